@@ -1,0 +1,1698 @@
+/*
+ * GFN2-xTB CPU oracle -- plain-C FP64 restatement of the reference CPU path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker and the timed CPU baseline.  The
+ * product library never links or loads this file.
+ *
+ * Every function cites the reference routine it restates (paths relative to
+ * the reference root).  The arithmetic follows the cited formulas; loop order
+ * and data layout are this file's own.
+ */
+#include "gfn2_oracle.h"
+
+#include <dlfcn.h>
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../xtbloom_amd/data/gfn2_tables.h"
+
+#define ST_OK 0
+#define ST_INVALID 1
+#define ST_INTERNAL 6
+#define ST_NOT_CONVERGED 7
+#define ST_EIGEN_FAILED 8
+
+static const double kPi = 3.141592653589793238462643383279502884;
+static const double kEvToHartree = 1.0 / 27.21138505;     /* h0.cpp:20 */
+static const double kAngstromToBohr = 1.8897261246204404; /* h0.cpp:21, coordination.cpp:27 */
+static const double kIntegralCutoff = 25.0;               /* common/integrals.hpp */
+
+/* ------------------------------------------------------------------ */
+/* LAPACK provider (LP64 LAPACKE/CBLAS, scipy-openblas symbol prefix) */
+/* ------------------------------------------------------------------ */
+typedef int (*fn_dpotrf)(int, char, int, double*, int);
+typedef int (*fn_dpocon)(int, char, int, const double*, int, double, double*, double*, int*);
+typedef int (*fn_dsyevd)(int, char, char, int, double*, int, double*, double*, int, int*, int);
+typedef void (*fn_dtrsm)(int, int, int, int, int, int, int, double, const double*, int, double*,
+                         int);
+typedef void (*fn_dgemm)(int, int, int, int, int, int, double, const double*, int, const double*,
+                         int, double, double*, int);
+typedef void (*fn_setthreads)(int);
+static fn_dpotrf la_dpotrf;
+static fn_dpocon la_dpocon;
+static fn_dsyevd la_dsyevd;
+static fn_dtrsm la_dtrsm;
+static fn_dgemm la_dgemm;
+enum { kColMajor = 102, kNoTrans = 111, kCblasTrans = 112, kLower = 122, kLeft = 141, kRight = 142,
+       kNonUnit = 131 };
+
+int oracle_init(const char* path) {
+  void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return ST_INVALID;
+  la_dpotrf = (fn_dpotrf)dlsym(h, "scipy_LAPACKE_dpotrf_work");
+  la_dpocon = (fn_dpocon)dlsym(h, "scipy_LAPACKE_dpocon_work");
+  la_dsyevd = (fn_dsyevd)dlsym(h, "scipy_LAPACKE_dsyevd_work");
+  la_dtrsm = (fn_dtrsm)dlsym(h, "scipy_cblas_dtrsm");
+  la_dgemm = (fn_dgemm)dlsym(h, "scipy_cblas_dgemm");
+  fn_setthreads st = (fn_setthreads)dlsym(h, "scipy_openblas_set_num_threads");
+  if (st) st(1); /* one system per worker, sequential BLAS (gfn2_cpu_execution.cpp:1907) */
+  return (la_dpotrf && la_dpocon && la_dsyevd && la_dtrsm && la_dgemm) ? ST_OK : ST_INVALID;
+}
+
+void oracle_default_options(oracle_options* o) {
+  /* api.cpp:150-173 */
+  o->max_iterations = 250;
+  o->history = 8;
+  o->charge_tolerance = 1.0e-6;
+  o->energy_tolerance = 1.0e-8;
+  o->kt = 300.0 * 3.166808578545117e-6;
+  o->damping = 0.4;
+}
+
+/* ------------------------------------------------------------------ */
+/* Basis (gfn2/basis.cpp:66 make_basis_plan, common/sto.cpp expand)    */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int nat, nsh, nao, nprim;
+  const int32_t* Z;
+  int* atom_sh;  /* [nat+1] */
+  int* sh_atom;  /* [nsh] */
+  int* sh_l;     /* [nsh] */
+  int* sh_par;   /* [nsh] index into the parameter shell table */
+  int* sh_ao;    /* [nsh+1] */
+  int* sh_prim;  /* [nsh+1] */
+  int* ao_sh;    /* [nao] */
+  int* ao_atom;  /* [nao] */
+  double* alpha; /* [nprim] */
+  double* coef;  /* [nprim] */
+} Basis;
+
+static void basis_free(Basis* b) {
+  free(b->atom_sh); free(b->sh_atom); free(b->sh_l); free(b->sh_par); free(b->sh_ao);
+  free(b->sh_prim); free(b->ao_sh); free(b->ao_atom); free(b->alpha); free(b->coef);
+  memset(b, 0, sizeof(*b));
+}
+
+/* common/sto.cpp:20-68: select one Stewart row. */
+static int sto_row(int n, int l, int ng, const double** a, const double** c) {
+  if (n == 6 && ng == 6) {
+    if (l == 0) { *a = xtbp_sto_alpha6s; *c = xtbp_sto_coeff6s; return 1; }
+    if (l == 1) { *a = xtbp_sto_alpha6p; *c = xtbp_sto_coeff6p; return 1; }
+    return 0;
+  }
+  if (n <= 0 || n > 5 || l >= n || l > 4) return 0;
+  int type = l == 0 ? n - 1 : l == 1 ? n + 3 : l == 2 ? n + 6 : l == 3 ? n + 8 : n + 9;
+  if (type >= 15) return 0;
+  if (ng == 3) { *a = xtbp_sto_alpha3 + 3 * type; *c = xtbp_sto_coeff3 + 3 * type; return 1; }
+  if (ng == 4) { *a = xtbp_sto_alpha4 + 4 * type; *c = xtbp_sto_coeff4 + 4 * type; return 1; }
+  if (ng == 6) { *a = xtbp_sto_alpha6 + 6 * type; *c = xtbp_sto_coeff6 + 6 * type; return 1; }
+  return 0;
+}
+
+static int basis_make(int nat, const int32_t* Z, Basis* b) {
+  memset(b, 0, sizeof(*b));
+  b->nat = nat;
+  b->Z = Z;
+  int nsh = 0, nao = 0, nprim = 0;
+  for (int a = 0; a < nat; ++a) {
+    if (Z[a] < 1 || Z[a] > XTBP_NELEM) return ST_INVALID;
+    int off = xtbp_elem_shell_offset[Z[a] - 1], cnt = xtbp_elem_shell_count[Z[a] - 1];
+    for (int s = 0; s < cnt; ++s) {
+      int l = xtbp_shell_l[off + s];
+      if (l > 2) return ST_INVALID; /* integral engine supports s, p, d (integrals.cpp:177) */
+      nsh++; nao += 2 * l + 1; nprim += xtbp_shell_ngauss[off + s];
+    }
+  }
+  b->nsh = nsh; b->nao = nao; b->nprim = nprim;
+  b->atom_sh = malloc(sizeof(int) * (nat + 1));
+  b->sh_atom = malloc(sizeof(int) * nsh);
+  b->sh_l = malloc(sizeof(int) * nsh);
+  b->sh_par = malloc(sizeof(int) * nsh);
+  b->sh_ao = malloc(sizeof(int) * (nsh + 1));
+  b->sh_prim = malloc(sizeof(int) * (nsh + 1));
+  b->ao_sh = malloc(sizeof(int) * nao);
+  b->ao_atom = malloc(sizeof(int) * nao);
+  b->alpha = malloc(sizeof(double) * nprim);
+  b->coef = malloc(sizeof(double) * nprim);
+  int is = 0, io = 0, ip = 0;
+  static const double dfac[5] = {1.0, 1.0, 3.0, 15.0, 105.0};
+  for (int a = 0; a < nat; ++a) {
+    b->atom_sh[a] = is;
+    int off = xtbp_elem_shell_offset[Z[a] - 1], cnt = xtbp_elem_shell_count[Z[a] - 1];
+    int seen_l[5] = {0, 0, 0, 0, 0};
+    for (int s = 0; s < cnt; ++s, ++is) {
+      int p = off + s, l = xtbp_shell_l[p], ng = xtbp_shell_ngauss[p];
+      if (seen_l[l]) return ST_INVALID; /* GFN2 has no repeated l (gfn2/basis.hpp:14-19) */
+      seen_l[l] = 1;
+      b->sh_atom[is] = a; b->sh_l[is] = l; b->sh_par[is] = p;
+      b->sh_ao[is] = io; b->sh_prim[is] = ip;
+      const double *ta, *tc;
+      if (!sto_row(xtbp_shell_n[p], l, ng, &ta, &tc)) return ST_INVALID;
+      double z2 = xtbp_shell_slater[p] * xtbp_shell_slater[p];
+      for (int k = 0; k < ng; ++k) {
+        double al = ta[k] * z2;
+        double norm = pow(2.0 / kPi * al, 0.75) * pow(sqrt(4.0 * al), (double)l) / sqrt(dfac[l]);
+        b->alpha[ip + k] = al;
+        b->coef[ip + k] = tc[k] * norm;
+      }
+      for (int m = 0; m < 2 * l + 1; ++m) { b->ao_sh[io + m] = is; b->ao_atom[io + m] = a; }
+      io += 2 * l + 1;
+      ip += ng;
+    }
+  }
+  b->atom_sh[nat] = is; b->sh_ao[nsh] = io; b->sh_prim[nsh] = ip;
+  return ST_OK;
+}
+
+int oracle_basis_counts(int32_t nat, const int32_t* Z, int32_t* nshell, int32_t* nao,
+                        double* electrons) {
+  Basis b;
+  int st = basis_make(nat, Z, &b);
+  if (st == ST_OK) {
+    if (nshell) *nshell = b.nsh;
+    if (nao) *nao = b.nao;
+    if (electrons) {
+      double e = 0.0;
+      for (int s = 0; s < b.nsh; ++s) e += xtbp_shell_refocc[b.sh_par[s]];
+      *electrons = e;
+    }
+  }
+  basis_free(&b);
+  return st;
+}
+
+/* ------------------------------------------------------------------ */
+/* One-electron integrals (common/integrals.cpp:387-691)              */
+/* ------------------------------------------------------------------ */
+static const int kNcart[3] = {1, 3, 6};
+static const int kNsph[3] = {1, 3, 5};
+static const int kCart[3][6][3] = {
+    {{0, 0, 0}},
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+    {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}}};
+/* operator powers about the ket centre: x y z xx xy yy xz yz zz (integrals.cpp:64-74) */
+static const int kMoment[9][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {1, 1, 0},
+                                  {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}};
+#define SQ3 1.732050807568877293527446341505872367
+/* real-spherical transforms [sph][cart] (integrals.cpp:84-130) */
+static const double kTrans[3][5][6] = {
+    {{1.0}},
+    {{0, 1, 0}, {0, 0, 1}, {1, 0, 0}},
+    {{0, SQ3, 0, 0, 0, 0},
+     {0, 0, 0, 0, SQ3, 0},
+     {-0.5, 0, 0, -0.5, 0, 1.0},
+     {0, 0, SQ3, 0, 0, 0},
+     {0.5 * SQ3, 0, 0, -0.5 * SQ3, 0, 0}}};
+
+typedef struct {
+  /* spherical blocks [bra][ket]; moments are raw until made traceless */
+  double s[5][5];
+  double m[9][5][5];
+  double ds[3][5][5];    /* d/dR_ket, operator origin co-moving with the ket */
+  double dm[3][9][5][5];
+} PairBlock;
+
+static void make_traceless(double m[9][5][5], int nb, int nk) {
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nk; ++j) {
+      double tr = 0.5 * (m[3][i][j] + m[5][i][j] + m[8][i][j]);
+      m[3][i][j] = 1.5 * m[3][i][j] - tr;
+      m[4][i][j] *= 1.5;
+      m[5][i][j] = 1.5 * m[5][i][j] - tr;
+      m[6][i][j] *= 1.5;
+      m[7][i][j] *= 1.5;
+      m[8][i][j] = 1.5 * m[8][i][j] - tr;
+    }
+}
+
+static void to_spherical(int lb, int lk, const double cart[6][6], double out[5][5]) {
+  for (int i = 0; i < kNsph[lb]; ++i)
+    for (int j = 0; j < kNsph[lk]; ++j) {
+      double v = 0.0;
+      for (int c = 0; c < kNcart[lb]; ++c) {
+        double tb = kTrans[lb][i][c];
+        if (tb == 0.0) continue;
+        for (int d = 0; d < kNcart[lk]; ++d) v += tb * cart[c][d] * kTrans[lk][j][d];
+      }
+      out[i][j] = v;
+    }
+}
+
+/*
+ * Shell-pair block <bra| O |ket> with O in {1, (r-R_ket)^m}; v = R_ket - R_bra.
+ * Obara-Saika 1D overlap recursion (integrals.cpp:387-416); the ket-centre
+ * derivative of (x-K)^a exp(-a_k (x-K)^2) is 2 a_k (x-K)^(a+1) - a (x-K)^(a-1).
+ */
+static void shell_pair(const Basis* b, int bra, int ket, const double v[3], int moments,
+                       int gradient, PairBlock* out) {
+  const int lb = b->sh_l[bra], lk = b->sh_l[ket];
+  const int nb = kNcart[lb], nk = kNcart[lk];
+  double cs[6][6], cm[9][6][6], cds[3][6][6], cdm[3][9][6][6];
+  memset(cs, 0, sizeof(cs));
+  if (moments) memset(cm, 0, sizeof(cm));
+  if (gradient) { memset(cds, 0, sizeof(cds)); if (moments) memset(cdm, 0, sizeof(cdm)); }
+  const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  const int amax = lk + (moments ? 2 : 0) + (gradient ? 1 : 0);
+  for (int pk = b->sh_prim[ket]; pk < b->sh_prim[ket + 1]; ++pk) {
+    const double ak = b->alpha[pk];
+    for (int pb = b->sh_prim[bra]; pb < b->sh_prim[bra + 1]; ++pb) {
+      const double ab = b->alpha[pb];
+      const double p = ak + ab, ip = 1.0 / p;
+      const double est = ak * ab * r2 * ip;
+      if (est > kIntegralCutoff) continue;
+      const double pref = exp(-est) * pow(kPi * ip, 1.5) * b->coef[pk] * b->coef[pb];
+      double o[3][7][3];
+      for (int x = 0; x < 3; ++x) {
+        const double pK = -v[x] * ab * ip, pB = v[x] * ak * ip, h = 0.5 * ip;
+        memset(o[x], 0, sizeof(o[x]));
+        o[x][0][0] = 1.0;
+        for (int a = 0; a < amax; ++a)
+          o[x][a + 1][0] = pK * o[x][a][0] + (a > 0 ? a * h * o[x][a - 1][0] : 0.0);
+        for (int bb = 0; bb < lb; ++bb)
+          for (int a = 0; a <= amax; ++a)
+            o[x][a][bb + 1] = pB * o[x][a][bb] + h * ((a > 0 ? a * o[x][a - 1][bb] : 0.0) +
+                                                      (bb > 0 ? bb * o[x][a][bb - 1] : 0.0));
+      }
+      for (int i = 0; i < nb; ++i) {
+        const int* eb = kCart[lb][i];
+        for (int j = 0; j < nk; ++j) {
+          const int* ek = kCart[lk][j];
+          double f[3];
+          for (int x = 0; x < 3; ++x) f[x] = o[x][ek[x]][eb[x]];
+          cs[i][j] += pref * f[0] * f[1] * f[2];
+          if (gradient) {
+            for (int x = 0; x < 3; ++x) {
+              int a = ek[x];
+              double d = 2.0 * ak * o[x][a + 1][eb[x]] - (a > 0 ? a * o[x][a - 1][eb[x]] : 0.0);
+              cds[x][i][j] += pref * d * f[(x + 1) % 3] * f[(x + 2) % 3];
+            }
+          }
+          if (moments) {
+            for (int c = 0; c < 9; ++c) {
+              double g[3];
+              for (int x = 0; x < 3; ++x) g[x] = o[x][ek[x] + kMoment[c][x]][eb[x]];
+              cm[c][i][j] += pref * g[0] * g[1] * g[2];
+              if (gradient) {
+                for (int x = 0; x < 3; ++x) {
+                  int a = ek[x] + kMoment[c][x];
+                  double d =
+                      2.0 * ak * o[x][a + 1][eb[x]] - (a > 0 ? a * o[x][a - 1][eb[x]] : 0.0);
+                  cdm[x][c][i][j] += pref * d * g[(x + 1) % 3] * g[(x + 2) % 3];
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  to_spherical(lb, lk, cs, out->s);
+  if (gradient)
+    for (int x = 0; x < 3; ++x) to_spherical(lb, lk, cds[x], out->ds[x]);
+  if (moments) {
+    for (int c = 0; c < 9; ++c) to_spherical(lb, lk, cm[c], out->m[c]);
+    make_traceless(out->m, kNsph[lb], kNsph[lk]);
+    if (gradient) {
+      for (int x = 0; x < 3; ++x) {
+        for (int c = 0; c < 9; ++c) to_spherical(lb, lk, cdm[x][c], out->dm[x][c]);
+        make_traceless(out->dm[x], kNsph[lb], kNsph[lk]);
+      }
+    }
+  }
+}
+
+/*
+ * Dense S, D(3), Q(6) for one system: element (mu,nu) with operator origin on
+ * the atom of the ket AO nu (integrals.cpp:701-900, gfn2/integrals.hpp:25-45).
+ * Every ordered shell pair is evaluated directly with its own ket origin.
+ */
+static void build_integrals(const Basis* b, const double* xyz, double* S, double* D, double* Q) {
+  const int n = b->nao;
+  const size_t nn = (size_t)n * n;
+  PairBlock blk;
+  for (int i = 0; i < b->nsh; ++i) {
+    const int A = b->sh_atom[i];
+    for (int j = 0; j < b->nsh; ++j) {
+      const int C = b->sh_atom[j];
+      double v[3] = {xyz[3 * C] - xyz[3 * A], xyz[3 * C + 1] - xyz[3 * A + 1],
+                     xyz[3 * C + 2] - xyz[3 * A + 2]};
+      shell_pair(b, i, j, v, D != NULL, 0, &blk);
+      const int li = b->sh_l[i], lj = b->sh_l[j];
+      for (int u = 0; u < kNsph[li]; ++u)
+        for (int w = 0; w < kNsph[lj]; ++w) {
+          size_t idx = (size_t)(b->sh_ao[i] + u) * n + (b->sh_ao[j] + w);
+          S[idx] = blk.s[u][w];
+          if (D) {
+            for (int c = 0; c < 3; ++c) D[c * nn + idx] = blk.m[c][u][w];
+            for (int c = 0; c < 6; ++c) Q[c * nn + idx] = blk.m[3 + c][u][w];
+          }
+        }
+    }
+  }
+  /* S is exactly symmetric in the reference (one triangle mirrored). */
+  for (int r = 0; r < n; ++r)
+    for (int c = r + 1; c < n; ++c) S[(size_t)c * n + r] = S[(size_t)r * n + c];
+}
+
+/*
+ * gradient += sum_{mu,nu} aS*dS + aD.dD + aQ.dQ over inter-atomic ordered
+ * blocks (add_overlap_gradient_cpu integrals.cpp:1060, add_multipole_gradient_cpu
+ * :913).  d/dR_bra = -d/dR_ket by translation invariance.
+ */
+static void contract_integral_gradient(const Basis* b, const double* xyz, const double* aS,
+                                       const double* aD, const double* aQ, double* grad) {
+  const int n = b->nao;
+  const size_t nn = (size_t)n * n;
+  PairBlock blk;
+  for (int i = 0; i < b->nsh; ++i) {
+    const int A = b->sh_atom[i];
+    for (int j = 0; j < b->nsh; ++j) {
+      const int C = b->sh_atom[j];
+      if (A == C) continue;
+      double v[3] = {xyz[3 * C] - xyz[3 * A], xyz[3 * C + 1] - xyz[3 * A + 1],
+                     xyz[3 * C + 2] - xyz[3 * A + 2]};
+      shell_pair(b, i, j, v, 1, 1, &blk);
+      double g[3] = {0, 0, 0};
+      for (int u = 0; u < kNsph[b->sh_l[i]]; ++u)
+        for (int w = 0; w < kNsph[b->sh_l[j]]; ++w) {
+          size_t idx = (size_t)(b->sh_ao[i] + u) * n + (b->sh_ao[j] + w);
+          for (int x = 0; x < 3; ++x) {
+            double t = aS[idx] * blk.ds[x][u][w];
+            for (int c = 0; c < 3; ++c) t += aD[c * nn + idx] * blk.dm[x][c][u][w];
+            for (int c = 0; c < 6; ++c) t += aQ[c * nn + idx] * blk.dm[x][3 + c][u][w];
+            g[x] += t;
+          }
+        }
+      for (int x = 0; x < 3; ++x) { grad[3 * C + x] += g[x]; grad[3 * A + x] -= g[x]; }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* GFN2 coordination number (gfn2/coordination.cpp:144-320)           */
+/* ------------------------------------------------------------------ */
+static double logistic(double x) {
+  if (x >= 0.0) { double e = exp(-x); return 1.0 / (1.0 + e); }
+  double e = exp(x);
+  return e / (1.0 + e);
+}
+
+static double cn_radius(int Z) { return xtbp_covalent_radius_aa[Z - 1] * (4.0 / 3.0) * kAngstromToBohr; }
+
+static void dexp_count(double r, double rc, double* val, double* der) {
+  double f1 = logistic(10.0 * (rc / r - 1.0));
+  double f2 = logistic(20.0 * ((rc + 2.0) / r - 1.0));
+  *val = f1 * f2;
+  *der = -(1.0 / (r * r)) * (10.0 * rc * f1 * (1.0 - f1) * f2 + 20.0 * (rc + 2.0) * f2 * (1.0 - f2) * f1);
+}
+
+static int gfn2_cn(int nat, const int32_t* Z, const double* xyz, double* cn) {
+  for (int a = 0; a < nat; ++a) cn[a] = 0.0;
+  for (int a = 0; a < nat; ++a)
+    for (int c = 0; c < a; ++c) {
+      double dx = xyz[3 * a] - xyz[3 * c], dy = xyz[3 * a + 1] - xyz[3 * c + 1],
+             dz = xyz[3 * a + 2] - xyz[3 * c + 2];
+      double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < 1e-12) return ST_INVALID;
+      if (r2 > 625.0) continue;
+      double val, der;
+      dexp_count(sqrt(r2), cn_radius(Z[a]) + cn_radius(Z[c]), &val, &der);
+      cn[a] += val;
+      cn[c] += val;
+    }
+  return ST_OK;
+}
+
+static void gfn2_cn_gradient(int nat, const int32_t* Z, const double* xyz, const double* dEdcn,
+                             double* grad) {
+  for (int a = 0; a < nat; ++a)
+    for (int c = 0; c < a; ++c) {
+      double d[3] = {xyz[3 * a] - xyz[3 * c], xyz[3 * a + 1] - xyz[3 * c + 1],
+                     xyz[3 * a + 2] - xyz[3 * c + 2]};
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (r2 > 625.0) continue;
+      double r = sqrt(r2), val, der;
+      dexp_count(r, cn_radius(Z[a]) + cn_radius(Z[c]), &val, &der);
+      double sc = (dEdcn[a] + dEdcn[c]) * der / r;
+      for (int x = 0; x < 3; ++x) { grad[3 * a + x] += sc * d[x]; grad[3 * c + x] -= sc * d[x]; }
+    }
+}
+
+int oracle_coordination(int32_t nat, const int32_t* Z, const double* xyz, double* cn) {
+  return gfn2_cn(nat, Z, xyz, cn);
+}
+
+/* ------------------------------------------------------------------ */
+/* H0 (gfn2/h0.cpp:248-597)                                          */
+/* ------------------------------------------------------------------ */
+static double h0_pair_scale(const Basis* b, int s, int t) {
+  const int ps = b->sh_par[s], pt = b->sh_par[t];
+  const double zs = xtbp_shell_slater[ps], zt = xtbp_shell_slater[pt];
+  const double zeta = pow(2.0 * sqrt(zs * zt) / (zs + zt), XTBP_H0_WEXP);
+  const double den = xtbp_elem_en[b->Z[b->sh_atom[s]] - 1] - xtbp_elem_en[b->Z[b->sh_atom[t]] - 1];
+  const double en = 1.0 + XTBP_H0_ENSCALE * den * den;
+  return zeta * en * xtbp_kpair[b->sh_l[s] * 3 + b->sh_l[t]] * 1.0 /* no pair overrides */;
+}
+
+static double shell_level(const Basis* b, int s, const double* cn) {
+  const int p = b->sh_par[s];
+  return xtbp_shell_level_ev[p] * kEvToHartree -
+         xtbp_shell_kcn_ev[p] * kEvToHartree * cn[b->sh_atom[s]];
+}
+
+/* H0_mn = S_mn * (h_s + h_t)/2 * [k_st * poly_s(R) * poly_t(R)]_{A != B} */
+static void build_h0(const Basis* b, const double* xyz, const double* cn, const double* S,
+                     double* H0) {
+  const int n = b->nao;
+  for (int s = 0; s < b->nsh; ++s) {
+    const int A = b->sh_atom[s];
+    const double hs = shell_level(b, s, cn);
+    for (int t = 0; t < b->nsh; ++t) {
+      const int C = b->sh_atom[t];
+      const double ht = shell_level(b, t, cn);
+      double spatial = 1.0;
+      if (A != C) {
+        double dx = xyz[3 * A] - xyz[3 * C], dy = xyz[3 * A + 1] - xyz[3 * C + 1],
+               dz = xyz[3 * A + 2] - xyz[3 * C + 2];
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        double rr = sqrt(r / ((xtbp_atomic_radius_aa[b->Z[A] - 1] +
+                               xtbp_atomic_radius_aa[b->Z[C] - 1]) * kAngstromToBohr));
+        double poly = (1.0 + xtbp_shell_kpoly[b->sh_par[s]] * rr) *
+                      (1.0 + xtbp_shell_kpoly[b->sh_par[t]] * rr);
+        spatial = h0_pair_scale(b, s, t) * poly;
+      }
+      const double f = 0.5 * (hs + ht) * spatial;
+      for (int u = b->sh_ao[s]; u < b->sh_ao[s + 1]; ++u)
+        for (int w = b->sh_ao[t]; w < b->sh_ao[t + 1]; ++w)
+          H0[(size_t)u * n + w] = S[(size_t)u * n + w] * f;
+    }
+  }
+}
+
+/* add_h0_vjp_cpu (h0.cpp:469-597): dE/dS, dE/dCN and the explicit polynomial gradient. */
+static void h0_vjp(const Basis* b, const double* xyz, const double* cn, const double* S,
+                   const double* P, double* aS, double* dEdcn, double* grad) {
+  const int n = b->nao;
+  for (int s = 0; s < b->nsh; ++s) {
+    const int A = b->sh_atom[s];
+    const double hs = shell_level(b, s, cn);
+    for (int t = 0; t < b->nsh; ++t) {
+      const int C = b->sh_atom[t];
+      const double ht = shell_level(b, t, cn);
+      const double avg = 0.5 * (hs + ht);
+      double spatial = 1.0, dspatial = 0.0, d[3] = {0, 0, 0}, r = 0.0;
+      if (A != C) {
+        for (int x = 0; x < 3; ++x) d[x] = xyz[3 * A + x] - xyz[3 * C + x];
+        r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double rr = sqrt(r / ((xtbp_atomic_radius_aa[b->Z[A] - 1] +
+                               xtbp_atomic_radius_aa[b->Z[C] - 1]) * kAngstromToBohr));
+        double ks = xtbp_shell_kpoly[b->sh_par[s]], kt = xtbp_shell_kpoly[b->sh_par[t]];
+        double p1 = 1.0 + ks * rr, p2 = 1.0 + kt * rr;
+        double kp = h0_pair_scale(b, s, t);
+        spatial = kp * p1 * p2;
+        dspatial = kp * (ks * p2 + kt * p1) * rr / (2.0 * r);
+      }
+      const double f = avg * spatial;
+      double wsum = 0.0;
+      for (int u = b->sh_ao[s]; u < b->sh_ao[s + 1]; ++u)
+        for (int w = b->sh_ao[t]; w < b->sh_ao[t + 1]; ++w) {
+          size_t idx = (size_t)u * n + w;
+          aS[idx] += P[idx] * f;
+          wsum += P[idx] * S[idx];
+        }
+      const double lw = 0.5 * wsum * spatial;
+      dEdcn[A] -= xtbp_shell_kcn_ev[b->sh_par[s]] * kEvToHartree * lw;
+      dEdcn[C] -= xtbp_shell_kcn_ev[b->sh_par[t]] * kEvToHartree * lw;
+      if (A != C) {
+        const double sc = wsum * avg * dspatial / r;
+        for (int x = 0; x < 3; ++x) { grad[3 * A + x] += sc * d[x]; grad[3 * C + x] -= sc * d[x]; }
+      }
+    }
+  }
+}
+
+int oracle_integrals(int32_t nat, const int32_t* Z, const double* xyz, double* S, double* D,
+                     double* Q) {
+  Basis b;
+  int st = basis_make(nat, Z, &b);
+  if (st == ST_OK) build_integrals(&b, xyz, S, D, Q);
+  basis_free(&b);
+  return st;
+}
+
+int oracle_h0(int32_t nat, const int32_t* Z, const double* xyz, const double* S, double* H0) {
+  Basis b;
+  int st = basis_make(nat, Z, &b);
+  double* cn = malloc(sizeof(double) * (nat > 0 ? nat : 1));
+  if (st == ST_OK) st = gfn2_cn(nat, Z, xyz, cn);
+  if (st == ST_OK) build_h0(&b, xyz, cn, S, H0);
+  free(cn);
+  basis_free(&b);
+  return st;
+}
+
+/* ------------------------------------------------------------------ */
+/* ES2 / ES3 / point charges (es2.cpp:211-330, es3.hpp, external_point_charges.cpp) */
+/* ------------------------------------------------------------------ */
+static double shell_hardness(const Basis* b, int s) {
+  return xtbp_elem_gam[b->Z[b->sh_atom[s]] - 1] * xtbp_shell_hubbard_scale[b->sh_par[s]];
+}
+
+static void build_gamma(const Basis* b, const double* xyz, double* G) {
+  const int ns = b->nsh;
+  for (int s = 0; s < ns; ++s) {
+    const int A = b->sh_atom[s];
+    for (int t = 0; t < ns; ++t) {
+      const int C = b->sh_atom[t];
+      const double eta = 0.5 * (shell_hardness(b, s) + shell_hardness(b, t));
+      if (A == C) { G[s * ns + t] = eta; continue; }
+      double dx = xyz[3 * A] - xyz[3 * C], dy = xyz[3 * A + 1] - xyz[3 * C + 1],
+             dz = xyz[3 * A + 2] - xyz[3 * C + 2];
+      G[s * ns + t] = 1.0 / sqrt(dx * dx + dy * dy + dz * dz + 1.0 / (eta * eta));
+    }
+  }
+}
+
+static double es3_gamma(const Basis* b, int s) {
+  static const double scale[3] = {XTBP_ES3_SCALE_S, XTBP_ES3_SCALE_P, XTBP_ES3_SCALE_D};
+  return xtbp_elem_gam3[b->Z[b->sh_atom[s]] - 1] * scale[b->sh_l[s]];
+}
+
+/* V_s = sum_p Q_p / sqrt(r^2 + (2/(eta_s + gamma_p))^2) (external_point_charges.hpp:59) */
+static void pc_potential(const Basis* b, const double* xyz, int npc, const double* pxyz,
+                         const double* pq, const double* pg, double* V) {
+  for (int s = 0; s < b->nsh; ++s) {
+    const int A = b->sh_atom[s];
+    double v = 0.0;
+    for (int p = 0; p < npc; ++p) {
+      double dx = xyz[3 * A] - pxyz[3 * p], dy = xyz[3 * A + 1] - pxyz[3 * p + 1],
+             dz = xyz[3 * A + 2] - pxyz[3 * p + 2];
+      double ia = 2.0 / (shell_hardness(b, s) + pg[p]);
+      v += pq[p] / sqrt(dx * dx + dy * dy + dz * dz + ia * ia);
+    }
+    V[s] = v;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* AES2 (gfn2/aes2.cpp:355-706)                                       */
+/* ------------------------------------------------------------------ */
+static double mp_radius(int Z, double cn, double* dr_dcn) {
+  const double rad = xtbp_elem_mprad[Z - 1];
+  const double f = logistic(XTBP_MP_KEXP * (cn - xtbp_elem_mpvcn[Z - 1] - XTBP_MP_SHIFT));
+  if (dr_dcn) *dr_dcn = (XTBP_MP_RMAX - rad) * XTBP_MP_KEXP * f * (1.0 - f);
+  return rad + (XTBP_MP_RMAX - rad) * f;
+}
+
+typedef struct { double d[3], k3, k5, rho; } AesPair;
+
+static void aes2_pairs(int nat, const int32_t* Z, const double* xyz, const double* cn,
+                       AesPair* pairs) {
+  /* pair (i<j) stored at j*(j-1)/2 + i, vector R_i - R_j */
+  for (int j = 1; j < nat; ++j)
+    for (int i = 0; i < j; ++i) {
+      AesPair* p = &pairs[(size_t)j * (j - 1) / 2 + i];
+      for (int x = 0; x < 3; ++x) p->d[x] = xyz[3 * i + x] - xyz[3 * j + x];
+      double r = sqrt(p->d[0] * p->d[0] + p->d[1] * p->d[1] + p->d[2] * p->d[2]);
+      double rho = 0.5 * (mp_radius(Z[i], cn[i], NULL) + mp_radius(Z[j], cn[j], NULL));
+      double ir = 1.0 / r, ir3 = ir * ir * ir, ir5 = ir3 * ir * ir, sc = rho * ir;
+      p->k3 = ir3 / (1.0 + 6.0 * sc * sc * sc);
+      p->k5 = ir5 / (1.0 + 6.0 * sc * sc * sc * sc);
+      p->rho = rho;
+    }
+}
+
+static const double kQScale[6] = {1.0, 2.0, 1.0, 2.0, 2.0, 1.0};
+
+static void packed_tensor(const double* d, double k5, double sq[6]) {
+  sq[0] = d[0] * d[0] * k5; sq[1] = 2.0 * d[0] * d[1] * k5; sq[2] = d[1] * d[1] * k5;
+  sq[3] = 2.0 * d[0] * d[2] * k5; sq[4] = 2.0 * d[1] * d[2] * k5; sq[5] = d[2] * d[2] * k5;
+}
+
+static double aes2_energy(int nat, const int32_t* Z, const AesPair* pairs, const double* q,
+                          const double* dip, const double* quad) {
+  double e = 0.0;
+  for (int a = 0; a < nat; ++a) {
+    double d2 = 0.0, q2 = 0.0;
+    for (int x = 0; x < 3; ++x) d2 += dip[3 * a + x] * dip[3 * a + x];
+    for (int c = 0; c < 6; ++c) q2 += kQScale[c] * quad[6 * a + c] * quad[6 * a + c];
+    e += xtbp_elem_dkernel[Z[a] - 1] * d2 + xtbp_elem_qkernel[Z[a] - 1] * q2;
+  }
+  for (int j = 1; j < nat; ++j)
+    for (int i = 0; i < j; ++i) {
+      const AesPair* p = &pairs[(size_t)j * (j - 1) / 2 + i];
+      const double *di = dip + 3 * i, *dj = dip + 3 * j;
+      double pi = 0, pj = 0, dd = 0, cd = 0, sq[6];
+      for (int x = 0; x < 3; ++x) {
+        pi += p->d[x] * di[x]; pj += p->d[x] * dj[x]; dd += di[x] * dj[x];
+        cd += p->d[x] * (q[i] * dj[x] - q[j] * di[x]);
+      }
+      double r2 = p->d[0] * p->d[0] + p->d[1] * p->d[1] + p->d[2] * p->d[2];
+      packed_tensor(p->d, p->k5, sq);
+      double sqi = 0, sqj = 0;
+      for (int c = 0; c < 6; ++c) { sqi += sq[c] * quad[6 * i + c]; sqj += sq[c] * quad[6 * j + c]; }
+      e += p->k3 * cd + p->k5 * (r2 * dd - 3.0 * pi * pj) + q[i] * sqj + q[j] * sqi;
+    }
+  return e;
+}
+
+static void aes2_potential(int nat, const int32_t* Z, const AesPair* pairs, const double* q,
+                           const double* dip, const double* quad, double* vq, double* vd,
+                           double* vqq) {
+  for (int a = 0; a < nat; ++a) {
+    vq[a] = 0.0;
+    for (int x = 0; x < 3; ++x) vd[3 * a + x] = 2.0 * xtbp_elem_dkernel[Z[a] - 1] * dip[3 * a + x];
+    for (int c = 0; c < 6; ++c)
+      vqq[6 * a + c] = 2.0 * xtbp_elem_qkernel[Z[a] - 1] * kQScale[c] * quad[6 * a + c];
+  }
+  for (int j = 1; j < nat; ++j)
+    for (int i = 0; i < j; ++i) {
+      const AesPair* p = &pairs[(size_t)j * (j - 1) / 2 + i];
+      const double *di = dip + 3 * i, *dj = dip + 3 * j;
+      double sd[3], sq[6], pi = 0, pj = 0, sdi = 0, sdj = 0;
+      for (int x = 0; x < 3; ++x) {
+        sd[x] = p->d[x] * p->k3;
+        pi += p->d[x] * di[x]; pj += p->d[x] * dj[x];
+        sdi += sd[x] * di[x]; sdj += sd[x] * dj[x];
+      }
+      packed_tensor(p->d, p->k5, sq);
+      double r2 = p->d[0] * p->d[0] + p->d[1] * p->d[1] + p->d[2] * p->d[2];
+      double iso = r2 * p->k5;
+      double sqi = 0, sqj = 0;
+      for (int c = 0; c < 6; ++c) { sqi += sq[c] * quad[6 * i + c]; sqj += sq[c] * quad[6 * j + c]; }
+      vq[i] += sdj + sqj;
+      vq[j] += -sdi + sqi;
+      for (int x = 0; x < 3; ++x) {
+        vd[3 * i + x] += -q[j] * sd[x] + iso * dj[x] - 3.0 * p->k5 * p->d[x] * pj;
+        vd[3 * j + x] += q[i] * sd[x] + iso * di[x] - 3.0 * p->k5 * p->d[x] * pi;
+      }
+      for (int c = 0; c < 6; ++c) { vqq[6 * i + c] += q[j] * sq[c]; vqq[6 * j + c] += q[i] * sq[c]; }
+    }
+}
+
+/* add_aes2_vjp_cpu / pair_vjp (aes2.cpp:601-706): dE/dR at fixed multipoles and dE/dCN. */
+static void aes2_vjp(int nat, const int32_t* Z, const AesPair* pairs, const double* cn,
+                     const double* q, const double* dip, const double* quad, double* grad,
+                     double* dEdcn) {
+  for (int j = 1; j < nat; ++j)
+    for (int i = 0; i < j; ++i) {
+      const AesPair* p = &pairs[(size_t)j * (j - 1) / 2 + i];
+      const double *di = dip + 3 * i, *dj = dip + 3 * j, *Qi = quad + 6 * i, *Qj = quad + 6 * j;
+      double dri, drj;
+      mp_radius(Z[i], cn[i], &dri);
+      mp_radius(Z[j], cn[j], &drj);
+      double r = sqrt(p->d[0] * p->d[0] + p->d[1] * p->d[1] + p->d[2] * p->d[2]);
+      double sc = p->rho / r, sc3 = sc * sc * sc, sc4 = sc3 * sc;
+      double f3 = 1.0 / (1.0 + 6.0 * sc3), f5 = 1.0 / (1.0 + 6.0 * sc4);
+      double cdv[3], pi = 0, pj = 0, dd = 0;
+      for (int x = 0; x < 3; ++x) {
+        cdv[x] = q[i] * dj[x] - q[j] * di[x];
+        pi += p->d[x] * di[x]; pj += p->d[x] * dj[x]; dd += di[x] * dj[x];
+      }
+      double cdn = p->d[0] * cdv[0] + p->d[1] * cdv[1] + p->d[2] * cdv[2];
+      double r2 = r * r;
+      double ddn = r2 * dd - 3.0 * pi * pj;
+      double T[6];
+      for (int c = 0; c < 6; ++c) T[c] = q[i] * Qj[c] + q[j] * Qi[c];
+      double tv[3] = {T[0] * p->d[0] + T[1] * p->d[1] + T[3] * p->d[2],
+                      T[1] * p->d[0] + T[2] * p->d[1] + T[4] * p->d[2],
+                      T[3] * p->d[0] + T[4] * p->d[1] + T[5] * p->d[2]};
+      double cqn = p->d[0] * tv[0] + p->d[1] * tv[1] + p->d[2] * tv[2];
+      double ir = 1.0 / r;
+      double dk3r = -3.0 * f3 * p->k3 * ir, dk5r = -(1.0 + 4.0 * f5) * p->k5 * ir;
+      double irho = 1.0 / p->rho;
+      double dk3rho = -3.0 * (1.0 - f3) * p->k3 * irho, dk5rho = -4.0 * (1.0 - f5) * p->k5 * irho;
+      double k5n = ddn + cqn;
+      double dE_drho = dk3rho * cdn + dk5rho * k5n;
+      for (int x = 0; x < 3; ++x) {
+        double dddx = 2.0 * dd * p->d[x] - 3.0 * (pi * dj[x] + pj * di[x]);
+        double g = p->k3 * cdv[x] + dk3r * cdn * p->d[x] * ir + p->k5 * (dddx + 2.0 * tv[x]) +
+                   dk5r * k5n * p->d[x] * ir;
+        grad[3 * i + x] += g;
+        grad[3 * j + x] -= g;
+      }
+      dEdcn[i] += 0.5 * dE_drho * dri;
+      dEdcn[j] += 0.5 * dE_drho * drj;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* D4 (gfn2/d4.cpp)                                                   */
+/* ------------------------------------------------------------------ */
+static const double kD4K4 = 4.10451, kD4K5 = 19.08857, kD4K6 = 2.0 * 11.28174 * 11.28174;
+
+typedef struct {
+  int nat;
+  const int32_t* Z;
+  double* cn;      /* D4 CN */
+  double* pair;    /* [npair*5]: d(3), damp, ddamp (d4.hpp:44-52) */
+  double* r0;      /* [npair] damping radius */
+  double* rcov;    /* [npair] */
+  double* enf;     /* [npair] */
+} D4Geo;
+
+static size_t pidx(int i, int j) { /* i<j */ return (size_t)j * (j - 1) / 2 + i; }
+
+static void d4_free(D4Geo* g) { free(g->cn); free(g->pair); free(g->r0); free(g->rcov); free(g->enf); }
+
+static void d4_geometry(int nat, const int32_t* Z, const double* xyz, D4Geo* g) {
+  size_t np = (size_t)nat * (nat - 1) / 2;
+  g->nat = nat; g->Z = Z;
+  g->cn = calloc(nat, sizeof(double));
+  g->pair = calloc(np * 5 + 1, sizeof(double));
+  g->r0 = calloc(np + 1, sizeof(double));
+  g->rcov = calloc(np + 1, sizeof(double));
+  g->enf = calloc(np + 1, sizeof(double));
+  for (int j = 1; j < nat; ++j)
+    for (int i = 0; i < j; ++i) {
+      size_t k = pidx(i, j);
+      const int ei = Z[i] - 1, ej = Z[j] - 1;
+      g->rcov[k] = xtbp_d4_covalent_radius[ei] + xtbp_d4_covalent_radius[ej];
+      double den = fabs(xtbp_d4_en[ei] - xtbp_d4_en[ej]);
+      g->enf[k] = kD4K4 * exp(-pow(den + kD4K5, 2.0) / kD4K6);
+      double rrij = 3.0 * xtbp_d4_r4r2[ei] * xtbp_d4_r4r2[ej];
+      g->r0[k] = XTBP_DISP_A1 * sqrt(rrij) + XTBP_DISP_A2;
+      double* p = g->pair + 5 * k;
+      for (int x = 0; x < 3; ++x) p[x] = xyz[3 * i + x] - xyz[3 * j + x];
+      double r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+      if (r2 <= 900.0) {
+        double r = sqrt(r2), ex = 7.5 * (r - g->rcov[k]) / g->rcov[k];
+        double cnt = 0.5 * g->enf[k] * (1.0 + erf(-ex));
+        g->cn[i] += cnt; g->cn[j] += cnt;
+      }
+      if (r2 <= 2500.0) {
+        double r4 = r2 * r2, r6 = r4 * r2, r0 = g->r0[k], r02 = r0 * r0, r04 = r02 * r02,
+               r06 = r04 * r02;
+        double t6 = 1.0 / (r6 + r06), t8 = 1.0 / (r4 * r4 + r04 * r04);
+        p[3] = XTBP_DISP_S6 * t6 + XTBP_DISP_S8 * rrij * t8;
+        p[4] = XTBP_DISP_S6 * (-6.0 * r4 * t6 * t6) + XTBP_DISP_S8 * rrij * (-8.0 * r6 * t8 * t8);
+      }
+    }
+}
+
+static double zeta_scale(double a, double c, double qref, double qmod) {
+  if (qmod < 0.0) return exp(a);
+  return exp(a * (1.0 - exp(c * (1.0 - qref / qmod))));
+}
+static double zeta_scale_dq(double a, double c, double qref, double qmod) {
+  if (qmod < 0.0) return 0.0;
+  double inner = exp(c * (1.0 - qref / qmod));
+  return -a * c * inner * zeta_scale(a, c, qref, qmod) * qref / (qmod * qmod);
+}
+
+/* prepare_weight_slice (d4.cpp:298-366) */
+static void d4_weights(int Z, double cn, double q, double* w, double* dwdcn, double* dwdq) {
+  const int e = Z - 1, off = xtbp_d4_ref_offset[e], nref = xtbp_d4_ref_count[e];
+  double norm = 0.0, dnorm = 0.0, maxcn = -INFINITY;
+  for (int r = 0; r < nref; ++r) {
+    double rc = xtbp_d4_ref_cn[off + r];
+    if (rc > maxcn) maxcn = rc;
+    for (int k = 1; k <= xtbp_d4_ref_ngauss[off + r]; ++k) {
+      double f = 6.0 * k, dl = cn - rc, v = exp(-f * dl * dl);
+      norm += v; dnorm += 2.0 * f * (rc - cn) * v;
+    }
+  }
+  double inorm = fabs(norm) > 1.4916681462400413e-154 ? 1.0 / norm : 0.0;
+  double qmod = q + xtbp_d4_zeff[e], c = xtbp_d4_hardness[e] * 2.0;
+  for (int r = 0; r < XTBP_D4_MAXREF; ++r) { w[r] = 0.0; if (dwdcn) dwdcn[r] = 0.0; if (dwdq) dwdq[r] = 0.0; }
+  for (int r = 0; r < nref; ++r) {
+    double rc = xtbp_d4_ref_cn[off + r], num = 0.0, dnum = 0.0;
+    for (int k = 1; k <= xtbp_d4_ref_ngauss[off + r]; ++k) {
+      double f = 6.0 * k, dl = cn - rc, v = exp(-f * dl * dl);
+      num += v; dnum += 2.0 * f * (rc - cn) * v;
+    }
+    double cw = num * inorm;
+    if (!isfinite(cw) || inorm == 0.0) cw = fabs(maxcn - rc) < 1e-12 ? 1.0 : 0.0;
+    double dcw = inorm * (dnum - num * dnorm * inorm);
+    if (!isfinite(dcw) || inorm == 0.0) dcw = 0.0;
+    double qref = xtbp_d4_ref_q[off + r] + xtbp_d4_zeff[e];
+    double z = zeta_scale(3.0, c, qref, qmod);
+    w[r] = cw * z;
+    if (dwdcn) dwdcn[r] = dcw * z;
+    if (dwdq) dwdq[r] = cw * zeta_scale_dq(3.0, c, qref, qmod);
+  }
+}
+
+typedef struct { double c6, dcn_i, dcn_j, dq_i, dq_j; } C6Pair;
+
+static C6Pair d4_c6(int Zi, int Zj, const double* wi, const double* wj, const double* dci,
+                    const double* dcj, const double* dqi, const double* dqj) {
+  C6Pair out = {0, 0, 0, 0, 0};
+  const int oi = xtbp_d4_ref_offset[Zi - 1], ni = xtbp_d4_ref_count[Zi - 1];
+  const int oj = xtbp_d4_ref_offset[Zj - 1], nj = xtbp_d4_ref_count[Zj - 1];
+  for (int a = 0; a < ni; ++a)
+    for (int b = 0; b < nj; ++b) {
+      double c6 = xtbp_d4_c6[(size_t)(oi + a) * XTBP_D4_NREF + (oj + b)];
+      out.c6 += wi[a] * wj[b] * c6;
+      if (dci) {
+        out.dcn_i += dci[a] * wj[b] * c6;
+        out.dcn_j += wi[a] * dcj[b] * c6;
+        out.dq_i += dqi[a] * wj[b] * c6;
+        out.dq_j += wi[a] * dqj[b] * c6;
+      }
+    }
+  return out;
+}
+
+/* evaluate_d4_two_body_system_cpu (d4.cpp:896-1032) and its gradient (d4.cpp:1034-1118). */
+static double d4_two_body(const D4Geo* g, const double* q, double* vq, double* grad) {
+  const int nat = g->nat;
+  double* w = malloc(sizeof(double) * nat * XTBP_D4_MAXREF * 3);
+  double *dwc = w + nat * XTBP_D4_MAXREF, *dwq = dwc + nat * XTBP_D4_MAXREF;
+  double* dEdcn = calloc(nat, sizeof(double));
+  for (int a = 0; a < nat; ++a)
+    d4_weights(g->Z[a], g->cn[a], q[a], w + a * XTBP_D4_MAXREF, dwc + a * XTBP_D4_MAXREF,
+               dwq + a * XTBP_D4_MAXREF);
+  if (vq) for (int a = 0; a < nat; ++a) vq[a] = 0.0;
+  double e = 0.0;
+  for (int j = 1; j < nat; ++j)
+    for (int i = 0; i < j; ++i) {
+      const double* p = g->pair + 5 * pidx(i, j);
+      if (p[3] == 0.0) continue;
+      C6Pair c = d4_c6(g->Z[i], g->Z[j], w + i * XTBP_D4_MAXREF, w + j * XTBP_D4_MAXREF,
+                       dwc + i * XTBP_D4_MAXREF, dwc + j * XTBP_D4_MAXREF,
+                       dwq + i * XTBP_D4_MAXREF, dwq + j * XTBP_D4_MAXREF);
+      e -= c.c6 * p[3];
+      if (vq) { vq[i] -= c.dq_i * p[3]; vq[j] -= c.dq_j * p[3]; }
+      if (grad) {
+        for (int x = 0; x < 3; ++x) {
+          grad[3 * i + x] += -c.c6 * p[4] * p[x];
+          grad[3 * j + x] -= -c.c6 * p[4] * p[x];
+        }
+        dEdcn[i] -= c.dcn_i * p[3];
+        dEdcn[j] -= c.dcn_j * p[3];
+      }
+    }
+  if (grad) {
+    /* add_coordination_vjp (d4.cpp:424-453) */
+    const double isqpi = 0.5641895835477562869480794515607726;
+    for (int j = 1; j < nat; ++j)
+      for (int i = 0; i < j; ++i) {
+        size_t k = pidx(i, j);
+        const double* p = g->pair + 5 * k;
+        double r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        if (r2 > 900.0) continue;
+        double r = sqrt(r2), ex = 7.5 * (r - g->rcov[k]) / g->rcov[k];
+        double der = -g->enf[k] * 7.5 * exp(-ex * ex) * isqpi / g->rcov[k];
+        double sc = der * (dEdcn[i] + dEdcn[j]) / r;
+        for (int x = 0; x < 3; ++x) { grad[3 * i + x] += sc * p[x]; grad[3 * j + x] -= sc * p[x]; }
+      }
+  }
+  free(dEdcn);
+  free(w);
+  return e;
+}
+
+/* evaluate_d4_atm_cpu / add_d4_atm_gradient_cpu (d4.cpp:1120-1362), zero-charge weights. */
+static double d4_atm(const D4Geo* g, double* grad) {
+  const int nat = g->nat;
+  double* w = malloc(sizeof(double) * nat * XTBP_D4_MAXREF * 2 + 1);
+  double* dwc = w + nat * XTBP_D4_MAXREF;
+  double zero7[XTBP_D4_MAXREF];
+  for (int a = 0; a < nat; ++a)
+    d4_weights(g->Z[a], g->cn[a], 0.0, w + a * XTBP_D4_MAXREF, dwc + a * XTBP_D4_MAXREF, zero7);
+  size_t np = (size_t)nat * (nat - 1) / 2;
+  C6Pair* c6 = malloc(sizeof(C6Pair) * (np + 1));
+  for (int j = 1; j < nat; ++j)
+    for (int i = 0; i < j; ++i)
+      c6[pidx(i, j)] = d4_c6(g->Z[i], g->Z[j], w + i * XTBP_D4_MAXREF, w + j * XTBP_D4_MAXREF,
+                             dwc + i * XTBP_D4_MAXREF, dwc + j * XTBP_D4_MAXREF, zero7, zero7);
+  double* dEdcn = calloc(nat, sizeof(double));
+  double e = 0.0;
+  const double cut2 = 625.0, ex3 = 16.0 / 3.0;
+  for (int i = 2; i < nat; ++i)
+    for (int j = 1; j < i; ++j) {
+      const double* vij = g->pair + 5 * pidx(j, i);
+      double r2ij = vij[0] * vij[0] + vij[1] * vij[1] + vij[2] * vij[2];
+      if (r2ij > cut2) continue;
+      for (int k = 0; k < j; ++k) {
+        const double* vik = g->pair + 5 * pidx(k, i);
+        const double* vjk = g->pair + 5 * pidx(k, j);
+        double r2ik = vik[0] * vik[0] + vik[1] * vik[1] + vik[2] * vik[2];
+        double r2jk = vjk[0] * vjk[0] + vjk[1] * vjk[1] + vjk[2] * vjk[2];
+        if (r2ik > cut2 || r2jk > cut2) continue;
+        C6Pair cij = c6[pidx(j, i)], cik = c6[pidx(k, i)], cjk = c6[pidx(k, j)];
+        double r0 = g->r0[pidx(j, i)] * g->r0[pidx(k, i)] * g->r0[pidx(k, j)];
+        double r2p = r2ij * r2ik * r2jk, r1p = sqrt(r2p), r3p = r2p * r1p, r5p = r3p * r2p;
+        double rp = pow(r0 / r1p, ex3);
+        double damp = 1.0 / (1.0 + 6.0 * rp);
+        double ang = 0.375 * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik) * (-r2ij + r2jk + r2ik) / r5p +
+                     1.0 / r3p;
+        double c9 = -XTBP_DISP_S9 * sqrt(fabs(cij.c6 * cik.c6 * cjk.c6));
+        e -= ang * damp * c9;
+        if (!grad) continue;
+        double ddamp = -2.0 * 16.0 * rp * damp * damp;
+        double dg[3][3];
+        const double* vv[3] = {vij, vik, vjk};
+        double tg[3] = {r2ij, r2ik, r2jk}, o1[3] = {r2jk, r2jk, r2ik}, o2[3] = {r2ik, r2ij, r2ij};
+        for (int t = 0; t < 3; ++t) {
+          double T = tg[t], a = o1[t], b = o2[t];
+          double dang = -0.375 * (T * T * T + T * T * (a + b) + T * (3.0 * a * a + 2.0 * a * b + 3.0 * b * b) -
+                                  5.0 * (a - b) * (a - b) * (a + b)) / r5p;
+          double sc = c9 * (-dang * damp + ang * ddamp) / T;
+          for (int x = 0; x < 3; ++x) dg[t][x] = sc * vv[t][x];
+        }
+        for (int x = 0; x < 3; ++x) {
+          grad[3 * i + x] -= dg[0][x] + dg[1][x];
+          grad[3 * j + x] += dg[0][x] - dg[2][x];
+          grad[3 * k + x] += dg[1][x] + dg[2][x];
+        }
+        double de = ang * damp * c9;
+        /* pair_coefficient(i, j): first = i (its dcn_i slot); stored pairs are (lo, hi) */
+        dEdcn[i] -= 0.5 * de * (cij.dcn_j / cij.c6 + cik.dcn_j / cik.c6);
+        dEdcn[j] -= 0.5 * de * (cij.dcn_i / cij.c6 + cjk.dcn_j / cjk.c6);
+        dEdcn[k] -= 0.5 * de * (cik.dcn_i / cik.c6 + cjk.dcn_i / cjk.c6);
+      }
+    }
+  if (grad) {
+    const double isqpi = 0.5641895835477562869480794515607726;
+    for (int j = 1; j < nat; ++j)
+      for (int i = 0; i < j; ++i) {
+        size_t k = pidx(i, j);
+        const double* p = g->pair + 5 * k;
+        double r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        if (r2 > 900.0) continue;
+        double r = sqrt(r2), ex = 7.5 * (r - g->rcov[k]) / g->rcov[k];
+        double der = -g->enf[k] * 7.5 * exp(-ex * ex) * isqpi / g->rcov[k];
+        double sc = der * (dEdcn[i] + dEdcn[j]) / r;
+        for (int x = 0; x < 3; ++x) { grad[3 * i + x] += sc * p[x]; grad[3 * j + x] -= sc * p[x]; }
+      }
+  }
+  free(dEdcn);
+  free(c6);
+  free(w);
+  return e;
+}
+
+int oracle_d4_two_body(int32_t nat, const int32_t* Z, const double* xyz, const double* q,
+                       double* energy, double* potential, double* grad) {
+  D4Geo g;
+  d4_geometry(nat, Z, xyz, &g);
+  *energy = d4_two_body(&g, q, potential, grad);
+  d4_free(&g);
+  return ST_OK;
+}
+
+int oracle_d4_atm(int32_t nat, const int32_t* Z, const double* xyz, double* energy, double* grad) {
+  D4Geo g;
+  d4_geometry(nat, Z, xyz, &g);
+  *energy = d4_atm(&g, grad);
+  d4_free(&g);
+  return ST_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Repulsion (gfn2/repulsion.cpp:93-178)                              */
+/* ------------------------------------------------------------------ */
+double oracle_repulsion(int32_t nat, const int32_t* Z, const double* xyz, double* grad) {
+  double e = 0.0;
+  for (int a = 0; a < nat; ++a)
+    for (int c = 0; c < a; ++c) {
+      double d[3] = {xyz[3 * a] - xyz[3 * c], xyz[3 * a + 1] - xyz[3 * c + 1],
+                     xyz[3 * a + 2] - xyz[3 * c + 2]};
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (r2 > 625.0) continue;
+      double r = sqrt(r2);
+      int light = Z[a] <= 2 && Z[c] <= 2;
+      double k = light ? XTBP_REP_KLIGHT : XTBP_REP_KEXP;
+      double rk = light ? r : r * sqrt(r);
+      double alpha = sqrt(xtbp_elem_arep[Z[a] - 1]) * sqrt(xtbp_elem_arep[Z[c] - 1]);
+      double zz = xtbp_elem_zeff[Z[a] - 1] * xtbp_elem_zeff[Z[c] - 1];
+      double pe = zz * exp(-alpha * rk) / r;
+      e += pe;
+      if (grad) {
+        double f = (alpha * k * rk + 1.0) * pe / r2; /* force on a along d */
+        for (int x = 0; x < 3; ++x) { grad[3 * a + x] -= f * d[x]; grad[3 * c + x] += f * d[x]; }
+      }
+    }
+  return e;
+}
+
+/* ------------------------------------------------------------------ */
+/* Fermi occupations (gfn2/eigensolver.cpp:992-1161), long double bisection */
+/* ------------------------------------------------------------------ */
+static long double fermi_val(long double e, long double mu, long double t) {
+  long double a = (e - mu) / t;
+  if (a >= 0.0L) { long double x = expl(-a); return x / (1.0L + x); }
+  return 1.0L / (expl(a) + 1.0L);
+}
+static long double fermi_hole(long double e, long double mu, long double t) {
+  long double a = (e - mu) / t;
+  if (a >= 0.0L) return 1.0L / (expl(-a) + 1.0L);
+  long double x = expl(a);
+  return x / (1.0L + x);
+}
+static long double fermi_sum(const double* eps, int n, long double ref, long double mu,
+                             long double t, int holes) {
+  long double s = 0.0L;
+  for (int i = 0; i < n; ++i) {
+    long double e = (long double)eps[i] - ref;
+    s += holes ? fermi_hole(e, mu, t) : fermi_val(e, mu, t);
+  }
+  return s;
+}
+
+int oracle_fermi(int32_t n, const double* eps, double nel, double kt, double* occ, double* mu,
+                 double* entropy) {
+  *mu = 0.0;
+  *entropy = 0.0;
+  if (kt == 0.0) {
+    for (int i = 0; i < n; ++i) occ[i] = 0.0;
+    int full = (int)floor(nel);
+    if (full > n) full = n;
+    for (int i = 0; i < full; ++i) occ[i] = 1.0;
+    if (full < n) occ[full] = nel - full;
+    return 1;
+  }
+  if (nel == 0.0) { for (int i = 0; i < n; ++i) occ[i] = 0.0; return 1; }
+  long double target = nel, cap = n, t = kt;
+  if (target == cap) {
+    for (int i = 0; i < n; ++i) occ[i] = 1.0;
+    *mu = (double)((long double)eps[n - 1] + 50.0L * t);
+    return 1;
+  }
+  int holes = target > 0.5L * cap;
+  long double qt = holes ? cap - target : target;
+  long double steps = fmaxl(64.0L, -logl(qt / cap) + 8.0L);
+  long double ref = holes ? eps[n - 1] : eps[0];
+  long double lo0 = (long double)eps[0] - ref, hi0 = (long double)eps[n - 1] - ref;
+  long double scale = fmaxl(1.0L, fabsl(hi0 - lo0));
+  long double margin = steps * t + 64.0L * LDBL_EPSILON * scale;
+  long double lo = lo0 - margin, hi = hi0 + margin;
+  long double tol = 64.0L * (long double)DBL_EPSILON * qt;
+  for (int it = 0; it < 4096; ++it) {
+    long double mid = lo + 0.5L * (hi - lo);
+    long double q = fermi_sum(eps, n, ref, mid, t, holes);
+    if (fabsl(q - qt) <= tol) { lo = hi = mid; break; }
+    if (mid == lo || mid == hi) break;
+    if ((!holes && q < qt) || (holes && q > qt)) lo = mid; else hi = mid;
+  }
+  long double m = lo + 0.5L * (hi - lo);
+  *mu = (double)(ref + m);
+  long double s = 0.0L;
+  for (int i = 0; i < n; ++i) {
+    long double f = fermi_val((long double)eps[i] - ref, m, t);
+    occ[i] = (double)f;
+    long double fo = (long double)occ[i];
+    if (fo > 0.0L && fo < 1.0L) s -= fo * logl(fo) + (1.0L - fo) * logl(1.0L - fo);
+  }
+  *entropy = (double)s;
+  return 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Modified Broyden mixer (common/scc_mixer.cpp:500-745)              */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int dim, mem;
+  long iter;
+  double *cur, *prev, *prev_res, *df, *u, *omega; /* df,u: [mem*dim] */
+} Mixer;
+
+static int chol_solve(double* a, double* b, int n) {
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c <= r; ++c) {
+      double v = a[r * n + c];
+      for (int k = 0; k < c; ++k) v -= a[r * n + k] * a[c * n + k];
+      if (r == c) { if (!(v > 0.0)) return 0; a[r * n + c] = sqrt(v); }
+      else a[r * n + c] = v / a[c * n + c];
+    }
+  for (int r = 0; r < n; ++r) {
+    double v = b[r];
+    for (int c = 0; c < r; ++c) v -= a[r * n + c] * b[c];
+    b[r] = v / a[r * n + r];
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    double v = b[r];
+    for (int c = r + 1; c < n; ++c) v -= a[c * n + r] * b[c];
+    b[r] = v / a[r * n + r];
+  }
+  return 1;
+}
+
+/* raw -> mixed (in place into cur); returns residual rms, or NaN on failure */
+static double mixer_step(Mixer* m, const double* raw, double damp, double* out) {
+  const int n = m->dim;
+  double* res = malloc(sizeof(double) * n * 3);
+  double *dfn = res + n, *un = dfn + n;
+  double rs = 0.0;
+  for (int i = 0; i < n; ++i) { res[i] = raw[i] - m->cur[i]; rs += res[i] * res[i]; }
+  double rnorm = sqrt(rs), rms = rnorm / sqrt((double)n);
+  long newit = m->iter + 1;
+  if (newit == 1) {
+    for (int i = 0; i < n; ++i) out[i] = m->cur[i] + damp * res[i];
+  } else {
+    double d2 = 0.0;
+    for (int i = 0; i < n; ++i) { dfn[i] = res[i] - m->prev_res[i]; d2 += dfn[i] * dfn[i]; }
+    double den = fmax(sqrt(d2), DBL_EPSILON), inv = 1.0 / den;
+    for (int i = 0; i < n; ++i) {
+      dfn[i] *= inv;
+      un[i] = damp * dfn[i] + inv * (m->cur[i] - m->prev[i]);
+    }
+    double omega = rnorm > 0.01 / 100000.0 ? fmax(1.0, 0.01 / rnorm) : 100000.0;
+    int hc = (int)(m->mem < m->iter ? m->mem : m->iter);
+    long first = m->iter - hc + 1;
+    int slot_new = (int)((m->iter - 1) % m->mem);
+    int slots[64];
+    for (int h = 0; h < hc; ++h) slots[h] = (int)((first + h - 1) % m->mem);
+    double beta[64 * 64], coef[64];
+#define DFV(s) ((s) == slot_new ? dfn : m->df + (size_t)(s) * n)
+#define UV(s) ((s) == slot_new ? un : m->u + (size_t)(s) * n)
+#define OM(s) ((s) == slot_new ? omega : m->omega[s])
+    for (int r = 0; r < hc; ++r) {
+      const double* dr = DFV(slots[r]);
+      double dot = 0.0;
+      for (int i = 0; i < n; ++i) dot += dr[i] * res[i];
+      coef[r] = OM(slots[r]) * dot;
+      for (int c = 0; c < hc; ++c) {
+        const double* dc = DFV(slots[c]);
+        double ov = 0.0;
+        for (int i = 0; i < n; ++i) ov += dr[i] * dc[i];
+        double v = OM(slots[r]) * OM(slots[c]) * ov;
+        if (r == c) v += 0.01 * 0.01;
+        beta[r * hc + c] = v;
+      }
+    }
+    if (!chol_solve(beta, coef, hc)) { free(res); return NAN; }
+    for (int i = 0; i < n; ++i) {
+      double v = m->cur[i] + damp * res[i];
+      for (int h = 0; h < hc; ++h) v -= OM(slots[h]) * coef[h] * UV(slots[h])[i];
+      out[i] = v;
+    }
+    memcpy(m->df + (size_t)slot_new * n, dfn, sizeof(double) * n);
+    memcpy(m->u + (size_t)slot_new * n, un, sizeof(double) * n);
+    m->omega[slot_new] = omega;
+#undef DFV
+#undef UV
+#undef OM
+  }
+  memcpy(m->prev, m->cur, sizeof(double) * n);
+  memcpy(m->prev_res, res, sizeof(double) * n);
+  memcpy(m->cur, out, sizeof(double) * n);
+  m->iter = newit;
+  free(res);
+  return rms;
+}
+
+/* Drive the mixer with raw = current + residual[step] (tests/scc_mixer_test.cpp:254-319 contract). */
+int oracle_broyden(int32_t dim, int32_t memory, double damping, const double* initial,
+                   int32_t steps, const double* residuals, double* mixed_out) {
+  Mixer m;
+  memset(&m, 0, sizeof(m));
+  m.dim = dim; m.mem = memory;
+  m.cur = calloc(dim, sizeof(double)); m.prev = calloc(dim, sizeof(double));
+  m.prev_res = calloc(dim, sizeof(double));
+  m.df = calloc((size_t)dim * memory, sizeof(double));
+  m.u = calloc((size_t)dim * memory, sizeof(double));
+  m.omega = calloc(memory, sizeof(double));
+  memcpy(m.cur, initial, sizeof(double) * dim);
+  double* raw = malloc(sizeof(double) * dim);
+  int st = ST_OK;
+  for (int k = 0; k < steps && st == ST_OK; ++k) {
+    for (int i = 0; i < dim; ++i) raw[i] = m.cur[i] + residuals[(size_t)k * dim + i];
+    if (!isfinite(mixer_step(&m, raw, damping, mixed_out + (size_t)k * dim))) st = ST_INTERNAL;
+  }
+  free(raw); free(m.cur); free(m.prev); free(m.prev_res); free(m.df); free(m.u); free(m.omega);
+  return st;
+}
+
+/* ------------------------------------------------------------------ */
+/* Single point (gfn2_cpu_execution.cpp:1218-1601, scc_driver.cpp:706-1020, force.cpp:336-640) */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  Basis b;
+  int n, ns, nat;
+  double *S, *D, *Q, *H0, *G, *Vpc, *cn;
+  double *H, *C, *eps, *P, *W, *occ;
+  double *L;         /* Cholesky factor, column-major lower */
+  AesPair* aes;
+  D4Geo d4;
+} System;
+
+static void mulliken(const System* sy, const double* P, double* qsh, double* qat, double* dip,
+                     double* quad) {
+  const Basis* b = &sy->b;
+  const int n = sy->n;
+  const size_t nn = (size_t)n * n;
+  for (int s = 0; s < sy->ns; ++s) qsh[s] = 0.0;
+  for (int a = 0; a < sy->nat; ++a) {
+    for (int x = 0; x < 3; ++x) dip[3 * a + x] = 0.0;
+    for (int c = 0; c < 6; ++c) quad[6 * a + c] = 0.0;
+  }
+  /* q_sh = n0 - sum P*S over the ket AO's shell; d/Q over the ket AO's atom (mulliken.hpp:205-210) */
+  for (int r = 0; r < n; ++r)
+    for (int k = 0; k < n; ++k) {
+      size_t idx = (size_t)r * n + k;
+      const int s = b->ao_sh[k], a = b->ao_atom[k];
+      qsh[s] -= P[idx] * sy->S[idx];
+      for (int x = 0; x < 3; ++x) dip[3 * a + x] -= P[idx] * sy->D[x * nn + idx];
+      for (int c = 0; c < 6; ++c) quad[6 * a + c] -= P[idx] * sy->Q[c * nn + idx];
+    }
+  for (int a = 0; a < sy->nat; ++a) qat[a] = 0.0;
+  for (int s = 0; s < sy->ns; ++s) {
+    qsh[s] += xtbp_shell_refocc[b->sh_par[s]];
+    qat[b->sh_atom[s]] += qsh[s];
+  }
+}
+
+/* potentials from (mixed or raw) multipoles; vsh includes the atom potential */
+static void potentials(const System* sy, const double* qsh, const double* qat, const double* dip,
+                       const double* quad, double* vsh, double* vd, double* vqq, double* vat_scratch,
+                       double* d4v) {
+  const Basis* b = &sy->b;
+  const int ns = sy->ns;
+  for (int s = 0; s < ns; ++s) {
+    double v = 0.0;
+    for (int t = 0; t < ns; ++t) v += sy->G[s * ns + t] * qsh[t];
+    v += es3_gamma(b, s) * qsh[s] * qsh[s];
+    if (sy->Vpc) v += sy->Vpc[s];
+    vsh[s] = v;
+  }
+  aes2_potential(sy->nat, b->Z, sy->aes, qat, dip, quad, vat_scratch, vd, vqq);
+  d4_two_body(&sy->d4, qat, d4v, NULL);
+  for (int a = 0; a < sy->nat; ++a) vat_scratch[a] += d4v[a];
+  for (int s = 0; s < ns; ++s) vsh[s] += vat_scratch[b->sh_atom[s]];
+}
+
+/* H = H0 - S(u_m+u_n)/2 - D_mn v_D(A_n)/2 - D_nm v_D(A_m)/2 - (same for Q) (mulliken.hpp:222-237) */
+static void fock(const System* sy, const double* vsh, const double* vd, const double* vqq,
+                 double* H) {
+  const Basis* b = &sy->b;
+  const int n = sy->n;
+  const size_t nn = (size_t)n * n;
+  for (int r = 0; r < n; ++r) {
+    const int sr = b->ao_sh[r], ar = b->ao_atom[r];
+    for (int c = 0; c < n; ++c) {
+      const int sc = b->ao_sh[c], ac = b->ao_atom[c];
+      size_t f = (size_t)r * n + c, rv = (size_t)c * n + r;
+      double h = -0.5 * sy->S[f] * (vsh[sr] + vsh[sc]);
+      for (int x = 0; x < 3; ++x)
+        h -= 0.5 * (sy->D[x * nn + f] * vd[3 * ac + x] + sy->D[x * nn + rv] * vd[3 * ar + x]);
+      for (int x = 0; x < 6; ++x)
+        h -= 0.5 * (sy->Q[x * nn + f] * vqq[6 * ac + x] + sy->Q[x * nn + rv] * vqq[6 * ar + x]);
+      H[f] = sy->H0[f] + h;
+    }
+  }
+}
+
+/* solve_one_spin + occupations + density (eigensolver.cpp:1221-1400) */
+static int solve(System* sy, double nalpha, double nbeta, double kt, double* entropy) {
+  const int n = sy->n;
+  double* A = sy->C;
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) A[(size_t)c * n + r] = sy->H[(size_t)r * n + c];
+  la_dtrsm(kColMajor, kLeft, kLower, kNoTrans, kNonUnit, n, n, 1.0, sy->L, n, A, n);
+  la_dtrsm(kColMajor, kRight, kLower, kCblasTrans, kNonUnit, n, n, 1.0, sy->L, n, A, n);
+  for (int c = 0; c < n; ++c)
+    for (int r = c + 1; r < n; ++r) {
+      double avg = 0.5 * (A[r + (size_t)c * n] + A[c + (size_t)r * n]);
+      A[r + (size_t)c * n] = avg;
+      A[c + (size_t)r * n] = avg;
+    }
+  int lwork = 1 + 6 * n + 2 * n * n, liwork = 3 + 5 * n;
+  double* work = malloc(sizeof(double) * lwork);
+  int* iwork = malloc(sizeof(int) * liwork);
+  int info = la_dsyevd(kColMajor, 'V', 'L', n, A, n, sy->eps, work, lwork, iwork, liwork);
+  free(work);
+  free(iwork);
+  if (info != 0) return 0;
+  la_dtrsm(kColMajor, kLeft, kLower, kCblasTrans, kNonUnit, n, n, 1.0, sy->L, n, A, n);
+  double mu, sa, sb;
+  double* fa = sy->occ;
+  double* fb = sy->occ + n;
+  if (!oracle_fermi(n, sy->eps, nalpha, kt, fa, &mu, &sa)) return 0;
+  if (!oracle_fermi(n, sy->eps, nbeta, kt, fb, &mu, &sb)) return 0;
+  *entropy = sa + sb;
+  double* wc = malloc(sizeof(double) * (size_t)n * n);
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int o = 0; o < n; ++o) {
+      double wgt = fa[o] + fb[o];
+      if (pass == 1) wgt *= sy->eps[o];
+      for (int r = 0; r < n; ++r) wc[r + (size_t)o * n] = A[r + (size_t)o * n] * wgt;
+    }
+    /* P and W are symmetric, so column-major equals row-major */
+    la_dgemm(kColMajor, kNoTrans, kCblasTrans, n, n, n, 1.0, wc, n, A, n, 0.0, pass ? sy->W : sy->P, n);
+  }
+  free(wc);
+  return 1;
+}
+
+static void system_free(System* sy) {
+  free(sy->S); free(sy->D); free(sy->Q); free(sy->H0); free(sy->G); free(sy->Vpc); free(sy->cn);
+  free(sy->H); free(sy->C); free(sy->eps); free(sy->P); free(sy->W); free(sy->occ); free(sy->L);
+  free(sy->aes);
+  if (sy->d4.cn) d4_free(&sy->d4);
+  basis_free(&sy->b);
+}
+
+/* geometry refresh (gfn2_cpu_execution.cpp:1218-1262) + overlap factor (eigensolver.cpp:2335-2420) */
+static int system_setup(System* sy, int32_t nat, const int32_t* Z, const double* xyz, double charge,
+                        int32_t unpaired, int32_t npc, const double* pc_xyz, const double* pc_q,
+                        const double* pc_gamma, double* nalpha, double* nbeta) {
+  memset(sy, 0, sizeof(*sy));
+  int st = basis_make(nat, Z, &sy->b);
+  if (st != ST_OK) return st;
+  const int n = sy->b.nao, ns = sy->b.nsh;
+  sy->n = n; sy->ns = ns; sy->nat = nat;
+  const size_t nn = (size_t)n * n;
+  for (int a = 0; a < 3 * nat; ++a) if (!isfinite(xyz[a])) return ST_INVALID;
+  /* electron counts (wavefunction.cpp:614-636) */
+  double nel0 = 0.0;
+  for (int s = 0; s < ns; ++s) nel0 += xtbp_shell_refocc[sy->b.sh_par[s]];
+  double nel = nel0 - charge;
+  long nint = lround(nel);
+  if (nel < 0.0 || ((nint & 1) != (unpaired & 1))) return ST_INVALID;
+  *nalpha = 0.5 * (nel + unpaired);
+  *nbeta = 0.5 * (nel - unpaired);
+  sy->S = malloc(sizeof(double) * nn); sy->D = malloc(sizeof(double) * 3 * nn);
+  sy->Q = malloc(sizeof(double) * 6 * nn); sy->H0 = malloc(sizeof(double) * nn);
+  sy->G = malloc(sizeof(double) * ns * ns); sy->cn = malloc(sizeof(double) * nat);
+  sy->H = malloc(sizeof(double) * nn); sy->C = malloc(sizeof(double) * nn);
+  sy->eps = malloc(sizeof(double) * n); sy->P = malloc(sizeof(double) * nn);
+  sy->W = malloc(sizeof(double) * nn); sy->occ = malloc(sizeof(double) * 2 * n);
+  sy->L = malloc(sizeof(double) * nn);
+  sy->aes = malloc(sizeof(AesPair) * ((size_t)nat * (nat - 1) / 2 + 1));
+  if (npc > 0) sy->Vpc = malloc(sizeof(double) * ns);
+  st = gfn2_cn(nat, Z, xyz, sy->cn);
+  if (st != ST_OK) return st;
+  build_integrals(&sy->b, xyz, sy->S, sy->D, sy->Q);
+  build_h0(&sy->b, xyz, sy->cn, sy->S, sy->H0);
+  build_gamma(&sy->b, xyz, sy->G);
+  aes2_pairs(nat, Z, xyz, sy->cn, sy->aes);
+  d4_geometry(nat, Z, xyz, &sy->d4);
+  if (npc > 0) pc_potential(&sy->b, xyz, npc, pc_xyz, pc_q, pc_gamma, sy->Vpc);
+  double anorm = 0.0;
+  for (int c = 0; c < n; ++c) {
+    double s = 0.0;
+    for (int r = 0; r < n; ++r) s += fabs(sy->S[(size_t)r * n + c]);
+    if (s > anorm) anorm = s;
+  }
+  memcpy(sy->L, sy->S, sizeof(double) * nn);
+  int info = la_dpotrf(kColMajor, 'L', n, sy->L, n);
+  double rcond = 0.0;
+  if (info == 0) {
+    double* work = malloc(sizeof(double) * 3 * n);
+    int* iwork = malloc(sizeof(int) * n);
+    info = la_dpocon(kColMajor, 'L', n, sy->L, n, anorm, &rcond, work, iwork);
+    free(work); free(iwork);
+  }
+  for (int c = 0; c < n; ++c)
+    for (int r = 0; r < c; ++r) sy->L[r + (size_t)c * n] = 0.0;
+  if (info != 0 || !(rcond >= 1e-12)) return ST_EIGEN_FAILED;
+  return ST_OK;
+}
+
+/*
+ * One SCC step from the mixed state x = [qsh, dip(3N), quad(6N)] (scc_driver.cpp:706-860):
+ * potentials -> H -> eigensolve -> P -> raw Mulliken multipoles -> F = E - kT S.
+ */
+static int scc_step(System* sy, const double* x, double nalpha, double nbeta, double kt,
+                    double* raw, double* qat_scratch, double* F, double ecomp[8]) {
+  const int ns = sy->ns, nat = sy->nat;
+  const size_t nn = (size_t)sy->n * sy->n;
+  const double *qsh = x, *dip = x + ns, *quad = x + ns + 3 * nat;
+  double* vsh = malloc(sizeof(double) * (ns + 11 * nat));
+  double *vd = vsh + ns, *vqq = vd + 3 * nat, *vat = vqq + 6 * nat, *d4v = vat + nat;
+  for (int a = 0; a < nat; ++a) qat_scratch[a] = 0.0;
+  for (int s = 0; s < ns; ++s) qat_scratch[sy->b.sh_atom[s]] += qsh[s];
+  potentials(sy, qsh, qat_scratch, dip, quad, vsh, vd, vqq, vat, d4v);
+  fock(sy, vsh, vd, vqq, sy->H);
+  free(vsh);
+  double entropy = 0.0;
+  if (!solve(sy, nalpha, nbeta, kt, &entropy)) return ST_EIGEN_FAILED;
+  double *rq = raw, *rd = raw + ns, *rQ = raw + ns + 3 * nat;
+  mulliken(sy, sy->P, rq, qat_scratch, rd, rQ);
+  /* evaluate_scc_energy_system (scc_driver.cpp:2105-2286) */
+  double ecore = 0.0;
+  for (size_t k = 0; k < nn; ++k) ecore = fma(sy->H0[k], sy->P[k], ecore);
+  double ees2 = 0.0, ees3 = 0.0, epc = 0.0;
+  for (int s = 0; s < ns; ++s) {
+    double v = 0.0;
+    for (int t = 0; t < ns; ++t) v += sy->G[s * ns + t] * rq[t];
+    ees2 += 0.5 * rq[s] * v;
+    ees3 += es3_gamma(&sy->b, s) * rq[s] * rq[s] * rq[s] / 3.0;
+    if (sy->Vpc) epc = fma(rq[s], sy->Vpc[s], epc);
+  }
+  double eaes = aes2_energy(nat, sy->b.Z, sy->aes, qat_scratch, rd, rQ);
+  double ed4 = d4_two_body(&sy->d4, qat_scratch, NULL, NULL);
+  double e_int = ecore + ees2 + ees3 + eaes + ed4 + epc;
+  *F = fma(-kt, entropy, e_int);
+  ecomp[0] = ecore; ecomp[1] = ees2; ecomp[2] = ees3; ecomp[3] = eaes; ecomp[4] = ed4; ecomp[5] = epc;
+  return ST_OK;
+}
+
+int oracle_scc_replay(int32_t nat, const int32_t* Z, const double* xyz, double charge,
+                      int32_t unpaired, int32_t npc, const double* pc_xyz, const double* pc_q,
+                      const double* pc_gamma, double kt, const double* mixed, double* raw,
+                      double* hamiltonian, double* density, double* free_energy) {
+  if (!la_dsyevd) return ST_INVALID;
+  System sy;
+  double na, nb, ecomp[8];
+  int st = system_setup(&sy, nat, Z, xyz, charge, unpaired, npc, pc_xyz, pc_q, pc_gamma, &na, &nb);
+  if (st == ST_OK) {
+    double* qat = malloc(sizeof(double) * nat);
+    st = scc_step(&sy, mixed, na, nb, kt, raw, qat, free_energy, ecomp);
+    free(qat);
+    if (st == ST_OK) {
+      const size_t nn = (size_t)sy.n * sy.n;
+      if (hamiltonian) memcpy(hamiltonian, sy.H, sizeof(double) * nn);
+      if (density) memcpy(density, sy.P, sizeof(double) * nn);
+    }
+  }
+  system_free(&sy);
+  return st;
+}
+
+int oracle_singlepoint(int32_t nat, const int32_t* Z, const double* xyz, double charge,
+                       int32_t unpaired, int32_t npc, const double* pc_xyz, const double* pc_q,
+                       const double* pc_gamma, const oracle_options* opt, double* energy,
+                       double* grad_out, double* atomic_charges, double* pc_grad, double* dedq,
+                       int32_t* iterations, oracle_trace* tr) {
+  if (!la_dsyevd) return ST_INVALID;
+  oracle_options dflt;
+  if (!opt) { oracle_default_options(&dflt); opt = &dflt; }
+  System sy;
+  double nalpha = 0.0, nbeta = 0.0;
+  int status = system_setup(&sy, nat, Z, xyz, charge, unpaired, npc, pc_xyz, pc_q, pc_gamma,
+                            &nalpha, &nbeta);
+  if (status == ST_INVALID || sy.S == NULL) { system_free(&sy); return status == ST_OK ? ST_INVALID : status; }
+  const int n = sy.n, ns = sy.ns;
+  const size_t nn = (size_t)n * n;
+
+  /* FRESH SAD state (wavefunction.cpp:733-776) */
+  const int dim = ns + 9 * nat;
+  double* x = calloc(dim, sizeof(double));
+  double* raw = calloc(dim, sizeof(double));
+  double* qat = calloc(nat, sizeof(double));
+  {
+    double qa = charge / nat;
+    for (int a = 0; a < nat; ++a) {
+      double ref = 0.0;
+      for (int s = sy.b.atom_sh[a]; s < sy.b.atom_sh[a + 1]; ++s) ref += xtbp_shell_refocc[sy.b.sh_par[s]];
+      for (int s = sy.b.atom_sh[a]; s < sy.b.atom_sh[a + 1]; ++s)
+        x[s] = (xtbp_shell_refocc[sy.b.sh_par[s]] / ref) * qa;
+    }
+  }
+  Mixer mx;
+  memset(&mx, 0, sizeof(mx));
+  mx.dim = dim; mx.mem = opt->history;
+  mx.cur = calloc(dim, sizeof(double)); mx.prev = calloc(dim, sizeof(double));
+  mx.prev_res = calloc(dim, sizeof(double));
+  mx.df = calloc((size_t)dim * mx.mem, sizeof(double));
+  mx.u = calloc((size_t)dim * mx.mem, sizeof(double));
+  mx.omega = calloc(mx.mem, sizeof(double));
+  memcpy(mx.cur, x, sizeof(double) * dim);
+
+  double* mixed = malloc(sizeof(double) * dim);
+  double f_old = 0.0, F = 0.0, ecomp[8] = {0};
+  int it = status == ST_EIGEN_FAILED ? 1 : 0, converged = 0;
+  while (status == ST_OK && !converged && it < opt->max_iterations) {
+    status = scc_step(&sy, x, nalpha, nbeta, opt->kt, raw, qat, &F, ecomp);
+    if (status != ST_OK) { it++; break; }
+    double rms = mixer_step(&mx, raw, opt->damping, mixed);
+    if (!isfinite(rms)) { status = ST_INTERNAL; it++; break; }
+    double dF = F - (it == 0 ? 0.0 : f_old);
+    converged = rms < opt->charge_tolerance && fabs(dF) < opt->energy_tolerance;
+    if (tr && it < tr->capacity) {
+      if (tr->free_energy) tr->free_energy[it] = F;
+      if (tr->residual_rms) tr->residual_rms[it] = rms;
+    }
+    it++;
+    memcpy(x, converged ? raw : mixed, sizeof(double) * dim);
+    f_old = F;
+  }
+  free(mixed);
+  if (status == ST_OK && !converged) status = ST_NOT_CONVERGED;
+  if (tr) {
+    tr->recorded = it < tr->capacity ? it : tr->capacity;
+    if (tr->overlap) memcpy(tr->overlap, sy.S, sizeof(double) * nn);
+    if (tr->h0) memcpy(tr->h0, sy.H0, sizeof(double) * nn);
+    if (tr->dipole) memcpy(tr->dipole, sy.D, sizeof(double) * 3 * nn);
+    if (tr->quadrupole) memcpy(tr->quadrupole, sy.Q, sizeof(double) * 6 * nn);
+    if (tr->coordination) memcpy(tr->coordination, sy.cn, sizeof(double) * nat);
+  }
+  if (iterations) *iterations = it;
+
+  if (status == ST_OK) {
+    const double *qsh = x, *dip = x + ns, *quad = x + ns + 3 * nat;
+    double* vsh = malloc(sizeof(double) * (ns + 11 * nat));
+    double *vd = vsh + ns, *vqq = vd + 3 * nat, *vat = vqq + 6 * nat, *d4v = vat + nat;
+    for (int a = 0; a < nat; ++a) qat[a] = 0.0;
+    for (int s = 0; s < ns; ++s) qat[sy.b.sh_atom[s]] += qsh[s];
+    if (atomic_charges) memcpy(atomic_charges, qat, sizeof(double) * nat);
+    if (tr) {
+      if (tr->density) memcpy(tr->density, sy.P, sizeof(double) * nn);
+      if (tr->shell_charges) memcpy(tr->shell_charges, qsh, sizeof(double) * ns);
+      if (tr->dipoles) memcpy(tr->dipoles, dip, sizeof(double) * 3 * nat);
+      if (tr->quadrupoles) memcpy(tr->quadrupoles, quad, sizeof(double) * 6 * nat);
+    }
+    double* g = calloc(3 * nat, sizeof(double));
+    double erep = oracle_repulsion(nat, Z, xyz, g);
+    double eatm = d4_atm(&sy.d4, g);
+    ecomp[6] = erep; ecomp[7] = eatm;
+    if (energy) *energy = F + erep + eatm;
+    if (tr && tr->components) memcpy(tr->components, ecomp, sizeof(ecomp));
+
+    /* stationary potentials at the converged raw multipoles (gfn2_cpu_execution.cpp:1358-1441) */
+    potentials(&sy, qsh, qat, dip, quad, vsh, vd, vqq, vat, d4v);
+    /* force composer (force.cpp:501-612) */
+    double* aS = calloc(nn, sizeof(double));
+    double* aD = calloc(3 * nn, sizeof(double));
+    double* aQ = calloc(6 * nn, sizeof(double));
+    double* dEdcn = calloc(nat, sizeof(double));
+    h0_vjp(&sy.b, xyz, sy.cn, sy.S, sy.P, aS, dEdcn, g);
+    for (size_t k = 0; k < nn; ++k) aS[k] -= sy.W[k];
+    for (int r = 0; r < n; ++r) {
+      const int sr = sy.b.ao_sh[r], ar = sy.b.ao_atom[r];
+      for (int c = r; c < n; ++c) {
+        const int sc = sy.b.ao_sh[c], ac = sy.b.ao_atom[c];
+        size_t f = (size_t)r * n + c, rv = (size_t)c * n + r;
+        double pd = sy.P[f] + (f == rv ? 0.0 : sy.P[rv]);
+        aS[f] += pd * (-0.5 * (vsh[sr] + vsh[sc]));
+        for (int k = 0; k < 3; ++k) {
+          aD[k * nn + f] += -0.5 * pd * vd[3 * ac + k];
+          aD[k * nn + rv] += -0.5 * pd * vd[3 * ar + k];
+        }
+        for (int k = 0; k < 6; ++k) {
+          aQ[k * nn + f] += -0.5 * pd * vqq[6 * ac + k];
+          aQ[k * nn + rv] += -0.5 * pd * vqq[6 * ar + k];
+        }
+      }
+    }
+    contract_integral_gradient(&sy.b, xyz, aS, aD, aQ, g);
+    gfn2_cn_gradient(nat, Z, xyz, dEdcn, g);
+    /* ES2 gradient at fixed q (es2.cpp:989) */
+    for (int s = 0; s < ns; ++s) {
+      const int A = sy.b.sh_atom[s];
+      for (int t = 0; t < ns; ++t) {
+        const int C = sy.b.sh_atom[t];
+        if (A == C) continue;
+        double gm = sy.G[s * ns + t];
+        double sc = -0.5 * qsh[s] * qsh[t] * gm * gm * gm;
+        for (int k = 0; k < 3; ++k) {
+          double d = xyz[3 * A + k] - xyz[3 * C + k];
+          g[3 * A + k] += sc * d;
+          g[3 * C + k] -= sc * d;
+        }
+      }
+    }
+    for (int a = 0; a < nat; ++a) dEdcn[a] = 0.0;
+    aes2_vjp(nat, Z, sy.aes, sy.cn, qat, dip, quad, g, dEdcn);
+    gfn2_cn_gradient(nat, Z, xyz, dEdcn, g);
+    d4_two_body(&sy.d4, qat, NULL, g);
+    /* point charges (external_point_charges.cpp:389-491); dE/dQ_p is new (no reference) */
+    if (npc > 0) {
+      if (pc_grad) for (int k = 0; k < 3 * npc; ++k) pc_grad[k] = 0.0;
+      if (dedq) for (int p = 0; p < npc; ++p) dedq[p] = 0.0;
+      for (int s = 0; s < ns; ++s) {
+        const int A = sy.b.sh_atom[s];
+        for (int p = 0; p < npc; ++p) {
+          double d[3] = {xyz[3 * A] - pc_xyz[3 * p], xyz[3 * A + 1] - pc_xyz[3 * p + 1],
+                         xyz[3 * A + 2] - pc_xyz[3 * p + 2]};
+          double ia = 2.0 / (shell_hardness(&sy.b, s) + pc_gamma[p]);
+          double rs = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + ia * ia);
+          if (dedq) dedq[p] += qsh[s] / rs;
+          if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) continue;
+          double fs = qsh[s] * pc_q[p] / (rs * rs * rs);
+          for (int k = 0; k < 3; ++k) {
+            g[3 * A + k] -= fs * d[k];
+            if (pc_grad) pc_grad[3 * p + k] += fs * d[k];
+          }
+        }
+      }
+    }
+    if (grad_out) memcpy(grad_out, g, sizeof(double) * 3 * nat);
+    free(aS); free(aD); free(aQ); free(dEdcn); free(g); free(vsh);
+  }
+  free(x); free(raw); free(qat);
+  free(mx.cur); free(mx.prev); free(mx.prev_res); free(mx.df); free(mx.u); free(mx.omega);
+  system_free(&sy);
+  return status;
+}
+
+/* ------------------------------------------------------------------ */
+/* Batch driver: one system per worker (gfn2_cpu_execution.cpp:1855-1997) */
+/* ------------------------------------------------------------------ */
+#include <pthread.h>
+#include <unistd.h>
+
+typedef struct {
+  int64_t batch;
+  const int64_t *aoff, *poff;
+  const int32_t *Z, *uhf;
+  const double *xyz, *chg, *pxyz, *pq, *pg;
+  const oracle_options* opt;
+  double *e, *g, *q, *pgr;
+  int32_t *iters, *status;
+  long next;
+  pthread_mutex_t lock;
+} BatchJob;
+
+static void* batch_worker(void* arg) {
+  BatchJob* job = (BatchJob*)arg;
+  for (;;) {
+    pthread_mutex_lock(&job->lock);
+    long s = job->next++;
+    pthread_mutex_unlock(&job->lock);
+    if (s >= job->batch) break;
+    int64_t a0 = job->aoff[s], a1 = job->aoff[s + 1];
+    int64_t p0 = job->poff ? job->poff[s] : 0, p1 = job->poff ? job->poff[s + 1] : 0;
+    int nat = (int)(a1 - a0), npc = (int)(p1 - p0);
+    double e = NAN;
+    int32_t it = 0;
+    int st = oracle_singlepoint(nat, job->Z + a0, job->xyz + 3 * a0, job->chg[s], job->uhf[s], npc,
+                                npc ? job->pxyz + 3 * p0 : NULL, npc ? job->pq + p0 : NULL,
+                                npc ? job->pg + p0 : NULL, job->opt, &e,
+                                job->g ? job->g + 3 * a0 : NULL, job->q ? job->q + a0 : NULL,
+                                (job->pgr && npc) ? job->pgr + 3 * p0 : NULL, NULL, &it, NULL);
+    if (st != ST_OK) {
+      e = NAN;
+      if (job->g) for (int k = 0; k < 3 * nat; ++k) job->g[3 * a0 + k] = NAN;
+      if (job->q) for (int k = 0; k < nat; ++k) job->q[a0 + k] = NAN;
+      if (job->pgr) for (int k = 0; k < 3 * npc; ++k) job->pgr[3 * p0 + k] = NAN;
+    }
+    if (job->e) job->e[s] = e;
+    if (job->iters) job->iters[s] = it;
+    if (job->status) job->status[s] = st;
+  }
+  return NULL;
+}
+
+int oracle_batch(int64_t batch, const int64_t* atom_offsets, const int32_t* Z, const double* xyz,
+                 const double* charges, const int32_t* unpaired, const int64_t* point_offsets,
+                 const double* pc_xyz, const double* pc_q, const double* pc_gamma,
+                 const oracle_options* options, int32_t threads, double* energies, double* grad,
+                 double* atomic_charges, double* pc_grad, int32_t* iterations, int32_t* status) {
+  if (!la_dsyevd) return ST_INVALID;
+  BatchJob job = {batch, atom_offsets, point_offsets, Z, unpaired, xyz, charges, pc_xyz, pc_q,
+                  pc_gamma, options, energies, grad, atomic_charges, pc_grad, iterations, status,
+                  0, PTHREAD_MUTEX_INITIALIZER};
+  if (threads <= 0) threads = (int32_t)sysconf(_SC_NPROCESSORS_ONLN);
+  if (threads > 256) threads = 256;
+  pthread_t tid[256];
+  for (int t = 0; t < threads; ++t) pthread_create(&tid[t], NULL, batch_worker, &job);
+  for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+  return ST_OK;
+}

@@ -1,0 +1,236 @@
+"""ctypes binding of the C oracle (``liboracle_gfn2.so``).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, ``__graft_entry__.smoke()`` and
+the cpu_baseline leg of ``bench.py`` as the checker / CPU baseline; the
+product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import glob
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_gfn2.so")
+
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int32_p = ctypes.POINTER(ctypes.c_int32)
+_c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_int32), ("history", ctypes.c_int32),
+                ("charge_tolerance", ctypes.c_double), ("energy_tolerance", ctypes.c_double),
+                ("kt", ctypes.c_double), ("damping", ctypes.c_double)]
+
+
+class Trace(ctypes.Structure):
+    _fields_ = [("capacity", ctypes.c_int32), ("recorded", ctypes.c_int32)] + [
+        (name, _c_double_p) for name in (
+            "free_energy", "residual_rms", "overlap", "h0", "dipole", "quadrupole", "density",
+            "shell_charges", "dipoles", "quadrupoles", "coordination", "components")]
+
+
+def lapack_path() -> str:
+    import scipy
+    hits = sorted(glob.glob(os.path.join(os.path.dirname(scipy.__file__), "..", "scipy.libs",
+                                         "libscipy_openblas*.so")))
+    if not hits:
+        raise RuntimeError("scipy-openblas LP64 provider not found")
+    return os.path.abspath(hits[0])
+
+
+_LIB = None
+
+
+def build() -> str:
+    """Compile the oracle with gcc (idempotent; called by __graft_entry__.build)."""
+    import subprocess
+    src = os.path.join(HERE, "gfn2_oracle.c")
+    cmd = ["gcc", "-O2", "-std=c11", "-D_GNU_SOURCE", "-fPIC", "-shared", "-o", LIB_PATH, src,
+           "-ldl", "-lpthread", "-lm"]
+    subprocess.check_call(cmd)
+    return LIB_PATH
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_init.argtypes = [ctypes.c_char_p]
+        if L.oracle_init(lapack_path().encode()) != 0:
+            raise RuntimeError("oracle could not load the LAPACK provider")
+        L.oracle_singlepoint.argtypes = [
+            ctypes.c_int32, _c_int32_p, _c_double_p, ctypes.c_double, ctypes.c_int32,
+            ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p, ctypes.POINTER(Options),
+            _c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_int32_p,
+            ctypes.POINTER(Trace)]
+        L.oracle_batch.argtypes = [
+            ctypes.c_int64, _c_int64_p, _c_int32_p, _c_double_p, _c_double_p, _c_int32_p,
+            _c_int64_p, _c_double_p, _c_double_p, _c_double_p, ctypes.POINTER(Options),
+            ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_int32_p,
+            _c_int32_p]
+        L.oracle_scc_replay.argtypes = [
+            ctypes.c_int32, _c_int32_p, _c_double_p, ctypes.c_double, ctypes.c_int32,
+            ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p, ctypes.c_double, _c_double_p,
+            _c_double_p, _c_double_p, _c_double_p, _c_double_p]
+        L.oracle_broyden.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                     _c_double_p, ctypes.c_int32, _c_double_p, _c_double_p]
+        L.oracle_basis_counts.argtypes = [ctypes.c_int32, _c_int32_p, _c_int32_p, _c_int32_p,
+                                          _c_double_p]
+        L.oracle_coordination.argtypes = [ctypes.c_int32, _c_int32_p, _c_double_p, _c_double_p]
+        L.oracle_integrals.argtypes = [ctypes.c_int32, _c_int32_p, _c_double_p, _c_double_p,
+                                       _c_double_p, _c_double_p]
+        L.oracle_h0.argtypes = [ctypes.c_int32, _c_int32_p, _c_double_p, _c_double_p, _c_double_p]
+        L.oracle_repulsion.argtypes = [ctypes.c_int32, _c_int32_p, _c_double_p, _c_double_p]
+        L.oracle_repulsion.restype = ctypes.c_double
+        L.oracle_d4_two_body.argtypes = [ctypes.c_int32, _c_int32_p, _c_double_p, _c_double_p,
+                                         _c_double_p, _c_double_p, _c_double_p]
+        L.oracle_d4_atm.argtypes = [ctypes.c_int32, _c_int32_p, _c_double_p, _c_double_p,
+                                    _c_double_p]
+        L.oracle_fermi.argtypes = [ctypes.c_int32, _c_double_p, ctypes.c_double, ctypes.c_double,
+                                   _c_double_p, _c_double_p, _c_double_p]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_c_double_p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def options(max_iterations=250, charge_tolerance=1e-6, energy_tolerance=1e-8,
+            kt=300.0 * 3.166808578545117e-6, history=8, damping=0.4) -> Options:
+    return Options(max_iterations, history, charge_tolerance, energy_tolerance, kt, damping)
+
+
+@dataclass
+class Result:
+    status: int
+    energy: float
+    gradient: np.ndarray
+    charges: np.ndarray
+    iterations: int
+    pc_gradient: np.ndarray | None = None
+    dedq: np.ndarray | None = None
+    trace: dict | None = None
+
+
+def basis_counts(Z):
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    nsh, nao, nel = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+    st = lib().oracle_basis_counts(len(Z), _p(Z, _c_int32_p), ctypes.byref(nsh),
+                                   ctypes.byref(nao), ctypes.byref(nel))
+    if st != 0:
+        raise ValueError("unsupported element")
+    return nsh.value, nao.value, nel.value
+
+
+def singlepoint(Z, xyz, charge=0.0, uhf=0, pc_xyz=None, pc_q=None, pc_gamma=None, opts=None,
+                trace=False, trace_capacity=512) -> Result:
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    nat = len(Z)
+    npc = 0 if pc_q is None else len(pc_q)
+    if npc:
+        pc_xyz = np.ascontiguousarray(pc_xyz, dtype=np.float64).reshape(-1, 3)
+        pc_q = np.ascontiguousarray(pc_q, dtype=np.float64)
+        pc_gamma = np.ascontiguousarray(pc_gamma, dtype=np.float64)
+    e = ctypes.c_double(np.nan)
+    g = np.zeros((nat, 3))
+    q = np.zeros(nat)
+    pg = np.zeros((npc, 3)) if npc else None
+    dq = np.zeros(npc) if npc else None
+    it = ctypes.c_int32(0)
+    tr = None
+    bufs = {}
+    if trace:
+        nsh, nao, _ = basis_counts(Z)
+        nn = nao * nao
+        shapes = dict(free_energy=(trace_capacity,), residual_rms=(trace_capacity,),
+                      overlap=(nao, nao), h0=(nao, nao), dipole=(3, nao, nao),
+                      quadrupole=(6, nao, nao), density=(nao, nao), shell_charges=(nsh,),
+                      dipoles=(nat, 3), quadrupoles=(nat, 6), coordination=(nat,),
+                      components=(8,))
+        bufs = {k: np.zeros(s) for k, s in shapes.items()}
+        tr = Trace(trace_capacity, 0, *[_p(bufs[k]) for k in (
+            "free_energy", "residual_rms", "overlap", "h0", "dipole", "quadrupole", "density",
+            "shell_charges", "dipoles", "quadrupoles", "coordination", "components")])
+    o = opts or options()
+    st = lib().oracle_singlepoint(nat, _p(Z, _c_int32_p), _p(xyz), float(charge), int(uhf), npc,
+                                  _p(pc_xyz), _p(pc_q), _p(pc_gamma), ctypes.byref(o),
+                                  ctypes.byref(e), _p(g), _p(q), _p(pg), _p(dq), ctypes.byref(it),
+                                  ctypes.byref(tr) if tr is not None else None)
+    if trace:
+        bufs["recorded"] = tr.recorded
+        bufs["free_energy"] = bufs["free_energy"][:tr.recorded]
+        bufs["residual_rms"] = bufs["residual_rms"][:tr.recorded]
+    return Result(st, e.value, g, q, it.value, pg, dq, bufs if trace else None)
+
+
+def broyden(initial, residuals, memory=8, damping=0.4):
+    initial = np.ascontiguousarray(initial, dtype=np.float64)
+    residuals = np.ascontiguousarray(residuals, dtype=np.float64)
+    out = np.zeros_like(residuals)
+    st = lib().oracle_broyden(len(initial), memory, damping, _p(initial), len(residuals),
+                              _p(residuals), _p(out))
+    if st != 0:
+        raise RuntimeError("mixer failed")
+    return out
+
+
+def scc_replay(Z, xyz, mixed, charge=0.0, uhf=0, pc_xyz=None, pc_q=None, pc_gamma=None,
+               kt=300.0 * 3.166808578545117e-6):
+    """One SCC step from the mixed state [qsh, dip(3N), quad(6N)]; returns (raw, H, P, F)."""
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    mixed = np.ascontiguousarray(mixed, dtype=np.float64)
+    nsh, nao, _ = basis_counts(Z)
+    npc = 0 if pc_q is None else len(pc_q)
+    if npc:
+        pc_xyz = np.ascontiguousarray(pc_xyz, dtype=np.float64)
+        pc_q = np.ascontiguousarray(pc_q, dtype=np.float64)
+        pc_gamma = np.ascontiguousarray(pc_gamma, dtype=np.float64)
+    raw = np.zeros_like(mixed)
+    H = np.zeros((nao, nao))
+    P = np.zeros((nao, nao))
+    F = ctypes.c_double()
+    st = lib().oracle_scc_replay(len(Z), _p(Z, _c_int32_p), _p(xyz), float(charge), int(uhf), npc,
+                                 _p(pc_xyz), _p(pc_q), _p(pc_gamma), float(kt), _p(mixed), _p(raw),
+                                 _p(H), _p(P), ctypes.byref(F))
+    if st != 0:
+        raise RuntimeError(f"oracle replay failed with status {st}")
+    return raw, H, P, F.value
+
+
+def batch(atom_offsets, Z, xyz, charges, uhf, point_offsets=None, pc_xyz=None, pc_q=None,
+          pc_gamma=None, opts=None, threads=0, want_point_gradient=False):
+    atom_offsets = np.ascontiguousarray(atom_offsets, dtype=np.int64)
+    B = len(atom_offsets) - 1
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    charges = np.ascontiguousarray(charges, dtype=np.float64)
+    uhf = np.ascontiguousarray(uhf, dtype=np.int32)
+    e = np.zeros(B)
+    g = np.zeros((len(Z), 3))
+    q = np.zeros(len(Z))
+    it = np.zeros(B, dtype=np.int32)
+    st = np.zeros(B, dtype=np.int32)
+    pg = None
+    if point_offsets is not None:
+        point_offsets = np.ascontiguousarray(point_offsets, dtype=np.int64)
+        pc_xyz = np.ascontiguousarray(pc_xyz, dtype=np.float64)
+        pc_q = np.ascontiguousarray(pc_q, dtype=np.float64)
+        pc_gamma = np.ascontiguousarray(pc_gamma, dtype=np.float64)
+        if want_point_gradient:
+            pg = np.zeros((len(pc_q), 3))
+    o = opts or options()
+    lib().oracle_batch(B, _p(atom_offsets, _c_int64_p), _p(Z, _c_int32_p), _p(xyz), _p(charges),
+                       _p(uhf, _c_int32_p), _p(point_offsets, _c_int64_p), _p(pc_xyz), _p(pc_q),
+                       _p(pc_gamma), ctypes.byref(o), int(threads), _p(e), _p(g), _p(q), _p(pg),
+                       _p(it, _c_int32_p), _p(st, _c_int32_p))
+    return dict(energies=e, gradient=g, charges=q, iterations=it, status=st, pc_gradient=pg)
