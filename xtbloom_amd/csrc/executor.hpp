@@ -1,0 +1,71 @@
+// Host interface of the MI355X GFN2 executor (used by the C ABI layer).
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+
+namespace xtb {
+
+struct HostBatch {
+  int64_t batch = 0;
+  int64_t atoms = 0;
+  int64_t points = 0;
+  const int64_t* atom_offsets = nullptr;  // [B+1]
+  const int32_t* Z = nullptr;             // [N]
+  const double* xyz = nullptr;            // [3N]
+  const double* charges = nullptr;        // [B]
+  const int32_t* unpaired = nullptr;      // [B]
+  const int64_t* point_offsets = nullptr; // [B+1] or null
+  const double* pc_xyz = nullptr;         // [3P]
+  const double* pc_q = nullptr;           // [P]
+  const double* pc_gamma = nullptr;       // [P]
+};
+
+struct RunOptions {
+  int max_iterations = 250;
+  double charge_tolerance = 1e-6;
+  double energy_tolerance = 1e-8;
+  double kt = 300.0 * 3.166808578545117e-6;
+  int history = 8;
+  double damping = 0.4;
+  bool forces = true;
+};
+
+// Every pointer is optional (host memory); failed systems receive NaN slices.
+struct HostOutputs {
+  double* energies = nullptr;       // [B]
+  double* gradient = nullptr;       // [3N] dE/dR
+  double* charges = nullptr;        // [N]
+  double* pc_gradient = nullptr;    // [3P] dE/dR_point
+  double* pc_dedq = nullptr;        // [P]  dE/dQ_point
+  double* dipoles = nullptr;        // [3B]
+  int32_t* iterations = nullptr;    // [B]
+  uint8_t* converged = nullptr;     // [B]
+  int32_t* status = nullptr;        // [B]
+  // diagnostics (packed over the batch)
+  double* dbg_overlap = nullptr;    // [sum n^2]
+  double* dbg_h0 = nullptr;         // [sum n^2]
+  double* dbg_density = nullptr;    // [sum n^2]
+  double* dbg_cn = nullptr;         // [N]
+  double* dbg_components = nullptr; // [B*8] core es2 es3 aes2 d4 pc rep atm
+};
+
+struct Device;
+
+// Timing of the last run (milliseconds of device time per phase).
+struct RunTiming {
+  float setup_ms = 0.f, scc_ms = 0.f, force_ms = 0.f, total_ms = 0.f;
+  int scc_rounds = 0;
+};
+
+Device* device_create(int device_id, void* stream, std::string& error);
+void device_destroy(Device* dev);
+int device_id(const Device* dev);
+// returns an xtbloom status code; error receives a message on failure
+int run_batch(Device* dev, const HostBatch& batch, const RunOptions& options, HostOutputs& out,
+              RunTiming* timing, std::string& error);
+// Packed-matrix element count and AO count limit checks used by validation.
+int64_t batch_matrix_elements(const HostBatch& batch, int32_t* max_ao, std::string& error);
+
+}  // namespace xtb

@@ -1,0 +1,766 @@
+// MI355X (gfx950) GFN2-xTB batched single-point executor: host orchestration.
+//
+// One call = one ragged batch.  The host packs the topology once (basis,
+// offsets, per-atom/shell parameters), uploads it into a grow-only device arena
+// owned by the context, and drives the device kernels:
+//   geometry -> integrals/H0 -> Cholesky/inverse -> SCC rounds -> forces.
+// Each SCC round runs over all still-active systems; converged systems drop
+// out through a device-side active mask.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <vector>
+
+#include "../data/gfn2_tables.h"
+#include "executor.hpp"
+#include "gfn2_device.hpp"
+#include "gfn2_integrals.hpp"
+#include "gfn2_linalg.hpp"
+#include "gfn2_mix_force.hpp"
+#include "gfn2_scc.hpp"
+
+namespace xtb {
+
+namespace {
+
+constexpr double kEvToHartree = 1.0 / 27.21138505;
+constexpr double kAngstromToBohr = 1.8897261246204404;
+
+#define HIP_TRY(expr)                                                           \
+  do {                                                                          \
+    hipError_t _e = (expr);                                                     \
+    if (_e != hipSuccess) {                                                     \
+      error = std::string("HIP error ") + hipGetErrorString(_e) + " at " #expr; \
+      return 6;                                                                 \
+    }                                                                           \
+  } while (0)
+
+// Stewart STO-nG selection (reference common/sto.cpp:20-68).
+bool sto_row(int n, int l, int ng, const double** a, const double** c) {
+  if (n == 6 && ng == 6) {
+    if (l == 0) { *a = xtbp_sto_alpha6s; *c = xtbp_sto_coeff6s; return true; }
+    if (l == 1) { *a = xtbp_sto_alpha6p; *c = xtbp_sto_coeff6p; return true; }
+    return false;
+  }
+  if (n <= 0 || n > 5 || l >= n || l > 4) return false;
+  const int type = l == 0 ? n - 1 : l == 1 ? n + 3 : l == 2 ? n + 6 : l == 3 ? n + 8 : n + 9;
+  if (type >= 15) return false;
+  if (ng == 3) { *a = xtbp_sto_alpha3 + 3 * type; *c = xtbp_sto_coeff3 + 3 * type; return true; }
+  if (ng == 4) { *a = xtbp_sto_alpha4 + 4 * type; *c = xtbp_sto_coeff4 + 4 * type; return true; }
+  if (ng == 6) { *a = xtbp_sto_alpha6 + 6 * type; *c = xtbp_sto_coeff6 + 6 * type; return true; }
+  return false;
+}
+
+template <typename T>
+size_t align_bytes(size_t n) {
+  return ((n * sizeof(T)) + 255) & ~size_t(255);
+}
+
+// Host-side packed topology; every vector is uploaded verbatim.
+struct HostTopo {
+  int B = 0, N = 0, NS = 0, NAO = 0, NP = 0, NPC = 0;
+  std::vector<int> atom_off, shell_off, ao_off, vec_off, pc_off, order;
+  std::vector<int64_t> mat_off, gam_off, apair_off, sp_off;
+  std::vector<double> nalpha, nbeta, charges;
+  std::vector<int> Z, atom_sys, atom_sh0, at_d4ref, at_d4nref;
+  std::vector<double> xyz, at_cnrad, at_h0rad, at_en, at_dk, at_qk, at_mprad, at_mpvcn,
+      at_sqrt_arep, at_zeff, at_d4rcov, at_d4en, at_d4zeff, at_d4hard, at_d4r4r2;
+  std::vector<int> sh_atom, sh_l, sh_ao, sh_prim, sh_nprim;
+  std::vector<double> sh_level, sh_kcn, sh_kpoly, sh_zeta, sh_refocc, sh_eta, sh_gam3;
+  std::vector<int> ao_sh, ao_atom;
+  std::vector<double> alpha, coef;
+  std::vector<double> pc_xyz, pc_q, pc_gam;
+  std::vector<int> setup_status;
+  int max_ao = 0;
+};
+
+int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
+  const int B = (int)hb.batch;
+  T.B = B;
+  T.atom_off.assign(B + 1, 0);
+  T.shell_off.assign(B + 1, 0);
+  T.ao_off.assign(B + 1, 0);
+  T.vec_off.assign(B + 1, 0);
+  T.pc_off.assign(B + 1, 0);
+  T.mat_off.assign(B + 1, 0);
+  T.gam_off.assign(B + 1, 0);
+  T.apair_off.assign(B + 1, 0);
+  T.sp_off.assign(B + 1, 0);
+  T.nalpha.assign(B, 0.0);
+  T.nbeta.assign(B, 0.0);
+  T.charges.assign(hb.charges, hb.charges + B);
+  T.setup_status.assign(B, 0);
+  static const double dfac[5] = {1.0, 1.0, 3.0, 15.0, 105.0};
+  for (int s = 0; s < B; ++s) {
+    const int64_t a0 = hb.atom_offsets[s], a1 = hb.atom_offsets[s + 1];
+    T.atom_off[s] = (int)a0;
+    int nsh = 0, nao = 0;
+    double refel = 0.0;
+    for (int64_t a = a0; a < a1; ++a) {
+      const int z = hb.Z[a];
+      if (z < 1 || z > XTBP_NELEM) {
+        error = "atomic number outside the GFN2 parameter domain (1..86)";
+        return 1;
+      }
+      const int off = xtbp_elem_shell_offset[z - 1], cnt = xtbp_elem_shell_count[z - 1];
+      const int a_idx = (int)a;
+      if ((int)T.Z.size() != a_idx) { error = "internal atom ordering error"; return 6; }
+      T.Z.push_back(z);
+      T.atom_sys.push_back(s);
+      T.atom_sh0.push_back((int)T.sh_atom.size());
+      for (int k = 0; k < 3; ++k) T.xyz.push_back(hb.xyz[3 * a + k]);
+      T.at_cnrad.push_back(xtbp_covalent_radius_aa[z - 1] * (4.0 / 3.0) * kAngstromToBohr);
+      T.at_h0rad.push_back(xtbp_atomic_radius_aa[z - 1] * kAngstromToBohr);
+      T.at_en.push_back(xtbp_elem_en[z - 1]);
+      T.at_dk.push_back(xtbp_elem_dkernel[z - 1]);
+      T.at_qk.push_back(xtbp_elem_qkernel[z - 1]);
+      T.at_mprad.push_back(xtbp_elem_mprad[z - 1]);
+      T.at_mpvcn.push_back(xtbp_elem_mpvcn[z - 1]);
+      T.at_sqrt_arep.push_back(std::sqrt(xtbp_elem_arep[z - 1]));
+      T.at_zeff.push_back(xtbp_elem_zeff[z - 1]);
+      T.at_d4rcov.push_back(xtbp_d4_covalent_radius[z - 1]);
+      T.at_d4en.push_back(xtbp_d4_en[z - 1]);
+      T.at_d4zeff.push_back(xtbp_d4_zeff[z - 1]);
+      T.at_d4hard.push_back(xtbp_d4_hardness[z - 1]);
+      T.at_d4r4r2.push_back(xtbp_d4_r4r2[z - 1]);
+      T.at_d4ref.push_back(xtbp_d4_ref_offset[z - 1]);
+      T.at_d4nref.push_back(xtbp_d4_ref_count[z - 1]);
+      int seen[5] = {0, 0, 0, 0, 0};
+      for (int k = 0; k < cnt; ++k) {
+        const int p = off + k, l = xtbp_shell_l[p], ng = xtbp_shell_ngauss[p];
+        if (l > 2 || seen[l]) {
+          error = "element with f shells or repeated angular momenta is not supported";
+          return 3;
+        }
+        seen[l] = 1;
+        const double* ta;
+        const double* tc;
+        if (!sto_row(xtbp_shell_n[p], l, ng, &ta, &tc)) {
+          error = "missing STO-nG table row";
+          return 6;
+        }
+        const int gsh = (int)T.sh_atom.size();
+        T.sh_atom.push_back(a_idx);
+        T.sh_l.push_back(l);
+        T.sh_ao.push_back((int)T.ao_sh.size());
+        T.sh_prim.push_back((int)T.alpha.size());
+        T.sh_nprim.push_back(ng);
+        T.sh_level.push_back(xtbp_shell_level_ev[p] * kEvToHartree);
+        T.sh_kcn.push_back(xtbp_shell_kcn_ev[p] * kEvToHartree);
+        T.sh_kpoly.push_back(xtbp_shell_kpoly[p]);
+        T.sh_zeta.push_back(xtbp_shell_slater[p]);
+        T.sh_refocc.push_back(xtbp_shell_refocc[p]);
+        T.sh_eta.push_back(xtbp_elem_gam[z - 1] * xtbp_shell_hubbard_scale[p]);
+        const double s3[3] = {XTBP_ES3_SCALE_S, XTBP_ES3_SCALE_P, XTBP_ES3_SCALE_D};
+        T.sh_gam3.push_back(xtbp_elem_gam3[z - 1] * s3[l]);
+        const double z2 = xtbp_shell_slater[p] * xtbp_shell_slater[p];
+        for (int g = 0; g < ng; ++g) {
+          const double al = ta[g] * z2;
+          const double norm = std::pow(2.0 / kPi * al, 0.75) * std::pow(std::sqrt(4.0 * al), (double)l) /
+                              std::sqrt(dfac[l]);
+          T.alpha.push_back(al);
+          T.coef.push_back(tc[g] * norm);
+        }
+        for (int m = 0; m < 2 * l + 1; ++m) {
+          T.ao_sh.push_back(gsh);
+          T.ao_atom.push_back(a_idx);
+        }
+        nsh += 1;
+        nao += 2 * l + 1;
+        refel += xtbp_shell_refocc[p];
+      }
+    }
+    const int N = (int)(a1 - a0);
+    if (N <= 0) {
+      error = "every batch member needs at least one atom";
+      return 1;
+    }
+    if (nao > kMaxAO) {
+      error = "system exceeds the 1024-AO limit of the MI355X eigensolver";
+      return 3;
+    }
+    T.max_ao = std::max(T.max_ao, nao);
+    T.shell_off[s + 1] = T.shell_off[s] + nsh;
+    T.ao_off[s + 1] = T.ao_off[s] + nao;
+    T.vec_off[s + 1] = T.vec_off[s] + nsh + 9 * N;
+    T.mat_off[s + 1] = T.mat_off[s] + (int64_t)nao * nao;
+    T.gam_off[s + 1] = T.gam_off[s] + (int64_t)nsh * nsh;
+    T.apair_off[s + 1] = T.apair_off[s] + (int64_t)N * N;
+    T.sp_off[s + 1] = T.sp_off[s] + (int64_t)nsh * nsh;
+    // electron counts (reference wavefunction.cpp:614-636)
+    const double nel = refel - hb.charges[s];
+    const long nint = std::lround(nel);
+    if (!(nel >= 0.0) || ((nint & 1) != (hb.unpaired[s] & 1)) || hb.unpaired[s] < 0) {
+      error = "total electron count and unpaired-electron count have incompatible parity";
+      return 1;
+    }
+    T.nalpha[s] = 0.5 * (nel + hb.unpaired[s]);
+    T.nbeta[s] = 0.5 * (nel - hb.unpaired[s]);
+    if (T.nalpha[s] > nao || T.nbeta[s] > nao) {
+      error = "electron count exceeds the orbital space";
+      return 1;
+    }
+    T.pc_off[s + 1] = hb.point_offsets ? (int)hb.point_offsets[s + 1] : 0;
+  }
+  T.atom_off[B] = (int)hb.atom_offsets[B];
+  T.atom_sh0.push_back((int)T.sh_atom.size());
+  T.N = (int)T.Z.size();
+  T.NS = (int)T.sh_atom.size();
+  T.NAO = (int)T.ao_sh.size();
+  T.NP = (int)T.alpha.size();
+  T.NPC = T.pc_off[B];
+  if (T.NPC > 0) {
+    T.pc_xyz.assign(hb.pc_xyz, hb.pc_xyz + 3 * (size_t)T.NPC);
+    T.pc_q.assign(hb.pc_q, hb.pc_q + T.NPC);
+    T.pc_gam.assign(hb.pc_gamma, hb.pc_gamma + T.NPC);
+  }
+  // size-sorted visiting order (largest AO count first)
+  T.order.resize(B);
+  std::iota(T.order.begin(), T.order.end(), 0);
+  std::stable_sort(T.order.begin(), T.order.end(), [&](int a, int b) {
+    return (T.ao_off[a + 1] - T.ao_off[a]) > (T.ao_off[b + 1] - T.ao_off[b]);
+  });
+  return 0;
+}
+
+}  // namespace
+
+struct Device {
+  int id = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  // persistent parameter tables
+  double* d4_c6 = nullptr;
+  double* d4_refcn = nullptr;
+  double* d4_refq = nullptr;
+  int* d4_refng = nullptr;
+  double* kpair = nullptr;
+  // grow-only arena
+  char* arena = nullptr;
+  size_t arena_bytes = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+Device* device_create(int device_id, void* stream, std::string& error) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    error = "no ROCm device is visible";
+    return nullptr;
+  }
+  if (device_id < 0) device_id = 0;
+  if (device_id >= count) {
+    error = "device_id is out of range";
+    return nullptr;
+  }
+  if (hipSetDevice(device_id) != hipSuccess) {
+    error = "hipSetDevice failed";
+    return nullptr;
+  }
+  Device* d = new Device();
+  d->id = device_id;
+  if (stream) {
+    d->stream = static_cast<hipStream_t>(stream);
+  } else {
+    if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+      error = "hipStreamCreate failed";
+      delete d;
+      return nullptr;
+    }
+    d->own_stream = true;
+  }
+  bool ok = hipMalloc(&d->d4_c6, sizeof(double) * 262 * 262) == hipSuccess &&
+            hipMalloc(&d->d4_refcn, sizeof(double) * 262) == hipSuccess &&
+            hipMalloc(&d->d4_refq, sizeof(double) * 262) == hipSuccess &&
+            hipMalloc(&d->d4_refng, sizeof(int) * 262) == hipSuccess &&
+            hipMalloc(&d->kpair, sizeof(double) * 9) == hipSuccess;
+  ok = ok && hipMemcpy(d->d4_c6, xtbp_d4_c6, sizeof(double) * 262 * 262, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(d->d4_refcn, xtbp_d4_ref_cn, sizeof(double) * 262, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(d->d4_refq, xtbp_d4_ref_q, sizeof(double) * 262, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(d->d4_refng, xtbp_d4_ref_ngauss, sizeof(int) * 262, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(d->kpair, xtbp_kpair, sizeof(double) * 9, hipMemcpyHostToDevice) == hipSuccess;
+  for (int i = 0; i < 4 && ok; ++i) ok = hipEventCreate(&d->ev[i]) == hipSuccess;
+  if (!ok) {
+    error = "failed to allocate or upload the device parameter tables";
+    device_destroy(d);
+    return nullptr;
+  }
+  return d;
+}
+
+void device_destroy(Device* d) {
+  if (!d) return;
+  hipSetDevice(d->id);
+  if (d->stream) hipStreamSynchronize(d->stream);
+  hipFree(d->d4_c6);
+  hipFree(d->d4_refcn);
+  hipFree(d->d4_refq);
+  hipFree(d->d4_refng);
+  hipFree(d->kpair);
+  hipFree(d->arena);
+  for (auto& e : d->ev)
+    if (e) hipEventDestroy(e);
+  if (d->own_stream && d->stream) hipStreamDestroy(d->stream);
+  delete d;
+}
+
+int device_id(const Device* d) { return d ? d->id : -1; }
+
+int64_t batch_matrix_elements(const HostBatch& hb, int32_t* max_ao, std::string& error) {
+  int64_t total = 0;
+  int32_t mx = 0;
+  for (int64_t s = 0; s < hb.batch; ++s) {
+    int64_t nao = 0;
+    for (int64_t a = hb.atom_offsets[s]; a < hb.atom_offsets[s + 1]; ++a) {
+      const int z = hb.Z[a];
+      if (z < 1 || z > XTBP_NELEM) {
+        error = "atomic number outside the GFN2 parameter domain (1..86)";
+        return -1;
+      }
+      const int off = xtbp_elem_shell_offset[z - 1], cnt = xtbp_elem_shell_count[z - 1];
+      for (int k = 0; k < cnt; ++k) nao += 2 * xtbp_shell_l[off + k] + 1;
+    }
+    mx = std::max<int32_t>(mx, (int32_t)nao);
+    total += nao * nao;
+  }
+  if (max_ao) *max_ao = mx;
+  return total;
+}
+
+namespace {
+
+// Carves typed sub-buffers out of the device arena.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t n) {
+    T* p = reinterpret_cast<T*>(base + off);
+    off += align_bytes<T>(std::max<size_t>(n, 1));
+    return p;
+  }
+};
+
+template <typename T>
+size_t plan_bytes(size_t n) { return align_bytes<T>(std::max<size_t>(n, 1)); }
+
+__global__ void k_init_active(int B, const SysState* st, unsigned char* active, int* iters_fix,
+                              SysState* stw) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B) return;
+  const bool ok = st[s].status == 0;
+  active[s] = ok ? 1 : 0;
+  if (!ok) stw[s].iterations = 1;  // a failed overlap factorization is reported after one round
+  (void)iters_fix;
+}
+
+__global__ void k_valid(int B, const SysState* st, unsigned char* valid) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < B) valid[s] = (st[s].status == 0 && st[s].converged) ? 1 : 0;
+}
+
+}  // namespace
+
+int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutputs& out,
+              RunTiming* timing, std::string& error) {
+  if (hipSetDevice(dev->id) != hipSuccess) {
+    error = "hipSetDevice failed";
+    return 6;
+  }
+  HostTopo T;
+  int st = build_topology(hb, T, error);
+  if (st != 0) return st;
+  const int B = T.B, N = T.N, NS = T.NS, NAO = T.NAO, NPC = T.NPC;
+  const int64_t NN = T.mat_off[B];
+  const int64_t NG = T.gam_off[B];
+  const int64_t NAP = T.apair_off[B];
+  const int64_t NSP = T.sp_off[B];
+  const int NV = T.vec_off[B];
+  const int mem = std::max(1, opt.history);
+
+  // GEMM tile queues
+  std::vector<TileTask> full, lower;
+  for (int i = 0; i < B; ++i) {
+    const int s = T.order[i];
+    const int n = T.ao_off[s + 1] - T.ao_off[s];
+    const int nt = (n + kTile - 1) / kTile;
+    for (int a = 0; a < nt; ++a)
+      for (int b = 0; b < nt; ++b) {
+        full.push_back({s, a, b});
+        if (a >= b) lower.push_back({s, a, b});
+      }
+  }
+
+  // ---- arena layout ----
+  size_t bytes = 0;
+  auto addi = [&](size_t n) { bytes += plan_bytes<int>(n); };
+  auto addl = [&](size_t n) { bytes += plan_bytes<int64_t>(n); };
+  auto addd = [&](size_t n) { bytes += plan_bytes<double>(n); };
+  auto addb = [&](size_t n) { bytes += plan_bytes<unsigned char>(n); };
+  // topology
+  addi(B + 1); addi(B + 1); addi(B + 1); addi(B + 1); addi(B + 1); addi(B);
+  addl(B + 1); addl(B + 1); addl(B + 1); addl(B + 1);
+  addd(B); addd(B); addd(B);
+  addi(N); addi(N); addi(N + 1); addi(N); addi(N);
+  addd(3 * (size_t)N);
+  for (int k = 0; k < 15; ++k) addd(N);
+  for (int k = 0; k < 5; ++k) addi(NS);
+  for (int k = 0; k < 7; ++k) addd(NS);
+  addi(NAO); addi(NAO);
+  addd(T.NP); addd(T.NP);
+  addd(3 * (size_t)NPC); addd(NPC); addd(NPC);
+  bytes += plan_bytes<TileTask>(full.size()) + plan_bytes<TileTask>(lower.size());
+  // work arrays
+  const int kMat = 20;  // S, D3, Q6, H0, X, H, A, Z, Y, C, P, W, T
+  addd((size_t)kMat * NN);
+  addd(NG);                    // gamma
+  for (int k = 0; k < 7; ++k) addd(N);  // cn cnd4 mrad dmrad qat vat dedcn
+  addd(N);                     // dedcn4
+  addd((size_t)kD4MaxRef * N); addd((size_t)kD4MaxRef * N);
+  addd(NS); addd(NS);          // vpc, vsh
+  addd(3 * (size_t)N); addd(6 * (size_t)N);
+  for (int k = 0; k < 5; ++k) addd(NAO);  // eps fa fb occ fe
+  addd((size_t)NAO * 10);      // mulliken column partials
+  for (int k = 0; k < 5; ++k) addd(NV); // x raw cur prev prev_res
+  addd((size_t)mem * NV); addd((size_t)mem * NV);  // df, u
+  addd((size_t)mem * B);       // omega
+  bytes += plan_bytes<SysState>(B);
+  addb(B); addb(B);            // active, valid
+  addi(2);                     // counter
+  addi(B);                     // kocc
+  addd(3 * (size_t)N);         // grad
+  addd(3 * (size_t)NPC); addd(NPC);
+  addd(B); addd(B);            // erep, eatm
+  addd(2 * (size_t)NAP);       // ATM pair C6 staging
+  if (bytes > dev->arena_bytes) {
+    hipFree(dev->arena);
+    dev->arena = nullptr;
+    dev->arena_bytes = 0;
+    const size_t want = bytes + bytes / 4;
+    if (hipMalloc(&dev->arena, want) != hipSuccess) {
+      error = "device arena allocation failed";
+      return 4;
+    }
+    dev->arena_bytes = want;
+  }
+  Carver cv{dev->arena};
+  hipStream_t sm = dev->stream;
+  auto up = [&](auto* dst, const auto& vec) -> hipError_t {
+    if (vec.empty()) return hipSuccess;
+    return hipMemcpyAsync(dst, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice, sm);
+  };
+  Topo t{};
+  t.B = B;
+  int* d_atom_off = cv.take<int>(B + 1);
+  int* d_shell_off = cv.take<int>(B + 1);
+  int* d_ao_off = cv.take<int>(B + 1);
+  int* d_vec_off = cv.take<int>(B + 1);
+  int* d_pc_off = cv.take<int>(B + 1);
+  int* d_order = cv.take<int>(B);
+  int64_t* d_mat_off = cv.take<int64_t>(B + 1);
+  int64_t* d_gam_off = cv.take<int64_t>(B + 1);
+  int64_t* d_apair_off = cv.take<int64_t>(B + 1);
+  int64_t* d_sp_off = cv.take<int64_t>(B + 1);
+  double* d_nalpha = cv.take<double>(B);
+  double* d_nbeta = cv.take<double>(B);
+  double* d_charges = cv.take<double>(B);
+  int* d_Z = cv.take<int>(N);
+  int* d_atom_sys = cv.take<int>(N);
+  int* d_atom_sh0 = cv.take<int>(N + 1);
+  int* d_d4ref = cv.take<int>(N);
+  int* d_d4nref = cv.take<int>(N);
+  double* d_xyz = cv.take<double>(3 * (size_t)N);
+  double* at[15];
+  for (int k = 0; k < 15; ++k) at[k] = cv.take<double>(N);
+  int* shi[5];
+  for (int k = 0; k < 5; ++k) shi[k] = cv.take<int>(NS);
+  double* shd[7];
+  for (int k = 0; k < 7; ++k) shd[k] = cv.take<double>(NS);
+  int* d_ao_sh = cv.take<int>(NAO);
+  int* d_ao_atom = cv.take<int>(NAO);
+  double* d_alpha = cv.take<double>(T.NP);
+  double* d_coef = cv.take<double>(T.NP);
+  double* d_pcxyz = cv.take<double>(3 * (size_t)NPC);
+  double* d_pcq = cv.take<double>(NPC);
+  double* d_pcg = cv.take<double>(NPC);
+  TileTask* d_full = cv.take<TileTask>(full.size());
+  TileTask* d_lower = cv.take<TileTask>(lower.size());
+  HIP_TRY(up(d_atom_off, T.atom_off));
+  HIP_TRY(up(d_shell_off, T.shell_off));
+  HIP_TRY(up(d_ao_off, T.ao_off));
+  HIP_TRY(up(d_vec_off, T.vec_off));
+  HIP_TRY(up(d_pc_off, T.pc_off));
+  HIP_TRY(up(d_order, T.order));
+  HIP_TRY(up(d_mat_off, T.mat_off));
+  HIP_TRY(up(d_gam_off, T.gam_off));
+  HIP_TRY(up(d_apair_off, T.apair_off));
+  HIP_TRY(up(d_sp_off, T.sp_off));
+  HIP_TRY(up(d_nalpha, T.nalpha));
+  HIP_TRY(up(d_nbeta, T.nbeta));
+  HIP_TRY(up(d_charges, T.charges));
+  HIP_TRY(up(d_Z, T.Z));
+  HIP_TRY(up(d_atom_sys, T.atom_sys));
+  HIP_TRY(up(d_atom_sh0, T.atom_sh0));
+  HIP_TRY(up(d_d4ref, T.at_d4ref));
+  HIP_TRY(up(d_d4nref, T.at_d4nref));
+  HIP_TRY(up(d_xyz, T.xyz));
+  const std::vector<double>* atv[15] = {&T.at_cnrad, &T.at_h0rad, &T.at_en, &T.at_dk, &T.at_qk,
+                                        &T.at_mprad, &T.at_mpvcn, &T.at_sqrt_arep, &T.at_zeff,
+                                        &T.at_d4rcov, &T.at_d4en, &T.at_d4zeff, &T.at_d4hard,
+                                        &T.at_d4r4r2, &T.at_cnrad};
+  for (int k = 0; k < 15; ++k) HIP_TRY(up(at[k], *atv[k]));
+  const std::vector<int>* shiv[5] = {&T.sh_atom, &T.sh_l, &T.sh_ao, &T.sh_prim, &T.sh_nprim};
+  for (int k = 0; k < 5; ++k) HIP_TRY(up(shi[k], *shiv[k]));
+  const std::vector<double>* shdv[7] = {&T.sh_level, &T.sh_kcn, &T.sh_kpoly, &T.sh_zeta,
+                                        &T.sh_refocc, &T.sh_eta, &T.sh_gam3};
+  for (int k = 0; k < 7; ++k) HIP_TRY(up(shd[k], *shdv[k]));
+  HIP_TRY(up(d_ao_sh, T.ao_sh));
+  HIP_TRY(up(d_ao_atom, T.ao_atom));
+  HIP_TRY(up(d_alpha, T.alpha));
+  HIP_TRY(up(d_coef, T.coef));
+  HIP_TRY(up(d_pcxyz, T.pc_xyz));
+  HIP_TRY(up(d_pcq, T.pc_q));
+  HIP_TRY(up(d_pcg, T.pc_gam));
+  HIP_TRY(up(d_full, full));
+  HIP_TRY(up(d_lower, lower));
+
+  t.atom_off = d_atom_off; t.shell_off = d_shell_off; t.ao_off = d_ao_off; t.vec_off = d_vec_off;
+  t.pc_off = d_pc_off; t.mat_off = d_mat_off; t.gam_off = d_gam_off; t.apair_off = d_apair_off;
+  t.nalpha = d_nalpha; t.nbeta = d_nbeta;
+  t.Z = d_Z; t.xyz = d_xyz; t.atom_sys = d_atom_sys; t.atom_sh0 = d_atom_sh0;
+  t.at_cnrad = at[0]; t.at_h0rad = at[1]; t.at_en = at[2]; t.at_dk = at[3]; t.at_qk = at[4];
+  t.at_mprad = at[5]; t.at_mpvcn = at[6]; t.at_sqrt_arep = at[7]; t.at_zeff = at[8];
+  t.at_d4rcov = at[9]; t.at_d4en = at[10]; t.at_d4zeff = at[11]; t.at_d4hard = at[12];
+  t.at_d4r4r2 = at[13]; t.at_d4ref = d_d4ref; t.at_d4nref = d_d4nref;
+  t.sh_atom = shi[0]; t.sh_l = shi[1]; t.sh_ao = shi[2]; t.sh_prim = shi[3]; t.sh_nprim = shi[4];
+  t.sh_level = shd[0]; t.sh_kcn = shd[1]; t.sh_kpoly = shd[2]; t.sh_zeta = shd[3];
+  t.sh_refocc = shd[4]; t.sh_eta = shd[5]; t.sh_gam3 = shd[6];
+  t.ao_sh = d_ao_sh; t.ao_atom = d_ao_atom;
+  t.alpha = d_alpha; t.coef = d_coef;
+  t.pc_xyz = d_pcxyz; t.pc_q = d_pcq; t.pc_gam = d_pcg;
+  t.d4_c6 = dev->d4_c6; t.d4_refcn = dev->d4_refcn; t.d4_refq = dev->d4_refq;
+  t.d4_refng = dev->d4_refng; t.kpair = dev->kpair;
+
+  double* mats = cv.take<double>((size_t)kMat * NN);
+  double* dS = mats;
+  double* dD = mats + NN;
+  double* dQ = mats + 4 * NN;
+  double* dH0 = mats + 10 * NN;
+  double* dX = mats + 11 * NN;
+  double* dH = mats + 12 * NN;
+  double* dA = mats + 13 * NN;
+  double* dZ = mats + 14 * NN;
+  double* dY = mats + 15 * NN;
+  double* dC = mats + 16 * NN;
+  double* dP = mats + 17 * NN;
+  double* dW = mats + 18 * NN;
+  double* dT = mats + 19 * NN;
+  double* d_gam = cv.take<double>(NG);
+  double* d_cn = cv.take<double>(N);
+  double* d_cnd4 = cv.take<double>(N);
+  double* d_mrad = cv.take<double>(N);
+  double* d_dmrad = cv.take<double>(N);
+  double* d_qat = cv.take<double>(N);
+  double* d_vat = cv.take<double>(N);
+  double* d_dedcn = cv.take<double>(N);
+  double* d_dedcn4 = cv.take<double>(N);
+  double* d_w4 = cv.take<double>((size_t)kD4MaxRef * N);
+  double* d_dw4 = cv.take<double>((size_t)kD4MaxRef * N);
+  double* d_vpc = cv.take<double>(NS);
+  double* d_vsh = cv.take<double>(NS);
+  double* d_vd = cv.take<double>(3 * (size_t)N);
+  double* d_vqq = cv.take<double>(6 * (size_t)N);
+  double* d_eps = cv.take<double>(NAO);
+  double* d_fa = cv.take<double>(NAO);
+  double* d_fb = cv.take<double>(NAO);
+  double* d_occ = cv.take<double>(NAO);
+  double* d_fe = cv.take<double>(NAO);
+  double* d_colsum = cv.take<double>((size_t)NAO * 10);
+  MixerArrays mx{};
+  mx.x = cv.take<double>(NV);
+  mx.raw = cv.take<double>(NV);
+  mx.cur = cv.take<double>(NV);
+  mx.prev = cv.take<double>(NV);
+  mx.prev_res = cv.take<double>(NV);
+  mx.df = cv.take<double>((size_t)mem * NV);
+  mx.u = cv.take<double>((size_t)mem * NV);
+  mx.omega = cv.take<double>((size_t)mem * B);
+  mx.mem = mem;
+  mx.damping = opt.damping;
+  mx.qtol = opt.charge_tolerance;
+  mx.etol = opt.energy_tolerance;
+  mx.kt = opt.kt;
+  mx.max_iter = opt.max_iterations;
+  SysState* d_st = cv.take<SysState>(B);
+  unsigned char* d_active = cv.take<unsigned char>(B);
+  unsigned char* d_valid = cv.take<unsigned char>(B);
+  int* d_count = cv.take<int>(2);
+  int* d_kocc = cv.take<int>(B);
+  double* d_grad = cv.take<double>(3 * (size_t)N);
+  double* d_pcgrad = cv.take<double>(3 * (size_t)NPC);
+  double* d_dedq = cv.take<double>(NPC);
+  double* d_erep = cv.take<double>(B);
+  double* d_eatm = cv.take<double>(B);
+  double* d_c6pair = cv.take<double>(2 * (size_t)NAP);
+
+  HIP_TRY(hipMemsetAsync(d_st, 0, sizeof(SysState) * B, sm));
+  HIP_TRY(hipMemsetAsync(mx.df, 0, sizeof(double) * (size_t)mem * NV, sm));
+  HIP_TRY(hipMemsetAsync(mx.u, 0, sizeof(double) * (size_t)mem * NV, sm));
+  HIP_TRY(hipMemsetAsync(mx.omega, 0, sizeof(double) * (size_t)mem * B, sm));
+
+  HIP_TRY(hipEventRecord(dev->ev[0], sm));
+  // ---- geometry, integrals, overlap factorization ----
+  hipLaunchKernelGGL(k_geometry, dim3(B), dim3(256), 0, sm, t, d_order, d_cn, d_cnd4, d_mrad,
+                     d_dmrad, d_gam, d_vpc, mx, d_st, d_charges);
+  if (NSP > 0)
+    hipLaunchKernelGGL(k_integrals, dim3((unsigned)((NSP + 127) / 128)), dim3(128), 0, sm, t,
+                       d_sp_off, NSP, d_cn, dS, dD, dQ, dH0);
+  hipLaunchKernelGGL(k_cholesky_inverse, dim3(B), dim3(256), 0, sm, t, d_order, dS, dA, dX, d_st);
+  hipLaunchKernelGGL(k_init_active, dim3((B + 255) / 256), dim3(256), 0, sm, B, d_st, d_active,
+                     (int*)nullptr, d_st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(dev->ev[1], sm));
+
+  // ---- SCC rounds ----
+  const unsigned nfull = (unsigned)full.size(), nlower = (unsigned)lower.size();
+  int rounds = 0;
+  int* h_count = nullptr;
+  HIP_TRY(hipHostMalloc(&h_count, sizeof(int)));
+  for (int it = 0; it < opt.max_iterations; ++it) {
+    HIP_TRY(hipMemsetAsync(d_count, 0, sizeof(int), sm));
+    hipLaunchKernelGGL(k_potentials, dim3(B), dim3(256), 0, sm, t, d_order, d_active, mx.x,
+                       d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, d_dw4, d_vat, d_vsh, d_vd, d_vqq);
+    hipLaunchKernelGGL(k_fock, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, sm, t, NN,
+                       d_active, dS, dD, dQ, dH0, d_vsh, d_vd, d_vqq, dH);
+    // H' = X^T H X
+    hipLaunchKernelGGL((k_gemm<true, false, false>), dim3(nfull), dim3(256), 0, sm, t, d_full, dX,
+                       dH, dT, (const double*)nullptr, (const int*)nullptr, d_active);
+    hipLaunchKernelGGL((k_gemm<false, false, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower,
+                       dT, dX, dA, (const double*)nullptr, (const int*)nullptr, d_active);
+    hipLaunchKernelGGL(k_eigh, dim3(B), dim3(256), 0, sm, t, d_order, dA, dZ, dY, d_eps, d_st,
+                       d_active);
+    hipLaunchKernelGGL(k_fermi, dim3(B), dim3(64), 0, sm, t, d_order, d_active, d_eps, d_fa, d_fb,
+                       d_occ, d_fe, d_st, d_kocc, opt.kt);
+    // C = X Y (Y column-major -> read transposed), P = C diag(f) C^T
+    hipLaunchKernelGGL((k_gemm<false, true, false>), dim3(nfull), dim3(256), 0, sm, t, d_full, dX,
+                       dY, dC, (const double*)nullptr, (const int*)nullptr, d_active);
+    hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
+                       dC, dP, d_occ, d_kocc, d_active);
+    hipLaunchKernelGGL(k_mulliken, dim3(B), dim3(256), 0, sm, t, d_order, d_active, dP, dS, dD, dQ,
+                       dH0, d_colsum, mx.raw, d_st);
+    hipLaunchKernelGGL(k_energy_mix, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_cnd4,
+                       d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_count);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, sm));
+    HIP_TRY(hipStreamSynchronize(sm));
+    ++rounds;
+    if (*h_count == 0) break;
+  }
+  hipHostFree(h_count);
+  HIP_TRY(hipEventRecord(dev->ev[2], sm));
+
+  // ---- stationary energy and forces ----
+  hipLaunchKernelGGL(k_valid, dim3((B + 255) / 256), dim3(256), 0, sm, B, d_st, d_valid);
+  HIP_TRY(hipMemsetAsync(d_grad, 0, sizeof(double) * 3 * (size_t)N, sm));
+  HIP_TRY(hipMemsetAsync(d_dedcn, 0, sizeof(double) * (size_t)N, sm));
+  HIP_TRY(hipMemsetAsync(d_erep, 0, sizeof(double) * (size_t)B, sm));
+  HIP_TRY(hipMemsetAsync(d_eatm, 0, sizeof(double) * (size_t)B, sm));
+  if (NPC > 0) {
+    HIP_TRY(hipMemsetAsync(d_pcgrad, 0, sizeof(double) * 3 * (size_t)NPC, sm));
+    HIP_TRY(hipMemsetAsync(d_dedq, 0, sizeof(double) * (size_t)NPC, sm));
+  }
+  // potentials at the converged raw multipoles (reference gfn2_cpu_execution.cpp:1358-1441)
+  hipLaunchKernelGGL(k_potentials, dim3(B), dim3(256), 0, sm, t, d_order, d_valid, mx.x, d_cnd4,
+                     d_mrad, d_gam, d_vpc, d_qat, d_w4, d_dw4, d_vat, d_vsh, d_vd, d_vqq);
+  if (opt.forces) {
+    // W = C diag(f eps) C^T
+    hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
+                       dC, dW, d_fe, d_kocc, d_valid);
+    if (NAP > 0)
+      hipLaunchKernelGGL(k_force_pairs, dim3((unsigned)((NAP + 127) / 128)), dim3(128), 0, sm, t,
+                         NAP, d_valid, d_cn, dP, dW, dS, d_vsh, d_vd, d_vqq, d_grad, d_dedcn);
+  }
+  hipLaunchKernelGGL(k_force_atoms, dim3(B), dim3(256), 0, sm, t, d_order, d_valid, mx.x, d_qat,
+                     d_cnd4, d_mrad, d_dmrad, d_gam, d_w4, d_dw4, d_dedcn, d_dedcn4, d_grad,
+                     d_erep, d_pcgrad, d_dedq);
+  hipLaunchKernelGGL(k_force_atm, dim3(B), dim3(256), 0, sm, t, d_order, d_valid, d_cnd4, d_w4,
+                     d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(dev->ev[3], sm));
+
+  // ---- publication ----
+  std::vector<SysState> hst(B);
+  std::vector<double> herep(B), heatm(B), hx(NV), hqat(N), hgrad(3 * (size_t)N);
+  HIP_TRY(hipMemcpyAsync(hst.data(), d_st, sizeof(SysState) * B, hipMemcpyDeviceToHost, sm));
+  HIP_TRY(hipMemcpyAsync(herep.data(), d_erep, sizeof(double) * B, hipMemcpyDeviceToHost, sm));
+  HIP_TRY(hipMemcpyAsync(heatm.data(), d_eatm, sizeof(double) * B, hipMemcpyDeviceToHost, sm));
+  HIP_TRY(hipMemcpyAsync(hx.data(), mx.x, sizeof(double) * NV, hipMemcpyDeviceToHost, sm));
+  HIP_TRY(hipMemcpyAsync(hqat.data(), d_qat, sizeof(double) * N, hipMemcpyDeviceToHost, sm));
+  HIP_TRY(hipMemcpyAsync(hgrad.data(), d_grad, sizeof(double) * 3 * (size_t)N, hipMemcpyDeviceToHost, sm));
+  std::vector<double> hpcg, hdq;
+  if (NPC > 0) {
+    hpcg.resize(3 * (size_t)NPC);
+    hdq.resize(NPC);
+    HIP_TRY(hipMemcpyAsync(hpcg.data(), d_pcgrad, sizeof(double) * 3 * (size_t)NPC, hipMemcpyDeviceToHost, sm));
+    HIP_TRY(hipMemcpyAsync(hdq.data(), d_dedq, sizeof(double) * NPC, hipMemcpyDeviceToHost, sm));
+  }
+  if (out.dbg_overlap)
+    HIP_TRY(hipMemcpyAsync(out.dbg_overlap, dS, sizeof(double) * NN, hipMemcpyDeviceToHost, sm));
+  if (out.dbg_h0)
+    HIP_TRY(hipMemcpyAsync(out.dbg_h0, dH0, sizeof(double) * NN, hipMemcpyDeviceToHost, sm));
+  if (out.dbg_density)
+    HIP_TRY(hipMemcpyAsync(out.dbg_density, dP, sizeof(double) * NN, hipMemcpyDeviceToHost, sm));
+  if (out.dbg_cn)
+    HIP_TRY(hipMemcpyAsync(out.dbg_cn, d_cn, sizeof(double) * N, hipMemcpyDeviceToHost, sm));
+  HIP_TRY(hipStreamSynchronize(sm));
+  if (timing) {
+    hipEventElapsedTime(&timing->setup_ms, dev->ev[0], dev->ev[1]);
+    hipEventElapsedTime(&timing->scc_ms, dev->ev[1], dev->ev[2]);
+    hipEventElapsedTime(&timing->force_ms, dev->ev[2], dev->ev[3]);
+    hipEventElapsedTime(&timing->total_ms, dev->ev[0], dev->ev[3]);
+    timing->scc_rounds = rounds;
+  }
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (int s = 0; s < B; ++s) {
+    const SysState& z = hst[s];
+    int status = z.status;
+    if (status == 0 && !z.converged) status = 7;
+    const bool ok = status == 0;
+    const int a0 = T.atom_off[s], a1 = T.atom_off[s + 1];
+    const int ns = T.shell_off[s + 1] - T.shell_off[s];
+    const double* xs = hx.data() + T.vec_off[s];
+    if (out.status) out.status[s] = status;
+    if (out.iterations) out.iterations[s] = z.iterations;
+    if (out.converged) out.converged[s] = ok ? 1 : 0;
+    if (out.energies) out.energies[s] = ok ? z.free_energy + herep[s] + heatm[s] : nan;
+    for (int a = a0; a < a1; ++a) {
+      if (out.charges) out.charges[a] = ok ? hqat[a] : nan;
+      if (out.gradient)
+        for (int c = 0; c < 3; ++c) out.gradient[3 * a + c] = ok ? hgrad[3 * a + c] : nan;
+    }
+    if (out.dipoles) {
+      for (int c = 0; c < 3; ++c) {
+        double m = 0.0;
+        for (int a = a0; a < a1; ++a) m += T.xyz[3 * a + c] * hqat[a] + xs[ns + 3 * (a - a0) + c];
+        out.dipoles[3 * s + c] = ok ? m : nan;
+      }
+    }
+    for (int p = T.pc_off[s]; p < T.pc_off[s + 1]; ++p) {
+      if (out.pc_gradient)
+        for (int c = 0; c < 3; ++c) out.pc_gradient[3 * p + c] = ok ? hpcg[3 * p + c] : nan;
+      if (out.pc_dedq) out.pc_dedq[p] = ok ? hdq[p] : nan;
+    }
+    if (out.dbg_components) {
+      double* cp = out.dbg_components + 8 * s;
+      cp[0] = z.e_core; cp[1] = z.e_es2; cp[2] = z.e_es3; cp[3] = z.e_aes2;
+      cp[4] = z.e_d4; cp[5] = z.e_pc; cp[6] = herep[s]; cp[7] = heatm[s];
+    }
+  }
+  error.clear();
+  return 0;
+}
+
+}  // namespace xtb

@@ -1,0 +1,639 @@
+// Energy assembly + modified Broyden mixing, and the analytic force kernels.
+#pragma once
+
+#include "gfn2_scc.hpp"
+
+namespace xtb {
+
+// ---------------------------------------------------------------------------
+// SCC energy from the raw multipoles (reference scc_driver.cpp:2105-2286),
+// modified Broyden step (reference common/scc_mixer.cpp:559-745) and the
+// convergence gate rms < qtol and |dF| < etol (scc_driver.cpp:869-893).
+// One 256-thread block per active system.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_energy_mix(Topo t, const int* order, unsigned char* active,
+                                                    const double* cnd4, const double* mrad,
+                                                    const double* gam, const double* vpc,
+                                                    double* qat_raw, double* w4, MixerArrays mx,
+                                                    SysState* st, int* active_count) {
+  const int s = order[blockIdx.x];
+  if (!active[s]) return;
+  const int a0 = t.atom_off[s], a1 = t.atom_off[s + 1], N = a1 - a0;
+  const int s0 = t.shell_off[s], ns = t.shell_off[s + 1] - s0;
+  const int vo = t.vec_off[s], dim = t.vec_off[s + 1] - vo;
+  const double* raw = mx.raw + vo;
+  const double* qsh = raw;
+  const double* dip = raw + ns;
+  const double* quad = raw + ns + 3 * N;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  __shared__ double red[16];
+  __shared__ double beta[64 * 64];
+  __shared__ double coef[64];
+  __shared__ int slots[64];
+  __shared__ double sh_scalar[4];
+  for (int a = a0 + tid; a < a1; a += nt) {
+    double q = 0.0;
+    for (int sh = t.atom_sh0[a]; sh < t.atom_sh0[a + 1]; ++sh) q += qsh[sh - s0];
+    qat_raw[a] = q;
+    d4_weights(t, a, cnd4[a], q, w4 + kD4MaxRef * a, nullptr, nullptr);
+  }
+  __syncthreads();
+  // ES2 / ES3 / point charges
+  const double* G = gam + t.gam_off[s];
+  double e2 = 0.0, e3 = 0.0, epc = 0.0;
+  for (int i = tid; i < ns; i += nt) {
+    double v = 0.0;
+    for (int j = 0; j < ns; ++j) v += G[i * ns + j] * qsh[j];
+    const int sh = s0 + i;
+    e2 += 0.5 * qsh[i] * v;
+    e3 += t.sh_gam3[sh] * qsh[i] * qsh[i] * qsh[i] / 3.0;
+    epc = fma(qsh[i], vpc[sh], epc);
+  }
+  // AES2 and D4 two-body, pairs (a < b) owned by a
+  const double qscale[6] = {1.0, 2.0, 1.0, 2.0, 2.0, 1.0};
+  double eaes = 0.0, ed4 = 0.0;
+  for (int a = a0 + tid; a < a1; a += nt) {
+    const int la = a - a0;
+    const double* da = dip + 3 * la;
+    const double* Qa = quad + 6 * la;
+    double d2 = 0.0, q2 = 0.0;
+    for (int c = 0; c < 3; ++c) d2 += da[c] * da[c];
+    for (int c = 0; c < 6; ++c) q2 += qscale[c] * Qa[c] * Qa[c];
+    eaes += t.at_dk[a] * d2 + t.at_qk[a] * q2;
+    const double* ra = t.xyz + 3 * a;
+    for (int b = a + 1; b < a1; ++b) {
+      const int lb = b - a0;
+      const double* rb = t.xyz + 3 * b;
+      const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      double k3, k5;
+      aes2_kernels(sqrt(r2), 0.5 * (mrad[a] + mrad[b]), k3, k5);
+      const double* db = dip + 3 * lb;
+      const double* Qb = quad + 6 * lb;
+      double pa = 0, pb = 0, dd = 0, cd = 0;
+      for (int c = 0; c < 3; ++c) {
+        pa += r[c] * da[c];
+        pb += r[c] * db[c];
+        dd += da[c] * db[c];
+        cd += r[c] * (qat_raw[a] * db[c] - qat_raw[b] * da[c]);
+      }
+      const double sq[6] = {r[0] * r[0] * k5, 2.0 * r[0] * r[1] * k5, r[1] * r[1] * k5,
+                            2.0 * r[0] * r[2] * k5, 2.0 * r[1] * r[2] * k5, r[2] * r[2] * k5};
+      double sqa = 0, sqb = 0;
+      for (int c = 0; c < 6; ++c) { sqa += sq[c] * Qa[c]; sqb += sq[c] * Qb[c]; }
+      eaes += k3 * cd + k5 * (r2 * dd - 3.0 * pa * pb) + qat_raw[a] * sqb + qat_raw[b] * sqa;
+      double damp, ddamp;
+      d4_damping(t, a, b, r2, damp, ddamp);
+      if (damp != 0.0) ed4 -= d4_contract(t, a, b, w4 + kD4MaxRef * a, w4 + kD4MaxRef * b) * damp;
+    }
+  }
+  e2 = block_sum(e2, red);
+  e3 = block_sum(e3, red);
+  epc = block_sum(epc, red);
+  eaes = block_sum(eaes, red);
+  ed4 = block_sum(ed4, red);
+  const double eint = st[s].e_core + e2 + e3 + eaes + ed4 + epc;
+  const double F = fma(-mx.kt, st[s].entropy, eint);
+
+  // ---- modified Broyden ----
+  double* cur = mx.cur + vo;
+  double* prev = mx.prev + vo;
+  double* prev_res = mx.prev_res + vo;
+  const int mem = mx.mem;
+  double* dfh = mx.df + (int64_t)mem * vo;  // slot k at dfh + k*dim
+  double* uh = mx.u + (int64_t)mem * vo;
+  double* om = mx.omega + (int64_t)mem * s;
+  const int old_it = st[s].iterations;
+  double rs = 0.0;
+  for (int i = tid; i < dim; i += nt) {
+    const double res = raw[i] - cur[i];
+    rs += res * res;
+  }
+  rs = block_sum(rs, red);
+  const double rnorm = sqrt(rs), rms = rnorm / sqrt((double)dim);
+  double* xnew = mx.x + vo;
+  if (old_it == 0) {
+    for (int i = tid; i < dim; i += nt) {
+      const double res = raw[i] - cur[i];
+      xnew[i] = cur[i] + mx.damping * res;
+      prev[i] = cur[i];
+      prev_res[i] = res;
+    }
+  } else {
+    // delta F (normalized) and u into the new slot
+    const int slot_new = (old_it - 1) % mem;
+    double* dfn = dfh + (int64_t)slot_new * dim;
+    double* un = uh + (int64_t)slot_new * dim;
+    double d2 = 0.0;
+    for (int i = tid; i < dim; i += nt) {
+      const double res = raw[i] - cur[i];
+      const double v = res - prev_res[i];
+      d2 += v * v;
+    }
+    d2 = block_sum(d2, red);
+    const double inv = 1.0 / fmax(sqrt(d2), 2.220446049250313e-16);
+    for (int i = tid; i < dim; i += nt) {
+      const double res = raw[i] - cur[i];
+      const double v = (res - prev_res[i]) * inv;
+      dfn[i] = v;
+      un[i] = mx.damping * v + inv * (cur[i] - prev[i]);
+    }
+    const double omega = rnorm > 0.01 / 100000.0 ? fmax(1.0, 0.01 / rnorm) : 100000.0;
+    const int hc = min(mem, old_it);
+    const int first = old_it - hc + 1;
+    if (tid < hc) slots[tid] = (first + tid - 1) % mem;
+    if (tid == 0) om[slot_new] = omega;
+    __syncthreads();
+    // coefficients and beta matrix
+    for (int r = 0; r < hc; ++r) {
+      const double* dr = dfh + (int64_t)slots[r] * dim;
+      double dot = 0.0;
+      for (int i = tid; i < dim; i += nt) dot += dr[i] * (raw[i] - cur[i]);
+      dot = block_sum(dot, red);
+      if (tid == 0) coef[r] = om[slots[r]] * dot;
+      for (int c = 0; c <= r; ++c) {
+        const double* dc = dfh + (int64_t)slots[c] * dim;
+        double ov = 0.0;
+        for (int i = tid; i < dim; i += nt) ov += dr[i] * dc[i];
+        ov = block_sum(ov, red);
+        if (tid == 0) {
+          double val = om[slots[r]] * om[slots[c]] * ov;
+          if (r == c) val += 0.01 * 0.01;
+          beta[r * hc + c] = val;
+          beta[c * hc + r] = val;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int ok = 1;
+      for (int r = 0; r < hc && ok; ++r)
+        for (int c = 0; c <= r; ++c) {
+          double v = beta[r * hc + c];
+          for (int k = 0; k < c; ++k) v -= beta[r * hc + k] * beta[c * hc + k];
+          if (r == c) {
+            if (!(v > 0.0)) { ok = 0; break; }
+            beta[r * hc + c] = sqrt(v);
+          } else {
+            beta[r * hc + c] = v / beta[c * hc + c];
+          }
+        }
+      if (ok) {
+        for (int r = 0; r < hc; ++r) {
+          double v = coef[r];
+          for (int c = 0; c < r; ++c) v -= beta[r * hc + c] * coef[c];
+          coef[r] = v / beta[r * hc + r];
+        }
+        for (int r = hc - 1; r >= 0; --r) {
+          double v = coef[r];
+          for (int c = r + 1; c < hc; ++c) v -= beta[c * hc + r] * coef[c];
+          coef[r] = v / beta[r * hc + r];
+        }
+      }
+      sh_scalar[0] = ok ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int i = tid; i < dim; i += nt) {
+      const double res = raw[i] - cur[i];
+      double v = cur[i] + mx.damping * res;
+      for (int h = 0; h < hc; ++h) v -= om[slots[h]] * coef[h] * uh[(int64_t)slots[h] * dim + i];
+      xnew[i] = v;
+      prev[i] = cur[i];
+      prev_res[i] = res;
+    }
+    if (tid == 0 && sh_scalar[0] == 0.0) st[s].status = 6;
+  }
+  __syncthreads();
+  const double dF = F - (old_it == 0 ? 0.0 : st[s].f_old);
+  const bool conv = rms < mx.qtol && fabs(dF) < mx.etol;
+  for (int i = tid; i < dim; i += nt) {
+    cur[i] = xnew[i];
+    if (conv) xnew[i] = raw[i];
+  }
+  if (tid == 0) {
+    SysState& z = st[s];
+    z.e_es2 = e2; z.e_es3 = e3; z.e_aes2 = eaes; z.e_d4 = ed4; z.e_pc = epc;
+    z.free_energy = F;
+    z.f_old = F;
+    z.residual_rms = rms;
+    z.iterations = old_it + 1;
+    z.converged = conv ? 1 : 0;
+    bool stop = conv || z.status != 0;
+    if (!conv && z.status == 0 && z.iterations >= mx.max_iter) {
+      z.status = 7;
+      stop = true;
+    }
+    if (!isfinite(F) || !isfinite(rms)) { z.status = 6; stop = true; }
+    if (stop) active[s] = 0;
+    else atomicAdd(active_count, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forces.  Adjoints of the stationary energy (reference force.cpp:501-612):
+//   dE/dS_mn = P_mn (h0 factor_mn - (v_m + v_n)/2) - W_mn
+//   dE/dD_mn = -P_mn v_D(atom n),  dE/dQ_mn = -P_mn v_Q(atom n)
+// contracted with recomputed derivative integrals per ordered atom pair; the
+// H0 CN adjoint and polynomial gradient (h0.cpp:469-597) ride along.
+// ---------------------------------------------------------------------------
+template <int LB, int LK>
+__device__ void force_block(const Topo& t, int s, int si, int sj, const double* cn,
+                            const double* P, const double* W, const double* Sm,
+                            const double* vsh, const double* vd, const double* vqq, double gA[3],
+                            double gC[3], double& dcnA, double& dcnC) {
+  const int A = t.sh_atom[si], C = t.sh_atom[sj];
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t mo = t.mat_off[s];
+  const int r0 = t.sh_ao[si] - t.ao_off[s], c0 = t.sh_ao[sj] - t.ao_off[s];
+  const double hs = shell_level(t, si, cn), ht = shell_level(t, sj, cn);
+  const double avg = 0.5 * (hs + ht);
+  double sp = 1.0, dsp = 0.0, d[3] = {0, 0, 0}, r = 0.0;
+  if (A != C) {
+    for (int x = 0; x < 3; ++x) d[x] = t.xyz[3 * A + x] - t.xyz[3 * C + x];
+    r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    h0_spatial(t, si, sj, r, sp, dsp);
+  }
+  const double f = avg * sp;
+  const double sf = -0.5 * (vsh[si] + vsh[sj]);
+  double wsum = 0.0;
+  PairAdjoint<LB, LK> adj;
+#pragma unroll
+  for (int u = 0; u < 2 * LB + 1; ++u)
+#pragma unroll
+    for (int w = 0; w < 2 * LK + 1; ++w) {
+      const int64_t idx = mo + (int64_t)(r0 + u) * n + (c0 + w);
+      const double p = P[idx];
+      wsum += p * Sm[idx];
+      adj.a[0][u][w] = p * (f + sf) - W[idx];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) adj.a[1 + c][u][w] = -p * vd[3 * C + c];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) adj.a[4 + c][u][w] = -p * vqq[6 * C + c];
+    }
+  const double lw = 0.5 * wsum * sp;
+  dcnA -= t.sh_kcn[si] * lw;
+  dcnC -= t.sh_kcn[sj] * lw;
+  if (A == C) return;
+  const double sc = wsum * avg * dsp / r;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    gA[x] += sc * d[x];
+    gC[x] -= sc * d[x];
+  }
+  const double v[3] = {-d[0], -d[1], -d[2]};  // R_C - R_A
+  double g[3] = {0.0, 0.0, 0.0};
+  pair_gradient<LB, LK>(t, si, sj, v, adj, g);
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    gC[x] += g[x];
+    gA[x] -= g[x];
+  }
+}
+
+// one thread per ordered atom pair (A, C) of a system
+__global__ __launch_bounds__(128) void k_force_pairs(Topo t, const int64_t total,
+                                                     const unsigned char* valid, const double* cn,
+                                                     const double* P, const double* W,
+                                                     const double* Sm, const double* vsh,
+                                                     const double* vd, const double* vqq,
+                                                     double* grad, double* dedcn) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = t.B;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (t.apair_off[mid] <= g) lo = mid; else hi = mid;
+  }
+  const int s = lo;
+  if (!valid[s]) return;
+  const int a0 = t.atom_off[s], N = t.atom_off[s + 1] - a0;
+  const int64_t loc = g - t.apair_off[s];
+  const int A = a0 + (int)(loc / N), C = a0 + (int)(loc % N);
+  double gA[3] = {0, 0, 0}, gC[3] = {0, 0, 0}, dA = 0.0, dC = 0.0;
+  for (int si = t.atom_sh0[A]; si < t.atom_sh0[A + 1]; ++si)
+    for (int sj = t.atom_sh0[C]; sj < t.atom_sh0[C + 1]; ++sj) {
+      switch (t.sh_l[si] * 3 + t.sh_l[sj]) {
+        case 0: force_block<0, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+        case 1: force_block<0, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+        case 2: force_block<0, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+        case 3: force_block<1, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+        case 4: force_block<1, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+        case 5: force_block<1, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+        case 6: force_block<2, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+        case 7: force_block<2, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+        default: force_block<2, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+      }
+    }
+  if (A != C) {
+    for (int x = 0; x < 3; ++x) {
+      atomicAdd(&grad[3 * A + x], gA[x]);
+      atomicAdd(&grad[3 * C + x], gC[x]);
+    }
+  }
+  atomicAdd(&dedcn[A], dA);
+  atomicAdd(&dedcn[C], dC);
+}
+
+// AES2 fixed-multipole VJP of one unordered pair, first < second (reference aes2.cpp:601-706).
+__device__ void aes2_pair_vjp(const Topo& t, int i, int j, const double* mrad, const double* dmrad,
+                              const double* q, const double* dip, const double* quad, int a0,
+                              double gi[3], double& dcn_i, double& dcn_j) {
+  const double* ri = t.xyz + 3 * i;
+  const double* rj = t.xyz + 3 * j;
+  const double d[3] = {ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2]};
+  const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const double rho = 0.5 * (mrad[i] + mrad[j]);
+  double k3, k5;
+  aes2_kernels(r, rho, k3, k5);
+  const double sc = rho / r, sc3 = sc * sc * sc, sc4 = sc3 * sc;
+  const double f3 = 1.0 / (1.0 + 6.0 * sc3), f5 = 1.0 / (1.0 + 6.0 * sc4);
+  const double* di = dip + 3 * (i - a0);
+  const double* dj = dip + 3 * (j - a0);
+  const double* Qi = quad + 6 * (i - a0);
+  const double* Qj = quad + 6 * (j - a0);
+  const double qi = q[i], qj = q[j];
+  double cdv[3], pi = 0, pj = 0, dd = 0;
+  for (int x = 0; x < 3; ++x) {
+    cdv[x] = qi * dj[x] - qj * di[x];
+    pi += d[x] * di[x];
+    pj += d[x] * dj[x];
+    dd += di[x] * dj[x];
+  }
+  const double cdn = d[0] * cdv[0] + d[1] * cdv[1] + d[2] * cdv[2];
+  const double ddn = r * r * dd - 3.0 * pi * pj;
+  double T[6];
+  for (int c = 0; c < 6; ++c) T[c] = qi * Qj[c] + qj * Qi[c];
+  const double tv[3] = {T[0] * d[0] + T[1] * d[1] + T[3] * d[2], T[1] * d[0] + T[2] * d[1] + T[4] * d[2],
+                        T[3] * d[0] + T[4] * d[1] + T[5] * d[2]};
+  const double cqn = d[0] * tv[0] + d[1] * tv[1] + d[2] * tv[2];
+  const double ir = 1.0 / r;
+  const double dk3r = -3.0 * f3 * k3 * ir, dk5r = -(1.0 + 4.0 * f5) * k5 * ir;
+  const double irho = 1.0 / rho;
+  const double dk3rho = -3.0 * (1.0 - f3) * k3 * irho, dk5rho = -4.0 * (1.0 - f5) * k5 * irho;
+  const double k5n = ddn + cqn;
+  const double dE_drho = dk3rho * cdn + dk5rho * k5n;
+  for (int x = 0; x < 3; ++x) {
+    const double dddx = 2.0 * dd * d[x] - 3.0 * (pi * dj[x] + pj * di[x]);
+    gi[x] = k3 * cdv[x] + dk3r * cdn * d[x] * ir + k5 * (dddx + 2.0 * tv[x]) + dk5r * k5n * d[x] * ir;
+  }
+  dcn_i = 0.5 * dE_drho * dmrad[i];
+  dcn_j = 0.5 * dE_drho * dmrad[j];
+}
+
+// Per-system atom-level gradients: AES2 VJP, GFN2-CN chain (H0 + AES2 adjoints),
+// ES2, D4 two-body (+ D4-CN chain), repulsion energy/gradient and point charges.
+__global__ __launch_bounds__(256) void k_force_atoms(Topo t, const int* order,
+                                                     const unsigned char* valid,
+                                                     const double* xstate, const double* qat,
+                                                     const double* cnd4, const double* mrad,
+                                                     const double* dmrad, const double* gam,
+                                                     double* w4, double* dw4c, double* dedcn,
+                                                     double* dedcn4, double* grad, double* erep,
+                                                     double* pc_grad, double* dedq) {
+  const int s = order[blockIdx.x];
+  if (!valid[s]) return;
+  const int a0 = t.atom_off[s], a1 = t.atom_off[s + 1], N = a1 - a0;
+  const int s0 = t.shell_off[s], s1 = t.shell_off[s + 1], ns = s1 - s0;
+  const double* x = xstate + t.vec_off[s];
+  const double* qsh = x;
+  const double* dip = x + ns;
+  const double* quad = x + ns + 3 * N;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  __shared__ double red[16];
+  // phase 1: AES2 VJP (per atom, both pair orders) and D4 weights at the converged charges
+  for (int a = a0 + tid; a < a1; a += nt) {
+    double g[3] = {0, 0, 0}, dc = 0.0;
+    for (int b = a0; b < a1; ++b) {
+      if (b == a) continue;
+      const int i = min(a, b), j = max(a, b);
+      double gi[3], dci, dcj;
+      aes2_pair_vjp(t, i, j, mrad, dmrad, qat, dip, quad, a0, gi, dci, dcj);
+      const double sg = (a == i) ? 1.0 : -1.0;
+      for (int c = 0; c < 3; ++c) g[c] += sg * gi[c];
+      dc += (a == i) ? dci : dcj;
+    }
+    for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
+    dedcn[a] += dc;
+    d4_weights(t, a, cnd4[a], qat[a], w4 + kD4MaxRef * a, dw4c + kD4MaxRef * a, nullptr);
+  }
+  __syncthreads();
+  // phase 2: D4 two-body explicit gradient and D4-CN adjoint
+  for (int a = a0 + tid; a < a1; a += nt) {
+    const double* ra = t.xyz + 3 * a;
+    double g[3] = {0, 0, 0}, d4cn = 0.0;
+    for (int b = a0; b < a1; ++b) {
+      if (b == a) continue;
+      const double* rb = t.xyz + 3 * b;
+      const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      double damp, ddamp;
+      d4_damping(t, a, b, r2, damp, ddamp);
+      if (damp == 0.0) continue;
+      const double c6 = d4_contract(t, a, b, w4 + kD4MaxRef * a, w4 + kD4MaxRef * b);
+      const double dc6 = d4_contract(t, a, b, dw4c + kD4MaxRef * a, w4 + kD4MaxRef * b);
+      for (int c = 0; c < 3; ++c) g[c] += -c6 * ddamp * r[c];
+      d4cn -= dc6 * damp;
+    }
+    for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
+    dedcn4[a] = d4cn;
+  }
+  __syncthreads();
+  // phase 3: CN chains, ES2, repulsion (per atom, symmetric pair formulas)
+  double erp = 0.0;
+  for (int a = a0 + tid; a < a1; a += nt) {
+    const double* ra = t.xyz + 3 * a;
+    double g[3] = {0, 0, 0};
+    for (int b = a0; b < a1; ++b) {
+      if (b == a) continue;
+      const double* rb = t.xyz + 3 * b;
+      const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      const double rr = sqrt(r2);
+      double sc = 0.0;
+      if (r2 <= 625.0) {
+        double val, der;
+        dexp_count(rr, t.at_cnrad[a] + t.at_cnrad[b], val, der);
+        sc += (dedcn[a] + dedcn[b]) * der / rr;
+        // repulsion (reference repulsion.cpp:127-176)
+        const bool light = t.Z[a] <= 2 && t.Z[b] <= 2;
+        const double k = light ? 1.0 : 1.5;
+        const double rk = light ? rr : rr * sqrt(rr);
+        const double al = t.at_sqrt_arep[a] * t.at_sqrt_arep[b];
+        const double pe = t.at_zeff[a] * t.at_zeff[b] * exp(-al * rk) / rr;
+        if (a < b) erp += pe;
+        sc -= (al * k * rk + 1.0) * pe / r2;
+      }
+      if (r2 <= 900.0) {
+        double val, der;
+        d4_cn_count(t, a, b, rr, val, der);
+        sc += (dedcn4[a] + dedcn4[b]) * der / rr;
+      }
+      for (int c = 0; c < 3; ++c) g[c] += sc * r[c];
+    }
+    // ES2 at fixed q: -q_s q_t Gamma^3 (R_A - R_C) per shell pair on different atoms
+    const double* G = gam + t.gam_off[s];
+    for (int si = t.atom_sh0[a]; si < t.atom_sh0[a + 1]; ++si) {
+      for (int sj = s0; sj < s1; ++sj) {
+        const int C = t.sh_atom[sj];
+        if (C == a) continue;
+        const double gm = G[(si - s0) * ns + (sj - s0)];
+        const double f = -qsh[si - s0] * qsh[sj - s0] * gm * gm * gm;
+        for (int c = 0; c < 3; ++c) g[c] += f * (ra[c] - t.xyz[3 * C + c]);
+      }
+    }
+    for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
+  }
+  erp = block_sum(erp, red);
+  if (tid == 0) erep[s] = erp;
+  // phase 4: external point charges (reference external_point_charges.cpp:389-491)
+  const int p0 = t.pc_off[s], p1 = t.pc_off[s + 1];
+  if (p1 > p0) {
+    for (int a = a0 + tid; a < a1; a += nt) {
+      double g[3] = {0, 0, 0};
+      for (int sh = t.atom_sh0[a]; sh < t.atom_sh0[a + 1]; ++sh) {
+        const double qs = qsh[sh - s0];
+        for (int p = p0; p < p1; ++p) {
+          const double d[3] = {t.xyz[3 * a] - t.pc_xyz[3 * p], t.xyz[3 * a + 1] - t.pc_xyz[3 * p + 1],
+                               t.xyz[3 * a + 2] - t.pc_xyz[3 * p + 2]};
+          if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) continue;
+          const double ia = 2.0 / (t.sh_eta[sh] + t.pc_gam[p]);
+          const double rs2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + ia * ia;
+          const double rs = sqrt(rs2);
+          const double fs = qs * t.pc_q[p] / (rs2 * rs);
+          for (int c = 0; c < 3; ++c) g[c] -= fs * d[c];
+        }
+      }
+      for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
+    }
+    for (int p = p0 + tid; p < p1; p += nt) {
+      double g[3] = {0, 0, 0}, dq = 0.0;
+      for (int sh = s0; sh < s1; ++sh) {
+        const int a = t.sh_atom[sh];
+        const double d[3] = {t.xyz[3 * a] - t.pc_xyz[3 * p], t.xyz[3 * a + 1] - t.pc_xyz[3 * p + 1],
+                             t.xyz[3 * a + 2] - t.pc_xyz[3 * p + 2]};
+        const double ia = 2.0 / (t.sh_eta[sh] + t.pc_gam[p]);
+        const double rs2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + ia * ia;
+        const double rs = sqrt(rs2);
+        dq += qsh[sh - s0] / rs;
+        if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) continue;
+        const double fs = qsh[sh - s0] * t.pc_q[p] / (rs2 * rs);
+        for (int c = 0; c < 3; ++c) g[c] += fs * d[c];
+      }
+      for (int c = 0; c < 3; ++c) pc_grad[3 * p + c] = g[c];
+      dedq[p] = dq;
+    }
+  }
+}
+
+// D4 three-body ATM energy and gradient with zero-charge weights
+// (reference d4.cpp:1120-1362).  Pair data c6 / dc6/dcn are staged per ordered
+// pair; the triple loop accumulates into LDS and finishes with the D4-CN chain.
+__global__ __launch_bounds__(256) void k_force_atm(Topo t, const int* order, const unsigned char* valid,
+                                                   const double* cnd4, double* w4, double* dw4c,
+                                                   double* c6pair, double* dedcn4, double* grad,
+                                                   double* eatm) {
+  const int s = order[blockIdx.x];
+  if (!valid[s]) return;
+  const int a0 = t.atom_off[s], a1 = t.atom_off[s + 1], N = a1 - a0;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  __shared__ double red[16];
+  double* c6 = c6pair + 2 * t.apair_off[s];  // [N*N][2]: c6, dc6/dcn(first)
+  for (int a = a0 + tid; a < a1; a += nt) {
+    d4_weights(t, a, cnd4[a], 0.0, w4 + kD4MaxRef * a, dw4c + kD4MaxRef * a, nullptr);
+    dedcn4[a] = 0.0;
+  }
+  __syncthreads();
+  for (int64_t idx = tid; idx < (int64_t)N * N; idx += nt) {
+    const int i = a0 + (int)(idx / N), j = a0 + (int)(idx % N);
+    if (i == j) { c6[2 * idx] = 0.0; c6[2 * idx + 1] = 0.0; continue; }
+    c6[2 * idx] = d4_contract(t, i, j, w4 + kD4MaxRef * i, w4 + kD4MaxRef * j);
+    c6[2 * idx + 1] = d4_contract(t, i, j, dw4c + kD4MaxRef * i, w4 + kD4MaxRef * j);
+  }
+  __syncthreads();
+  double e = 0.0;
+  const double ex3 = 16.0 / 3.0;
+  // triples i > j > k; thread over (i, j) pairs
+  for (int64_t idx = tid; idx < (int64_t)N * N; idx += nt) {
+    const int li = (int)(idx / N), lj = (int)(idx % N);
+    if (lj >= li) continue;
+    const int i = a0 + li, j = a0 + lj;
+    const double* ri = t.xyz + 3 * i;
+    const double* rj = t.xyz + 3 * j;
+    const double vij[3] = {rj[0] - ri[0], rj[1] - ri[1], rj[2] - ri[2]};
+    const double r2ij = vij[0] * vij[0] + vij[1] * vij[1] + vij[2] * vij[2];
+    if (r2ij > 625.0) continue;
+    const double cij = c6[2 * ((int64_t)li * N + lj)];
+    double gi[3] = {0, 0, 0}, gj[3] = {0, 0, 0}, dci = 0.0, dcj = 0.0;
+    for (int lk = 0; lk < lj; ++lk) {
+      const int k = a0 + lk;
+      const double* rk = t.xyz + 3 * k;
+      const double vik[3] = {rk[0] - ri[0], rk[1] - ri[1], rk[2] - ri[2]};
+      const double vjk[3] = {rk[0] - rj[0], rk[1] - rj[1], rk[2] - rj[2]};
+      const double r2ik = vik[0] * vik[0] + vik[1] * vik[1] + vik[2] * vik[2];
+      const double r2jk = vjk[0] * vjk[0] + vjk[1] * vjk[1] + vjk[2] * vjk[2];
+      if (r2ik > 625.0 || r2jk > 625.0) continue;
+      const double cik = c6[2 * ((int64_t)li * N + lk)], cjk = c6[2 * ((int64_t)lj * N + lk)];
+      const double r0 = d4_r0(t, j, i) * d4_r0(t, k, i) * d4_r0(t, k, j);
+      const double r2p = r2ij * r2ik * r2jk, r1p = sqrt(r2p), r3p = r2p * r1p, r5p = r3p * r2p;
+      const double rp = pow(r0 / r1p, ex3);
+      const double damp = 1.0 / (1.0 + 6.0 * rp);
+      const double ang = 0.375 * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik) * (-r2ij + r2jk + r2ik) / r5p +
+                         1.0 / r3p;
+      const double c9 = -5.0 * sqrt(fabs(cij * cik * cjk));
+      e -= ang * damp * c9;
+      const double ddamp = -2.0 * 16.0 * rp * damp * damp;
+      const double tg[3] = {r2ij, r2ik, r2jk}, o1[3] = {r2jk, r2jk, r2ik}, o2[3] = {r2ik, r2ij, r2ij};
+      const double* vv[3] = {vij, vik, vjk};
+      double dg[3][3];
+      for (int q = 0; q < 3; ++q) {
+        const double T = tg[q], a = o1[q], b = o2[q];
+        const double dang = -0.375 * (T * T * T + T * T * (a + b) + T * (3.0 * a * a + 2.0 * a * b + 3.0 * b * b) -
+                                      5.0 * (a - b) * (a - b) * (a + b)) / r5p;
+        const double scq = c9 * (-dang * damp + ang * ddamp) / T;
+        for (int c = 0; c < 3; ++c) dg[q][c] = scq * vv[q][c];
+      }
+      for (int c = 0; c < 3; ++c) {
+        gi[c] -= dg[0][c] + dg[1][c];
+        gj[c] += dg[0][c] - dg[2][c];
+        atomicAdd(&grad[3 * k + c], dg[1][c] + dg[2][c]);
+      }
+      const double de = ang * damp * c9;
+      // dc6(x, y)/dcn_x is stored at [x][y] slot 1
+      const double dcij_i = c6[2 * ((int64_t)li * N + lj) + 1], dcij_j = c6[2 * ((int64_t)lj * N + li) + 1];
+      const double dcik_i = c6[2 * ((int64_t)li * N + lk) + 1], dcik_k = c6[2 * ((int64_t)lk * N + li) + 1];
+      const double dcjk_j = c6[2 * ((int64_t)lj * N + lk) + 1], dcjk_k = c6[2 * ((int64_t)lk * N + lj) + 1];
+      dci -= 0.5 * de * (dcij_i / cij + dcik_i / cik);
+      dcj -= 0.5 * de * (dcij_j / cij + dcjk_j / cjk);
+      atomicAdd(&dedcn4[k], -0.5 * de * (dcik_k / cik + dcjk_k / cjk));
+    }
+    for (int c = 0; c < 3; ++c) {
+      atomicAdd(&grad[3 * i + c], gi[c]);
+      atomicAdd(&grad[3 * j + c], gj[c]);
+    }
+    atomicAdd(&dedcn4[i], dci);
+    atomicAdd(&dedcn4[j], dcj);
+  }
+  e = block_sum(e, red);
+  if (tid == 0) eatm[s] = e;
+  __syncthreads();
+  // D4-CN chain of the ATM adjoint
+  for (int a = a0 + tid; a < a1; a += nt) {
+    const double* ra = t.xyz + 3 * a;
+    double g[3] = {0, 0, 0};
+    for (int b = a0; b < a1; ++b) {
+      if (b == a) continue;
+      const double* rb = t.xyz + 3 * b;
+      const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      if (r2 > 900.0) continue;
+      const double rr = sqrt(r2);
+      double val, der;
+      d4_cn_count(t, a, b, rr, val, der);
+      const double sc = (dedcn4[a] + dedcn4[b]) * der / rr;
+      for (int c = 0; c < 3; ++c) g[c] += sc * r[c];
+    }
+    for (int c = 0; c < 3; ++c) atomicAdd(&grad[3 * a + c], g[c]);
+  }
+}
+
+}  // namespace xtb

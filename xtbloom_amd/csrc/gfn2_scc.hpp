@@ -1,0 +1,460 @@
+// Geometry, integral, SCC-iteration kernels of the MI355X GFN2 executor.
+#pragma once
+
+#include "gfn2_device.hpp"
+#include "gfn2_integrals.hpp"
+
+namespace xtb {
+
+
+
+// Mixer arrays, one slice per system at vec_off (vectors) / mem*vec_off (history).
+struct MixerArrays {
+  double* x;         // current SCC state used to build potentials ([qsh, dip, quad])
+  double* raw;       // Mulliken output of the latest iteration
+  double* cur;       // Broyden current input
+  double* prev;
+  double* prev_res;
+  double* df;        // [mem * dim]
+  double* u;         // [mem * dim]
+  double* omega;     // [B * mem]
+  int mem;
+  double damping;
+  double qtol, etol, kt;
+  int max_iter;
+};
+
+// ---------------------------------------------------------------------------
+// geometry: CN (GFN2 dexp), D4 CN, AES2 radii, ES2 Gamma, point-charge shell
+// potential, FRESH SAD state and mixer reset.  One block per system.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_geometry(Topo t, const int* order, double* cn, double* cnd4,
+                                                  double* mrad, double* dmrad, double* gam,
+                                                  double* vpc, MixerArrays mx, SysState* st,
+                                                  const double* charges) {
+  const int s = order[blockIdx.x];
+  const int a0 = t.atom_off[s], a1 = t.atom_off[s + 1], N = a1 - a0;
+  const int s0 = t.shell_off[s], s1 = t.shell_off[s + 1], ns = s1 - s0;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int a = a0 + tid; a < a1; a += nt) {
+    double c = 0.0, c4 = 0.0;
+    const double* ra = t.xyz + 3 * a;
+    for (int b = a0; b < a1; ++b) {
+      if (b == a) continue;
+      const double* rb = t.xyz + 3 * b;
+      const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double r = sqrt(r2);
+      if (r2 <= 625.0) {
+        double val, der;
+        dexp_count(r, t.at_cnrad[a] + t.at_cnrad[b], val, der);
+        c += val;
+      }
+      if (r2 <= 900.0) {
+        double val, der;
+        d4_cn_count(t, a, b, r, val, der);
+        c4 += val;
+      }
+    }
+    cn[a] = c;
+    cnd4[a] = c4;
+    double dr;
+    mrad[a] = mp_radius(t, a, c, &dr);
+    dmrad[a] = dr;
+  }
+  // ES2 Gamma (reference gfn2/es2.cpp:211-300): arithmetic hardness average
+  double* G = gam + t.gam_off[s];
+  for (int idx = tid; idx < ns * ns; idx += nt) {
+    const int i = idx / ns, j = idx % ns;
+    const int si = s0 + i, sj = s0 + j;
+    const int A = t.sh_atom[si], C = t.sh_atom[sj];
+    const double eta = 0.5 * (t.sh_eta[si] + t.sh_eta[sj]);
+    if (A == C) {
+      G[idx] = eta;
+    } else {
+      const double dx = t.xyz[3 * A] - t.xyz[3 * C], dy = t.xyz[3 * A + 1] - t.xyz[3 * C + 1],
+                   dz = t.xyz[3 * A + 2] - t.xyz[3 * C + 2];
+      G[idx] = 1.0 / sqrt(dx * dx + dy * dy + dz * dz + 1.0 / (eta * eta));
+    }
+  }
+  // point-charge shell potential (reference external_point_charges.hpp:59)
+  const int p0 = t.pc_off[s], p1 = t.pc_off[s + 1];
+  for (int sh = s0 + tid; sh < s1; sh += nt) {
+    const int A = t.sh_atom[sh];
+    double v = 0.0;
+    for (int p = p0; p < p1; ++p) {
+      const double dx = t.xyz[3 * A] - t.pc_xyz[3 * p], dy = t.xyz[3 * A + 1] - t.pc_xyz[3 * p + 1],
+                   dz = t.xyz[3 * A + 2] - t.pc_xyz[3 * p + 2];
+      const double ia = 2.0 / (t.sh_eta[sh] + t.pc_gam[p]);
+      v += t.pc_q[p] / sqrt(dx * dx + dy * dy + dz * dz + ia * ia);
+    }
+    vpc[sh] = v;
+  }
+  // FRESH state (reference wavefunction.cpp:733-776)
+  const int vo = t.vec_off[s], dim = t.vec_off[s + 1] - vo;
+  for (int i = tid; i < dim; i += nt) {
+    double val = 0.0;
+    if (i < ns) {
+      const int sh = s0 + i, A = t.sh_atom[sh];
+      double ref = 0.0;
+      for (int q = t.atom_sh0[A]; q < t.atom_sh0[A + 1]; ++q) ref += t.sh_refocc[q];
+      val = (t.sh_refocc[sh] / ref) * (charges[s] / N);
+    }
+    mx.x[vo + i] = val;
+    mx.cur[vo + i] = val;
+    mx.prev[vo + i] = 0.0;
+    mx.prev_res[vo + i] = 0.0;
+  }
+  if (tid == 0) {
+    SysState z = {};
+    z.status = st[s].status;  // keep setup failures
+    st[s] = z;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// S, D, Q and H0 of every ordered shell pair, one thread per pair.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double shell_level(const Topo& t, int sh, const double* cn) {
+  return t.sh_level[sh] - t.sh_kcn[sh] * cn[t.sh_atom[sh]];
+}
+
+__device__ __forceinline__ double h0_pair_scale(const Topo& t, int si, int sj) {
+  const double zs = t.sh_zeta[si], zt = t.sh_zeta[sj];
+  const double zeta = sqrt(2.0 * sqrt(zs * zt) / (zs + zt));  // wexp = 0.5
+  const double den = t.at_en[t.sh_atom[si]] - t.at_en[t.sh_atom[sj]];
+  return zeta * (1.0 + 0.02 * den * den) * t.kpair[t.sh_l[si] * 3 + t.sh_l[sj]];
+}
+
+// spatial scale and its derivative d/dR (reference h0.cpp:407-440, 537-560)
+__device__ __forceinline__ void h0_spatial(const Topo& t, int si, int sj, double r, double& sp,
+                                           double& dsp) {
+  const int A = t.sh_atom[si], C = t.sh_atom[sj];
+  const double rr = sqrt(r / (t.at_h0rad[A] + t.at_h0rad[C]));
+  const double ks = t.sh_kpoly[si], kt = t.sh_kpoly[sj];
+  const double p1 = 1.0 + ks * rr, p2 = 1.0 + kt * rr;
+  const double kp = h0_pair_scale(t, si, sj);
+  sp = kp * p1 * p2;
+  dsp = kp * (ks * p2 + kt * p1) * rr / (2.0 * r);
+}
+
+template <int LB, int LK>
+__device__ void integral_block(const Topo& t, int s, int si, int sj, const double* cn, double* S,
+                               double* D, double* Q, double* H0) {
+  const int A = t.sh_atom[si], C = t.sh_atom[sj];
+  const double v[3] = {t.xyz[3 * C] - t.xyz[3 * A], t.xyz[3 * C + 1] - t.xyz[3 * A + 1],
+                       t.xyz[3 * C + 2] - t.xyz[3 * A + 2]};
+  PairBlock<LB, LK> blk;
+  pair_integrals<LB, LK, true>(t, si, sj, v, blk);
+  double f = 0.5 * (shell_level(t, si, cn) + shell_level(t, sj, cn));
+  if (A != C) {
+    const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double sp, dsp;
+    h0_spatial(t, si, sj, r, sp, dsp);
+    f *= sp;
+  }
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t nn = (int64_t)n * n;
+  const int64_t mo = t.mat_off[s];
+  const int r0 = t.sh_ao[si] - t.ao_off[s], c0 = t.sh_ao[sj] - t.ao_off[s];
+#pragma unroll
+  for (int u = 0; u < 2 * LB + 1; ++u)
+#pragma unroll
+    for (int w = 0; w < 2 * LK + 1; ++w) {
+      const int64_t idx = mo + (int64_t)(r0 + u) * n + (c0 + w);
+      // S is mirrored from the bra <= ket triangle so it is exactly symmetric
+      const double sval = (si <= sj) ? blk.s[u][w] : 0.0;
+      if (si <= sj) {
+        S[idx] = sval;
+        S[mo + (int64_t)(c0 + w) * n + (r0 + u)] = sval;
+        H0[idx] = sval * f;
+        H0[mo + (int64_t)(c0 + w) * n + (r0 + u)] = sval * f;
+      }
+      const int64_t loc = (int64_t)(r0 + u) * n + (c0 + w);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) D[3 * mo + c * nn + loc] = blk.m[c][u][w];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) Q[6 * mo + c * nn + loc] = blk.m[3 + c][u][w];
+    }
+}
+
+// pairs are enumerated per system as (i, j) over ns x ns; sp_off[B+1] prefix
+__global__ __launch_bounds__(128) void k_integrals(Topo t, const int64_t* sp_off, int64_t total,
+                                                   const double* cn, double* S, double* D,
+                                                   double* Q, double* H0) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = t.B;  // find s with sp_off[s] <= g < sp_off[s+1]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (sp_off[mid] <= g) lo = mid; else hi = mid;
+  }
+  const int s = lo;
+  const int ns = t.shell_off[s + 1] - t.shell_off[s];
+  const int64_t loc = g - sp_off[s];
+  const int si = t.shell_off[s] + (int)(loc / ns), sj = t.shell_off[s] + (int)(loc % ns);
+  const int key = t.sh_l[si] * 3 + t.sh_l[sj];
+  switch (key) {
+    case 0: integral_block<0, 0>(t, s, si, sj, cn, S, D, Q, H0); break;
+    case 1: integral_block<0, 1>(t, s, si, sj, cn, S, D, Q, H0); break;
+    case 2: integral_block<0, 2>(t, s, si, sj, cn, S, D, Q, H0); break;
+    case 3: integral_block<1, 0>(t, s, si, sj, cn, S, D, Q, H0); break;
+    case 4: integral_block<1, 1>(t, s, si, sj, cn, S, D, Q, H0); break;
+    case 5: integral_block<1, 2>(t, s, si, sj, cn, S, D, Q, H0); break;
+    case 6: integral_block<2, 0>(t, s, si, sj, cn, S, D, Q, H0); break;
+    case 7: integral_block<2, 1>(t, s, si, sj, cn, S, D, Q, H0); break;
+    default: integral_block<2, 2>(t, s, si, sj, cn, S, D, Q, H0); break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Potentials from the state x: v_sh (with the atom potential folded in), v_D, v_Q
+// (reference scc_driver.cpp:2363-2560, aes2.cpp:451-545, d4.cpp:896-1032).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_potentials(Topo t, const int* order,
+                                                    const unsigned char* active, const double* xs,
+                                                    const double* cnd4, const double* mrad,
+                                                    const double* gam, const double* vpc,
+                                                    double* qat, double* w4, double* dw4,
+                                                    double* vat, double* vsh, double* vd,
+                                                    double* vqq) {
+  const int s = order[blockIdx.x];
+  if (active && !active[s]) return;
+  const int a0 = t.atom_off[s], a1 = t.atom_off[s + 1], N = a1 - a0;
+  const int s0 = t.shell_off[s], s1 = t.shell_off[s + 1], ns = s1 - s0;
+  const double* x = xs + t.vec_off[s];
+  const double* qsh = x;
+  const double* dip = x + ns;
+  const double* quad = x + ns + 3 * N;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int a = a0 + tid; a < a1; a += nt) {
+    double q = 0.0;
+    for (int sh = t.atom_sh0[a]; sh < t.atom_sh0[a + 1]; ++sh) q += qsh[sh - s0];
+    qat[a] = q;
+    d4_weights(t, a, cnd4[a], q, w4 + kD4MaxRef * a, nullptr, dw4 + kD4MaxRef * a);
+  }
+  __syncthreads();
+  const double* G = gam + t.gam_off[s];
+  for (int i = tid; i < ns; i += nt) {
+    double v = 0.0;
+    for (int j = 0; j < ns; ++j) v += G[i * ns + j] * qsh[j];
+    const int sh = s0 + i;
+    v += t.sh_gam3[sh] * qsh[i] * qsh[i];
+    v += vpc[sh];
+    vsh[sh] = v;
+  }
+  const double qscale[6] = {1.0, 2.0, 1.0, 2.0, 2.0, 1.0};
+  for (int a = a0 + tid; a < a1; a += nt) {
+    const int la = a - a0;
+    const double* ra = t.xyz + 3 * a;
+    double pq = 0.0, pd[3], pQ[6];
+    for (int c = 0; c < 3; ++c) pd[c] = 2.0 * t.at_dk[a] * dip[3 * la + c];
+    for (int c = 0; c < 6; ++c) pQ[c] = 2.0 * t.at_qk[a] * qscale[c] * quad[6 * la + c];
+    double pd4 = 0.0;
+    const double* dwa = dw4 + kD4MaxRef * a;
+    for (int b = a0; b < a1; ++b) {
+      if (b == a) continue;
+      const int lb = b - a0;
+      const double* rb = t.xyz + 3 * b;
+      const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      const double rr = sqrt(r2);
+      double k3, k5;
+      aes2_kernels(rr, 0.5 * (mrad[a] + mrad[b]), k3, k5);
+      const double* db = dip + 3 * lb;
+      const double* Qb = quad + 6 * lb;
+      const double sq[6] = {r[0] * r[0] * k5, 2.0 * r[0] * r[1] * k5, r[1] * r[1] * k5,
+                            2.0 * r[0] * r[2] * k5, 2.0 * r[1] * r[2] * k5, r[2] * r[2] * k5};
+      const double rdb = r[0] * db[0] + r[1] * db[1] + r[2] * db[2];
+      double sqQ = 0.0;
+      for (int c = 0; c < 6; ++c) sqQ += sq[c] * Qb[c];
+      const double qb = qat[b];
+      pq += k3 * rdb + sqQ;
+      for (int c = 0; c < 3; ++c) pd[c] += -qb * k3 * r[c] + k5 * (r2 * db[c] - 3.0 * r[c] * rdb);
+      for (int c = 0; c < 6; ++c) pQ[c] += qb * sq[c];
+      double damp, ddamp;
+      d4_damping(t, a, b, r2, damp, ddamp);
+      if (damp != 0.0) pd4 -= d4_contract(t, a, b, dwa, w4 + kD4MaxRef * b) * damp;
+    }
+    vat[a] = pq + pd4;
+    for (int c = 0; c < 3; ++c) vd[3 * a + c] = pd[c];
+    for (int c = 0; c < 6; ++c) vqq[6 * a + c] = pQ[c];
+  }
+  __syncthreads();
+  for (int sh = s0 + tid; sh < s1; sh += nt) vsh[sh] += vat[t.sh_atom[sh]];
+}
+
+// H = H0 - S (u_m+u_n)/2 - D_mn.v(A_n)/2 - D_nm.v(A_m)/2 - (Q likewise)   (mulliken.hpp:222-237)
+__global__ __launch_bounds__(256) void k_fock(Topo t, int64_t total, const unsigned char* active,
+                                              const double* S, const double* D, const double* Q,
+                                              const double* H0, const double* vsh,
+                                              const double* vd, const double* vqq, double* H) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = t.B;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (t.mat_off[mid] <= g) lo = mid; else hi = mid;
+  }
+  const int s = lo;
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t mo = t.mat_off[s], loc = g - mo, nn = (int64_t)n * n;
+  const int r = (int)(loc / n), c = (int)(loc % n);
+  const int gr = t.ao_off[s] + r, gc = t.ao_off[s] + c;
+  const int ar = t.ao_atom[gr], ac = t.ao_atom[gc];
+  const int64_t f = mo + loc, rvl = (int64_t)c * n + r;
+  const double* Ds = D + 3 * mo;
+  const double* Qs = Q + 6 * mo;
+  double h = -0.5 * S[f] * (vsh[t.ao_sh[gr]] + vsh[t.ao_sh[gc]]);
+#pragma unroll
+  for (int x = 0; x < 3; ++x) h -= 0.5 * (Ds[x * nn + loc] * vd[3 * ac + x] + Ds[x * nn + rvl] * vd[3 * ar + x]);
+#pragma unroll
+  for (int x = 0; x < 6; ++x) h -= 0.5 * (Qs[x * nn + loc] * vqq[6 * ac + x] + Qs[x * nn + rvl] * vqq[6 * ar + x]);
+  H[f] = H0[f] + h;
+}
+
+// ---------------------------------------------------------------------------
+// Fermi occupations per system (reference eigensolver.cpp:992-1161), FP64
+// bisection on the shifted frame; one wavefront per system.
+// ---------------------------------------------------------------------------
+__device__ double fermi_sum_wave(const double* eps, int n, double ref, double mu, double kt,
+                                 bool holes) {
+  double acc = 0.0;
+  for (int i = threadIdx.x & 63; i < n; i += 64) {
+    const double a = ((eps[i] - ref) - mu) / kt;
+    double f;
+    if (holes) f = a >= 0.0 ? 1.0 / (exp(-a) + 1.0) : exp(a) / (1.0 + exp(a));
+    else f = a >= 0.0 ? exp(-a) / (1.0 + exp(-a)) : 1.0 / (exp(a) + 1.0);
+    acc += f;
+  }
+  return wave_sum(acc);
+}
+
+__device__ void fermi_wave(const double* eps, int n, double nel, double kt, double* occ,
+                           double& entropy) {
+  const int lane = threadIdx.x & 63;
+  entropy = 0.0;
+  if (nel == 0.0) {
+    for (int i = lane; i < n; i += 64) occ[i] = 0.0;
+    return;
+  }
+  if (kt == 0.0) {
+    const int full = (int)floor(nel);
+    for (int i = lane; i < n; i += 64) occ[i] = i < full ? 1.0 : (i == full ? nel - full : 0.0);
+    return;
+  }
+  if (nel == (double)n) {
+    for (int i = lane; i < n; i += 64) occ[i] = 1.0;
+    return;
+  }
+  const bool holes = nel > 0.5 * n;
+  const double qt = holes ? n - nel : nel;
+  const double steps = fmax(64.0, -log(qt / n) + 8.0);
+  const double ref = holes ? eps[n - 1] : eps[0];
+  const double lo0 = eps[0] - ref, hi0 = eps[n - 1] - ref;
+  const double margin = steps * kt + 64.0 * 2.220446049250313e-16 * fmax(1.0, fabs(hi0 - lo0));
+  double lo = lo0 - margin, hi = hi0 + margin;
+  const double tol = 64.0 * 2.220446049250313e-16 * qt;
+  for (int it = 0; it < 4096; ++it) {
+    const double mid = lo + 0.5 * (hi - lo);
+    const double q = fermi_sum_wave(eps, n, ref, mid, kt, holes);
+    if (fabs(q - qt) <= tol) { lo = hi = mid; break; }
+    if (mid == lo || mid == hi) break;
+    if ((!holes && q < qt) || (holes && q > qt)) lo = mid; else hi = mid;
+  }
+  const double mu = lo + 0.5 * (hi - lo);
+  double sacc = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const double a = ((eps[i] - ref) - mu) / kt;
+    const double f = a >= 0.0 ? exp(-a) / (1.0 + exp(-a)) : 1.0 / (exp(a) + 1.0);
+    occ[i] = f;
+    if (f > 0.0 && f < 1.0) sacc -= f * log(f) + (1.0 - f) * log(1.0 - f);
+  }
+  entropy = wave_sum(sacc);
+}
+
+// occ_total[i] = f_alpha + f_beta; fe[i] = occ_total * eps; kocc = last nonzero + 1
+__global__ __launch_bounds__(64) void k_fermi(Topo t, const int* order, const unsigned char* active,
+                                              const double* eps_all, double* fa_all, double* fb_all,
+                                              double* occ_all, double* fe_all, SysState* st,
+                                              int* kocc, double kt) {
+  const int s = order[blockIdx.x];
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int o = t.ao_off[s];
+  double sa, sb;
+  fermi_wave(eps_all + o, n, t.nalpha[s], kt, fa_all + o, sa);
+  fermi_wave(eps_all + o, n, t.nbeta[s], kt, fb_all + o, sb);
+  int last = 0;
+  for (int i = threadIdx.x; i < n; i += 64) {
+    const double f = fa_all[o + i] + fb_all[o + i];
+    occ_all[o + i] = f;
+    fe_all[o + i] = f * eps_all[o + i];
+    if (f != 0.0) last = max(last, i + 1);
+  }
+  for (int off = 32; off > 0; off >>= 1) last = max(last, __shfl_xor(last, off, 64));
+  if (threadIdx.x == 0) {
+    st[s].entropy = sa + sb;
+    st[s].kocc = last;
+    kocc[s] = last;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mulliken raw multipoles and Tr(P H0) (reference mulliken_kernels_impl.hpp:40-115).
+// colsum holds [n][10] column partials per system at ao_off*10.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mulliken(Topo t, const int* order,
+                                                  const unsigned char* active, const double* P,
+                                                  const double* S, const double* D,
+                                                  const double* Q, const double* H0,
+                                                  double* colsum, double* raw, SysState* st) {
+  const int s = order[blockIdx.x];
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t mo = t.mat_off[s], nn = (int64_t)n * n;
+  const int a0 = t.atom_off[s], N = t.atom_off[s + 1] - a0;
+  const int s0 = t.shell_off[s], ns = t.shell_off[s + 1] - s0;
+  double* cs = colsum + (int64_t)t.ao_off[s] * 10;
+  __shared__ double red[16];
+  double core = 0.0;
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    double acc[10];
+    for (int k = 0; k < 10; ++k) acc[k] = 0.0;
+    const double* Ds = D + 3 * mo;
+    const double* Qs = Q + 6 * mo;
+    for (int r = 0; r < n; ++r) {
+      const int64_t loc = (int64_t)r * n + c;
+      const int64_t idx = mo + loc;
+      const double p = P[idx];
+      acc[0] = fma(-p, S[idx], acc[0]);
+      for (int k = 0; k < 3; ++k) acc[1 + k] = fma(-p, Ds[k * nn + loc], acc[1 + k]);
+      for (int k = 0; k < 6; ++k) acc[4 + k] = fma(-p, Qs[k * nn + loc], acc[4 + k]);
+      core = fma(H0[idx], p, core);
+    }
+    for (int k = 0; k < 10; ++k) cs[(int64_t)c * 10 + k] = acc[k];
+  }
+  core = block_sum(core, red);
+  __syncthreads();
+  double* rv = raw + t.vec_off[s];
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    const int sh = s0 + i;
+    double q = t.sh_refocc[sh];
+    for (int ao = t.sh_ao[sh]; ao < t.sh_ao[sh] + 2 * t.sh_l[sh] + 1; ++ao)
+      q += cs[(int64_t)(ao - t.ao_off[s]) * 10];
+    rv[i] = q;
+  }
+  for (int i = threadIdx.x; i < N * 9; i += blockDim.x) {
+    const int la = i / 9, k = i % 9, a = a0 + la;
+    double v = 0.0;
+    const int aoa = t.sh_ao[t.atom_sh0[a]];
+    const int aob = t.sh_ao[t.atom_sh0[a + 1] - 1] + 2 * t.sh_l[t.atom_sh0[a + 1] - 1] + 1;
+    for (int ao = aoa; ao < aob; ++ao) v += cs[(int64_t)(ao - t.ao_off[s]) * 10 + 1 + k];
+    if (k < 3) rv[ns + 3 * la + k] = v;
+    else rv[ns + 3 * N + 6 * la + (k - 3)] = v;
+  }
+  if (threadIdx.x == 0) st[s].e_core = core;
+}
+
+}  // namespace xtb

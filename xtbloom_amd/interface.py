@@ -1,0 +1,459 @@
+"""Python host layer of the MI355X backend, mirroring xTBloom's ``Calculator`` API.
+
+The classes keep the reference's names, argument meaning and result keys
+(reference python/xtbloom/interface.py: PointCharge :187, Structure :291,
+Context :500, Result :1433, BatchResult :1506, Calculator :1792,
+BatchCalculator :2028) with the backend map extended by ``"hip"`` ->
+``BACKEND_ROCM`` (reference interface.py:154-158 has auto/cpu/cuda).  Every
+call goes through ``xtbloom_compute`` of ``libxtbloom_amd.so``; there is no
+Python or CPU compute path.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+
+from . import library
+
+_METHODS = {"GFN2-XTB": library.MODEL_GFN2_XTB, "GFN2": library.MODEL_GFN2_XTB,
+            "GFN1-XTB": library.MODEL_GFN1_XTB, "GFN1": library.MODEL_GFN1_XTB}
+_BACKENDS = {"auto": library.BACKEND_AUTO, "cpu": library.BACKEND_CPU,
+             "cuda": library.BACKEND_CUDA, "hip": library.BACKEND_ROCM,
+             "rocm": library.BACKEND_ROCM}
+
+
+class XTBloomError(RuntimeError):
+    """A non-success status from the C ABI."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{library.status_string(status)}: {message}")
+        self.status = status
+
+
+class XTBloomValueError(ValueError):
+    pass
+
+
+def _check(status: int):
+    if status != library.STATUS_SUCCESS:
+        raise XTBloomError(status, library.last_error())
+
+
+@dataclass
+class PointCharge:
+    """External point charges (positions bohr, charges e, gammas Hartree)."""
+    positions: np.ndarray
+    charges: np.ndarray
+    gammas: np.ndarray
+
+    def __post_init__(self):
+        self.positions = np.ascontiguousarray(self.positions, dtype=float)
+        self.charges = np.ascontiguousarray(self.charges, dtype=float)
+        self.gammas = np.ascontiguousarray(self.gammas, dtype=float)
+        if self.positions.ndim != 2 or self.positions.shape[1] != 3:
+            raise XTBloomValueError("point charge positions must have shape (n, 3)")
+        n = len(self.positions)
+        if self.charges.shape != (n,) or self.gammas.shape != (n,) or n == 0:
+            raise XTBloomValueError("point charge values/gammas must match a nonempty position set")
+        if not (np.isfinite(self.positions).all() and np.isfinite(self.charges).all()
+                and np.isfinite(self.gammas).all()):
+            raise XTBloomValueError("point charge inputs must be finite")
+
+
+class Structure:
+    """One molecule: atomic numbers, positions (bohr), charge, uhf, optional point charges."""
+
+    def __init__(self, numbers, positions, charge: float = 0.0, uhf: int | None = None,
+                 point_charges: PointCharge | None = None):
+        self._numbers = np.ascontiguousarray(numbers, dtype=np.int32)
+        self._positions = np.ascontiguousarray(positions, dtype=float).reshape(-1, 3)
+        if self._numbers.ndim != 1 or len(self._numbers) != len(self._positions):
+            raise XTBloomValueError("numbers and positions must describe the same atoms")
+        self._charge = float(charge)
+        self._uhf = int(uhf or 0)
+        self._point_charges = point_charges
+
+    numbers = property(lambda self: self._numbers)
+    positions = property(lambda self: self._positions)
+    charge = property(lambda self: self._charge)
+    uhf = property(lambda self: self._uhf)
+    point_charges = property(lambda self: self._point_charges)
+
+    def update(self, positions=None, point_charges=None):
+        if positions is not None:
+            pos = np.ascontiguousarray(positions, dtype=float).reshape(-1, 3)
+            if pos.shape != self._positions.shape:
+                raise XTBloomValueError("updated positions must keep the atom count")
+            self._positions = pos
+        if point_charges is not None:
+            self._point_charges = point_charges
+
+
+class Context:
+    """Owns one native context (one device, one stream)."""
+
+    def __init__(self, backend: str | int = "hip", device_id: int | None = None, stream=None):
+        lib = library.load()
+        self.backend_code = _BACKENDS[backend] if isinstance(backend, str) else int(backend)
+        opts = library.ContextOptions()
+        _check(lib.xtbloom_context_options_init(ctypes.byref(opts), ctypes.sizeof(opts)))
+        opts.backend = self.backend_code
+        opts.device_id = -1 if device_id is None else int(device_id)
+        opts.stream = stream
+        handle = ctypes.c_void_p()
+        _check(lib.xtbloom_context_create(ctypes.byref(opts), ctypes.byref(handle)))
+        self._handle = handle
+
+    @property
+    def handle(self):
+        return self._handle
+
+    @property
+    def backend(self) -> int:
+        return library.load().xtbloom_context_get_backend(self._handle)
+
+    @property
+    def device_id(self) -> int:
+        return library.load().xtbloom_context_get_device_id(self._handle)
+
+    def close(self):
+        if getattr(self, "_handle", None):
+            library.load().xtbloom_context_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _const(arr, space=library.MEMORY_HOST):
+    b = library.ConstBuffer()
+    if arr is None:
+        return b
+    b.data = arr.ctypes.data
+    b.size_bytes = arr.nbytes
+    b.memory_space = space
+    return b
+
+
+def _out(arr):
+    b = library.Buffer()
+    if arr is None:
+        return b
+    b.data = arr.ctypes.data
+    b.size_bytes = arr.nbytes
+    b.memory_space = library.MEMORY_HOST
+    return b
+
+
+@dataclass
+class ComputedBatch:
+    """Flat outputs of one native call (forces are -dE/dR)."""
+    atom_offsets: np.ndarray
+    point_offsets: np.ndarray | None
+    energies: np.ndarray
+    forces: np.ndarray
+    charges: np.ndarray
+    point_charge_forces: np.ndarray | None
+    dipole_moments: np.ndarray | None
+    scc_iterations: np.ndarray
+    scc_converged: np.ndarray
+    per_system_status: np.ndarray
+    point_charge_potential_gradient: np.ndarray | None = None
+    extension: dict | None = None
+
+
+def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=None, uhf=None,
+                   point_offsets=None, point_positions=None, point_values=None,
+                   point_gammas=None, *, model=library.MODEL_GFN2_XTB,
+                   flags=library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES,
+                   max_scc_iterations=250, charge_tolerance=1e-6, energy_tolerance=1e-8,
+                   electronic_temperature=300.0, scc_mixer_history=8, scc_mixer_damping=0.4,
+                   extension=False, diagnostics=False) -> ComputedBatch:
+    """One ``xtbloom_compute`` over flat ragged arrays (the ``xtbloom_batch_t`` layout)."""
+    lib = library.load()
+    atom_offsets = np.ascontiguousarray(atom_offsets, dtype=np.int64)
+    numbers = np.ascontiguousarray(numbers, dtype=np.int32)
+    positions = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    B, N = len(atom_offsets) - 1, len(numbers)
+    charges = np.zeros(B) if charges is None else np.ascontiguousarray(charges, dtype=np.float64)
+    uhf = np.zeros(B, np.int32) if uhf is None else np.ascontiguousarray(uhf, dtype=np.int32)
+    P = 0
+    if point_offsets is not None:
+        point_offsets = np.ascontiguousarray(point_offsets, dtype=np.int64)
+        point_positions = np.ascontiguousarray(point_positions, dtype=np.float64).reshape(-1, 3)
+        point_values = np.ascontiguousarray(point_values, dtype=np.float64)
+        point_gammas = np.ascontiguousarray(point_gammas, dtype=np.float64)
+        P = len(point_values)
+        if P == 0:
+            point_offsets = point_positions = point_values = point_gammas = None
+    batch = library.Batch()
+    _check(lib.xtbloom_batch_init(ctypes.byref(batch), ctypes.sizeof(batch)))
+    batch.batch_size, batch.total_atoms, batch.total_point_charges = B, N, P
+    batch.atom_offsets = _const(atom_offsets)
+    batch.atomic_numbers = _const(numbers)
+    batch.positions = _const(positions)
+    batch.molecular_charges = _const(charges)
+    batch.unpaired_electrons = _const(uhf)
+    if P:
+        batch.point_charge_offsets = _const(point_offsets)
+        batch.point_charge_positions = _const(point_positions)
+        batch.point_charge_values = _const(point_values)
+        batch.point_charge_gammas = _const(point_gammas)
+    opts = library.ComputeOptions()
+    _check(lib.xtbloom_compute_options_init(ctypes.byref(opts), ctypes.sizeof(opts)))
+    opts.model = model
+    opts.flags = flags if P else flags & ~library.COMPUTE_POINT_CHARGE_FORCES
+    opts.max_scc_iterations = max_scc_iterations
+    opts.charge_tolerance = charge_tolerance
+    opts.energy_tolerance = energy_tolerance
+    opts.electronic_temperature = electronic_temperature * library.KELVIN_TO_HARTREE
+    opts.scc_mixer_history = scc_mixer_history
+    opts.scc_mixer_damping = scc_mixer_damping
+    energies = np.full(B, np.nan)
+    forces = np.full((N, 3), np.nan)
+    q = np.full(N, np.nan)
+    pcf = np.full((P, 3), np.nan) if P else None
+    dip = np.full((B, 3), np.nan) if flags & library.COMPUTE_DIPOLE_MOMENTS else None
+    iters = np.zeros(B, np.int32)
+    conv = np.zeros(B, np.uint8)
+    status = np.zeros(B, np.int32)
+    res = library.BatchResult()
+    _check(lib.xtbloom_batch_result_init(ctypes.byref(res), ctypes.sizeof(res)))
+    res.energies = _out(energies)
+    res.forces = _out(forces)
+    res.atomic_charges = _out(q)
+    res.point_charge_forces = _out(pcf)
+    res.dipole_moments = _out(dip)
+    res.scc_iterations = _out(iters)
+    res.scc_converged = _out(conv)
+    res.per_system_status = _out(status)
+    ext_out = None
+    dedq = None
+    if extension or diagnostics:
+        ext = library.Extension()
+        ext.struct_size = ctypes.sizeof(ext)
+        ext.api_version = library.API_VERSION
+        bufs = {}
+        if P:
+            dedq = np.full(P, np.nan)
+            ext.point_charge_potential_gradient = dedq.ctypes.data
+        if diagnostics:
+            nn = int(lib.xtbloom_amd_packed_matrix_elements(ctypes.byref(batch)))
+            for name in ("overlap", "core_hamiltonian", "density"):
+                bufs[name] = np.zeros(nn)
+                setattr(ext, name, bufs[name].ctypes.data)
+            bufs["coordination_numbers"] = np.zeros(N)
+            ext.coordination_numbers = bufs["coordination_numbers"].ctypes.data
+            bufs["energy_components"] = np.zeros((B, 8))
+            ext.energy_components = bufs["energy_components"].ctypes.data
+        _check(lib.xtbloom_amd_compute_extended(context.handle, ctypes.byref(batch),
+                                                ctypes.byref(opts), ctypes.byref(res),
+                                                ctypes.byref(ext)))
+        ext_out = dict(bufs, setup_ms=ext.setup_ms, scc_ms=ext.scc_ms, force_ms=ext.force_ms,
+                       total_ms=ext.total_ms, scc_rounds=ext.scc_rounds)
+    else:
+        _check(lib.xtbloom_compute(context.handle, ctypes.byref(batch), ctypes.byref(opts),
+                                   ctypes.byref(res)))
+    return ComputedBatch(atom_offsets, point_offsets, energies, forces, q, pcf, dip, iters,
+                         conv.astype(bool), status, dedq, ext_out)
+
+
+class Result:
+    """Single-system results; keys as in the reference (``gradient`` = -forces)."""
+
+    def __init__(self, computed: ComputedBatch, index: int = 0):
+        b, e = int(computed.atom_offsets[index]), int(computed.atom_offsets[index + 1])
+        self.energy = float(computed.energies[index])
+        self.forces = computed.forces[b:e].copy()
+        self.charges = computed.charges[b:e].copy()
+        self.point_charge_forces = None
+        self.point_charge_potential_gradient = None
+        if computed.point_charge_forces is not None:
+            pb, pe = int(computed.point_offsets[index]), int(computed.point_offsets[index + 1])
+            self.point_charge_forces = computed.point_charge_forces[pb:pe].copy()
+            if computed.point_charge_potential_gradient is not None:
+                self.point_charge_potential_gradient = \
+                    computed.point_charge_potential_gradient[pb:pe].copy()
+        self.dipole_moments = (computed.dipole_moments[index].copy()
+                               if computed.dipole_moments is not None else None)
+        self.scc_iterations = int(computed.scc_iterations[index])
+        self.scc_converged = bool(computed.scc_converged[index])
+        self.scc_status = int(computed.per_system_status[index])
+        self._natoms = e - b
+
+    _keys = ("energy", "energies", "forces", "gradient", "charges", "point_charge_forces",
+             "dipole_moments", "scc_iterations", "scc_converged", "scc_status", "natoms")
+
+    def get(self, attribute: str):
+        if attribute == "energies":
+            return np.asarray([self.energy])
+        if attribute == "gradient":
+            return -self.forces
+        if attribute == "natoms":
+            return self._natoms
+        if attribute not in self._keys:
+            raise XTBloomValueError(f"attribute {attribute!r} is not available in this result")
+        return getattr(self, attribute)
+
+    __getitem__ = get
+
+    def dict(self):
+        return {k: self.get(k) for k in self._keys}
+
+
+class BatchResult:
+    def __init__(self, computed: ComputedBatch):
+        self._computed = computed
+        self.energies = computed.energies
+        self.scc_iterations = computed.scc_iterations
+        self.scc_converged = computed.scc_converged
+        self.per_system_status = computed.per_system_status
+
+    def __len__(self):
+        return len(self.energies)
+
+    def __getitem__(self, index: int) -> Result:
+        return Result(self._computed, index)
+
+    def failed_indices(self) -> np.ndarray:
+        return np.nonzero(self.per_system_status != library.STATUS_SUCCESS)[0]
+
+    def raise_for_status(self):
+        bad = self.failed_indices()
+        if len(bad):
+            raise XTBloomError(int(self.per_system_status[bad[0]]),
+                               f"{len(bad)} system(s) failed, first index {int(bad[0])}")
+
+    def get(self, attribute: str):
+        c = self._computed
+        table = {"energies": c.energies, "forces": c.forces, "gradient": -c.forces,
+                 "charges": c.charges, "point_charge_forces": c.point_charge_forces,
+                 "dipole_moments": c.dipole_moments, "scc_iterations": c.scc_iterations,
+                 "scc_converged": c.scc_converged, "scc_status": c.per_system_status}
+        if attribute not in table:
+            raise XTBloomValueError(f"attribute {attribute!r} is not available in this result")
+        return table[attribute]
+
+
+def _flatten(structures: Sequence[Structure]):
+    offs = np.zeros(len(structures) + 1, np.int64)
+    offs[1:] = np.cumsum([len(s.numbers) for s in structures])
+    numbers = np.concatenate([s.numbers for s in structures])
+    positions = np.concatenate([s.positions for s in structures])
+    charges = np.array([s.charge for s in structures])
+    uhf = np.array([s.uhf for s in structures], np.int32)
+    kw = {}
+    if any(s.point_charges is not None for s in structures):
+        poff = np.zeros(len(structures) + 1, np.int64)
+        poff[1:] = np.cumsum([0 if s.point_charges is None else len(s.point_charges.charges)
+                              for s in structures])
+        pcs = [s.point_charges for s in structures if s.point_charges is not None]
+        kw = dict(point_offsets=poff,
+                  point_positions=np.concatenate([p.positions for p in pcs]),
+                  point_values=np.concatenate([p.charges for p in pcs]),
+                  point_gammas=np.concatenate([p.gammas for p in pcs]))
+    return offs, numbers, positions, charges, uhf, kw
+
+
+class _Settings:
+    def __init__(self, **kw):
+        self.max_scc_iterations = kw.get("max_scc_iterations", 250)
+        self.charge_tolerance = kw.get("charge_tolerance", 1.0e-6)
+        self.energy_tolerance = kw.get("energy_tolerance", 1.0e-8)
+        self.electronic_temperature = kw.get("electronic_temperature", 300.0)
+        self.scc_mixer_history = kw.get("scc_mixer_history", library.DEFAULT_SCC_MIXER_HISTORY)
+        self.scc_mixer_damping = kw.get("scc_mixer_damping", library.DEFAULT_SCC_MIXER_DAMPING)
+
+    def set(self, attribute, value):
+        aliases = {"max-iter": "max_scc_iterations", "temperature": "electronic_temperature"}
+        attribute = aliases.get(attribute, attribute)
+        if not hasattr(self, attribute):
+            raise XTBloomValueError(f"unknown setting {attribute!r}")
+        setattr(self, attribute, value)
+
+    def kwargs(self):
+        return dict(vars(self))
+
+
+def _resolve_method(method: str) -> int:
+    key = method.upper()
+    if key not in _METHODS:
+        raise XTBloomValueError(f"unknown method {method!r}")
+    return _METHODS[key]
+
+
+class Calculator(Structure):
+    """Single-point GFN2-xTB calculator on the MI355X backend (tblite-like API)."""
+
+    def __init__(self, method: str, numbers, positions, charge: float = 0.0, uhf: int | None = None,
+                 point_charges: PointCharge | None = None, *, backend: str | int = "hip",
+                 device_id: int | None = None, **settings):
+        super().__init__(numbers, positions, charge, uhf, point_charges)
+        self._model = _resolve_method(method)
+        self._method = method
+        self._context = Context(backend, device_id)
+        self._settings = _Settings(**settings)
+
+    @property
+    def backend(self) -> int:
+        return self._context.backend
+
+    @property
+    def method(self) -> str:
+        return self._method
+
+    def set(self, attribute: str, value):
+        self._settings.set(attribute, value)
+
+    def singlepoint(self) -> Result:
+        flags = library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES
+        if self.point_charges is not None:
+            flags |= library.COMPUTE_POINT_CHARGE_FORCES
+        offs, numbers, positions, charges, uhf, kw = _flatten([self])
+        computed = compute_arrays(self._context, offs, numbers, positions, charges, uhf, **kw,
+                                  model=self._model, flags=flags, extension=True,
+                                  **self._settings.kwargs())
+        BatchResult(computed).raise_for_status()
+        return Result(computed, 0)
+
+    def close(self):
+        self._context.close()
+
+
+class BatchCalculator:
+    """Batched GFN2-xTB over many structures in one native call."""
+
+    def __init__(self, structures: Sequence[Structure], method: str = "GFN2-xTB", *,
+                 backend: str | int = "hip", device_id: int | None = None, **settings):
+        if not structures:
+            raise XTBloomValueError("a batch needs at least one structure")
+        self._structures = list(structures)
+        self._model = _resolve_method(method)
+        self._context = Context(backend, device_id)
+        self._settings = _Settings(**settings)
+
+    @property
+    def backend(self) -> int:
+        return self._context.backend
+
+    def set(self, attribute, value):
+        self._settings.set(attribute, value)
+
+    def compute(self, *, flags=None) -> BatchResult:
+        offs, numbers, positions, charges, uhf, kw = _flatten(self._structures)
+        if flags is None:
+            flags = library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES
+            if kw:
+                flags |= library.COMPUTE_POINT_CHARGE_FORCES
+        computed = compute_arrays(self._context, offs, numbers, positions, charges, uhf, **kw,
+                                  model=self._model, flags=flags, **self._settings.kwargs())
+        return BatchResult(computed)
+
+    def close(self):
+        self._context.close()
