@@ -15,6 +15,42 @@
 extern "C" {
 #endif
 
+/* Kernel families of one call, the index space of kernel_stats below. */
+typedef enum xtbloom_amd_kernel {
+  XTBLOOM_AMD_KERNEL_GEOMETRY = 0,   /* CN, D4 CN, ES2 Gamma, point-charge potential, SAD */
+  XTBLOOM_AMD_KERNEL_INTEGRALS = 1,  /* S, D, Q, H0 per shell pair */
+  XTBLOOM_AMD_KERNEL_CHOLESKY = 2,   /* S = L L^T, X = L^-T */
+  XTBLOOM_AMD_KERNEL_POTENTIALS = 3, /* ES2/ES3/AES2/D4 potentials */
+  XTBLOOM_AMD_KERNEL_FOCK = 4,       /* H = H0 + multipole potential terms */
+  XTBLOOM_AMD_KERNEL_REDUCE = 5,     /* H' = X^T H X (two GEMM launches) */
+  XTBLOOM_AMD_KERNEL_TRIDIAG = 6,    /* Householder tridiagonalization */
+  XTBLOOM_AMD_KERNEL_FERMI = 7,      /* occupations and entropy */
+  XTBLOOM_AMD_KERNEL_BACK = 8,       /* C = X Y */
+  XTBLOOM_AMD_KERNEL_DENSITY = 9,    /* P = C f C^T */
+  XTBLOOM_AMD_KERNEL_MULLIKEN = 10,  /* shell charges and atomic multipoles */
+  XTBLOOM_AMD_KERNEL_ENERGY_MIX = 11,/* SCC energy, Broyden step, convergence */
+  XTBLOOM_AMD_KERNEL_ENERGY_WEIGHTED = 12, /* W = C f eps C^T */
+  XTBLOOM_AMD_KERNEL_FORCE_PAIRS = 13,     /* derivative integrals x adjoints */
+  XTBLOOM_AMD_KERNEL_FORCE_ATOMS = 14,     /* AES2/D4/ES2/CN/repulsion/point-charge gradients */
+  XTBLOOM_AMD_KERNEL_FORCE_ATM = 15,       /* D4 three-body energy and gradient */
+  XTBLOOM_AMD_KERNEL_TRIDIAG_EIGEN = 16,   /* bisection + inverse iteration on T */
+  XTBLOOM_AMD_KERNEL_BACKTRANSFORM = 17,   /* eigenvectors of H' = Q Z */
+  XTBLOOM_AMD_KERNEL_COUNT = 18
+} xtbloom_amd_kernel_t;
+
+/* Device time of one kernel family, summed over the call's launches (HIP
+ * events recorded on the context stream around each launch), and the
+ * algorithmic work of those launches: FP64-equivalent flops for the dense
+ * linear algebra, minimal HBM bytes for the streaming kernels (0 when the
+ * family has no figure). */
+typedef struct xtbloom_amd_kernel_stat {
+  int32_t launches;
+  int32_t reserved;
+  double total_ms;
+  double flops;
+  double bytes;
+} xtbloom_amd_kernel_stat_t;
+
 /* Host buffers; every pointer may be NULL.  Matrices are packed batch-major
  * (system s occupies sum_{r<s} n_r^2 .. + n_s^2, row-major), sized by
  * xtbloom_amd_packed_matrix_elements(). */
@@ -27,6 +63,7 @@ typedef struct xtbloom_amd_extension {
   double* density;                         /* [sum n^2] converged P */
   double* coordination_numbers;            /* [N] GFN2 CN */
   double* energy_components;               /* [8B] core, es2, es3, aes2, d4, pc, rep, atm */
+  xtbloom_amd_kernel_stat_t* kernel_stats; /* [XTBLOOM_AMD_KERNEL_COUNT]; NULL = no per-launch events */
   /* device timing of the call, milliseconds (outputs) */
   float setup_ms;
   float scc_ms;

@@ -42,7 +42,7 @@ def compare(batch, out, label, opts=None, iter_tol=1):
     worst = dict(E=0.0, F=0.0, q=0.0, PF=0.0, it=0)
     for i in range(batch.batch_size):
         s = batch.system(i)
-        kw = {k: s[k] for k in ("pc_xyz", "pc_q", "pc_gamma") if k in s}
+        kw = {k: s[k] for k in ("pc_xyz", "pc_q", "pc_gamma") if k in s and len(s["pc_q"])}
         ref = O.singlepoint(s["Z"], s["xyz"], charge=s["charge"], uhf=s["uhf"], opts=opts, **kw)
         a0, a1 = batch.atom_offsets[i], batch.atom_offsets[i + 1]
         assert ref.status == 0, (label, i)
@@ -51,7 +51,7 @@ def compare(batch, out, label, opts=None, iter_tol=1):
         worst["F"] = max(worst["F"], np.abs(out.forces[a0:a1] + ref.gradient).max())
         worst["q"] = max(worst["q"], np.abs(out.charges[a0:a1] - ref.charges).max())
         worst["it"] = max(worst["it"], abs(int(out.scc_iterations[i]) - ref.iterations))
-        if "pc_q" in s:
+        if kw:
             p0, p1 = batch.point_offsets[i], batch.point_offsets[i + 1]
             worst["PF"] = max(worst["PF"],
                               np.abs(out.point_charge_forces[p0:p1] + ref.pc_gradient).max())

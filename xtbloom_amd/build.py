@@ -47,7 +47,8 @@ def build(verbose: bool = False) -> str:
         with ThreadPoolExecutor(max_workers=len(SOURCES)) as pool:
             objs = list(pool.map(one, SOURCES))
         tmp_lib = LIB_PATH + ".tmp"
-        cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp_lib] + objs
+        cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC",
+               "-Wl,--version-script=" + os.path.join(CSRC, "xtbloom.map"), "-o", tmp_lib] + objs
         subprocess.run(cmd, check=True)
         os.replace(tmp_lib, LIB_PATH)
     return LIB_PATH

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -18,6 +19,11 @@
 #include "../../include/xtbloom/xtbloom.h"
 #include "../../include/xtbloom_amd/extensions.h"
 #include "executor.hpp"
+
+static_assert(sizeof(xtb::KernelStat) == sizeof(xtbloom_amd_kernel_stat_t), "kernel stat layout");
+static_assert(offsetof(xtb::KernelStat, bytes) == offsetof(xtbloom_amd_kernel_stat_t, bytes),
+              "kernel stat layout");
+static_assert((int)xtb::kKernelCount == (int)XTBLOOM_AMD_KERNEL_COUNT, "kernel family count");
 
 struct xtbloom_context {
   xtb::Device* device = nullptr;
@@ -289,6 +295,7 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, const xtbloom_batch_t*
     out.dbg_density = ext->density;
     out.dbg_cn = ext->coordination_numbers;
     out.dbg_components = ext->energy_components;
+    out.kernel_stats = reinterpret_cast<xtb::KernelStat*>(ext->kernel_stats);
   }
   std::string err;
   xtb::RunTiming timing;

@@ -32,6 +32,20 @@ struct RunOptions {
   bool forces = true;
 };
 
+// Per-kernel-family timing record (layout of xtbloom_amd_kernel_stat_t).
+struct KernelStat {
+  int32_t launches;
+  int32_t reserved;
+  double total_ms;
+  double flops;
+  double bytes;
+};
+enum KernelKind {
+  kKGeometry, kKIntegrals, kKCholesky, kKPotentials, kKFock, kKReduce, kKTridiag, kKFermi, kKBack,
+  kKDensity, kKMulliken, kKEnergyMix, kKEnergyWeighted, kKForcePairs, kKForceAtoms, kKForceAtm,
+  kKTridiagEigen, kKBacktransform, kKernelCount
+};
+
 // Every pointer is optional (host memory); failed systems receive NaN slices.
 struct HostOutputs {
   double* energies = nullptr;       // [B]
@@ -49,6 +63,7 @@ struct HostOutputs {
   double* dbg_density = nullptr;    // [sum n^2]
   double* dbg_cn = nullptr;         // [N]
   double* dbg_components = nullptr; // [B*8] core es2 es3 aes2 d4 pc rep atm
+  KernelStat* kernel_stats = nullptr; // [kKernelCount]; enables per-launch events
 };
 
 struct Device;

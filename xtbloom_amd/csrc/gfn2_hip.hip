@@ -243,6 +243,7 @@ struct Device {
   char* arena = nullptr;
   size_t arena_bytes = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<hipEvent_t> probe_pool;  // per-launch timing events, grow-only
 };
 
 Device* device_create(int device_id, void* stream, std::string& error) {
@@ -293,17 +294,18 @@ Device* device_create(int device_id, void* stream, std::string& error) {
 
 void device_destroy(Device* d) {
   if (!d) return;
-  hipSetDevice(d->id);
-  if (d->stream) hipStreamSynchronize(d->stream);
-  hipFree(d->d4_c6);
-  hipFree(d->d4_refcn);
-  hipFree(d->d4_refq);
-  hipFree(d->d4_refng);
-  hipFree(d->kpair);
-  hipFree(d->arena);
+  (void)hipSetDevice(d->id);
+  if (d->stream) (void)hipStreamSynchronize(d->stream);
+  (void)hipFree(d->d4_c6);
+  (void)hipFree(d->d4_refcn);
+  (void)hipFree(d->d4_refq);
+  (void)hipFree(d->d4_refng);
+  (void)hipFree(d->kpair);
+  (void)hipFree(d->arena);
   for (auto& e : d->ev)
-    if (e) hipEventDestroy(e);
-  if (d->own_stream && d->stream) hipStreamDestroy(d->stream);
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : d->probe_pool) (void)hipEventDestroy(e);
+  if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
   delete d;
 }
 
@@ -381,8 +383,10 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   const int NV = T.vec_off[B];
   const int mem = std::max(1, opt.history);
 
-  // GEMM tile queues
+  // GEMM tile queues and eigenvector back-transformation column blocks
   std::vector<TileTask> full, lower;
+  std::vector<BackTask> back;
+  size_t back_lds = 0;
   for (int i = 0; i < B; ++i) {
     const int s = T.order[i];
     const int n = T.ao_off[s + 1] - T.ao_off[s];
@@ -392,6 +396,10 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
         full.push_back({s, a, b});
         if (a >= b) lower.push_back({s, a, b});
       }
+    int cw = 64;
+    while (cw > 4 && (size_t)(n * cw + n + 256) * sizeof(double) > kBackLdsBytes) cw /= 2;
+    back_lds = std::max(back_lds, (size_t)(n * cw + n + 256) * sizeof(double));
+    for (int j0 = 0; j0 < n; j0 += cw) back.push_back({s, j0, cw, 0});
   }
 
   // ---- arena layout ----
@@ -413,6 +421,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(T.NP); addd(T.NP);
   addd(3 * (size_t)NPC); addd(NPC); addd(NPC);
   bytes += plan_bytes<TileTask>(full.size()) + plan_bytes<TileTask>(lower.size());
+  bytes += plan_bytes<BackTask>(back.size());
   // work arrays
   const int kMat = 20;  // S, D3, Q6, H0, X, H, A, Z, Y, C, P, W, T
   addd((size_t)kMat * NN);
@@ -423,6 +432,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(NS); addd(NS);          // vpc, vsh
   addd(3 * (size_t)N); addd(6 * (size_t)N);
   for (int k = 0; k < 5; ++k) addd(NAO);  // eps fa fb occ fe
+  addd(NAO); addd(NAO); addi(NAO);        // tridiagonal d, e and eigenvalue ranks
   addd((size_t)NAO * 10);      // mulliken column partials
   for (int k = 0; k < 5; ++k) addd(NV); // x raw cur prev prev_res
   addd((size_t)mem * NV); addd((size_t)mem * NV);  // df, u
@@ -436,7 +446,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(B); addd(B);            // erep, eatm
   addd(2 * (size_t)NAP);       // ATM pair C6 staging
   if (bytes > dev->arena_bytes) {
-    hipFree(dev->arena);
+    (void)hipFree(dev->arena);
     dev->arena = nullptr;
     dev->arena_bytes = 0;
     const size_t want = bytes + bytes / 4;
@@ -488,6 +498,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   double* d_pcg = cv.take<double>(NPC);
   TileTask* d_full = cv.take<TileTask>(full.size());
   TileTask* d_lower = cv.take<TileTask>(lower.size());
+  BackTask* d_back = cv.take<BackTask>(back.size());
   HIP_TRY(up(d_atom_off, T.atom_off));
   HIP_TRY(up(d_shell_off, T.shell_off));
   HIP_TRY(up(d_ao_off, T.ao_off));
@@ -526,6 +537,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   HIP_TRY(up(d_pcg, T.pc_gam));
   HIP_TRY(up(d_full, full));
   HIP_TRY(up(d_lower, lower));
+  HIP_TRY(up(d_back, back));
 
   t.atom_off = d_atom_off; t.shell_off = d_shell_off; t.ao_off = d_ao_off; t.vec_off = d_vec_off;
   t.pc_off = d_pc_off; t.mat_off = d_mat_off; t.gam_off = d_gam_off; t.apair_off = d_apair_off;
@@ -578,6 +590,9 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   double* d_fb = cv.take<double>(NAO);
   double* d_occ = cv.take<double>(NAO);
   double* d_fe = cv.take<double>(NAO);
+  double* d_td = cv.take<double>(NAO);
+  double* d_te = cv.take<double>(NAO);
+  int* d_rank = cv.take<int>(NAO);
   double* d_colsum = cv.take<double>((size_t)NAO * 10);
   MixerArrays mx{};
   mx.x = cv.take<double>(NV);
@@ -611,18 +626,66 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   HIP_TRY(hipMemsetAsync(mx.u, 0, sizeof(double) * (size_t)mem * NV, sm));
   HIP_TRY(hipMemsetAsync(mx.omega, 0, sizeof(double) * (size_t)mem * B, sm));
 
+  // ---- optional per-launch timing (HIP events on this stream) ----
+  KernelStat* ks = out.kernel_stats;
+  struct ProbeRec {
+    int kind;
+    double flops, bytes;
+  };
+  std::vector<ProbeRec> probes;
+  auto probe_begin = [&]() -> hipError_t {
+    if (!ks) return hipSuccess;
+    while (dev->probe_pool.size() < 2 * (probes.size() + 1)) {
+      hipEvent_t e;
+      const hipError_t err = hipEventCreate(&e);
+      if (err != hipSuccess) return err;
+      dev->probe_pool.push_back(e);
+    }
+    return hipEventRecord(dev->probe_pool[2 * probes.size()], sm);
+  };
+  auto probe_end = [&](int kind, double flops, double bytes) -> hipError_t {
+    if (!ks) return hipSuccess;
+    const hipError_t err = hipEventRecord(dev->probe_pool[2 * probes.size() + 1], sm);
+    probes.push_back({kind, flops, bytes});
+    return err;
+  };
+  // algorithmic work of a system subset: sum n^2 and sum n^3
+  std::vector<unsigned char> h_active(B, 1);
+  auto work = [&](const std::vector<unsigned char>& mask, double& n2, double& n3) {
+    n2 = n3 = 0.0;
+    for (int s = 0; s < B; ++s) {
+      if (!mask[s]) continue;
+      const double n = T.ao_off[s + 1] - T.ao_off[s];
+      n2 += n * n;
+      n3 += n * n * n;
+    }
+  };
+  double all2, all3;
+  work(h_active, all2, all3);
+
   HIP_TRY(hipEventRecord(dev->ev[0], sm));
   // ---- geometry, integrals, overlap factorization ----
+  HIP_TRY(probe_begin());
   hipLaunchKernelGGL(k_geometry, dim3(B), dim3(256), 0, sm, t, d_order, d_cn, d_cnd4, d_mrad,
                      d_dmrad, d_gam, d_vpc, mx, d_st, d_charges);
-  if (NSP > 0)
+  HIP_TRY(probe_end(kKGeometry, 0.0, 0.0));
+  if (NSP > 0) {
+    HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_integrals, dim3((unsigned)((NSP + 127) / 128)), dim3(128), 0, sm, t,
                        d_sp_off, NSP, d_cn, dS, dD, dQ, dH0);
+    HIP_TRY(probe_end(kKIntegrals, 0.0, 88.0 * all2));  // S + 3 D + 6 Q + H0 written
+  }
+  HIP_TRY(probe_begin());
   hipLaunchKernelGGL(k_cholesky_inverse, dim3(B), dim3(256), 0, sm, t, d_order, dS, dA, dX, d_st);
+  HIP_TRY(probe_end(kKCholesky, all3 / 3.0, 0.0));
   hipLaunchKernelGGL(k_init_active, dim3((B + 255) / 256), dim3(256), 0, sm, B, d_st, d_active,
                      (int*)nullptr, d_st);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(dev->ev[1], sm));
+  if (ks) {
+    HIP_TRY(hipMemcpyAsync(h_active.data(), d_active, B, hipMemcpyDeviceToHost, sm));
+    HIP_TRY(hipStreamSynchronize(sm));
+  }
 
   // ---- SCC rounds ----
   const unsigned nfull = (unsigned)full.size(), nlower = (unsigned)lower.size();
@@ -630,40 +693,78 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   int* h_count = nullptr;
   HIP_TRY(hipHostMalloc(&h_count, sizeof(int)));
   for (int it = 0; it < opt.max_iterations; ++it) {
+    double a2 = 0.0, a3 = 0.0;
+    if (ks) work(h_active, a2, a3);
     HIP_TRY(hipMemsetAsync(d_count, 0, sizeof(int), sm));
+    HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_potentials, dim3(B), dim3(256), 0, sm, t, d_order, d_active, mx.x,
                        d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, d_dw4, d_vat, d_vsh, d_vd, d_vqq);
+    HIP_TRY(probe_end(kKPotentials, 0.0, 0.0));
+    HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_fock, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, sm, t, NN,
                        d_active, dS, dD, dQ, dH0, d_vsh, d_vd, d_vqq, dH);
+    HIP_TRY(probe_end(kKFock, 0.0, 96.0 * a2));  // reads S, D(3), Q(6), H0; writes H
     // H' = X^T H X
+    HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<true, false, false>), dim3(nfull), dim3(256), 0, sm, t, d_full, dX,
                        dH, dT, (const double*)nullptr, (const int*)nullptr, d_active);
+    HIP_TRY(probe_end(kKReduce, a3, 0.0));
+    HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<false, false, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower,
                        dT, dX, dA, (const double*)nullptr, (const int*)nullptr, d_active);
-    hipLaunchKernelGGL(k_eigh, dim3(B), dim3(256), 0, sm, t, d_order, dA, dZ, dY, d_eps, d_st,
-                       d_active);
+    HIP_TRY(probe_end(kKReduce, a3, 0.0));
+    // eigenpairs of H' (A is overwritten by the reflectors; T, H, C, P, W serve as scratch)
+    HIP_TRY(probe_begin());
+    hipLaunchKernelGGL(k_tridiag, dim3(B), dim3(256), 0, sm, t, d_order, dA, d_td, d_te, d_active);
+    HIP_TRY(probe_end(kKTridiag, (4.0 / 3.0) * a3, 0.0));
+    HIP_TRY(probe_begin());
+    hipLaunchKernelGGL(k_tridiag_eigen, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_td, d_te,
+                       dZ, EigenScratch{dT, dH, dC, dP, dW}, d_rank, d_eps);
+    HIP_TRY(probe_end(kKTridiagEigen, 0.0, 0.0));
+    HIP_TRY(probe_begin());
+    hipLaunchKernelGGL(k_backtransform, dim3((unsigned)back.size()), dim3(256), back_lds, sm, t, d_back,
+                       dA, dZ, dY, d_rank, d_active);
+    HIP_TRY(probe_end(kKBacktransform, 2.0 * a3, 0.0));
+    HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_fermi, dim3(B), dim3(64), 0, sm, t, d_order, d_active, d_eps, d_fa, d_fb,
                        d_occ, d_fe, d_st, d_kocc, opt.kt);
-    // C = X Y (Y column-major -> read transposed), P = C diag(f) C^T
-    hipLaunchKernelGGL((k_gemm<false, true, false>), dim3(nfull), dim3(256), 0, sm, t, d_full, dX,
+    HIP_TRY(probe_end(kKFermi, 0.0, 0.0));
+    // C = X Y (columns of Y are the eigenvectors), P = C diag(f) C^T
+    HIP_TRY(probe_begin());
+    hipLaunchKernelGGL((k_gemm<false, false, false>), dim3(nfull), dim3(256), 0, sm, t, d_full, dX,
                        dY, dC, (const double*)nullptr, (const int*)nullptr, d_active);
+    HIP_TRY(probe_end(kKBack, a3, 0.0));
+    HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
                        dC, dP, d_occ, d_kocc, d_active);
+    HIP_TRY(probe_end(kKDensity, 2.0 * a3, 0.0));
+    HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_mulliken, dim3(B), dim3(256), 0, sm, t, d_order, d_active, dP, dS, dD, dQ,
                        dH0, d_colsum, mx.raw, d_st);
+    HIP_TRY(probe_end(kKMulliken, 0.0, 88.0 * a2));  // reads P, S, D(3), Q(6), H0
+    HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_energy_mix, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_cnd4,
                        d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_count);
+    HIP_TRY(probe_end(kKEnergyMix, 0.0, 0.0));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, sm));
+    if (ks) HIP_TRY(hipMemcpyAsync(h_active.data(), d_active, B, hipMemcpyDeviceToHost, sm));
     HIP_TRY(hipStreamSynchronize(sm));
     ++rounds;
     if (*h_count == 0) break;
   }
-  hipHostFree(h_count);
+  (void)hipHostFree(h_count);
   HIP_TRY(hipEventRecord(dev->ev[2], sm));
 
   // ---- stationary energy and forces ----
   hipLaunchKernelGGL(k_valid, dim3((B + 255) / 256), dim3(256), 0, sm, B, d_st, d_valid);
+  std::vector<unsigned char> h_valid(B, 1);
+  double v2 = 0.0, v3 = 0.0;
+  if (ks) {
+    HIP_TRY(hipMemcpyAsync(h_valid.data(), d_valid, B, hipMemcpyDeviceToHost, sm));
+    HIP_TRY(hipStreamSynchronize(sm));
+    work(h_valid, v2, v3);
+  }
   HIP_TRY(hipMemsetAsync(d_grad, 0, sizeof(double) * 3 * (size_t)N, sm));
   HIP_TRY(hipMemsetAsync(d_dedcn, 0, sizeof(double) * (size_t)N, sm));
   HIP_TRY(hipMemsetAsync(d_erep, 0, sizeof(double) * (size_t)B, sm));
@@ -673,21 +774,32 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(hipMemsetAsync(d_dedq, 0, sizeof(double) * (size_t)NPC, sm));
   }
   // potentials at the converged raw multipoles (reference gfn2_cpu_execution.cpp:1358-1441)
+  HIP_TRY(probe_begin());
   hipLaunchKernelGGL(k_potentials, dim3(B), dim3(256), 0, sm, t, d_order, d_valid, mx.x, d_cnd4,
                      d_mrad, d_gam, d_vpc, d_qat, d_w4, d_dw4, d_vat, d_vsh, d_vd, d_vqq);
+  HIP_TRY(probe_end(kKPotentials, 0.0, 0.0));
   if (opt.forces) {
     // W = C diag(f eps) C^T
+    HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
                        dC, dW, d_fe, d_kocc, d_valid);
-    if (NAP > 0)
+    HIP_TRY(probe_end(kKEnergyWeighted, 2.0 * v3, 0.0));
+    if (NAP > 0) {
+      HIP_TRY(probe_begin());
       hipLaunchKernelGGL(k_force_pairs, dim3((unsigned)((NAP + 127) / 128)), dim3(128), 0, sm, t,
                          NAP, d_valid, d_cn, dP, dW, dS, d_vsh, d_vd, d_vqq, d_grad, d_dedcn);
+      HIP_TRY(probe_end(kKForcePairs, 0.0, 24.0 * v2));  // reads P, W, S
+    }
   }
+  HIP_TRY(probe_begin());
   hipLaunchKernelGGL(k_force_atoms, dim3(B), dim3(256), 0, sm, t, d_order, d_valid, mx.x, d_qat,
                      d_cnd4, d_mrad, d_dmrad, d_gam, d_w4, d_dw4, d_dedcn, d_dedcn4, d_grad,
                      d_erep, d_pcgrad, d_dedq);
+  HIP_TRY(probe_end(kKForceAtoms, 0.0, 0.0));
+  HIP_TRY(probe_begin());
   hipLaunchKernelGGL(k_force_atm, dim3(B), dim3(256), 0, sm, t, d_order, d_valid, d_cnd4, d_w4,
                      d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm);
+  HIP_TRY(probe_end(kKForceAtm, 0.0, 0.0));
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(dev->ev[3], sm));
 
@@ -717,11 +829,23 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(hipMemcpyAsync(out.dbg_cn, d_cn, sizeof(double) * N, hipMemcpyDeviceToHost, sm));
   HIP_TRY(hipStreamSynchronize(sm));
   if (timing) {
-    hipEventElapsedTime(&timing->setup_ms, dev->ev[0], dev->ev[1]);
-    hipEventElapsedTime(&timing->scc_ms, dev->ev[1], dev->ev[2]);
-    hipEventElapsedTime(&timing->force_ms, dev->ev[2], dev->ev[3]);
-    hipEventElapsedTime(&timing->total_ms, dev->ev[0], dev->ev[3]);
+    HIP_TRY(hipEventElapsedTime(&timing->setup_ms, dev->ev[0], dev->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&timing->scc_ms, dev->ev[1], dev->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&timing->force_ms, dev->ev[2], dev->ev[3]));
+    HIP_TRY(hipEventElapsedTime(&timing->total_ms, dev->ev[0], dev->ev[3]));
     timing->scc_rounds = rounds;
+  }
+  if (ks) {
+    for (int k = 0; k < kKernelCount; ++k) ks[k] = KernelStat{0, 0, 0.0, 0.0, 0.0};
+    for (size_t i = 0; i < probes.size(); ++i) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, dev->probe_pool[2 * i], dev->probe_pool[2 * i + 1]));
+      KernelStat& k = ks[probes[i].kind];
+      k.launches += 1;
+      k.total_ms += ms;
+      k.flops += probes[i].flops;
+      k.bytes += probes[i].bytes;
+    }
   }
   const double nan = std::numeric_limits<double>::quiet_NaN();
   for (int s = 0; s < B; ++s) {

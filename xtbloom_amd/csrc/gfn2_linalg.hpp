@@ -17,6 +17,8 @@ struct TileTask {
 
 constexpr int kTile = 64;
 constexpr int kTileK = 16;
+// dynamic LDS budget of one back-transformation block (column block + reflector + partial dots)
+constexpr size_t kBackLdsBytes = 65536;
 
 // C[sys] = sum_k opA(A)[i][k] * scale[k] * opB(B)[k][j]   (row-major n x n operands)
 //   TA: A is read transposed (A[k][i]); TB: B is read transposed (B[j][k]).
@@ -196,33 +198,40 @@ __global__ __launch_bounds__(256) void k_cholesky_inverse(Topo t, const int* ord
 }
 
 // ---------------------------------------------------------------------------
-// Symmetric eigensolver, one 256-thread block per system.
-//   A (n x n, symmetric, row-major == column-major) is destroyed: its strict
-//   lower triangle receives the Householder vectors.  Z receives the
-//   eigenvectors (column-major: vector j is Z[j*n .. j*n+n)), w the ascending
-//   eigenvalues.  status[s] = 8 when QL fails to converge.
+// Symmetric eigensolver in three launches (all FP64):
+//   k_tridiag         Householder reduction A = Q T Q^T, one 256-thread block per
+//                     system; T's diagonal/off-diagonal go to d/e, the reflectors
+//                     (H = I - 2 v v^T, |v| = 1) stay in A's strict lower triangle.
+//   k_tridiag_eigen   eigenpairs of T, one block per system: T is split where an
+//                     off-diagonal is negligible, every eigenvalue is found by
+//                     Sturm-count bisection (one thread per eigenvalue), every
+//                     eigenvector by inverse iteration with a partially pivoted
+//                     tridiagonal LU (one thread per cluster of close eigenvalues,
+//                     modified Gram-Schmidt inside a cluster).  Eigenvalues are
+//                     ranked ascending (ties by index).
+//   k_backtransform   Y = Q Z on column blocks staged in LDS, many blocks per
+//                     system; column j of Z lands in column rank[j] of Y.
+// Layouts: Z and Y are row-major (row i = component i, column j = vector j).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_eigh(Topo t, const int* order, double* Aall, double* Zall,
-                                              double* Yall, double* wall, SysState* status,
-                                              const unsigned char* active) {
+constexpr double kEps = 2.220446049250313e-16;
+constexpr double kSafeMin = 2.2250738585072014e-308;
+constexpr double kClusterTol = 1e-6;  // relative gap (to the block 1-norm) below which vectors are orthogonalized
+constexpr int kInverseIterations = 3;
+
+__global__ __launch_bounds__(256) void k_tridiag(Topo t, const int* order, double* Aall, double* dall,
+                                                 double* eall, const unsigned char* active) {
   const int s = order[blockIdx.x];
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
   const int64_t mo = t.mat_off[s];
-  double* A = Aall + mo;  // column-major view
-  double* Z = Zall + mo;  // column-major eigenvectors of T
-  double* Y = Yall + mo;  // column-major eigenvectors of A (sorted)
-  double* w = wall + t.ao_off[s];
+  double* A = Aall + mo;  // column-major view of the symmetric matrix
+  double* d = dall + t.ao_off[s];
+  double* e = eall + t.ao_off[s];
   __shared__ double v[kMaxAO];
   __shared__ double pq[kMaxAO];
-  __shared__ double d[kMaxAO];
-  __shared__ double e[kMaxAO];
   __shared__ double red[16];
-  __shared__ double bc[2];
-  __shared__ int ctrl[4];
+  __shared__ double bc[1];
   const int tid = threadIdx.x, nt = blockDim.x;
-
-  // ---- Householder tridiagonalization ----
   for (int k = 0; k + 2 < n; ++k) {
     const int m = n - k - 1;
     double ss = 0.0;
@@ -237,8 +246,7 @@ __global__ __launch_bounds__(256) void k_eigh(Topo t, const int* order, double* 
       const double xn = sqrt(ss);
       d[k] = A[k + (int64_t)k * n];
       if (xn == 0.0 || ss - x0 * x0 == 0.0) {
-        // already reduced: no reflection
-        e[k] = x0;
+        e[k] = x0;  // already reduced: no reflection
         bc[0] = 0.0;
       } else {
         const double alpha = x0 >= 0.0 ? -xn : xn;
@@ -261,7 +269,7 @@ __global__ __launch_bounds__(256) void k_eigh(Topo t, const int* order, double* 
       A[(k + 1 + i) + (int64_t)k * n] = v[i];
     }
     __syncthreads();
-    // p = A22 v
+    // p = A22 v, q = p - (v^T p) v
     double kp = 0.0;
     for (int i = tid; i < m; i += nt) {
       double p = 0.0;
@@ -287,113 +295,288 @@ __global__ __launch_bounds__(256) void k_eigh(Topo t, const int* order, double* 
     }
     d[n - 1] = A[(n - 1) + (int64_t)(n - 1) * n];
     e[n - 1] = 0.0;
-    ctrl[3] = 0;
   }
-  // Z = I
-  for (int64_t idx = tid; idx < (int64_t)n * n; idx += nt) Z[idx] = (idx % (n + 1) == 0) ? 1.0 : 0.0;
-  __syncthreads();
+}
 
-  // ---- implicit QL with Wilkinson-type shift; rotations applied to Z rows ----
-  // pq[] holds cosines, v[] holds sines of the current sweep.
-  for (int l = 0; l < n; ++l) {
-    int iter = 0;
-    for (;;) {
-      if (tid == 0) {
-        int m = l;
-        for (; m < n - 1; ++m) {
-          const double dd = fabs(d[m]) + fabs(d[m + 1]);
-          if (fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
-        }
-        ctrl[0] = m;
-        ctrl[1] = m;  // last rotation index applied is ctrl[1]..m-1 -> set below
-        if (m != l) {
-          if (++iter > 60) {
-            ctrl[3] = 1;
-            ctrl[0] = l;  // give up on this eigenvalue
-          } else {
-            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = hypot(g, 1.0);
-            g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
-            double sn = 1.0, cs = 1.0, p = 0.0;
-            int i = m - 1;
-            bool underflow = false;
-            for (; i >= l; --i) {
-              double f = sn * e[i], b = cs * e[i];
-              r = hypot(f, g);
-              e[i + 1] = r;
-              if (r == 0.0) {
-                d[i + 1] -= p;
-                e[m] = 0.0;
-                underflow = true;
-                break;
-              }
-              sn = f / r;
-              cs = g / r;
-              g = d[i + 1] - p;
-              r = (d[i] - g) * sn + 2.0 * cs * b;
-              p = sn * r;
-              d[i + 1] = g + p;
-              g = cs * r - b;
-              pq[i] = cs;
-              v[i] = sn;
-            }
-            ctrl[1] = underflow ? i + 1 : l;  // rotations i in [ctrl1, m-1]
-            if (!underflow) {
-              d[l] -= p;
-              e[l] = g;
-              e[m] = 0.0;
-            }
-          }
-        }
-      }
-      __syncthreads();
-      const int m = ctrl[0], lo = ctrl[1];
-      if (m == l) break;
-      // apply rotations i = m-1 .. lo to columns (i, i+1) of Z, row by row
-      for (int r = tid; r < n; r += nt) {
-        double f = Z[r + (int64_t)m * n];
-        for (int i = m - 1; i >= lo; --i) {
-          const double zi = Z[r + (int64_t)i * n];
-          Z[r + (int64_t)(i + 1) * n] = v[i] * zi + pq[i] * f;
-          f = pq[i] * zi - v[i] * f;
-        }
-        Z[r + (int64_t)lo * n] = f;
-      }
-      __syncthreads();
-    }
+// number of eigenvalues of T[b0:b1) below x (LDL^T pivots, tiny pivots set to -pivmin)
+__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int b0, int b1,
+                                           double x, double pivmin) {
+  double q = d[b0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int c = q <= 0.0 ? 1 : 0;
+  for (int i = b0 + 1; i < b1; ++i) {
+    q = (d[i] - x) - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q <= 0.0 ? 1 : 0;
+  }
+  return c;
+}
+
+// deterministic start vector in [-1, 1)
+__device__ __forceinline__ double start_component(int j, int i) {
+  uint64_t z = (uint64_t)(j + 1) * 0x9E3779B97F4A7C15ull + (uint64_t)(i + 1) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// Per-system scratch for the LU factors of one shifted block, interleaved by
+// vector (element i of vector j at [i * n + j]) so that a wavefront's threads
+// touch consecutive addresses.
+struct EigenScratch {
+  double* u0;    // pivot row diagonal
+  double* u1;    // first superdiagonal
+  double* u2;    // second superdiagonal (fill-in from row interchanges)
+  double* mult;  // elimination multipliers
+  double* piv;   // 1.0 where rows k, k+1 were interchanged
+};
+
+__global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
+                                                       const unsigned char* active,
+                                                       const double* dall, const double* eall,
+                                                       double* Zall, EigenScratch sc, int* rank_all,
+                                                       double* wall) {
+  const int s = order[blockIdx.x];
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t mo = t.mat_off[s];
+  double* Z = Zall + mo;
+  double* F0 = sc.u0 + mo;
+  double* F1 = sc.u1 + mo;
+  double* F2 = sc.u2 + mo;
+  double* FM = sc.mult + mo;
+  double* FP = sc.piv + mo;
+  __shared__ double d[kMaxAO];
+  __shared__ double e[kMaxAO];   // split off-diagonal (negligible entries zeroed)
+  __shared__ double e2[kMaxAO];
+  __shared__ double lam[kMaxAO];
+  __shared__ double red[16];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double emax2 = 0.0;
+  for (int i = tid; i < n; i += nt) {
+    d[i] = dall[t.ao_off[s] + i];
+    double ei = i + 1 < n ? eall[t.ao_off[s] + i] : 0.0;
+    e[i] = ei;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += nt) {
+    double ei = e[i];
+    if (i + 1 < n && ei * ei <= kEps * kEps * fabs(d[i] * d[i + 1]) + kSafeMin) ei = 0.0;
+    e2[i] = ei * ei;
+    emax2 = fmax(emax2, ei * ei);
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += nt)
+    if (e2[i] == 0.0) e[i] = 0.0;
+  // block-wide max of e^2 for the pivot floor
+  {
+    double m2 = emax2;
+    for (int off = 32; off > 0; off >>= 1) m2 = fmax(m2, __shfl_xor(m2, off, 64));
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = m2;
+    __syncthreads();
+    m2 = 0.0;
+    for (int w = 0; w < (nt >> 6); ++w) m2 = fmax(m2, red[w]);
+    emax2 = m2;
     __syncthreads();
   }
-  if (tid == 0 && ctrl[3]) status[s].status = 8;
+  const double pivmin = kSafeMin * fmax(1.0, emax2);
 
-  // ---- back-transformation Z <- H_0 ... H_{n-3} Z (waves over columns) ----
-  const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+  // ---- eigenvalues: thread per index, bisection inside the unreduced block ----
+  for (int j = tid; j < n; j += nt) {
+    int b0 = j, b1 = j + 1;
+    while (b0 > 0 && e[b0 - 1] != 0.0) --b0;
+    while (b1 < n && e[b1 - 1] != 0.0) ++b1;
+    const int k = j - b0, bn = b1 - b0;
+    if (bn == 1) {
+      lam[j] = d[j];
+      continue;
+    }
+    double gl = 1e300, gu = -1e300, tnorm = 0.0;
+    for (int i = b0; i < b1; ++i) {
+      const double off = (i > b0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < b1 ? fabs(e[i]) : 0.0);
+      gl = fmin(gl, d[i] - off);
+      gu = fmax(gu, d[i] + off);
+    }
+    tnorm = fmax(fabs(gl), fabs(gu));
+    const double fudge = 2.0 * kEps * tnorm * bn + 4.0 * pivmin;
+    double lo = gl - fudge, hi = gu + fudge;
+    for (int it = 0; it < 200; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      if (hi - lo <= 2.0 * kEps * fmax(fabs(lo), fabs(hi)) + pivmin || mid == lo || mid == hi) break;
+      if (sturm_count(d, e2, b0, b1, mid, pivmin) <= k) lo = mid; else hi = mid;
+    }
+    lam[j] = 0.5 * (lo + hi);
+  }
+  __syncthreads();
+
+  // ---- eigenvectors: thread per cluster start ----
+  for (int j = tid; j < n; j += nt) {
+    int b0 = j, b1 = j + 1;
+    while (b0 > 0 && e[b0 - 1] != 0.0) --b0;
+    while (b1 < n && e[b1 - 1] != 0.0) ++b1;
+    const int bn = b1 - b0;
+    double onenrm = 0.0;
+    for (int i = b0; i < b1; ++i)
+      onenrm = fmax(onenrm, fabs(d[i]) + (i > b0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < b1 ? fabs(e[i]) : 0.0));
+    const double gtol = kClusterTol * fmax(onenrm, kSafeMin);
+    if (j > b0 && lam[j] - lam[j - 1] <= gtol) continue;  // not a cluster start
+    int g1 = j + 1;
+    while (g1 < b1 && lam[g1] - lam[g1 - 1] <= gtol) ++g1;
+    const double ptol = kEps * fmax(onenrm, kSafeMin);       // smallest admissible pivot
+    const double pertol = 10.0 * kEps * fmax(onenrm, kSafeMin);
+    double xprev = 0.0;
+    for (int jj = j; jj < g1; ++jj) {
+      // rows outside the block are zero
+      for (int i = 0; i < b0; ++i) Z[(int64_t)i * n + jj] = 0.0;
+      for (int i = b1; i < n; ++i) Z[(int64_t)i * n + jj] = 0.0;
+      if (bn == 1) {
+        Z[(int64_t)b0 * n + jj] = 1.0;
+        continue;
+      }
+      double x = lam[jj];
+      if (jj > j && x - xprev < pertol) x = xprev + pertol;
+      xprev = x;
+      // LU with partial pivoting of T_b - x I
+      double c0 = d[b0] - x, c1 = e[b0], c2 = 0.0;
+      for (int k = 0; k < bn - 1; ++k) {
+        const int gi = b0 + k;
+        const double n0 = e[gi], n1 = d[gi + 1] - x, n2 = (k + 2 < bn) ? e[gi + 1] : 0.0;
+        const int64_t at = (int64_t)gi * n + jj;
+        if (fabs(c0) >= fabs(n0)) {
+          const double m = c0 != 0.0 ? n0 / c0 : 0.0;
+          F0[at] = c0; F1[at] = c1; F2[at] = c2; FM[at] = m; FP[at] = 0.0;
+          c0 = n1 - m * c1;
+          c1 = n2 - m * c2;
+        } else {
+          const double m = c0 / n0;
+          F0[at] = n0; F1[at] = n1; F2[at] = n2; FM[at] = m; FP[at] = 1.0;
+          const double t0 = c1 - m * n1, t1 = c2 - m * n2;
+          c0 = t0;
+          c1 = t1;
+        }
+        c2 = 0.0;
+      }
+      {
+        const int64_t at = (int64_t)(b1 - 1) * n + jj;
+        F0[at] = c0; F1[at] = 0.0; F2[at] = 0.0; FM[at] = 0.0; FP[at] = 0.0;
+      }
+      for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] = start_component(jj, i - b0);
+      for (int it = 0; it <= kInverseIterations; ++it) {
+        // orthogonalize against the cluster members already finished
+        for (int p = j; p < jj; ++p) {
+          double dot = 0.0;
+          for (int i = b0; i < b1; ++i) dot += Z[(int64_t)i * n + p] * Z[(int64_t)i * n + jj];
+          for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] -= dot * Z[(int64_t)i * n + p];
+        }
+        if (it == kInverseIterations) break;
+        // scale to unit max-norm, then solve (T_b - x I) y = r
+        double rmax = 0.0;
+        for (int i = b0; i < b1; ++i) rmax = fmax(rmax, fabs(Z[(int64_t)i * n + jj]));
+        const double rs = rmax > 0.0 ? 1.0 / rmax : 1.0;
+        for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] *= rs;
+        for (int k = 0; k < bn - 1; ++k) {
+          const int64_t at = (int64_t)(b0 + k) * n + jj, nx = at + n;
+          double rk = Z[at], rn = Z[nx];
+          if (FP[at] != 0.0) { const double tmp = rk; rk = rn; rn = tmp; }
+          Z[at] = rk;
+          Z[nx] = rn - FM[at] * rk;
+        }
+        for (int k = bn - 1; k >= 0; --k) {
+          const int64_t at = (int64_t)(b0 + k) * n + jj;
+          double r = Z[at];
+          if (k + 1 < bn) r -= F1[at] * Z[at + n];
+          if (k + 2 < bn) r -= F2[at] * Z[at + 2 * (int64_t)n];
+          double u = F0[at];
+          if (fabs(u) < ptol) u = u < 0.0 ? -ptol : ptol;
+          Z[at] = r / u;
+        }
+        // normalize
+        double nrm = 0.0, ymax = 0.0;
+        for (int i = b0; i < b1; ++i) ymax = fmax(ymax, fabs(Z[(int64_t)i * n + jj]));
+        const double ys = ymax > 0.0 ? 1.0 / ymax : 1.0;
+        for (int i = b0; i < b1; ++i) {
+          const double y = Z[(int64_t)i * n + jj] * ys;
+          nrm += y * y;
+        }
+        const double sc2 = ys / sqrt(nrm);
+        for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] *= sc2;
+      }
+      // final normalization; sign: largest-magnitude component positive
+      double nrm = 0.0, big = 0.0;
+      for (int i = b0; i < b1; ++i) {
+        const double y = Z[(int64_t)i * n + jj];
+        nrm += y * y;
+        if (fabs(y) > fabs(big)) big = y;
+      }
+      const double f = (big < 0.0 ? -1.0 : 1.0) / sqrt(nrm);
+      for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] *= f;
+    }
+  }
+  // ---- ascending rank (stable on ties) ----
+  for (int j = tid; j < n; j += nt) {
+    const double lj = lam[j];
+    int r = 0;
+    for (int i = 0; i < n; ++i) {
+      const double li = lam[i];
+      r += (li < lj) || (li == lj && i < j);
+    }
+    rank_all[t.ao_off[s] + j] = r;
+    wall[t.ao_off[s] + r] = lj;
+  }
+}
+
+// One column block of one system: Y[:, rank[j0 .. j0+cw)] = Q Z[:, j0 .. j0+cw).
+struct BackTask {
+  int sys;
+  int j0;
+  int cw;
+  int pad;
+};
+
+__global__ __launch_bounds__(256) void k_backtransform(Topo t, const BackTask* tasks, const double* Aall,
+                                                       const double* Zall, double* Yall,
+                                                       const int* rank_all,
+                                                       const unsigned char* active) {
+  extern __shared__ double lds[];
+  const BackTask task = tasks[blockIdx.x];
+  const int s = task.sys;
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t mo = t.mat_off[s];
+  const double* A = Aall + mo;
+  const double* Z = Zall + mo;
+  double* Y = Yall + mo;
+  const int* rank = rank_all + t.ao_off[s];
+  const int cw = task.cw, j0 = task.j0;
+  double* Zb = lds;               // [n][cw]
+  double* hv = lds + n * cw;      // [n]
+  double* part = hv + n;          // [256]
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int idx = tid; idx < n * cw; idx += nt) {
+    const int i = idx / cw, c = idx % cw;
+    Zb[idx] = (j0 + c < n) ? Z[(int64_t)i * n + j0 + c] : 0.0;
+  }
+  const int c = tid % cw, rg = tid / cw, nrg = nt / cw;
   for (int k = n - 3; k >= 0; --k) {
     const int m = n - k - 1;
-    const double* hv = A + (k + 1) + (int64_t)k * n;
-    for (int j = wid; j < n; j += nw) {
-      double* zc = Z + (int64_t)j * n + (k + 1);
-      double dot = 0.0;
-      for (int i = lane; i < m; i += 64) dot += hv[i] * zc[i];
-      dot = wave_sum(dot);
-      for (int i = lane; i < m; i += 64) zc[i] -= 2.0 * dot * hv[i];
-    }
     __syncthreads();
-  }
-  // ---- ascending order: rank sort, stable on ties ----
-  for (int j = tid; j < n; j += nt) {
-    const double dj = d[j];
-    int rank = 0;
-    for (int i = 0; i < n; ++i) {
-      const double di = d[i];
-      rank += (di < dj) || (di == dj && i < j);
-    }
-    pq[j] = (double)rank;
+    for (int i = tid; i < m; i += nt) hv[i] = A[(k + 1 + i) + (int64_t)k * n];
+    __syncthreads();
+    double p = 0.0;
+    for (int i = rg; i < m; i += nrg) p += hv[i] * Zb[(k + 1 + i) * cw + c];
+    part[rg * cw + c] = p;
+    __syncthreads();
+    double dot = 0.0;
+    for (int r = 0; r < nrg; ++r) dot += part[r * cw + c];
+    dot *= 2.0;
+    for (int i = rg; i < m; i += nrg) Zb[(k + 1 + i) * cw + c] -= dot * hv[i];
   }
   __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    const int rk = (int)pq[j];
-    for (int i = tid; i < n; i += nt) Y[(int64_t)rk * n + i] = Z[(int64_t)j * n + i];
-    if (tid == 0) w[rk] = d[j];
+  for (int idx = tid; idx < n * cw; idx += nt) {
+    const int i = idx / cw, cc = idx % cw;
+    if (j0 + cc < n) Y[(int64_t)i * n + rank[j0 + cc]] = Zb[idx];
   }
 }
 

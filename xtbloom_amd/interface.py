@@ -174,7 +174,7 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                    flags=library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES,
                    max_scc_iterations=250, charge_tolerance=1e-6, energy_tolerance=1e-8,
                    electronic_temperature=300.0, scc_mixer_history=8, scc_mixer_damping=0.4,
-                   extension=False, diagnostics=False) -> ComputedBatch:
+                   extension=False, diagnostics=False, kernel_stats=False) -> ComputedBatch:
     """One ``xtbloom_compute`` over flat ragged arrays (the ``xtbloom_batch_t`` layout)."""
     lib = library.load()
     atom_offsets = np.ascontiguousarray(atom_offsets, dtype=np.int64)
@@ -235,7 +235,7 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
     res.per_system_status = _out(status)
     ext_out = None
     dedq = None
-    if extension or diagnostics:
+    if extension or diagnostics or kernel_stats:
         ext = library.Extension()
         ext.struct_size = ctypes.sizeof(ext)
         ext.api_version = library.API_VERSION
@@ -252,16 +252,127 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
             ext.coordination_numbers = bufs["coordination_numbers"].ctypes.data
             bufs["energy_components"] = np.zeros((B, 8))
             ext.energy_components = bufs["energy_components"].ctypes.data
+        stats = (library.KernelStat * len(library.KERNEL_FAMILIES))() if kernel_stats else None
+        if stats is not None:
+            ext.kernel_stats = ctypes.addressof(stats)
         _check(lib.xtbloom_amd_compute_extended(context.handle, ctypes.byref(batch),
                                                 ctypes.byref(opts), ctypes.byref(res),
                                                 ctypes.byref(ext)))
         ext_out = dict(bufs, setup_ms=ext.setup_ms, scc_ms=ext.scc_ms, force_ms=ext.force_ms,
                        total_ms=ext.total_ms, scc_rounds=ext.scc_rounds)
+        if stats is not None:
+            ext_out["kernel_stats"] = _stats_dict(stats)
     else:
         _check(lib.xtbloom_compute(context.handle, ctypes.byref(batch), ctypes.byref(opts),
                                    ctypes.byref(res)))
     return ComputedBatch(atom_offsets, point_offsets, energies, forces, q, pcf, dip, iters,
                          conv.astype(bool), status, dedq, ext_out)
+
+
+def _stats_dict(stats) -> dict:
+    return {name: dict(launches=st.launches, ms=st.total_ms, flops=st.flops, bytes=st.bytes)
+            for name, st in zip(library.KERNEL_FAMILIES, stats)}
+
+
+def _tensor_buffer(cls, tensor):
+    b = cls()
+    if tensor is None:
+        return b
+    if not tensor.is_contiguous():
+        raise XTBloomValueError("tensor buffers must be contiguous")
+    b.data = tensor.data_ptr()
+    b.size_bytes = tensor.numel() * tensor.element_size()
+    b.memory_space = library.MEMORY_ROCM_DEVICE if tensor.is_cuda else library.MEMORY_HOST
+    return b
+
+
+def compute_tensors(context: Context, atom_offsets, numbers, positions, charges, uhf,
+                    point_offsets=None, point_positions=None, point_values=None,
+                    point_gammas=None, *, flags=library.COMPUTE_ENERGY | library.COMPUTE_FORCES
+                    | library.COMPUTE_ATOMIC_CHARGES, max_scc_iterations=250,
+                    charge_tolerance=1e-6, energy_tolerance=1e-8, electronic_temperature=300.0,
+                    scc_mixer_history=8, scc_mixer_damping=0.4, outputs=None,
+                    point_charge_potential_gradient=False, kernel_stats=False):
+    """``xtbloom_compute`` on torch tensors (device-resident inputs and outputs stay on the GPU).
+
+    Inputs follow the ``xtbloom_batch_t`` dtypes (int64 offsets, int32 numbers and
+    unpaired counts, float64 coordinates).  ``outputs`` may pass preallocated
+    tensors (keys energies, forces, charges, point_charge_forces, scc_iterations,
+    scc_converged, per_system_status, point_charge_potential_gradient); missing
+    ones are allocated on the positions' device.  Returns (outputs, kernel stats).
+    """
+    import torch
+
+    lib = library.load()
+    dev = positions.device
+    B, N = atom_offsets.numel() - 1, numbers.numel()
+    P = 0 if point_values is None else point_values.numel()
+    batch = library.Batch()
+    _check(lib.xtbloom_batch_init(ctypes.byref(batch), ctypes.sizeof(batch)))
+    batch.batch_size, batch.total_atoms, batch.total_point_charges = B, N, P
+    batch.atom_offsets = _tensor_buffer(library.ConstBuffer, atom_offsets)
+    batch.atomic_numbers = _tensor_buffer(library.ConstBuffer, numbers)
+    batch.positions = _tensor_buffer(library.ConstBuffer, positions)
+    batch.molecular_charges = _tensor_buffer(library.ConstBuffer, charges)
+    batch.unpaired_electrons = _tensor_buffer(library.ConstBuffer, uhf)
+    if P:
+        batch.point_charge_offsets = _tensor_buffer(library.ConstBuffer, point_offsets)
+        batch.point_charge_positions = _tensor_buffer(library.ConstBuffer, point_positions)
+        batch.point_charge_values = _tensor_buffer(library.ConstBuffer, point_values)
+        batch.point_charge_gammas = _tensor_buffer(library.ConstBuffer, point_gammas)
+    opts = library.ComputeOptions()
+    _check(lib.xtbloom_compute_options_init(ctypes.byref(opts), ctypes.sizeof(opts)))
+    opts.flags = flags if P else flags & ~library.COMPUTE_POINT_CHARGE_FORCES
+    opts.max_scc_iterations = max_scc_iterations
+    opts.charge_tolerance = charge_tolerance
+    opts.energy_tolerance = energy_tolerance
+    opts.electronic_temperature = electronic_temperature * library.KELVIN_TO_HARTREE
+    opts.scc_mixer_history = scc_mixer_history
+    opts.scc_mixer_damping = scc_mixer_damping
+    out = dict(outputs or {})
+    f64 = dict(dtype=torch.float64, device=dev)
+    out.setdefault("energies", torch.empty(B, **f64))
+    if flags & library.COMPUTE_FORCES:
+        out.setdefault("forces", torch.empty((N, 3), **f64))
+    if flags & library.COMPUTE_ATOMIC_CHARGES:
+        out.setdefault("charges", torch.empty(N, **f64))
+    if P and flags & library.COMPUTE_POINT_CHARGE_FORCES:
+        out.setdefault("point_charge_forces", torch.empty((P, 3), **f64))
+    out.setdefault("scc_iterations", torch.empty(B, dtype=torch.int32, device=dev))
+    out.setdefault("scc_converged", torch.empty(B, dtype=torch.uint8, device=dev))
+    out.setdefault("per_system_status", torch.empty(B, dtype=torch.int32, device=dev))
+    res = library.BatchResult()
+    _check(lib.xtbloom_batch_result_init(ctypes.byref(res), ctypes.sizeof(res)))
+    res.energies = _tensor_buffer(library.Buffer, out.get("energies"))
+    res.forces = _tensor_buffer(library.Buffer, out.get("forces"))
+    res.atomic_charges = _tensor_buffer(library.Buffer, out.get("charges"))
+    res.point_charge_forces = _tensor_buffer(library.Buffer, out.get("point_charge_forces"))
+    res.scc_iterations = _tensor_buffer(library.Buffer, out["scc_iterations"])
+    res.scc_converged = _tensor_buffer(library.Buffer, out["scc_converged"])
+    res.per_system_status = _tensor_buffer(library.Buffer, out["per_system_status"])
+    ext = library.Extension()
+    ext.struct_size = ctypes.sizeof(ext)
+    ext.api_version = library.API_VERSION
+    host_dedq = None
+    if P and point_charge_potential_gradient:
+        # the extension's outputs are host buffers
+        host_dedq = np.empty(P)
+        ext.point_charge_potential_gradient = host_dedq.ctypes.data
+    stats = (library.KernelStat * len(library.KERNEL_FAMILIES))() if kernel_stats else None
+    if stats is not None:
+        ext.kernel_stats = ctypes.addressof(stats)
+    if dev.type == "cuda":
+        torch.cuda.synchronize(dev)
+    _check(lib.xtbloom_amd_compute_extended(context.handle, ctypes.byref(batch),
+                                            ctypes.byref(opts), ctypes.byref(res),
+                                            ctypes.byref(ext)))
+    if host_dedq is not None:
+        out["point_charge_potential_gradient"] = torch.from_numpy(host_dedq).to(dev)
+    timing = dict(setup_ms=ext.setup_ms, scc_ms=ext.scc_ms, force_ms=ext.force_ms,
+                  total_ms=ext.total_ms, scc_rounds=ext.scc_rounds)
+    if stats is not None:
+        timing["kernel_stats"] = _stats_dict(stats)
+    return out, timing
 
 
 class Result:

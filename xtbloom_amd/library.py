@@ -105,12 +105,24 @@ class BatchResult(ctypes.Structure):
                 ("wiberg_orders", Buffer), ("spin_populations", Buffer)]
 
 
+KERNEL_FAMILIES = ("geometry", "integrals", "cholesky", "potentials", "fock", "reduce", "tridiag",
+                   "fermi", "back", "density", "mulliken", "energy_mix", "energy_weighted",
+                   "force_pairs", "force_atoms", "force_atm", "tridiag_eigen", "backtransform")
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("total_ms", ctypes.c_double), ("flops", ctypes.c_double),
+                ("bytes", ctypes.c_double)]
+
+
 class Extension(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("api_version", ctypes.c_uint32),
                 ("point_charge_potential_gradient", ctypes.c_void_p),
                 ("overlap", ctypes.c_void_p), ("core_hamiltonian", ctypes.c_void_p),
                 ("density", ctypes.c_void_p), ("coordination_numbers", ctypes.c_void_p),
-                ("energy_components", ctypes.c_void_p), ("setup_ms", ctypes.c_float),
+                ("energy_components", ctypes.c_void_p), ("kernel_stats", ctypes.c_void_p),
+                ("setup_ms", ctypes.c_float),
                 ("scc_ms", ctypes.c_float), ("force_ms", ctypes.c_float),
                 ("total_ms", ctypes.c_float), ("scc_rounds", ctypes.c_int32),
                 ("reserved", ctypes.c_uint32)]
@@ -122,6 +134,7 @@ assert ctypes.sizeof(Buffer) == 24
 assert ctypes.sizeof(Batch) == 456
 assert ctypes.sizeof(ComputeOptions) == 80
 assert ctypes.sizeof(BatchResult) == 280
+assert ctypes.sizeof(KernelStat) == 32
 
 EXPORTED_SYMBOLS = (
     "xtbloom_version_string", "xtbloom_status_string", "xtbloom_get_last_error",
