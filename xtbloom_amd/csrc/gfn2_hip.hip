@@ -715,7 +715,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(probe_end(kKReduce, a3, 0.0));
     // eigenpairs of H' (A is overwritten by the reflectors; T, H, C, P, W serve as scratch)
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL(k_tridiag, dim3(B), dim3(256), 0, sm, t, d_order, dA, d_td, d_te, d_active);
+    hipLaunchKernelGGL(k_tridiag, dim3(B), dim3(512), 0, sm, t, d_order, dA, d_td, d_te, d_active);
     HIP_TRY(probe_end(kKTridiag, (4.0 / 3.0) * a3, 0.0));
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_tridiag_eigen, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_td, d_te,

@@ -3,6 +3,8 @@
 // (Householder tridiagonalization + implicit QL + back-transformation).
 #pragma once
 
+#include <type_traits>
+
 #include "gfn2_device.hpp"
 
 namespace xtb {
@@ -218,19 +220,9 @@ constexpr double kSafeMin = 2.2250738585072014e-308;
 constexpr double kClusterTol = 1e-6;  // relative gap (to the block 1-norm) below which vectors are orthogonalized
 constexpr int kInverseIterations = 3;
 
-__global__ __launch_bounds__(256) void k_tridiag(Topo t, const int* order, double* Aall, double* dall,
-                                                 double* eall, const unsigned char* active) {
-  const int s = order[blockIdx.x];
-  if (active && !active[s]) return;
-  const int n = t.ao_off[s + 1] - t.ao_off[s];
-  const int64_t mo = t.mat_off[s];
-  double* A = Aall + mo;  // column-major view of the symmetric matrix
-  double* d = dall + t.ao_off[s];
-  double* e = eall + t.ao_off[s];
-  __shared__ double v[kMaxAO];
-  __shared__ double pq[kMaxAO];
-  __shared__ double red[16];
-  __shared__ double bc[1];
+// Householder reduction with the trailing matrix in global memory (any n).
+__device__ __forceinline__ void tridiag_global(double* A, int n, double* d, double* e, double* v, double* pq,
+                               double* red, double* bc) {
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int k = 0; k + 2 < n; ++k) {
     const int m = n - k - 1;
@@ -298,6 +290,189 @@ __global__ __launch_bounds__(256) void k_tridiag(Topo t, const int* order, doubl
   }
 }
 
+// Householder reduction with the lower triangle held in registers, n <= 16 NR,
+// 512 threads.  Thread (lr, lc) = (tid / 32, tid % 32) owns element
+// (16 TI + lr, 32 TJ + lc) of every 16 x 32 tile that meets the lower triangle
+// (TJ <= TI / 2), so the symmetric matvec and the rank-2 update of each step
+// never touch memory.  Only the reflector (LDS, and its column of A for the
+// back-transformation), the matvec partials (LDS) and d/e leave the registers.
+// Stored form: strictly-lower elements as is, diagonal elements halved, the
+// strictly-upper part of tiles crossing the diagonal as zero.  Then every slot
+// contributes x v_j to p_i and x v_i to p_j, which is the full symmetric
+// product, and the rank-2 update keeps the form with factor 2 / 1 / 0.
+constexpr int tri_slots(int NR) { return NR == 0 ? 0 : tri_slots(NR - 1) + (NR - 1) / 2 + 1; }
+// slot -> (row tile, column tile) of the register layout; folded at compile time
+constexpr int tri_row(int slot) {
+  int TI = 0;
+  while (tri_slots(TI + 1) <= slot) ++TI;
+  return TI;
+}
+constexpr int tri_col(int slot) { return slot - tri_slots(tri_row(slot)); }
+// tile entirely strictly below the diagonal (every row index > every column index)
+constexpr bool tri_strict(int TI, int TJ) { return 16 * TI > 32 * TJ + 31; }
+
+// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+template <int NR>
+__device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, double* e, double* v,
+                                                  double* pq, double* colpart, double* red,
+                                                  double* bc) {
+  constexpr int NC = NR / 2;
+  constexpr int NE = tri_slots(NR);
+  const int tid = threadIdx.x, lr = tid >> 5, lc = tid & 31, w = tid >> 6;
+  double a[NE];
+  static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+    constexpr int sl = decltype(S)::value;
+    constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+    const int i = 16 * TI + lr, j = 32 * TJ + lc;
+    // A is symmetric: lanes read row i along j contiguously
+    const double x = (i < n && j < n) ? A[j + (int64_t)i * n] : 0.0;
+    if constexpr (tri_strict(TI, TJ)) a[sl] = x;
+    else a[sl] = i > j ? x : (i == j ? 0.5 * x : 0.0);
+  });
+  for (int k = 0; k + 2 < n; ++k) {
+    const int Tk = k >> 5, ck = k & 31;
+    if (tid <= k || tid >= n) v[tid] = 0.0;
+    if (lc == ck) {
+      static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+        constexpr int sl = decltype(S)::value;
+        constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+        if (TJ == Tk) {
+          const int i = 16 * TI + lr;
+          if (i > k && i < n) v[i] = a[sl];
+          if (i == k) bc[1] = 2.0 * a[sl];
+        }
+      });
+    }
+    __syncthreads();
+    const double xi = (tid > k && tid < n) ? v[tid] : 0.0;
+    const double ss = block_sum(xi * xi, red);
+    if (tid == 0) {
+      const double x0 = v[k + 1];
+      const double xn = sqrt(ss);
+      d[k] = bc[1];
+      if (xn == 0.0 || ss - x0 * x0 == 0.0) {
+        e[k] = x0;  // already reduced: no reflection
+        bc[0] = 0.0;
+      } else {
+        const double alpha = x0 >= 0.0 ? -xn : xn;
+        const double v0 = x0 - alpha;
+        const double vv = ss - x0 * x0 + v0 * v0;
+        e[k] = alpha;
+        v[k + 1] = v0;
+        bc[0] = 1.0 / sqrt(vv);
+      }
+    }
+    __syncthreads();
+    const double inv = bc[0];
+    if (inv == 0.0) {
+      if (tid > k && tid < n) A[tid + (int64_t)k * n] = 0.0;
+      continue;
+    }
+    if (tid > k && tid < n) {
+      const double vi = v[tid] * inv;
+      v[tid] = vi;
+      A[tid + (int64_t)k * n] = vi;
+    }
+    __syncthreads();
+    // p = A v (v = 0 outside the trailing block); a row tile's partial is
+    // reduced over its 32 lanes as soon as its last slot is done
+    double pc[NC];
+#pragma unroll
+    for (int TJ = 0; TJ < NC; ++TJ) pc[TJ] = 0.0;
+    double pr = 0.0;
+    static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+      constexpr int sl = decltype(S)::value;
+      constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+      if constexpr (TJ == 0) __builtin_amdgcn_sched_barrier(0);  // one row tile at a time
+      const double x = a[sl];
+      pr = fma(x, v[32 * TJ + lc], pr);
+      pc[TJ] = fma(x, v[16 * TI + lr], pc[TJ]);
+      if constexpr (TJ == TI / 2) {  // last slot of row tile TI
+        pr += __shfl_xor(pr, 16, 32);
+        pr += __shfl_xor(pr, 8, 32);
+        pr += __shfl_xor(pr, 4, 32);
+        pr += __shfl_xor(pr, 2, 32);
+        pr += __shfl_xor(pr, 1, 32);
+        if (lc == 0) pq[16 * TI + lr] = pr;
+        pr = 0.0;
+      }
+    });
+#pragma unroll
+    for (int TJ = 0; TJ < NC; ++TJ) {
+      const double c = pc[TJ] + __shfl_xor(pc[TJ], 32, 64);
+      if ((tid & 63) < 32) colpart[w * 256 + 32 * TJ + lc] = c;
+    }
+    __syncthreads();
+    double p = 0.0;
+    if (tid > k && tid < n) {
+      p = pq[tid];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) p += colpart[r * 256 + tid];
+    }
+    const double vt = v[tid];
+    const double K = block_sum(vt * p, red);
+    pq[tid] = p - K * vt;  // q; zero outside the trailing block
+    __syncthreads();
+    // A -= 2 (v q^T + q v^T); frozen rows/columns see v = q = 0
+    static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+      constexpr int sl = decltype(S)::value;
+      constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+      if constexpr (TJ == 0) __builtin_amdgcn_sched_barrier(0);  // one row tile at a time
+      const int i = 16 * TI + lr, j = 32 * TJ + lc;
+      const double u = v[i] * pq[j] + pq[i] * v[j];
+      if constexpr (tri_strict(TI, TJ)) a[sl] -= 2.0 * u;
+      else a[sl] -= (i > j ? 2.0 : (i == j ? 1.0 : 0.0)) * u;
+    });
+    __syncthreads();
+  }
+  static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+    constexpr int sl = decltype(S)::value;
+    constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+    const int i = 16 * TI + lr, j = 32 * TJ + lc;
+    if (i == n - 2 && j == n - 2) d[n - 2] = 2.0 * a[sl];
+    if (i == n - 1 && j == n - 2) e[n - 2] = a[sl];
+    if (i == n - 1 && j == n - 1) d[n - 1] = 2.0 * a[sl];
+  });
+  if (tid == 0) e[n - 1] = 0.0;
+}
+
+// 512 threads per system (two waves per SIMD, 256 VGPRs per thread)
+__global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, double* Aall, double* dall,
+                                                 double* eall, const unsigned char* active) {
+  const int s = order[blockIdx.x];
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  double* A = Aall + t.mat_off[s];  // column-major view of the symmetric matrix
+  double* d = dall + t.ao_off[s];
+  double* e = eall + t.ao_off[s];
+  __shared__ double v[kMaxAO];
+  __shared__ double pq[kMaxAO];
+  __shared__ double colpart[8 * 256];
+  __shared__ double red[16];
+  __shared__ double bc[2];
+  if (n <= 64) tridiag_registers<4>(A, n, d, e, v, pq, colpart, red, bc);
+  else if (n <= 128) tridiag_registers<8>(A, n, d, e, v, pq, colpart, red, bc);
+  else if (n <= 192) tridiag_registers<12>(A, n, d, e, v, pq, colpart, red, bc);
+  else if (n <= 256) tridiag_registers<16>(A, n, d, e, v, pq, colpart, red, bc);
+  else tridiag_global(A, n, d, e, v, pq, red, bc);
+}
+
+// x / y via the hardware reciprocal refined by one Newton step (a few ulp,
+// enough for the sign decisions of a Sturm count)
+__device__ __forceinline__ double fast_div(double x, double y) {
+  double r = __builtin_amdgcn_rcp(y);
+  r = fma(r, fma(-y, r, 1.0), r);
+  return x * r;
+}
+
 // number of eigenvalues of T[b0:b1) below x (LDL^T pivots, tiny pivots set to -pivmin)
 __device__ __forceinline__ int sturm_count(const double* d, const double* e2, int b0, int b1,
                                            double x, double pivmin) {
@@ -305,7 +480,7 @@ __device__ __forceinline__ int sturm_count(const double* d, const double* e2, in
   if (fabs(q) < pivmin) q = -pivmin;
   int c = q <= 0.0 ? 1 : 0;
   for (int i = b0 + 1; i < b1; ++i) {
-    q = (d[i] - x) - e2[i - 1] / q;
+    q = (d[i] - x) - fast_div(e2[i - 1], q);
     if (fabs(q) < pivmin) q = -pivmin;
     c += q <= 0.0 ? 1 : 0;
   }
@@ -332,6 +507,60 @@ struct EigenScratch {
   double* piv;   // 1.0 where rows k, k+1 were interchanged
 };
 
+// Eigenvector of an isolated eigenvalue x of the unreduced block T[b0:b1) from
+// its twisted factorization: forward pivots D+ (in F0) and backward pivots D-
+// (in F1) of T - x I, twist index r = argmin |D+_r + D-_r - (d_r - x)|, then
+// z_r = 1, z_i = -(e_i / D+_i) z_{i+1} below r and z_i = -(e_{i-1} / D-_i) z_{i-1}
+// above r; normalized with the largest component positive.  Returns false
+// (leaving Z to be overwritten) if the recurrences produced a non-finite value.
+__device__ __forceinline__ bool twisted_vector(const double* d, const double* e, int b0, int b1,
+                                               double x, double pivmin, int n, int jj,
+                                               double* __restrict__ Dp, double* __restrict__ Dm,
+                                               double* __restrict__ Z) {
+  double q = d[b0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  Dp[(int64_t)b0 * n + jj] = q;
+  for (int i = b0 + 1; i < b1; ++i) {
+    q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    Dp[(int64_t)i * n + jj] = q;
+  }
+  q = d[b1 - 1] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  Dm[(int64_t)(b1 - 1) * n + jj] = q;
+  int r = b1 - 1;
+  double gmin = fabs(Dp[(int64_t)(b1 - 1) * n + jj] + q - (d[b1 - 1] - x));
+  for (int i = b1 - 2; i >= b0; --i) {
+    q = (d[i] - x) - e[i] * e[i] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    Dm[(int64_t)i * n + jj] = q;
+    const double g = fabs(Dp[(int64_t)i * n + jj] + q - (d[i] - x));
+    if (g < gmin) {
+      gmin = g;
+      r = i;
+    }
+  }
+  double z = 1.0, nrm = 1.0, big = 1.0;
+  Z[(int64_t)r * n + jj] = 1.0;
+  for (int i = r - 1; i >= b0; --i) {
+    z = -(e[i] / Dp[(int64_t)i * n + jj]) * z;
+    Z[(int64_t)i * n + jj] = z;
+    nrm += z * z;
+    if (fabs(z) > fabs(big)) big = z;
+  }
+  z = 1.0;
+  for (int i = r + 1; i < b1; ++i) {
+    z = -(e[i - 1] / Dm[(int64_t)i * n + jj]) * z;
+    Z[(int64_t)i * n + jj] = z;
+    nrm += z * z;
+    if (fabs(z) > fabs(big)) big = z;
+  }
+  if (!isfinite(nrm) || !(nrm > 0.0)) return false;
+  const double f = (big < 0.0 ? -1.0 : 1.0) / sqrt(nrm);
+  for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] *= f;
+  return true;
+}
+
 __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
                                                        const unsigned char* active,
                                                        const double* dall, const double* eall,
@@ -341,12 +570,12 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
   const int64_t mo = t.mat_off[s];
-  double* Z = Zall + mo;
-  double* F0 = sc.u0 + mo;
-  double* F1 = sc.u1 + mo;
-  double* F2 = sc.u2 + mo;
-  double* FM = sc.mult + mo;
-  double* FP = sc.piv + mo;
+  double* __restrict__ Z = Zall + mo;
+  double* __restrict__ F0 = sc.u0 + mo;
+  double* __restrict__ F1 = sc.u1 + mo;
+  double* __restrict__ F2 = sc.u2 + mo;
+  double* __restrict__ FM = sc.mult + mo;
+  double* __restrict__ FP = sc.piv + mo;
   __shared__ double d[kMaxAO];
   __shared__ double e[kMaxAO];   // split off-diagonal (negligible entries zeroed)
   __shared__ double e2[kMaxAO];
@@ -438,6 +667,7 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
       double x = lam[jj];
       if (jj > j && x - xprev < pertol) x = xprev + pertol;
       xprev = x;
+      if (g1 - j == 1 && twisted_vector(d, e, b0, b1, x, pivmin, n, jj, F0, F1, Z)) continue;
       // LU with partial pivoting of T_b - x I
       double c0 = d[b0] - x, c1 = e[b0], c2 = 0.0;
       for (int k = 0; k < bn - 1; ++k) {
