@@ -229,12 +229,31 @@ __device__ __forceinline__ void d4_cn_count(const Topo& t, int a, int b, double 
 }
 
 // ---------------------------------------------------------------------------
-// Warp / block reductions (wavefront of 64 lanes)
+// Wavefront / block reductions (64-lane wavefronts) built on DPP moves: two
+// quad permutations and two row rotations leave every lane with its 16-lane row
+// sum, row broadcasts 15 and 31 accumulate the rows into lane 63, which is then
+// read back as a wave-uniform value.  update_dpp with old = 0 makes lanes
+// without a source (or outside the row mask) contribute zero.
 // ---------------------------------------------------------------------------
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_move<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x124, 0xf>(v);  // row_ror:4
+  v += dpp_move<0x128, 0xf>(v);  // row_ror:8
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // Block-wide sum for blockDim.x <= 1024; red must hold blockDim.x/64 doubles.

@@ -244,6 +244,7 @@ struct Device {
   size_t arena_bytes = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<hipEvent_t> probe_pool;  // per-launch timing events, grow-only
+  size_t lds_limit = 65536;            // dynamic LDS granted to the back-transformation
 };
 
 Device* device_create(int device_id, void* stream, std::string& error) {
@@ -284,6 +285,13 @@ Device* device_create(int device_id, void* stream, std::string& error) {
        hipMemcpy(d->d4_refng, xtbp_d4_ref_ngauss, sizeof(int) * 262, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(d->kpair, xtbp_kpair, sizeof(double) * 9, hipMemcpyHostToDevice) == hipSuccess;
   for (int i = 0; i < 4 && ok; ++i) ok = hipEventCreate(&d->ev[i]) == hipSuccess;
+  // gfx950 grants up to 160 KB of LDS to one workgroup; opt the back-transformation in
+  int lds = 0;
+  if (ok && hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device_id) == hipSuccess &&
+      lds > 65536 &&
+      hipFuncSetAttribute((const void*)k_backtransform, hipFuncAttributeMaxDynamicSharedMemorySize, lds) ==
+          hipSuccess)
+    d->lds_limit = (size_t)lds;
   if (!ok) {
     error = "failed to allocate or upload the device parameter tables";
     device_destroy(d);
@@ -386,6 +394,9 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   // GEMM tile queues and eigenvector back-transformation column blocks
   std::vector<TileTask> full, lower;
   std::vector<BackTask> back;
+  std::vector<ElemChunk> chunks;
+  std::vector<RowChunk> rowchunks;
+  std::vector<int64_t> part_off(B + 1, 0);
   size_t back_lds = 0;
   for (int i = 0; i < B; ++i) {
     const int s = T.order[i];
@@ -397,9 +408,22 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
         if (a >= b) lower.push_back({s, a, b});
       }
     int cw = 64;
-    while (cw > 4 && (size_t)(n * cw + n + 256) * sizeof(double) > kBackLdsBytes) cw /= 2;
-    back_lds = std::max(back_lds, (size_t)(n * cw + n + 256) * sizeof(double));
+    while (cw > 1 && (size_t)(n * cw + 2 * n + kBackThreads) * sizeof(double) > dev->lds_limit) cw /= 2;
+    back_lds = std::max(back_lds, (size_t)(n * cw + 2 * n + kBackThreads) * sizeof(double));
     for (int j0 = 0; j0 < n; j0 += cw) back.push_back({s, j0, cw, 0});
+    for (int e0 = 0; e0 < n * n; e0 += kFockChunk) chunks.push_back({s, e0});
+  }
+  // Mulliken row chunks, partials laid out system by system
+  for (int s = 0; s < B; ++s) {
+    const int n = T.ao_off[s + 1] - T.ao_off[s];
+    const int nch = (n + kMullRows - 1) / kMullRows;
+    part_off[s + 1] = part_off[s] + (int64_t)nch * n * 11;
+  }
+  for (int i = 0; i < B; ++i) {
+    const int s = T.order[i];
+    const int n = T.ao_off[s + 1] - T.ao_off[s];
+    for (int j = 0; j * kMullRows < n; ++j)
+      rowchunks.push_back({s, j * kMullRows, part_off[s] + (int64_t)j * n * 11});
   }
 
   // ---- arena layout ----
@@ -422,6 +446,10 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(3 * (size_t)NPC); addd(NPC); addd(NPC);
   bytes += plan_bytes<TileTask>(full.size()) + plan_bytes<TileTask>(lower.size());
   bytes += plan_bytes<BackTask>(back.size());
+  bytes += plan_bytes<ElemChunk>(chunks.size());
+  bytes += plan_bytes<RowChunk>(rowchunks.size());
+  addl(B + 1);
+  addd((size_t)part_off[B]);  // Mulliken chunk partials
   // work arrays
   const int kMat = 20;  // S, D3, Q6, H0, X, H, A, Z, Y, C, P, W, T
   addd((size_t)kMat * NN);
@@ -499,6 +527,10 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   TileTask* d_full = cv.take<TileTask>(full.size());
   TileTask* d_lower = cv.take<TileTask>(lower.size());
   BackTask* d_back = cv.take<BackTask>(back.size());
+  ElemChunk* d_chunks = cv.take<ElemChunk>(chunks.size());
+  RowChunk* d_rowchunks = cv.take<RowChunk>(rowchunks.size());
+  int64_t* d_part_off = cv.take<int64_t>(B + 1);
+  double* d_mpart = cv.take<double>((size_t)part_off[B]);
   HIP_TRY(up(d_atom_off, T.atom_off));
   HIP_TRY(up(d_shell_off, T.shell_off));
   HIP_TRY(up(d_ao_off, T.ao_off));
@@ -538,6 +570,9 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   HIP_TRY(up(d_full, full));
   HIP_TRY(up(d_lower, lower));
   HIP_TRY(up(d_back, back));
+  HIP_TRY(up(d_chunks, chunks));
+  HIP_TRY(up(d_rowchunks, rowchunks));
+  HIP_TRY(up(d_part_off, part_off));
 
   t.atom_off = d_atom_off; t.shell_off = d_shell_off; t.ao_off = d_ao_off; t.vec_off = d_vec_off;
   t.pc_off = d_pc_off; t.mat_off = d_mat_off; t.gam_off = d_gam_off; t.apair_off = d_apair_off;
@@ -676,7 +711,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(probe_end(kKIntegrals, 0.0, 88.0 * all2));  // S + 3 D + 6 Q + H0 written
   }
   HIP_TRY(probe_begin());
-  hipLaunchKernelGGL(k_cholesky_inverse, dim3(B), dim3(256), 0, sm, t, d_order, dS, dA, dX, d_st);
+  hipLaunchKernelGGL(k_cholesky_inverse, dim3(B), dim3(512), 0, sm, t, d_order, dS, dA, dX, d_st);
   HIP_TRY(probe_end(kKCholesky, all3 / 3.0, 0.0));
   hipLaunchKernelGGL(k_init_active, dim3((B + 255) / 256), dim3(256), 0, sm, B, d_st, d_active,
                      (int*)nullptr, d_st);
@@ -701,7 +736,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
                        d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, d_dw4, d_vat, d_vsh, d_vd, d_vqq);
     HIP_TRY(probe_end(kKPotentials, 0.0, 0.0));
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL(k_fock, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, sm, t, NN,
+    hipLaunchKernelGGL(k_fock, dim3((unsigned)chunks.size()), dim3(256), 0, sm, t, d_chunks,
                        d_active, dS, dD, dQ, dH0, d_vsh, d_vd, d_vqq, dH);
     HIP_TRY(probe_end(kKFock, 0.0, 96.0 * a2));  // reads S, D(3), Q(6), H0; writes H
     // H' = X^T H X
@@ -722,7 +757,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
                        dZ, EigenScratch{dT, dH, dC, dP, dW}, d_rank, d_eps);
     HIP_TRY(probe_end(kKTridiagEigen, 0.0, 0.0));
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL(k_backtransform, dim3((unsigned)back.size()), dim3(256), back_lds, sm, t, d_back,
+    hipLaunchKernelGGL(k_backtransform, dim3((unsigned)back.size()), dim3(kBackThreads), back_lds, sm, t, d_back,
                        dA, dZ, dY, d_rank, d_active);
     HIP_TRY(probe_end(kKBacktransform, 2.0 * a3, 0.0));
     HIP_TRY(probe_begin());
@@ -739,8 +774,10 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
                        dC, dP, d_occ, d_kocc, d_active);
     HIP_TRY(probe_end(kKDensity, 2.0 * a3, 0.0));
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL(k_mulliken, dim3(B), dim3(256), 0, sm, t, d_order, d_active, dP, dS, dD, dQ,
-                       dH0, d_colsum, mx.raw, d_st);
+    hipLaunchKernelGGL(k_mulliken_rows, dim3((unsigned)rowchunks.size()), dim3(256), 0, sm, t,
+                       d_rowchunks, d_active, dP, dS, dD, dQ, dH0, d_mpart);
+    hipLaunchKernelGGL(k_mulliken_finish, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_part_off,
+                       d_mpart, d_colsum, mx.raw, d_st);
     HIP_TRY(probe_end(kKMulliken, 0.0, 88.0 * a2));  // reads P, S, D(3), Q(6), H0
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_energy_mix, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_cnd4,

@@ -19,8 +19,6 @@ struct TileTask {
 
 constexpr int kTile = 64;
 constexpr int kTileK = 16;
-// dynamic LDS budget of one back-transformation block (column block + reflector + partial dots)
-constexpr size_t kBackLdsBytes = 65536;
 
 // C[sys] = sum_k opA(A)[i][k] * scale[k] * opB(B)[k][j]   (row-major n x n operands)
 //   TA: A is read transposed (A[k][i]); TB: B is read transposed (B[j][k]).
@@ -99,20 +97,11 @@ __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, con
   }
 }
 
-// Per-system Cholesky S = L L^T and X = L^{-T} (row-major; X upper triangular).
-// L is built in `work` (column-major lower).  status[s] = 8 on a non-positive
-// pivot or when the conditioning bound 1/(|S|_1 |L^-1|_1 |L^-T|_1) < 1e-12.
-__global__ __launch_bounds__(256) void k_cholesky_inverse(Topo t, const int* order, const double* S,
-                                                          double* work, double* X, SysState* status) {
-  const int s = order[blockIdx.x];
-  const int n = t.ao_off[s + 1] - t.ao_off[s];
-  const int64_t mo = t.mat_off[s];
-  const double* Sm = S + mo;
-  double* L = work + mo;  // column-major: L[i + j*n]
-  double* Xm = X + mo;
-  __shared__ double red[16];
-  __shared__ int bad;
-  if (threadIdx.x == 0) bad = 0;
+// Cholesky S = L L^T and X = L^{-T} with everything in global memory (any n);
+// returns 1/(|S|_1 |X|_1 |X|_inf) through `bound`, or sets *bad on a
+// non-positive pivot.  `mx` holds blockDim.x doubles.
+__device__ void cholesky_global(const double* Sm, double* L, double* Xm, int n, double* mx, int& bad,
+                                double& bound) {
   for (int64_t idx = threadIdx.x; idx < (int64_t)n * n; idx += blockDim.x) L[idx] = Sm[idx];
   __syncthreads();
   // |S|_1
@@ -122,11 +111,9 @@ __global__ __launch_bounds__(256) void k_cholesky_inverse(Topo t, const int* ord
     for (int i = 0; i < n; ++i) sacc += fabs(Sm[(int64_t)i * n + j]);
     colmax = fmax(colmax, sacc);
   }
-  // block max via shared array
-  __shared__ double mx[256];
   mx[threadIdx.x] = colmax;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + w]);
     __syncthreads();
   }
@@ -179,7 +166,7 @@ __global__ __launch_bounds__(256) void k_cholesky_inverse(Topo t, const int* ord
   }
   mx[threadIdx.x] = c1;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + w]);
     __syncthreads();
   }
@@ -187,16 +174,13 @@ __global__ __launch_bounds__(256) void k_cholesky_inverse(Topo t, const int* ord
   __syncthreads();
   mx[threadIdx.x] = ci;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + w]);
     __syncthreads();
   }
   const double n2 = mx[0];
-  if (threadIdx.x == 0) {
-    const double rcond_bound = 1.0 / (snorm * n1 * n2);
-    if (bad || !(rcond_bound >= 1e-12) || !isfinite(rcond_bound)) status[s].status = 8;
-  }
-  (void)red;
+  bound = 1.0 / (snorm * n1 * n2);
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -320,10 +304,12 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
+constexpr int kTriPad = 257;  // row stride of the partial-sum buffers (bank spread)
+
 template <int NR>
 __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, double* e, double* v,
-                                                  double* pq, double* colpart, double* red,
-                                                  double* bc) {
+                                                  double* pq, double* rowpart, double* colpart,
+                                                  double* red, double* bc) {
   constexpr int NC = NR / 2;
   constexpr int NE = tri_slots(NR);
   const int tid = threadIdx.x, lr = tid >> 5, lc = tid & 31, w = tid >> 6;
@@ -336,6 +322,8 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
     const double x = (i < n && j < n) ? A[j + (int64_t)i * n] : 0.0;
     if constexpr (tri_strict(TI, TJ)) a[sl] = x;
     else a[sl] = i > j ? x : (i == j ? 0.5 * x : 0.0);
+    // keep one row tile of loads in flight (address registers would spill the matrix)
+    if constexpr (TJ == TI / 2) __builtin_amdgcn_sched_barrier(0);
   });
   for (int k = 0; k + 2 < n; ++k) {
     const int Tk = k >> 5, ck = k & 31;
@@ -371,19 +359,18 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
       }
     }
     __syncthreads();
+    // no reflection (inv = 0) runs the same path with v = 0: p, q and the
+    // update vanish and the stored reflector is zero
     const double inv = bc[0];
-    if (inv == 0.0) {
-      if (tid > k && tid < n) A[tid + (int64_t)k * n] = 0.0;
-      continue;
-    }
     if (tid > k && tid < n) {
       const double vi = v[tid] * inv;
       v[tid] = vi;
       A[tid + (int64_t)k * n] = vi;
     }
     __syncthreads();
-    // p = A v (v = 0 outside the trailing block); a row tile's partial is
-    // reduced over its 32 lanes as soon as its last slot is done
+    // p = A v (v = 0 outside the trailing block).  Lane partials go to LDS,
+    // transposed so that thread i sums its row's 32 + 16 partials without
+    // cross-lane shuffles: rowpart[lc][i] from x v_j, colpart[lr][j] from x v_i.
     double pc[NC];
 #pragma unroll
     for (int TJ = 0; TJ < NC; ++TJ) pc[TJ] = 0.0;
@@ -391,31 +378,23 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
     static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
       constexpr int sl = decltype(S)::value;
       constexpr int TI = tri_row(sl), TJ = tri_col(sl);
-      if constexpr (TJ == 0) __builtin_amdgcn_sched_barrier(0);  // one row tile at a time
       const double x = a[sl];
       pr = fma(x, v[32 * TJ + lc], pr);
       pc[TJ] = fma(x, v[16 * TI + lr], pc[TJ]);
       if constexpr (TJ == TI / 2) {  // last slot of row tile TI
-        pr += __shfl_xor(pr, 16, 32);
-        pr += __shfl_xor(pr, 8, 32);
-        pr += __shfl_xor(pr, 4, 32);
-        pr += __shfl_xor(pr, 2, 32);
-        pr += __shfl_xor(pr, 1, 32);
-        if (lc == 0) pq[16 * TI + lr] = pr;
+        rowpart[lc * kTriPad + 16 * TI + lr] = pr;
         pr = 0.0;
       }
     });
 #pragma unroll
-    for (int TJ = 0; TJ < NC; ++TJ) {
-      const double c = pc[TJ] + __shfl_xor(pc[TJ], 32, 64);
-      if ((tid & 63) < 32) colpart[w * 256 + 32 * TJ + lc] = c;
-    }
+    for (int TJ = 0; TJ < NC; ++TJ) colpart[lr * kTriPad + 32 * TJ + lc] = pc[TJ];
     __syncthreads();
     double p = 0.0;
     if (tid > k && tid < n) {
-      p = pq[tid];
-#pragma unroll
-      for (int r = 0; r < 8; ++r) p += colpart[r * 256 + tid];
+#pragma unroll 8
+      for (int r = 0; r < 32; ++r) p += rowpart[r * kTriPad + tid];
+#pragma unroll 8
+      for (int r = 0; r < 16; ++r) p += colpart[r * kTriPad + tid];
     }
     const double vt = v[tid];
     const double K = block_sum(vt * p, red);
@@ -425,7 +404,6 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
     static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
       constexpr int sl = decltype(S)::value;
       constexpr int TI = tri_row(sl), TJ = tri_col(sl);
-      if constexpr (TJ == 0) __builtin_amdgcn_sched_barrier(0);  // one row tile at a time
       const int i = 16 * TI + lr, j = 32 * TJ + lc;
       const double u = v[i] * pq[j] + pq[i] * v[j];
       if constexpr (tri_strict(TI, TJ)) a[sl] -= 2.0 * u;
@@ -455,14 +433,192 @@ __global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, doubl
   double* e = eall + t.ao_off[s];
   __shared__ double v[kMaxAO];
   __shared__ double pq[kMaxAO];
-  __shared__ double colpart[8 * 256];
+  __shared__ double rowpart[32 * kTriPad];
+  __shared__ double colpart[16 * kTriPad];
   __shared__ double red[16];
   __shared__ double bc[2];
-  if (n <= 64) tridiag_registers<4>(A, n, d, e, v, pq, colpart, red, bc);
-  else if (n <= 128) tridiag_registers<8>(A, n, d, e, v, pq, colpart, red, bc);
-  else if (n <= 192) tridiag_registers<12>(A, n, d, e, v, pq, colpart, red, bc);
-  else if (n <= 256) tridiag_registers<16>(A, n, d, e, v, pq, colpart, red, bc);
+  if (n <= 64) tridiag_registers<4>(A, n, d, e, v, pq, rowpart, colpart, red, bc);
+  else if (n <= 128) tridiag_registers<8>(A, n, d, e, v, pq, rowpart, colpart, red, bc);
+  else if (n <= 192) tridiag_registers<12>(A, n, d, e, v, pq, rowpart, colpart, red, bc);
+  else if (n <= 256) tridiag_registers<16>(A, n, d, e, v, pq, rowpart, colpart, red, bc);
   else tridiag_global(A, n, d, e, v, pq, red, bc);
+}
+
+// Overlap Cholesky S = L L^T (L column-major into `L`) and X = L^{-T} (row-major,
+// upper triangular, into `Xm`) with the lower triangle in registers (same
+// 16 x 32 tile layout as tridiag_registers, 512 threads, n <= 16 NR):
+//   factorization  per column k: the owners broadcast column k through LDS, the
+//                  trailing lower triangle takes the rank-1 update l l^T / l_kk;
+//   inverse        Gauss-Jordan on the identity: row k of L^{-1} is scaled by
+//                  1/L_kk, then rows i > k lose L_ik times it;
+//   bound          1/(|S|_1 |X|_1 |X|_inf) from LDS-transposed row/column sums.
+template <int NR>
+__device__ __forceinline__ void cholesky_registers(const double* Sm, double* L, double* Xm, int n,
+                                                   double* lcol, double* xr, double* rowpart,
+                                                   double* colpart, double* red, int& bad,
+                                                   double& bound) {
+  constexpr int NE = tri_slots(NR);
+  const int tid = threadIdx.x, lr = tid >> 5, lc = tid & 31;
+  double a[NE];
+  static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+    constexpr int sl = decltype(S)::value;
+    constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+    const int i = 16 * TI + lr, j = 32 * TJ + lc;
+    a[sl] = (i < n && j < n && i >= j) ? Sm[(int64_t)i * n + j] : 0.0;
+    if constexpr (TJ == TI / 2) __builtin_amdgcn_sched_barrier(0);
+  });
+  // |S|_1 (columns, coalesced)
+  double colmax = 0.0;
+  for (int j = tid; j < n; j += blockDim.x) {
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) acc += fabs(Sm[(int64_t)i * n + j]);
+    colmax = fmax(colmax, acc);
+  }
+  // lcol is zero outside [k, n): the rank-1 update then needs no masks (column k
+  // itself and the strict upper part of diagonal tiles take harmless updates
+  // that are never read again)
+  if (tid >= n) lcol[tid] = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const int Tk = k >> 5, ck = k & 31;
+    if (tid < k) lcol[tid] = 0.0;
+    if (lc == ck) {
+      static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+        constexpr int sl = decltype(S)::value;
+        constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+        if (TJ == Tk) {
+          const int i = 16 * TI + lr;
+          if (i >= k && i < n) lcol[i] = a[sl];
+        }
+      });
+    }
+    __syncthreads();
+    double piv = lcol[k];
+    if (!(piv > 0.0)) {
+      if (tid == 0) bad = 1;
+      piv = 1.0;
+    }
+    const double sq = sqrt(piv);
+    if (tid >= k && tid < n) L[tid + (int64_t)k * n] = tid == k ? sq : lcol[tid] / sq;
+    const double ip = 1.0 / piv;
+    static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+      constexpr int sl = decltype(S)::value;
+      constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+      a[sl] -= lcol[16 * TI + lr] * lcol[32 * TJ + lc] * ip;
+    });
+    __syncthreads();
+  }
+  // L^{-1} in the same registers
+  static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+    constexpr int sl = decltype(S)::value;
+    constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+    const int i = 16 * TI + lr, j = 32 * TJ + lc;
+    a[sl] = (i == j && i < n) ? 1.0 : 0.0;
+  });
+  // lcol holds column k of L below the diagonal (zero elsewhere), xr row k of
+  // L^{-1} (zero right of the diagonal), so the update needs no masks either
+  for (int k = 0; k < n; ++k) {
+    if (tid < kMaxAO) {
+      lcol[tid] = (tid > k && tid < n) ? L[tid + (int64_t)k * n] : 0.0;
+      if (tid > k) xr[tid] = 0.0;
+    }
+    __syncthreads();
+    const double il = 1.0 / L[k + (int64_t)k * n];
+    const bool rowk = lr == (k & 15);
+    static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+      constexpr int sl = decltype(S)::value;
+      constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+      const int j = 32 * TJ + lc;
+      const bool own = rowk && TI == (k >> 4) && j <= k;
+      const double nv = own ? a[sl] * il : a[sl];
+      a[sl] = nv;  // select, not a branch: keeps the register array free of copies
+      if (own) xr[j] = nv;
+    });
+    __syncthreads();
+    static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+      constexpr int sl = decltype(S)::value;
+      constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+      a[sl] -= lcol[16 * TI + lr] * xr[32 * TJ + lc];
+    });
+    __syncthreads();
+  }
+  // X[r][c] = Linv[c][r]: the registers hold Linv(i, j), i >= j; X's strict lower part is zero
+  for (int64_t idx = tid; idx < (int64_t)n * n; idx += blockDim.x) {
+    const int r = (int)(idx / n), c = (int)(idx - (int64_t)r * n);
+    if (c < r) Xm[idx] = 0.0;
+  }
+  double pr = 0.0;
+  double pc[NR / 2];
+#pragma unroll
+  for (int TJ = 0; TJ < NR / 2; ++TJ) pc[TJ] = 0.0;
+  static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+    constexpr int sl = decltype(S)::value;
+    constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+    const int i = 16 * TI + lr, j = 32 * TJ + lc;
+    const double x = (i >= j && i < n) ? a[sl] : 0.0;
+    if (i >= j && i < n) Xm[(int64_t)j * n + i] = x;
+    pr += fabs(x);
+    pc[TJ] += fabs(x);
+    if constexpr (TJ == TI / 2) {
+      rowpart[lc * kTriPad + 16 * TI + lr] = pr;
+      pr = 0.0;
+      __builtin_amdgcn_sched_barrier(0);  // one row tile of stores in flight
+    }
+  });
+#pragma unroll
+  for (int TJ = 0; TJ < NR / 2; ++TJ) colpart[lr * kTriPad + 32 * TJ + lc] = pc[TJ];
+  // rows of Linv never reach tiles outside the stored set; fill the gaps with zeros
+  for (int idx = tid; idx < 32 * kTriPad; idx += blockDim.x) {
+    const int row = idx % kTriPad;
+    if (row >= 16 * NR && row < kTriPad) rowpart[idx] = 0.0;
+  }
+  __syncthreads();
+  double rs = 0.0, cs = 0.0;
+  if (tid < n) {
+    for (int r = 0; r < 32; ++r) rs += rowpart[r * kTriPad + tid];
+    for (int r = 0; r < 16; ++r) cs += colpart[r * kTriPad + tid];
+  }
+  // block maxima: |S|_1, |X|_1 = max row sum of Linv, |X|_inf = max column sum of Linv
+  double m[3] = {colmax, rs, cs};
+  for (int q = 0; q < 3; ++q) {
+    double v = m[q];
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    v = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, red[w]);
+    m[q] = v;
+  }
+  bound = 1.0 / (m[0] * m[1] * m[2]);
+  __syncthreads();
+}
+
+// Per-system overlap factorization, 512 threads: X = L^{-T} with L in `work`
+// (column-major).  status = 8 on a non-positive pivot or when the conditioning
+// bound 1/(|S|_1 |X|_1 |X|_inf) < 1e-12 (the bound never exceeds rcond_1).
+__global__ __launch_bounds__(512) void k_cholesky_inverse(Topo t, const int* order, const double* S,
+                                                          double* work, double* X, SysState* status) {
+  const int s = order[blockIdx.x];
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t mo = t.mat_off[s];
+  const double* Sm = S + mo;
+  double* L = work + mo;
+  double* Xm = X + mo;
+  __shared__ double lcol[kMaxAO];
+  __shared__ double xr[kMaxAO];
+  __shared__ double rowpart[32 * kTriPad];
+  __shared__ double colpart[16 * kTriPad];
+  __shared__ double red[16];
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  double bound = 0.0;
+  if (n <= 64) cholesky_registers<4>(Sm, L, Xm, n, lcol, xr, rowpart, colpart, red, bad, bound);
+  else if (n <= 128) cholesky_registers<8>(Sm, L, Xm, n, lcol, xr, rowpart, colpart, red, bad, bound);
+  else if (n <= 192) cholesky_registers<12>(Sm, L, Xm, n, lcol, xr, rowpart, colpart, red, bad, bound);
+  else if (n <= 256) cholesky_registers<16>(Sm, L, Xm, n, lcol, xr, rowpart, colpart, red, bad, bound);
+  else cholesky_global(Sm, L, Xm, n, rowpart, bad, bound);
+  if (threadIdx.x == 0 && (bad || !(bound >= 1e-12) || !isfinite(bound))) status[s].status = 8;
 }
 
 // x / y via the hardware reciprocal refined by one Newton step (a few ulp,
@@ -765,35 +921,55 @@ struct BackTask {
   int pad;
 };
 
-__global__ __launch_bounds__(256) void k_backtransform(Topo t, const BackTask* tasks, const double* Aall,
-                                                       const double* Zall, double* Yall,
-                                                       const int* rank_all,
-                                                       const unsigned char* active) {
+// 512 threads; dynamic LDS = (n cw + 2 n + 512) doubles: the column block, two
+// reflector buffers and the partial dots.  The next reflector is fetched into
+// registers while the current one is applied, so each reflector costs two
+// barriers and no exposed global-load latency.
+constexpr int kBackThreads = 512;
+
+__global__ __launch_bounds__(kBackThreads) void k_backtransform(Topo t, const BackTask* tasks,
+                                                                const double* Aall,
+                                                                const double* Zall, double* Yall,
+                                                                const int* rank_all,
+                                                                const unsigned char* active) {
   extern __shared__ double lds[];
   const BackTask task = tasks[blockIdx.x];
   const int s = task.sys;
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
   const int64_t mo = t.mat_off[s];
-  const double* A = Aall + mo;
-  const double* Z = Zall + mo;
-  double* Y = Yall + mo;
+  const double* __restrict__ A = Aall + mo;
+  const double* __restrict__ Z = Zall + mo;
+  double* __restrict__ Y = Yall + mo;
   const int* rank = rank_all + t.ao_off[s];
   const int cw = task.cw, j0 = task.j0;
-  double* Zb = lds;               // [n][cw]
-  double* hv = lds + n * cw;      // [n]
-  double* part = hv + n;          // [256]
+  double* Zb = lds;                 // [n][cw]
+  double* hvb = lds + n * cw;       // [2][n]
+  double* part = hvb + 2 * n;       // [512]
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int idx = tid; idx < n * cw; idx += nt) {
     const int i = idx / cw, c = idx % cw;
     Zb[idx] = (j0 + c < n) ? Z[(int64_t)i * n + j0 + c] : 0.0;
   }
   const int c = tid % cw, rg = tid / cw, nrg = nt / cw;
-  for (int k = n - 3; k >= 0; --k) {
+  // reflector k occupies A[k+1 .. n-1, k] (column-major); m = n - k - 1 <= 1023
+  int k = n - 3;
+  if (k >= 0) {
+    double* hv = hvb + (k & 1) * n;
+    for (int i = tid; i < n - k - 1; i += nt) hv[i] = A[(k + 1 + i) + (int64_t)k * n];
+  }
+  __syncthreads();
+  for (; k >= 0; --k) {
     const int m = n - k - 1;
-    __syncthreads();
-    for (int i = tid; i < m; i += nt) hv[i] = A[(k + 1 + i) + (int64_t)k * n];
-    __syncthreads();
+    const double* hv = hvb + (k & 1) * n;
+    // prefetch reflector k - 1 (length m + 1) into registers
+    double r0 = 0.0, r1 = 0.0;
+    const bool more = k >= 1;
+    if (more) {
+      const int kn = k - 1;
+      if (tid < m + 1) r0 = A[(kn + 1 + tid) + (int64_t)kn * n];
+      if (tid + nt < m + 1) r1 = A[(kn + 1 + tid + nt) + (int64_t)kn * n];
+    }
     double p = 0.0;
     for (int i = rg; i < m; i += nrg) p += hv[i] * Zb[(k + 1 + i) * cw + c];
     part[rg * cw + c] = p;
@@ -802,8 +978,13 @@ __global__ __launch_bounds__(256) void k_backtransform(Topo t, const BackTask* t
     for (int r = 0; r < nrg; ++r) dot += part[r * cw + c];
     dot *= 2.0;
     for (int i = rg; i < m; i += nrg) Zb[(k + 1 + i) * cw + c] -= dot * hv[i];
+    if (more) {
+      double* hn = hvb + ((k - 1) & 1) * n;
+      if (tid < m + 1) hn[tid] = r0;
+      if (tid + nt < m + 1) hn[tid + nt] = r1;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   for (int idx = tid; idx < n * cw; idx += nt) {
     const int i = idx / cw, cc = idx % cw;
     if (j0 + cc < n) Y[(int64_t)i * n + rank[j0 + cc]] = Zb[idx];

@@ -285,33 +285,72 @@ __global__ __launch_bounds__(256) void k_potentials(Topo t, const int* order,
 }
 
 // H = H0 - S (u_m+u_n)/2 - D_mn.v(A_n)/2 - D_nm.v(A_m)/2 - (Q likewise)   (mulliken.hpp:222-237)
-__global__ __launch_bounds__(256) void k_fock(Topo t, int64_t total, const unsigned char* active,
-                                              const double* S, const double* D, const double* Q,
-                                              const double* H0, const double* vsh,
-                                              const double* vd, const double* vqq, double* H) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  int lo = 0, hi = t.B;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (t.mat_off[mid] <= g) lo = mid; else hi = mid;
-  }
-  const int s = lo;
+//
+// The transposed blocks follow from the element's own S, D, Q (moments about
+// the ket atom): with d = R(A_n) - R(A_m),
+//   D_nm,a = D_mn,a + d_a S_mn
+//   M_nm,ab = M_mn,ab + d_a D_mn,b + d_b D_mn,a + d_a d_b S_mn      (raw second moments)
+// and the traceless Q = 3/2 M - 1/2 tr(M) 1, so only coalesced reads are needed.
+// One block = 1024 consecutive elements of one system (host-built chunk list).
+struct ElemChunk {
+  int sys;
+  int start;  // first element (row-major) of the chunk inside the system's n x n matrix
+};
+constexpr int kFockChunk = 1024;
+
+__global__ __launch_bounds__(256) void k_fock(Topo t, const ElemChunk* chunks,
+                                              const unsigned char* active,
+                                              const double* __restrict__ S,
+                                              const double* __restrict__ D,
+                                              const double* __restrict__ Q,
+                                              const double* __restrict__ H0,
+                                              const double* __restrict__ vsh,
+                                              const double* __restrict__ vd,
+                                              const double* __restrict__ vqq,
+                                              double* __restrict__ H) {
+  const ElemChunk ch = chunks[blockIdx.x];
+  const int s = ch.sys;
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
-  const int64_t mo = t.mat_off[s], loc = g - mo, nn = (int64_t)n * n;
-  const int r = (int)(loc / n), c = (int)(loc % n);
-  const int gr = t.ao_off[s] + r, gc = t.ao_off[s] + c;
-  const int ar = t.ao_atom[gr], ac = t.ao_atom[gc];
-  const int64_t f = mo + loc, rvl = (int64_t)c * n + r;
+  const int o = t.ao_off[s];
+  const int64_t mo = t.mat_off[s], nn = (int64_t)n * n;
   const double* Ds = D + 3 * mo;
   const double* Qs = Q + 6 * mo;
-  double h = -0.5 * S[f] * (vsh[t.ao_sh[gr]] + vsh[t.ao_sh[gc]]);
 #pragma unroll
-  for (int x = 0; x < 3; ++x) h -= 0.5 * (Ds[x * nn + loc] * vd[3 * ac + x] + Ds[x * nn + rvl] * vd[3 * ar + x]);
+  for (int q = 0; q < kFockChunk / 256; ++q) {
+    const int e = ch.start + q * 256 + threadIdx.x;
+    if (e >= nn) break;
+    const int r = e / n, c = e - r * n;
+    const int ar = t.ao_atom[o + r], ac = t.ao_atom[o + c];
+    const double sv = S[mo + e];
+    double dm[3], qm[6];
 #pragma unroll
-  for (int x = 0; x < 6; ++x) h -= 0.5 * (Qs[x * nn + loc] * vqq[6 * ac + x] + Qs[x * nn + rvl] * vqq[6 * ar + x]);
-  H[f] = H0[f] + h;
+    for (int x = 0; x < 3; ++x) dm[x] = Ds[x * nn + e];
+#pragma unroll
+    for (int x = 0; x < 6; ++x) qm[x] = Qs[x * nn + e];
+    const double dl[3] = {t.xyz[3 * ac] - t.xyz[3 * ar], t.xyz[3 * ac + 1] - t.xyz[3 * ar + 1],
+                          t.xyz[3 * ac + 2] - t.xyz[3 * ar + 2]};
+    // transposed dipole and traceless quadrupole ([xx, xy, yy, xz, yz, zz])
+    double dt[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) dt[x] = dm[x] + dl[x] * sv;
+    const double dd = dl[0] * dm[0] + dl[1] * dm[1] + dl[2] * dm[2];
+    const double l2 = dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2];
+    const double trc = 0.5 * (2.0 * dd + l2 * sv);
+    const double qt[6] = {
+        qm[0] + 1.5 * (2.0 * dl[0] * dm[0] + dl[0] * dl[0] * sv) - trc,
+        qm[1] + 1.5 * (dl[0] * dm[1] + dl[1] * dm[0] + dl[0] * dl[1] * sv),
+        qm[2] + 1.5 * (2.0 * dl[1] * dm[1] + dl[1] * dl[1] * sv) - trc,
+        qm[3] + 1.5 * (dl[0] * dm[2] + dl[2] * dm[0] + dl[0] * dl[2] * sv),
+        qm[4] + 1.5 * (dl[1] * dm[2] + dl[2] * dm[1] + dl[1] * dl[2] * sv),
+        qm[5] + 1.5 * (2.0 * dl[2] * dm[2] + dl[2] * dl[2] * sv) - trc};
+    double h = -0.5 * sv * (vsh[t.ao_sh[o + r]] + vsh[t.ao_sh[o + c]]);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) h -= 0.5 * (dm[x] * vd[3 * ac + x] + dt[x] * vd[3 * ar + x]);
+#pragma unroll
+    for (int x = 0; x < 6; ++x) h -= 0.5 * (qm[x] * vqq[6 * ac + x] + qt[x] * vqq[6 * ar + x]);
+    H[mo + e] = H0[mo + e] + h;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -402,41 +441,87 @@ __global__ __launch_bounds__(64) void k_fermi(Topo t, const int* order, const un
 }
 
 // ---------------------------------------------------------------------------
-// Mulliken raw multipoles and Tr(P H0) (reference mulliken_kernels_impl.hpp:40-115).
-// colsum holds [n][10] column partials per system at ao_off*10.
+// Mulliken raw multipoles and Tr(P H0) (reference mulliken_kernels_impl.hpp:40-115),
+// in two deterministic phases:
+//   k_mulliken_rows    one block per 32-row chunk of a system: per column c the
+//                      partial sums -P.S, -P.D(3), -P.Q(6) and P.H0 over the
+//                      chunk's rows, stored at part[off + 11 c + k]
+//   k_mulliken_finish  one block per system: chunk partials added in chunk
+//                      order, then shell charges, atomic dipoles/quadrupoles
+//                      (by ket AO) and the core energy.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mulliken(Topo t, const int* order,
-                                                  const unsigned char* active, const double* P,
-                                                  const double* S, const double* D,
-                                                  const double* Q, const double* H0,
-                                                  double* colsum, double* raw, SysState* st) {
-  const int s = order[blockIdx.x];
+struct RowChunk {
+  int sys;
+  int r0;
+  int64_t off;  // first partial of this chunk in the partial buffer
+};
+constexpr int kMullRows = 32;
+
+__global__ __launch_bounds__(256) void k_mulliken_rows(Topo t, const RowChunk* chunks,
+                                                       const unsigned char* active,
+                                                       const double* __restrict__ P,
+                                                       const double* __restrict__ S,
+                                                       const double* __restrict__ D,
+                                                       const double* __restrict__ Q,
+                                                       const double* __restrict__ H0,
+                                                       double* __restrict__ part) {
+  const RowChunk ch = chunks[blockIdx.x];
+  const int s = ch.sys;
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
   const int64_t mo = t.mat_off[s], nn = (int64_t)n * n;
+  const double* Ds = D + 3 * mo;
+  const double* Qs = Q + 6 * mo;
+  const int r1 = min(ch.r0 + kMullRows, n);
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    double acc[11];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) acc[k] = 0.0;
+    for (int r = ch.r0; r < r1; ++r) {
+      const int64_t loc = (int64_t)r * n + c;
+      const double p = P[mo + loc];
+      acc[0] = fma(-p, S[mo + loc], acc[0]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc[1 + k] = fma(-p, Ds[k * nn + loc], acc[1 + k]);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[4 + k] = fma(-p, Qs[k * nn + loc], acc[4 + k]);
+      acc[10] = fma(H0[mo + loc], p, acc[10]);
+    }
+#pragma unroll
+    for (int k = 0; k < 11; ++k) part[ch.off + 11 * (int64_t)c + k] = acc[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mulliken_finish(Topo t, const int* order,
+                                                         const unsigned char* active,
+                                                         const int64_t* part_off,
+                                                         const double* __restrict__ part,
+                                                         double* colsum, double* raw,
+                                                         SysState* st) {
+  const int s = order[blockIdx.x];
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
   const int a0 = t.atom_off[s], N = t.atom_off[s + 1] - a0;
   const int s0 = t.shell_off[s], ns = t.shell_off[s + 1] - s0;
+  const int nch = (n + kMullRows - 1) / kMullRows;
+  const double* ps = part + part_off[s];
   double* cs = colsum + (int64_t)t.ao_off[s] * 10;
   __shared__ double red[16];
   double core = 0.0;
   for (int c = threadIdx.x; c < n; c += blockDim.x) {
-    double acc[10];
-    for (int k = 0; k < 10; ++k) acc[k] = 0.0;
-    const double* Ds = D + 3 * mo;
-    const double* Qs = Q + 6 * mo;
-    for (int r = 0; r < n; ++r) {
-      const int64_t loc = (int64_t)r * n + c;
-      const int64_t idx = mo + loc;
-      const double p = P[idx];
-      acc[0] = fma(-p, S[idx], acc[0]);
-      for (int k = 0; k < 3; ++k) acc[1 + k] = fma(-p, Ds[k * nn + loc], acc[1 + k]);
-      for (int k = 0; k < 6; ++k) acc[4 + k] = fma(-p, Qs[k * nn + loc], acc[4 + k]);
-      core = fma(H0[idx], p, core);
+    double acc[11];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) acc[k] = 0.0;
+    for (int j = 0; j < nch; ++j) {
+      const double* pj = ps + (int64_t)j * n * 11 + 11 * (int64_t)c;
+#pragma unroll
+      for (int k = 0; k < 11; ++k) acc[k] += pj[k];
     }
+#pragma unroll
     for (int k = 0; k < 10; ++k) cs[(int64_t)c * 10 + k] = acc[k];
+    core += acc[10];
   }
   core = block_sum(core, red);
-  __syncthreads();
   double* rv = raw + t.vec_off[s];
   for (int i = threadIdx.x; i < ns; i += blockDim.x) {
     const int sh = s0 + i;
