@@ -289,8 +289,8 @@ Device* device_create(int device_id, void* stream, std::string& error) {
   int lds = 0;
   if (ok && hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device_id) == hipSuccess &&
       lds > 65536 &&
-      hipFuncSetAttribute((const void*)k_backtransform, hipFuncAttributeMaxDynamicSharedMemorySize, lds) ==
-          hipSuccess)
+      hipFuncSetAttribute((const void*)k_backtransform, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          lds - (int)kBackStaticLds) == hipSuccess)
     d->lds_limit = (size_t)lds;
   if (!ok) {
     error = "failed to allocate or upload the device parameter tables";
@@ -407,9 +407,12 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
         full.push_back({s, a, b});
         if (a >= b) lower.push_back({s, a, b});
       }
-    int cw = 64;
-    while (cw > 1 && (size_t)(n * cw + 2 * n + kBackThreads) * sizeof(double) > dev->lds_limit) cw /= 2;
-    back_lds = std::max(back_lds, (size_t)(n * cw + 2 * n + kBackThreads) * sizeof(double));
+    int cw = 64;  // register path (n <= 256): 64 columns per block, no dynamic LDS
+    if (n > 256) {
+      const size_t budget = dev->lds_limit - kBackStaticLds;
+      while (cw > 1 && (size_t)(n * cw + 2 * n + kBackThreads) * sizeof(double) > budget) cw /= 2;
+      back_lds = std::max(back_lds, (size_t)(n * cw + 2 * n + kBackThreads) * sizeof(double));
+    }
     for (int j0 = 0; j0 < n; j0 += cw) back.push_back({s, j0, cw, 0});
     for (int e0 = 0; e0 < n * n; e0 += kFockChunk) chunks.push_back({s, e0});
   }
@@ -461,6 +464,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(3 * (size_t)N); addd(6 * (size_t)N);
   for (int k = 0; k < 5; ++k) addd(NAO);  // eps fa fb occ fe
   addd(NAO); addd(NAO); addi(NAO);        // tridiagonal d, e and eigenvalue ranks
+  addd(B);                                // per-system eigenvalue change of the last round
+  addd(NAO);                              // eigenvector normalization factors
   addd((size_t)NAO * 10);      // mulliken column partials
   for (int k = 0; k < 5; ++k) addd(NV); // x raw cur prev prev_res
   addd((size_t)mem * NV); addd((size_t)mem * NV);  // df, u
@@ -625,6 +630,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   double* d_fb = cv.take<double>(NAO);
   double* d_occ = cv.take<double>(NAO);
   double* d_fe = cv.take<double>(NAO);
+  double* d_shift = cv.take<double>(B);
+  double* d_zscale = cv.take<double>(NAO);
   double* d_td = cv.take<double>(NAO);
   double* d_te = cv.take<double>(NAO);
   int* d_rank = cv.take<int>(NAO);
@@ -754,11 +761,12 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(probe_end(kKTridiag, (4.0 / 3.0) * a3, 0.0));
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_tridiag_eigen, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_td, d_te,
-                       dZ, EigenScratch{dT, dH, dC, dP, dW}, d_rank, d_eps);
+                       dZ, EigenScratch{dT, dH, dC, dP, dW}, d_rank, d_eps, d_shift, it,
+                       d_zscale);
     HIP_TRY(probe_end(kKTridiagEigen, 0.0, 0.0));
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_backtransform, dim3((unsigned)back.size()), dim3(kBackThreads), back_lds, sm, t, d_back,
-                       dA, dZ, dY, d_rank, d_active);
+                       dA, dZ, dY, d_rank, d_zscale, d_active);
     HIP_TRY(probe_end(kKBacktransform, 2.0 * a3, 0.0));
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_fermi, dim3(B), dim3(64), 0, sm, t, d_order, d_active, d_eps, d_fa, d_fb,

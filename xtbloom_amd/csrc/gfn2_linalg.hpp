@@ -643,6 +643,149 @@ __device__ __forceinline__ int sturm_count(const double* d, const double* e2, in
   return c;
 }
 
+// Sturm count of T[b0:b1) - x I (as sturm_count) together with
+// L(x) = d/dx log|det(T - x I)| = sum_i 1/(x - lambda_i), accumulated from the
+// pivot recurrence q_i = (d_i - x) - e_{i-1}^2 / q_{i-1},
+// q_i' = -1 + e_{i-1}^2 q_{i-1}' / q_{i-1}^2, L = sum q_i' / q_i.
+__device__ __forceinline__ int sturm_count_logd(const double* d, const double* e2, int b0, int b1,
+                                                double x, double pivmin, double& L) {
+  double q = d[b0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int c = q <= 0.0 ? 1 : 0;
+  double r = fast_div(1.0, q), dq = -1.0;
+  double acc = dq * r;
+  for (int i = b0 + 1; i < b1; ++i) {
+    const double tq = e2[i - 1] * r;
+    dq = fma(tq * r, dq, -1.0);
+    q = (d[i] - x) - tq;
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q <= 0.0 ? 1 : 0;
+    r = fast_div(1.0, q);
+    acc = fma(dq, r, acc);
+  }
+  L = acc;
+  return c;
+}
+
+// Eigenvalues of T by Sturm-count bracketing with Newton steps on
+// det(T - x I) once a bracket isolates exactly its eigenvalue (a step outside
+// the bracket falls back to bisection).  Converged when the bracket or the
+// Newton step is below 2 eps |T_b| + pivmin, the accuracy of bisection.  A
+// guess (the previous SCC round's eigenvalue) first narrows the bracket to
+// [guess - delta, guess + delta] where the counts allow.
+//
+// The lanes of the block share the eigenvalues through an LDS counter: a lane
+// whose eigenvalue converged takes the next index, so a wave's cost follows
+// the total work instead of its slowest eigenvalue.  One trip of the loop is
+// one pass over the block.
+struct EigState {
+  int j, b0, b1, k, it, phase;  // phase 0/1: guess bracket ends, 2: main iteration
+  int clo, chi;
+  double lo, hi, x, tol, guess, dl;
+  double dxo;  // size of the step before the last one (Newton acceptance)
+};
+
+// Gershgorin interval and tolerance of the unreduced block [b0, b1)
+__device__ __forceinline__ void block_bounds(const double* d, const double* e, int b0, int b1,
+                                             double pivmin, double& lo, double& hi, double& tol) {
+  double gl = 1e300, gu = -1e300;
+  for (int i = b0; i < b1; ++i) {
+    const double off = (i > b0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < b1 ? fabs(e[i]) : 0.0);
+    gl = fmin(gl, d[i] - off);
+    gu = fmax(gu, d[i] + off);
+  }
+  const double tnorm = fmax(fabs(gl), fabs(gu));
+  const double fudge = 2.0 * kEps * tnorm * (b1 - b0) + 4.0 * pivmin;
+  lo = gl - fudge;
+  hi = gu + fudge;
+  tol = 2.0 * kEps * tnorm + pivmin;
+}
+
+// starts eigenvalue j; when T is unsplit the whole-matrix bounds are passed in
+__device__ __forceinline__ bool eig_start(EigState& st, int j, const double* d, const double* e,
+                                          int n, double pivmin, bool unsplit, double glo,
+                                          double ghi, double gtol, bool guess_ok, double guess,
+                                          double delta, double* lam) {
+  int b0 = 0, b1 = n;
+  if (!unsplit) {
+    b0 = j;
+    b1 = j + 1;
+    while (b0 > 0 && e[b0 - 1] != 0.0) --b0;
+    while (b1 < n && e[b1 - 1] != 0.0) ++b1;
+  }
+  if (b1 - b0 == 1) {
+    lam[j] = d[j];
+    return false;
+  }
+  st.j = j; st.b0 = b0; st.b1 = b1; st.k = j - b0; st.it = 0;
+  if (unsplit) {
+    st.lo = glo; st.hi = ghi; st.tol = gtol;
+  } else {
+    block_bounds(d, e, b0, b1, pivmin, st.lo, st.hi, st.tol);
+  }
+  st.clo = 0;
+  st.chi = b1 - b0;
+  const bool g = guess_ok && guess > st.lo && guess < st.hi;
+  st.guess = guess;
+  st.dl = fmax(delta, 4.0 * st.tol);
+  st.phase = g ? 0 : 2;
+  st.x = g ? guess - st.dl : 0.5 * (st.lo + st.hi);
+  st.dxo = st.hi - st.lo;
+  return true;
+}
+
+// one pass at st.x; returns true (result in `out`) when the eigenvalue has converged
+__device__ __forceinline__ bool eig_step(EigState& st, const double* d, const double* e2,
+                                         double pivmin, double& out) {
+  double L;
+  const int c = sturm_count_logd(d, e2, st.b0, st.b1, st.x, pivmin, L);
+  if (st.phase < 2) {
+    // guess bracket: only narrow where the count confirms the side
+    if (st.x > st.lo && st.x < st.hi) {
+      if (c <= st.k) { st.lo = st.x; st.clo = c; } else { st.hi = st.x; st.chi = c; }
+    }
+    if (st.phase == 0) {
+      st.phase = 1;
+      st.x = st.guess + st.dl;
+      if (st.x < st.hi) return false;
+    }
+    st.phase = 2;
+    st.x = (st.guess > st.lo && st.guess < st.hi) ? st.guess : 0.5 * (st.lo + st.hi);
+    st.dxo = st.hi - st.lo;
+    return false;
+  }
+  if (c <= st.k) { st.lo = st.x; st.clo = c; } else { st.hi = st.x; st.chi = c; }
+  if (st.hi - st.lo <= st.tol || ++st.it >= 256) {
+    out = 0.5 * (st.lo + st.hi);
+    return true;
+  }
+  // Newton only on an isolating bracket and only while it converges: the step
+  // must stay inside and be less than half the step before last (otherwise
+  // bisect), which rules out slow one-sided creeping
+  const double dxo = st.dxo;
+  if (st.clo == st.k && st.chi == st.k + 1 && L != 0.0 && isfinite(L)) {
+    const double step = -1.0 / L;
+    if (fabs(step) <= st.tol) {
+      out = st.x + step;
+      return true;
+    }
+    const double xn = st.x + step;
+    if (xn > st.lo && xn < st.hi && fabs(step) < 0.5 * dxo) {
+      st.dxo = fabs(step);
+      st.x = xn;
+      return false;
+    }
+  }
+  const double mid = 0.5 * (st.lo + st.hi);
+  if (mid == st.lo || mid == st.hi) {
+    out = mid;
+    return true;
+  }
+  st.dxo = fabs(mid - st.x);
+  st.x = mid;
+  return false;
+}
+
 // deterministic start vector in [-1, 1)
 __device__ __forceinline__ double start_component(int j, int i) {
   uint64_t z = (uint64_t)(j + 1) * 0x9E3779B97F4A7C15ull + (uint64_t)(i + 1) * 0xBF58476D1CE4E5B9ull;
@@ -667,15 +810,18 @@ struct EigenScratch {
 // its twisted factorization: forward pivots D+ (in F0) and backward pivots D-
 // (in F1) of T - x I, twist index r = argmin |D+_r + D-_r - (d_r - x)|, then
 // z_r = 1, z_i = -(e_i / D+_i) z_{i+1} below r and z_i = -(e_{i-1} / D-_i) z_{i-1}
-// above r; normalized with the largest component positive.  Returns false
-// (leaving Z to be overwritten) if the recurrences produced a non-finite value.
+// above r.  Z receives the unnormalized vector and `scale` the factor that
+// normalizes it with the largest component positive (applied when the vector
+// is back-transformed).  Returns false (leaving Z to be overwritten) if the
+// recurrences produced a non-finite value.
 __device__ __forceinline__ bool twisted_vector(const double* d, const double* e, int b0, int b1,
                                                double x, double pivmin, int n, int jj,
                                                double* __restrict__ Dp, double* __restrict__ Dm,
-                                               double* __restrict__ Z) {
+                                               double* __restrict__ Z, double& scale) {
   double q = d[b0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   Dp[(int64_t)b0 * n + jj] = q;
+#pragma unroll 4
   for (int i = b0 + 1; i < b1; ++i) {
     q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
     if (fabs(q) < pivmin) q = -pivmin;
@@ -686,6 +832,7 @@ __device__ __forceinline__ bool twisted_vector(const double* d, const double* e,
   Dm[(int64_t)(b1 - 1) * n + jj] = q;
   int r = b1 - 1;
   double gmin = fabs(Dp[(int64_t)(b1 - 1) * n + jj] + q - (d[b1 - 1] - x));
+#pragma unroll 4
   for (int i = b1 - 2; i >= b0; --i) {
     q = (d[i] - x) - e[i] * e[i] / q;
     if (fabs(q) < pivmin) q = -pivmin;
@@ -698,6 +845,7 @@ __device__ __forceinline__ bool twisted_vector(const double* d, const double* e,
   }
   double z = 1.0, nrm = 1.0, big = 1.0;
   Z[(int64_t)r * n + jj] = 1.0;
+#pragma unroll 4
   for (int i = r - 1; i >= b0; --i) {
     z = -(e[i] / Dp[(int64_t)i * n + jj]) * z;
     Z[(int64_t)i * n + jj] = z;
@@ -705,6 +853,7 @@ __device__ __forceinline__ bool twisted_vector(const double* d, const double* e,
     if (fabs(z) > fabs(big)) big = z;
   }
   z = 1.0;
+#pragma unroll 4
   for (int i = r + 1; i < b1; ++i) {
     z = -(e[i - 1] / Dm[(int64_t)i * n + jj]) * z;
     Z[(int64_t)i * n + jj] = z;
@@ -712,16 +861,19 @@ __device__ __forceinline__ bool twisted_vector(const double* d, const double* e,
     if (fabs(z) > fabs(big)) big = z;
   }
   if (!isfinite(nrm) || !(nrm > 0.0)) return false;
-  const double f = (big < 0.0 ? -1.0 : 1.0) / sqrt(nrm);
-  for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] *= f;
+  scale = (big < 0.0 ? -1.0 : 1.0) / sqrt(nrm);
   return true;
 }
 
+// From the second SCC round on, wall holds the previous round's ascending
+// eigenvalues on entry; shift[s] carries the largest eigenvalue change of the
+// system's previous round (written here for the next one).
 __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
                                                        const unsigned char* active,
                                                        const double* dall, const double* eall,
                                                        double* Zall, EigenScratch sc, int* rank_all,
-                                                       double* wall) {
+                                                       double* wall, double* shift, int round,
+                                                       double* zscale_all) {
   const int s = order[blockIdx.x];
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
@@ -732,6 +884,7 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
   double* __restrict__ F2 = sc.u2 + mo;
   double* __restrict__ FM = sc.mult + mo;
   double* __restrict__ FP = sc.piv + mo;
+  double* zscale = zscale_all + t.ao_off[s];
   __shared__ double d[kMaxAO];
   __shared__ double e[kMaxAO];   // split off-diagonal (negligible entries zeroed)
   __shared__ double e2[kMaxAO];
@@ -768,31 +921,74 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
   }
   const double pivmin = kSafeMin * fmax(1.0, emax2);
 
-  // ---- eigenvalues: thread per index, bisection inside the unreduced block ----
-  for (int j = tid; j < n; j += nt) {
-    int b0 = j, b1 = j + 1;
-    while (b0 > 0 && e[b0 - 1] != 0.0) --b0;
-    while (b1 < n && e[b1 - 1] != 0.0) ++b1;
-    const int k = j - b0, bn = b1 - b0;
-    if (bn == 1) {
-      lam[j] = d[j];
-      continue;
-    }
-    double gl = 1e300, gu = -1e300, tnorm = 0.0;
-    for (int i = b0; i < b1; ++i) {
-      const double off = (i > b0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < b1 ? fabs(e[i]) : 0.0);
+  // ---- eigenvalues: lanes draw indices from a shared counter ----
+  // the previous round's eigenvalues are guesses when T did not split
+  __shared__ int nsplit, next_j;
+  if (tid == 0) { nsplit = 0; next_j = 0; }
+  __syncthreads();
+  for (int i = tid; i + 1 < n; i += nt)
+    if (e[i] == 0.0) atomicAdd(&nsplit, 1);
+  __syncthreads();
+  const bool unsplit = nsplit == 0;
+  const bool have_old = round >= 1 && unsplit;
+  const bool guess_ok = round >= 2 && unsplit;
+  const double delta = guess_ok ? 2.0 * shift[s] : 0.0;
+  // whole-matrix Gershgorin bounds (used when T is unsplit)
+  __shared__ double gbound[3];
+  if (unsplit) {
+    double gl = 1e300, gu = -1e300;
+    for (int i = tid; i < n; i += nt) {
+      const double off = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
       gl = fmin(gl, d[i] - off);
       gu = fmax(gu, d[i] + off);
     }
-    tnorm = fmax(fabs(gl), fabs(gu));
-    const double fudge = 2.0 * kEps * tnorm * bn + 4.0 * pivmin;
-    double lo = gl - fudge, hi = gu + fudge;
-    for (int it = 0; it < 200; ++it) {
-      const double mid = 0.5 * (lo + hi);
-      if (hi - lo <= 2.0 * kEps * fmax(fabs(lo), fabs(hi)) + pivmin || mid == lo || mid == hi) break;
-      if (sturm_count(d, e2, b0, b1, mid, pivmin) <= k) lo = mid; else hi = mid;
+    for (int off = 32; off > 0; off >>= 1) {
+      gl = fmin(gl, __shfl_xor(gl, off, 64));
+      gu = fmax(gu, __shfl_xor(gu, off, 64));
     }
-    lam[j] = 0.5 * (lo + hi);
+    if ((tid & 63) == 0) { red[tid >> 6] = gl; red[8 + (tid >> 6)] = gu; }
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < (nt >> 6); ++w) { gl = fmin(gl, red[w]); gu = fmax(gu, red[8 + w]); }
+      const double tnorm = fmax(fabs(gl), fabs(gu));
+      const double fudge = 2.0 * kEps * tnorm * n + 4.0 * pivmin;
+      gbound[0] = gl - fudge;
+      gbound[1] = gu + fudge;
+      gbound[2] = 2.0 * kEps * tnorm + pivmin;
+    }
+    __syncthreads();
+  }
+  double dmax = 0.0;
+  {
+    EigState st;
+    bool busy = false;
+    for (;;) {
+      if (!busy) {
+        const int j = atomicAdd(&next_j, 1);
+        if (j >= n) break;
+        const double g = have_old ? wall[t.ao_off[s] + j] : 0.0;
+        busy = eig_start(st, j, d, e, n, pivmin, unsplit, gbound[0], gbound[1], gbound[2], guess_ok,
+                         g, delta, lam);
+        if (!busy) continue;
+      }
+      double x;
+      if (eig_step(st, d, e2, pivmin, x)) {
+        lam[st.j] = x;
+        if (have_old) dmax = fmax(dmax, fabs(x - wall[t.ao_off[s] + st.j]));
+        busy = false;
+      }
+    }
+  }
+  // largest eigenvalue change of this round (block max), for the next round's brackets
+  for (int off = 32; off > 0; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+  __syncthreads();
+  if ((tid & 63) == 0) red[tid >> 6] = dmax;
+  __syncthreads();
+  if (tid == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (nt >> 6); ++w) m = fmax(m, red[w]);
+    // first round: no change measured yet, take a generous bracket
+    shift[s] = have_old ? m : 1.0;
   }
   __syncthreads();
 
@@ -816,6 +1012,7 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
       // rows outside the block are zero
       for (int i = 0; i < b0; ++i) Z[(int64_t)i * n + jj] = 0.0;
       for (int i = b1; i < n; ++i) Z[(int64_t)i * n + jj] = 0.0;
+      zscale[jj] = 1.0;
       if (bn == 1) {
         Z[(int64_t)b0 * n + jj] = 1.0;
         continue;
@@ -823,7 +1020,13 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
       double x = lam[jj];
       if (jj > j && x - xprev < pertol) x = xprev + pertol;
       xprev = x;
-      if (g1 - j == 1 && twisted_vector(d, e, b0, b1, x, pivmin, n, jj, F0, F1, Z)) continue;
+      if (g1 - j == 1) {
+        double f;
+        if (twisted_vector(d, e, b0, b1, x, pivmin, n, jj, F0, F1, Z, f)) {
+          zscale[jj] = f;
+          continue;
+        }
+      }
       // LU with partial pivoting of T_b - x I
       double c0 = d[b0] - x, c1 = e[b0], c2 = 0.0;
       for (int k = 0; k < bn - 1; ++k) {
@@ -921,35 +1124,100 @@ struct BackTask {
   int pad;
 };
 
-// 512 threads; dynamic LDS = (n cw + 2 n + 512) doubles: the column block, two
-// reflector buffers and the partial dots.  The next reflector is fetched into
-// registers while the current one is applied, so each reflector costs two
-// barriers and no exposed global-load latency.
+// 512 threads.  n <= 256: the 64-column block lives in registers (thread
+// (c, rg) = (tid % 64, tid / 64) holds rows rg, rg + 8, ...), reflectors are
+// staged in LDS zero-padded to absolute row indices, so every thread applies
+// every reflector without masks and a reflector costs two barriers.  n > 256:
+// the column block lives in dynamic LDS ((n cw + 2 n + 512) doubles).  In both
+// paths the next reflector is fetched into registers while the current one is
+// applied.
 constexpr int kBackThreads = 512;
+constexpr int kBackRegRows = 32;  // rows per thread of the register path (n <= 256)
+constexpr int kBackChunk = 8;     // reflectors fetched per global-load round trip
+constexpr size_t kBackStaticLds = (2 * kBackChunk * 256 + kBackThreads) * sizeof(double);
 
-__global__ __launch_bounds__(kBackThreads) void k_backtransform(Topo t, const BackTask* tasks,
-                                                                const double* Aall,
-                                                                const double* Zall, double* Yall,
-                                                                const int* rank_all,
-                                                                const unsigned char* active) {
-  extern __shared__ double lds[];
-  const BackTask task = tasks[blockIdx.x];
-  const int s = task.sys;
-  if (active && !active[s]) return;
-  const int n = t.ao_off[s + 1] - t.ao_off[s];
-  const int64_t mo = t.mat_off[s];
-  const double* __restrict__ A = Aall + mo;
-  const double* __restrict__ Z = Zall + mo;
-  double* __restrict__ Y = Yall + mo;
-  const int* rank = rank_all + t.ao_off[s];
-  const int cw = task.cw, j0 = task.j0;
+__device__ __forceinline__ void backtransform_registers(const double* __restrict__ A,
+                                                        const double* __restrict__ Z,
+                                                        double* __restrict__ Y, const int* rank,
+                                                        const double* zscale, int n, int j0,
+                                                        double* ring, double* part) {
+  const int tid = threadIdx.x, c = tid & 63, rg = tid >> 6;
+  const bool col_ok = j0 + c < n;
+  const double f = col_ok ? zscale[j0 + c] : 0.0;
+  double z[kBackRegRows];
+#pragma unroll
+  for (int q = 0; q < kBackRegRows; ++q) {
+    const int i = rg + 8 * q;
+    z[q] = (i < n && col_ok) ? f * Z[(int64_t)i * n + j0 + c] : 0.0;
+  }
+  // Reflectors are applied in the order k = n-3, ..., 0, in chunks of
+  // kBackChunk; chunk ch holds k = n-3 - ch*kBackChunk - u at ring slot
+  // ((ch & 1) * kBackChunk + u) * 256, zero-padded to absolute rows.
+  constexpr int kVals = kBackChunk * 256 / kBackThreads;
+  auto fetch = [&](int ch, double (&r)[kVals]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int w = 0; w < kVals; ++w) {
+      const int v = tid + kBackThreads * w, u = v >> 8, i = v & 255;
+      const int k = n - 3 - ch * kBackChunk - u;
+      r[w] = (k >= 0 && i > k && i < n) ? A[i + (int64_t)k * n] : 0.0;
+    }
+  };
+  auto stage = [&](int ch, const double (&r)[kVals]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int w = 0; w < kVals; ++w) ring[(ch & 1) * kBackChunk * 256 + tid + kBackThreads * w] = r[w];
+  };
+  const int nref = n - 2;  // reflectors k = 0 .. n-3
+  double rn[kVals];
+  if (nref > 0) {
+    fetch(0, rn);
+    stage(0, rn);
+  }
+  __syncthreads();
+  for (int ch = 0; ch * kBackChunk < nref; ++ch) {
+    const bool more = (ch + 1) * kBackChunk < nref;
+    if (more) fetch(ch + 1, rn);  // in flight while this chunk is applied
+    for (int u = 0; u < kBackChunk; ++u) {
+      const int k = n - 3 - ch * kBackChunk - u;
+      if (k < 0) break;
+      const double* hv = ring + ((ch & 1) * kBackChunk + u) * 256;
+      double p = 0.0;
+#pragma unroll
+      for (int q = 0; q < kBackRegRows; ++q) p = fma(hv[rg + 8 * q], z[q], p);
+      part[rg * 64 + c] = p;
+      __syncthreads();
+      double dot = 0.0;
+#pragma unroll
+      for (int w = 0; w < 8; ++w) dot += part[w * 64 + c];
+      dot *= 2.0;
+#pragma unroll
+      for (int q = 0; q < kBackRegRows; ++q) z[q] = fma(-dot, hv[rg + 8 * q], z[q]);
+      __syncthreads();
+    }
+    if (more) stage(ch + 1, rn);
+    __syncthreads();
+  }
+  if (col_ok) {
+    const int dst = rank[j0 + c];
+#pragma unroll
+    for (int q = 0; q < kBackRegRows; ++q) {
+      const int i = rg + 8 * q;
+      if (i < n) Y[(int64_t)i * n + dst] = z[q];
+    }
+  }
+}
+
+__device__ __forceinline__ void backtransform_lds(const double* __restrict__ A,
+                                                  const double* __restrict__ Z,
+                                                  double* __restrict__ Y, const int* rank,
+                                                  const double* zscale, int n, int j0, int cw,
+                                                  double* lds) {
   double* Zb = lds;                 // [n][cw]
   double* hvb = lds + n * cw;       // [2][n]
   double* part = hvb + 2 * n;       // [512]
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int idx = tid; idx < n * cw; idx += nt) {
     const int i = idx / cw, c = idx % cw;
-    Zb[idx] = (j0 + c < n) ? Z[(int64_t)i * n + j0 + c] : 0.0;
+    Zb[idx] = (j0 + c < n) ? zscale[j0 + c] * Z[(int64_t)i * n + j0 + c] : 0.0;
   }
   const int c = tid % cw, rg = tid / cw, nrg = nt / cw;
   // reflector k occupies A[k+1 .. n-1, k] (column-major); m = n - k - 1 <= 1023
@@ -962,7 +1230,6 @@ __global__ __launch_bounds__(kBackThreads) void k_backtransform(Topo t, const Ba
   for (; k >= 0; --k) {
     const int m = n - k - 1;
     const double* hv = hvb + (k & 1) * n;
-    // prefetch reflector k - 1 (length m + 1) into registers
     double r0 = 0.0, r1 = 0.0;
     const bool more = k >= 1;
     if (more) {
@@ -975,7 +1242,7 @@ __global__ __launch_bounds__(kBackThreads) void k_backtransform(Topo t, const Ba
     part[rg * cw + c] = p;
     __syncthreads();
     double dot = 0.0;
-    for (int r = 0; r < nrg; ++r) dot += part[r * cw + c];
+    for (int w = 0; w < nrg; ++w) dot += part[w * cw + c];
     dot *= 2.0;
     for (int i = rg; i < m; i += nrg) Zb[(k + 1 + i) * cw + c] -= dot * hv[i];
     if (more) {
@@ -989,6 +1256,28 @@ __global__ __launch_bounds__(kBackThreads) void k_backtransform(Topo t, const Ba
     const int i = idx / cw, cc = idx % cw;
     if (j0 + cc < n) Y[(int64_t)i * n + rank[j0 + cc]] = Zb[idx];
   }
+}
+
+__global__ __launch_bounds__(kBackThreads) void k_backtransform(Topo t, const BackTask* tasks,
+                                                                const double* Aall,
+                                                                const double* Zall, double* Yall,
+                                                                const int* rank_all,
+                                                                const double* zscale_all,
+                                                                const unsigned char* active) {
+  extern __shared__ double lds[];
+  __shared__ double ring[2 * kBackChunk * 256];
+  __shared__ double part[kBackThreads];
+  const BackTask task = tasks[blockIdx.x];
+  const int s = task.sys;
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t mo = t.mat_off[s];
+  const int* rank = rank_all + t.ao_off[s];
+  const double* zscale = zscale_all + t.ao_off[s];
+  if (n <= 256)
+    backtransform_registers(Aall + mo, Zall + mo, Yall + mo, rank, zscale, n, task.j0, ring, part);
+  else
+    backtransform_lds(Aall + mo, Zall + mo, Yall + mo, rank, zscale, n, task.j0, task.cw, lds);
 }
 
 }  // namespace xtb
