@@ -416,6 +416,22 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     for (int j0 = 0; j0 < n; j0 += cw) back.push_back({s, j0, cw, 0});
     for (int e0 = 0; e0 < n * n; e0 += kFockChunk) chunks.push_back({s, e0});
   }
+  // systems split by whether any shell is a d shell: [0] s/p only, [1] with d;
+  // per class the prefix of ordered shell pairs and of ordered atom pairs
+  std::vector<int> cls_sys[2];
+  std::vector<int64_t> cls_sp[2], cls_ap[2];
+  for (int c = 0; c < 2; ++c) { cls_sp[c].push_back(0); cls_ap[c].push_back(0); }
+  for (int i = 0; i < B; ++i) {
+    const int s = T.order[i];
+    bool has_d = false;
+    for (int sh = T.shell_off[s]; sh < T.shell_off[s + 1]; ++sh) has_d = has_d || T.sh_l[sh] == 2;
+    const int c = has_d ? 1 : 0;
+    const int64_t ns = T.shell_off[s + 1] - T.shell_off[s];
+    const int64_t na = T.atom_off[s + 1] - T.atom_off[s];
+    cls_sys[c].push_back(s);
+    cls_sp[c].push_back(cls_sp[c].back() + ns * ns);
+    cls_ap[c].push_back(cls_ap[c].back() + na * na);
+  }
   // Mulliken row chunks, partials laid out system by system
   for (int s = 0; s < B; ++s) {
     const int n = T.ao_off[s + 1] - T.ao_off[s];
@@ -451,6 +467,11 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   bytes += plan_bytes<BackTask>(back.size());
   bytes += plan_bytes<ElemChunk>(chunks.size());
   bytes += plan_bytes<RowChunk>(rowchunks.size());
+  for (int c = 0; c < 2; ++c) {
+    addi(cls_sys[c].size());
+    addl(cls_sp[c].size());
+    addl(cls_ap[c].size());
+  }
   addl(B + 1);
   addd((size_t)part_off[B]);  // Mulliken chunk partials
   // work arrays
@@ -534,6 +555,17 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   BackTask* d_back = cv.take<BackTask>(back.size());
   ElemChunk* d_chunks = cv.take<ElemChunk>(chunks.size());
   RowChunk* d_rowchunks = cv.take<RowChunk>(rowchunks.size());
+  SysList sp_list[2], ap_list[2];
+  for (int c = 0; c < 2; ++c) {
+    int* ls = cv.take<int>(cls_sys[c].size());
+    int64_t* lsp = cv.take<int64_t>(cls_sp[c].size());
+    int64_t* lap = cv.take<int64_t>(cls_ap[c].size());
+    HIP_TRY(up(ls, cls_sys[c]));
+    HIP_TRY(up(lsp, cls_sp[c]));
+    HIP_TRY(up(lap, cls_ap[c]));
+    sp_list[c] = SysList{ls, lsp, (int)cls_sys[c].size()};
+    ap_list[c] = SysList{ls, lap, (int)cls_sys[c].size()};
+  }
   int64_t* d_part_off = cv.take<int64_t>(B + 1);
   double* d_mpart = cv.take<double>((size_t)part_off[B]);
   HIP_TRY(up(d_atom_off, T.atom_off));
@@ -713,8 +745,12 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   HIP_TRY(probe_end(kKGeometry, 0.0, 0.0));
   if (NSP > 0) {
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL(k_integrals, dim3((unsigned)((NSP + 127) / 128)), dim3(128), 0, sm, t,
-                       d_sp_off, NSP, d_cn, dS, dD, dQ, dH0);
+    if (cls_sp[0].back() > 0)
+      hipLaunchKernelGGL((k_integrals<false>), dim3((unsigned)((cls_sp[0].back() + 127) / 128)), dim3(128), 0,
+                         sm, t, sp_list[0], cls_sp[0].back(), d_cn, dS, dD, dQ, dH0);
+    if (cls_sp[1].back() > 0)
+      hipLaunchKernelGGL((k_integrals<true>), dim3((unsigned)((cls_sp[1].back() + 127) / 128)), dim3(128), 0,
+                         sm, t, sp_list[1], cls_sp[1].back(), d_cn, dS, dD, dQ, dH0);
     HIP_TRY(probe_end(kKIntegrals, 0.0, 88.0 * all2));  // S + 3 D + 6 Q + H0 written
   }
   HIP_TRY(probe_begin());
@@ -831,8 +867,16 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(probe_end(kKEnergyWeighted, 2.0 * v3, 0.0));
     if (NAP > 0) {
       HIP_TRY(probe_begin());
-      hipLaunchKernelGGL(k_force_pairs, dim3((unsigned)((NAP + 127) / 128)), dim3(128), 0, sm, t,
-                         NAP, d_valid, d_cn, dP, dW, dS, d_vsh, d_vd, d_vqq, d_grad, d_dedcn);
+      for (int c = 0; c < 2; ++c) {
+        const int64_t np = cls_ap[c].back();
+        if (np == 0) continue;
+        if (c == 0)
+          hipLaunchKernelGGL((k_force_pairs<false>), dim3((unsigned)((np + 127) / 128)), dim3(128), 0, sm, t,
+                             ap_list[c], np, d_valid, d_cn, dP, dW, dS, d_vsh, d_vd, d_vqq, d_grad, d_dedcn);
+        else
+          hipLaunchKernelGGL((k_force_pairs<true>), dim3((unsigned)((np + 127) / 128)), dim3(128), 0, sm, t,
+                             ap_list[c], np, d_valid, d_cn, dP, dW, dS, d_vsh, d_vd, d_vqq, d_grad, d_dedcn);
+      }
       HIP_TRY(probe_end(kKForcePairs, 0.0, 24.0 * v2));  // reads P, W, S
     }
   }

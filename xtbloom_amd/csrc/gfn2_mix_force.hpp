@@ -290,8 +290,10 @@ __device__ void force_block(const Topo& t, int s, int si, int sj, const double* 
   }
 }
 
-// one thread per ordered atom pair (A, C) of a system
-__global__ __launch_bounds__(128) void k_force_pairs(Topo t, const int64_t total,
+// one thread per ordered atom pair (A, C) of a system (N x N items); HAS_D as
+// for k_integrals
+template <bool HAS_D>
+__global__ __launch_bounds__(128) void k_force_pairs(Topo t, SysList list, const int64_t total,
                                                      const unsigned char* valid, const double* cn,
                                                      const double* P, const double* W,
                                                      const double* Sm, const double* vsh,
@@ -299,29 +301,34 @@ __global__ __launch_bounds__(128) void k_force_pairs(Topo t, const int64_t total
                                                      double* grad, double* dedcn) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
-  int lo = 0, hi = t.B;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (t.apair_off[mid] <= g) lo = mid; else hi = mid;
-  }
-  const int s = lo;
+  int64_t loc;
+  const int s = syslist_find(list, g, loc);
   if (!valid[s]) return;
   const int a0 = t.atom_off[s], N = t.atom_off[s + 1] - a0;
-  const int64_t loc = g - t.apair_off[s];
   const int A = a0 + (int)(loc / N), C = a0 + (int)(loc % N);
   double gA[3] = {0, 0, 0}, gC[3] = {0, 0, 0}, dA = 0.0, dC = 0.0;
   for (int si = t.atom_sh0[A]; si < t.atom_sh0[A + 1]; ++si)
     for (int sj = t.atom_sh0[C]; sj < t.atom_sh0[C + 1]; ++sj) {
-      switch (t.sh_l[si] * 3 + t.sh_l[sj]) {
-        case 0: force_block<0, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-        case 1: force_block<0, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-        case 2: force_block<0, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-        case 3: force_block<1, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-        case 4: force_block<1, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-        case 5: force_block<1, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-        case 6: force_block<2, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-        case 7: force_block<2, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-        default: force_block<2, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+      const int key = t.sh_l[si] * 3 + t.sh_l[sj];
+      if constexpr (!HAS_D) {
+        switch (key) {
+          case 0: force_block<0, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 1: force_block<0, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 3: force_block<1, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          default: force_block<1, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+        }
+      } else {
+        switch (key) {
+          case 0: force_block<0, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 1: force_block<0, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 2: force_block<0, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 3: force_block<1, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 4: force_block<1, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 5: force_block<1, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 6: force_block<2, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 7: force_block<2, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          default: force_block<2, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+        }
       }
     }
   if (A != C) {
@@ -527,7 +534,8 @@ __global__ __launch_bounds__(256) void k_force_atoms(Topo t, const int* order,
 
 // D4 three-body ATM energy and gradient with zero-charge weights
 // (reference d4.cpp:1120-1362).  Pair data c6 / dc6/dcn are staged per ordered
-// pair; the triple loop accumulates into LDS and finishes with the D4-CN chain.
+// pair; the triple loop accumulates gradient and CN adjoint with LDS atomics
+// (one system per block, at most kMaxAO atoms) and finishes with the D4-CN chain.
 __global__ __launch_bounds__(256) void k_force_atm(Topo t, const int* order, const unsigned char* valid,
                                                    const double* cnd4, double* w4, double* dw4c,
                                                    double* c6pair, double* dedcn4, double* grad,
@@ -537,10 +545,14 @@ __global__ __launch_bounds__(256) void k_force_atm(Topo t, const int* order, con
   const int a0 = t.atom_off[s], a1 = t.atom_off[s + 1], N = a1 - a0;
   const int tid = threadIdx.x, nt = blockDim.x;
   __shared__ double red[16];
+  __shared__ double gsh[3 * kMaxAO];
+  __shared__ double csh[kMaxAO];
   double* c6 = c6pair + 2 * t.apair_off[s];  // [N*N][2]: c6, dc6/dcn(first)
   for (int a = a0 + tid; a < a1; a += nt) {
     d4_weights(t, a, cnd4[a], 0.0, w4 + kD4MaxRef * a, dw4c + kD4MaxRef * a, nullptr);
-    dedcn4[a] = 0.0;
+    const int la = a - a0;
+    gsh[3 * la] = gsh[3 * la + 1] = gsh[3 * la + 2] = 0.0;
+    csh[la] = 0.0;
   }
   __syncthreads();
   for (int64_t idx = tid; idx < (int64_t)N * N; idx += nt) {
@@ -595,7 +607,7 @@ __global__ __launch_bounds__(256) void k_force_atm(Topo t, const int* order, con
       for (int c = 0; c < 3; ++c) {
         gi[c] -= dg[0][c] + dg[1][c];
         gj[c] += dg[0][c] - dg[2][c];
-        atomicAdd(&grad[3 * k + c], dg[1][c] + dg[2][c]);
+        atomicAdd(&gsh[3 * lk + c], dg[1][c] + dg[2][c]);
       }
       const double de = ang * damp * c9;
       // dc6(x, y)/dcn_x is stored at [x][y] slot 1
@@ -604,14 +616,14 @@ __global__ __launch_bounds__(256) void k_force_atm(Topo t, const int* order, con
       const double dcjk_j = c6[2 * ((int64_t)lj * N + lk) + 1], dcjk_k = c6[2 * ((int64_t)lk * N + lj) + 1];
       dci -= 0.5 * de * (dcij_i / cij + dcik_i / cik);
       dcj -= 0.5 * de * (dcij_j / cij + dcjk_j / cjk);
-      atomicAdd(&dedcn4[k], -0.5 * de * (dcik_k / cik + dcjk_k / cjk));
+      atomicAdd(&csh[lk], -0.5 * de * (dcik_k / cik + dcjk_k / cjk));
     }
     for (int c = 0; c < 3; ++c) {
-      atomicAdd(&grad[3 * i + c], gi[c]);
-      atomicAdd(&grad[3 * j + c], gj[c]);
+      atomicAdd(&gsh[3 * li + c], gi[c]);
+      atomicAdd(&gsh[3 * lj + c], gj[c]);
     }
-    atomicAdd(&dedcn4[i], dci);
-    atomicAdd(&dedcn4[j], dcj);
+    atomicAdd(&csh[li], dci);
+    atomicAdd(&csh[lj], dcj);
   }
   e = block_sum(e, red);
   if (tid == 0) eatm[s] = e;
@@ -619,7 +631,7 @@ __global__ __launch_bounds__(256) void k_force_atm(Topo t, const int* order, con
   // D4-CN chain of the ATM adjoint
   for (int a = a0 + tid; a < a1; a += nt) {
     const double* ra = t.xyz + 3 * a;
-    double g[3] = {0, 0, 0};
+    double g[3] = {gsh[3 * (a - a0)], gsh[3 * (a - a0) + 1], gsh[3 * (a - a0) + 2]};
     for (int b = a0; b < a1; ++b) {
       if (b == a) continue;
       const double* rb = t.xyz + 3 * b;
@@ -629,10 +641,11 @@ __global__ __launch_bounds__(256) void k_force_atm(Topo t, const int* order, con
       const double rr = sqrt(r2);
       double val, der;
       d4_cn_count(t, a, b, rr, val, der);
-      const double sc = (dedcn4[a] + dedcn4[b]) * der / rr;
+      const double sc = (csh[a - a0] + csh[b - a0]) * der / rr;
       for (int c = 0; c < 3; ++c) g[c] += sc * r[c];
     }
-    for (int c = 0; c < 3; ++c) atomicAdd(&grad[3 * a + c], g[c]);
+    for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
+    dedcn4[a] = csh[a - a0];
   }
 }
 

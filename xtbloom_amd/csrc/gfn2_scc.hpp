@@ -178,32 +178,57 @@ __device__ void integral_block(const Topo& t, int s, int si, int sj, const doubl
     }
 }
 
-// pairs are enumerated per system as (i, j) over ns x ns; sp_off[B+1] prefix
-__global__ __launch_bounds__(128) void k_integrals(Topo t, const int64_t* sp_off, int64_t total,
+// A subset of the batch's systems with a prefix of per-system work items;
+// kernels find their system by binary search over the prefix.
+struct SysList {
+  const int* sys;      // [count] system indices
+  const int64_t* off;  // [count + 1] work-item prefix
+  int count;
+};
+
+__device__ __forceinline__ int syslist_find(const SysList& l, int64_t g, int64_t& loc) {
+  int lo = 0, hi = l.count;  // off[lo] <= g < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (l.off[mid] <= g) lo = mid; else hi = mid;
+  }
+  loc = g - l.off[lo];
+  return l.sys[lo];
+}
+
+// One thread per ordered shell pair (i, j) of a system (ns x ns items).  HAS_D
+// selects the instantiation: systems whose shells are all s/p use the smaller
+// one, so they do not pay the register budget of d-shell blocks.
+template <bool HAS_D>
+__global__ __launch_bounds__(128) void k_integrals(Topo t, SysList list, int64_t total,
                                                    const double* cn, double* S, double* D,
                                                    double* Q, double* H0) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
-  int lo = 0, hi = t.B;  // find s with sp_off[s] <= g < sp_off[s+1]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (sp_off[mid] <= g) lo = mid; else hi = mid;
-  }
-  const int s = lo;
+  int64_t loc;
+  const int s = syslist_find(list, g, loc);
   const int ns = t.shell_off[s + 1] - t.shell_off[s];
-  const int64_t loc = g - sp_off[s];
   const int si = t.shell_off[s] + (int)(loc / ns), sj = t.shell_off[s] + (int)(loc % ns);
   const int key = t.sh_l[si] * 3 + t.sh_l[sj];
-  switch (key) {
-    case 0: integral_block<0, 0>(t, s, si, sj, cn, S, D, Q, H0); break;
-    case 1: integral_block<0, 1>(t, s, si, sj, cn, S, D, Q, H0); break;
-    case 2: integral_block<0, 2>(t, s, si, sj, cn, S, D, Q, H0); break;
-    case 3: integral_block<1, 0>(t, s, si, sj, cn, S, D, Q, H0); break;
-    case 4: integral_block<1, 1>(t, s, si, sj, cn, S, D, Q, H0); break;
-    case 5: integral_block<1, 2>(t, s, si, sj, cn, S, D, Q, H0); break;
-    case 6: integral_block<2, 0>(t, s, si, sj, cn, S, D, Q, H0); break;
-    case 7: integral_block<2, 1>(t, s, si, sj, cn, S, D, Q, H0); break;
-    default: integral_block<2, 2>(t, s, si, sj, cn, S, D, Q, H0); break;
+  if constexpr (!HAS_D) {
+    switch (key) {
+      case 0: integral_block<0, 0>(t, s, si, sj, cn, S, D, Q, H0); break;
+      case 1: integral_block<0, 1>(t, s, si, sj, cn, S, D, Q, H0); break;
+      case 3: integral_block<1, 0>(t, s, si, sj, cn, S, D, Q, H0); break;
+      default: integral_block<1, 1>(t, s, si, sj, cn, S, D, Q, H0); break;
+    }
+  } else {
+    switch (key) {
+      case 0: integral_block<0, 0>(t, s, si, sj, cn, S, D, Q, H0); break;
+      case 1: integral_block<0, 1>(t, s, si, sj, cn, S, D, Q, H0); break;
+      case 2: integral_block<0, 2>(t, s, si, sj, cn, S, D, Q, H0); break;
+      case 3: integral_block<1, 0>(t, s, si, sj, cn, S, D, Q, H0); break;
+      case 4: integral_block<1, 1>(t, s, si, sj, cn, S, D, Q, H0); break;
+      case 5: integral_block<1, 2>(t, s, si, sj, cn, S, D, Q, H0); break;
+      case 6: integral_block<2, 0>(t, s, si, sj, cn, S, D, Q, H0); break;
+      case 7: integral_block<2, 1>(t, s, si, sj, cn, S, D, Q, H0); break;
+      default: integral_block<2, 2>(t, s, si, sj, cn, S, D, Q, H0); break;
+    }
   }
 }
 
