@@ -144,6 +144,16 @@ def test_large_system(ctx):
     compare(b, run(ctx, b), "long1")
 
 
+def test_system_beyond_lds_staging(ctx):
+    # 272 atoms: pair sums from the global staging slice, eigensolver beyond
+    # its register-resident sizes; a small molecule rides along in the batch.
+    b = workloads.make("long64", seed=3, batch=1, kmin=90, kmax=90)
+    w = workloads.make("water8", seed=4, batch=1)
+    both = workloads.from_systems([b.system(0), w.system(0)], b.names + w.names)
+    print("case long272+water", both.names, flush=True)
+    compare(both, run(ctx, both), "long272+water")
+
+
 def test_failure_isolation_not_converged(ctx):
     b = workloads.make("water8", seed=9, batch=3)
     print("case max_iter=2", flush=True)

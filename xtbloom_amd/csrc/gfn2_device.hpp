@@ -49,6 +49,7 @@ struct Topo {
   const double* at_d4r4r2;
   const int* at_d4ref;      // first global reference
   const int* at_d4nref;
+  const int* perm;          // [N] atoms of each system grouped by D4 element (stable)
   // per shell (global index)
   const int* sh_atom;       // global atom
   const int* sh_l;
@@ -267,6 +268,28 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int i = 0; i < nw; ++i) s += red[i];
   __syncthreads();
   return s;
+}
+
+// K sums at once; red holds at least K * (blockDim/64) doubles.  Three barriers
+// for all K values instead of three per value.
+template <int K>
+__device__ __forceinline__ void block_sum_n(double (&v)[K], double* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[k * nw + wid] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < nw; ++i) s += red[k * nw + i];
+    v[k] = s;
+  }
+  __syncthreads();
 }
 
 }  // namespace xtb

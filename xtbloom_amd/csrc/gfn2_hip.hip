@@ -66,7 +66,7 @@ struct HostTopo {
   std::vector<int> atom_off, shell_off, ao_off, vec_off, pc_off, order;
   std::vector<int64_t> mat_off, gam_off, apair_off, sp_off;
   std::vector<double> nalpha, nbeta, charges;
-  std::vector<int> Z, atom_sys, atom_sh0, at_d4ref, at_d4nref;
+  std::vector<int> Z, atom_sys, atom_sh0, at_d4ref, at_d4nref, perm;
   std::vector<double> xyz, at_cnrad, at_h0rad, at_en, at_dk, at_qk, at_mprad, at_mpvcn,
       at_sqrt_arep, at_zeff, at_d4rcov, at_d4en, at_d4zeff, at_d4hard, at_d4r4r2;
   std::vector<int> sh_atom, sh_l, sh_ao, sh_prim, sh_nprim;
@@ -76,6 +76,7 @@ struct HostTopo {
   std::vector<double> pc_xyz, pc_q, pc_gam;
   std::vector<int> setup_status;
   int max_ao = 0;
+  int max_atoms = 0;
 };
 
 int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
@@ -217,6 +218,15 @@ int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
     T.pc_xyz.assign(hb.pc_xyz, hb.pc_xyz + 3 * (size_t)T.NPC);
     T.pc_q.assign(hb.pc_q, hb.pc_q + T.NPC);
     T.pc_gam.assign(hb.pc_gamma, hb.pc_gamma + T.NPC);
+  }
+  // atoms of each system grouped by D4 element, stable inside a group
+  T.perm.resize(T.N);
+  for (int s = 0; s < B; ++s) {
+    const int a0 = T.atom_off[s], a1 = T.atom_off[s + 1];
+    T.max_atoms = std::max(T.max_atoms, a1 - a0);
+    std::iota(T.perm.begin() + a0, T.perm.begin() + a1, a0);
+    std::stable_sort(T.perm.begin() + a0, T.perm.begin() + a1,
+                     [&](int a, int b) { return T.at_d4ref[a] < T.at_d4ref[b]; });
   }
   // size-sorted visiting order (largest AO count first)
   T.order.resize(B);
@@ -499,6 +509,9 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(3 * (size_t)NPC); addd(NPC);
   addd(B); addd(B);            // erep, eatm
   addd(2 * (size_t)NAP);       // ATM pair C6 staging
+  addi(N);                     // element-grouped atom order
+  addd((size_t)kPairStageDoubles * N); addi((size_t)kPairStageInts * N);  // pair staging (large systems)
+  addd((size_t)mem * mem * B);  // Broyden overlaps
   if (bytes > dev->arena_bytes) {
     (void)hipFree(dev->arena);
     dev->arena = nullptr;
@@ -694,6 +707,22 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   double* d_erep = cv.take<double>(B);
   double* d_eatm = cv.take<double>(B);
   double* d_c6pair = cv.take<double>(2 * (size_t)NAP);
+  int* d_perm = cv.take<int>(N);
+  double* d_gstage = cv.take<double>((size_t)kPairStageDoubles * N);
+  int* d_gstagei = cv.take<int>((size_t)kPairStageInts * N);
+  mx.ovl = cv.take<double>((size_t)mem * mem * B);
+  HIP_TRY(up(d_perm, T.perm));
+  t.perm = d_perm;
+  const bool big_systems = T.max_atoms > kPairStageAtoms;
+  auto launch_potentials = [&](const unsigned char* mask) {
+    hipLaunchKernelGGL(k_potentials<true>, dim3(B), dim3(256), 0, sm, t, d_order, mask, mx.x,
+                       d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, d_dw4, d_vat, d_vsh, d_vd, d_vqq,
+                       d_gstage, d_gstagei);
+    if (big_systems)
+      hipLaunchKernelGGL(k_potentials<false>, dim3(B), dim3(256), 0, sm, t, d_order, mask, mx.x,
+                         d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, d_dw4, d_vat, d_vsh, d_vd,
+                         d_vqq, d_gstage, d_gstagei);
+  };
 
   HIP_TRY(hipMemsetAsync(d_st, 0, sizeof(SysState) * B, sm));
   HIP_TRY(hipMemsetAsync(mx.df, 0, sizeof(double) * (size_t)mem * NV, sm));
@@ -775,8 +804,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     if (ks) work(h_active, a2, a3);
     HIP_TRY(hipMemsetAsync(d_count, 0, sizeof(int), sm));
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL(k_potentials, dim3(B), dim3(256), 0, sm, t, d_order, d_active, mx.x,
-                       d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, d_dw4, d_vat, d_vsh, d_vd, d_vqq);
+    launch_potentials(d_active);
     HIP_TRY(probe_end(kKPotentials, 0.0, 0.0));
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_fock, dim3((unsigned)chunks.size()), dim3(256), 0, sm, t, d_chunks,
@@ -824,8 +852,13 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
                        d_mpart, d_colsum, mx.raw, d_st);
     HIP_TRY(probe_end(kKMulliken, 0.0, 88.0 * a2));  // reads P, S, D(3), Q(6), H0
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL(k_energy_mix, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_cnd4,
-                       d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_count);
+    hipLaunchKernelGGL(k_energy_mix<true>, dim3(B), dim3(256), 0, sm, t, d_order, d_active,
+                       d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_count, d_gstage,
+                       d_gstagei);
+    if (big_systems)
+      hipLaunchKernelGGL(k_energy_mix<false>, dim3(B), dim3(256), 0, sm, t, d_order, d_active,
+                         d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_count, d_gstage,
+                         d_gstagei);
     HIP_TRY(probe_end(kKEnergyMix, 0.0, 0.0));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, sm));
@@ -856,8 +889,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   }
   // potentials at the converged raw multipoles (reference gfn2_cpu_execution.cpp:1358-1441)
   HIP_TRY(probe_begin());
-  hipLaunchKernelGGL(k_potentials, dim3(B), dim3(256), 0, sm, t, d_order, d_valid, mx.x, d_cnd4,
-                     d_mrad, d_gam, d_vpc, d_qat, d_w4, d_dw4, d_vat, d_vsh, d_vd, d_vqq);
+  launch_potentials(d_valid);
   HIP_TRY(probe_end(kKPotentials, 0.0, 0.0));
   if (opt.forces) {
     // W = C diag(f eps) C^T

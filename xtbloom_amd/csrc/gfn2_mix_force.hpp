@@ -9,16 +9,20 @@ namespace xtb {
 // SCC energy from the raw multipoles (reference scc_driver.cpp:2105-2286),
 // modified Broyden step (reference common/scc_mixer.cpp:559-745) and the
 // convergence gate rms < qtol and |dF| < etol (scc_driver.cpp:869-893).
-// One 256-thread block per active system.
+// One 256-thread block per active system; the pair energies come from the
+// shared pair sums (STAGED as for k_potentials).
 // ---------------------------------------------------------------------------
+template <bool STAGED>
 __global__ __launch_bounds__(256) void k_energy_mix(Topo t, const int* order, unsigned char* active,
                                                     const double* cnd4, const double* mrad,
                                                     const double* gam, const double* vpc,
                                                     double* qat_raw, double* w4, MixerArrays mx,
-                                                    SysState* st, int* active_count) {
+                                                    SysState* st, int* active_count,
+                                                    double* gstage, int* gstagei) {
   const int s = order[blockIdx.x];
   if (!active[s]) return;
   const int a0 = t.atom_off[s], a1 = t.atom_off[s + 1], N = a1 - a0;
+  if (STAGED != (N <= kPairStageAtoms)) return;
   const int s0 = t.shell_off[s], ns = t.shell_off[s + 1] - s0;
   const int vo = t.vec_off[s], dim = t.vec_off[s + 1] - vo;
   const double* raw = mx.raw + vo;
@@ -27,71 +31,67 @@ __global__ __launch_bounds__(256) void k_energy_mix(Topo t, const int* order, un
   const double* quad = raw + ns + 3 * N;
   const int tid = threadIdx.x, nt = blockDim.x;
   __shared__ double red[16];
-  __shared__ double beta[64 * 64];
+  // the atom staging area while the pair sums run, the Broyden beta matrix after
+  __shared__ double lbuf[STAGED ? kPairStageDoubles * kPairStageAtoms : 64 * 64];
+  __shared__ int libuf[STAGED ? kPairStageInts * kPairStageAtoms : 1];
   __shared__ double coef[64];
   __shared__ int slots[64];
   __shared__ double sh_scalar[4];
-  for (int a = a0 + tid; a < a1; a += nt) {
+  double* buf = STAGED ? lbuf : gstage + (size_t)kPairStageDoubles * a0;
+  int* ibuf = STAGED ? libuf : gstagei + (size_t)kPairStageInts * a0;
+  const PairStage S = pair_stage(buf, ibuf, N);
+  for (int p = tid; p < N; p += nt) {
+    const int a = t.perm[a0 + p];
     double q = 0.0;
     for (int sh = t.atom_sh0[a]; sh < t.atom_sh0[a + 1]; ++sh) q += qsh[sh - s0];
     qat_raw[a] = q;
-    d4_weights(t, a, cnd4[a], q, w4 + kD4MaxRef * a, nullptr, nullptr);
+    pair_stage_atom(t, S, N, p, a, a - a0, dip, quad, q, cnd4, mrad, w4, nullptr);
   }
   __syncthreads();
-  // ES2 / ES3 / point charges
-  const double* G = gam + t.gam_off[s];
-  double e2 = 0.0, e3 = 0.0, epc = 0.0;
+  // ES2 / ES3 / point charges (Gamma symmetric: coalesced column reads)
+  const double* Gm = gam + t.gam_off[s];
+  double e[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // es2 es3 pc aes2 d4
   for (int i = tid; i < ns; i += nt) {
     double v = 0.0;
-    for (int j = 0; j < ns; ++j) v += G[i * ns + j] * qsh[j];
+    for (int j = 0; j < ns; ++j) v += Gm[j * ns + i] * qsh[j];
     const int sh = s0 + i;
-    e2 += 0.5 * qsh[i] * v;
-    e3 += t.sh_gam3[sh] * qsh[i] * qsh[i] * qsh[i] / 3.0;
-    epc = fma(qsh[i], vpc[sh], epc);
+    e[0] += 0.5 * qsh[i] * v;
+    e[1] += t.sh_gam3[sh] * qsh[i] * qsh[i] * qsh[i] / 3.0;
+    e[2] = fma(qsh[i], vpc[sh], e[2]);
   }
-  // AES2 and D4 two-body, pairs (a < b) owned by a
+  // AES2 and D4 two-body: every ordered pair once, halved.  The AES2 pair
+  // energy equals q_a V_q + d_a.V_d + Q_a.V_Q of atom a in the field of b,
+  // symmetric in (a, b).
   const double qscale[6] = {1.0, 2.0, 1.0, 2.0, 2.0, 1.0};
-  double eaes = 0.0, ed4 = 0.0;
-  for (int a = a0 + tid; a < a1; a += nt) {
-    const int la = a - a0;
-    const double* da = dip + 3 * la;
-    const double* Qa = quad + 6 * la;
-    double d2 = 0.0, q2 = 0.0;
-    for (int c = 0; c < 3; ++c) d2 += da[c] * da[c];
-    for (int c = 0; c < 6; ++c) q2 += qscale[c] * Qa[c] * Qa[c];
-    eaes += t.at_dk[a] * d2 + t.at_qk[a] * q2;
-    const double* ra = t.xyz + 3 * a;
-    for (int b = a + 1; b < a1; ++b) {
-      const int lb = b - a0;
-      const double* rb = t.xyz + 3 * b;
-      const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
-      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-      double k3, k5;
-      aes2_kernels(sqrt(r2), 0.5 * (mrad[a] + mrad[b]), k3, k5);
-      const double* db = dip + 3 * lb;
-      const double* Qb = quad + 6 * lb;
-      double pa = 0, pb = 0, dd = 0, cd = 0;
-      for (int c = 0; c < 3; ++c) {
-        pa += r[c] * da[c];
-        pb += r[c] * db[c];
-        dd += da[c] * db[c];
-        cd += r[c] * (qat_raw[a] * db[c] - qat_raw[b] * da[c]);
+  const int G = pair_lanes(N, nt), g = tid & (G - 1), per = nt / G;
+  const int L = (N + G - 1) / G;
+  for (int base = 0; base < N; base += per) {
+    const int pa = base + tid / G;
+    if (pa < N) {
+      PairSums o;
+      pair_sums(t, S, N, pa, min(N, g * L), min(N, (g + 1) * L), o);
+      pair_sums_reduce(o, G);
+      if (g == 0) {
+        const int a = t.perm[a0 + pa];
+        double d2 = 0.0, q2 = 0.0, pair = S.q[pa] * o.pq, ew = 0.0;
+        for (int c = 0; c < 3; ++c) {
+          const double dc = S.dp[c * N + pa];
+          d2 += dc * dc;
+          pair += dc * o.pd[c];
+        }
+        for (int c = 0; c < 6; ++c) {
+          const double qc = S.qp[c * N + pa];
+          q2 += qscale[c] * qc * qc;
+          pair += qc * o.pQ[c];
+        }
+        for (int r = 0; r < kD4MaxRef; ++r) ew += S.w[r * N + pa] * o.z[r];
+        e[3] += t.at_dk[a] * d2 + t.at_qk[a] * q2 + 0.5 * pair;
+        e[4] -= 0.5 * ew;
       }
-      const double sq[6] = {r[0] * r[0] * k5, 2.0 * r[0] * r[1] * k5, r[1] * r[1] * k5,
-                            2.0 * r[0] * r[2] * k5, 2.0 * r[1] * r[2] * k5, r[2] * r[2] * k5};
-      double sqa = 0, sqb = 0;
-      for (int c = 0; c < 6; ++c) { sqa += sq[c] * Qa[c]; sqb += sq[c] * Qb[c]; }
-      eaes += k3 * cd + k5 * (r2 * dd - 3.0 * pa * pb) + qat_raw[a] * sqb + qat_raw[b] * sqa;
-      double damp, ddamp;
-      d4_damping(t, a, b, r2, damp, ddamp);
-      if (damp != 0.0) ed4 -= d4_contract(t, a, b, w4 + kD4MaxRef * a, w4 + kD4MaxRef * b) * damp;
     }
   }
-  e2 = block_sum(e2, red);
-  e3 = block_sum(e3, red);
-  epc = block_sum(epc, red);
-  eaes = block_sum(eaes, red);
-  ed4 = block_sum(ed4, red);
+  block_sum_n<5>(e, red);
+  const double e2 = e[0], e3 = e[1], epc = e[2], eaes = e[3], ed4 = e[4];
   const double eint = st[s].e_core + e2 + e3 + eaes + ed4 + epc;
   const double F = fma(-mx.kt, st[s].entropy, eint);
 
@@ -104,13 +104,17 @@ __global__ __launch_bounds__(256) void k_energy_mix(Topo t, const int* order, un
   double* uh = mx.u + (int64_t)mem * vo;
   double* om = mx.omega + (int64_t)mem * s;
   const int old_it = st[s].iterations;
-  double rs = 0.0;
+  double* beta = lbuf;
+  // |res|^2 and |res - res_prev|^2 in one pass
+  double rr[2] = {0.0, 0.0};
   for (int i = tid; i < dim; i += nt) {
     const double res = raw[i] - cur[i];
-    rs += res * res;
+    const double v = res - prev_res[i];
+    rr[0] += res * res;
+    rr[1] += v * v;
   }
-  rs = block_sum(rs, red);
-  const double rnorm = sqrt(rs), rms = rnorm / sqrt((double)dim);
+  block_sum_n<2>(rr, red);
+  const double rnorm = sqrt(rr[0]), rms = rnorm / sqrt((double)dim);
   double* xnew = mx.x + vo;
   if (old_it == 0) {
     for (int i = tid; i < dim; i += nt) {
@@ -124,14 +128,7 @@ __global__ __launch_bounds__(256) void k_energy_mix(Topo t, const int* order, un
     const int slot_new = (old_it - 1) % mem;
     double* dfn = dfh + (int64_t)slot_new * dim;
     double* un = uh + (int64_t)slot_new * dim;
-    double d2 = 0.0;
-    for (int i = tid; i < dim; i += nt) {
-      const double res = raw[i] - cur[i];
-      const double v = res - prev_res[i];
-      d2 += v * v;
-    }
-    d2 = block_sum(d2, red);
-    const double inv = 1.0 / fmax(sqrt(d2), 2.220446049250313e-16);
+    const double inv = 1.0 / fmax(sqrt(rr[1]), 2.220446049250313e-16);
     for (int i = tid; i < dim; i += nt) {
       const double res = raw[i] - cur[i];
       const double v = (res - prev_res[i]) * inv;
@@ -144,27 +141,42 @@ __global__ __launch_bounds__(256) void k_energy_mix(Topo t, const int* order, un
     if (tid < hc) slots[tid] = (first + tid - 1) % mem;
     if (tid == 0) om[slot_new] = omega;
     __syncthreads();
-    // coefficients and beta matrix
-    for (int r = 0; r < hc; ++r) {
-      const double* dr = dfh + (int64_t)slots[r] * dim;
-      double dot = 0.0;
-      for (int i = tid; i < dim; i += nt) dot += dr[i] * (raw[i] - cur[i]);
-      dot = block_sum(dot, red);
-      if (tid == 0) coef[r] = om[slots[r]] * dot;
-      for (int c = 0; c <= r; ++c) {
-        const double* dc = dfh + (int64_t)slots[c] * dim;
-        double ov = 0.0;
-        for (int i = tid; i < dim; i += nt) ov += dr[i] * dc[i];
-        ov = block_sum(ov, red);
-        if (tid == 0) {
-          double val = om[slots[r]] * om[slots[c]] * ov;
-          if (r == c) val += 0.01 * 0.01;
-          beta[r * hc + c] = val;
-          beta[c * hc + r] = val;
+    // 2*hc dot products, one wavefront each: coef_r = omega_r <df_r|res> and
+    // the new slot's overlaps with every slot of the window.  Overlaps among
+    // older slots are kept from earlier rounds (a slot's df never changes).
+    double* ov = mx.ovl + (int64_t)mem * mem * s;
+    {
+      const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+      for (int task = wid; task < 2 * hc; task += nw) {
+        const int r = task < hc ? task : task - hc;
+        const double* dr = dfh + (int64_t)slots[r] * dim;
+        double dot = 0.0;
+        if (task < hc) {
+          for (int i = lane; i < dim; i += 64) dot = fma(dr[i], raw[i] - cur[i], dot);
+        } else {
+          for (int i = lane; i < dim; i += 64) dot = fma(dr[i], dfn[i], dot);
+        }
+        dot = wave_sum(dot);
+        if (lane == 0) {
+          if (task < hc) {
+            coef[r] = om[slots[r]] * dot;
+          } else {
+            ov[slot_new * mem + slots[r]] = dot;
+            ov[slots[r] * mem + slot_new] = dot;
+          }
         }
       }
     }
     __syncthreads();
+    if (tid == 0) {
+      for (int r = 0; r < hc; ++r)
+        for (int c = 0; c <= r; ++c) {
+          double val = om[slots[r]] * om[slots[c]] * ov[slots[r] * mem + slots[c]];
+          if (r == c) val += 0.01 * 0.01;
+          beta[r * hc + c] = val;
+          beta[c * hc + r] = val;
+        }
+    }
     if (tid == 0) {
       int ok = 1;
       for (int r = 0; r < hc && ok; ++r)

@@ -18,6 +18,7 @@ struct MixerArrays {
   double* df;        // [mem * dim]
   double* u;         // [mem * dim]
   double* omega;     // [B * mem]
+  double* ovl;       // [B * mem * mem]  <df_r|df_c> by slot pair
   int mem;
   double damping;
   double qtol, etol, kt;
@@ -233,77 +234,239 @@ __global__ __launch_bounds__(128) void k_integrals(Topo t, SysList list, int64_t
 }
 
 // ---------------------------------------------------------------------------
+// Atom-pair sums shared by the potentials and the SCC energy: the AES2
+// multipole potentials of every atom from all other atoms (reference
+// aes2.cpp:451-545) and the D4 two-body contraction (reference
+// d4.cpp:896-1032).
+//
+// One block per system.  The atoms' data are staged as a structure of arrays
+// in the system's element-grouped order (Topo::perm): in LDS for systems of at
+// most kPairStageAtoms atoms, in a global scratch slice otherwise.  G lanes
+// share one atom (G = the largest power of two with G*N <= blockDim, at most
+// 64); each takes a contiguous run of partners and the G partial sums are
+// combined with xor shuffles inside the wavefront.
+//
+// D4: sum_b damp_ab w_a^T C(a,b) w_b = w_a^T sum_j C(e_a, j) y_j with
+// y_j = sum over partners b of element j of damp_ab w_b.  Partners are visited
+// grouped by element, so the 7x7 reference block of C6 is applied once per
+// element run instead of once per pair.
+// ---------------------------------------------------------------------------
+constexpr int kPairStageAtoms = 256;
+constexpr int kPairStageDoubles = 22;  // doubles per staged atom
+constexpr int kPairStageInts = 2;      // ints per staged atom
+
+struct PairStage {
+  double* x;    // [3][N] positions
+  double* mr;   // [N] AES2 multipole radius
+  double* q;    // [N] atomic charge
+  double* dp;   // [3][N] dipole
+  double* qp;   // [6][N] traceless quadrupole (xx xy yy xz yz zz)
+  double* w;    // [7][N] D4 reference weights
+  double* r42;  // [N] D4 <r4>/<r2>
+  int* ref;     // [N] first D4 reference of the element
+  int* nref;    // [N] number of D4 references
+};
+
+__device__ __forceinline__ PairStage pair_stage(double* buf, int* ibuf, int N) {
+  PairStage p;
+  p.x = buf;
+  p.mr = buf + 3 * N;
+  p.q = buf + 4 * N;
+  p.dp = buf + 5 * N;
+  p.qp = buf + 8 * N;
+  p.w = buf + 14 * N;
+  p.r42 = buf + 21 * N;
+  p.ref = ibuf;
+  p.nref = ibuf + N;
+  return p;
+}
+
+// Stage atom p (element-grouped position) of the system: geometry, radius,
+// multipoles from the state vector, D4 weights at charge q (written to w4 and
+// optionally dw4 = dw/dq as well), and the element's D4 data.
+// dip / quad: the system's dipole and quadrupole blocks of the state vector;
+// la: the atom's index inside the system.
+__device__ __forceinline__ void pair_stage_atom(const Topo& t, const PairStage& S, int N, int p,
+                                                int a, int la, const double* dip,
+                                                const double* quad, double q, const double* cnd4,
+                                                const double* mrad, double* w4, double* dw4) {
+  for (int c = 0; c < 3; ++c) {
+    S.x[c * N + p] = t.xyz[3 * a + c];
+    S.dp[c * N + p] = dip[3 * la + c];
+  }
+  for (int c = 0; c < 6; ++c) S.qp[c * N + p] = quad[6 * la + c];
+  S.mr[p] = mrad[a];
+  S.q[p] = q;
+  S.r42[p] = t.at_d4r4r2[a];
+  S.ref[p] = t.at_d4ref[a];
+  S.nref[p] = t.at_d4nref[a];
+  d4_weights(t, a, cnd4[a], q, w4 + kD4MaxRef * a, nullptr,
+             dw4 ? dw4 + kD4MaxRef * a : nullptr);
+  for (int r = 0; r < kD4MaxRef; ++r) S.w[r * N + p] = w4[kD4MaxRef * a + r];
+}
+
+struct PairSums {
+  double pq;      // potential on the atom's charge
+  double pd[3];   // potential on its dipole
+  double pQ[6];   // potential on its quadrupole (off-diagonals doubled)
+  double z[7];    // sum_j C(e_a, j) y_j
+};
+
+// Partners pb0 <= pb < pb1 (element-grouped positions) of atom pa.
+__device__ __forceinline__ void pair_sums(const Topo& t, const PairStage& S, int N, int pa,
+                                          int pb0, int pb1, PairSums& o) {
+  const double xa = S.x[pa], ya = S.x[N + pa], za = S.x[2 * N + pa];
+  const double mra = S.mr[pa], r42a = S.r42[pa];
+  const int oa = S.ref[pa], na = S.nref[pa];
+  o.pq = 0.0;
+  for (int c = 0; c < 3; ++c) o.pd[c] = 0.0;
+  for (int c = 0; c < 6; ++c) o.pQ[c] = 0.0;
+  for (int r = 0; r < 7; ++r) o.z[r] = 0.0;
+  double y[7];
+  for (int k = 0; k < 7; ++k) y[k] = 0.0;
+  int cur = -1, curn = 0;
+  auto flush = [&]() __attribute__((always_inline)) {
+    if (cur < 0) return;
+#pragma unroll
+    for (int r = 0; r < 7; ++r) {
+      if (r < na) {
+        const double* row = t.d4_c6 + (size_t)(oa + r) * 262 + cur;
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+          if (k < curn) acc = fma(row[k], y[k], acc);
+        o.z[r] += acc;
+      }
+    }
+  };
+  for (int pb = pb0; pb < pb1; ++pb) {
+    const int ob = S.ref[pb];
+    if (ob != cur) {
+      flush();
+      cur = ob;
+      curn = S.nref[pb];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) y[k] = 0.0;
+    }
+    const double r[3] = {xa - S.x[pb], ya - S.x[N + pb], za - S.x[2 * N + pb]};
+    const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    const bool self = pb == pa;
+    // AES2: the self "pair" gets zero kernels (r = 0), which keeps the loop
+    // body branch-free.
+    double k3, k5;
+    aes2_kernels(self ? 1.0 : sqrt(r2), 0.5 * (mra + S.mr[pb]), k3, k5);
+    k3 = self ? 0.0 : k3;
+    k5 = self ? 0.0 : k5;
+    const double db[3] = {S.dp[pb], S.dp[N + pb], S.dp[2 * N + pb]};
+    const double sq[6] = {r[0] * r[0] * k5, 2.0 * r[0] * r[1] * k5, r[1] * r[1] * k5,
+                          2.0 * r[0] * r[2] * k5, 2.0 * r[1] * r[2] * k5, r[2] * r[2] * k5};
+    const double rdb = r[0] * db[0] + r[1] * db[1] + r[2] * db[2];
+    double sqQ = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sqQ = fma(sq[c], S.qp[c * N + pb], sqQ);
+    const double qb = S.q[pb];
+    o.pq += k3 * rdb + sqQ;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o.pd[c] += -qb * k3 * r[c] + k5 * (r2 * db[c] - 3.0 * r[c] * rdb);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o.pQ[c] += qb * sq[c];
+    // D4 rational damping (reference d4.cpp:774-786)
+    double damp = 0.0;
+    if (!self && r2 <= 2500.0) {
+      const double rrij = 3.0 * r42a * S.r42[pb];
+      const double r0 = 0.52 * sqrt(rrij) + 5.0;
+      const double r4 = r2 * r2, r6 = r4 * r2, r02 = r0 * r0, r04 = r02 * r02, r06 = r04 * r02;
+      damp = 1.0 / (r6 + r06) + 2.7 * rrij / (r4 * r4 + r04 * r04);
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) y[k] = fma(damp, S.w[k * N + pb], y[k]);
+  }
+  flush();
+}
+
+// Sum the PairSums of the G consecutive lanes that share one atom.
+__device__ __forceinline__ void pair_sums_reduce(PairSums& o, int G) {
+  for (int off = G >> 1; off > 0; off >>= 1) {
+    o.pq += __shfl_xor(o.pq, off);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o.pd[c] += __shfl_xor(o.pd[c], off);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o.pQ[c] += __shfl_xor(o.pQ[c], off);
+#pragma unroll
+    for (int r = 0; r < 7; ++r) o.z[r] += __shfl_xor(o.z[r], off);
+  }
+}
+
+__device__ __forceinline__ int pair_lanes(int N, int nt) {
+  int G = 1;
+  while (G < 64 && 2 * G * N <= nt) G *= 2;
+  return G;
+}
+
 // Potentials from the state x: v_sh (with the atom potential folded in), v_D, v_Q
 // (reference scc_driver.cpp:2363-2560, aes2.cpp:451-545, d4.cpp:896-1032).
-// ---------------------------------------------------------------------------
+// STAGED = the instantiation for systems of at most kPairStageAtoms atoms; the
+// other one serves the rest from the global staging slice.
+template <bool STAGED>
 __global__ __launch_bounds__(256) void k_potentials(Topo t, const int* order,
                                                     const unsigned char* active, const double* xs,
                                                     const double* cnd4, const double* mrad,
                                                     const double* gam, const double* vpc,
                                                     double* qat, double* w4, double* dw4,
                                                     double* vat, double* vsh, double* vd,
-                                                    double* vqq) {
+                                                    double* vqq, double* gstage, int* gstagei) {
   const int s = order[blockIdx.x];
   if (active && !active[s]) return;
   const int a0 = t.atom_off[s], a1 = t.atom_off[s + 1], N = a1 - a0;
+  if (STAGED != (N <= kPairStageAtoms)) return;
   const int s0 = t.shell_off[s], s1 = t.shell_off[s + 1], ns = s1 - s0;
   const double* x = xs + t.vec_off[s];
   const double* qsh = x;
-  const double* dip = x + ns;
-  const double* quad = x + ns + 3 * N;
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int a = a0 + tid; a < a1; a += nt) {
+  __shared__ double lbuf[STAGED ? kPairStageDoubles * kPairStageAtoms : 1];
+  __shared__ int libuf[STAGED ? kPairStageInts * kPairStageAtoms : 1];
+  double* buf = STAGED ? lbuf : gstage + (size_t)kPairStageDoubles * a0;
+  int* ibuf = STAGED ? libuf : gstagei + (size_t)kPairStageInts * a0;
+  const PairStage S = pair_stage(buf, ibuf, N);
+  for (int p = tid; p < N; p += nt) {
+    const int a = t.perm[a0 + p];
     double q = 0.0;
     for (int sh = t.atom_sh0[a]; sh < t.atom_sh0[a + 1]; ++sh) q += qsh[sh - s0];
     qat[a] = q;
-    d4_weights(t, a, cnd4[a], q, w4 + kD4MaxRef * a, nullptr, dw4 + kD4MaxRef * a);
+    pair_stage_atom(t, S, N, p, a, a - a0, x + ns, x + ns + 3 * N, q, cnd4, mrad, w4, dw4);
   }
   __syncthreads();
-  const double* G = gam + t.gam_off[s];
+  // ES2 + ES3 + point charges; Gamma is symmetric, so column i is read
+  // (coalesced across the block) in place of row i.
+  const double* Gm = gam + t.gam_off[s];
   for (int i = tid; i < ns; i += nt) {
     double v = 0.0;
-    for (int j = 0; j < ns; ++j) v += G[i * ns + j] * qsh[j];
+    for (int j = 0; j < ns; ++j) v += Gm[j * ns + i] * qsh[j];
     const int sh = s0 + i;
     v += t.sh_gam3[sh] * qsh[i] * qsh[i];
     v += vpc[sh];
     vsh[sh] = v;
   }
   const double qscale[6] = {1.0, 2.0, 1.0, 2.0, 2.0, 1.0};
-  for (int a = a0 + tid; a < a1; a += nt) {
-    const int la = a - a0;
-    const double* ra = t.xyz + 3 * a;
-    double pq = 0.0, pd[3], pQ[6];
-    for (int c = 0; c < 3; ++c) pd[c] = 2.0 * t.at_dk[a] * dip[3 * la + c];
-    for (int c = 0; c < 6; ++c) pQ[c] = 2.0 * t.at_qk[a] * qscale[c] * quad[6 * la + c];
-    double pd4 = 0.0;
-    const double* dwa = dw4 + kD4MaxRef * a;
-    for (int b = a0; b < a1; ++b) {
-      if (b == a) continue;
-      const int lb = b - a0;
-      const double* rb = t.xyz + 3 * b;
-      const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
-      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-      const double rr = sqrt(r2);
-      double k3, k5;
-      aes2_kernels(rr, 0.5 * (mrad[a] + mrad[b]), k3, k5);
-      const double* db = dip + 3 * lb;
-      const double* Qb = quad + 6 * lb;
-      const double sq[6] = {r[0] * r[0] * k5, 2.0 * r[0] * r[1] * k5, r[1] * r[1] * k5,
-                            2.0 * r[0] * r[2] * k5, 2.0 * r[1] * r[2] * k5, r[2] * r[2] * k5};
-      const double rdb = r[0] * db[0] + r[1] * db[1] + r[2] * db[2];
-      double sqQ = 0.0;
-      for (int c = 0; c < 6; ++c) sqQ += sq[c] * Qb[c];
-      const double qb = qat[b];
-      pq += k3 * rdb + sqQ;
-      for (int c = 0; c < 3; ++c) pd[c] += -qb * k3 * r[c] + k5 * (r2 * db[c] - 3.0 * r[c] * rdb);
-      for (int c = 0; c < 6; ++c) pQ[c] += qb * sq[c];
-      double damp, ddamp;
-      d4_damping(t, a, b, r2, damp, ddamp);
-      if (damp != 0.0) pd4 -= d4_contract(t, a, b, dwa, w4 + kD4MaxRef * b) * damp;
+  const int G = pair_lanes(N, nt), g = tid & (G - 1), per = nt / G;
+  const int L = (N + G - 1) / G;
+  for (int base = 0; base < N; base += per) {
+    const int pa = base + tid / G;
+    if (pa < N) {
+      PairSums o;
+      pair_sums(t, S, N, pa, min(N, g * L), min(N, (g + 1) * L), o);
+      pair_sums_reduce(o, G);
+      if (g == 0) {
+        const int a = t.perm[a0 + pa];
+        double pd4 = 0.0;
+        for (int r = 0; r < kD4MaxRef; ++r) pd4 -= dw4[kD4MaxRef * a + r] * o.z[r];
+        vat[a] = o.pq + pd4;
+        for (int c = 0; c < 3; ++c)
+          vd[3 * a + c] = 2.0 * t.at_dk[a] * S.dp[c * N + pa] + o.pd[c];
+        for (int c = 0; c < 6; ++c)
+          vqq[6 * a + c] = 2.0 * t.at_qk[a] * qscale[c] * S.qp[c * N + pa] + o.pQ[c];
+      }
     }
-    vat[a] = pq + pd4;
-    for (int c = 0; c < 3; ++c) vd[3 * a + c] = pd[c];
-    for (int c = 0; c < 6; ++c) vqq[6 * a + c] = pQ[c];
   }
   __syncthreads();
   for (int sh = s0 + tid; sh < s1; sh += nt) vsh[sh] += vat[t.sh_atom[sh]];
