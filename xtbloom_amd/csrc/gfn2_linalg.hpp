@@ -308,11 +308,11 @@ constexpr int kTriPad = 257;  // row stride of the partial-sum buffers (bank spr
 
 template <int NR>
 __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, double* e, double* v,
-                                                  double* pq, double* rowpart, double* colpart,
-                                                  double* red, double* bc) {
+                                                  double* col, double* pq, double* rowpart,
+                                                  double* colpart) {
   constexpr int NC = NR / 2;
   constexpr int NE = tri_slots(NR);
-  const int tid = threadIdx.x, lr = tid >> 5, lc = tid & 31, w = tid >> 6;
+  const int tid = threadIdx.x, lr = tid >> 5, lc = tid & 31, lane = tid & 63;
   double a[NE];
   static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
     constexpr int sl = decltype(S)::value;
@@ -325,48 +325,45 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
     // keep one row tile of loads in flight (address registers would spill the matrix)
     if constexpr (TJ == TI / 2) __builtin_amdgcn_sched_barrier(0);
   });
-  for (int k = 0; k + 2 < n; ++k) {
-    const int Tk = k >> 5, ck = k & 31;
-    if (tid <= k || tid >= n) v[tid] = 0.0;
-    if (lc == ck) {
-      static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
-        constexpr int sl = decltype(S)::value;
-        constexpr int TI = tri_row(sl), TJ = tri_col(sl);
-        if (TJ == Tk) {
-          const int i = 16 * TI + lr;
-          if (i > k && i < n) v[i] = a[sl];
-          if (i == k) bc[1] = 2.0 * a[sl];
-        }
-      });
-    }
-    __syncthreads();
-    const double xi = (tid > k && tid < n) ? v[tid] : 0.0;
-    const double ss = block_sum(xi * xi, red);
-    if (tid == 0) {
-      const double x0 = v[k + 1];
-      const double xn = sqrt(ss);
-      d[k] = bc[1];
-      if (xn == 0.0 || ss - x0 * x0 == 0.0) {
-        e[k] = x0;  // already reduced: no reflection
-        bc[0] = 0.0;
-      } else {
-        const double alpha = x0 >= 0.0 ? -xn : xn;
-        const double v0 = x0 - alpha;
-        const double vv = ss - x0 * x0 + v0 * v0;
-        e[k] = alpha;
-        v[k + 1] = v0;
-        bc[0] = 1.0 / sqrt(vv);
+  // col holds column k of the reduced matrix (its owners publish column k+1
+  // right after the update of step k); rows >= n stay zero.
+  col[tid] = 0.0;
+  __syncthreads();
+  if (lc == 0) {
+    static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+      constexpr int sl = decltype(S)::value;
+      constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+      if constexpr (TJ == 0) {
+        const int i = 16 * TI + lr;
+        if (i < n) col[i] = a[sl];
       }
+    });
+  }
+  __syncthreads();
+  for (int k = 0; k + 2 < n; ++k) {
+    // Reflector of column k, formed redundantly (and identically) by every
+    // wavefront: v = (x - alpha e_{k+1}) / |x - alpha e_{k+1}|.
+    double part = 0.0;
+    for (int i = k + 1 + lane; i < n; i += 64) {
+      const double x = col[i];
+      part = fma(x, x, part);
     }
-    __syncthreads();
+    const double ss = wave_sum(part);
+    const double x0 = col[k + 1];
+    const double xn = sqrt(ss);
+    const bool noref = xn == 0.0 || ss - x0 * x0 == 0.0;  // already reduced
+    const double alpha = x0 >= 0.0 ? -xn : xn;
+    const double v0 = x0 - alpha;
+    const double inv = noref ? 0.0 : 1.0 / sqrt(ss - x0 * x0 + v0 * v0);
+    if (tid == 0) {
+      d[k] = 2.0 * col[k];
+      e[k] = noref ? x0 : alpha;
+    }
     // no reflection (inv = 0) runs the same path with v = 0: p, q and the
     // update vanish and the stored reflector is zero
-    const double inv = bc[0];
-    if (tid > k && tid < n) {
-      const double vi = v[tid] * inv;
-      v[tid] = vi;
-      A[tid + (int64_t)k * n] = vi;
-    }
+    const double vi = (tid > k && tid < n) ? inv * (tid == k + 1 ? v0 : col[tid]) : 0.0;
+    v[tid] = vi;
+    if (tid > k && tid < n) A[tid + (int64_t)k * n] = vi;
     __syncthreads();
     // p = A v (v = 0 outside the trailing block).  Lane partials go to LDS,
     // transposed so that thread i sums its row's 32 + 16 partials without
@@ -396,19 +393,36 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
 #pragma unroll 8
       for (int r = 0; r < 16; ++r) p += colpart[r * kTriPad + tid];
     }
-    const double vt = v[tid];
-    const double K = block_sum(vt * p, red);
-    pq[tid] = p - K * vt;  // q; zero outside the trailing block
+    pq[tid] = p;
     __syncthreads();
-    // A -= 2 (v q^T + q v^T); frozen rows/columns see v = q = 0
+    // K = v.p, again per wavefront; with q = p - K v,
+    // v_i q_j + q_i v_j = v_i p_j + (p_i - 2 K v_i) v_j.
+    part = 0.0;
+    for (int i = k + 1 + lane; i < n; i += 64) part = fma(v[i], pq[i], part);
+    const double K2 = 2.0 * wave_sum(part);
+    double ri = 0.0;
+    // A -= 2 (v q^T + q v^T); frozen rows/columns see v = p = 0
     static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
       constexpr int sl = decltype(S)::value;
       constexpr int TI = tri_row(sl), TJ = tri_col(sl);
       const int i = 16 * TI + lr, j = 32 * TJ + lc;
-      const double u = v[i] * pq[j] + pq[i] * v[j];
+      if constexpr (TJ == 0) ri = fma(-K2, v[i], pq[i]);
+      const double u = fma(v[i], pq[j], ri * v[j]);
       if constexpr (tri_strict(TI, TJ)) a[sl] -= 2.0 * u;
       else a[sl] -= (i > j ? 2.0 : (i == j ? 1.0 : 0.0)) * u;
     });
+    // publish column k+1 for the next step
+    const int Tk1 = (k + 1) >> 5, ck1 = (k + 1) & 31;
+    if (lc == ck1) {
+      static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+        constexpr int sl = decltype(S)::value;
+        constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+        if (TJ == Tk1) {
+          const int i = 16 * TI + lr;
+          if (i < n) col[i] = a[sl];
+        }
+      });
+    }
     __syncthreads();
   }
   static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
@@ -433,14 +447,15 @@ __global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, doubl
   double* e = eall + t.ao_off[s];
   __shared__ double v[kMaxAO];
   __shared__ double pq[kMaxAO];
+  __shared__ double col[512];  // published column of the register path
   __shared__ double rowpart[32 * kTriPad];
   __shared__ double colpart[16 * kTriPad];
   __shared__ double red[16];
   __shared__ double bc[2];
-  if (n <= 64) tridiag_registers<4>(A, n, d, e, v, pq, rowpart, colpart, red, bc);
-  else if (n <= 128) tridiag_registers<8>(A, n, d, e, v, pq, rowpart, colpart, red, bc);
-  else if (n <= 192) tridiag_registers<12>(A, n, d, e, v, pq, rowpart, colpart, red, bc);
-  else if (n <= 256) tridiag_registers<16>(A, n, d, e, v, pq, rowpart, colpart, red, bc);
+  if (n <= 64) tridiag_registers<4>(A, n, d, e, v, col, pq, rowpart, colpart);
+  else if (n <= 128) tridiag_registers<8>(A, n, d, e, v, col, pq, rowpart, colpart);
+  else if (n <= 192) tridiag_registers<12>(A, n, d, e, v, col, pq, rowpart, colpart);
+  else if (n <= 256) tridiag_registers<16>(A, n, d, e, v, col, pq, rowpart, colpart);
   else tridiag_global(A, n, d, e, v, pq, red, bc);
 }
 
