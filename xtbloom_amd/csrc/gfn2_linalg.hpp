@@ -365,27 +365,39 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
     v[tid] = vi;
     if (tid > k && tid < n) A[tid + (int64_t)k * n] = vi;
     __syncthreads();
-    // p = A v (v = 0 outside the trailing block).  Lane partials go to LDS,
-    // transposed so that thread i sums its row's 32 + 16 partials without
-    // cross-lane shuffles: rowpart[lc][i] from x v_j, colpart[lr][j] from x v_i.
-    double pc[NC];
+    // p = A v (v = 0 outside the trailing block), column tile by column tile:
+    // a column tile whose columns are all <= k is frozen (v = p = 0 there) and
+    // skipped, which leaves about (n-k)^2/2 elements of work per step.  Lane
+    // partials go to LDS, transposed so that thread i sums its row's 32 + 16
+    // partials without cross-lane shuffles: rowpart[lc][i] from x v_j,
+    // colpart[lr][j] from x v_i.
+    double vr[NR], pr[NR];
 #pragma unroll
-    for (int TJ = 0; TJ < NC; ++TJ) pc[TJ] = 0.0;
-    double pr = 0.0;
-    static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
-      constexpr int sl = decltype(S)::value;
-      constexpr int TI = tri_row(sl), TJ = tri_col(sl);
-      const double x = a[sl];
-      pr = fma(x, v[32 * TJ + lc], pr);
-      pc[TJ] = fma(x, v[16 * TI + lr], pc[TJ]);
-      if constexpr (TJ == TI / 2) {  // last slot of row tile TI
-        rowpart[lc * kTriPad + 16 * TI + lr] = pr;
-        pr = 0.0;
+    for (int TI = 0; TI < NR; ++TI) {
+      vr[TI] = v[16 * TI + lr];
+      pr[TI] = 0.0;
+    }
+    static_for<0, NC>([&](auto CJ) __attribute__((always_inline)) {
+      constexpr int TJ = decltype(CJ)::value;
+      if (32 * TJ + 31 > k) {
+        const double vj = v[32 * TJ + lc];
+        double pc = 0.0;
+        static_for<2 * TJ, NR>([&](auto CI) __attribute__((always_inline)) {
+          constexpr int TI = decltype(CI)::value;
+          constexpr int sl = tri_slots(TI) + TJ;
+          const double x = a[sl];
+          pr[TI] = fma(x, vj, pr[TI]);
+          pc = fma(x, vr[TI], pc);
+        });
+        colpart[lr * kTriPad + 32 * TJ + lc] = pc;
       }
     });
 #pragma unroll
-    for (int TJ = 0; TJ < NC; ++TJ) colpart[lr * kTriPad + 32 * TJ + lc] = pc[TJ];
+    for (int TI = 0; TI < NR; ++TI) rowpart[lc * kTriPad + 16 * TI + lr] = pr[TI];
     __syncthreads();
+    // only partials of live tiles are read: row i > k gets rowpart from its own
+    // row tile (written every step) and colpart from column tiles TJ with
+    // 32 TJ + 31 >= i > k, all of them live
     double p = 0.0;
     if (tid > k && tid < n) {
 #pragma unroll 8
@@ -400,16 +412,24 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
     part = 0.0;
     for (int i = k + 1 + lane; i < n; i += 64) part = fma(v[i], pq[i], part);
     const double K2 = 2.0 * wave_sum(part);
-    double ri = 0.0;
-    // A -= 2 (v q^T + q v^T); frozen rows/columns see v = p = 0
-    static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
-      constexpr int sl = decltype(S)::value;
-      constexpr int TI = tri_row(sl), TJ = tri_col(sl);
-      const int i = 16 * TI + lr, j = 32 * TJ + lc;
-      if constexpr (TJ == 0) ri = fma(-K2, v[i], pq[i]);
-      const double u = fma(v[i], pq[j], ri * v[j]);
-      if constexpr (tri_strict(TI, TJ)) a[sl] -= 2.0 * u;
-      else a[sl] -= (i > j ? 2.0 : (i == j ? 1.0 : 0.0)) * u;
+    double rr[NR];
+#pragma unroll
+    for (int TI = 0; TI < NR; ++TI) rr[TI] = fma(-K2, vr[TI], pq[16 * TI + lr]);
+    // A -= 2 (v q^T + q v^T) over the live column tiles
+    static_for<0, NC>([&](auto CJ) __attribute__((always_inline)) {
+      constexpr int TJ = decltype(CJ)::value;
+      if (32 * TJ + 31 > k) {
+        const int j = 32 * TJ + lc;
+        const double vj = v[j], pj = pq[j];
+        static_for<2 * TJ, NR>([&](auto CI) __attribute__((always_inline)) {
+          constexpr int TI = decltype(CI)::value;
+          constexpr int sl = tri_slots(TI) + TJ;
+          const int i = 16 * TI + lr;
+          const double u = fma(vr[TI], pj, rr[TI] * vj);
+          if constexpr (tri_strict(TI, TJ)) a[sl] -= 2.0 * u;
+          else a[sl] -= (i > j ? 2.0 : (i == j ? 1.0 : 0.0)) * u;
+        });
+      }
     });
     // publish column k+1 for the next step
     const int Tk1 = (k + 1) >> 5, ck1 = (k + 1) & 31;
