@@ -205,6 +205,14 @@ def test_invalid_input_leaves_outputs(ctx):
     with pytest.raises(interface.XTBloomError) as err:
         interface.compute_arrays(ctx, b.atom_offsets, b.atomic_numbers, bad)
     assert err.value.status == library.STATUS_INVALID_ARGUMENT
+    # coincident atoms in the second system (reference coordination.cpp:236-246)
+    close = b.positions.copy()
+    close[5] = close[3] + np.array([0.0, 3e-7, 0.0])
+    print("case coincident", flush=True)
+    with pytest.raises(interface.XTBloomError) as err:
+        interface.compute_arrays(ctx, b.atom_offsets, b.atomic_numbers, close)
+    assert err.value.status == library.STATUS_INVALID_ARGUMENT
+    assert "coincident" in str(err.value)
 
 
 def test_point_charge_potential_gradient_fd(ctx):

@@ -7,12 +7,14 @@
 // exception crosses the boundary.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
 #include <limits>
 #include <mutex>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -221,15 +223,25 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
   const double* chg = reinterpret_cast<const double*>(st.charges.data());
   for (int64_t i = 0; i < B; ++i)
     if (!std::isfinite(chg[i])) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "molecular charges must be finite");
-  // coincident atoms make CN/ES/repulsion undefined (reference coordination.cpp:236-246)
-  for (int64_t sys = 0; sys < B; ++sys)
-    for (int64_t a = aoff[sys]; a < aoff[sys + 1]; ++a)
-      for (int64_t b = aoff[sys]; b < a; ++b) {
-        const double dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1],
-                     dz = xyz[3 * a + 2] - xyz[3 * b + 2];
-        if (dx * dx + dy * dy + dz * dz < 1e-12)
-          return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "coincident or near-coincident atoms");
-      }
+  // coincident atoms make CN/ES/repulsion undefined (reference coordination.cpp:236-246).
+  // r^2 < 1e-12 needs |dx| < 1e-6: sweep each system's atoms in x order and
+  // compare only neighbours inside that window.
+  {
+    std::vector<int64_t> idx;
+    for (int64_t sys = 0; sys < B; ++sys) {
+      idx.resize(aoff[sys + 1] - aoff[sys]);
+      std::iota(idx.begin(), idx.end(), aoff[sys]);
+      std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return xyz[3 * a] < xyz[3 * b]; });
+      for (size_t i = 0; i < idx.size(); ++i)
+        for (size_t j = i + 1; j < idx.size() && xyz[3 * idx[j]] - xyz[3 * idx[i]] < 1e-6; ++j) {
+          const int64_t a = idx[i], b = idx[j];
+          const double dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1],
+                       dz = xyz[3 * a + 2] - xyz[3 * b + 2];
+          if (dx * dx + dy * dy + dz * dz < 1e-12)
+            return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "coincident or near-coincident atoms");
+        }
+    }
+  }
   if (P > 0) {
     const int64_t* po = reinterpret_cast<const int64_t*>(st.poff.data());
     if (po[0] != 0 || po[B] != P) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "point_charge_offsets must start at 0 and end at total_point_charges");

@@ -55,6 +55,39 @@ bool sto_row(int n, int l, int ng, const double** a, const double** c) {
   return false;
 }
 
+// Normalized primitives of every shell parameter set, built once per process:
+// the exponents are zeta^2 alpha_g and the contraction coefficients carry the
+// primitive normalization (2 alpha/pi)^{3/4} (4 alpha)^{l/2} / sqrt((2l-1)!!).
+struct ShellPrims {
+  int ng = 0;     // 0: no STO-nG row for this parameter set
+  double alpha[6];
+  double coef[6];
+};
+
+const ShellPrims* shell_prims() {
+  static const std::vector<ShellPrims> table = [] {
+    static const double dfac[3] = {1.0, 1.0, 3.0};
+    std::vector<ShellPrims> t(XTBP_NSHELLPAR);
+    for (int p = 0; p < XTBP_NSHELLPAR; ++p) {
+      const int l = xtbp_shell_l[p], ng = xtbp_shell_ngauss[p];
+      const double* ta;
+      const double* tc;
+      if (l > 2 || ng > 6 || !sto_row(xtbp_shell_n[p], l, ng, &ta, &tc)) continue;
+      const double z2 = xtbp_shell_slater[p] * xtbp_shell_slater[p];
+      t[p].ng = ng;
+      for (int g = 0; g < ng; ++g) {
+        const double al = ta[g] * z2;
+        const double norm = std::pow(2.0 / kPi * al, 0.75) *
+                            std::pow(std::sqrt(4.0 * al), (double)l) / std::sqrt(dfac[l]);
+        t[p].alpha[g] = al;
+        t[p].coef[g] = tc[g] * norm;
+      }
+    }
+    return t;
+  }();
+  return table.data();
+}
+
 template <typename T>
 size_t align_bytes(size_t n) {
   return ((n * sizeof(T)) + 255) & ~size_t(255);
@@ -95,7 +128,26 @@ int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
   T.nbeta.assign(B, 0.0);
   T.charges.assign(hb.charges, hb.charges + B);
   T.setup_status.assign(B, 0);
-  static const double dfac[5] = {1.0, 1.0, 3.0, 15.0, 105.0};
+  const ShellPrims* prims = shell_prims();
+  {
+    // upper bounds: 3 shells (s, p, d), 9 AOs and 18 primitives per atom
+    const size_t N = (size_t)hb.atom_offsets[B];
+    for (auto* v : {&T.Z, &T.atom_sys, &T.at_d4ref, &T.at_d4nref}) v->reserve(N);
+    T.atom_sh0.reserve(N + 1);
+    T.xyz.reserve(3 * N);
+    for (auto* v : {&T.at_cnrad, &T.at_h0rad, &T.at_en, &T.at_dk, &T.at_qk, &T.at_mprad,
+                    &T.at_mpvcn, &T.at_sqrt_arep, &T.at_zeff, &T.at_d4rcov, &T.at_d4en,
+                    &T.at_d4zeff, &T.at_d4hard, &T.at_d4r4r2})
+      v->reserve(N);
+    for (auto* v : {&T.sh_atom, &T.sh_l, &T.sh_ao, &T.sh_prim, &T.sh_nprim}) v->reserve(3 * N);
+    for (auto* v : {&T.sh_level, &T.sh_kcn, &T.sh_kpoly, &T.sh_zeta, &T.sh_refocc, &T.sh_eta,
+                    &T.sh_gam3})
+      v->reserve(3 * N);
+    T.ao_sh.reserve(9 * N);
+    T.ao_atom.reserve(9 * N);
+    T.alpha.reserve(18 * N);
+    T.coef.reserve(18 * N);
+  }
   for (int s = 0; s < B; ++s) {
     const int64_t a0 = hb.atom_offsets[s], a1 = hb.atom_offsets[s + 1];
     T.atom_off[s] = (int)a0;
@@ -138,9 +190,8 @@ int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
           return 3;
         }
         seen[l] = 1;
-        const double* ta;
-        const double* tc;
-        if (!sto_row(xtbp_shell_n[p], l, ng, &ta, &tc)) {
+        const ShellPrims& sp = prims[p];
+        if (sp.ng == 0) {
           error = "missing STO-nG table row";
           return 6;
         }
@@ -158,14 +209,8 @@ int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
         T.sh_eta.push_back(xtbp_elem_gam[z - 1] * xtbp_shell_hubbard_scale[p]);
         const double s3[3] = {XTBP_ES3_SCALE_S, XTBP_ES3_SCALE_P, XTBP_ES3_SCALE_D};
         T.sh_gam3.push_back(xtbp_elem_gam3[z - 1] * s3[l]);
-        const double z2 = xtbp_shell_slater[p] * xtbp_shell_slater[p];
-        for (int g = 0; g < ng; ++g) {
-          const double al = ta[g] * z2;
-          const double norm = std::pow(2.0 / kPi * al, 0.75) * std::pow(std::sqrt(4.0 * al), (double)l) /
-                              std::sqrt(dfac[l]);
-          T.alpha.push_back(al);
-          T.coef.push_back(tc[g] * norm);
-        }
+        T.alpha.insert(T.alpha.end(), sp.alpha, sp.alpha + ng);
+        T.coef.insert(T.coef.end(), sp.coef, sp.coef + ng);
         for (int m = 0; m < 2 * l + 1; ++m) {
           T.ao_sh.push_back(gsh);
           T.ao_atom.push_back(a_idx);
@@ -255,6 +300,10 @@ struct Device {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<hipEvent_t> probe_pool;  // per-launch timing events, grow-only
   size_t lds_limit = 65536;            // dynamic LDS granted to the back-transformation
+  // grow-only pinned staging: [0, 256) the per-round active count, then the
+  // call's host-to-device uploads (pinned, so each copy is a plain async DMA)
+  char* pinned = nullptr;
+  size_t pinned_bytes = 0;
 };
 
 Device* device_create(int device_id, void* stream, std::string& error) {
@@ -320,6 +369,7 @@ void device_destroy(Device* d) {
   (void)hipFree(d->d4_refng);
   (void)hipFree(d->kpair);
   (void)hipFree(d->arena);
+  if (d->pinned) (void)hipHostFree(d->pinned);
   for (auto& e : d->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : d->probe_pool) (void)hipEventDestroy(e);
@@ -525,9 +575,17 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   }
   Carver cv{dev->arena};
   hipStream_t sm = dev->stream;
+  // uploads are collected here and issued together from pinned staging right
+  // before the first kernel (the source vectors live until the call returns)
+  struct Upload {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  };
+  std::vector<Upload> uploads;
   auto up = [&](auto* dst, const auto& vec) -> hipError_t {
-    if (vec.empty()) return hipSuccess;
-    return hipMemcpyAsync(dst, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice, sm);
+    if (!vec.empty()) uploads.push_back({dst, vec.data(), vec.size() * sizeof(vec[0])});
+    return hipSuccess;
   };
   Topo t{};
   t.B = B;
@@ -766,6 +824,26 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   double all2, all3;
   work(h_active, all2, all3);
 
+  {
+    size_t need = 256;
+    for (const Upload& u : uploads) need += align_bytes<char>(u.bytes);
+    // copies of an earlier call that failed midway may still read the staging
+    HIP_TRY(hipStreamSynchronize(sm));
+    if (need > dev->pinned_bytes) {
+      if (dev->pinned) (void)hipHostFree(dev->pinned);
+      dev->pinned = nullptr;
+      dev->pinned_bytes = 0;
+      HIP_TRY(hipHostMalloc((void**)&dev->pinned, need + need / 4));
+      dev->pinned_bytes = need + need / 4;
+    }
+    size_t off = 256;
+    for (const Upload& u : uploads) {
+      std::memcpy(dev->pinned + off, u.src, u.bytes);
+      HIP_TRY(hipMemcpyAsync(u.dst, dev->pinned + off, u.bytes, hipMemcpyHostToDevice, sm));
+      off += align_bytes<char>(u.bytes);
+    }
+  }
+  int* h_count = reinterpret_cast<int*>(dev->pinned);
   HIP_TRY(hipEventRecord(dev->ev[0], sm));
   // ---- geometry, integrals, overlap factorization ----
   HIP_TRY(probe_begin());
@@ -797,8 +875,6 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   // ---- SCC rounds ----
   const unsigned nfull = (unsigned)full.size(), nlower = (unsigned)lower.size();
   int rounds = 0;
-  int* h_count = nullptr;
-  HIP_TRY(hipHostMalloc(&h_count, sizeof(int)));
   for (int it = 0; it < opt.max_iterations; ++it) {
     double a2 = 0.0, a3 = 0.0;
     if (ks) work(h_active, a2, a3);
@@ -867,7 +943,6 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     ++rounds;
     if (*h_count == 0) break;
   }
-  (void)hipHostFree(h_count);
   HIP_TRY(hipEventRecord(dev->ev[2], sm));
 
   // ---- stationary energy and forces ----
