@@ -24,7 +24,7 @@ typedef enum xtbloom_amd_kernel {
   XTBLOOM_AMD_KERNEL_FOCK = 4,       /* H = H0 + multipole potential terms */
   XTBLOOM_AMD_KERNEL_REDUCE = 5,     /* H' = X^T H X (two GEMM launches) */
   XTBLOOM_AMD_KERNEL_TRIDIAG = 6,    /* Householder tridiagonalization */
-  XTBLOOM_AMD_KERNEL_FERMI = 7,      /* occupations and entropy */
+  XTBLOOM_AMD_KERNEL_FERMI = 7,      /* occupations (computed inside TRIDIAG_EIGEN: no launches) */
   XTBLOOM_AMD_KERNEL_BACK = 8,       /* C = X Y */
   XTBLOOM_AMD_KERNEL_DENSITY = 9,    /* P = C f C^T */
   XTBLOOM_AMD_KERNEL_MULLIKEN = 10,  /* shell charges and atomic multipoles */
@@ -33,7 +33,7 @@ typedef enum xtbloom_amd_kernel {
   XTBLOOM_AMD_KERNEL_FORCE_PAIRS = 13,     /* derivative integrals x adjoints */
   XTBLOOM_AMD_KERNEL_FORCE_ATOMS = 14,     /* AES2/D4/ES2/CN/repulsion/point-charge gradients */
   XTBLOOM_AMD_KERNEL_FORCE_ATM = 15,       /* D4 three-body energy and gradient */
-  XTBLOOM_AMD_KERNEL_TRIDIAG_EIGEN = 16,   /* bisection + inverse iteration on T */
+  XTBLOOM_AMD_KERNEL_TRIDIAG_EIGEN = 16,   /* eigenvalues of T, occupations, occupied eigenvectors */
   XTBLOOM_AMD_KERNEL_BACKTRANSFORM = 17,   /* eigenvectors of H' = Q Z */
   XTBLOOM_AMD_KERNEL_COUNT = 18
 } xtbloom_amd_kernel_t;

@@ -544,7 +544,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(NS); addd(NS);          // vpc, vsh
   addd(3 * (size_t)N); addd(6 * (size_t)N);
   for (int k = 0; k < 5; ++k) addd(NAO);  // eps fa fb occ fe
-  addd(NAO); addd(NAO); addi(NAO);        // tridiagonal d, e and eigenvalue ranks
+  addd(NAO); addd(NAO);                   // tridiagonal d and e
   addd(B);                                // per-system eigenvalue change of the last round
   addd(NAO);                              // eigenvector normalization factors
   addd((size_t)NAO * 10);      // mulliken column partials
@@ -737,7 +737,6 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   double* d_zscale = cv.take<double>(NAO);
   double* d_td = cv.take<double>(NAO);
   double* d_te = cv.take<double>(NAO);
-  int* d_rank = cv.take<int>(NAO);
   double* d_colsum = cv.take<double>((size_t)NAO * 10);
   MixerArrays mx{};
   mx.x = cv.take<double>(NV);
@@ -821,6 +820,17 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
       n3 += n * n * n;
     }
   };
+  // sum n^2 kocc of a subset (the eigenvector-limited products)
+  std::vector<int> h_kocc(B, 0);
+  auto work_occ = [&](const std::vector<unsigned char>& mask) {
+    double w = 0.0;
+    for (int s = 0; s < B; ++s) {
+      if (!mask[s]) continue;
+      const double n = T.ao_off[s + 1] - T.ao_off[s];
+      w += n * n * h_kocc[s];
+    }
+    return w;
+  };
   double all2, all3;
   work(h_active, all2, all3);
 
@@ -878,6 +888,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   for (int it = 0; it < opt.max_iterations; ++it) {
     double a2 = 0.0, a3 = 0.0;
     if (ks) work(h_active, a2, a3);
+    const std::vector<unsigned char> round_active = ks ? h_active : std::vector<unsigned char>();
+    size_t pr_bt = 0, pr_back = 0, pr_dens = 0;  // probes whose work depends on kocc
     HIP_TRY(hipMemsetAsync(d_count, 0, sizeof(int), sm));
     HIP_TRY(probe_begin());
     launch_potentials(d_active);
@@ -889,38 +901,40 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     // H' = X^T H X
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<true, false, false>), dim3(nfull), dim3(256), 0, sm, t, d_full, dX,
-                       dH, dT, (const double*)nullptr, (const int*)nullptr, d_active);
-    HIP_TRY(probe_end(kKReduce, a3, 0.0));
+                       dH, dT, (const double*)nullptr, (const int*)nullptr, (const int*)nullptr,
+                       d_active);
+    HIP_TRY(probe_end(kKReduce, 2.0 * a3, 0.0));
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<false, false, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower,
-                       dT, dX, dA, (const double*)nullptr, (const int*)nullptr, d_active);
-    HIP_TRY(probe_end(kKReduce, a3, 0.0));
+                       dT, dX, dA, (const double*)nullptr, (const int*)nullptr,
+                       (const int*)nullptr, d_active);
+    HIP_TRY(probe_end(kKReduce, a3, 0.0));  // symmetric: lower tiles only
     // eigenpairs of H' (A is overwritten by the reflectors; T, H, C, P, W serve as scratch)
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_tridiag, dim3(B), dim3(512), 0, sm, t, d_order, dA, d_td, d_te, d_active);
     HIP_TRY(probe_end(kKTridiag, (4.0 / 3.0) * a3, 0.0));
     HIP_TRY(probe_begin());
+    // eigenvalues, occupations (kocc), eigenvectors of the kocc lowest levels
     hipLaunchKernelGGL(k_tridiag_eigen, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_td, d_te,
-                       dZ, EigenScratch{dT, dH, dC, dP, dW}, d_rank, d_eps, d_shift, it,
-                       d_zscale);
+                       dZ, EigenScratch{dT, dH, dC, dP, dW}, d_eps, d_shift, it, d_zscale,
+                       FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt});
     HIP_TRY(probe_end(kKTridiagEigen, 0.0, 0.0));
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_backtransform, dim3((unsigned)back.size()), dim3(kBackThreads), back_lds, sm, t, d_back,
-                       dA, dZ, dY, d_rank, d_zscale, d_active);
-    HIP_TRY(probe_end(kKBacktransform, 2.0 * a3, 0.0));
-    HIP_TRY(probe_begin());
-    hipLaunchKernelGGL(k_fermi, dim3(B), dim3(64), 0, sm, t, d_order, d_active, d_eps, d_fa, d_fb,
-                       d_occ, d_fe, d_st, d_kocc, opt.kt);
-    HIP_TRY(probe_end(kKFermi, 0.0, 0.0));
-    // C = X Y (columns of Y are the eigenvectors), P = C diag(f) C^T
+                       dA, dZ, dY, d_kocc, d_zscale, d_active);
+    HIP_TRY(probe_end(kKBacktransform, 0.0, 0.0));
+    pr_bt = probes.size();
+    // C = X Y (the kocc leading columns), P = C diag(f) C^T
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<false, false, false>), dim3(nfull), dim3(256), 0, sm, t, d_full, dX,
-                       dY, dC, (const double*)nullptr, (const int*)nullptr, d_active);
-    HIP_TRY(probe_end(kKBack, a3, 0.0));
+                       dY, dC, (const double*)nullptr, (const int*)nullptr, d_kocc, d_active);
+    HIP_TRY(probe_end(kKBack, 0.0, 0.0));
+    pr_back = probes.size();
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
-                       dC, dP, d_occ, d_kocc, d_active);
-    HIP_TRY(probe_end(kKDensity, 2.0 * a3, 0.0));
+                       dC, dP, d_occ, d_kocc, (const int*)nullptr, d_active);
+    HIP_TRY(probe_end(kKDensity, 0.0, 0.0));
+    pr_dens = probes.size();
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_mulliken_rows, dim3((unsigned)rowchunks.size()), dim3(256), 0, sm, t,
                        d_rowchunks, d_active, dP, dS, dD, dQ, dH0, d_mpart);
@@ -938,8 +952,19 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(probe_end(kKEnergyMix, 0.0, 0.0));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, sm));
-    if (ks) HIP_TRY(hipMemcpyAsync(h_active.data(), d_active, B, hipMemcpyDeviceToHost, sm));
+    if (ks) {
+      HIP_TRY(hipMemcpyAsync(h_active.data(), d_active, B, hipMemcpyDeviceToHost, sm));
+      HIP_TRY(hipMemcpyAsync(h_kocc.data(), d_kocc, sizeof(int) * B, hipMemcpyDeviceToHost, sm));
+    }
     HIP_TRY(hipStreamSynchronize(sm));
+    if (ks) {
+      // algorithmic flops of the kocc-limited products of this round:
+      // back-transformation 2 n^2 kocc, C = X Y 2 n^2 kocc, P (lower half) n^2 kocc
+      const double w = work_occ(round_active);
+      probes[pr_bt - 1].flops = 2.0 * w;
+      probes[pr_back - 1].flops = 2.0 * w;
+      probes[pr_dens - 1].flops = w;
+    }
     ++rounds;
     if (*h_count == 0) break;
   }
@@ -970,8 +995,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     // W = C diag(f eps) C^T
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
-                       dC, dW, d_fe, d_kocc, d_valid);
-    HIP_TRY(probe_end(kKEnergyWeighted, 2.0 * v3, 0.0));
+                       dC, dW, d_fe, d_kocc, (const int*)nullptr, d_valid);
+    HIP_TRY(probe_end(kKEnergyWeighted, work_occ(h_valid), 0.0));  // lower half, K = kocc
     if (NAP > 0) {
       HIP_TRY(probe_begin());
       for (int c = 0; c < 2; ++c) {

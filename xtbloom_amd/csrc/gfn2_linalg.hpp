@@ -22,15 +22,18 @@ constexpr int kTileK = 16;
 
 // C[sys] = sum_k opA(A)[i][k] * scale[k] * opB(B)[k][j]   (row-major n x n operands)
 //   TA: A is read transposed (A[k][i]); TB: B is read transposed (B[j][k]).
-//   K per system: kdim[sys] if given, else n.
+//   K per system: kdim[sys] if given, else n.  ncol (optional): tiles whose
+//   columns all lie at or past ncol[sys] are skipped.
 //   SYMM: only tiles with ti >= tj are queued; the kernel mirrors them.
 template <bool TA, bool TB, bool SYMM>
 __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, const double* A,
                                               const double* Bm, double* C, const double* scale,
-                                              const int* kdim, const unsigned char* active) {
+                                              const int* kdim, const int* ncol,
+                                              const unsigned char* active) {
   const TileTask task = tasks[blockIdx.x];
   const int s = task.sys;
   if (active && !active[s]) return;
+  if (ncol && task.tj * kTile >= ncol[s]) return;  // columns past ncol are not needed
   const int n = t.ao_off[s + 1] - t.ao_off[s];
   const int64_t mo = t.mat_off[s];
   const double* a = A + mo;
@@ -900,15 +903,93 @@ __device__ __forceinline__ bool twisted_vector(const double* d, const double* e,
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// Fermi occupations of one spin channel (reference eigensolver.cpp:992-1161),
+// FP64 bisection on the shifted frame; one wavefront.
+// ---------------------------------------------------------------------------
+__device__ double fermi_sum_wave(const double* eps, int n, double ref, double mu, double kt,
+                                 bool holes) {
+  double acc = 0.0;
+  for (int i = threadIdx.x & 63; i < n; i += 64) {
+    const double a = ((eps[i] - ref) - mu) / kt;
+    double f;
+    if (holes) f = a >= 0.0 ? 1.0 / (exp(-a) + 1.0) : exp(a) / (1.0 + exp(a));
+    else f = a >= 0.0 ? exp(-a) / (1.0 + exp(-a)) : 1.0 / (exp(a) + 1.0);
+    acc += f;
+  }
+  return wave_sum(acc);
+}
+
+__device__ void fermi_wave(const double* eps, int n, double nel, double kt, double* occ,
+                           double& entropy) {
+  const int lane = threadIdx.x & 63;
+  entropy = 0.0;
+  if (nel == 0.0) {
+    for (int i = lane; i < n; i += 64) occ[i] = 0.0;
+    return;
+  }
+  if (kt == 0.0) {
+    const int full = (int)floor(nel);
+    for (int i = lane; i < n; i += 64) occ[i] = i < full ? 1.0 : (i == full ? nel - full : 0.0);
+    return;
+  }
+  if (nel == (double)n) {
+    for (int i = lane; i < n; i += 64) occ[i] = 1.0;
+    return;
+  }
+  const bool holes = nel > 0.5 * n;
+  const double qt = holes ? n - nel : nel;
+  const double steps = fmax(64.0, -log(qt / n) + 8.0);
+  const double ref = holes ? eps[n - 1] : eps[0];
+  const double lo0 = eps[0] - ref, hi0 = eps[n - 1] - ref;
+  const double margin = steps * kt + 64.0 * 2.220446049250313e-16 * fmax(1.0, fabs(hi0 - lo0));
+  double lo = lo0 - margin, hi = hi0 + margin;
+  const double tol = 64.0 * 2.220446049250313e-16 * qt;
+  for (int it = 0; it < 4096; ++it) {
+    const double mid = lo + 0.5 * (hi - lo);
+    const double q = fermi_sum_wave(eps, n, ref, mid, kt, holes);
+    if (fabs(q - qt) <= tol) { lo = hi = mid; break; }
+    if (mid == lo || mid == hi) break;
+    if ((!holes && q < qt) || (holes && q > qt)) lo = mid; else hi = mid;
+  }
+  const double mu = lo + 0.5 * (hi - lo);
+  double sacc = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const double a = ((eps[i] - ref) - mu) / kt;
+    const double f = a >= 0.0 ? exp(-a) / (1.0 + exp(-a)) : 1.0 / (exp(a) + 1.0);
+    occ[i] = f;
+    if (f > 0.0 && f < 1.0) sacc -= f * log(f) + (1.0 - f) * log(1.0 - f);
+  }
+  entropy = wave_sum(sacc);
+}
+
+// Occupations are negligible below kOccFloor: orbitals past the last one
+// above it add less than 1e-30 of a unit occupation to P and W, far below
+// FP64 resolution, and get no eigenvector.
+constexpr double kOccFloor = 1e-30;
+
+// Fermi outputs of the eigen kernel, per orbital in ascending eigenvalue order.
+struct FermiOut {
+  double* fa;    // alpha occupations
+  double* fb;    // beta occupations
+  double* occ;   // fa + fb
+  double* fe;    // occ * eps
+  SysState* st;  // entropy and kocc
+  int* kocc;     // [B] leading orbitals with occ > kOccFloor
+  double kt;
+};
+
+// Eigenvalues of T (all), then the Fermi occupations, then eigenvectors for the
+// kocc lowest eigenvalues only, written to Z column = ascending rank.
 // From the second SCC round on, wall holds the previous round's ascending
 // eigenvalues on entry; shift[s] carries the largest eigenvalue change of the
 // system's previous round (written here for the next one).
 __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
                                                        const unsigned char* active,
                                                        const double* dall, const double* eall,
-                                                       double* Zall, EigenScratch sc, int* rank_all,
+                                                       double* Zall, EigenScratch sc,
                                                        double* wall, double* shift, int round,
-                                                       double* zscale_all) {
+                                                       double* zscale_all, FermiOut fo) {
   const int s = order[blockIdx.x];
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
@@ -1027,8 +1108,51 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
   }
   __syncthreads();
 
-  // ---- eigenvectors: thread per cluster start ----
+  // ---- ascending rank (stable on ties) and the sorted spectrum ----
+  __shared__ int rnk[kMaxAO];
+  __shared__ int kocc_sh;
+  double* srt = e2;  // e2 is not needed after the eigenvalues
   for (int j = tid; j < n; j += nt) {
+    const double lj = lam[j];
+    int r = 0;
+    for (int i = 0; i < n; ++i) {
+      const double li = lam[i];
+      r += (li < lj) || (li == lj && i < j);
+    }
+    rnk[j] = r;
+    srt[r] = lj;
+    wall[t.ao_off[s] + r] = lj;
+  }
+  __syncthreads();
+  // ---- occupations (first wavefront) ----
+  if (tid < 64) {
+    const int o = t.ao_off[s];
+    double sa, sb;
+    fermi_wave(srt, n, t.nalpha[s], fo.kt, fo.fa + o, sa);
+    fermi_wave(srt, n, t.nbeta[s], fo.kt, fo.fb + o, sb);
+    int last = 0;
+    for (int i = tid; i < n; i += 64) {
+      const double f = fo.fa[o + i] + fo.fb[o + i];
+      fo.occ[o + i] = f;
+      fo.fe[o + i] = f * srt[i];
+      if (f > kOccFloor) last = max(last, i + 1);
+    }
+    for (int off = 32; off > 0; off >>= 1) last = max(last, __shfl_xor(last, off, 64));
+    if (tid == 0) {
+      fo.st[s].entropy = sa + sb;
+      fo.st[s].kocc = last;
+      fo.kocc[s] = last;
+      kocc_sh = last;
+    }
+  }
+  __syncthreads();
+  const int kocc = kocc_sh;
+
+  // ---- eigenvectors: thread per cluster start ----
+  // A cluster is needed when its first (lowest) member is below kocc; every
+  // member of a needed cluster is computed so the cluster stays orthogonal.
+  for (int j = tid; j < n; j += nt) {
+    if (rnk[j] >= kocc) continue;
     int b0 = j, b1 = j + 1;
     while (b0 > 0 && e[b0 - 1] != 0.0) --b0;
     while (b1 < n && e[b1 - 1] != 0.0) ++b1;
@@ -1044,12 +1168,13 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
     const double pertol = 10.0 * kEps * fmax(onenrm, kSafeMin);
     double xprev = 0.0;
     for (int jj = j; jj < g1; ++jj) {
+      const int cj = rnk[jj];  // Z column of eigenvalue jj
       // rows outside the block are zero
-      for (int i = 0; i < b0; ++i) Z[(int64_t)i * n + jj] = 0.0;
-      for (int i = b1; i < n; ++i) Z[(int64_t)i * n + jj] = 0.0;
-      zscale[jj] = 1.0;
+      for (int i = 0; i < b0; ++i) Z[(int64_t)i * n + cj] = 0.0;
+      for (int i = b1; i < n; ++i) Z[(int64_t)i * n + cj] = 0.0;
+      zscale[cj] = 1.0;
       if (bn == 1) {
-        Z[(int64_t)b0 * n + jj] = 1.0;
+        Z[(int64_t)b0 * n + cj] = 1.0;
         continue;
       }
       double x = lam[jj];
@@ -1057,8 +1182,8 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
       xprev = x;
       if (g1 - j == 1) {
         double f;
-        if (twisted_vector(d, e, b0, b1, x, pivmin, n, jj, F0, F1, Z, f)) {
-          zscale[jj] = f;
+        if (twisted_vector(d, e, b0, b1, x, pivmin, n, cj, F0, F1, Z, f)) {
+          zscale[cj] = f;
           continue;
         }
       }
@@ -1067,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
       for (int k = 0; k < bn - 1; ++k) {
         const int gi = b0 + k;
         const double n0 = e[gi], n1 = d[gi + 1] - x, n2 = (k + 2 < bn) ? e[gi + 1] : 0.0;
-        const int64_t at = (int64_t)gi * n + jj;
+        const int64_t at = (int64_t)gi * n + cj;
         if (fabs(c0) >= fabs(n0)) {
           const double m = c0 != 0.0 ? n0 / c0 : 0.0;
           F0[at] = c0; F1[at] = c1; F2[at] = c2; FM[at] = m; FP[at] = 0.0;
@@ -1083,32 +1208,32 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
         c2 = 0.0;
       }
       {
-        const int64_t at = (int64_t)(b1 - 1) * n + jj;
+        const int64_t at = (int64_t)(b1 - 1) * n + cj;
         F0[at] = c0; F1[at] = 0.0; F2[at] = 0.0; FM[at] = 0.0; FP[at] = 0.0;
       }
-      for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] = start_component(jj, i - b0);
+      for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + cj] = start_component(jj, i - b0);
       for (int it = 0; it <= kInverseIterations; ++it) {
         // orthogonalize against the cluster members already finished
         for (int p = j; p < jj; ++p) {
           double dot = 0.0;
-          for (int i = b0; i < b1; ++i) dot += Z[(int64_t)i * n + p] * Z[(int64_t)i * n + jj];
-          for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] -= dot * Z[(int64_t)i * n + p];
+          for (int i = b0; i < b1; ++i) dot += Z[(int64_t)i * n + rnk[p]] * Z[(int64_t)i * n + cj];
+          for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + cj] -= dot * Z[(int64_t)i * n + rnk[p]];
         }
         if (it == kInverseIterations) break;
         // scale to unit max-norm, then solve (T_b - x I) y = r
         double rmax = 0.0;
-        for (int i = b0; i < b1; ++i) rmax = fmax(rmax, fabs(Z[(int64_t)i * n + jj]));
+        for (int i = b0; i < b1; ++i) rmax = fmax(rmax, fabs(Z[(int64_t)i * n + cj]));
         const double rs = rmax > 0.0 ? 1.0 / rmax : 1.0;
-        for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] *= rs;
+        for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + cj] *= rs;
         for (int k = 0; k < bn - 1; ++k) {
-          const int64_t at = (int64_t)(b0 + k) * n + jj, nx = at + n;
+          const int64_t at = (int64_t)(b0 + k) * n + cj, nx = at + n;
           double rk = Z[at], rn = Z[nx];
           if (FP[at] != 0.0) { const double tmp = rk; rk = rn; rn = tmp; }
           Z[at] = rk;
           Z[nx] = rn - FM[at] * rk;
         }
         for (int k = bn - 1; k >= 0; --k) {
-          const int64_t at = (int64_t)(b0 + k) * n + jj;
+          const int64_t at = (int64_t)(b0 + k) * n + cj;
           double r = Z[at];
           if (k + 1 < bn) r -= F1[at] * Z[at + n];
           if (k + 2 < bn) r -= F2[at] * Z[at + 2 * (int64_t)n];
@@ -1118,40 +1243,30 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
         }
         // normalize
         double nrm = 0.0, ymax = 0.0;
-        for (int i = b0; i < b1; ++i) ymax = fmax(ymax, fabs(Z[(int64_t)i * n + jj]));
+        for (int i = b0; i < b1; ++i) ymax = fmax(ymax, fabs(Z[(int64_t)i * n + cj]));
         const double ys = ymax > 0.0 ? 1.0 / ymax : 1.0;
         for (int i = b0; i < b1; ++i) {
-          const double y = Z[(int64_t)i * n + jj] * ys;
+          const double y = Z[(int64_t)i * n + cj] * ys;
           nrm += y * y;
         }
         const double sc2 = ys / sqrt(nrm);
-        for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] *= sc2;
+        for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + cj] *= sc2;
       }
       // final normalization; sign: largest-magnitude component positive
       double nrm = 0.0, big = 0.0;
       for (int i = b0; i < b1; ++i) {
-        const double y = Z[(int64_t)i * n + jj];
+        const double y = Z[(int64_t)i * n + cj];
         nrm += y * y;
         if (fabs(y) > fabs(big)) big = y;
       }
       const double f = (big < 0.0 ? -1.0 : 1.0) / sqrt(nrm);
-      for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + jj] *= f;
+      for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + cj] *= f;
     }
-  }
-  // ---- ascending rank (stable on ties) ----
-  for (int j = tid; j < n; j += nt) {
-    const double lj = lam[j];
-    int r = 0;
-    for (int i = 0; i < n; ++i) {
-      const double li = lam[i];
-      r += (li < lj) || (li == lj && i < j);
-    }
-    rank_all[t.ao_off[s] + j] = r;
-    wall[t.ao_off[s] + r] = lj;
   }
 }
 
-// One column block of one system: Y[:, rank[j0 .. j0+cw)] = Q Z[:, j0 .. j0+cw).
+// One column block of one system: Y[:, j0 .. j0+cw) = Q Z[:, j0 .. j0+cw), for
+// the columns below the system's kocc (Z holds eigenvectors by ascending rank).
 struct BackTask {
   int sys;
   int j0;
@@ -1173,11 +1288,11 @@ constexpr size_t kBackStaticLds = (2 * kBackChunk * 256 + kBackThreads) * sizeof
 
 __device__ __forceinline__ void backtransform_registers(const double* __restrict__ A,
                                                         const double* __restrict__ Z,
-                                                        double* __restrict__ Y, const int* rank,
+                                                        double* __restrict__ Y, int kocc,
                                                         const double* zscale, int n, int j0,
                                                         double* ring, double* part) {
   const int tid = threadIdx.x, c = tid & 63, rg = tid >> 6;
-  const bool col_ok = j0 + c < n;
+  const bool col_ok = j0 + c < kocc;
   const double f = col_ok ? zscale[j0 + c] : 0.0;
   double z[kBackRegRows];
 #pragma unroll
@@ -1232,7 +1347,7 @@ __device__ __forceinline__ void backtransform_registers(const double* __restrict
     __syncthreads();
   }
   if (col_ok) {
-    const int dst = rank[j0 + c];
+    const int dst = j0 + c;
 #pragma unroll
     for (int q = 0; q < kBackRegRows; ++q) {
       const int i = rg + 8 * q;
@@ -1243,7 +1358,7 @@ __device__ __forceinline__ void backtransform_registers(const double* __restrict
 
 __device__ __forceinline__ void backtransform_lds(const double* __restrict__ A,
                                                   const double* __restrict__ Z,
-                                                  double* __restrict__ Y, const int* rank,
+                                                  double* __restrict__ Y, int kocc,
                                                   const double* zscale, int n, int j0, int cw,
                                                   double* lds) {
   double* Zb = lds;                 // [n][cw]
@@ -1252,7 +1367,7 @@ __device__ __forceinline__ void backtransform_lds(const double* __restrict__ A,
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int idx = tid; idx < n * cw; idx += nt) {
     const int i = idx / cw, c = idx % cw;
-    Zb[idx] = (j0 + c < n) ? zscale[j0 + c] * Z[(int64_t)i * n + j0 + c] : 0.0;
+    Zb[idx] = (j0 + c < kocc) ? zscale[j0 + c] * Z[(int64_t)i * n + j0 + c] : 0.0;
   }
   const int c = tid % cw, rg = tid / cw, nrg = nt / cw;
   // reflector k occupies A[k+1 .. n-1, k] (column-major); m = n - k - 1 <= 1023
@@ -1289,14 +1404,14 @@ __device__ __forceinline__ void backtransform_lds(const double* __restrict__ A,
   }
   for (int idx = tid; idx < n * cw; idx += nt) {
     const int i = idx / cw, cc = idx % cw;
-    if (j0 + cc < n) Y[(int64_t)i * n + rank[j0 + cc]] = Zb[idx];
+    if (j0 + cc < kocc) Y[(int64_t)i * n + j0 + cc] = Zb[idx];
   }
 }
 
 __global__ __launch_bounds__(kBackThreads) void k_backtransform(Topo t, const BackTask* tasks,
                                                                 const double* Aall,
                                                                 const double* Zall, double* Yall,
-                                                                const int* rank_all,
+                                                                const int* kocc_all,
                                                                 const double* zscale_all,
                                                                 const unsigned char* active) {
   extern __shared__ double lds[];
@@ -1307,12 +1422,13 @@ __global__ __launch_bounds__(kBackThreads) void k_backtransform(Topo t, const Ba
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
   const int64_t mo = t.mat_off[s];
-  const int* rank = rank_all + t.ao_off[s];
+  const int kocc = kocc_all[s];
+  if (task.j0 >= kocc) return;
   const double* zscale = zscale_all + t.ao_off[s];
   if (n <= 256)
-    backtransform_registers(Aall + mo, Zall + mo, Yall + mo, rank, zscale, n, task.j0, ring, part);
+    backtransform_registers(Aall + mo, Zall + mo, Yall + mo, kocc, zscale, n, task.j0, ring, part);
   else
-    backtransform_lds(Aall + mo, Zall + mo, Yall + mo, rank, zscale, n, task.j0, task.cw, lds);
+    backtransform_lds(Aall + mo, Zall + mo, Yall + mo, kocc, zscale, n, task.j0, task.cw, lds);
 }
 
 }  // namespace xtb

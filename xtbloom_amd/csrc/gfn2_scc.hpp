@@ -542,93 +542,6 @@ __global__ __launch_bounds__(256) void k_fock(Topo t, const ElemChunk* chunks,
 }
 
 // ---------------------------------------------------------------------------
-// Fermi occupations per system (reference eigensolver.cpp:992-1161), FP64
-// bisection on the shifted frame; one wavefront per system.
-// ---------------------------------------------------------------------------
-__device__ double fermi_sum_wave(const double* eps, int n, double ref, double mu, double kt,
-                                 bool holes) {
-  double acc = 0.0;
-  for (int i = threadIdx.x & 63; i < n; i += 64) {
-    const double a = ((eps[i] - ref) - mu) / kt;
-    double f;
-    if (holes) f = a >= 0.0 ? 1.0 / (exp(-a) + 1.0) : exp(a) / (1.0 + exp(a));
-    else f = a >= 0.0 ? exp(-a) / (1.0 + exp(-a)) : 1.0 / (exp(a) + 1.0);
-    acc += f;
-  }
-  return wave_sum(acc);
-}
-
-__device__ void fermi_wave(const double* eps, int n, double nel, double kt, double* occ,
-                           double& entropy) {
-  const int lane = threadIdx.x & 63;
-  entropy = 0.0;
-  if (nel == 0.0) {
-    for (int i = lane; i < n; i += 64) occ[i] = 0.0;
-    return;
-  }
-  if (kt == 0.0) {
-    const int full = (int)floor(nel);
-    for (int i = lane; i < n; i += 64) occ[i] = i < full ? 1.0 : (i == full ? nel - full : 0.0);
-    return;
-  }
-  if (nel == (double)n) {
-    for (int i = lane; i < n; i += 64) occ[i] = 1.0;
-    return;
-  }
-  const bool holes = nel > 0.5 * n;
-  const double qt = holes ? n - nel : nel;
-  const double steps = fmax(64.0, -log(qt / n) + 8.0);
-  const double ref = holes ? eps[n - 1] : eps[0];
-  const double lo0 = eps[0] - ref, hi0 = eps[n - 1] - ref;
-  const double margin = steps * kt + 64.0 * 2.220446049250313e-16 * fmax(1.0, fabs(hi0 - lo0));
-  double lo = lo0 - margin, hi = hi0 + margin;
-  const double tol = 64.0 * 2.220446049250313e-16 * qt;
-  for (int it = 0; it < 4096; ++it) {
-    const double mid = lo + 0.5 * (hi - lo);
-    const double q = fermi_sum_wave(eps, n, ref, mid, kt, holes);
-    if (fabs(q - qt) <= tol) { lo = hi = mid; break; }
-    if (mid == lo || mid == hi) break;
-    if ((!holes && q < qt) || (holes && q > qt)) lo = mid; else hi = mid;
-  }
-  const double mu = lo + 0.5 * (hi - lo);
-  double sacc = 0.0;
-  for (int i = lane; i < n; i += 64) {
-    const double a = ((eps[i] - ref) - mu) / kt;
-    const double f = a >= 0.0 ? exp(-a) / (1.0 + exp(-a)) : 1.0 / (exp(a) + 1.0);
-    occ[i] = f;
-    if (f > 0.0 && f < 1.0) sacc -= f * log(f) + (1.0 - f) * log(1.0 - f);
-  }
-  entropy = wave_sum(sacc);
-}
-
-// occ_total[i] = f_alpha + f_beta; fe[i] = occ_total * eps; kocc = last nonzero + 1
-__global__ __launch_bounds__(64) void k_fermi(Topo t, const int* order, const unsigned char* active,
-                                              const double* eps_all, double* fa_all, double* fb_all,
-                                              double* occ_all, double* fe_all, SysState* st,
-                                              int* kocc, double kt) {
-  const int s = order[blockIdx.x];
-  if (active && !active[s]) return;
-  const int n = t.ao_off[s + 1] - t.ao_off[s];
-  const int o = t.ao_off[s];
-  double sa, sb;
-  fermi_wave(eps_all + o, n, t.nalpha[s], kt, fa_all + o, sa);
-  fermi_wave(eps_all + o, n, t.nbeta[s], kt, fb_all + o, sb);
-  int last = 0;
-  for (int i = threadIdx.x; i < n; i += 64) {
-    const double f = fa_all[o + i] + fb_all[o + i];
-    occ_all[o + i] = f;
-    fe_all[o + i] = f * eps_all[o + i];
-    if (f != 0.0) last = max(last, i + 1);
-  }
-  for (int off = 32; off > 0; off >>= 1) last = max(last, __shfl_xor(last, off, 64));
-  if (threadIdx.x == 0) {
-    st[s].entropy = sa + sb;
-    st[s].kocc = last;
-    kocc[s] = last;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Mulliken raw multipoles and Tr(P H0) (reference mulliken_kernels_impl.hpp:40-115),
 // in two deterministic phases:
 //   k_mulliken_rows    one block per 32-row chunk of a system: per column c the
