@@ -545,6 +545,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(3 * (size_t)N); addd(6 * (size_t)N);
   for (int k = 0; k < 5; ++k) addd(NAO);  // eps fa fb occ fe
   addd(NAO); addd(NAO);                   // tridiagonal d and e
+  addd((size_t)kBackChunk * NAO + (size_t)kBackChunk * kBackChunk * B);  // WY factors (wy_offset)
   addd(B);                                // per-system eigenvalue change of the last round
   addd(NAO);                              // eigenvector normalization factors
   addd((size_t)NAO * 10);      // mulliken column partials
@@ -736,6 +737,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   double* d_shift = cv.take<double>(B);
   double* d_zscale = cv.take<double>(NAO);
   double* d_td = cv.take<double>(NAO);
+  double* d_wyT = cv.take<double>((size_t)kBackChunk * NAO + (size_t)kBackChunk * kBackChunk * B);
   double* d_te = cv.take<double>(NAO);
   double* d_colsum = cv.take<double>((size_t)NAO * 10);
   MixerArrays mx{};
@@ -920,8 +922,9 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
                        FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt});
     HIP_TRY(probe_end(kKTridiagEigen, 0.0, 0.0));
     HIP_TRY(probe_begin());
+    hipLaunchKernelGGL(k_wy_factors, dim3(B), dim3(256), 0, sm, t, d_order, d_active, dA, d_wyT);
     hipLaunchKernelGGL(k_backtransform, dim3((unsigned)back.size()), dim3(kBackThreads), back_lds, sm, t, d_back,
-                       dA, dZ, dY, d_kocc, d_zscale, d_active);
+                       dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
     HIP_TRY(probe_end(kKBacktransform, 0.0, 0.0));
     pr_bt = probes.size();
     // C = X Y (the kocc leading columns), P = C diag(f) C^T

@@ -1275,46 +1275,143 @@ struct BackTask {
 };
 
 // 512 threads.  n <= 256: the 64-column block lives in registers (thread
-// (c, rg) = (tid % 64, tid / 64) holds rows rg, rg + 8, ...), reflectors are
-// staged in LDS zero-padded to absolute row indices, so every thread applies
-// every reflector without masks and a reflector costs two barriers.  n > 256:
-// the column block lives in dynamic LDS ((n cw + 2 n + 512) doubles).  In both
-// paths the next reflector is fetched into registers while the current one is
-// applied.
+// (c, rg) = (tid % 64, tid / 64) holds rows rg, rg + 8, ...) and the
+// reflectors are applied kBackChunk at a time in compact WY form,
+//   H_{k_7} ... H_{k_0} = I - V T V^T,   V = [v_{k_0} .. v_{k_7}],
+// with k_u = n-3 - ch kBackChunk - u and T lower triangular (k_wy_factors):
+// one LDS reduction (V^T z for the whole chunk) and two barriers per chunk.
+// Reflectors are staged in LDS zero-padded to absolute row indices, so every
+// thread applies every reflector without masks; the next chunk is fetched into
+// registers while the current one is applied.  n > 256: the column block lives
+// in dynamic LDS ((n cw + 2 n + 512) doubles) and reflectors are applied one at
+// a time.
 constexpr int kBackThreads = 512;
 constexpr int kBackRegRows = 32;  // rows per thread of the register path (n <= 256)
-constexpr int kBackChunk = 8;     // reflectors fetched per global-load round trip
-constexpr size_t kBackStaticLds = (2 * kBackChunk * 256 + kBackThreads) * sizeof(double);
+constexpr int kBackChunk = 8;     // reflectors per WY block
+constexpr size_t kBackStaticLds =
+    (2 * kBackChunk * 256 + kBackChunk * kBackThreads + kBackChunk * 64) * sizeof(double);
+
+// T factor of every WY block of a system, T_mi at Tall[wy_offset + 64 ch + 8 m + i]
+// (a system holds ceil((n-2)/8) blocks of 64 entries: at most 8 n + 40):
+//   T_mm = 2,  T_mi = -2 sum_{l=i}^{m-1} (v_m . v_l) T_li  (i < m),
+// which follows from H_m (I - V T V^T) = I - [V v_m] [[T, 0], [-2 v_m^T V T, 2]] [V v_m]^T.
+// One wavefront per block; the inner products are over rows > k_i, where both
+// reflectors live.
+// start of system s's WY factors: 8 per AO of the preceding systems plus 64 per
+// preceding system covers the 8 n + 40 entries each one needs
+__device__ __forceinline__ int64_t wy_offset(const Topo& t, int s) {
+  return (int64_t)kBackChunk * t.ao_off[s] + (int64_t)kBackChunk * kBackChunk * s;
+}
+
+// One block per system, one WY block per wavefront at a time: the wavefront's
+// eight reflectors are staged in LDS, lane p + 32 h (p < 28) forms the inner
+// product of pair p over rows of parity h, and lane 0 runs the recurrence.
+__global__ __launch_bounds__(256) void k_wy_factors(Topo t, const int* order,
+                                                    const unsigned char* active, const double* Aall,
+                                                    double* Tall) {
+  const int s = order[blockIdx.x];
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  if (n > 256) return;  // the LDS path applies reflectors one at a time
+  const double* A = Aall + t.mat_off[s];
+  double* Ts = Tall + wy_offset(t, s);
+  constexpr int kPairs = kBackChunk * (kBackChunk - 1) / 2;  // 28
+  __shared__ double vs[4][kBackChunk][256];  // per wavefront, zero-padded to absolute rows
+  __shared__ double gs[4][32];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nref = n - 2, nch = (nref + kBackChunk - 1) / kBackChunk;
+  // pair of this lane: p -> (m, i), m > i, in row-major order of the strict lower triangle
+  const int p = lane & 31, h = lane >> 5;
+  int pm = 1, pi = 0;
+  for (int q = 0; q < p && q < kPairs - 1; ++q) {
+    if (++pi == pm) { ++pm; pi = 0; }
+  }
+  for (int cg = 0; cg < nch; cg += 4) {
+    const int ch = cg + wid;  // >= nch: this wavefront idles but joins the barriers
+    const int kb = n - 3 - ch * kBackChunk;
+    for (int idx = lane; idx < kBackChunk * 256; idx += 64) {
+      const int u = idx >> 8, r = idx & 255, k = kb - u;
+      vs[wid][u][r] = (ch < nch && k >= 0 && r > k && r < n) ? A[r + (int64_t)k * n] : 0.0;
+    }
+    __syncthreads();
+    double acc = 0.0;
+    if (p < kPairs)
+      for (int r = h; r < n; r += 2) acc = fma(vs[wid][pm][r], vs[wid][pi][r], acc);
+    acc += __shfl_xor(acc, 32, 64);
+    if (h == 0 && p < kPairs) gs[wid][p] = acc;
+    __syncthreads();
+    if (lane == 0 && ch < nch) {
+      double g[kBackChunk][kBackChunk];
+      {
+        int q = 0;
+#pragma unroll
+        for (int m = 1; m < kBackChunk; ++m)
+#pragma unroll
+          for (int i = 0; i < m; ++i) g[m][i] = gs[wid][q++];
+      }
+      double T[kBackChunk][kBackChunk];
+#pragma unroll
+      for (int m = 0; m < kBackChunk; ++m) {
+#pragma unroll
+        for (int i = 0; i < kBackChunk; ++i) T[m][i] = 0.0;
+        if (kb - m < 0) continue;  // past the last reflector: zero row
+        T[m][m] = 2.0;
+#pragma unroll
+        for (int i = 0; i < m; ++i) {
+          double tacc = 0.0;
+#pragma unroll
+          for (int l = i; l < m; ++l) tacc = fma(g[m][l], T[l][i], tacc);
+          T[m][i] = -2.0 * tacc;
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < kBackChunk; ++m)
+#pragma unroll
+        for (int i = 0; i < kBackChunk; ++i) Ts[ch * kBackChunk * kBackChunk + m * kBackChunk + i] = T[m][i];
+    }
+  }
+}
 
 __device__ __forceinline__ void backtransform_registers(const double* __restrict__ A,
                                                         const double* __restrict__ Z,
+                                                        const double* __restrict__ Tf,
                                                         double* __restrict__ Y, int kocc,
                                                         const double* zscale, int n, int j0,
-                                                        double* ring, double* part) {
-  const int tid = threadIdx.x, c = tid & 63, rg = tid >> 6;
-  const bool col_ok = j0 + c < kocc;
-  const double f = col_ok ? zscale[j0 + c] : 0.0;
-  double z[kBackRegRows];
+                                                        double* ring, double* part, double* wf) {
+  // thread (w, cg, rl): wavefront w, column group cg = lane / 4 (columns
+  // j0 + 4 cg + j, j < 4), row lane rl = lane % 4; rows rgp + 32 q (q < 8) with
+  // rgp = 4 w + rl.  Each reflector entry read from LDS feeds four columns.
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int cg = lane >> 2, rl = lane & 3, rgp = 4 * w + rl;
+  constexpr int kRows = 8, kCols = 4;
+  double z[kRows][kCols];
 #pragma unroll
-  for (int q = 0; q < kBackRegRows; ++q) {
-    const int i = rg + 8 * q;
-    z[q] = (i < n && col_ok) ? f * Z[(int64_t)i * n + j0 + c] : 0.0;
+  for (int j = 0; j < kCols; ++j) {
+    const int c = j0 + kCols * cg + j;
+    const bool ok = c < kocc;
+    const double f = ok ? zscale[c] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) {
+      const int i = rgp + 32 * q;
+      z[q][j] = (ok && i < n) ? f * Z[(int64_t)i * n + c] : 0.0;
+    }
   }
-  // Reflectors are applied in the order k = n-3, ..., 0, in chunks of
-  // kBackChunk; chunk ch holds k = n-3 - ch*kBackChunk - u at ring slot
-  // ((ch & 1) * kBackChunk + u) * 256, zero-padded to absolute rows.
+  // chunk ch holds k = n-3 - ch*kBackChunk - u at ring slot
+  // ((ch & 1) * kBackChunk + u) * 256, zero-padded to absolute rows; its T
+  // factor at Tf[64 ch].
   constexpr int kVals = kBackChunk * 256 / kBackThreads;
+  constexpr int kTT = kBackChunk * kBackChunk;
   auto fetch = [&](int ch, double (&r)[kVals]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int w = 0; w < kVals; ++w) {
-      const int v = tid + kBackThreads * w, u = v >> 8, i = v & 255;
+    for (int v = 0; v < kVals; ++v) {
+      const int x = tid + kBackThreads * v, u = x >> 8, i = x & 255;
       const int k = n - 3 - ch * kBackChunk - u;
-      r[w] = (k >= 0 && i > k && i < n) ? A[i + (int64_t)k * n] : 0.0;
+      r[v] = (k >= 0 && i > k && i < n) ? A[i + (int64_t)k * n] : 0.0;
     }
   };
   auto stage = [&](int ch, const double (&r)[kVals]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int w = 0; w < kVals; ++w) ring[(ch & 1) * kBackChunk * 256 + tid + kBackThreads * w] = r[w];
+    for (int v = 0; v < kVals; ++v) ring[(ch & 1) * kBackChunk * 256 + tid + kBackThreads * v] = r[v];
   };
   const int nref = n - 2;  // reflectors k = 0 .. n-3
   double rn[kVals];
@@ -1326,32 +1423,85 @@ __device__ __forceinline__ void backtransform_registers(const double* __restrict
   for (int ch = 0; ch * kBackChunk < nref; ++ch) {
     const bool more = (ch + 1) * kBackChunk < nref;
     if (more) fetch(ch + 1, rn);  // in flight while this chunk is applied
+    const double* hv = ring + (ch & 1) * kBackChunk * 256;
+    // T is read-only and wave-uniform: the constant address space puts it in
+    // SGPRs through scalar loads
+    const __attribute__((address_space(4))) double* tt =
+        (const __attribute__((address_space(4))) double*)(Tf + ch * kTT);
+    // V^T z: per reflector, this thread's rows, summed over the four row lanes
+    // of the column group (DPP quad permutations), then over the wavefronts
+#pragma unroll
     for (int u = 0; u < kBackChunk; ++u) {
-      const int k = n - 3 - ch * kBackChunk - u;
-      if (k < 0) break;
-      const double* hv = ring + ((ch & 1) * kBackChunk + u) * 256;
-      double p = 0.0;
+      double p[kCols] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int q = 0; q < kBackRegRows; ++q) p = fma(hv[rg + 8 * q], z[q], p);
-      part[rg * 64 + c] = p;
-      __syncthreads();
-      double dot = 0.0;
+      for (int q = 0; q < kRows; ++q) {
+        const double h = hv[u * 256 + rgp + 32 * q];
 #pragma unroll
-      for (int w = 0; w < 8; ++w) dot += part[w * 64 + c];
-      dot *= 2.0;
+        for (int j = 0; j < kCols; ++j) p[j] = fma(h, z[q][j], p[j]);
+      }
 #pragma unroll
-      for (int q = 0; q < kBackRegRows; ++q) z[q] = fma(-dot, hv[rg + 8 * q], z[q]);
-      __syncthreads();
+      for (int j = 0; j < kCols; ++j) {
+        p[j] += dpp_move<0xb1, 0xf>(p[j]);  // lanes 0<->1, 2<->3
+        p[j] += dpp_move<0x4e, 0xf>(p[j]);  // lanes 0<->2, 1<->3
+      }
+      if (rl == 0) {
+#pragma unroll
+        for (int j = 0; j < kCols; ++j) part[(u * 8 + w) * 64 + kCols * cg + j] = p[j];
+      }
+    }
+    __syncthreads();
+    {
+      const int u = tid >> 6, c = tid & 63;
+      double acc = 0.0;
+#pragma unroll
+      for (int g = 0; g < 8; ++g) acc += part[(u * 8 + g) * 64 + c];
+      wf[u * 64 + c] = acc;
+    }
+    __syncthreads();
+    // y = T w (T lower triangular; in place, last row first)
+    double y[kBackChunk][kCols];
+#pragma unroll
+    for (int u = 0; u < kBackChunk; ++u)
+#pragma unroll
+      for (int j = 0; j < kCols; ++j) y[u][j] = wf[u * 64 + kCols * cg + j];
+#pragma unroll
+    for (int m = kBackChunk - 1; m >= 0; --m)
+#pragma unroll
+      for (int j = 0; j < kCols; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i <= m; ++i) acc = fma(tt[m * kBackChunk + i], y[i][j], acc);
+        y[m][j] = acc;
+      }
+    // z -= V y.  Empty asm with a memory clobber pins the program order so the
+    // reflector entries of one row are loaded only when that row is updated
+    // (hoisting all 64 up front spills the column block).
+#pragma unroll
+    for (int u = 0; u < kBackChunk; ++u)
+#pragma unroll
+      for (int j = 0; j < kCols; ++j) asm volatile("" : "+v"(y[u][j])::"memory");
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) {
+#pragma unroll
+      for (int u = 0; u < kBackChunk; ++u) {
+        const double h = hv[u * 256 + rgp + 32 * q];
+#pragma unroll
+        for (int j = 0; j < kCols; ++j) z[q][j] = fma(-h, y[u][j], z[q][j]);
+      }
+#pragma unroll
+      for (int j = 0; j < kCols; ++j) asm volatile("" : "+v"(z[q][j])::"memory");
     }
     if (more) stage(ch + 1, rn);
     __syncthreads();
   }
-  if (col_ok) {
-    const int dst = j0 + c;
 #pragma unroll
-    for (int q = 0; q < kBackRegRows; ++q) {
-      const int i = rg + 8 * q;
-      if (i < n) Y[(int64_t)i * n + dst] = z[q];
+  for (int j = 0; j < kCols; ++j) {
+    const int c = j0 + kCols * cg + j;
+    if (c >= kocc) continue;
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) {
+      const int i = rgp + 32 * q;
+      if (i < n) Y[(int64_t)i * n + c] = z[q][j];
     }
   }
 }
@@ -1410,13 +1560,15 @@ __device__ __forceinline__ void backtransform_lds(const double* __restrict__ A,
 
 __global__ __launch_bounds__(kBackThreads) void k_backtransform(Topo t, const BackTask* tasks,
                                                                 const double* Aall,
-                                                                const double* Zall, double* Yall,
+                                                                const double* Zall,
+                                                                const double* Tall, double* Yall,
                                                                 const int* kocc_all,
                                                                 const double* zscale_all,
                                                                 const unsigned char* active) {
   extern __shared__ double lds[];
   __shared__ double ring[2 * kBackChunk * 256];
-  __shared__ double part[kBackThreads];
+  __shared__ double part[kBackChunk * kBackThreads];
+  __shared__ double wf[kBackChunk * 64];
   const BackTask task = tasks[blockIdx.x];
   const int s = task.sys;
   if (active && !active[s]) return;
@@ -1426,7 +1578,8 @@ __global__ __launch_bounds__(kBackThreads) void k_backtransform(Topo t, const Ba
   if (task.j0 >= kocc) return;
   const double* zscale = zscale_all + t.ao_off[s];
   if (n <= 256)
-    backtransform_registers(Aall + mo, Zall + mo, Yall + mo, kocc, zscale, n, task.j0, ring, part);
+    backtransform_registers(Aall + mo, Zall + mo, Tall + wy_offset(t, s),
+                            Yall + mo, kocc, zscale, n, task.j0, ring, part, wf);
   else
     backtransform_lds(Aall + mo, Zall + mo, Yall + mo, kocc, zscale, n, task.j0, task.cw, lds);
 }
