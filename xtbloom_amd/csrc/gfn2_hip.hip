@@ -559,7 +559,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(3 * (size_t)N);         // grad
   addd(3 * (size_t)NPC); addd(NPC);
   addd(B); addd(B);            // erep, eatm
-  addd(2 * (size_t)NAP);       // ATM pair C6 staging
+  addd(kAtmFields * (size_t)NAP);  // ATM per-pair factors
   addi(N);                     // element-grouped atom order
   addd((size_t)kPairStageDoubles * N); addi((size_t)kPairStageInts * N);  // pair staging (large systems)
   addd((size_t)mem * mem * B);  // Broyden overlaps
@@ -765,7 +765,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   double* d_dedq = cv.take<double>(NPC);
   double* d_erep = cv.take<double>(B);
   double* d_eatm = cv.take<double>(B);
-  double* d_c6pair = cv.take<double>(2 * (size_t)NAP);
+  double* d_c6pair = cv.take<double>(kAtmFields * (size_t)NAP);
   int* d_perm = cv.take<int>(N);
   double* d_gstage = cv.take<double>((size_t)kPairStageDoubles * N);
   int* d_gstagei = cv.take<int>((size_t)kPairStageInts * N);

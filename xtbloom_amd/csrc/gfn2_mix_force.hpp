@@ -545,12 +545,23 @@ __global__ __launch_bounds__(256) void k_force_atoms(Topo t, const int* order,
 }
 
 // D4 three-body ATM energy and gradient with zero-charge weights
-// (reference d4.cpp:1120-1362).  Pair data c6 / dc6/dcn are staged per ordered
-// pair; the triple loop accumulates gradient and CN adjoint with LDS atomics
-// (one system per block, at most kMaxAO atoms) and finishes with the D4-CN chain.
+// (reference d4.cpp:1120-1362).  Every factor of a triple's term that belongs
+// to one pair is computed once per ordered pair and staged (structure of
+// arrays at 5 apair_off[s]):
+//   r2 = |R_x - R_y|^2,  ir = 1/|R_x - R_y|,  q = (R0_xy / |R_x - R_y|)^(16/3),
+//   sc = sqrt|C6_xy|,    dcr = (dC6_xy/dCN_x) / C6_xy,
+// so that with r1p = r_ij r_ik r_jk, (R0_ij R0_ik R0_jk / r1p)^(16/3) = q_ij q_ik q_jk
+// and sqrt|C6_ij C6_ik C6_jk| = sc_ij sc_ik sc_jk: a triple needs no power,
+// square root or division except the damping's reciprocal.  One thread per pair
+// (j, k), j > k, loops over i > j, so the lanes of a wavefront share the loop
+// length; gradient and CN adjoint of i go to LDS atomics (one system per block,
+// at most kMaxAO atoms), those of j and k stay in registers.  The D4-CN chain
+// finishes the kernel.
+constexpr int kAtmFields = 5;
+
 __global__ __launch_bounds__(256) void k_force_atm(Topo t, const int* order, const unsigned char* valid,
                                                    const double* cnd4, double* w4, double* dw4c,
-                                                   double* c6pair, double* dedcn4, double* grad,
+                                                   double* pairbuf, double* dedcn4, double* grad,
                                                    double* eatm) {
   const int s = order[blockIdx.x];
   if (!valid[s]) return;
@@ -559,105 +570,128 @@ __global__ __launch_bounds__(256) void k_force_atm(Topo t, const int* order, con
   __shared__ double red[16];
   __shared__ double gsh[3 * kMaxAO];
   __shared__ double csh[kMaxAO];
-  double* c6 = c6pair + 2 * t.apair_off[s];  // [N*N][2]: c6, dc6/dcn(first)
+  __shared__ double xs[3 * kMaxAO];
+  const int64_t NN = (int64_t)N * N;
+  double* pr2 = pairbuf + kAtmFields * t.apair_off[s];
+  double* pir = pr2 + NN;
+  double* pq = pr2 + 2 * NN;
+  double* psc = pr2 + 3 * NN;
+  double* pdc = pr2 + 4 * NN;
   for (int a = a0 + tid; a < a1; a += nt) {
     d4_weights(t, a, cnd4[a], 0.0, w4 + kD4MaxRef * a, dw4c + kD4MaxRef * a, nullptr);
     const int la = a - a0;
-    gsh[3 * la] = gsh[3 * la + 1] = gsh[3 * la + 2] = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      gsh[3 * la + c] = 0.0;
+      xs[3 * la + c] = t.xyz[3 * a + c];
+    }
     csh[la] = 0.0;
   }
   __syncthreads();
-  for (int64_t idx = tid; idx < (int64_t)N * N; idx += nt) {
-    const int i = a0 + (int)(idx / N), j = a0 + (int)(idx % N);
-    if (i == j) { c6[2 * idx] = 0.0; c6[2 * idx + 1] = 0.0; continue; }
-    c6[2 * idx] = d4_contract(t, i, j, w4 + kD4MaxRef * i, w4 + kD4MaxRef * j);
-    c6[2 * idx + 1] = d4_contract(t, i, j, dw4c + kD4MaxRef * i, w4 + kD4MaxRef * j);
+  const double ex3 = 16.0 / 3.0;
+  for (int64_t idx = tid; idx < NN; idx += nt) {
+    const int lx = (int)(idx / N), ly = (int)(idx % N);
+    if (lx == ly) continue;
+    const int x = a0 + lx, y = a0 + ly;
+    const double d0 = xs[3 * ly] - xs[3 * lx], d1 = xs[3 * ly + 1] - xs[3 * lx + 1],
+                 d2 = xs[3 * ly + 2] - xs[3 * lx + 2];
+    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    const double r = sqrt(r2);
+    const double c6 = d4_contract(t, x, y, w4 + kD4MaxRef * x, w4 + kD4MaxRef * y);
+    const double dc6 = d4_contract(t, x, y, dw4c + kD4MaxRef * x, w4 + kD4MaxRef * y);
+    pr2[idx] = r2;
+    pir[idx] = 1.0 / r;
+    pq[idx] = pow(d4_r0(t, x, y) / r, ex3);
+    psc[idx] = sqrt(fabs(c6));
+    pdc[idx] = dc6 / c6;
   }
   __syncthreads();
   double e = 0.0;
-  const double ex3 = 16.0 / 3.0;
-  // triples i > j > k; thread over (i, j) pairs
-  for (int64_t idx = tid; idx < (int64_t)N * N; idx += nt) {
-    const int li = (int)(idx / N), lj = (int)(idx % N);
-    if (lj >= li) continue;
-    const int i = a0 + li, j = a0 + lj;
-    const double* ri = t.xyz + 3 * i;
-    const double* rj = t.xyz + 3 * j;
-    const double vij[3] = {rj[0] - ri[0], rj[1] - ri[1], rj[2] - ri[2]};
-    const double r2ij = vij[0] * vij[0] + vij[1] * vij[1] + vij[2] * vij[2];
-    if (r2ij > 625.0) continue;
-    const double cij = c6[2 * ((int64_t)li * N + lj)];
-    double gi[3] = {0, 0, 0}, gj[3] = {0, 0, 0}, dci = 0.0, dcj = 0.0;
-    for (int lk = 0; lk < lj; ++lk) {
-      const int k = a0 + lk;
-      const double* rk = t.xyz + 3 * k;
-      const double vik[3] = {rk[0] - ri[0], rk[1] - ri[1], rk[2] - ri[2]};
-      const double vjk[3] = {rk[0] - rj[0], rk[1] - rj[1], rk[2] - rj[2]};
-      const double r2ik = vik[0] * vik[0] + vik[1] * vik[1] + vik[2] * vik[2];
-      const double r2jk = vjk[0] * vjk[0] + vjk[1] * vjk[1] + vjk[2] * vjk[2];
-      if (r2ik > 625.0 || r2jk > 625.0) continue;
-      const double cik = c6[2 * ((int64_t)li * N + lk)], cjk = c6[2 * ((int64_t)lj * N + lk)];
-      const double r0 = d4_r0(t, j, i) * d4_r0(t, k, i) * d4_r0(t, k, j);
-      const double r2p = r2ij * r2ik * r2jk, r1p = sqrt(r2p), r3p = r2p * r1p, r5p = r3p * r2p;
-      const double rp = pow(r0 / r1p, ex3);
+  // pairs (j, k), j > k, enumerated row by row (p = j (j-1)/2 + k): all lanes
+  // busy, and consecutive lanes share j and with it the length of the i loop
+  const int64_t ntri = (int64_t)N * (N - 1) / 2;
+  for (int64_t p = tid; p < ntri; p += nt) {
+    int lj = (int)((1.0 + sqrt(1.0 + 8.0 * (double)p)) * 0.5);
+    while ((int64_t)lj * (lj - 1) / 2 > p) --lj;
+    while ((int64_t)(lj + 1) * lj / 2 <= p) ++lj;
+    const int lk = (int)(p - (int64_t)lj * (lj - 1) / 2);
+    const int64_t jk = (int64_t)lj * N + lk;
+    const double r2jk = pr2[jk];
+    if (r2jk > 625.0) continue;
+    const double irjk = pir[jk], qjk = pq[jk], scjk = psc[jk];
+    const double dcjk_j = pdc[jk], dcjk_k = pdc[(int64_t)lk * N + lj];
+    const double vjk[3] = {xs[3 * lk] - xs[3 * lj], xs[3 * lk + 1] - xs[3 * lj + 1],
+                           xs[3 * lk + 2] - xs[3 * lj + 2]};
+    double gj[3] = {0, 0, 0}, gk[3] = {0, 0, 0}, dcj = 0.0, dck = 0.0;
+    for (int li = lj + 1; li < N; ++li) {
+      const int64_t ij = (int64_t)li * N + lj, ik = (int64_t)li * N + lk;
+      const double r2ij = pr2[ij], r2ik = pr2[ik];
+      if (r2ij > 625.0 || r2ik > 625.0) continue;
+      const double ir1 = pir[ij] * pir[ik] * irjk;
+      const double ir3 = ir1 * ir1 * ir1, ir5 = ir3 * ir1 * ir1;
+      const double rp = pq[ij] * pq[ik] * qjk;
       const double damp = 1.0 / (1.0 + 6.0 * rp);
-      const double ang = 0.375 * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik) * (-r2ij + r2jk + r2ik) / r5p +
-                         1.0 / r3p;
-      const double c9 = -5.0 * sqrt(fabs(cij * cik * cjk));
-      e -= ang * damp * c9;
+      const double ang = 0.375 * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik) * (-r2ij + r2jk + r2ik) * ir5 + ir3;
+      const double c9 = -5.0 * psc[ij] * psc[ik] * scjk;
+      const double de = ang * damp * c9;
+      e -= de;
       const double ddamp = -2.0 * 16.0 * rp * damp * damp;
+      const double vij[3] = {xs[3 * lj] - xs[3 * li], xs[3 * lj + 1] - xs[3 * li + 1],
+                             xs[3 * lj + 2] - xs[3 * li + 2]};
+      const double vik[3] = {xs[3 * lk] - xs[3 * li], xs[3 * lk + 1] - xs[3 * li + 1],
+                             xs[3 * lk + 2] - xs[3 * li + 2]};
       const double tg[3] = {r2ij, r2ik, r2jk}, o1[3] = {r2jk, r2jk, r2ik}, o2[3] = {r2ik, r2ij, r2ij};
+      const double irr[3] = {pir[ij], pir[ik], irjk};
       const double* vv[3] = {vij, vik, vjk};
       double dg[3][3];
+#pragma unroll
       for (int q = 0; q < 3; ++q) {
         const double T = tg[q], a = o1[q], b = o2[q];
         const double dang = -0.375 * (T * T * T + T * T * (a + b) + T * (3.0 * a * a + 2.0 * a * b + 3.0 * b * b) -
-                                      5.0 * (a - b) * (a - b) * (a + b)) / r5p;
-        const double scq = c9 * (-dang * damp + ang * ddamp) / T;
+                                      5.0 * (a - b) * (a - b) * (a + b)) * ir5;
+        const double scq = c9 * (-dang * damp + ang * ddamp) * (irr[q] * irr[q]);
+#pragma unroll
         for (int c = 0; c < 3; ++c) dg[q][c] = scq * vv[q][c];
       }
+#pragma unroll
       for (int c = 0; c < 3; ++c) {
-        gi[c] -= dg[0][c] + dg[1][c];
+        atomicAdd(&gsh[3 * li + c], -(dg[0][c] + dg[1][c]));
         gj[c] += dg[0][c] - dg[2][c];
-        atomicAdd(&gsh[3 * lk + c], dg[1][c] + dg[2][c]);
+        gk[c] += dg[1][c] + dg[2][c];
       }
-      const double de = ang * damp * c9;
-      // dc6(x, y)/dcn_x is stored at [x][y] slot 1
-      const double dcij_i = c6[2 * ((int64_t)li * N + lj) + 1], dcij_j = c6[2 * ((int64_t)lj * N + li) + 1];
-      const double dcik_i = c6[2 * ((int64_t)li * N + lk) + 1], dcik_k = c6[2 * ((int64_t)lk * N + li) + 1];
-      const double dcjk_j = c6[2 * ((int64_t)lj * N + lk) + 1], dcjk_k = c6[2 * ((int64_t)lk * N + lj) + 1];
-      dci -= 0.5 * de * (dcij_i / cij + dcik_i / cik);
-      dcj -= 0.5 * de * (dcij_j / cij + dcjk_j / cjk);
-      atomicAdd(&csh[lk], -0.5 * de * (dcik_k / cik + dcjk_k / cjk));
+      // dC6(x, y)/dCN_x / C6(x, y) sits at the ordered pair [x][y]
+      atomicAdd(&csh[li], -0.5 * de * (pdc[ij] + pdc[ik]));
+      dcj -= 0.5 * de * (pdc[(int64_t)lj * N + li] + dcjk_j);
+      dck -= 0.5 * de * (pdc[(int64_t)lk * N + li] + dcjk_k);
     }
     for (int c = 0; c < 3; ++c) {
-      atomicAdd(&gsh[3 * li + c], gi[c]);
       atomicAdd(&gsh[3 * lj + c], gj[c]);
+      atomicAdd(&gsh[3 * lk + c], gk[c]);
     }
-    atomicAdd(&csh[li], dci);
     atomicAdd(&csh[lj], dcj);
+    atomicAdd(&csh[lk], dck);
   }
   e = block_sum(e, red);
   if (tid == 0) eatm[s] = e;
   __syncthreads();
   // D4-CN chain of the ATM adjoint
   for (int a = a0 + tid; a < a1; a += nt) {
-    const double* ra = t.xyz + 3 * a;
-    double g[3] = {gsh[3 * (a - a0)], gsh[3 * (a - a0) + 1], gsh[3 * (a - a0) + 2]};
+    const int la = a - a0;
+    double g[3] = {gsh[3 * la], gsh[3 * la + 1], gsh[3 * la + 2]};
     for (int b = a0; b < a1; ++b) {
       if (b == a) continue;
-      const double* rb = t.xyz + 3 * b;
-      const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+      const int lb = b - a0;
+      const double r[3] = {xs[3 * la] - xs[3 * lb], xs[3 * la + 1] - xs[3 * lb + 1],
+                           xs[3 * la + 2] - xs[3 * lb + 2]};
       const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
       if (r2 > 900.0) continue;
       const double rr = sqrt(r2);
       double val, der;
       d4_cn_count(t, a, b, rr, val, der);
-      const double sc = (csh[a - a0] + csh[b - a0]) * der / rr;
+      const double sc = (csh[la] + csh[lb]) * der / rr;
       for (int c = 0; c < 3; ++c) g[c] += sc * r[c];
     }
     for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
-    dedcn4[a] = csh[a - a0];
+    dedcn4[a] = csh[la];
   }
 }
 
