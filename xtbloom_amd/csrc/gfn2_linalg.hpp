@@ -403,10 +403,17 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
     // 32 TJ + 31 >= i > k, all of them live
     double p = 0.0;
     if (tid > k && tid < n) {
-#pragma unroll 8
-      for (int r = 0; r < 32; ++r) p += rowpart[r * kTriPad + tid];
-#pragma unroll 8
-      for (int r = 0; r < 16; ++r) p += colpart[r * kTriPad + tid];
+      // eight independent partial sums: the loads overlap and the add chain is short
+      double ps[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int r = 0; r < 32; r += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ps[u] += rowpart[(r + u) * kTriPad + tid];
+      }
+      for (int r = 0; r < 16; r += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ps[u] += colpart[(r + u) * kTriPad + tid];
+      }
+      p = ((ps[0] + ps[1]) + (ps[2] + ps[3])) + ((ps[4] + ps[5]) + (ps[6] + ps[7]));
     }
     pq[tid] = p;
     __syncthreads();
@@ -424,13 +431,17 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
       if (32 * TJ + 31 > k) {
         const int j = 32 * TJ + lc;
         const double vj = v[j], pj = pq[j];
+        const double vjm = -2.0 * vj, pjm = -2.0 * pj;  // strict tiles: two FMAs per element
         static_for<2 * TJ, NR>([&](auto CI) __attribute__((always_inline)) {
           constexpr int TI = decltype(CI)::value;
           constexpr int sl = tri_slots(TI) + TJ;
           const int i = 16 * TI + lr;
-          const double u = fma(vr[TI], pj, rr[TI] * vj);
-          if constexpr (tri_strict(TI, TJ)) a[sl] -= 2.0 * u;
-          else a[sl] -= (i > j ? 2.0 : (i == j ? 1.0 : 0.0)) * u;
+          if constexpr (tri_strict(TI, TJ)) {
+            a[sl] = fma(rr[TI], vjm, fma(vr[TI], pjm, a[sl]));
+          } else {
+            const double u = fma(vr[TI], pj, rr[TI] * vj);
+            a[sl] -= (i > j ? 2.0 : (i == j ? 1.0 : 0.0)) * u;
+          }
         });
       }
     });
