@@ -419,88 +419,120 @@ __global__ __launch_bounds__(256) void k_force_atoms(Topo t, const int* order,
   const double* quad = x + ns + 3 * N;
   const int tid = threadIdx.x, nt = blockDim.x;
   __shared__ double red[16];
+  // Each atom's partner loops are split over G lanes (G as for the pair sums of
+  // the potentials), partial results combined with xor shuffles.
+  const int G = pair_lanes(N, nt), gl = tid & (G - 1), per = nt / G;
+  const int L = (N + G - 1) / G, Ls = (ns + G - 1) / G;
+  auto lanes_sum = [&](double& v) __attribute__((always_inline)) {
+    for (int off = G >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  };
   // phase 1: AES2 VJP (per atom, both pair orders) and D4 weights at the converged charges
-  for (int a = a0 + tid; a < a1; a += nt) {
+  for (int base = 0; base < N; base += per) {
+    const int la = base + tid / G;
+    const bool act = la < N;
+    const int a = a0 + (act ? la : 0);
     double g[3] = {0, 0, 0}, dc = 0.0;
-    for (int b = a0; b < a1; ++b) {
-      if (b == a) continue;
-      const int i = min(a, b), j = max(a, b);
-      double gi[3], dci, dcj;
-      aes2_pair_vjp(t, i, j, mrad, dmrad, qat, dip, quad, a0, gi, dci, dcj);
-      const double sg = (a == i) ? 1.0 : -1.0;
-      for (int c = 0; c < 3; ++c) g[c] += sg * gi[c];
-      dc += (a == i) ? dci : dcj;
+    if (act) {
+      for (int b = a0 + gl * L; b < a0 + min(N, (gl + 1) * L); ++b) {
+        if (b == a) continue;
+        const int i = min(a, b), j = max(a, b);
+        double gi[3], dci, dcj;
+        aes2_pair_vjp(t, i, j, mrad, dmrad, qat, dip, quad, a0, gi, dci, dcj);
+        const double sg = (a == i) ? 1.0 : -1.0;
+        for (int c = 0; c < 3; ++c) g[c] += sg * gi[c];
+        dc += (a == i) ? dci : dcj;
+      }
     }
-    for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
-    dedcn[a] += dc;
-    d4_weights(t, a, cnd4[a], qat[a], w4 + kD4MaxRef * a, dw4c + kD4MaxRef * a, nullptr);
+    for (int c = 0; c < 3; ++c) lanes_sum(g[c]);
+    lanes_sum(dc);
+    if (act && gl == 0) {
+      for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
+      dedcn[a] += dc;
+      d4_weights(t, a, cnd4[a], qat[a], w4 + kD4MaxRef * a, dw4c + kD4MaxRef * a, nullptr);
+    }
   }
   __syncthreads();
   // phase 2: D4 two-body explicit gradient and D4-CN adjoint
-  for (int a = a0 + tid; a < a1; a += nt) {
+  for (int base = 0; base < N; base += per) {
+    const int la = base + tid / G;
+    const bool act = la < N;
+    const int a = a0 + (act ? la : 0);
     const double* ra = t.xyz + 3 * a;
     double g[3] = {0, 0, 0}, d4cn = 0.0;
-    for (int b = a0; b < a1; ++b) {
-      if (b == a) continue;
-      const double* rb = t.xyz + 3 * b;
-      const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
-      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-      double damp, ddamp;
-      d4_damping(t, a, b, r2, damp, ddamp);
-      if (damp == 0.0) continue;
-      const double c6 = d4_contract(t, a, b, w4 + kD4MaxRef * a, w4 + kD4MaxRef * b);
-      const double dc6 = d4_contract(t, a, b, dw4c + kD4MaxRef * a, w4 + kD4MaxRef * b);
-      for (int c = 0; c < 3; ++c) g[c] += -c6 * ddamp * r[c];
-      d4cn -= dc6 * damp;
+    if (act) {
+      for (int b = a0 + gl * L; b < a0 + min(N, (gl + 1) * L); ++b) {
+        if (b == a) continue;
+        const double* rb = t.xyz + 3 * b;
+        const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+        const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        double damp, ddamp;
+        d4_damping(t, a, b, r2, damp, ddamp);
+        if (damp == 0.0) continue;
+        const double c6 = d4_contract(t, a, b, w4 + kD4MaxRef * a, w4 + kD4MaxRef * b);
+        const double dc6 = d4_contract(t, a, b, dw4c + kD4MaxRef * a, w4 + kD4MaxRef * b);
+        for (int c = 0; c < 3; ++c) g[c] += -c6 * ddamp * r[c];
+        d4cn -= dc6 * damp;
+      }
     }
-    for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
-    dedcn4[a] = d4cn;
+    for (int c = 0; c < 3; ++c) lanes_sum(g[c]);
+    lanes_sum(d4cn);
+    if (act && gl == 0) {
+      for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
+      dedcn4[a] = d4cn;
+    }
   }
   __syncthreads();
   // phase 3: CN chains, ES2, repulsion (per atom, symmetric pair formulas)
   double erp = 0.0;
-  for (int a = a0 + tid; a < a1; a += nt) {
+  const double* Gm = gam + t.gam_off[s];
+  for (int base = 0; base < N; base += per) {
+    const int la = base + tid / G;
+    const bool act = la < N;
+    const int a = a0 + (act ? la : 0);
     const double* ra = t.xyz + 3 * a;
     double g[3] = {0, 0, 0};
-    for (int b = a0; b < a1; ++b) {
-      if (b == a) continue;
-      const double* rb = t.xyz + 3 * b;
-      const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
-      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-      const double rr = sqrt(r2);
-      double sc = 0.0;
-      if (r2 <= 625.0) {
-        double val, der;
-        dexp_count(rr, t.at_cnrad[a] + t.at_cnrad[b], val, der);
-        sc += (dedcn[a] + dedcn[b]) * der / rr;
-        // repulsion (reference repulsion.cpp:127-176)
-        const bool light = t.Z[a] <= 2 && t.Z[b] <= 2;
-        const double k = light ? 1.0 : 1.5;
-        const double rk = light ? rr : rr * sqrt(rr);
-        const double al = t.at_sqrt_arep[a] * t.at_sqrt_arep[b];
-        const double pe = t.at_zeff[a] * t.at_zeff[b] * exp(-al * rk) / rr;
-        if (a < b) erp += pe;
-        sc -= (al * k * rk + 1.0) * pe / r2;
+    if (act) {
+      for (int b = a0 + gl * L; b < a0 + min(N, (gl + 1) * L); ++b) {
+        if (b == a) continue;
+        const double* rb = t.xyz + 3 * b;
+        const double r[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+        const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        const double rr = sqrt(r2);
+        double sc = 0.0;
+        if (r2 <= 625.0) {
+          double val, der;
+          dexp_count(rr, t.at_cnrad[a] + t.at_cnrad[b], val, der);
+          sc += (dedcn[a] + dedcn[b]) * der / rr;
+          // repulsion (reference repulsion.cpp:127-176)
+          const bool light = t.Z[a] <= 2 && t.Z[b] <= 2;
+          const double k = light ? 1.0 : 1.5;
+          const double rk = light ? rr : rr * sqrt(rr);
+          const double al = t.at_sqrt_arep[a] * t.at_sqrt_arep[b];
+          const double pe = t.at_zeff[a] * t.at_zeff[b] * exp(-al * rk) / rr;
+          if (a < b) erp += pe;
+          sc -= (al * k * rk + 1.0) * pe / r2;
+        }
+        if (r2 <= 900.0) {
+          double val, der;
+          d4_cn_count(t, a, b, rr, val, der);
+          sc += (dedcn4[a] + dedcn4[b]) * der / rr;
+        }
+        for (int c = 0; c < 3; ++c) g[c] += sc * r[c];
       }
-      if (r2 <= 900.0) {
-        double val, der;
-        d4_cn_count(t, a, b, rr, val, der);
-        sc += (dedcn4[a] + dedcn4[b]) * der / rr;
+      // ES2 at fixed q: -q_s q_t Gamma^3 (R_A - R_C) per shell pair on different atoms
+      for (int si = t.atom_sh0[a]; si < t.atom_sh0[a + 1]; ++si) {
+        for (int sj = s0 + gl * Ls; sj < s0 + min(ns, (gl + 1) * Ls); ++sj) {
+          const int C = t.sh_atom[sj];
+          if (C == a) continue;
+          const double gm = Gm[(si - s0) * ns + (sj - s0)];
+          const double f = -qsh[si - s0] * qsh[sj - s0] * gm * gm * gm;
+          for (int c = 0; c < 3; ++c) g[c] += f * (ra[c] - t.xyz[3 * C + c]);
+        }
       }
-      for (int c = 0; c < 3; ++c) g[c] += sc * r[c];
     }
-    // ES2 at fixed q: -q_s q_t Gamma^3 (R_A - R_C) per shell pair on different atoms
-    const double* G = gam + t.gam_off[s];
-    for (int si = t.atom_sh0[a]; si < t.atom_sh0[a + 1]; ++si) {
-      for (int sj = s0; sj < s1; ++sj) {
-        const int C = t.sh_atom[sj];
-        if (C == a) continue;
-        const double gm = G[(si - s0) * ns + (sj - s0)];
-        const double f = -qsh[si - s0] * qsh[sj - s0] * gm * gm * gm;
-        for (int c = 0; c < 3; ++c) g[c] += f * (ra[c] - t.xyz[3 * C + c]);
-      }
-    }
-    for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
+    for (int c = 0; c < 3; ++c) lanes_sum(g[c]);
+    if (act && gl == 0)
+      for (int c = 0; c < 3; ++c) grad[3 * a + c] += g[c];
   }
   erp = block_sum(erp, red);
   if (tid == 0) erep[s] = erp;
