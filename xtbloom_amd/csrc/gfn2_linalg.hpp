@@ -703,15 +703,29 @@ __device__ __forceinline__ int sturm_count_logd(const double* d, const double* e
   int c = q <= 0.0 ? 1 : 0;
   double r = fast_div(1.0, q), dq = -1.0;
   double acc = dq * r;
-  for (int i = b0 + 1; i < b1; ++i) {
-    const double tq = e2[i - 1] * r;
+  auto step = [&](double di, double ei2) __attribute__((always_inline)) {
+    const double tq = ei2 * r;
     dq = fma(tq * r, dq, -1.0);
-    q = (d[i] - x) - tq;
+    q = (di - x) - tq;
     if (fabs(q) < pivmin) q = -pivmin;
     c += q <= 0.0 ? 1 : 0;
     r = fast_div(1.0, q);
     acc = fma(dq, r, acc);
+  };
+  // the recurrence is a serial chain: load eight entries of d and e^2 ahead so
+  // that LDS latency stays off it
+  int i = b0 + 1;
+  for (; i + 7 < b1; i += 8) {
+    double dv[8], ev[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      dv[u] = d[i + u];
+      ev[u] = e2[i - 1 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) step(dv[u], ev[u]);
   }
+  for (; i < b1; ++i) step(d[i], e2[i - 1]);
   L = acc;
   return c;
 }
@@ -730,7 +744,7 @@ __device__ __forceinline__ int sturm_count_logd(const double* d, const double* e
 struct EigState {
   int j, b0, b1, k, it, phase;  // phase 0/1: guess bracket ends, 2: main iteration
   int clo, chi;
-  double lo, hi, x, tol, guess, dl;
+  double lo, hi, x, tol, guess, dl, dh;
   double dxo;  // size of the step before the last one (Newton acceptance)
 };
 
@@ -754,7 +768,7 @@ __device__ __forceinline__ void block_bounds(const double* d, const double* e, i
 __device__ __forceinline__ bool eig_start(EigState& st, int j, const double* d, const double* e,
                                           int n, double pivmin, bool unsplit, double glo,
                                           double ghi, double gtol, bool guess_ok, double guess,
-                                          double delta, double* lam) {
+                                          double gprev, double gnext, double delta, double* lam) {
   int b0 = 0, b1 = n;
   if (!unsplit) {
     b0 = j;
@@ -776,7 +790,12 @@ __device__ __forceinline__ bool eig_start(EigState& st, int j, const double* d, 
   st.chi = b1 - b0;
   const bool g = guess_ok && guess > st.lo && guess < st.hi;
   st.guess = guess;
-  st.dl = fmax(delta, 4.0 * st.tol);
+  // guess bracket: at most the largest change of the last round, and at most
+  // half the distance to the neighbouring guesses, so that the bracket usually
+  // isolates the eigenvalue at once (Newton then starts from the guess)
+  const double dmin = 4.0 * st.tol;
+  st.dl = fmax(fmin(delta, 0.5 * (guess - gprev)), dmin);
+  st.dh = fmax(fmin(delta, 0.5 * (gnext - guess)), dmin);
   st.phase = g ? 0 : 2;
   st.x = g ? guess - st.dl : 0.5 * (st.lo + st.hi);
   st.dxo = st.hi - st.lo;
@@ -795,7 +814,7 @@ __device__ __forceinline__ bool eig_step(EigState& st, const double* d, const do
     }
     if (st.phase == 0) {
       st.phase = 1;
-      st.x = st.guess + st.dl;
+      st.x = st.guess + st.dh;
       if (st.x < st.hi) return false;
     }
     st.phase = 2;
@@ -867,47 +886,85 @@ __device__ __forceinline__ bool twisted_vector(const double* d, const double* e,
                                                double x, double pivmin, int n, int jj,
                                                double* __restrict__ Dp, double* __restrict__ Dm,
                                                double* __restrict__ Z, double& scale) {
+  // Every loop below is a serial chain; the entries it consumes (d, e from
+  // LDS, pivots from global memory) are loaded eight at a time ahead of it.
+  constexpr int U = 8;
+  auto at = [&](int i) __attribute__((always_inline)) { return (int64_t)i * n + jj; };
   double q = d[b0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
-  Dp[(int64_t)b0 * n + jj] = q;
-#pragma unroll 4
-  for (int i = b0 + 1; i < b1; ++i) {
-    q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
-    if (fabs(q) < pivmin) q = -pivmin;
-    Dp[(int64_t)i * n + jj] = q;
+  Dp[at(b0)] = q;
+  {
+    auto step = [&](int i, double di, double ei) __attribute__((always_inline)) {
+      q = (di - x) - fast_div(ei * ei, q);
+      if (fabs(q) < pivmin) q = -pivmin;
+      Dp[at(i)] = q;
+    };
+    int i = b0 + 1;
+    for (; i + U - 1 < b1; i += U) {
+      double dv[U], ev[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) { dv[u] = d[i + u]; ev[u] = e[i - 1 + u]; }
+#pragma unroll
+      for (int u = 0; u < U; ++u) step(i + u, dv[u], ev[u]);
+    }
+    for (; i < b1; ++i) step(i, d[i], e[i - 1]);
   }
   q = d[b1 - 1] - x;
   if (fabs(q) < pivmin) q = -pivmin;
-  Dm[(int64_t)(b1 - 1) * n + jj] = q;
+  Dm[at(b1 - 1)] = q;
   int r = b1 - 1;
-  double gmin = fabs(Dp[(int64_t)(b1 - 1) * n + jj] + q - (d[b1 - 1] - x));
-#pragma unroll 4
-  for (int i = b1 - 2; i >= b0; --i) {
-    q = (d[i] - x) - e[i] * e[i] / q;
-    if (fabs(q) < pivmin) q = -pivmin;
-    Dm[(int64_t)i * n + jj] = q;
-    const double g = fabs(Dp[(int64_t)i * n + jj] + q - (d[i] - x));
-    if (g < gmin) {
-      gmin = g;
-      r = i;
+  double gmin = fabs(Dp[at(b1 - 1)] + q - (d[b1 - 1] - x));
+  {
+    auto step = [&](int i, double di, double ei, double dpi) __attribute__((always_inline)) {
+      q = (di - x) - fast_div(ei * ei, q);
+      if (fabs(q) < pivmin) q = -pivmin;
+      Dm[at(i)] = q;
+      const double g = fabs(dpi + q - (di - x));
+      if (g < gmin) {
+        gmin = g;
+        r = i;
+      }
+    };
+    int i = b1 - 2;
+    for (; i - U + 1 >= b0; i -= U) {
+      double dv[U], ev[U], pv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) { dv[u] = d[i - u]; ev[u] = e[i - u]; pv[u] = Dp[at(i - u)]; }
+#pragma unroll
+      for (int u = 0; u < U; ++u) step(i - u, dv[u], ev[u], pv[u]);
     }
+    for (; i >= b0; --i) step(i, d[i], e[i], Dp[at(i)]);
   }
   double z = 1.0, nrm = 1.0, big = 1.0;
-  Z[(int64_t)r * n + jj] = 1.0;
-#pragma unroll 4
-  for (int i = r - 1; i >= b0; --i) {
-    z = -(e[i] / Dp[(int64_t)i * n + jj]) * z;
-    Z[(int64_t)i * n + jj] = z;
+  Z[at(r)] = 1.0;
+  auto put = [&](int i, double ratio) __attribute__((always_inline)) {
+    z = -ratio * z;
+    Z[at(i)] = z;
     nrm += z * z;
     if (fabs(z) > fabs(big)) big = z;
+  };
+  {
+    int i = r - 1;
+    for (; i - U + 1 >= b0; i -= U) {
+      double rv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) rv[u] = e[i - u] / Dp[at(i - u)];
+#pragma unroll
+      for (int u = 0; u < U; ++u) put(i - u, rv[u]);
+    }
+    for (; i >= b0; --i) put(i, e[i] / Dp[at(i)]);
   }
   z = 1.0;
-#pragma unroll 4
-  for (int i = r + 1; i < b1; ++i) {
-    z = -(e[i - 1] / Dm[(int64_t)i * n + jj]) * z;
-    Z[(int64_t)i * n + jj] = z;
-    nrm += z * z;
-    if (fabs(z) > fabs(big)) big = z;
+  {
+    int i = r + 1;
+    for (; i + U - 1 < b1; i += U) {
+      double rv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) rv[u] = e[i - 1 + u] / Dm[at(i + u)];
+#pragma unroll
+      for (int u = 0; u < U; ++u) put(i + u, rv[u]);
+    }
+    for (; i < b1; ++i) put(i, e[i - 1] / Dm[at(i)]);
   }
   if (!isfinite(nrm) || !(nrm > 0.0)) return false;
   scale = (big < 0.0 ? -1.0 : 1.0) / sqrt(nrm);
@@ -1093,9 +1150,12 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
       if (!busy) {
         const int j = atomicAdd(&next_j, 1);
         if (j >= n) break;
-        const double g = have_old ? wall[t.ao_off[s] + j] : 0.0;
+        const double* wo = wall + t.ao_off[s];
+        const double g = have_old ? wo[j] : 0.0;
+        const double gp = (have_old && j > 0) ? wo[j - 1] : -1e300;
+        const double gn = (have_old && j + 1 < n) ? wo[j + 1] : 1e300;
         busy = eig_start(st, j, d, e, n, pivmin, unsplit, gbound[0], gbound[1], gbound[2], guess_ok,
-                         g, delta, lam);
+                         g, gp, gn, delta, lam);
         if (!busy) continue;
       }
       double x;
