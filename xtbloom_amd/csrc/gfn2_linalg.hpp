@@ -1521,46 +1521,39 @@ __device__ __forceinline__ void backtransform_registers(const double* __restrict
       }
     }
     __syncthreads();
+    // w = V^T z for reflector u, column c (thread (u, c))
+    const int ur = tid >> 6, cr = tid & 63;
     {
-      const int u = tid >> 6, c = tid & 63;
       double acc = 0.0;
 #pragma unroll
-      for (int g = 0; g < 8; ++g) acc += part[(u * 8 + g) * 64 + c];
-      wf[u * 64 + c] = acc;
+      for (int g = 0; g < 8; ++g) acc += part[(ur * 8 + g) * 64 + cr];
+      wf[ur * 64 + cr] = acc;
     }
     __syncthreads();
-    // y = T w (T lower triangular; in place, last row first)
-    double y[kBackChunk][kCols];
+    // y = T w (T lower triangular), thread (u, c); y goes to part[u][0][c]
+    {
+      double acc = 0.0;
+      for (int i = 0; i <= ur; ++i) acc = fma(tt[ur * kBackChunk + i], wf[i * 64 + cr], acc);
+      part[ur * 8 * 64 + cr] = acc;
+    }
+    __syncthreads();
+    // z -= V y, one reflector at a time (only its four y values live)
 #pragma unroll
-    for (int u = 0; u < kBackChunk; ++u)
+    for (int u = 0; u < kBackChunk; ++u) {
+      double yj[kCols];
 #pragma unroll
-      for (int j = 0; j < kCols; ++j) y[u][j] = wf[u * 64 + kCols * cg + j];
+      for (int j = 0; j < kCols; ++j) yj[j] = part[u * 8 * 64 + kCols * cg + j];
 #pragma unroll
-    for (int m = kBackChunk - 1; m >= 0; --m)
-#pragma unroll
-      for (int j = 0; j < kCols; ++j) {
-        double acc = 0.0;
-#pragma unroll
-        for (int i = 0; i <= m; ++i) acc = fma(tt[m * kBackChunk + i], y[i][j], acc);
-        y[m][j] = acc;
-      }
-    // z -= V y.  Empty asm with a memory clobber pins the program order so the
-    // reflector entries of one row are loaded only when that row is updated
-    // (hoisting all 64 up front spills the column block).
-#pragma unroll
-    for (int u = 0; u < kBackChunk; ++u)
-#pragma unroll
-      for (int j = 0; j < kCols; ++j) asm volatile("" : "+v"(y[u][j])::"memory");
-#pragma unroll
-    for (int q = 0; q < kRows; ++q) {
-#pragma unroll
-      for (int u = 0; u < kBackChunk; ++u) {
+      for (int q = 0; q < kRows; ++q) {
         const double h = hv[u * 256 + rgp + 32 * q];
 #pragma unroll
-        for (int j = 0; j < kCols; ++j) z[q][j] = fma(-h, y[u][j], z[q][j]);
+        for (int j = 0; j < kCols; ++j) z[q][j] = fma(-h, yj[j], z[q][j]);
       }
+      // Empty asm with a memory clobber pins the program order, so that only
+      // this reflector's entries are in flight (hoisting all of them spills).
 #pragma unroll
-      for (int j = 0; j < kCols; ++j) asm volatile("" : "+v"(z[q][j])::"memory");
+      for (int q = 0; q < kRows; ++q)
+        asm volatile("" : "+v"(z[q][0]), "+v"(z[q][1]), "+v"(z[q][2]), "+v"(z[q][3])::"memory");
     }
     if (more) stage(ch + 1, rn);
     __syncthreads();
@@ -1629,7 +1622,8 @@ __device__ __forceinline__ void backtransform_lds(const double* __restrict__ A,
   }
 }
 
-__global__ __launch_bounds__(kBackThreads) void k_backtransform(Topo t, const BackTask* tasks,
+// two blocks per CU (four wavefronts per SIMD): at most 128 VGPRs
+__global__ __launch_bounds__(kBackThreads, 4) void k_backtransform(Topo t, const BackTask* tasks,
                                                                 const double* Aall,
                                                                 const double* Zall,
                                                                 const double* Tall, double* Yall,
