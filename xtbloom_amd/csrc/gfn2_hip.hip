@@ -913,7 +913,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(probe_end(kKReduce, a3, 0.0));  // symmetric: lower tiles only
     // eigenpairs of H' (A is overwritten by the reflectors; T, H, C, P, W serve as scratch)
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL(k_tridiag, dim3(B), dim3(512), 0, sm, t, d_order, dA, d_td, d_te, d_active);
+    hipLaunchKernelGGL(k_tridiag, dim3(B), dim3(512), 0, sm, t, d_order, dA, d_td, d_te, d_wyT,
+                       d_active);
     HIP_TRY(probe_end(kKTridiag, (4.0 / 3.0) * a3, 0.0));
     HIP_TRY(probe_begin());
     // eigenvalues, occupations (kocc), eigenvectors of the kocc lowest levels
@@ -922,7 +923,6 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
                        FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt});
     HIP_TRY(probe_end(kKTridiagEigen, 0.0, 0.0));
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL(k_wy_factors, dim3(B), dim3(256), 0, sm, t, d_order, d_active, dA, d_wyT);
     hipLaunchKernelGGL(k_backtransform, dim3((unsigned)back.size()), dim3(kBackThreads), back_lds, sm, t, d_back,
                        dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
     HIP_TRY(probe_end(kKBacktransform, 0.0, 0.0));

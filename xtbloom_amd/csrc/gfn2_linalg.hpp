@@ -470,9 +470,79 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
   if (tid == 0) e[n - 1] = 0.0;
 }
 
+// The back-transformation applies the reflectors in compact WY blocks of
+// kBackChunk:  H_{k_7} ... H_{k_0} = I - V T V^T,  V = [v_{k_0} .. v_{k_7}],
+// k_u = n-3 - ch kBackChunk - u, T lower triangular with
+//   T_mm = 2,  T_mi = -2 sum_{l=i}^{m-1} (v_m . v_l) T_li  (i < m),
+// which follows from H_m (I - V T V^T) = I - [V v_m] [[T, 0], [-2 v_m^T V T, 2]] [V v_m]^T.
+// T_mi of block ch sits at Tall[wy_offset + 64 ch + 8 m + i]; a system holds
+// ceil((n-2)/8) blocks of 64 entries, at most 8 n + 40.
+constexpr int kBackChunk = 8;  // reflectors per WY block
+__device__ __forceinline__ int64_t wy_offset(const Topo& t, int s) {
+  return (int64_t)kBackChunk * t.ao_off[s] + (int64_t)kBackChunk * kBackChunk * s;
+}
+
+// T factors of all WY blocks of one system from its stored reflectors (A's
+// strict lower part, column k = reflector k), by the whole block: one thread
+// per (block, pair) inner product over rows > k_i (where both reflectors live),
+// then one thread per block runs the recurrence.  gsh holds 28 doubles per block.
+__device__ __forceinline__ void wy_factors(const double* A, int n, double* Ts, double* gsh) {
+  constexpr int kPairs = kBackChunk * (kBackChunk - 1) / 2;  // 28
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int nref = n - 2, nch = (nref + kBackChunk - 1) / kBackChunk;
+  for (int task = tid; task < nch * kPairs; task += nt) {
+    const int ch = task / kPairs;
+    int p = task - ch * kPairs, m = 1, i = 0;
+    while (p > 0) {
+      --p;
+      if (++i == m) { ++m; i = 0; }
+    }
+    const int kb = n - 3 - ch * kBackChunk, km = kb - m, ki = kb - i;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (km >= 0) {
+      const double* vm = A + (int64_t)km * n;
+      const double* vi = A + (int64_t)ki * n;
+      int r = ki + 1;
+      for (; r + 3 < n; r += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = fma(vm[r + u], vi[r + u], acc[u]);
+      }
+      for (; r < n; ++r) acc[0] = fma(vm[r], vi[r], acc[0]);
+    }
+    gsh[task] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  }
+  __syncthreads();
+  for (int ch = tid; ch < nch; ch += nt) {
+    const int kb = n - 3 - ch * kBackChunk;
+    const double* g = gsh + ch * kPairs;  // (v_m . v_l) at m (m-1) / 2 + l
+    double T[kBackChunk][kBackChunk];
+#pragma unroll
+    for (int m = 0; m < kBackChunk; ++m) {
+#pragma unroll
+      for (int i = 0; i < kBackChunk; ++i) T[m][i] = 0.0;
+      if (kb - m < 0) continue;  // past the last reflector: zero row
+      T[m][m] = 2.0;
+#pragma unroll
+      for (int i = 0; i < m; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int l = i; l < m; ++l) acc = fma(g[m * (m - 1) / 2 + l], T[l][i], acc);
+        T[m][i] = -2.0 * acc;
+      }
+    }
+    double* To = Ts + ch * kBackChunk * kBackChunk;
+#pragma unroll
+    for (int m = 0; m < kBackChunk; ++m)
+#pragma unroll
+      for (int i = 0; i < kBackChunk; ++i) To[m * kBackChunk + i] = T[m][i];
+  }
+  __syncthreads();
+}
+
 // 512 threads per system (two waves per SIMD, 256 VGPRs per thread)
 __global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, double* Aall, double* dall,
-                                                 double* eall, const unsigned char* active) {
+                                                 double* eall, double* Tall,
+                                                 const unsigned char* active) {
   const int s = order[blockIdx.x];
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
@@ -491,6 +561,12 @@ __global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, doubl
   else if (n <= 192) tridiag_registers<12>(A, n, d, e, v, col, pq, rowpart, colpart);
   else if (n <= 256) tridiag_registers<16>(A, n, d, e, v, col, pq, rowpart, colpart);
   else tridiag_global(A, n, d, e, v, pq, red, bc);
+  // WY factors for the register back-transformation (n <= 256); the stored
+  // reflectors must be visible to the whole block first
+  if (n <= 256 && n > 2) {
+    __syncthreads();
+    wy_factors(A, n, Tall + wy_offset(t, s), rowpart);
+  }
 }
 
 // Overlap Cholesky S = L L^T (L column-major into `L`) and X = L^{-T} (row-major,
@@ -1349,7 +1425,7 @@ struct BackTask {
 // (c, rg) = (tid % 64, tid / 64) holds rows rg, rg + 8, ...) and the
 // reflectors are applied kBackChunk at a time in compact WY form,
 //   H_{k_7} ... H_{k_0} = I - V T V^T,   V = [v_{k_0} .. v_{k_7}],
-// with k_u = n-3 - ch kBackChunk - u and T lower triangular (k_wy_factors):
+// with k_u = n-3 - ch kBackChunk - u and T lower triangular (wy_factors):
 // one LDS reduction (V^T z for the whole chunk) and two barriers per chunk.
 // Reflectors are staged in LDS zero-padded to absolute row indices, so every
 // thread applies every reflector without masks; the next chunk is fetched into
@@ -1358,90 +1434,8 @@ struct BackTask {
 // a time.
 constexpr int kBackThreads = 512;
 constexpr int kBackRegRows = 32;  // rows per thread of the register path (n <= 256)
-constexpr int kBackChunk = 8;     // reflectors per WY block
 constexpr size_t kBackStaticLds =
     (2 * kBackChunk * 256 + kBackChunk * kBackThreads + kBackChunk * 64) * sizeof(double);
-
-// T factor of every WY block of a system, T_mi at Tall[wy_offset + 64 ch + 8 m + i]
-// (a system holds ceil((n-2)/8) blocks of 64 entries: at most 8 n + 40):
-//   T_mm = 2,  T_mi = -2 sum_{l=i}^{m-1} (v_m . v_l) T_li  (i < m),
-// which follows from H_m (I - V T V^T) = I - [V v_m] [[T, 0], [-2 v_m^T V T, 2]] [V v_m]^T.
-// One wavefront per block; the inner products are over rows > k_i, where both
-// reflectors live.
-// start of system s's WY factors: 8 per AO of the preceding systems plus 64 per
-// preceding system covers the 8 n + 40 entries each one needs
-__device__ __forceinline__ int64_t wy_offset(const Topo& t, int s) {
-  return (int64_t)kBackChunk * t.ao_off[s] + (int64_t)kBackChunk * kBackChunk * s;
-}
-
-// One block per system, one WY block per wavefront at a time: the wavefront's
-// eight reflectors are staged in LDS, lane p + 32 h (p < 28) forms the inner
-// product of pair p over rows of parity h, and lane 0 runs the recurrence.
-__global__ __launch_bounds__(256) void k_wy_factors(Topo t, const int* order,
-                                                    const unsigned char* active, const double* Aall,
-                                                    double* Tall) {
-  const int s = order[blockIdx.x];
-  if (active && !active[s]) return;
-  const int n = t.ao_off[s + 1] - t.ao_off[s];
-  if (n > 256) return;  // the LDS path applies reflectors one at a time
-  const double* A = Aall + t.mat_off[s];
-  double* Ts = Tall + wy_offset(t, s);
-  constexpr int kPairs = kBackChunk * (kBackChunk - 1) / 2;  // 28
-  __shared__ double vs[4][kBackChunk][256];  // per wavefront, zero-padded to absolute rows
-  __shared__ double gs[4][32];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int nref = n - 2, nch = (nref + kBackChunk - 1) / kBackChunk;
-  // pair of this lane: p -> (m, i), m > i, in row-major order of the strict lower triangle
-  const int p = lane & 31, h = lane >> 5;
-  int pm = 1, pi = 0;
-  for (int q = 0; q < p && q < kPairs - 1; ++q) {
-    if (++pi == pm) { ++pm; pi = 0; }
-  }
-  for (int cg = 0; cg < nch; cg += 4) {
-    const int ch = cg + wid;  // >= nch: this wavefront idles but joins the barriers
-    const int kb = n - 3 - ch * kBackChunk;
-    for (int idx = lane; idx < kBackChunk * 256; idx += 64) {
-      const int u = idx >> 8, r = idx & 255, k = kb - u;
-      vs[wid][u][r] = (ch < nch && k >= 0 && r > k && r < n) ? A[r + (int64_t)k * n] : 0.0;
-    }
-    __syncthreads();
-    double acc = 0.0;
-    if (p < kPairs)
-      for (int r = h; r < n; r += 2) acc = fma(vs[wid][pm][r], vs[wid][pi][r], acc);
-    acc += __shfl_xor(acc, 32, 64);
-    if (h == 0 && p < kPairs) gs[wid][p] = acc;
-    __syncthreads();
-    if (lane == 0 && ch < nch) {
-      double g[kBackChunk][kBackChunk];
-      {
-        int q = 0;
-#pragma unroll
-        for (int m = 1; m < kBackChunk; ++m)
-#pragma unroll
-          for (int i = 0; i < m; ++i) g[m][i] = gs[wid][q++];
-      }
-      double T[kBackChunk][kBackChunk];
-#pragma unroll
-      for (int m = 0; m < kBackChunk; ++m) {
-#pragma unroll
-        for (int i = 0; i < kBackChunk; ++i) T[m][i] = 0.0;
-        if (kb - m < 0) continue;  // past the last reflector: zero row
-        T[m][m] = 2.0;
-#pragma unroll
-        for (int i = 0; i < m; ++i) {
-          double tacc = 0.0;
-#pragma unroll
-          for (int l = i; l < m; ++l) tacc = fma(g[m][l], T[l][i], tacc);
-          T[m][i] = -2.0 * tacc;
-        }
-      }
-#pragma unroll
-      for (int m = 0; m < kBackChunk; ++m)
-#pragma unroll
-        for (int i = 0; i < kBackChunk; ++i) Ts[ch * kBackChunk * kBackChunk + m * kBackChunk + i] = T[m][i];
-    }
-  }
-}
 
 __device__ __forceinline__ void backtransform_registers(const double* __restrict__ A,
                                                         const double* __restrict__ Z,
