@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="ragged512")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-stride", type=int, default=4,
+    ap.add_argument("--cpu-stride", type=int, default=2,
                     help="CPU baseline / parity sample: every k-th system")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
