@@ -81,7 +81,7 @@ def cpu_baseline(batch, gpu_out, sample_stride):
         dit = max(dit, abs(int(it[i]) - int(ref["iterations"][k])))
     atoms = int(sub.atom_offsets[-1])
     base = {"value": len(idx) / wall, "unit": "systems/s", "cores": threads, "kind": "port",
-            "sample": f"{len(idx)} of {batch.batch_size} systems (every {sample_stride}th, "
+            "sample": f"{len(idx)} of {batch.batch_size} systems (stride {sample_stride}, "
                       f"{atoms} atoms), FP64 C restatement, one system per thread, "
                       f"{wall:.2f} s wall"}
     parity = {"systems": len(idx), "max_abs_dE": de, "max_abs_dF": df, "max_abs_dq": dq,

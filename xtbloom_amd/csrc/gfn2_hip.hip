@@ -902,15 +902,15 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(probe_end(kKFock, 0.0, 96.0 * a2));  // reads S, D(3), Q(6), H0; writes H
     // H' = X^T H X
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL((k_gemm<true, false, false>), dim3(nfull), dim3(256), 0, sm, t, d_full, dX,
-                       dH, dT, (const double*)nullptr, (const int*)nullptr, (const int*)nullptr,
-                       d_active);
-    HIP_TRY(probe_end(kKReduce, 2.0 * a3, 0.0));
+    hipLaunchKernelGGL((k_gemm<true, false, false, 1>), dim3(nfull), dim3(256), 0, sm, t, d_full,
+                       dX, dH, dT, (const double*)nullptr, (const int*)nullptr,
+                       (const int*)nullptr, d_active);
+    HIP_TRY(probe_end(kKReduce, a3, 0.0));  // X upper triangular: n^3
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL((k_gemm<false, false, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower,
+    hipLaunchKernelGGL((k_gemm<false, false, true, 2>), dim3(nlower), dim3(256), 0, sm, t, d_lower,
                        dT, dX, dA, (const double*)nullptr, (const int*)nullptr,
                        (const int*)nullptr, d_active);
-    HIP_TRY(probe_end(kKReduce, a3, 0.0));  // symmetric: lower tiles only
+    HIP_TRY(probe_end(kKReduce, a3 / 3.0, 0.0));  // symmetric, X upper triangular: n^3/3
     // eigenpairs of H' (A is overwritten by the reflectors; T, H, C, P, W serve as scratch)
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_tridiag, dim3(B), dim3(512), 0, sm, t, d_order, dA, d_td, d_te, d_wyT,
@@ -929,8 +929,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     pr_bt = probes.size();
     // C = X Y (the kocc leading columns), P = C diag(f) C^T
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL((k_gemm<false, false, false>), dim3(nfull), dim3(256), 0, sm, t, d_full, dX,
-                       dY, dC, (const double*)nullptr, (const int*)nullptr, d_kocc, d_active);
+    hipLaunchKernelGGL((k_gemm<false, false, false, 3>), dim3(nfull), dim3(256), 0, sm, t, d_full,
+                       dX, dY, dC, (const double*)nullptr, (const int*)nullptr, d_kocc, d_active);
     HIP_TRY(probe_end(kKBack, 0.0, 0.0));
     pr_back = probes.size();
     HIP_TRY(probe_begin());
@@ -962,10 +962,11 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(hipStreamSynchronize(sm));
     if (ks) {
       // algorithmic flops of the kocc-limited products of this round:
-      // back-transformation 2 n^2 kocc, C = X Y 2 n^2 kocc, P (lower half) n^2 kocc
+      // back-transformation 2 n^2 kocc, C = X Y n^2 kocc (X upper triangular),
+      // P (lower half) n^2 kocc
       const double w = work_occ(round_active);
       probes[pr_bt - 1].flops = 2.0 * w;
-      probes[pr_back - 1].flops = 2.0 * w;
+      probes[pr_back - 1].flops = w;
       probes[pr_dens - 1].flops = w;
     }
     ++rounds;

@@ -25,7 +25,13 @@ constexpr int kTileK = 16;
 //   K per system: kdim[sys] if given, else n.  ncol (optional): tiles whose
 //   columns all lie at or past ncol[sys] are skipped.
 //   SYMM: only tiles with ti >= tj are queued; the kernel mirrors them.
-template <bool TA, bool TB, bool SYMM>
+//   TRI: the K range a tile needs when an operand is upper triangular (its
+//   strict lower part stored as zero, as for X = L^-T), so the skipped
+//   products are exact zeros:
+//     1  A read transposed and upper triangular (X^T H):  k < i0 + 64
+//     2  B upper triangular (T X):                         k < j0 + 64
+//     3  A upper triangular (X Y):                         k >= i0
+template <bool TA, bool TB, bool SYMM, int TRI = 0>
 __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, const double* A,
                                               const double* Bm, double* C, const double* scale,
                                               const int* kdim, const int* ncol,
@@ -50,7 +56,11 @@ __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, con
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
-  for (int k0 = 0; k0 < K; k0 += kTileK) {
+  int kbeg = 0, kend = K;
+  if constexpr (TRI == 1) kend = min(K, i0 + kTile);
+  if constexpr (TRI == 2) kend = min(K, j0 + kTile);
+  if constexpr (TRI == 3) kbeg = i0;
+  for (int k0 = kbeg; k0 < kend; k0 += kTileK) {
     // load A tile: rows i0..i0+63, k k0..k0+15 -> As[k][i]
     for (int idx = threadIdx.x; idx < kTile * kTileK; idx += 256) {
       int kk, ii;
