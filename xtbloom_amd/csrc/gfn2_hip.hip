@@ -774,11 +774,11 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   t.perm = d_perm;
   const bool big_systems = T.max_atoms > kPairStageAtoms;
   auto launch_potentials = [&](const unsigned char* mask) {
-    hipLaunchKernelGGL(k_potentials<true>, dim3(B), dim3(256), 0, sm, t, d_order, mask, mx.x,
+    hipLaunchKernelGGL(k_potentials<true>, dim3(B), dim3(512), 0, sm, t, d_order, mask, mx.x,
                        d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, d_dw4, d_vat, d_vsh, d_vd, d_vqq,
                        d_gstage, d_gstagei);
     if (big_systems)
-      hipLaunchKernelGGL(k_potentials<false>, dim3(B), dim3(256), 0, sm, t, d_order, mask, mx.x,
+      hipLaunchKernelGGL(k_potentials<false>, dim3(B), dim3(512), 0, sm, t, d_order, mask, mx.x,
                          d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, d_dw4, d_vat, d_vsh, d_vd,
                          d_vqq, d_gstage, d_gstagei);
   };
@@ -945,11 +945,11 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
                        d_mpart, d_colsum, mx.raw, d_st);
     HIP_TRY(probe_end(kKMulliken, 0.0, 88.0 * a2));  // reads P, S, D(3), Q(6), H0
     HIP_TRY(probe_begin());
-    hipLaunchKernelGGL(k_energy_mix<true>, dim3(B), dim3(256), 0, sm, t, d_order, d_active,
+    hipLaunchKernelGGL(k_energy_mix<true>, dim3(B), dim3(512), 0, sm, t, d_order, d_active,
                        d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_count, d_gstage,
                        d_gstagei);
     if (big_systems)
-      hipLaunchKernelGGL(k_energy_mix<false>, dim3(B), dim3(256), 0, sm, t, d_order, d_active,
+      hipLaunchKernelGGL(k_energy_mix<false>, dim3(B), dim3(512), 0, sm, t, d_order, d_active,
                          d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_count, d_gstage,
                          d_gstagei);
     HIP_TRY(probe_end(kKEnergyMix, 0.0, 0.0));

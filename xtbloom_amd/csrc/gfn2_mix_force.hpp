@@ -9,11 +9,11 @@ namespace xtb {
 // SCC energy from the raw multipoles (reference scc_driver.cpp:2105-2286),
 // modified Broyden step (reference common/scc_mixer.cpp:559-745) and the
 // convergence gate rms < qtol and |dF| < etol (scc_driver.cpp:869-893).
-// One 256-thread block per active system; the pair energies come from the
+// One 512-thread block per active system; the pair energies come from the
 // shared pair sums (STAGED as for k_potentials).
 // ---------------------------------------------------------------------------
 template <bool STAGED>
-__global__ __launch_bounds__(256) void k_energy_mix(Topo t, const int* order, unsigned char* active,
+__global__ __launch_bounds__(512) void k_energy_mix(Topo t, const int* order, unsigned char* active,
                                                     const double* cnd4, const double* mrad,
                                                     const double* gam, const double* vpc,
                                                     double* qat_raw, double* w4, MixerArrays mx,
@@ -30,7 +30,7 @@ __global__ __launch_bounds__(256) void k_energy_mix(Topo t, const int* order, un
   const double* dip = raw + ns;
   const double* quad = raw + ns + 3 * N;
   const int tid = threadIdx.x, nt = blockDim.x;
-  __shared__ double red[16];
+  __shared__ double red[64];  // block_sum_n: K values per wavefront
   // the atom staging area while the pair sums run, the Broyden beta matrix after
   __shared__ double lbuf[STAGED ? kPairStageDoubles * kPairStageAtoms : 64 * 64];
   __shared__ int libuf[STAGED ? kPairStageInts * kPairStageAtoms : 1];

@@ -408,7 +408,7 @@ __device__ __forceinline__ int pair_lanes(int N, int nt) {
 // STAGED = the instantiation for systems of at most kPairStageAtoms atoms; the
 // other one serves the rest from the global staging slice.
 template <bool STAGED>
-__global__ __launch_bounds__(256) void k_potentials(Topo t, const int* order,
+__global__ __launch_bounds__(512) void k_potentials(Topo t, const int* order,
                                                     const unsigned char* active, const double* xs,
                                                     const double* cnd4, const double* mrad,
                                                     const double* gam, const double* vpc,
