@@ -298,6 +298,7 @@ struct Device {
   char* arena = nullptr;
   size_t arena_bytes = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t round_ev[2] = {nullptr, nullptr};  // end of the SCC rounds of each parity
   std::vector<hipEvent_t> probe_pool;  // per-launch timing events, grow-only
   size_t lds_limit = 65536;            // dynamic LDS granted to the back-transformation
   // grow-only pinned staging: [0, 256) the per-round active count, then the
@@ -344,6 +345,8 @@ Device* device_create(int device_id, void* stream, std::string& error) {
        hipMemcpy(d->d4_refng, xtbp_d4_ref_ngauss, sizeof(int) * 262, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(d->kpair, xtbp_kpair, sizeof(double) * 9, hipMemcpyHostToDevice) == hipSuccess;
   for (int i = 0; i < 4 && ok; ++i) ok = hipEventCreate(&d->ev[i]) == hipSuccess;
+  for (int i = 0; i < 2 && ok; ++i)
+    ok = hipEventCreateWithFlags(&d->round_ev[i], hipEventDisableTiming) == hipSuccess;
   // gfx950 grants up to 160 KB of LDS to one workgroup; opt the back-transformation in
   int lds = 0;
   if (ok && hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device_id) == hipSuccess &&
@@ -370,6 +373,8 @@ void device_destroy(Device* d) {
   (void)hipFree(d->kpair);
   (void)hipFree(d->arena);
   if (d->pinned) (void)hipHostFree(d->pinned);
+  for (auto& e : d->round_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto& e : d->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : d->probe_pool) (void)hipEventDestroy(e);
@@ -837,7 +842,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   work(h_active, all2, all3);
 
   {
-    size_t need = 256;
+    size_t need = 256 + 2 * align_bytes<unsigned char>(B) + 2 * align_bytes<int>(B);
     for (const Upload& u : uploads) need += align_bytes<char>(u.bytes);
     // copies of an earlier call that failed midway may still read the staging
     HIP_TRY(hipStreamSynchronize(sm));
@@ -848,14 +853,29 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
       HIP_TRY(hipHostMalloc((void**)&dev->pinned, need + need / 4));
       dev->pinned_bytes = need + need / 4;
     }
-    size_t off = 256;
+    size_t off = 256 + 2 * align_bytes<unsigned char>(B) + 2 * align_bytes<int>(B);
     for (const Upload& u : uploads) {
       std::memcpy(dev->pinned + off, u.src, u.bytes);
       HIP_TRY(hipMemcpyAsync(u.dst, dev->pinned + off, u.bytes, hipMemcpyHostToDevice, sm));
       off += align_bytes<char>(u.bytes);
     }
   }
+  // pinned per-parity round results: active counts, and (with kernel stats)
+  // snapshots of the active mask and kocc at the end of each round
   int* h_count = reinterpret_cast<int*>(dev->pinned);
+  unsigned char* h_act_snap[2];
+  int* h_kocc_snap[2];
+  {
+    char* q = dev->pinned + 256;
+    for (int k = 0; k < 2; ++k) {
+      h_act_snap[k] = reinterpret_cast<unsigned char*>(q);
+      q += align_bytes<unsigned char>(B);
+    }
+    for (int k = 0; k < 2; ++k) {
+      h_kocc_snap[k] = reinterpret_cast<int*>(q);
+      q += align_bytes<int>(B);
+    }
+  }
   HIP_TRY(hipEventRecord(dev->ev[0], sm));
   // ---- geometry, integrals, overlap factorization ----
   HIP_TRY(probe_begin());
@@ -885,37 +905,88 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   }
 
   // ---- SCC rounds ----
+  // Round it+1 is enqueued before the host waits for round it's active count,
+  // so the device never idles on the host.  Once a round reports no active
+  // system the loop stops; the round already enqueued behind it finds every
+  // system inactive and its blocks return at once.  Per-round work figures
+  // of the kernel stats are filled in when the round's active set and kocc
+  // are known (one round later).
   const unsigned nfull = (unsigned)full.size(), nlower = (unsigned)lower.size();
   int rounds = 0;
-  for (int it = 0; it < opt.max_iterations; ++it) {
+  struct RoundProbes {
+    size_t fock, red1, red2, tri, mull, bt, back, dens;  // probe index + 1 (0: none)
+  };
+  std::vector<RoundProbes> rprobes;
+  std::vector<std::vector<unsigned char>> start_set;  // active set at the start of each round
+  if (ks) start_set.push_back(h_active);
+  auto last_probe = [&]() -> size_t { return ks ? probes.size() : 0; };
+  auto patch_round = [&](int r, const std::vector<int>& kocc_after) {
+    const RoundProbes& rp = rprobes[r];
     double a2 = 0.0, a3 = 0.0;
-    if (ks) work(h_active, a2, a3);
-    const std::vector<unsigned char> round_active = ks ? h_active : std::vector<unsigned char>();
-    size_t pr_bt = 0, pr_back = 0, pr_dens = 0;  // probes whose work depends on kocc
-    HIP_TRY(hipMemsetAsync(d_count, 0, sizeof(int), sm));
+    work(start_set[r], a2, a3);
+    h_kocc = kocc_after;
+    const double w = work_occ(start_set[r]);
+    auto set = [&](size_t idx, double flops, double bytes) {
+      if (idx) { probes[idx - 1].flops = flops; probes[idx - 1].bytes = bytes; }
+    };
+    set(rp.fock, 0.0, 96.0 * a2);      // reads S, D(3), Q(6), H0; writes H
+    set(rp.red1, a3, 0.0);             // X^T H, X upper triangular: n^3
+    set(rp.red2, a3 / 3.0, 0.0);       // (X^T H) X, symmetric: n^3/3
+    set(rp.tri, (4.0 / 3.0) * a3, 0.0);
+    set(rp.mull, 0.0, 88.0 * a2);      // reads P, S, D(3), Q(6), H0
+    // back-transformation 2 n^2 kocc, C = X Y n^2 kocc (X upper triangular),
+    // P (lower half) n^2 kocc
+    set(rp.bt, 2.0 * w, 0.0);
+    set(rp.back, w, 0.0);
+    set(rp.dens, w, 0.0);
+  };
+  // waits for round r and records its completion; `count` receives the number
+  // of systems still active after it (the return value is a status code)
+  auto finish_round = [&](int r, int& count) -> int {
+    const int par = r & 1;
+    HIP_TRY(hipEventSynchronize(dev->round_ev[par]));
+    ++rounds;
+    if (ks) {
+      start_set.emplace_back(h_act_snap[par], h_act_snap[par] + B);
+      patch_round(r, std::vector<int>(h_kocc_snap[par], h_kocc_snap[par] + B));
+    }
+    count = h_count[par];
+    return 0;
+  };
+  int enqueued = 0;
+  bool stopped = false;
+  for (int it = 0; it < opt.max_iterations; ++it) {
+    const int par = it & 1;
+    int* d_cnt = d_count + par;
+    RoundProbes rp{};
+    HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(int), sm));
     HIP_TRY(probe_begin());
     launch_potentials(d_active);
     HIP_TRY(probe_end(kKPotentials, 0.0, 0.0));
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_fock, dim3((unsigned)chunks.size()), dim3(256), 0, sm, t, d_chunks,
                        d_active, dS, dD, dQ, dH0, d_vsh, d_vd, d_vqq, dH);
-    HIP_TRY(probe_end(kKFock, 0.0, 96.0 * a2));  // reads S, D(3), Q(6), H0; writes H
+    HIP_TRY(probe_end(kKFock, 0.0, 0.0));
+    rp.fock = last_probe();
     // H' = X^T H X
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<true, false, false, 1>), dim3(nfull), dim3(256), 0, sm, t, d_full,
                        dX, dH, dT, (const double*)nullptr, (const int*)nullptr,
                        (const int*)nullptr, d_active);
-    HIP_TRY(probe_end(kKReduce, a3, 0.0));  // X upper triangular: n^3
+    HIP_TRY(probe_end(kKReduce, 0.0, 0.0));
+    rp.red1 = last_probe();
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<false, false, true, 2>), dim3(nlower), dim3(256), 0, sm, t, d_lower,
                        dT, dX, dA, (const double*)nullptr, (const int*)nullptr,
                        (const int*)nullptr, d_active);
-    HIP_TRY(probe_end(kKReduce, a3 / 3.0, 0.0));  // symmetric, X upper triangular: n^3/3
+    HIP_TRY(probe_end(kKReduce, 0.0, 0.0));
+    rp.red2 = last_probe();
     // eigenpairs of H' (A is overwritten by the reflectors; T, H, C, P, W serve as scratch)
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_tridiag, dim3(B), dim3(512), 0, sm, t, d_order, dA, d_td, d_te, d_wyT,
                        d_active);
-    HIP_TRY(probe_end(kKTridiag, (4.0 / 3.0) * a3, 0.0));
+    HIP_TRY(probe_end(kKTridiag, 0.0, 0.0));
+    rp.tri = last_probe();
     HIP_TRY(probe_begin());
     // eigenvalues, occupations (kocc), eigenvectors of the kocc lowest levels
     hipLaunchKernelGGL(k_tridiag_eigen, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_td, d_te,
@@ -926,51 +997,59 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     hipLaunchKernelGGL(k_backtransform, dim3((unsigned)back.size()), dim3(kBackThreads), back_lds, sm, t, d_back,
                        dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
     HIP_TRY(probe_end(kKBacktransform, 0.0, 0.0));
-    pr_bt = probes.size();
+    rp.bt = last_probe();
     // C = X Y (the kocc leading columns), P = C diag(f) C^T
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<false, false, false, 3>), dim3(nfull), dim3(256), 0, sm, t, d_full,
                        dX, dY, dC, (const double*)nullptr, (const int*)nullptr, d_kocc, d_active);
     HIP_TRY(probe_end(kKBack, 0.0, 0.0));
-    pr_back = probes.size();
+    rp.back = last_probe();
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
                        dC, dP, d_occ, d_kocc, (const int*)nullptr, d_active);
     HIP_TRY(probe_end(kKDensity, 0.0, 0.0));
-    pr_dens = probes.size();
+    rp.dens = last_probe();
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_mulliken_rows, dim3((unsigned)rowchunks.size()), dim3(256), 0, sm, t,
                        d_rowchunks, d_active, dP, dS, dD, dQ, dH0, d_mpart);
     hipLaunchKernelGGL(k_mulliken_finish, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_part_off,
                        d_mpart, d_colsum, mx.raw, d_st);
-    HIP_TRY(probe_end(kKMulliken, 0.0, 88.0 * a2));  // reads P, S, D(3), Q(6), H0
+    HIP_TRY(probe_end(kKMulliken, 0.0, 0.0));
+    rp.mull = last_probe();
     HIP_TRY(probe_begin());
     hipLaunchKernelGGL(k_energy_mix<true>, dim3(B), dim3(512), 0, sm, t, d_order, d_active,
-                       d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_count, d_gstage,
+                       d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_cnt, d_gstage,
                        d_gstagei);
     if (big_systems)
       hipLaunchKernelGGL(k_energy_mix<false>, dim3(B), dim3(512), 0, sm, t, d_order, d_active,
-                         d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_count, d_gstage,
+                         d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_cnt, d_gstage,
                          d_gstagei);
     HIP_TRY(probe_end(kKEnergyMix, 0.0, 0.0));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, sm));
+    HIP_TRY(hipMemcpyAsync(h_count + par, d_cnt, sizeof(int), hipMemcpyDeviceToHost, sm));
     if (ks) {
-      HIP_TRY(hipMemcpyAsync(h_active.data(), d_active, B, hipMemcpyDeviceToHost, sm));
-      HIP_TRY(hipMemcpyAsync(h_kocc.data(), d_kocc, sizeof(int) * B, hipMemcpyDeviceToHost, sm));
+      HIP_TRY(hipMemcpyAsync(h_act_snap[par], d_active, B, hipMemcpyDeviceToHost, sm));
+      HIP_TRY(hipMemcpyAsync(h_kocc_snap[par], d_kocc, sizeof(int) * B, hipMemcpyDeviceToHost, sm));
     }
-    HIP_TRY(hipStreamSynchronize(sm));
-    if (ks) {
-      // algorithmic flops of the kocc-limited products of this round:
-      // back-transformation 2 n^2 kocc, C = X Y n^2 kocc (X upper triangular),
-      // P (lower half) n^2 kocc
-      const double w = work_occ(round_active);
-      probes[pr_bt - 1].flops = 2.0 * w;
-      probes[pr_back - 1].flops = w;
-      probes[pr_dens - 1].flops = w;
+    HIP_TRY(hipEventRecord(dev->round_ev[par], sm));
+    rprobes.push_back(rp);
+    enqueued = it + 1;
+    if (it >= 1) {
+      int still_active = 0;
+      if (const int st_round = finish_round(it - 1, still_active)) return st_round;
+      if (still_active == 0) {
+        stopped = true;
+        break;
+      }
     }
-    ++rounds;
-    if (*h_count == 0) break;
+  }
+  // the last enqueued round: either the speculative one behind a finished
+  // batch (no active system) or the final round at the iteration limit
+  if (enqueued > 0) {
+    const int last = enqueued - 1;
+    HIP_TRY(hipEventSynchronize(dev->round_ev[last & 1]));
+    if (ks) patch_round(last, std::vector<int>(h_kocc_snap[last & 1], h_kocc_snap[last & 1] + B));
+    if (!stopped) ++rounds;
   }
   HIP_TRY(hipEventRecord(dev->ev[2], sm));
 
