@@ -52,7 +52,13 @@ def to_device(batch, torch, dev):
     return t
 
 
-def cpu_baseline(batch, gpu_out, sample_stride):
+WORKLOAD_TEXT = {
+    "ragged512": "ragged alkanes CnH2n+2, n~U{2..38}, p=0.5 terminal OH/NH2",
+    "c20h42_128": "uniform C20H42 conformers (62 atoms, 122 AOs), torsions N(180, 15 deg)",
+}
+
+
+def cpu_baseline(batch, gpu_out, sample_stride, opts=None):
     """FP64 C restatement on a strided sample; returns (baseline dict, parity dict)."""
     import numpy as np
 
@@ -64,7 +70,7 @@ def cpu_baseline(batch, gpu_out, sample_stride):
     oracle.lib()
     t0 = time.perf_counter()
     ref = oracle.batch(sub.atom_offsets, sub.atomic_numbers, sub.positions,
-                       sub.molecular_charges, sub.unpaired_electrons, threads=threads)
+                       sub.molecular_charges, sub.unpaired_electrons, opts=opts, threads=threads)
     wall = time.perf_counter() - t0
     e = gpu_out["energies"].cpu().numpy()
     f = gpu_out["forces"].cpu().numpy()
@@ -99,6 +105,12 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=2,
                     help="CPU baseline / parity sample: every k-th system")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=None, help="override the workload's batch size")
+    ap.add_argument("--charge-tol", type=float, default=1e-6,
+                    help="SCC charge tolerance (the ABI default; the reference's published "
+                         "cross-engine numbers use 1e-4)")
+    ap.add_argument("--energy-tol", type=float, default=1e-8,
+                    help="SCC energy tolerance (the ABI default; published numbers use 1e-6)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the batched GFN2 path runs every system of a batch "
@@ -111,14 +123,16 @@ def main():
     from xtbloom_amd import interface, library, workloads
 
     library.load()
-    batch = workloads.make(args.workload, seed=args.seed)
+    over = {} if args.batch is None else {"batch": args.batch}
+    batch = workloads.make(args.workload, seed=args.seed, **over)
+    tol = dict(charge_tolerance=args.charge_tol, energy_tolerance=args.energy_tol)
     dev = torch.device("cuda", 0)
     inp = to_device(batch, torch, dev)
     ctx = interface.Context("hip", 0)
     flags = library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES
     out = None
     for _ in range(args.warmup):
-        out, _ = interface.compute_tensors(ctx, **inp, flags=flags, outputs=out)
+        out, _ = interface.compute_tensors(ctx, **inp, flags=flags, outputs=out, **tol)
     torch.cuda.synchronize()
     agg = {name: dict(launches=0, ms=0.0, flops=0.0, bytes=0.0)
            for name in library.KERNEL_FAMILIES}
@@ -127,7 +141,7 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         out, timing = interface.compute_tensors(ctx, **inp, flags=flags, outputs=out,
-                                                kernel_stats=True)
+                                                kernel_stats=True, **tol)
         for name, st in timing["kernel_stats"].items():
             for k in ("launches", "ms", "flops", "bytes"):
                 agg[name][k] += st[k]
@@ -162,7 +176,8 @@ def main():
 
     cpu = parity = None
     if not args.no_cpu:
-        cpu, parity = cpu_baseline(batch, out, args.cpu_stride)
+        from oracle import oracle as _o
+        cpu, parity = cpu_baseline(batch, out, args.cpu_stride, _o.options(**tol))
     n_atoms = int(batch.atom_offsets[-1])
     n_ao = int(sum(workloads_nao(batch)))
     line = {
@@ -170,10 +185,11 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded alkane generator)",
-        "config": {"workload": f"{args.workload}: {batch.batch_size} ragged alkanes CnH2n+2, "
-                               "n~U{2..38}, p=0.5 terminal OH/NH2, seed " + str(args.seed),
+        "config": {"workload": f"{args.workload}: {batch.batch_size} "
+                               f"{WORKLOAD_TEXT.get(args.workload, 'systems')}, seed {args.seed}",
                    "systems": batch.batch_size, "atoms": n_atoms, "aos": n_ao,
-                   "scc": "FRESH, charge 1e-6, energy 1e-8, max 250, Broyden 8/0.4, 300 K",
+                   "scc": f"FRESH, charge {args.charge_tol:g}, energy {args.energy_tol:g}, max 250, "
+                          "Broyden 8/0.4, 300 K",
                    "path": "plain FP64 HIP", "inputs": "device-resident (ROCm buffers)"},
         "roofline": roof,
         "cpu_baseline": cpu,
