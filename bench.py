@@ -179,7 +179,14 @@ def main():
         from oracle import oracle as _o
         cpu, parity = cpu_baseline(batch, out, args.cpu_stride, _o.options(**tol))
     n_atoms = int(batch.atom_offsets[-1])
-    n_ao = int(sum(workloads_nao(batch)))
+    nao = workloads_nao(batch)
+    n_ao = int(sum(nao))
+    # whole-path FP64-equivalent work, F(n, k) = (25/3) k n^3 + (7/3) n^3 per system
+    # with k its SCC iterations (Cholesky, per-iteration TRSM, tridiagonalization,
+    # back-transformation, C = XY and P, and W once), over the wall time of a step
+    iters = out["scc_iterations"].cpu().numpy()
+    fpath = sum((25.0 / 3.0) * float(k) * n ** 3 + (7.0 / 3.0) * n ** 3 for n, k in zip(nao, iters))
+    path_tflops = fpath / (ms_per_step * 1e-3) / 1e12
     line = {
         "metric": METRIC, "value": value, "unit": "systems/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -192,6 +199,9 @@ def main():
                           "Broyden 8/0.4, 300 K",
                    "path": "plain FP64 HIP", "inputs": "device-resident (ROCm buffers)"},
         "roofline": roof,
+        "path_fp64_equivalent": {"flops_per_step": fpath, "achieved": path_tflops,
+                                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                 "frac": path_tflops / FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
         "scc_rounds": rounds,
