@@ -70,7 +70,9 @@ typedef struct xtbloom_amd_extension {
   float force_ms;
   float total_ms;
   int32_t scc_rounds;
-  uint32_t reserved;
+  /* input: bit f set = per-launch events for kernel family f only; 0 = every
+   * family (fewer events perturb concurrent lanes less) */
+  uint32_t kernel_stats_mask;
 } xtbloom_amd_extension_t;
 
 /* xtbloom_compute plus the extension outputs above. */

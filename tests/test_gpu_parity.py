@@ -197,6 +197,34 @@ def test_batch_equals_sequential(ctx):
         assert np.abs(o1.forces - out.forces[a0:a1]).max() < 1e-10
 
 
+def test_lanes_match_single_lane(ctx, monkeypatch):
+    # 96 systems: three concurrent lanes of 32 against one lane, with point
+    # charges on every system so that the point-charge slices are cut too
+    b = workloads.make("embedded256", seed=11, batch=96, kmin=2, kmax=10, points=4)
+    runs = {}
+    for lanes in ("1", "2", "3"):
+        monkeypatch.setenv("XTBLOOM_AMD_LANES", lanes)
+        print("case lanes", lanes, flush=True)
+        runs[lanes] = run(ctx, b, diagnostics=True, kernel_stats=True)
+    one = runs["1"]
+    for lanes in ("2", "3"):
+        out = runs[lanes]
+        for name in ("energies", "charges", "scc_iterations", "per_system_status"):
+            assert np.array_equal(getattr(out, name), getattr(one, name)), (lanes, name)
+        # gradients are accumulated with floating-point atomics: equal up to
+        # the order of the additions
+        for name in ("forces", "point_charge_forces", "point_charge_potential_gradient"):
+            assert np.abs(getattr(out, name) - getattr(one, name)).max() < 1e-11, (lanes, name)
+        for name in ("overlap", "core_hamiltonian", "density", "coordination_numbers",
+                     "energy_components"):
+            assert np.array_equal(out.extension[name], one.extension[name]), (lanes, name)
+        assert out.extension["scc_rounds"] == one.extension["scc_rounds"]
+        st1, st = one.extension["kernel_stats"], out.extension["kernel_stats"]
+        # every lane launches the per-call kernels once
+        assert st["geometry"]["launches"] == int(lanes) * st1["geometry"]["launches"]
+        assert abs(st["tridiag"]["flops"] - st1["tridiag"]["flops"]) <= 1e-9 * st1["tridiag"]["flops"]
+
+
 def test_invalid_input_leaves_outputs(ctx):
     b = workloads.make("water8", seed=2, batch=2)
     bad = b.positions.copy()

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libxtbloom_amd.so")
-SOURCES = ["gfn2_hip.hip", "api.cpp"]
+SOURCES = ["gfn2_hip.hip", "api.cpp", "lanes.cpp"]
 ARCH = os.environ.get("XTBLOOM_AMD_ARCH", "gfx950")
 
 

@@ -29,6 +29,9 @@ static_assert((int)xtb::kKernelCount == (int)XTBLOOM_AMD_KERNEL_COUNT, "kernel f
 
 struct xtbloom_context {
   xtb::Device* device = nullptr;
+  // additional execution lanes (own streams and arenas) for large batches,
+  // created on first use
+  std::vector<xtb::Device*> lanes;
   std::mutex lock;  // one context is one serialized transaction domain (reference backend.hpp:28-32)
 };
 
@@ -308,12 +311,22 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, const xtbloom_batch_t*
     out.dbg_cn = ext->coordination_numbers;
     out.dbg_components = ext->energy_components;
     out.kernel_stats = reinterpret_cast<xtb::KernelStat*>(ext->kernel_stats);
+    out.kernel_stats_mask = ext->kernel_stats_mask;
   }
   std::string err;
   xtb::RunTiming timing;
   int rc = 0;
   try {
-    rc = xtb::run_batch(context->device, hb, ro, out, &timing, err);
+    const int want = xtb::lanes_for_batch(B);
+    while ((int)context->lanes.size() + 1 < want) {
+      std::string lane_err;
+      xtb::Device* d = xtb::device_create(xtb::device_id(context->device), nullptr, lane_err);
+      if (d == nullptr) break;  // fewer lanes: the batch runs in larger slices
+      context->lanes.push_back(d);
+    }
+    std::vector<xtb::Device*> lanes{context->device};
+    for (int k = 0; k + 1 < want && k < (int)context->lanes.size(); ++k) lanes.push_back(context->lanes[k]);
+    rc = xtb::run_batch_lanes(lanes, hb, ro, out, &timing, err);
   } catch (const std::bad_alloc&) {
     return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "host allocation failed");
   } catch (...) {
@@ -444,6 +457,7 @@ xtbloom_status_t xtbloom_context_create(const xtbloom_context_options_t* options
 
 void xtbloom_context_destroy(xtbloom_context_t* context) {
   if (context == nullptr) return;
+  for (xtb::Device* d : context->lanes) xtb::device_destroy(d);
   xtb::device_destroy(context->device);
   delete context;
 }

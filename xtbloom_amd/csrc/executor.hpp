@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 namespace xtb {
 
@@ -64,6 +65,7 @@ struct HostOutputs {
   double* dbg_cn = nullptr;         // [N]
   double* dbg_components = nullptr; // [B*8] core es2 es3 aes2 d4 pc rep atm
   KernelStat* kernel_stats = nullptr; // [kKernelCount]; enables per-launch events
+  uint32_t kernel_stats_mask = 0;     // bit f: events for family f (0 = every family)
 };
 
 struct Device;
@@ -80,7 +82,21 @@ int device_id(const Device* dev);
 // returns an xtbloom status code; error receives a message on failure
 int run_batch(Device* dev, const HostBatch& batch, const RunOptions& options, HostOutputs& out,
               RunTiming* timing, std::string& error);
-// Packed-matrix element count and AO count limit checks used by validation.
-int64_t batch_matrix_elements(const HostBatch& batch, int32_t* max_ao, std::string& error);
+// Packed-matrix element count and AO count limit checks used by validation;
+// system_aos (optional, [B]) receives each system's AO count.
+int64_t batch_matrix_elements(const HostBatch& batch, int32_t* max_ao, std::string& error,
+                              int32_t* system_aos = nullptr);
+
+// Runs a batch as up to `lanes.size()` contiguous, cost-balanced slices at
+// the same time, one host thread and one device stream per slice (lanes[0]
+// runs in the calling thread).  Results are identical to run_batch on the
+// whole batch: every system is computed independently.  Returns an xtbloom
+// status code like run_batch.
+int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& batch,
+                    const RunOptions& options, HostOutputs& out, RunTiming* timing,
+                    std::string& error);
+// Lane count for a batch: min(configured lanes, batch / kLaneMinSystems), at
+// least 1.  The configured count is XTBLOOM_AMD_LANES (default 1).
+int lanes_for_batch(int64_t batch);
 
 }  // namespace xtb

@@ -384,7 +384,8 @@ void device_destroy(Device* d) {
 
 int device_id(const Device* d) { return d ? d->id : -1; }
 
-int64_t batch_matrix_elements(const HostBatch& hb, int32_t* max_ao, std::string& error) {
+int64_t batch_matrix_elements(const HostBatch& hb, int32_t* max_ao, std::string& error,
+                              int32_t* system_aos) {
   int64_t total = 0;
   int32_t mx = 0;
   for (int64_t s = 0; s < hb.batch; ++s) {
@@ -399,6 +400,7 @@ int64_t batch_matrix_elements(const HostBatch& hb, int32_t* max_ao, std::string&
       for (int k = 0; k < cnt; ++k) nao += 2 * xtbp_shell_l[off + k] + 1;
     }
     mx = std::max<int32_t>(mx, (int32_t)nao);
+    if (system_aos) system_aos[s] = (int32_t)nao;
     total += nao * nao;
   }
   if (max_ao) *max_ao = mx;
@@ -800,8 +802,11 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     double flops, bytes;
   };
   std::vector<ProbeRec> probes;
-  auto probe_begin = [&]() -> hipError_t {
-    if (!ks) return hipSuccess;
+  // families outside out.kernel_stats_mask (0 = every family) get no events
+  bool rec = false;  // the current begin/end pair records events
+  auto probe_begin = [&](int kind) -> hipError_t {
+    rec = ks && (out.kernel_stats_mask == 0u || ((out.kernel_stats_mask >> kind) & 1u));
+    if (!rec) return hipSuccess;
     while (dev->probe_pool.size() < 2 * (probes.size() + 1)) {
       hipEvent_t e;
       const hipError_t err = hipEventCreate(&e);
@@ -811,7 +816,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     return hipEventRecord(dev->probe_pool[2 * probes.size()], sm);
   };
   auto probe_end = [&](int kind, double flops, double bytes) -> hipError_t {
-    if (!ks) return hipSuccess;
+    if (!rec) return hipSuccess;
     const hipError_t err = hipEventRecord(dev->probe_pool[2 * probes.size() + 1], sm);
     probes.push_back({kind, flops, bytes});
     return err;
@@ -878,12 +883,12 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   }
   HIP_TRY(hipEventRecord(dev->ev[0], sm));
   // ---- geometry, integrals, overlap factorization ----
-  HIP_TRY(probe_begin());
+  HIP_TRY(probe_begin(kKGeometry));
   hipLaunchKernelGGL(k_geometry, dim3(B), dim3(256), 0, sm, t, d_order, d_cn, d_cnd4, d_mrad,
                      d_dmrad, d_gam, d_vpc, mx, d_st, d_charges);
   HIP_TRY(probe_end(kKGeometry, 0.0, 0.0));
   if (NSP > 0) {
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKIntegrals));
     if (cls_sp[0].back() > 0)
       hipLaunchKernelGGL((k_integrals<false>), dim3((unsigned)((cls_sp[0].back() + 127) / 128)), dim3(128), 0,
                          sm, t, sp_list[0], cls_sp[0].back(), d_cn, dS, dD, dQ, dH0);
@@ -892,7 +897,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
                          sm, t, sp_list[1], cls_sp[1].back(), d_cn, dS, dD, dQ, dH0);
     HIP_TRY(probe_end(kKIntegrals, 0.0, 88.0 * all2));  // S + 3 D + 6 Q + H0 written
   }
-  HIP_TRY(probe_begin());
+  HIP_TRY(probe_begin(kKCholesky));
   hipLaunchKernelGGL(k_cholesky_inverse, dim3(B), dim3(512), 0, sm, t, d_order, dS, dA, dX, d_st);
   HIP_TRY(probe_end(kKCholesky, all3 / 3.0, 0.0));
   hipLaunchKernelGGL(k_init_active, dim3((B + 255) / 256), dim3(256), 0, sm, B, d_st, d_active,
@@ -919,7 +924,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   std::vector<RoundProbes> rprobes;
   std::vector<std::vector<unsigned char>> start_set;  // active set at the start of each round
   if (ks) start_set.push_back(h_active);
-  auto last_probe = [&]() -> size_t { return ks ? probes.size() : 0; };
+  auto last_probe = [&]() -> size_t { return rec ? probes.size() : 0; };
   auto patch_round = [&](int r, const std::vector<int>& kocc_after) {
     const RoundProbes& rp = rprobes[r];
     double a2 = 0.0, a3 = 0.0;
@@ -960,63 +965,63 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     int* d_cnt = d_count + par;
     RoundProbes rp{};
     HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(int), sm));
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKPotentials));
     launch_potentials(d_active);
     HIP_TRY(probe_end(kKPotentials, 0.0, 0.0));
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKFock));
     hipLaunchKernelGGL(k_fock, dim3((unsigned)chunks.size()), dim3(256), 0, sm, t, d_chunks,
                        d_active, dS, dD, dQ, dH0, d_vsh, d_vd, d_vqq, dH);
     HIP_TRY(probe_end(kKFock, 0.0, 0.0));
     rp.fock = last_probe();
     // H' = X^T H X
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKReduce));
     hipLaunchKernelGGL((k_gemm<true, false, false, 1>), dim3(nfull), dim3(256), 0, sm, t, d_full,
                        dX, dH, dT, (const double*)nullptr, (const int*)nullptr,
                        (const int*)nullptr, d_active);
     HIP_TRY(probe_end(kKReduce, 0.0, 0.0));
     rp.red1 = last_probe();
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKReduce));
     hipLaunchKernelGGL((k_gemm<false, false, true, 2>), dim3(nlower), dim3(256), 0, sm, t, d_lower,
                        dT, dX, dA, (const double*)nullptr, (const int*)nullptr,
                        (const int*)nullptr, d_active);
     HIP_TRY(probe_end(kKReduce, 0.0, 0.0));
     rp.red2 = last_probe();
     // eigenpairs of H' (A is overwritten by the reflectors; T, H, C, P, W serve as scratch)
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKTridiag));
     hipLaunchKernelGGL(k_tridiag, dim3(B), dim3(512), 0, sm, t, d_order, dA, d_td, d_te, d_wyT,
                        d_active);
     HIP_TRY(probe_end(kKTridiag, 0.0, 0.0));
     rp.tri = last_probe();
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKTridiagEigen));
     // eigenvalues, occupations (kocc), eigenvectors of the kocc lowest levels
     hipLaunchKernelGGL(k_tridiag_eigen, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_td, d_te,
                        dZ, EigenScratch{dT, dH, dC, dP, dW}, d_eps, d_shift, it, d_zscale,
                        FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt});
     HIP_TRY(probe_end(kKTridiagEigen, 0.0, 0.0));
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKBacktransform));
     hipLaunchKernelGGL(k_backtransform, dim3((unsigned)back.size()), dim3(kBackThreads), back_lds, sm, t, d_back,
                        dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
     HIP_TRY(probe_end(kKBacktransform, 0.0, 0.0));
     rp.bt = last_probe();
     // C = X Y (the kocc leading columns), P = C diag(f) C^T
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKBack));
     hipLaunchKernelGGL((k_gemm<false, false, false, 3>), dim3(nfull), dim3(256), 0, sm, t, d_full,
                        dX, dY, dC, (const double*)nullptr, (const int*)nullptr, d_kocc, d_active);
     HIP_TRY(probe_end(kKBack, 0.0, 0.0));
     rp.back = last_probe();
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKDensity));
     hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
                        dC, dP, d_occ, d_kocc, (const int*)nullptr, d_active);
     HIP_TRY(probe_end(kKDensity, 0.0, 0.0));
     rp.dens = last_probe();
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKMulliken));
     hipLaunchKernelGGL(k_mulliken_rows, dim3((unsigned)rowchunks.size()), dim3(256), 0, sm, t,
                        d_rowchunks, d_active, dP, dS, dD, dQ, dH0, d_mpart);
     hipLaunchKernelGGL(k_mulliken_finish, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_part_off,
                        d_mpart, d_colsum, mx.raw, d_st);
     HIP_TRY(probe_end(kKMulliken, 0.0, 0.0));
     rp.mull = last_probe();
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKEnergyMix));
     hipLaunchKernelGGL(k_energy_mix<true>, dim3(B), dim3(512), 0, sm, t, d_order, d_active,
                        d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_cnt, d_gstage,
                        d_gstagei);
@@ -1071,17 +1076,17 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(hipMemsetAsync(d_dedq, 0, sizeof(double) * (size_t)NPC, sm));
   }
   // potentials at the converged raw multipoles (reference gfn2_cpu_execution.cpp:1358-1441)
-  HIP_TRY(probe_begin());
+  HIP_TRY(probe_begin(kKPotentials));
   launch_potentials(d_valid);
   HIP_TRY(probe_end(kKPotentials, 0.0, 0.0));
   if (opt.forces) {
     // W = C diag(f eps) C^T
-    HIP_TRY(probe_begin());
+    HIP_TRY(probe_begin(kKEnergyWeighted));
     hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
                        dC, dW, d_fe, d_kocc, (const int*)nullptr, d_valid);
     HIP_TRY(probe_end(kKEnergyWeighted, work_occ(h_valid), 0.0));  // lower half, K = kocc
     if (NAP > 0) {
-      HIP_TRY(probe_begin());
+      HIP_TRY(probe_begin(kKForcePairs));
       for (int c = 0; c < 2; ++c) {
         const int64_t np = cls_ap[c].back();
         if (np == 0) continue;
@@ -1095,12 +1100,12 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
       HIP_TRY(probe_end(kKForcePairs, 0.0, 24.0 * v2));  // reads P, W, S
     }
   }
-  HIP_TRY(probe_begin());
+  HIP_TRY(probe_begin(kKForceAtoms));
   hipLaunchKernelGGL(k_force_atoms, dim3(B), dim3(256), 0, sm, t, d_order, d_valid, mx.x, d_qat,
                      d_cnd4, d_mrad, d_dmrad, d_gam, d_w4, d_dw4, d_dedcn, d_dedcn4, d_grad,
                      d_erep, d_pcgrad, d_dedq);
   HIP_TRY(probe_end(kKForceAtoms, 0.0, 0.0));
-  HIP_TRY(probe_begin());
+  HIP_TRY(probe_begin(kKForceAtm));
   hipLaunchKernelGGL(k_force_atm, dim3(B), dim3(256), 0, sm, t, d_order, d_valid, d_cnd4, d_w4,
                      d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm);
   HIP_TRY(probe_end(kKForceAtm, 0.0, 0.0));

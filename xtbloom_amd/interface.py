@@ -255,6 +255,7 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
         stats = (library.KernelStat * len(library.KERNEL_FAMILIES))() if kernel_stats else None
         if stats is not None:
             ext.kernel_stats = ctypes.addressof(stats)
+            ext.kernel_stats_mask = _stats_mask(kernel_stats)
         _check(lib.xtbloom_amd_compute_extended(context.handle, ctypes.byref(batch),
                                                 ctypes.byref(opts), ctypes.byref(res),
                                                 ctypes.byref(ext)))
@@ -267,6 +268,16 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                                    ctypes.byref(res)))
     return ComputedBatch(atom_offsets, point_offsets, energies, forces, q, pcf, dip, iters,
                          conv.astype(bool), status, dedq, ext_out)
+
+
+def _stats_mask(kernel_stats) -> int:
+    """``True``: events for every kernel family; an iterable of family names: those only."""
+    if kernel_stats is True:
+        return 0
+    mask = 0
+    for name in kernel_stats:
+        mask |= 1 << library.KERNEL_FAMILIES.index(name)
+    return mask
 
 
 def _stats_dict(stats) -> dict:
@@ -361,6 +372,7 @@ def compute_tensors(context: Context, atom_offsets, numbers, positions, charges,
     stats = (library.KernelStat * len(library.KERNEL_FAMILIES))() if kernel_stats else None
     if stats is not None:
         ext.kernel_stats = ctypes.addressof(stats)
+        ext.kernel_stats_mask = _stats_mask(kernel_stats)
     if dev.type == "cuda":
         torch.cuda.synchronize(dev)
     _check(lib.xtbloom_amd_compute_extended(context.handle, ctypes.byref(batch),

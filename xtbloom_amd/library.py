@@ -125,7 +125,7 @@ class Extension(ctypes.Structure):
                 ("setup_ms", ctypes.c_float),
                 ("scc_ms", ctypes.c_float), ("force_ms", ctypes.c_float),
                 ("total_ms", ctypes.c_float), ("scc_rounds", ctypes.c_int32),
-                ("reserved", ctypes.c_uint32)]
+                ("kernel_stats_mask", ctypes.c_uint32)]
 
 
 # Byte images asserted by the reference ABI for LP64 (reference xtbloom.h:320-343).
