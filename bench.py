@@ -113,6 +113,15 @@ def main():
                          "cross-engine numbers use 1e-4)")
     ap.add_argument("--energy-tol", type=float, default=1e-8,
                     help="SCC energy tolerance (the ABI default; published numbers use 1e-6)")
+    ap.add_argument("--scc-start", choices=("fresh", "warm"), default="fresh",
+                    help="warm: the warmup call converges FRESH, every timed call starts WARM "
+                         "from the previous call's converged state")
+    ap.add_argument("--warm-step", type=float, default=0.0,
+                    help="with --scc-start warm: Gaussian displacement (bohr) of every atom "
+                         "before each timed call (0: the same geometry)")
+    ap.add_argument("--plan", action="store_true",
+                    help="timed calls through a fixed-topology plan (no per-launch events: "
+                         "the roofline entry then comes from the warmup calls)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the batched GFN2 path runs every system of a batch "
@@ -154,13 +163,38 @@ def main():
     agg = new_agg()
     rounds = 0
     device_ms = 0.0
+    plan = None
+    if args.plan:
+        plan = interface.Plan(ctx, batch.atom_offsets, batch.atomic_numbers, batch.positions,
+                              batch.molecular_charges, batch.unpaired_electrons, flags=flags,
+                              **tol)
+        # the plan's own checkpoint for WARM calls
+        out, _ = interface.compute_tensors(ctx, **inp, flags=flags, outputs=out, plan=plan, **tol)
+    start = library.SCC_START_WARM if args.scc_start == "warm" else library.SCC_START_FRESH
+    geoms = [inp["positions"]] * args.steps
+    if args.scc_start == "warm" and args.warm_step > 0.0:
+        gen = torch.Generator(device=dev).manual_seed(args.seed)
+        geoms, cur = [], inp["positions"]
+        for _ in range(args.steps):
+            cur = cur + args.warm_step * torch.randn(cur.shape, generator=gen, device=dev,
+                                                     dtype=cur.dtype)
+            geoms.append(cur)
+    # the timed loop sums iteration counts on the device: load torch's reduction
+    # kernel outside the timed region
+    iter_sum = 0 * int(out["scc_iterations"].sum())
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out, timing = interface.compute_tensors(ctx, **inp, flags=flags, outputs=out,
-                                                kernel_stats=[dom], **tol)
-        add(agg, timing["kernel_stats"])
+    for k in range(args.steps):
+        step_inp = dict(inp, positions=geoms[k])
+        out, timing = interface.compute_tensors(ctx, **step_inp, flags=flags, outputs=out,
+                                                kernel_stats=None if plan else [dom],
+                                                scc_start_mode=start, plan=plan, **tol)
+        if not plan:
+            add(agg, timing["kernel_stats"])
         rounds = max(rounds, timing["scc_rounds"])
         device_ms += timing["total_ms"]
+        if args.scc_start == "warm":
+            iter_sum += int(out["scc_iterations"].sum())
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = 1000.0 * wall / args.steps
@@ -168,7 +202,7 @@ def main():
     value = batch.batch_size / (ms_per_step / 1000.0)
 
     # roofline of the kernel family with the largest device time
-    d = agg[dom]
+    d = wagg[dom] if plan else agg[dom]
     avg_ms = d["ms"] / max(d["launches"], 1)
     if d["flops"] > 0.0:
         per_launch = d["flops"] / d["launches"]
@@ -180,8 +214,10 @@ def main():
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None}
-    roof.update(kernel=dom, launches_per_step=d["launches"] / args.steps,
+    roof.update(kernel=dom, launches_per_step=d["launches"] / (max(args.warmup, 1) if plan else args.steps),
                 avg_launch_ms=avg_ms, algorithmic_per_launch=per_launch)
+    if plan:
+        roof["source"] = "warmup calls (plan calls record no events)"
     nw = max(args.warmup, 1)
     total_kernel_ms = sum(v["ms"] for v in wagg.values())
     kernels = {k: {"ms_per_step": v["ms"] / nw, "launches_per_step": v["launches"] / nw,
@@ -189,7 +225,7 @@ def main():
                for k, v in sorted(wagg.items(), key=lambda kv: -kv[1]["ms"]) if v["launches"]}
 
     cpu = parity = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.scc_start == "fresh":
         from oracle import oracle as _o
         cpu, parity = cpu_baseline(batch, out, args.cpu_stride, _o.options(**tol))
     n_atoms = int(batch.atom_offsets[-1])
@@ -209,9 +245,12 @@ def main():
         "config": {"workload": f"{args.workload}: {batch.batch_size} "
                                f"{WORKLOAD_TEXT.get(args.workload, 'systems')}, seed {args.seed}",
                    "systems": batch.batch_size, "atoms": n_atoms, "aos": n_ao,
-                   "scc": f"FRESH, charge {args.charge_tol:g}, energy {args.energy_tol:g}, max 250, "
+                   "scc": f"{args.scc_start.upper()}"
+                          + (f" (step {args.warm_step:g} bohr)" if args.scc_start == "warm" else "")
+                          + f", charge {args.charge_tol:g}, energy {args.energy_tol:g}, max 250, "
                           "Broyden 8/0.4, 300 K",
-                   "path": "plain FP64 HIP", "inputs": "device-resident (ROCm buffers)",
+                   "path": "plain FP64 HIP" + (", fixed-topology plan" if plan else ""),
+                   "inputs": "device-resident (ROCm buffers)",
                    "lanes": lanes_for_batch(batch.batch_size)},
         "roofline": roof,
         "path_fp64_equivalent": {"flops_per_step": fpath, "achieved": path_tflops,
@@ -220,11 +259,15 @@ def main():
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
         "scc_rounds": rounds,
+        "mean_scc_iterations": (iter_sum / (args.steps * batch.batch_size)
+                                if args.scc_start == "warm" else float(iters.mean())),
         "failed_systems": int((status != 0).sum()),
         "device_ms_per_step": device_ms / args.steps,
         "kernels": kernels,
         "kernels_source": "warmup calls, events around every launch, summed over lanes",
     }
+    if plan:
+        plan.close()
     ctx.close()
     print(json.dumps(line), flush=True)
 

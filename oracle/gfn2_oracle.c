@@ -1470,6 +1470,17 @@ int oracle_singlepoint(int32_t nat, const int32_t* Z, const double* xyz, double 
                        const double* pc_gamma, const oracle_options* opt, double* energy,
                        double* grad_out, double* atomic_charges, double* pc_grad, double* dedq,
                        int32_t* iterations, oracle_trace* tr) {
+  return oracle_singlepoint_state(nat, Z, xyz, charge, unpaired, npc, pc_xyz, pc_q, pc_gamma, opt,
+                                  energy, grad_out, atomic_charges, pc_grad, dedq, iterations, tr,
+                                  NULL, NULL);
+}
+
+int oracle_singlepoint_state(int32_t nat, const int32_t* Z, const double* xyz, double charge,
+                             int32_t unpaired, int32_t npc, const double* pc_xyz, const double* pc_q,
+                             const double* pc_gamma, const oracle_options* opt, double* energy,
+                             double* grad_out, double* atomic_charges, double* pc_grad, double* dedq,
+                             int32_t* iterations, oracle_trace* tr, const double* state_in,
+                             double* state_out) {
   if (!la_dsyevd) return ST_INVALID;
   oracle_options dflt;
   if (!opt) { oracle_default_options(&dflt); opt = &dflt; }
@@ -1481,12 +1492,16 @@ int oracle_singlepoint(int32_t nat, const int32_t* Z, const double* xyz, double 
   const int n = sy.n, ns = sy.ns;
   const size_t nn = (size_t)n * n;
 
-  /* FRESH SAD state (wavefunction.cpp:733-776) */
+  /* FRESH SAD state (wavefunction.cpp:733-776), or the WARM checkpoint's
+   * converged multipoles (gfn2_cpu_execution.cpp:1261-1268: the restored
+   * wavefunction seeds a reset mixer) */
   const int dim = ns + 9 * nat;
   double* x = calloc(dim, sizeof(double));
   double* raw = calloc(dim, sizeof(double));
   double* qat = calloc(nat, sizeof(double));
-  {
+  if (state_in) {
+    memcpy(x, state_in, sizeof(double) * dim);
+  } else {
     double qa = charge / nat;
     for (int a = 0; a < nat; ++a) {
       double ref = 0.0;
@@ -1525,6 +1540,7 @@ int oracle_singlepoint(int32_t nat, const int32_t* Z, const double* xyz, double 
   }
   free(mixed);
   if (status == ST_OK && !converged) status = ST_NOT_CONVERGED;
+  if (state_out) memcpy(state_out, x, sizeof(double) * dim);
   if (tr) {
     tr->recorded = it < tr->capacity ? it : tr->capacity;
     if (tr->overlap) memcpy(tr->overlap, sy.S, sizeof(double) * nn);

@@ -66,6 +66,19 @@ int oracle_singlepoint(int32_t nat, const int32_t* Z, const double* xyz, double 
                        int32_t* iterations, oracle_trace* trace);
 
 /*
+ * oracle_singlepoint with the SCC multipole state exposed: state_in (NULL =
+ * FRESH SAD start) is the state a WARM start begins from, state_out receives
+ * the final state.  The state holds nshell shell charges, then 3 nat atomic
+ * dipole and 6 nat atomic quadrupole components ([xx, xy, yy, xz, yz, zz]).
+ */
+int oracle_singlepoint_state(int32_t nat, const int32_t* Z, const double* xyz, double charge,
+                             int32_t unpaired, int32_t npc, const double* pc_xyz, const double* pc_q,
+                             const double* pc_gamma, const oracle_options* options, double* energy,
+                             double* grad, double* atomic_charges, double* pc_grad, double* dedq,
+                             int32_t* iterations, oracle_trace* trace, const double* state_in,
+                             double* state_out);
+
+/*
  * Ragged batch with one system per worker thread (threads <= 0: all cores).
  * Layout follows xtbloom_batch_t: atom_offsets[B+1], point_offsets[B+1] or NULL.
  * Failed systems get NaN energy/gradient/charge slices.

@@ -70,6 +70,8 @@ def lib():
             ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p, ctypes.POINTER(Options),
             _c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_int32_p,
             ctypes.POINTER(Trace)]
+        L.oracle_singlepoint_state.argtypes = L.oracle_singlepoint.argtypes + [
+            _c_double_p, _c_double_p]
         L.oracle_batch.argtypes = [
             ctypes.c_int64, _c_int64_p, _c_int32_p, _c_double_p, _c_double_p, _c_int32_p,
             _c_int64_p, _c_double_p, _c_double_p, _c_double_p, ctypes.POINTER(Options),
@@ -118,6 +120,7 @@ class Result:
     pc_gradient: np.ndarray | None = None
     dedq: np.ndarray | None = None
     trace: dict | None = None
+    state: np.ndarray | None = None
 
 
 def basis_counts(Z):
@@ -131,7 +134,10 @@ def basis_counts(Z):
 
 
 def singlepoint(Z, xyz, charge=0.0, uhf=0, pc_xyz=None, pc_q=None, pc_gamma=None, opts=None,
-                trace=False, trace_capacity=512) -> Result:
+                trace=False, trace_capacity=512, warm_state=None) -> Result:
+    """One single point.  ``warm_state`` (the ``state`` of an earlier converged
+    result of the same system) starts the SCC from that state instead of the
+    FRESH SAD guess; ``Result.state`` is always the final SCC multipole state."""
     Z = np.ascontiguousarray(Z, dtype=np.int32)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
     nat = len(Z)
@@ -161,15 +167,21 @@ def singlepoint(Z, xyz, charge=0.0, uhf=0, pc_xyz=None, pc_q=None, pc_gamma=None
             "free_energy", "residual_rms", "overlap", "h0", "dipole", "quadrupole", "density",
             "shell_charges", "dipoles", "quadrupoles", "coordination", "components")])
     o = opts or options()
-    st = lib().oracle_singlepoint(nat, _p(Z, _c_int32_p), _p(xyz), float(charge), int(uhf), npc,
-                                  _p(pc_xyz), _p(pc_q), _p(pc_gamma), ctypes.byref(o),
-                                  ctypes.byref(e), _p(g), _p(q), _p(pg), _p(dq), ctypes.byref(it),
-                                  ctypes.byref(tr) if tr is not None else None)
+    nsh, _, _ = basis_counts(Z)
+    state = np.zeros(nsh + 9 * nat)
+    if warm_state is not None:
+        warm_state = np.ascontiguousarray(warm_state, dtype=np.float64)
+        if warm_state.shape != state.shape:
+            raise ValueError("warm_state does not belong to this system")
+    st = lib().oracle_singlepoint_state(
+        nat, _p(Z, _c_int32_p), _p(xyz), float(charge), int(uhf), npc, _p(pc_xyz), _p(pc_q),
+        _p(pc_gamma), ctypes.byref(o), ctypes.byref(e), _p(g), _p(q), _p(pg), _p(dq),
+        ctypes.byref(it), ctypes.byref(tr) if tr is not None else None, _p(warm_state), _p(state))
     if trace:
         bufs["recorded"] = tr.recorded
         bufs["free_energy"] = bufs["free_energy"][:tr.recorded]
         bufs["residual_rms"] = bufs["residual_rms"][:tr.recorded]
-    return Result(st, e.value, g, q, it.value, pg, dq, bufs if trace else None)
+    return Result(st, e.value, g, q, it.value, pg, dq, bufs if trace else None, state)
 
 
 def broyden(initial, residuals, memory=8, damping=0.4):

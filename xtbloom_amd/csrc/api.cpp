@@ -12,6 +12,7 @@
 #include <cstddef>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <numeric>
@@ -27,12 +28,57 @@ static_assert(offsetof(xtb::KernelStat, bytes) == offsetof(xtbloom_amd_kernel_st
               "kernel stat layout");
 static_assert((int)xtb::kKernelCount == (int)XTBLOOM_AMD_KERNEL_COUNT, "kernel family count");
 
+// Identity of a call for strict WARM starts and fixed-topology plans
+// (reference xtbloom.h:502-520): the topology (atom offsets, element numbers,
+// point-charge offsets and hardnesses, spin channels), molecular charges,
+// unpaired electrons and the compute policy.  Geometry (atom and point-charge
+// positions, point-charge values) is not part of it.
+struct CallIdentity {
+  std::vector<char> atom_offsets, Z, charges, unpaired, point_offsets, pc_gamma, spin;
+  uint32_t flags = 0;
+  int32_t model = 0, max_iterations = 0, history = 0, determinism = 0;
+  double charge_tolerance = 0.0, energy_tolerance = 0.0, kt = 0.0, damping = 0.0;
+  bool operator==(const CallIdentity& o) const {
+    // doubles compare by their bits: the identity is exact
+    auto same = [](double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; };
+    return atom_offsets == o.atom_offsets && Z == o.Z && charges == o.charges && unpaired == o.unpaired &&
+           point_offsets == o.point_offsets && pc_gamma == o.pc_gamma && spin == o.spin && flags == o.flags &&
+           model == o.model && max_iterations == o.max_iterations && history == o.history &&
+           determinism == o.determinism && same(charge_tolerance, o.charge_tolerance) &&
+           same(energy_tolerance, o.energy_tolerance) && same(kt, o.kt) && same(damping, o.damping);
+  }
+};
+
+// Converged SCC multipole state of the latest fully converged call, the
+// starting point of the next strict WARM call with the same identity
+// (reference gfn2_cpu_execution.cpp:829-838, 1313-1325, 1470-1487).
+struct WarmCheckpoint {
+  bool valid = false;
+  CallIdentity id;
+  std::vector<double> state;
+};
+
 struct xtbloom_context {
   xtb::Device* device = nullptr;
   // additional execution lanes (own streams and arenas) for large batches,
   // created on first use
   std::vector<xtb::Device*> lanes;
+  WarmCheckpoint warm;  // checkpoint of xtbloom_compute calls
   std::mutex lock;  // one context is one serialized transaction domain (reference backend.hpp:28-32)
+};
+
+// Fixed-topology plan (reference xtbloom.h:939-991): the identity it was
+// created for, its own WARM checkpoint, the host topology of each lane slice
+// and the workspace sizes reserved at creation.
+struct xtbloom_plan {
+  xtbloom_context_t* context = nullptr;
+  CallIdentity id;
+  WarmCheckpoint warm;
+  std::vector<xtb::TopoCache*> caches;
+  uint64_t device_bytes = 0, host_bytes = 0;
+  ~xtbloom_plan() {
+    for (xtb::TopoCache* c : caches) xtb::topo_cache_destroy(c);
+  }
 };
 
 namespace {
@@ -115,14 +161,16 @@ struct Staged {
   std::vector<char> atom_offsets, Z, xyz, charges, unpaired, poff, pxyz, pq, pg, spin;
 };
 
+// result may be NULL (plan creation: no outputs are checked).
 xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_compute_options_t* options,
                                     const xtbloom_batch_result_t* result, Staged& st,
-                                    xtb::HostBatch& hb, xtb::RunOptions& ro, uint32_t& flags) {
+                                    xtb::HostBatch& hb, xtb::RunOptions& ro, uint32_t& flags, bool& warm) {
+  warm = false;
   if (!valid_header(batch, XTBLOOM_BATCH_V1_SIZE))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "batch is NULL, too small, or has a different api_version");
   if (!valid_header(options, XTBLOOM_COMPUTE_OPTIONS_V1_SIZE))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "compute options are NULL, too small, or have a different api_version");
-  if (!valid_header(result, XTBLOOM_BATCH_RESULT_V1_SIZE))
+  if (result != nullptr && !valid_header(result, XTBLOOM_BATCH_RESULT_V1_SIZE))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "batch result is NULL, too small, or has a different api_version");
   const int64_t B = batch->batch_size, N = batch->total_atoms, P = batch->total_point_charges;
   if (B <= 0 || N <= 0 || N < B || P < 0 || B > (int64_t)1 << 28 || N > (int64_t)1 << 30 || P > (int64_t)1 << 30)
@@ -147,10 +195,10 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
   ro.energy_tolerance = options->energy_tolerance;
   ro.kt = options->electronic_temperature;
   if (options->struct_size >= XTBLOOM_COMPUTE_OPTIONS_V2_SIZE) {
-    if (options->scc_start_mode == XTBLOOM_SCC_START_WARM)
-      return fail(XTBLOOM_STATUS_NOT_SUPPORTED, "WARM SCC starts are not implemented by the ROCm backend");
-    if (options->scc_start_mode != XTBLOOM_SCC_START_FRESH && options->scc_start_mode != 0)
-      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "unknown scc_start_mode");
+    if (options->scc_start_mode != XTBLOOM_SCC_START_FRESH && options->scc_start_mode != XTBLOOM_SCC_START_WARM &&
+        options->scc_start_mode != 0)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "scc_start_mode must be XTBLOOM_SCC_START_FRESH or XTBLOOM_SCC_START_WARM");
+    warm = options->scc_start_mode == XTBLOOM_SCC_START_WARM;
   }
   if (options->struct_size >= XTBLOOM_COMPUTE_OPTIONS_V3_SIZE &&
       options->scc_mixer == XTBLOOM_SCC_MIXER_MODIFIED_BROYDEN) {
@@ -169,20 +217,21 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
   const bool want_pf = (flags & XTBLOOM_COMPUTE_POINT_CHARGE_FORCES) && P > 0;
   const bool want_d = flags & XTBLOOM_COMPUTE_DIPOLE_MOMENTS;
   xtbloom_status_t s;
-  if ((s = check_output(result->energies, sizeof(double) * B, "energies", want_e)) ||
+  if (result != nullptr && (
+      (s = check_output(result->energies, sizeof(double) * B, "energies", want_e)) ||
       (s = check_output(result->forces, sizeof(double) * 3 * N, "forces", want_f)) ||
       (s = check_output(result->atomic_charges, sizeof(double) * N, "atomic_charges", want_q)) ||
       (s = check_output(result->point_charge_forces, sizeof(double) * 3 * P, "point_charge_forces", want_pf)) ||
       (s = check_output(result->scc_iterations, sizeof(int32_t) * B, "scc_iterations", true)) ||
       (s = check_output(result->scc_converged, sizeof(uint8_t) * B, "scc_converged", true)) ||
-      (s = check_output(result->per_system_status, sizeof(int32_t) * B, "per_system_status", true)))
+      (s = check_output(result->per_system_status, sizeof(int32_t) * B, "per_system_status", true))))
     return s;
-  if (want_d) {
+  if (want_d && result != nullptr) {
     if (result->struct_size < XTBLOOM_BATCH_RESULT_V2_SIZE)
       return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "dipole moments need an ABI-v2 batch result");
     if ((s = check_output(result->dipole_moments, sizeof(double) * 3 * B, "dipole_moments", true))) return s;
   }
-  if (result->struct_size >= XTBLOOM_BATCH_RESULT_V2_SIZE &&
+  if (result != nullptr && result->struct_size >= XTBLOOM_BATCH_RESULT_V2_SIZE &&
       (result->quadrupole_moments.data || result->wiberg_orders.data || result->spin_populations.data))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "reserved result outputs must be NULL");
   // unsupported optional inputs on this backend
@@ -276,7 +325,45 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
   return XTBLOOM_STATUS_SUCCESS;
 }
 
-xtbloom_status_t compute_impl(xtbloom_context_t* context, const xtbloom_batch_t* batch,
+CallIdentity make_identity(const Staged& st, const xtbloom_compute_options_t* options,
+                           const xtb::RunOptions& ro, uint32_t flags) {
+  CallIdentity id;
+  id.atom_offsets = st.atom_offsets;
+  id.Z = st.Z;
+  id.charges = st.charges;
+  id.unpaired = st.unpaired;
+  id.point_offsets = st.poff;
+  id.pc_gamma = st.pg;
+  id.spin = st.spin;
+  id.flags = flags;
+  id.model = options->model;
+  id.max_iterations = ro.max_iterations;
+  id.history = ro.history;
+  id.determinism = options->struct_size >= XTBLOOM_COMPUTE_OPTIONS_V3_SIZE ? options->determinism : 0;
+  id.charge_tolerance = ro.charge_tolerance;
+  id.energy_tolerance = ro.energy_tolerance;
+  id.kt = ro.kt;
+  id.damping = ro.damping;
+  return id;
+}
+
+// Creates the context's extra lanes a batch of B systems runs on; returns
+// the lanes to use (the context device first).
+std::vector<xtb::Device*> lanes_for(xtbloom_context_t* context, int64_t B) {
+  const int want = xtb::lanes_for_batch(B);
+  while ((int)context->lanes.size() + 1 < want) {
+    std::string lane_err;
+    xtb::Device* d = xtb::device_create(xtb::device_id(context->device), nullptr, lane_err);
+    if (d == nullptr) break;  // fewer lanes: the batch runs in larger slices
+    context->lanes.push_back(d);
+  }
+  std::vector<xtb::Device*> lanes{context->device};
+  for (int k = 0; k + 1 < want && k < (int)context->lanes.size(); ++k) lanes.push_back(context->lanes[k]);
+  return lanes;
+}
+
+// xtbloom_compute (plan == NULL) and xtbloom_plan_compute.
+xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, const xtbloom_batch_t* batch,
                               const xtbloom_compute_options_t* options, xtbloom_batch_result_t* result,
                               xtbloom_amd_extension_t* ext) {
   if (context == nullptr || context->device == nullptr)
@@ -286,11 +373,40 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, const xtbloom_batch_t*
   xtb::HostBatch hb;
   xtb::RunOptions ro;
   uint32_t flags = 0;
-  xtbloom_status_t s = validate_and_stage(batch, options, result, st, hb, ro, flags);
+  bool warm = false;
+  xtbloom_status_t s = validate_and_stage(batch, options, result, st, hb, ro, flags, warm);
   if (s != XTBLOOM_STATUS_SUCCESS) return s;
   if (ext != nullptr && !valid_header(ext, sizeof(xtbloom_amd_extension_t)))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "extension descriptor is invalid");
+  CallIdentity id = make_identity(st, options, ro, flags);
+  if (plan != nullptr && !(id == plan->id))
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT,
+                "the batch topology or compute policy does not match the fixed-topology plan");
+  // strict WARM: the latest fully converged call with the same identity, never
+  // a fallback to FRESH (reference xtbloom.h:502-523)
+  WarmCheckpoint& cp = plan ? plan->warm : context->warm;
+  std::vector<double> warm_state;
+  if (warm) {
+    if (!cp.valid || !(cp.id == id))
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT,
+                  "WARM SCC start requires a previous fully converged call with identical topology and "
+                  "compute policy on this context");
+    warm_state.swap(cp.state);
+    ro.warm_state = warm_state.data();
+  }
+  // an accepted call consumes the checkpoint whatever its start mode
+  cp.valid = false;
+  cp.state.clear();
   const int64_t B = hb.batch, N = hb.atoms, P = hb.points;
+  std::vector<int32_t> state_dim(B);
+  {
+    std::string dim_err;
+    if (xtb::batch_matrix_elements(hb, nullptr, dim_err, nullptr, state_dim.data()) < 0)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, dim_err);
+  }
+  int64_t nstate = 0;
+  for (int32_t d : state_dim) nstate += d;
+  std::vector<double> scc_state((size_t)nstate);
   std::vector<double> energies(B), grad(3 * N), charges(N), pcgrad(3 * P), dipoles(3 * B);
   std::vector<int32_t> iters(B), status(B);
   std::vector<uint8_t> conv(B);
@@ -313,26 +429,24 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, const xtbloom_batch_t*
     out.kernel_stats = reinterpret_cast<xtb::KernelStat*>(ext->kernel_stats);
     out.kernel_stats_mask = ext->kernel_stats_mask;
   }
+  out.scc_state = scc_state.data();
   std::string err;
   xtb::RunTiming timing;
   int rc = 0;
   try {
-    const int want = xtb::lanes_for_batch(B);
-    while ((int)context->lanes.size() + 1 < want) {
-      std::string lane_err;
-      xtb::Device* d = xtb::device_create(xtb::device_id(context->device), nullptr, lane_err);
-      if (d == nullptr) break;  // fewer lanes: the batch runs in larger slices
-      context->lanes.push_back(d);
-    }
-    std::vector<xtb::Device*> lanes{context->device};
-    for (int k = 0; k + 1 < want && k < (int)context->lanes.size(); ++k) lanes.push_back(context->lanes[k]);
-    rc = xtb::run_batch_lanes(lanes, hb, ro, out, &timing, err);
+    rc = xtb::run_batch_lanes(lanes_for(context, B), hb, ro, out, &timing, err,
+                              plan ? &plan->caches : nullptr);
   } catch (const std::bad_alloc&) {
     return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "host allocation failed");
   } catch (...) {
     return fail(XTBLOOM_STATUS_INTERNAL_ERROR, "unexpected exception in the ROCm executor");
   }
   if (rc != 0) return fail(rc, err);
+  if (std::all_of(status.begin(), status.end(), [](int32_t v) { return v == 0; })) {
+    cp.id = std::move(id);
+    cp.state = std::move(scc_state);
+    cp.valid = true;
+  }
   if (ext) {
     ext->setup_ms = timing.setup_ms;
     ext->scc_ms = timing.scc_ms;
@@ -473,14 +587,14 @@ int32_t xtbloom_context_get_device_id(const xtbloom_context_t* context) {
 xtbloom_status_t xtbloom_compute(xtbloom_context_t* context, const xtbloom_batch_t* batch,
                                  const xtbloom_compute_options_t* options,
                                  xtbloom_batch_result_t* result) {
-  return compute_impl(context, batch, options, result, nullptr);
+  return compute_impl(context, nullptr, batch, options, result, nullptr);
 }
 
 xtbloom_status_t xtbloom_amd_compute_extended(xtbloom_context_t* context, const xtbloom_batch_t* batch,
                                               const xtbloom_compute_options_t* options,
                                               xtbloom_batch_result_t* result,
                                               xtbloom_amd_extension_t* extension) {
-  return compute_impl(context, batch, options, result, extension);
+  return compute_impl(context, nullptr, batch, options, result, extension);
 }
 
 int64_t xtbloom_amd_packed_matrix_elements(const xtbloom_batch_t* batch) {
@@ -519,18 +633,76 @@ xtbloom_status_t xtbloom_compute_enqueue(xtbloom_context_t*, const xtbloom_batch
                                          const xtbloom_batch_result_t*, xtbloom_request_t*) {
   return not_supported("xtbloom_compute_enqueue");
 }
-xtbloom_status_t xtbloom_plan_create(xtbloom_context_t*, const xtbloom_batch_t*,
-                                     const xtbloom_compute_options_t*, xtbloom_plan_t** plan) {
-  if (plan) *plan = nullptr;
-  return not_supported("fixed-topology plan");
+// ---- fixed-topology plans (reference xtbloom.h:939-991) ----
+xtbloom_status_t xtbloom_plan_create(xtbloom_context_t* context, const xtbloom_batch_t* batch,
+                                     const xtbloom_compute_options_t* options, xtbloom_plan_t** plan) {
+  if (plan == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "plan output is NULL");
+  *plan = nullptr;
+  if (context == nullptr || context->device == nullptr)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "context is NULL");
+  std::lock_guard<std::mutex> guard(context->lock);
+  Staged st;
+  xtb::HostBatch hb;
+  xtb::RunOptions ro;
+  uint32_t flags = 0;
+  bool warm = false;
+  xtbloom_status_t s = validate_and_stage(batch, options, nullptr, st, hb, ro, flags, warm);
+  if (s != XTBLOOM_STATUS_SUCCESS) return s;
+  try {
+    std::unique_ptr<xtbloom_plan_t> p(new xtbloom_plan_t());
+    p->context = context;
+    p->id = make_identity(st, options, ro, flags);
+    for (int k = 0; k < 4; ++k) p->caches.push_back(xtb::topo_cache_create());
+    // reserve: host topology of every lane slice and the device workspace
+    xtb::RunOptions reserve = ro;
+    reserve.reserve_only = true;
+    xtb::HostOutputs none;
+    xtb::RunTiming timing;
+    std::string err;
+    const int rc = xtb::run_batch_lanes(lanes_for(context, hb.batch), hb, reserve, none, &timing, err,
+                                        &p->caches);
+    if (rc != 0) return fail(rc, err);
+    std::vector<int32_t> dims(hb.batch);
+    if (xtb::batch_matrix_elements(hb, nullptr, err, nullptr, dims.data()) < 0)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, err);
+    int64_t nstate = 0;
+    for (int32_t d : dims) nstate += d;
+    p->device_bytes = timing.workspace_bytes;
+    // per call: the staged inputs, the host outputs and the SCC state (twice:
+    // the call's and the checkpoint)
+    p->host_bytes = (uint64_t)(st.atom_offsets.size() + st.Z.size() + st.xyz.size() + st.charges.size() +
+                               st.unpaired.size() + st.poff.size() + st.pxyz.size() + st.pq.size() +
+                               st.pg.size() + st.spin.size()) +
+                    sizeof(double) * (uint64_t)(5 * hb.batch + 4 * hb.atoms + 4 * hb.points + 2 * nstate) +
+                    (sizeof(int32_t) * 2 + 1) * (uint64_t)hb.batch;
+    *plan = p.release();
+  } catch (const std::bad_alloc&) {
+    return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "plan allocation failed");
+  }
+  g_last_error.clear();
+  return XTBLOOM_STATUS_SUCCESS;
 }
-void xtbloom_plan_destroy(xtbloom_plan_t*) {}
-xtbloom_status_t xtbloom_plan_query_workspace(const xtbloom_plan_t*, xtbloom_workspace_query_t*) {
-  return not_supported("fixed-topology plan");
+
+void xtbloom_plan_destroy(xtbloom_plan_t* plan) { delete plan; }
+
+xtbloom_status_t xtbloom_plan_query_workspace(const xtbloom_plan_t* plan, xtbloom_workspace_query_t* query) {
+  if (plan == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "plan is NULL");
+  if (!valid_header(query, XTBLOOM_WORKSPACE_QUERY_V1_SIZE))
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "workspace query is NULL, too small, or has a different api_version");
+  if (query->compute_flags != plan->id.flags)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "compute_flags must equal the plan's compute flags");
+  query->host_required_bytes = plan->host_bytes;
+  query->host_required_alignment = alignof(std::max_align_t);
+  query->device_required_bytes = plan->device_bytes;
+  query->device_required_alignment = 256;
+  g_last_error.clear();
+  return XTBLOOM_STATUS_SUCCESS;
 }
-xtbloom_status_t xtbloom_plan_compute(xtbloom_plan_t*, const xtbloom_batch_t*,
-                                      const xtbloom_compute_options_t*, xtbloom_batch_result_t*) {
-  return not_supported("fixed-topology plan");
+
+xtbloom_status_t xtbloom_plan_compute(xtbloom_plan_t* plan, const xtbloom_batch_t* batch,
+                                      const xtbloom_compute_options_t* options, xtbloom_batch_result_t* result) {
+  if (plan == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "plan is NULL");
+  return compute_impl(plan->context, plan, batch, options, result, nullptr);
 }
 xtbloom_status_t xtbloom_plan_compute_enqueue(xtbloom_plan_t*, const xtbloom_batch_t*,
                                               const xtbloom_compute_options_t*,

@@ -1,6 +1,7 @@
 // Host interface of the MI355X GFN2 executor (used by the C ABI layer).
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include <string>
@@ -31,6 +32,12 @@ struct RunOptions {
   int history = 8;
   double damping = 0.4;
   bool forces = true;
+  // WARM start: the SCC multipole state to start from instead of the FRESH
+  // SAD state, per system (shell charges, then 3 dipole and 6 quadrupole
+  // components per atom), systems in batch order; null = FRESH
+  const double* warm_state = nullptr;
+  // size the device workspace for the batch and return before any kernel
+  bool reserve_only = false;
 };
 
 // Per-kernel-family timing record (layout of xtbloom_amd_kernel_stat_t).
@@ -64,37 +71,53 @@ struct HostOutputs {
   double* dbg_density = nullptr;    // [sum n^2]
   double* dbg_cn = nullptr;         // [N]
   double* dbg_components = nullptr; // [B*8] core es2 es3 aes2 d4 pc rep atm
+  double* scc_state = nullptr;        // [sum (shells + 9 atoms)] final SCC multipole state
   KernelStat* kernel_stats = nullptr; // [kKernelCount]; enables per-launch events
   uint32_t kernel_stats_mask = 0;     // bit f: events for family f (0 = every family)
 };
 
 struct Device;
+// Host topology of one batch (everything but the geometry), reusable by
+// later calls with the same topology (fixed-topology plans).
+struct TopoCache;
+TopoCache* topo_cache_create();
+void topo_cache_destroy(TopoCache* cache);
+// Binds the cache to `tag` (which slice of which lane split); a new tag
+// empties it.  Returns whether the cache holds a topology.
+bool topo_cache_retag(TopoCache* cache, int64_t tag);
 
 // Timing of the last run (milliseconds of device time per phase).
 struct RunTiming {
   float setup_ms = 0.f, scc_ms = 0.f, force_ms = 0.f, total_ms = 0.f;
   int scc_rounds = 0;
+  size_t workspace_bytes = 0;  // device workspace the call needs
 };
 
 Device* device_create(int device_id, void* stream, std::string& error);
 void device_destroy(Device* dev);
 int device_id(const Device* dev);
 // returns an xtbloom status code; error receives a message on failure
+// `cache` (optional) holds the host topology: when it is filled the call
+// refreshes only the geometry in it, otherwise it builds and keeps it.  The
+// caller guarantees that a filled cache belongs to the batch's topology.
 int run_batch(Device* dev, const HostBatch& batch, const RunOptions& options, HostOutputs& out,
-              RunTiming* timing, std::string& error);
+              RunTiming* timing, std::string& error, TopoCache* cache = nullptr);
 // Packed-matrix element count and AO count limit checks used by validation;
-// system_aos (optional, [B]) receives each system's AO count.
+// system_aos (optional, [B]) receives each system's AO count and
+// system_state (optional, [B]) the length of its SCC multipole state.
 int64_t batch_matrix_elements(const HostBatch& batch, int32_t* max_ao, std::string& error,
-                              int32_t* system_aos = nullptr);
+                              int32_t* system_aos = nullptr, int32_t* system_state = nullptr);
 
 // Runs a batch as up to `lanes.size()` contiguous, cost-balanced slices at
 // the same time, one host thread and one device stream per slice (lanes[0]
 // runs in the calling thread).  Results are identical to run_batch on the
 // whole batch: every system is computed independently.  Returns an xtbloom
-// status code like run_batch.
+// status code like run_batch; warm_state and scc_state are cut per slice.
+// caches (optional) holds one topology cache per lane; a cache is reused only
+// for the same slice of the same lane count.
 int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& batch,
                     const RunOptions& options, HostOutputs& out, RunTiming* timing,
-                    std::string& error);
+                    std::string& error, const std::vector<TopoCache*>* caches = nullptr);
 // Lane count for a batch: min(configured lanes, batch / kLaneMinSystems), at
 // least 1.  The configured count is XTBLOOM_AMD_LANES (default 1).
 int lanes_for_batch(int64_t batch);

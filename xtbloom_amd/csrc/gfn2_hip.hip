@@ -284,6 +284,22 @@ int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
 
 }  // namespace
 
+struct TopoCache {
+  HostTopo topo;
+  bool valid = false;
+  int64_t tag = -1;  // which slice of which lane split the topology belongs to
+};
+
+TopoCache* topo_cache_create() { return new TopoCache(); }
+void topo_cache_destroy(TopoCache* cache) { delete cache; }
+bool topo_cache_retag(TopoCache* cache, int64_t tag) {
+  if (cache->tag != tag) {
+    cache->valid = false;
+    cache->tag = tag;
+  }
+  return cache->valid;
+}
+
 struct Device {
   int id = 0;
   hipStream_t stream = nullptr;
@@ -385,11 +401,11 @@ void device_destroy(Device* d) {
 int device_id(const Device* d) { return d ? d->id : -1; }
 
 int64_t batch_matrix_elements(const HostBatch& hb, int32_t* max_ao, std::string& error,
-                              int32_t* system_aos) {
+                              int32_t* system_aos, int32_t* system_state) {
   int64_t total = 0;
   int32_t mx = 0;
   for (int64_t s = 0; s < hb.batch; ++s) {
-    int64_t nao = 0;
+    int64_t nao = 0, nsh = 0;
     for (int64_t a = hb.atom_offsets[s]; a < hb.atom_offsets[s + 1]; ++a) {
       const int z = hb.Z[a];
       if (z < 1 || z > XTBP_NELEM) {
@@ -398,7 +414,10 @@ int64_t batch_matrix_elements(const HostBatch& hb, int32_t* max_ao, std::string&
       }
       const int off = xtbp_elem_shell_offset[z - 1], cnt = xtbp_elem_shell_count[z - 1];
       for (int k = 0; k < cnt; ++k) nao += 2 * xtbp_shell_l[off + k] + 1;
+      nsh += cnt;
     }
+    if (system_state)
+      system_state[s] = (int32_t)(nsh + 9 * (hb.atom_offsets[s + 1] - hb.atom_offsets[s]));
     mx = std::max<int32_t>(mx, (int32_t)nao);
     if (system_aos) system_aos[s] = (int32_t)nao;
     total += nao * nao;
@@ -442,14 +461,24 @@ __global__ void k_valid(int B, const SysState* st, unsigned char* valid) {
 }  // namespace
 
 int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutputs& out,
-              RunTiming* timing, std::string& error) {
+              RunTiming* timing, std::string& error, TopoCache* cache) {
   if (hipSetDevice(dev->id) != hipSuccess) {
     error = "hipSetDevice failed";
     return 6;
   }
-  HostTopo T;
-  int st = build_topology(hb, T, error);
-  if (st != 0) return st;
+  HostTopo local;
+  HostTopo& T = cache ? cache->topo : local;
+  if (cache && cache->valid) {
+    // same topology: only the geometry changes
+    T.xyz.assign(hb.xyz, hb.xyz + 3 * (size_t)hb.atoms);
+    T.pc_xyz.assign(hb.pc_xyz, hb.pc_xyz + 3 * (size_t)T.NPC);
+    T.pc_q.assign(hb.pc_q, hb.pc_q + T.NPC);
+  } else {
+    if (cache) cache->valid = false;
+    const int st = build_topology(hb, T, error);
+    if (st != 0) return st;
+    if (cache) cache->valid = true;
+  }
   const int B = T.B, N = T.N, NS = T.NS, NAO = T.NAO, NPC = T.NPC;
   const int64_t NN = T.mat_off[B];
   const int64_t NG = T.gam_off[B];
@@ -581,6 +610,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     }
     dev->arena_bytes = want;
   }
+  if (timing) timing->workspace_bytes = bytes;
+  if (opt.reserve_only) return 0;
   Carver cv{dev->arena};
   hipStream_t sm = dev->stream;
   // uploads are collected here and issued together from pinned staging right
@@ -758,6 +789,9 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   mx.omega = cv.take<double>((size_t)mem * B);
   mx.mem = mem;
   mx.damping = opt.damping;
+  // WARM start: the state arrives in raw (free until the first Mulliken
+  // step) and replaces the FRESH state right after the geometry kernel
+  if (opt.warm_state) uploads.push_back({mx.raw, opt.warm_state, sizeof(double) * (size_t)NV});
   mx.qtol = opt.charge_tolerance;
   mx.etol = opt.energy_tolerance;
   mx.kt = opt.kt;
@@ -887,6 +921,10 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   hipLaunchKernelGGL(k_geometry, dim3(B), dim3(256), 0, sm, t, d_order, d_cn, d_cnd4, d_mrad,
                      d_dmrad, d_gam, d_vpc, mx, d_st, d_charges);
   HIP_TRY(probe_end(kKGeometry, 0.0, 0.0));
+  if (opt.warm_state) {
+    HIP_TRY(hipMemcpyAsync(mx.x, mx.raw, sizeof(double) * (size_t)NV, hipMemcpyDeviceToDevice, sm));
+    HIP_TRY(hipMemcpyAsync(mx.cur, mx.raw, sizeof(double) * (size_t)NV, hipMemcpyDeviceToDevice, sm));
+  }
   if (NSP > 0) {
     HIP_TRY(probe_begin(kKIntegrals));
     if (cls_sp[0].back() > 0)
@@ -1137,6 +1175,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   if (out.dbg_cn)
     HIP_TRY(hipMemcpyAsync(out.dbg_cn, d_cn, sizeof(double) * N, hipMemcpyDeviceToHost, sm));
   HIP_TRY(hipStreamSynchronize(sm));
+  if (out.scc_state) std::copy(hx.begin(), hx.end(), out.scc_state);
   if (timing) {
     HIP_TRY(hipEventElapsedTime(&timing->setup_ms, dev->ev[0], dev->ev[1]));
     HIP_TRY(hipEventElapsedTime(&timing->scc_ms, dev->ev[1], dev->ev[2]));

@@ -41,12 +41,16 @@ struct Slice {
   std::vector<KernelStat> stats;
   RunTiming timing;
   std::string error;
+  TopoCache* cache = nullptr;
   int rc = 0;
 };
 
+// exact identity of a slice: s0, s1 < 2^29 (batch sizes are below 2^28), K <= 4
+int64_t slice_tag(int64_t s0, int64_t s1, int K) { return (s0 << 34) | (s1 << 4) | K; }
+
 int run_guarded(Device* dev, Slice& sl, const RunOptions& opt) {
   try {
-    return run_batch(dev, sl.batch, opt, sl.out, &sl.timing, sl.error);
+    return run_batch(dev, sl.batch, opt, sl.out, &sl.timing, sl.error, sl.cache);
   } catch (const std::bad_alloc&) {
     sl.error = "host allocation failed";
     return 4;  // XTBLOOM_STATUS_ALLOCATION_FAILED
@@ -70,11 +74,16 @@ int lanes_for_batch(int64_t batch) {
 }
 
 int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, const RunOptions& opt,
-                    HostOutputs& out, RunTiming* timing, std::string& error) {
+                    HostOutputs& out, RunTiming* timing, std::string& error,
+                    const std::vector<TopoCache*>* caches) {
   const int K = (int)std::min<int64_t>((int64_t)lanes.size(), hb.batch);
-  if (K <= 1) return run_batch(lanes[0], hb, opt, out, timing, error);
-  std::vector<int32_t> nao(hb.batch);
-  if (batch_matrix_elements(hb, nullptr, error, nao.data()) < 0) return 1;
+  if (K <= 1) {
+    TopoCache* cache = caches ? (*caches)[0] : nullptr;
+    if (cache) topo_cache_retag(cache, slice_tag(0, hb.batch, 1));
+    return run_batch(lanes[0], hb, opt, out, timing, error, cache);
+  }
+  std::vector<int32_t> nao(hb.batch), ndim(hb.batch);
+  if (batch_matrix_elements(hb, nullptr, error, nao.data(), ndim.data()) < 0) return 1;
 
   // cut points: equal shares of sum n^3, every slice non-empty
   std::vector<double> cost(hb.batch + 1, 0.0);
@@ -89,8 +98,12 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
     cut[k] = c;
   }
 
-  std::vector<int64_t> mat_prefix(hb.batch + 1, 0);
-  for (int64_t s = 0; s < hb.batch; ++s) mat_prefix[s + 1] = mat_prefix[s] + (int64_t)nao[s] * nao[s];
+  std::vector<int64_t> mat_prefix(hb.batch + 1, 0), state_prefix(hb.batch + 1, 0);
+  for (int64_t s = 0; s < hb.batch; ++s) {
+    mat_prefix[s + 1] = mat_prefix[s] + (int64_t)nao[s] * nao[s];
+    state_prefix[s + 1] = state_prefix[s] + ndim[s];
+  }
+  std::vector<RunOptions> sopt(K, opt);
 
   std::vector<Slice> sl(K);
   for (int k = 0; k < K; ++k) {
@@ -137,6 +150,10 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
     o.dbg_density = at(out.dbg_density, m0);
     o.dbg_cn = at(out.dbg_cn, a0);
     o.dbg_components = at(out.dbg_components, 8 * x.s0);
+    o.scc_state = at(out.scc_state, state_prefix[x.s0]);
+    sopt[k].warm_state = at(opt.warm_state, state_prefix[x.s0]);
+    x.cache = caches ? (*caches)[k] : nullptr;
+    if (x.cache) topo_cache_retag(x.cache, slice_tag(x.s0, x.s1, K));
     if (out.kernel_stats) {
       x.stats.assign(kKernelCount, KernelStat{0, 0, 0.0, 0.0, 0.0});
       o.kernel_stats = x.stats.data();
@@ -149,14 +166,14 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
   int spawned = 1;
   try {
     for (int k = 1; k < K; ++k) {
-      workers.emplace_back([&, k]() { sl[k].rc = run_guarded(lanes[k], sl[k], opt); });
+      workers.emplace_back([&, k]() { sl[k].rc = run_guarded(lanes[k], sl[k], sopt[k]); });
       ++spawned;
     }
   } catch (...) {
     // could not start a thread: the remaining slices run in this thread
   }
-  sl[0].rc = run_guarded(lanes[0], sl[0], opt);
-  for (int k = spawned; k < K; ++k) sl[k].rc = run_guarded(lanes[k], sl[k], opt);
+  sl[0].rc = run_guarded(lanes[0], sl[0], sopt[0]);
+  for (int k = spawned; k < K; ++k) sl[k].rc = run_guarded(lanes[k], sl[k], sopt[k]);
   for (auto& w : workers) w.join();
 
   for (int k = 0; k < K; ++k) {
@@ -185,6 +202,7 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
       t.force_ms = std::max(t.force_ms, x.timing.force_ms);
       t.total_ms = std::max(t.total_ms, x.timing.total_ms);
       t.scc_rounds = std::max(t.scc_rounds, x.timing.scc_rounds);
+      t.workspace_bytes += x.timing.workspace_bytes;
     }
     *timing = t;
   }

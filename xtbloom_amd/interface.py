@@ -168,15 +168,14 @@ class ComputedBatch:
     extension: dict | None = None
 
 
-def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=None, uhf=None,
-                   point_offsets=None, point_positions=None, point_values=None,
-                   point_gammas=None, *, model=library.MODEL_GFN2_XTB,
-                   flags=library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES,
-                   max_scc_iterations=250, charge_tolerance=1e-6, energy_tolerance=1e-8,
-                   electronic_temperature=300.0, scc_mixer_history=8, scc_mixer_damping=0.4,
-                   extension=False, diagnostics=False, kernel_stats=False) -> ComputedBatch:
-    """One ``xtbloom_compute`` over flat ragged arrays (the ``xtbloom_batch_t`` layout)."""
-    lib = library.load()
+_DEFAULT_FLAGS = library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES
+
+
+def _descriptors(lib, atom_offsets, numbers, positions, charges, uhf, point_offsets,
+                 point_positions, point_values, point_gammas, model, flags, max_scc_iterations,
+                 charge_tolerance, energy_tolerance, electronic_temperature, scc_mixer_history,
+                 scc_mixer_damping, scc_start_mode):
+    """Batch and compute-options descriptors over flat host arrays (kept alive by the caller)."""
     atom_offsets = np.ascontiguousarray(atom_offsets, dtype=np.int64)
     numbers = np.ascontiguousarray(numbers, dtype=np.int32)
     positions = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
@@ -215,6 +214,33 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
     opts.electronic_temperature = electronic_temperature * library.KELVIN_TO_HARTREE
     opts.scc_mixer_history = scc_mixer_history
     opts.scc_mixer_damping = scc_mixer_damping
+    opts.scc_start_mode = scc_start_mode
+    keep = (atom_offsets, numbers, positions, charges, uhf, point_offsets, point_positions,
+            point_values, point_gammas)
+    return batch, opts, keep, (B, N, P)
+
+
+def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=None, uhf=None,
+                   point_offsets=None, point_positions=None, point_values=None,
+                   point_gammas=None, *, model=library.MODEL_GFN2_XTB, flags=_DEFAULT_FLAGS,
+                   max_scc_iterations=250, charge_tolerance=1e-6, energy_tolerance=1e-8,
+                   electronic_temperature=300.0, scc_mixer_history=8, scc_mixer_damping=0.4,
+                   scc_start_mode=library.SCC_START_FRESH, plan: "Plan | None" = None,
+                   extension=False, diagnostics=False, kernel_stats=False) -> ComputedBatch:
+    """One ``xtbloom_compute`` over flat ragged arrays (the ``xtbloom_batch_t`` layout).
+
+    ``scc_start_mode=library.SCC_START_WARM`` starts from the converged state of
+    the latest fully converged call with the same topology and policy (strict:
+    no such call raises ``XTBloomError`` with INVALID_ARGUMENT).  With ``plan``
+    the call goes through ``xtbloom_plan_compute`` of that plan.
+    """
+    lib = library.load()
+    batch, opts, keep, (B, N, P) = _descriptors(
+        lib, atom_offsets, numbers, positions, charges, uhf, point_offsets, point_positions,
+        point_values, point_gammas, model, flags, max_scc_iterations, charge_tolerance,
+        energy_tolerance, electronic_temperature, scc_mixer_history, scc_mixer_damping,
+        scc_start_mode)
+    atom_offsets, point_offsets = keep[0], keep[5]
     energies = np.full(B, np.nan)
     forces = np.full((N, 3), np.nan)
     q = np.full(N, np.nan)
@@ -235,7 +261,12 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
     res.per_system_status = _out(status)
     ext_out = None
     dedq = None
-    if extension or diagnostics or kernel_stats:
+    if plan is not None:
+        if extension or diagnostics or kernel_stats:
+            raise XTBloomValueError("plan calls take no extension outputs")
+        _check(lib.xtbloom_plan_compute(plan.handle, ctypes.byref(batch), ctypes.byref(opts),
+                                        ctypes.byref(res)))
+    elif extension or diagnostics or kernel_stats:
         ext = library.Extension()
         ext.struct_size = ctypes.sizeof(ext)
         ext.api_version = library.API_VERSION
@@ -268,6 +299,75 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                                    ctypes.byref(res)))
     return ComputedBatch(atom_offsets, point_offsets, energies, forces, q, pcf, dip, iters,
                          conv.astype(bool), status, dedq, ext_out)
+
+
+class Plan:
+    """Fixed-topology plan (``xtbloom_plan_create``) over flat ragged arrays.
+
+    The plan binds the topology (atom offsets, element numbers, molecular
+    charges, unpaired electrons, point-charge offsets and gammas) and the
+    compute policy, and reserves the device workspace; :meth:`compute` takes
+    new geometry only (reference ``python/xtbloom/library.py`` ``Plan``).  A
+    plan keeps its own WARM checkpoint.  Close it before its context.
+    """
+
+    def __init__(self, context: Context, atom_offsets, numbers, positions, charges=None,
+                 uhf=None, point_offsets=None, point_positions=None, point_values=None,
+                 point_gammas=None, *, model=library.MODEL_GFN2_XTB, flags=_DEFAULT_FLAGS,
+                 max_scc_iterations=250, charge_tolerance=1e-6, energy_tolerance=1e-8,
+                 electronic_temperature=300.0, scc_mixer_history=8, scc_mixer_damping=0.4):
+        lib = library.load()
+        self._policy = dict(model=model, flags=flags, max_scc_iterations=max_scc_iterations,
+                            charge_tolerance=charge_tolerance, energy_tolerance=energy_tolerance,
+                            electronic_temperature=electronic_temperature,
+                            scc_mixer_history=scc_mixer_history,
+                            scc_mixer_damping=scc_mixer_damping)
+        batch, opts, keep, _ = _descriptors(
+            lib, atom_offsets, numbers, positions, charges, uhf, point_offsets, point_positions,
+            point_values, point_gammas, scc_start_mode=library.SCC_START_FRESH, **self._policy)
+        self._topology = dict(atom_offsets=keep[0], numbers=keep[1], charges=keep[3],
+                              uhf=keep[4], point_offsets=keep[5], point_gammas=keep[8])
+        self._flags = int(opts.flags)
+        handle = ctypes.c_void_p()
+        _check(lib.xtbloom_plan_create(context.handle, ctypes.byref(batch), ctypes.byref(opts),
+                                       ctypes.byref(handle)))
+        self._handle = handle
+        self._context = context  # the plan must not outlive its context
+
+    @property
+    def handle(self):
+        if not self._handle:
+            raise XTBloomError(library.STATUS_INVALID_ARGUMENT, "fixed-topology plan is closed")
+        return self._handle
+
+    def compute(self, positions, point_positions=None, point_values=None, *,
+                scc_start_mode=library.SCC_START_FRESH) -> ComputedBatch:
+        """One ``xtbloom_plan_compute`` with new geometry."""
+        return compute_arrays(self._context, positions=positions, point_positions=point_positions,
+                              point_values=point_values, scc_start_mode=scc_start_mode, plan=self,
+                              **self._topology, **self._policy)
+
+    def query_workspace(self) -> dict:
+        """Reserved workspace (``xtbloom_plan_query_workspace``)."""
+        lib = library.load()
+        q = library.WorkspaceQuery()
+        _check(lib.xtbloom_workspace_query_init(ctypes.byref(q), ctypes.sizeof(q)))
+        q.compute_flags = self._flags
+        _check(lib.xtbloom_plan_query_workspace(self.handle, ctypes.byref(q)))
+        return dict(host_bytes=q.host_required_bytes, host_alignment=q.host_required_alignment,
+                    device_bytes=q.device_required_bytes,
+                    device_alignment=q.device_required_alignment)
+
+    def close(self):
+        if getattr(self, "_handle", None):
+            library.load().xtbloom_plan_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _stats_mask(kernel_stats) -> int:
@@ -303,7 +403,8 @@ def compute_tensors(context: Context, atom_offsets, numbers, positions, charges,
                     | library.COMPUTE_ATOMIC_CHARGES, max_scc_iterations=250,
                     charge_tolerance=1e-6, energy_tolerance=1e-8, electronic_temperature=300.0,
                     scc_mixer_history=8, scc_mixer_damping=0.4, outputs=None,
-                    point_charge_potential_gradient=False, kernel_stats=False):
+                    point_charge_potential_gradient=False, kernel_stats=False,
+                    scc_start_mode=library.SCC_START_FRESH, plan: "Plan | None" = None):
     """``xtbloom_compute`` on torch tensors (device-resident inputs and outputs stay on the GPU).
 
     Inputs follow the ``xtbloom_batch_t`` dtypes (int64 offsets, int32 numbers and
@@ -340,6 +441,7 @@ def compute_tensors(context: Context, atom_offsets, numbers, positions, charges,
     opts.electronic_temperature = electronic_temperature * library.KELVIN_TO_HARTREE
     opts.scc_mixer_history = scc_mixer_history
     opts.scc_mixer_damping = scc_mixer_damping
+    opts.scc_start_mode = scc_start_mode
     out = dict(outputs or {})
     f64 = dict(dtype=torch.float64, device=dev)
     out.setdefault("energies", torch.empty(B, **f64))
@@ -375,6 +477,13 @@ def compute_tensors(context: Context, atom_offsets, numbers, positions, charges,
         ext.kernel_stats_mask = _stats_mask(kernel_stats)
     if dev.type == "cuda":
         torch.cuda.synchronize(dev)
+    if plan is not None:
+        # plan calls carry no extension: no dE/dQ and no kernel statistics
+        if host_dedq is not None or stats is not None:
+            raise XTBloomValueError("plan calls take no extension outputs")
+        _check(lib.xtbloom_plan_compute(plan.handle, ctypes.byref(batch), ctypes.byref(opts),
+                                        ctypes.byref(res)))
+        return out, dict(setup_ms=0.0, scc_ms=0.0, force_ms=0.0, total_ms=0.0, scc_rounds=0)
     _check(lib.xtbloom_amd_compute_extended(context.handle, ctypes.byref(batch),
                                             ctypes.byref(opts), ctypes.byref(res),
                                             ctypes.byref(ext)))
@@ -504,6 +613,25 @@ class _Settings:
         return dict(vars(self))
 
 
+def _compute_auto_warm(context: Context, warm_start: bool, *args, **kw) -> ComputedBatch:
+    """compute_arrays with the reference calculators' opt-in automatic warm start
+    (reference python/xtbloom/interface.py:1274-1281, 1366-1388): WARM when the
+    context holds a fully converged checkpoint, one FRESH retry when the native
+    gate rejects it (changed identity), and the readiness flag mirrored from the
+    call's statuses."""
+    ready = warm_start and getattr(context, "_warm_ready", False)
+    mode = library.SCC_START_WARM if ready else library.SCC_START_FRESH
+    try:
+        computed = compute_arrays(context, *args, scc_start_mode=mode, **kw)
+    except XTBloomError as error:
+        if mode != library.SCC_START_WARM or error.status != library.STATUS_INVALID_ARGUMENT:
+            raise
+        computed = compute_arrays(context, *args, scc_start_mode=library.SCC_START_FRESH, **kw)
+    context._warm_ready = bool(np.all(computed.per_system_status == 0) and
+                               np.all(computed.scc_converged))
+    return computed
+
+
 def _resolve_method(method: str) -> int:
     key = method.upper()
     if key not in _METHODS:
@@ -516,12 +644,15 @@ class Calculator(Structure):
 
     def __init__(self, method: str, numbers, positions, charge: float = 0.0, uhf: int | None = None,
                  point_charges: PointCharge | None = None, *, backend: str | int = "hip",
-                 device_id: int | None = None, **settings):
+                 device_id: int | None = None, warm_start: bool = False, **settings):
         super().__init__(numbers, positions, charge, uhf, point_charges)
         self._model = _resolve_method(method)
         self._method = method
         self._context = Context(backend, device_id)
         self._settings = _Settings(**settings)
+        # warm_start=True seeds each SCC from the previous fully converged call
+        # with the same topology and settings (native SCC_START_WARM)
+        self._warm_start = bool(warm_start)
 
     @property
     def backend(self) -> int:
@@ -539,9 +670,9 @@ class Calculator(Structure):
         if self.point_charges is not None:
             flags |= library.COMPUTE_POINT_CHARGE_FORCES
         offs, numbers, positions, charges, uhf, kw = _flatten([self])
-        computed = compute_arrays(self._context, offs, numbers, positions, charges, uhf, **kw,
-                                  model=self._model, flags=flags, extension=True,
-                                  **self._settings.kwargs())
+        computed = _compute_auto_warm(self._context, self._warm_start, offs, numbers, positions,
+                                      charges, uhf, **kw, model=self._model, flags=flags,
+                                      extension=True, **self._settings.kwargs())
         BatchResult(computed).raise_for_status()
         return Result(computed, 0)
 
@@ -553,13 +684,15 @@ class BatchCalculator:
     """Batched GFN2-xTB over many structures in one native call."""
 
     def __init__(self, structures: Sequence[Structure], method: str = "GFN2-xTB", *,
-                 backend: str | int = "hip", device_id: int | None = None, **settings):
+                 backend: str | int = "hip", device_id: int | None = None,
+                 warm_start: bool = False, **settings):
         if not structures:
             raise XTBloomValueError("a batch needs at least one structure")
         self._structures = list(structures)
         self._model = _resolve_method(method)
         self._context = Context(backend, device_id)
         self._settings = _Settings(**settings)
+        self._warm_start = bool(warm_start)
 
     @property
     def backend(self) -> int:
@@ -574,8 +707,9 @@ class BatchCalculator:
             flags = library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES
             if kw:
                 flags |= library.COMPUTE_POINT_CHARGE_FORCES
-        computed = compute_arrays(self._context, offs, numbers, positions, charges, uhf, **kw,
-                                  model=self._model, flags=flags, **self._settings.kwargs())
+        computed = _compute_auto_warm(self._context, self._warm_start, offs, numbers, positions,
+                                      charges, uhf, **kw, model=self._model, flags=flags,
+                                      **self._settings.kwargs())
         return BatchResult(computed)
 
     def close(self):

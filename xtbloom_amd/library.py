@@ -128,6 +128,15 @@ class Extension(ctypes.Structure):
                 ("kernel_stats_mask", ctypes.c_uint32)]
 
 
+class WorkspaceQuery(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("api_version", ctypes.c_uint32),
+                ("compute_flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("host_required_bytes", ctypes.c_uint64),
+                ("host_required_alignment", ctypes.c_uint32),
+                ("device_required_bytes", ctypes.c_uint64),
+                ("device_required_alignment", ctypes.c_uint32), ("reserved_v2", ctypes.c_uint32)]
+
+
 # Byte images asserted by the reference ABI for LP64 (reference xtbloom.h:320-343).
 assert ctypes.sizeof(ContextOptions) == 32
 assert ctypes.sizeof(Buffer) == 24
@@ -135,6 +144,7 @@ assert ctypes.sizeof(Batch) == 456
 assert ctypes.sizeof(ComputeOptions) == 80
 assert ctypes.sizeof(BatchResult) == 280
 assert ctypes.sizeof(KernelStat) == 32
+assert ctypes.sizeof(WorkspaceQuery) == 48 and WorkspaceQuery.device_required_bytes.offset == 32
 
 EXPORTED_SYMBOLS = (
     "xtbloom_version_string", "xtbloom_status_string", "xtbloom_get_last_error",
@@ -198,6 +208,18 @@ def load() -> ctypes.CDLL:
     lib.xtbloom_amd_compute_extended.restype = ctypes.c_int32
     lib.xtbloom_amd_packed_matrix_elements.argtypes = [ctypes.POINTER(Batch)]
     lib.xtbloom_amd_packed_matrix_elements.restype = ctypes.c_int64
+    lib.xtbloom_workspace_query_init.argtypes = [ctypes.POINTER(WorkspaceQuery), ctypes.c_size_t]
+    lib.xtbloom_workspace_query_init.restype = ctypes.c_int32
+    lib.xtbloom_plan_create.argtypes = [vp, ctypes.POINTER(Batch), ctypes.POINTER(ComputeOptions),
+                                        ctypes.POINTER(vp)]
+    lib.xtbloom_plan_create.restype = ctypes.c_int32
+    lib.xtbloom_plan_destroy.argtypes = [vp]
+    lib.xtbloom_plan_destroy.restype = None
+    lib.xtbloom_plan_query_workspace.argtypes = [vp, ctypes.POINTER(WorkspaceQuery)]
+    lib.xtbloom_plan_query_workspace.restype = ctypes.c_int32
+    lib.xtbloom_plan_compute.argtypes = [vp, ctypes.POINTER(Batch), ctypes.POINTER(ComputeOptions),
+                                         ctypes.POINTER(BatchResult)]
+    lib.xtbloom_plan_compute.restype = ctypes.c_int32
     _LIB = lib
     return lib
 
