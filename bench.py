@@ -214,6 +214,15 @@ def main():
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    # HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated on a copy)
+    tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{args.workload}_fp64.json")
+    if os.path.exists(tpath) and args.scc_start == "fresh" and args.batch is None:
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        if tr.get("kernel") == dom:
+            roof["traffic"] = tr["traffic_bytes_per_launch"]
+            roof["traffic_source"] = f"profiles/hbm_traffic_{args.workload}_fp64.json (rocprofv3 --pmc)"
     roof.update(kernel=dom, launches_per_step=d["launches"] / (max(args.warmup, 1) if plan else args.steps),
                 avg_launch_ms=avg_ms, algorithmic_per_launch=per_launch)
     if plan:
