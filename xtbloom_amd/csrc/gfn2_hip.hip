@@ -490,7 +490,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   // GEMM tile queues and eigenvector back-transformation column blocks
   std::vector<TileTask> full, lower;
   std::vector<BackTask> back;
-  std::vector<ElemChunk> chunks;
+  std::vector<FockTile> chunks;
   std::vector<RowChunk> rowchunks;
   std::vector<int64_t> part_off(B + 1, 0);
   size_t back_lds = 0;
@@ -510,7 +510,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
       back_lds = std::max(back_lds, (size_t)(n * cw + 2 * n + kBackThreads) * sizeof(double));
     }
     for (int j0 = 0; j0 < n; j0 += cw) back.push_back({s, j0, cw, 0});
-    for (int e0 = 0; e0 < n * n; e0 += kFockChunk) chunks.push_back({s, e0});
+    for (int r0 = 0; r0 < n; r0 += kFockTile)
+      for (int c0 = r0; c0 < n; c0 += kFockTile) chunks.push_back({s, r0, c0});
   }
   // systems split by whether any shell is a d shell: [0] s/p only, [1] with d;
   // per class the prefix of ordered shell pairs and of ordered atom pairs
@@ -561,7 +562,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(3 * (size_t)NPC); addd(NPC); addd(NPC);
   bytes += plan_bytes<TileTask>(full.size()) + plan_bytes<TileTask>(lower.size());
   bytes += plan_bytes<BackTask>(back.size());
-  bytes += plan_bytes<ElemChunk>(chunks.size());
+  bytes += plan_bytes<FockTile>(chunks.size());
   bytes += plan_bytes<RowChunk>(rowchunks.size());
   for (int c = 0; c < 2; ++c) {
     addi(cls_sys[c].size());
@@ -663,7 +664,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   TileTask* d_full = cv.take<TileTask>(full.size());
   TileTask* d_lower = cv.take<TileTask>(lower.size());
   BackTask* d_back = cv.take<BackTask>(back.size());
-  ElemChunk* d_chunks = cv.take<ElemChunk>(chunks.size());
+  FockTile* d_chunks = cv.take<FockTile>(chunks.size());
   RowChunk* d_rowchunks = cv.take<RowChunk>(rowchunks.size());
   SysList sp_list[2], ap_list[2];
   for (int c = 0; c < 2; ++c) {
@@ -972,7 +973,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     auto set = [&](size_t idx, double flops, double bytes) {
       if (idx) { probes[idx - 1].flops = flops; probes[idx - 1].bytes = bytes; }
     };
-    set(rp.fock, 0.0, 96.0 * a2);      // reads S, D(3), Q(6), H0; writes H
+    set(rp.fock, 0.0, 52.0 * a2);      // reads half of S, D(3), Q(6), H0 (H symmetric); writes H
     set(rp.red1, a3, 0.0);             // X^T H, X upper triangular: n^3
     set(rp.red2, a3 / 3.0, 0.0);       // (X^T H) X, symmetric: n^3/3
     set(rp.tri, (4.0 / 3.0) * a3, 0.0);

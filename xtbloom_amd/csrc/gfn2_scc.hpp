@@ -478,15 +478,56 @@ __global__ __launch_bounds__(512) void k_potentials(Topo t, const int* order,
 // the ket atom): with d = R(A_n) - R(A_m),
 //   D_nm,a = D_mn,a + d_a S_mn
 //   M_nm,ab = M_mn,ab + d_a D_mn,b + d_b D_mn,a + d_a d_b S_mn      (raw second moments)
-// and the traceless Q = 3/2 M - 1/2 tr(M) 1, so only coalesced reads are needed.
-// One block = 1024 consecutive elements of one system (host-built chunk list).
-struct ElemChunk {
+// and the traceless Q = 3/2 M - 1/2 tr(M) 1.  Element (m, n) therefore needs
+// only S, D, Q, H0 at (m, n), and H is symmetric: one block reads a 32 x 32
+// tile on or above the diagonal and writes it and, through LDS, its mirror
+// below the diagonal, so S, D, Q and H0 are read about once per pair.
+struct FockTile {
   int sys;
-  int start;  // first element (row-major) of the chunk inside the system's n x n matrix
+  int r0, c0;  // tile origin, r0 <= c0
 };
-constexpr int kFockChunk = 1024;
+constexpr int kFockTile = 32;
 
-__global__ __launch_bounds__(256) void k_fock(Topo t, const ElemChunk* chunks,
+// D and traceless Q of element (n, m) from those of element (m, n), m on atom
+// ar and n on atom ac ([xx, xy, yy, xz, yz, zz])
+__device__ __forceinline__ void transposed_moments(const Topo& t, int ar, int ac, double sv,
+                                                   const double (&dm)[3], const double (&qm)[6],
+                                                   double (&dt)[3], double (&qt)[6]) {
+  const double dl[3] = {t.xyz[3 * ac] - t.xyz[3 * ar], t.xyz[3 * ac + 1] - t.xyz[3 * ar + 1],
+                        t.xyz[3 * ac + 2] - t.xyz[3 * ar + 2]};
+#pragma unroll
+  for (int x = 0; x < 3; ++x) dt[x] = dm[x] + dl[x] * sv;
+  const double dd = dl[0] * dm[0] + dl[1] * dm[1] + dl[2] * dm[2];
+  const double l2 = dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2];
+  const double trc = 0.5 * (2.0 * dd + l2 * sv);
+  qt[0] = qm[0] + 1.5 * (2.0 * dl[0] * dm[0] + dl[0] * dl[0] * sv) - trc;
+  qt[1] = qm[1] + 1.5 * (dl[0] * dm[1] + dl[1] * dm[0] + dl[0] * dl[1] * sv);
+  qt[2] = qm[2] + 1.5 * (2.0 * dl[1] * dm[1] + dl[1] * dl[1] * sv) - trc;
+  qt[3] = qm[3] + 1.5 * (dl[0] * dm[2] + dl[2] * dm[0] + dl[0] * dl[2] * sv);
+  qt[4] = qm[4] + 1.5 * (dl[1] * dm[2] + dl[2] * dm[1] + dl[1] * dl[2] * sv);
+  qt[5] = qm[5] + 1.5 * (2.0 * dl[2] * dm[2] + dl[2] * dl[2] * sv) - trc;
+}
+
+__device__ __forceinline__ double fock_element(const Topo& t, int o, int r, int c, double sv,
+                                               const double (&dm)[3], const double (&qm)[6],
+                                               double h0, const double* __restrict__ vsh,
+                                               const double* __restrict__ vd,
+                                               const double* __restrict__ vqq) {
+  const int ar = t.ao_atom[o + r], ac = t.ao_atom[o + c];
+  double dt[3], qt[6];
+  transposed_moments(t, ar, ac, sv, dm, qm, dt, qt);
+  double h = -0.5 * sv * (vsh[t.ao_sh[o + r]] + vsh[t.ao_sh[o + c]]);
+#pragma unroll
+  for (int x = 0; x < 3; ++x) h -= 0.5 * (dm[x] * vd[3 * ac + x] + dt[x] * vd[3 * ar + x]);
+#pragma unroll
+  for (int x = 0; x < 6; ++x) h -= 0.5 * (qm[x] * vqq[6 * ac + x] + qt[x] * vqq[6 * ar + x]);
+  return h0 + h;
+}
+
+// One block = one tile (host-built list of the tiles with r0 <= c0).  Thread
+// (rg, cl) = (tid / 32, tid % 32) handles rows rg, rg + 8, rg + 16, rg + 24
+// of column cl: row segments of 32 doubles are read and written whole.
+__global__ __launch_bounds__(256) void k_fock(Topo t, const FockTile* tiles,
                                               const unsigned char* active,
                                               const double* __restrict__ S,
                                               const double* __restrict__ D,
@@ -496,48 +537,40 @@ __global__ __launch_bounds__(256) void k_fock(Topo t, const ElemChunk* chunks,
                                               const double* __restrict__ vd,
                                               const double* __restrict__ vqq,
                                               double* __restrict__ H) {
-  const ElemChunk ch = chunks[blockIdx.x];
-  const int s = ch.sys;
+  const FockTile tl = tiles[blockIdx.x];
+  const int s = tl.sys;
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
   const int o = t.ao_off[s];
   const int64_t mo = t.mat_off[s], nn = (int64_t)n * n;
   const double* Ds = D + 3 * mo;
   const double* Qs = Q + 6 * mo;
+  __shared__ double mirror[kFockTile][kFockTile + 1];
+  const int cl = threadIdx.x & 31, rg = threadIdx.x >> 5;
+  const int c = tl.c0 + cl;
 #pragma unroll
-  for (int q = 0; q < kFockChunk / 256; ++q) {
-    const int e = ch.start + q * 256 + threadIdx.x;
-    if (e >= nn) break;
-    const int r = e / n, c = e - r * n;
-    const int ar = t.ao_atom[o + r], ac = t.ao_atom[o + c];
-    const double sv = S[mo + e];
-    double dm[3], qm[6];
+  for (int i = 0; i < 4; ++i) {
+    const int rl = rg + 8 * i, r = tl.r0 + rl;
+    if (r < n && c < n) {
+      const int64_t e = (int64_t)r * n + c;
+      double dm[3], qm[6];
 #pragma unroll
-    for (int x = 0; x < 3; ++x) dm[x] = Ds[x * nn + e];
+      for (int x = 0; x < 3; ++x) dm[x] = Ds[x * nn + e];
 #pragma unroll
-    for (int x = 0; x < 6; ++x) qm[x] = Qs[x * nn + e];
-    const double dl[3] = {t.xyz[3 * ac] - t.xyz[3 * ar], t.xyz[3 * ac + 1] - t.xyz[3 * ar + 1],
-                          t.xyz[3 * ac + 2] - t.xyz[3 * ar + 2]};
-    // transposed dipole and traceless quadrupole ([xx, xy, yy, xz, yz, zz])
-    double dt[3];
+      for (int x = 0; x < 6; ++x) qm[x] = Qs[x * nn + e];
+      const double h = fock_element(t, o, r, c, S[mo + e], dm, qm, H0[mo + e], vsh, vd, vqq);
+      H[mo + e] = h;
+      mirror[cl][rl] = h;
+    }
+  }
+  if (tl.r0 == tl.c0) return;  // a diagonal tile holds both halves itself (uniform per block)
+  __syncthreads();
+  // H(c, r) = H(r, c): row c0 + rg + 8 i of H, columns r0 .. r0 + 31
+  const int rc = tl.r0 + cl;
 #pragma unroll
-    for (int x = 0; x < 3; ++x) dt[x] = dm[x] + dl[x] * sv;
-    const double dd = dl[0] * dm[0] + dl[1] * dm[1] + dl[2] * dm[2];
-    const double l2 = dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2];
-    const double trc = 0.5 * (2.0 * dd + l2 * sv);
-    const double qt[6] = {
-        qm[0] + 1.5 * (2.0 * dl[0] * dm[0] + dl[0] * dl[0] * sv) - trc,
-        qm[1] + 1.5 * (dl[0] * dm[1] + dl[1] * dm[0] + dl[0] * dl[1] * sv),
-        qm[2] + 1.5 * (2.0 * dl[1] * dm[1] + dl[1] * dl[1] * sv) - trc,
-        qm[3] + 1.5 * (dl[0] * dm[2] + dl[2] * dm[0] + dl[0] * dl[2] * sv),
-        qm[4] + 1.5 * (dl[1] * dm[2] + dl[2] * dm[1] + dl[1] * dl[2] * sv),
-        qm[5] + 1.5 * (2.0 * dl[2] * dm[2] + dl[2] * dl[2] * sv) - trc};
-    double h = -0.5 * sv * (vsh[t.ao_sh[o + r]] + vsh[t.ao_sh[o + c]]);
-#pragma unroll
-    for (int x = 0; x < 3; ++x) h -= 0.5 * (dm[x] * vd[3 * ac + x] + dt[x] * vd[3 * ar + x]);
-#pragma unroll
-    for (int x = 0; x < 6; ++x) h -= 0.5 * (qm[x] * vqq[6 * ac + x] + qt[x] * vqq[6 * ar + x]);
-    H[mo + e] = H0[mo + e] + h;
+  for (int i = 0; i < 4; ++i) {
+    const int ml = rg + 8 * i, m = tl.c0 + ml;
+    if (m < n && rc < n) H[mo + (int64_t)m * n + rc] = mirror[ml][cl];
   }
 }
 
