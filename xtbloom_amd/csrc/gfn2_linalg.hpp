@@ -31,6 +31,14 @@ constexpr int kTileK = 16;
 //     1  A read transposed and upper triangular (X^T H):  k < i0 + 64
 //     2  B upper triangular (T X):                         k < j0 + 64
 //     3  A upper triangular (X Y):                         k >= i0
+// FP64 matrix-core product of one 16 x 16 x 4 step, v_mfma_f64_16x16x4f64:
+// lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and accumulates
+// C[l / 16 + 4 r][l % 16], r = 0..3 (layout checked on gfx950).
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ mfma_d4 mfma_16x16x4(double a, double b, mfma_d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 template <bool TA, bool TB, bool SYMM, int TRI = 0>
 __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, const double* A,
                                               const double* Bm, double* C, const double* scale,
@@ -50,19 +58,27 @@ __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, con
   const int i0 = task.ti * kTile, j0 = task.tj * kTile;
   __shared__ double As[kTileK][kTile + 1];
   __shared__ double Bs[kTileK][kTile + 1];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // 16 x 16 threads, 4 x 4 each
-  double acc[4][4];
+  // wave w owns the 32 x 32 quadrant (w / 2, w % 2): 2 x 2 blocks of 16 x 16
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int qi = 32 * (w >> 1), qj = 32 * (w & 1);
+  const int l16 = lane & 15, lk = lane >> 4;
+  mfma_d4 acc[2][2];
 #pragma unroll
-  for (int p = 0; p < 4; ++p)
+  for (int p = 0; p < 2; ++p)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
+    for (int q = 0; q < 2; ++q) acc[p][q] = mfma_d4{0.0, 0.0, 0.0, 0.0};
   int kbeg = 0, kend = K;
   if constexpr (TRI == 1) kend = min(K, i0 + kTile);
   if constexpr (TRI == 2) kend = min(K, j0 + kTile);
   if constexpr (TRI == 3) kbeg = i0;
-  for (int k0 = kbeg; k0 < kend; k0 += kTileK) {
-    // load A tile: rows i0..i0+63, k k0..k0+15 -> As[k][i]
-    for (int idx = threadIdx.x; idx < kTile * kTileK; idx += 256) {
+  // Stage k0 of A and B is fetched into registers one step ahead: its global
+  // loads are in flight while the matrix cores work on the previous stage.
+  constexpr int kPer = kTile * kTileK / 256;  // elements of each operand per thread
+  double pa[kPer], pb[kPer];
+  auto fetch = [&](int k0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int idx = threadIdx.x + 256 * u;
       int kk, ii;
       if (TA) { ii = idx % kTile; kk = idx / kTile; } else { kk = idx % kTileK; ii = idx / kTileK; }
       const int gi = i0 + ii, gk = k0 + kk;
@@ -71,43 +87,56 @@ __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, con
         val = TA ? a[(int64_t)gk * n + gi] : a[(int64_t)gi * n + gk];
         if (sc) val *= sc[gk];
       }
-      As[kk][ii] = val;
+      pa[u] = val;
     }
-    for (int idx = threadIdx.x; idx < kTile * kTileK; idx += 256) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int idx = threadIdx.x + 256 * u;
       int kk, jj;
       if (TB) { kk = idx % kTileK; jj = idx / kTileK; } else { jj = idx % kTile; kk = idx / kTile; }
       const int gj = j0 + jj, gk = k0 + kk;
-      double val = 0.0;
-      if (gj < n && gk < K) val = TB ? b[(int64_t)gj * n + gk] : b[(int64_t)gk * n + gj];
-      Bs[kk][jj] = val;
+      pb[u] = (gj < n && gk < K) ? (TB ? b[(int64_t)gj * n + gk] : b[(int64_t)gk * n + gj]) : 0.0;
+    }
+  };
+  if (kbeg < kend) fetch(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += kTileK) {
+    // A tile rows i0..i0+63 -> As[k][i], B tile -> Bs[k][j]
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int idx = threadIdx.x + 256 * u;
+      if (TA) As[idx / kTile][idx % kTile] = pa[u]; else As[idx % kTileK][idx / kTileK] = pa[u];
+      if (TB) Bs[idx % kTileK][idx / kTileK] = pb[u]; else Bs[idx / kTile][idx % kTile] = pb[u];
     }
     __syncthreads();
+    if (k0 + kTileK < kend) fetch(k0 + kTileK);
 #pragma unroll
-    for (int kk = 0; kk < kTileK; ++kk) {
-      double ar[4], br[4];
+    for (int k4 = 0; k4 < kTileK; k4 += 4) {
+      double af[2], bf[2];
 #pragma unroll
-      for (int p = 0; p < 4; ++p) ar[p] = As[kk][ty + 16 * p];
+      for (int p = 0; p < 2; ++p) af[p] = As[k4 + lk][qi + 16 * p + l16];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) br[q] = Bs[kk][tx + 16 * q];
+      for (int q = 0; q < 2; ++q) bf[q] = Bs[k4 + lk][qj + 16 * q + l16];
 #pragma unroll
-      for (int p = 0; p < 4; ++p)
+      for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[p][q] = fma(ar[p], br[q], acc[p][q]);
+        for (int q = 0; q < 2; ++q) acc[p][q] = mfma_16x16x4(af[p], bf[q], acc[p][q]);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    const int gi = i0 + ty + 16 * p;
-    if (gi >= n) continue;
+  for (int p = 0; p < 2; ++p)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int gj = j0 + tx + 16 * q;
-      if (gj >= n) continue;
-      c[(int64_t)gi * n + gj] = acc[p][q];
-      if (SYMM && task.ti != task.tj) c[(int64_t)gj * n + gi] = acc[p][q];
+    for (int r = 0; r < 4; ++r) {
+      const int gi = i0 + qi + 16 * p + lk + 4 * r;
+      if (gi >= n) continue;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int gj = j0 + qj + 16 * q + l16;
+        if (gj >= n) continue;
+        c[(int64_t)gi * n + gj] = acc[p][q][r];
+        if (SYMM && task.ti != task.tj) c[(int64_t)gj * n + gi] = acc[p][q][r];
+      }
     }
-  }
 }
 
 // Cholesky S = L L^T and X = L^{-T} with everything in global memory (any n);
