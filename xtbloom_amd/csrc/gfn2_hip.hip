@@ -491,7 +491,6 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   std::vector<TileTask> full, lower;
   std::vector<BackTask> back;
   std::vector<FockTile> chunks;
-  std::vector<RowChunk> rowchunks;
   std::vector<int64_t> part_off(B + 1, 0);
   size_t back_lds = 0;
   for (int i = 0; i < B; ++i) {
@@ -529,17 +528,12 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     cls_sp[c].push_back(cls_sp[c].back() + ns * ns);
     cls_ap[c].push_back(cls_ap[c].back() + na * na);
   }
-  // Mulliken row chunks, partials laid out system by system
+  // Mulliken partials (one per 32-row chunk and column), system by system;
+  // k_mulliken_rows runs over the Fock tile list
   for (int s = 0; s < B; ++s) {
     const int n = T.ao_off[s + 1] - T.ao_off[s];
     const int nch = (n + kMullRows - 1) / kMullRows;
     part_off[s + 1] = part_off[s] + (int64_t)nch * n * 11;
-  }
-  for (int i = 0; i < B; ++i) {
-    const int s = T.order[i];
-    const int n = T.ao_off[s + 1] - T.ao_off[s];
-    for (int j = 0; j * kMullRows < n; ++j)
-      rowchunks.push_back({s, j * kMullRows, part_off[s] + (int64_t)j * n * 11});
   }
 
   // ---- arena layout ----
@@ -563,7 +557,6 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   bytes += plan_bytes<TileTask>(full.size()) + plan_bytes<TileTask>(lower.size());
   bytes += plan_bytes<BackTask>(back.size());
   bytes += plan_bytes<FockTile>(chunks.size());
-  bytes += plan_bytes<RowChunk>(rowchunks.size());
   for (int c = 0; c < 2; ++c) {
     addi(cls_sys[c].size());
     addl(cls_sp[c].size());
@@ -665,7 +658,6 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   TileTask* d_lower = cv.take<TileTask>(lower.size());
   BackTask* d_back = cv.take<BackTask>(back.size());
   FockTile* d_chunks = cv.take<FockTile>(chunks.size());
-  RowChunk* d_rowchunks = cv.take<RowChunk>(rowchunks.size());
   SysList sp_list[2], ap_list[2];
   for (int c = 0; c < 2; ++c) {
     int* ls = cv.take<int>(cls_sys[c].size());
@@ -719,7 +711,6 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   HIP_TRY(up(d_lower, lower));
   HIP_TRY(up(d_back, back));
   HIP_TRY(up(d_chunks, chunks));
-  HIP_TRY(up(d_rowchunks, rowchunks));
   HIP_TRY(up(d_part_off, part_off));
 
   t.atom_off = d_atom_off; t.shell_off = d_shell_off; t.ao_off = d_ao_off; t.vec_off = d_vec_off;
@@ -977,7 +968,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     set(rp.red1, a3, 0.0);             // X^T H, X upper triangular: n^3
     set(rp.red2, a3 / 3.0, 0.0);       // (X^T H) X, symmetric: n^3/3
     set(rp.tri, (4.0 / 3.0) * a3, 0.0);
-    set(rp.mull, 0.0, 88.0 * a2);      // reads P, S, D(3), Q(6), H0
+    set(rp.mull, 0.0, 48.0 * a2);      // reads half of P, S, D(3), Q(6), H0 (P symmetric)
     // back-transformation 2 n^2 kocc, C = X Y n^2 kocc (X upper triangular),
     // P (lower half) n^2 kocc
     set(rp.bt, 2.0 * w, 0.0);
@@ -1054,8 +1045,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(probe_end(kKDensity, 0.0, 0.0));
     rp.dens = last_probe();
     HIP_TRY(probe_begin(kKMulliken));
-    hipLaunchKernelGGL(k_mulliken_rows, dim3((unsigned)rowchunks.size()), dim3(256), 0, sm, t,
-                       d_rowchunks, d_active, dP, dS, dD, dQ, dH0, d_mpart);
+    hipLaunchKernelGGL(k_mulliken_rows, dim3((unsigned)chunks.size()), dim3(256), 0, sm, t,
+                       d_chunks, d_active, d_part_off, dP, dS, dD, dQ, dH0, d_mpart);
     hipLaunchKernelGGL(k_mulliken_finish, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_part_off,
                        d_mpart, d_colsum, mx.raw, d_st);
     HIP_TRY(probe_end(kKMulliken, 0.0, 0.0));

@@ -577,52 +577,110 @@ __global__ __launch_bounds__(256) void k_fock(Topo t, const FockTile* tiles,
 // ---------------------------------------------------------------------------
 // Mulliken raw multipoles and Tr(P H0) (reference mulliken_kernels_impl.hpp:40-115),
 // in two deterministic phases:
-//   k_mulliken_rows    one block per 32-row chunk of a system: per column c the
-//                      partial sums -P.S, -P.D(3), -P.Q(6) and P.H0 over the
-//                      chunk's rows, stored at part[off + 11 c + k]
+//   k_mulliken_rows    one block per Fock tile (32 x 32, on or above the
+//                      diagonal).  For every column c the partial sums -P.S,
+//                      -P.D(3), -P.Q(6) and P.H0 over the 32 rows of one tile
+//                      row go to part[off(chunk) + 11 c + k].  Tile (R, C)
+//                      gives the partials of its columns over rows R (moments
+//                      as stored) and, P being symmetric, those of its rows
+//                      over columns C from the transposed moments, so P, S,
+//                      D, Q and H0 are read once per pair of elements.
 //   k_mulliken_finish  one block per system: chunk partials added in chunk
 //                      order, then shell charges, atomic dipoles/quadrupoles
 //                      (by ket AO) and the core energy.
 // ---------------------------------------------------------------------------
-struct RowChunk {
-  int sys;
-  int r0;
-  int64_t off;  // first partial of this chunk in the partial buffer
-};
 constexpr int kMullRows = 32;
+static_assert(kMullRows == kFockTile, "Mulliken chunks are the rows of one Fock tile");
 
-__global__ __launch_bounds__(256) void k_mulliken_rows(Topo t, const RowChunk* chunks,
+// Sum over the 32 lanes of each half wavefront; lanes 31 and 63 hold the sums.
+__device__ __forceinline__ double half_wave_sum(double v) {
+  v += dpp_move<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x124, 0xf>(v);  // row_ror:4
+  v += dpp_move<0x128, 0xf>(v);  // row_ror:8
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  return v;
+}
+
+// Thread (grp, lane) = (tid / 32, tid % 32) holds rows r0 + grp + 8 i
+// (i = 0..3) of column c0 + lane.
+__global__ __launch_bounds__(256) void k_mulliken_rows(Topo t, const FockTile* tiles,
                                                        const unsigned char* active,
+                                                       const int64_t* __restrict__ part_off,
                                                        const double* __restrict__ P,
                                                        const double* __restrict__ S,
                                                        const double* __restrict__ D,
                                                        const double* __restrict__ Q,
                                                        const double* __restrict__ H0,
                                                        double* __restrict__ part) {
-  const RowChunk ch = chunks[blockIdx.x];
-  const int s = ch.sys;
+  const FockTile tl = tiles[blockIdx.x];
+  const int s = tl.sys;
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int o = t.ao_off[s];
   const int64_t mo = t.mat_off[s], nn = (int64_t)n * n;
   const double* Ds = D + 3 * mo;
   const double* Qs = Q + 6 * mo;
-  const int r1 = min(ch.r0 + kMullRows, n);
-  for (int c = threadIdx.x; c < n; c += blockDim.x) {
-    double acc[11];
+  double* ps = part + part_off[s];
+  __shared__ double red[8][11][kMullRows + 1];
+  const int lane = threadIdx.x & 31, grp = threadIdx.x >> 5;
+  const bool mirror = tl.r0 != tl.c0;  // uniform per block
+  const int c = tl.c0 + lane;
+  const int ac = c < n ? t.ao_atom[o + c] : 0;
+  double col[11];
 #pragma unroll
-    for (int k = 0; k < 11; ++k) acc[k] = 0.0;
-    for (int r = ch.r0; r < r1; ++r) {
+  for (int k = 0; k < 11; ++k) col[k] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = tl.r0 + grp + 8 * i;
+    const bool in = r < n && c < n;
+    double p = 0.0, sv = 0.0, h0 = 0.0, dm[3] = {0.0, 0.0, 0.0}, qm[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (in) {
       const int64_t loc = (int64_t)r * n + c;
-      const double p = P[mo + loc];
-      acc[0] = fma(-p, S[mo + loc], acc[0]);
+      p = P[mo + loc];
+      sv = S[mo + loc];
+      h0 = H0[mo + loc];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) acc[1 + k] = fma(-p, Ds[k * nn + loc], acc[1 + k]);
+      for (int k = 0; k < 3; ++k) dm[k] = Ds[k * nn + loc];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) acc[4 + k] = fma(-p, Qs[k * nn + loc], acc[4 + k]);
-      acc[10] = fma(H0[mo + loc], p, acc[10]);
+      for (int k = 0; k < 6; ++k) qm[k] = Qs[k * nn + loc];
     }
+    col[0] = fma(-p, sv, col[0]);
 #pragma unroll
-    for (int k = 0; k < 11; ++k) part[ch.off + 11 * (int64_t)c + k] = acc[k];
+    for (int k = 0; k < 3; ++k) col[1 + k] = fma(-p, dm[k], col[1 + k]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) col[4 + k] = fma(-p, qm[k], col[4 + k]);
+    col[10] = fma(h0, p, col[10]);
+    if (mirror) {
+      // row r's partial over the tile's 32 columns (the element (c, r) moments)
+      double dt[3] = {0.0, 0.0, 0.0}, qt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      if (in) transposed_moments(t, t.ao_atom[o + r], ac, sv, dm, qm, dt, qt);
+      double rowv[11];
+      rowv[0] = half_wave_sum(-p * sv);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) rowv[1 + k] = half_wave_sum(-p * dt[k]);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) rowv[4 + k] = half_wave_sum(-p * qt[k]);
+      rowv[10] = half_wave_sum(h0 * p);
+      if (lane == 31 && r < n) {
+        double* dst = ps + (int64_t)(tl.c0 / kMullRows) * n * 11 + 11 * (int64_t)r;
+#pragma unroll
+        for (int k = 0; k < 11; ++k) dst[k] = rowv[k];
+      }
+    }
+  }
+  // column partials over the tile's 32 rows: the eight row groups added in order
+#pragma unroll
+  for (int k = 0; k < 11; ++k) red[grp][k][lane] = col[k];
+  __syncthreads();
+  for (int q = threadIdx.x; q < 11 * kMullRows; q += blockDim.x) {
+    const int k = q / kMullRows, cl = q % kMullRows;
+    if (tl.c0 + cl < n) {
+      double v = 0.0;
+#pragma unroll
+      for (int g = 0; g < 8; ++g) v += red[g][k][cl];
+      ps[(int64_t)(tl.r0 / kMullRows) * n * 11 + 11 * (int64_t)(tl.c0 + cl) + k] = v;
+    }
   }
 }
 
