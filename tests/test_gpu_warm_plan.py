@@ -239,3 +239,21 @@ def test_calculator_warm_start_for_dynamics():
     finally:
         warm.close()
         cold.close()
+
+
+def test_context_topology_cache_follows_the_batch(ctx):
+    # one context keeps the host topology of its previous call: alternating
+    # topologies and moving geometry must give what a new context gives
+    a, b = small_batch(16), workloads.make("ragged512", seed=17, batch=6, kmax=9)
+    pos_a2 = moved(a, 0.02, 18)
+    print("case topology-cache", flush=True)
+    seq = [run(ctx, a), run(ctx, b), run(ctx, a), run(ctx, a, positions=pos_a2), run(ctx, b)]
+    other = interface.Context("hip", 0)
+    try:
+        ref_a, ref_b, ref_a2 = run(other, a), run(other, b), run(other, a, positions=pos_a2)
+    finally:
+        other.close()
+    for got, ref in zip(seq, (ref_a, ref_b, ref_a, ref_a2, ref_b)):
+        assert np.array_equal(got.energies, ref.energies)
+        assert np.array_equal(got.scc_iterations, ref.scc_iterations)
+        assert np.abs(got.forces - ref.forces).max() < 1e-11

@@ -64,8 +64,23 @@ struct xtbloom_context {
   // created on first use
   std::vector<xtb::Device*> lanes;
   WarmCheckpoint warm;  // checkpoint of xtbloom_compute calls
+  // host topology of the previous xtbloom_compute call, reused when the next
+  // call has the same topology (only its geometry is then refreshed)
+  std::vector<xtb::TopoCache*> topo;
+  CallIdentity topo_id;
+  ~xtbloom_context() {
+    for (xtb::TopoCache* c : topo) xtb::topo_cache_destroy(c);
+  }
   std::mutex lock;  // one context is one serialized transaction domain (reference backend.hpp:28-32)
 };
+
+// The inputs the packed host topology depends on: atom offsets, element
+// numbers, molecular charges, unpaired electrons, point-charge offsets and
+// gammas.
+bool same_topology(const CallIdentity& a, const CallIdentity& b) {
+  return a.atom_offsets == b.atom_offsets && a.Z == b.Z && a.charges == b.charges &&
+         a.unpaired == b.unpaired && a.point_offsets == b.point_offsets && a.pc_gamma == b.pc_gamma;
+}
 
 // Fixed-topology plan (reference xtbloom.h:939-991): the identity it was
 // created for, its own WARM checkpoint, the host topology of each lane slice
@@ -434,8 +449,20 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, 
   xtb::RunTiming timing;
   int rc = 0;
   try {
-    rc = xtb::run_batch_lanes(lanes_for(context, B), hb, ro, out, &timing, err,
-                              plan ? &plan->caches : nullptr);
+    const std::vector<xtb::TopoCache*>* caches = nullptr;
+    if (plan) {
+      caches = &plan->caches;
+    } else {
+      // the context's own topology cache: emptied when the topology changes
+      if (context->topo.empty())
+        for (int k = 0; k < 4; ++k) context->topo.push_back(xtb::topo_cache_create());
+      if (!same_topology(context->topo_id, id)) {
+        for (xtb::TopoCache* c : context->topo) xtb::topo_cache_retag(c, -1);
+        context->topo_id = id;
+      }
+      caches = &context->topo;
+    }
+    rc = xtb::run_batch_lanes(lanes_for(context, B), hb, ro, out, &timing, err, caches);
   } catch (const std::bad_alloc&) {
     return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "host allocation failed");
   } catch (...) {

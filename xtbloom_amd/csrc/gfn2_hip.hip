@@ -474,7 +474,10 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     T.pc_xyz.assign(hb.pc_xyz, hb.pc_xyz + 3 * (size_t)T.NPC);
     T.pc_q.assign(hb.pc_q, hb.pc_q + T.NPC);
   } else {
-    if (cache) cache->valid = false;
+    if (cache) {
+      cache->valid = false;
+      T = HostTopo();  // build_topology appends to empty vectors
+    }
     const int st = build_topology(hb, T, error);
     if (st != 0) return st;
     if (cache) cache->valid = true;
