@@ -56,8 +56,8 @@ __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, con
   const double* sc = scale ? scale + t.ao_off[s] : nullptr;
   const int K = kdim ? kdim[s] : n;
   const int i0 = task.ti * kTile, j0 = task.tj * kTile;
-  __shared__ double As[kTileK][kTile + 1];
-  __shared__ double Bs[kTileK][kTile + 1];
+  __shared__ double As[2][kTileK][kTile + 1];  // two stages: one written while the other is read
+  __shared__ double Bs[2][kTileK][kTile + 1];
   // wave w owns the 32 x 32 quadrant (w / 2, w % 2): 2 x 2 blocks of 16 x 16
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int qi = 32 * (w >> 1), qj = 32 * (w & 1);
@@ -98,29 +98,38 @@ __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, con
       pb[u] = (gj < n && gk < K) ? (TB ? b[(int64_t)gj * n + gk] : b[(int64_t)gk * n + gj]) : 0.0;
     }
   };
-  if (kbeg < kend) fetch(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += kTileK) {
-    // A tile rows i0..i0+63 -> As[k][i], B tile -> Bs[k][j]
+  // A tile rows i0..i0+63 -> As[buf][k][i], B tile -> Bs[buf][k][j]
+  auto stash = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int idx = threadIdx.x + 256 * u;
-      if (TA) As[idx / kTile][idx % kTile] = pa[u]; else As[idx % kTileK][idx / kTileK] = pa[u];
-      if (TB) Bs[idx % kTileK][idx / kTileK] = pb[u]; else Bs[idx / kTile][idx % kTile] = pb[u];
+      if (TA) As[buf][idx / kTile][idx % kTile] = pa[u]; else As[buf][idx % kTileK][idx / kTileK] = pa[u];
+      if (TB) Bs[buf][idx % kTileK][idx / kTileK] = pb[u]; else Bs[buf][idx / kTile][idx % kTile] = pb[u];
     }
-    __syncthreads();
-    if (k0 + kTileK < kend) fetch(k0 + kTileK);
+  };
+  if (kbeg < kend) {
+    fetch(kbeg);
+    stash(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kbeg; k0 < kend; k0 += kTileK, buf ^= 1) {
+    const bool more = k0 + kTileK < kend;
+    if (more) fetch(k0 + kTileK);
 #pragma unroll
     for (int k4 = 0; k4 < kTileK; k4 += 4) {
       double af[2], bf[2];
 #pragma unroll
-      for (int p = 0; p < 2; ++p) af[p] = As[k4 + lk][qi + 16 * p + l16];
+      for (int p = 0; p < 2; ++p) af[p] = As[buf][k4 + lk][qi + 16 * p + l16];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) bf[q] = Bs[k4 + lk][qj + 16 * q + l16];
+      for (int q = 0; q < 2; ++q) bf[q] = Bs[buf][k4 + lk][qj + 16 * q + l16];
 #pragma unroll
       for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int q = 0; q < 2; ++q) acc[p][q] = mfma_16x16x4(af[p], bf[q], acc[p][q]);
     }
+    // the other stage was last read before the previous barrier
+    if (more) stash(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
