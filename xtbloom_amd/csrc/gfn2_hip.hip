@@ -1134,12 +1134,12 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     }
   }
   HIP_TRY(probe_begin(kKForceAtoms));
-  hipLaunchKernelGGL(k_force_atoms, dim3(B), dim3(256), 0, sm, t, d_order, d_valid, mx.x, d_qat,
+  hipLaunchKernelGGL(k_force_atoms, dim3(B), dim3(512), 0, sm, t, d_order, d_valid, mx.x, d_qat,
                      d_cnd4, d_mrad, d_dmrad, d_gam, d_w4, d_dw4, d_dedcn, d_dedcn4, d_grad,
                      d_erep, d_pcgrad, d_dedq);
   HIP_TRY(probe_end(kKForceAtoms, 0.0, 0.0));
   HIP_TRY(probe_begin(kKForceAtm));
-  hipLaunchKernelGGL(k_force_atm, dim3(B), dim3(256), 0, sm, t, d_order, d_valid, d_cnd4, d_w4,
+  hipLaunchKernelGGL(k_force_atm, dim3(B), dim3(512), 0, sm, t, d_order, d_valid, d_cnd4, d_w4,
                      d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm);
   HIP_TRY(probe_end(kKForceAtm, 0.0, 0.0));
   HIP_TRY(hipGetLastError());

@@ -401,7 +401,7 @@ __device__ void aes2_pair_vjp(const Topo& t, int i, int j, const double* mrad, c
 
 // Per-system atom-level gradients: AES2 VJP, GFN2-CN chain (H0 + AES2 adjoints),
 // ES2, D4 two-body (+ D4-CN chain), repulsion energy/gradient and point charges.
-__global__ __launch_bounds__(256) void k_force_atoms(Topo t, const int* order,
+__global__ __launch_bounds__(512) void k_force_atoms(Topo t, const int* order,
                                                      const unsigned char* valid,
                                                      const double* xstate, const double* qat,
                                                      const double* cnd4, const double* mrad,
@@ -591,7 +591,7 @@ __global__ __launch_bounds__(256) void k_force_atoms(Topo t, const int* order,
 // finishes the kernel.
 constexpr int kAtmFields = 5;
 
-__global__ __launch_bounds__(256) void k_force_atm(Topo t, const int* order, const unsigned char* valid,
+__global__ __launch_bounds__(512) void k_force_atm(Topo t, const int* order, const unsigned char* valid,
                                                    const double* cnd4, double* w4, double* dw4c,
                                                    double* pairbuf, double* dedcn4, double* grad,
                                                    double* eatm) {
