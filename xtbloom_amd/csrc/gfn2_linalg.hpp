@@ -673,10 +673,17 @@ __device__ __forceinline__ void cholesky_registers(const double* Sm, double* L, 
     const double sq = sqrt(piv);
     if (tid >= k && tid < n) L[tid + (int64_t)k * n] = tid == k ? sq : lcol[tid] / sq;
     const double ip = 1.0 / piv;
-    static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
-      constexpr int sl = decltype(S)::value;
-      constexpr int TI = tri_row(sl), TJ = tri_col(sl);
-      a[sl] -= lcol[16 * TI + lr] * lcol[32 * TJ + lc] * ip;
+    // column tiles entirely left of k are final (lcol is zero there): skipped
+    static_for<0, NR / 2>([&](auto CJ) __attribute__((always_inline)) {
+      constexpr int TJ = decltype(CJ)::value;
+      if (32 * TJ + 31 >= k) {
+        const double lj = lcol[32 * TJ + lc] * ip;
+        static_for<2 * TJ, NR>([&](auto CI) __attribute__((always_inline)) {
+          constexpr int TI = decltype(CI)::value;
+          constexpr int sl = tri_slots(TI) + TJ;
+          a[sl] -= lcol[16 * TI + lr] * lj;
+        });
+      }
     });
     __syncthreads();
   }
@@ -707,10 +714,17 @@ __device__ __forceinline__ void cholesky_registers(const double* Sm, double* L, 
       if (own) xr[j] = nv;
     });
     __syncthreads();
-    static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
-      constexpr int sl = decltype(S)::value;
-      constexpr int TI = tri_row(sl), TJ = tri_col(sl);
-      a[sl] -= lcol[16 * TI + lr] * xr[32 * TJ + lc];
+    // xr is zero right of column k: column tiles entirely right of k are skipped
+    static_for<0, NR / 2>([&](auto CJ) __attribute__((always_inline)) {
+      constexpr int TJ = decltype(CJ)::value;
+      if (32 * TJ <= k) {
+        const double xj = xr[32 * TJ + lc];
+        static_for<2 * TJ, NR>([&](auto CI) __attribute__((always_inline)) {
+          constexpr int TI = decltype(CI)::value;
+          constexpr int sl = tri_slots(TI) + TJ;
+          a[sl] -= lcol[16 * TI + lr] * xj;
+        });
+      }
     });
     __syncthreads();
   }
