@@ -407,8 +407,10 @@ __device__ __forceinline__ int pair_lanes(int N, int nt) {
 // (reference scc_driver.cpp:2363-2560, aes2.cpp:451-545, d4.cpp:896-1032).
 // STAGED = the instantiation for systems of at most kPairStageAtoms atoms; the
 // other one serves the rest from the global staging slice.
+// Bounded to 128 VGPRs (four waves per SIMD): two blocks share a CU, so the
+// whole batch runs in one wave of blocks (a few spilled values cost less).
 template <bool STAGED>
-__global__ __launch_bounds__(512) void k_potentials(Topo t, const int* order,
+__global__ __launch_bounds__(512, 4) void k_potentials(Topo t, const int* order,
                                                     const unsigned char* active, const double* xs,
                                                     const double* cnd4, const double* mrad,
                                                     const double* gam, const double* vpc,
