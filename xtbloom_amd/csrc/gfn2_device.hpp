@@ -244,14 +244,6 @@ __device__ __forceinline__ double dpp_move(double x) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// A wave-uniform double moved to scalar registers (readfirstlane on both halves).
-__device__ __forceinline__ double wave_uniform(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readfirstlane((int)b);
-  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
 __device__ __forceinline__ double wave_sum(double v) {
   v += dpp_move<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
   v += dpp_move<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
