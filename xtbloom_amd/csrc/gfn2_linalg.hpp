@@ -356,6 +356,9 @@ __device__ __forceinline__ void static_for(F&& f) {
 }
 
 constexpr int kTriPad = 257;  // row stride of the partial-sum buffers (bank spread)
+constexpr int kTriTail = 64;    // trailing columns reduced by one wavefront
+constexpr int kTriTailLd = 65;  // row stride of that trailing block in LDS
+static_assert(kTriTail * kTriTailLd <= 32 * kTriPad, "the trailing block reuses the row partials");
 
 template <int NR>
 __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, double* e, double* v,
@@ -391,7 +394,8 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
     });
   }
   __syncthreads();
-  for (int k = 0; k + 2 < n; ++k) {
+  int k = 0;
+  for (; k + 2 < n && n - k > kTriTail; ++k) {
     // Reflector of column k, formed redundantly (and identically) by every
     // wavefront: v = (x - alpha e_{k+1}) / |x - alpha e_{k+1}|.
     double part = 0.0;
@@ -507,15 +511,86 @@ __device__ __forceinline__ void tridiag_registers(double* A, int n, double* d, d
     }
     __syncthreads();
   }
+  if (k + 2 >= n) {  // n < 3: nothing was reduced
+    static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
+      constexpr int sl = decltype(S)::value;
+      constexpr int TI = tri_row(sl), TJ = tri_col(sl);
+      const int i = 16 * TI + lr, j = 32 * TJ + lc;
+      if (i == n - 2 && j == n - 2) d[n - 2] = 2.0 * a[sl];
+      if (i == n - 1 && j == n - 2) e[n - 2] = a[sl];
+      if (i == n - 1 && j == n - 1) d[n - 1] = 2.0 * a[sl];
+    });
+    if (tid == 0) e[n - 1] = 0.0;
+    return;
+  }
+  // The last n - k0 <= kTriTail columns: one wavefront finishes them on the
+  // trailing block, stored whole (symmetric, full diagonal) in LDS, without
+  // block barriers.  Lane l owns row k + 1 + l of the step-k update.
+  const int k0 = k;
+  double* tb = rowpart;  // (n - k0) x kTriTailLd
   static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
     constexpr int sl = decltype(S)::value;
     constexpr int TI = tri_row(sl), TJ = tri_col(sl);
     const int i = 16 * TI + lr, j = 32 * TJ + lc;
-    if (i == n - 2 && j == n - 2) d[n - 2] = 2.0 * a[sl];
-    if (i == n - 1 && j == n - 2) e[n - 2] = a[sl];
-    if (i == n - 1 && j == n - 1) d[n - 1] = 2.0 * a[sl];
+    if (i >= k0 && j >= k0 && i < n && j < n && i >= j) {
+      const double x = a[sl];
+      if (i > j) {
+        tb[(i - k0) * kTriTailLd + (j - k0)] = x;
+        tb[(j - k0) * kTriTailLd + (i - k0)] = x;
+      } else {
+        tb[(i - k0) * kTriTailLd + (i - k0)] = 2.0 * x;
+      }
+    }
   });
-  if (tid == 0) e[n - 1] = 0.0;
+  __syncthreads();
+  if (tid < 64) {
+    const int l = tid;
+    auto lds_fence = [] __attribute__((always_inline)) { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); };
+    for (; k + 2 < n; ++k) {
+      const int m1 = n - k - 1;  // rows below the diagonal
+      const int rl = k + 1 + l - k0, ck = k - k0;
+      const double x = l < m1 ? tb[rl * kTriTailLd + ck] : 0.0;
+      const double ss = wave_sum(x * x);
+      const double x0 = tb[(ck + 1) * kTriTailLd + ck];
+      const double xn = sqrt(ss);
+      const bool noref = xn == 0.0 || ss - x0 * x0 == 0.0;  // already reduced
+      const double alpha = x0 >= 0.0 ? -xn : xn;
+      const double v0 = x0 - alpha;
+      const double inv = noref ? 0.0 : 1.0 / sqrt(ss - x0 * x0 + v0 * v0);
+      if (l == 0) {
+        d[k] = tb[ck * kTriTailLd + ck];
+        e[k] = noref ? x0 : alpha;
+      }
+      const double vl = l < m1 ? inv * (l == 0 ? v0 : x) : 0.0;
+      if (l < m1) A[(k + 1 + l) + (int64_t)k * n] = vl;
+      v[l] = vl;
+      lds_fence();
+      // p = A v over the trailing rows (v broadcast from LDS)
+      double p = 0.0;
+      if (l < m1) {
+        const double* row = tb + rl * kTriTailLd + ck + 1;
+        for (int j = 0; j < m1; ++j) p = fma(row[j], v[j], p);
+      }
+      // K = v.p; with q = p - K v,  A -= 2 (v q^T + q v^T) = 2 (v p^T + (p - 2 K v) v^T)
+      const double K2 = 2.0 * wave_sum(vl * p);
+      const double rq = fma(-K2, vl, p);
+      pq[l] = p;
+      lds_fence();
+      if (l < m1) {
+        double* row = tb + rl * kTriTailLd + ck + 1;
+        for (int j = 0; j < m1; ++j) row[j] -= 2.0 * (vl * pq[j] + rq * v[j]);
+      }
+      lds_fence();
+    }
+    if (l == 0) {
+      const int c2 = n - 2 - k0;
+      d[n - 2] = tb[c2 * kTriTailLd + c2];
+      e[n - 2] = tb[(c2 + 1) * kTriTailLd + c2];
+      d[n - 1] = tb[(c2 + 1) * kTriTailLd + c2 + 1];
+      e[n - 1] = 0.0;
+    }
+  }
+  __syncthreads();
 }
 
 // The back-transformation applies the reflectors in compact WY blocks of
