@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -287,8 +288,10 @@ int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
 struct TopoCache {
   HostTopo topo;
   bool valid = false;
-  int64_t tag = -1;  // which slice of which lane split the topology belongs to
+  int64_t tag = -1;         // which slice of which lane split the topology belongs to
+  uint64_t generation = 0;  // process-unique id of the current contents
 };
+std::atomic<uint64_t> g_topo_generation{0};
 
 TopoCache* topo_cache_create() { return new TopoCache(); }
 void topo_cache_destroy(TopoCache* cache) { delete cache; }
@@ -321,6 +324,10 @@ struct Device {
   // call's host-to-device uploads (pinned, so each copy is a plain async DMA)
   char* pinned = nullptr;
   size_t pinned_bytes = 0;
+  // the arena holds the uploaded topology of cache generation resident_gen
+  // (0: none), laid out for a call of resident_bytes arena bytes
+  uint64_t resident_gen = 0;
+  size_t resident_bytes = 0;
 };
 
 Device* device_create(int device_id, void* stream, std::string& error) {
@@ -468,7 +475,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   }
   HostTopo local;
   HostTopo& T = cache ? cache->topo : local;
-  if (cache && cache->valid) {
+  const bool cache_hit = cache && cache->valid;
+  if (cache_hit) {
     // same topology: only the geometry changes
     T.xyz.assign(hb.xyz, hb.xyz + 3 * (size_t)hb.atoms);
     T.pc_xyz.assign(hb.pc_xyz, hb.pc_xyz + 3 * (size_t)T.NPC);
@@ -480,7 +488,10 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     }
     const int st = build_topology(hb, T, error);
     if (st != 0) return st;
-    if (cache) cache->valid = true;
+    if (cache) {
+      cache->valid = true;
+      cache->generation = ++g_topo_generation;
+    }
   }
   const int B = T.B, N = T.N, NS = T.NS, NAO = T.NAO, NPC = T.NPC;
   const int64_t NN = T.mat_off[B];
@@ -600,6 +611,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     (void)hipFree(dev->arena);
     dev->arena = nullptr;
     dev->arena_bytes = 0;
+    dev->resident_gen = 0;
     const size_t want = bytes + bytes / 4;
     if (hipMalloc(&dev->arena, want) != hipSuccess) {
       error = "device arena allocation failed";
@@ -622,6 +634,12 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   auto up = [&](auto* dst, const auto& vec) -> hipError_t {
     if (!vec.empty()) uploads.push_back({dst, vec.data(), vec.size() * sizeof(vec[0])});
     return hipSuccess;
+  };
+  // Topology arrays stay in the arena between calls: a call with the same
+  // cached topology and the same arena layout uploads only the geometry.
+  const bool resident = cache_hit && cache->generation == dev->resident_gen && bytes == dev->resident_bytes;
+  auto up_topo = [&](auto* dst, const auto& vec) -> hipError_t {
+    return resident ? hipSuccess : up(dst, vec);
   };
   Topo t{};
   t.B = B;
@@ -666,55 +684,55 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     int* ls = cv.take<int>(cls_sys[c].size());
     int64_t* lsp = cv.take<int64_t>(cls_sp[c].size());
     int64_t* lap = cv.take<int64_t>(cls_ap[c].size());
-    HIP_TRY(up(ls, cls_sys[c]));
-    HIP_TRY(up(lsp, cls_sp[c]));
-    HIP_TRY(up(lap, cls_ap[c]));
+    HIP_TRY(up_topo(ls, cls_sys[c]));
+    HIP_TRY(up_topo(lsp, cls_sp[c]));
+    HIP_TRY(up_topo(lap, cls_ap[c]));
     sp_list[c] = SysList{ls, lsp, (int)cls_sys[c].size()};
     ap_list[c] = SysList{ls, lap, (int)cls_sys[c].size()};
   }
   int64_t* d_part_off = cv.take<int64_t>(B + 1);
   double* d_mpart = cv.take<double>((size_t)part_off[B]);
-  HIP_TRY(up(d_atom_off, T.atom_off));
-  HIP_TRY(up(d_shell_off, T.shell_off));
-  HIP_TRY(up(d_ao_off, T.ao_off));
-  HIP_TRY(up(d_vec_off, T.vec_off));
-  HIP_TRY(up(d_pc_off, T.pc_off));
-  HIP_TRY(up(d_order, T.order));
-  HIP_TRY(up(d_mat_off, T.mat_off));
-  HIP_TRY(up(d_gam_off, T.gam_off));
-  HIP_TRY(up(d_apair_off, T.apair_off));
-  HIP_TRY(up(d_sp_off, T.sp_off));
-  HIP_TRY(up(d_nalpha, T.nalpha));
-  HIP_TRY(up(d_nbeta, T.nbeta));
-  HIP_TRY(up(d_charges, T.charges));
-  HIP_TRY(up(d_Z, T.Z));
-  HIP_TRY(up(d_atom_sys, T.atom_sys));
-  HIP_TRY(up(d_atom_sh0, T.atom_sh0));
-  HIP_TRY(up(d_d4ref, T.at_d4ref));
-  HIP_TRY(up(d_d4nref, T.at_d4nref));
+  HIP_TRY(up_topo(d_atom_off, T.atom_off));
+  HIP_TRY(up_topo(d_shell_off, T.shell_off));
+  HIP_TRY(up_topo(d_ao_off, T.ao_off));
+  HIP_TRY(up_topo(d_vec_off, T.vec_off));
+  HIP_TRY(up_topo(d_pc_off, T.pc_off));
+  HIP_TRY(up_topo(d_order, T.order));
+  HIP_TRY(up_topo(d_mat_off, T.mat_off));
+  HIP_TRY(up_topo(d_gam_off, T.gam_off));
+  HIP_TRY(up_topo(d_apair_off, T.apair_off));
+  HIP_TRY(up_topo(d_sp_off, T.sp_off));
+  HIP_TRY(up_topo(d_nalpha, T.nalpha));
+  HIP_TRY(up_topo(d_nbeta, T.nbeta));
+  HIP_TRY(up_topo(d_charges, T.charges));
+  HIP_TRY(up_topo(d_Z, T.Z));
+  HIP_TRY(up_topo(d_atom_sys, T.atom_sys));
+  HIP_TRY(up_topo(d_atom_sh0, T.atom_sh0));
+  HIP_TRY(up_topo(d_d4ref, T.at_d4ref));
+  HIP_TRY(up_topo(d_d4nref, T.at_d4nref));
   HIP_TRY(up(d_xyz, T.xyz));
   const std::vector<double>* atv[15] = {&T.at_cnrad, &T.at_h0rad, &T.at_en, &T.at_dk, &T.at_qk,
                                         &T.at_mprad, &T.at_mpvcn, &T.at_sqrt_arep, &T.at_zeff,
                                         &T.at_d4rcov, &T.at_d4en, &T.at_d4zeff, &T.at_d4hard,
                                         &T.at_d4r4r2, &T.at_cnrad};
-  for (int k = 0; k < 15; ++k) HIP_TRY(up(at[k], *atv[k]));
+  for (int k = 0; k < 15; ++k) HIP_TRY(up_topo(at[k], *atv[k]));
   const std::vector<int>* shiv[5] = {&T.sh_atom, &T.sh_l, &T.sh_ao, &T.sh_prim, &T.sh_nprim};
-  for (int k = 0; k < 5; ++k) HIP_TRY(up(shi[k], *shiv[k]));
+  for (int k = 0; k < 5; ++k) HIP_TRY(up_topo(shi[k], *shiv[k]));
   const std::vector<double>* shdv[7] = {&T.sh_level, &T.sh_kcn, &T.sh_kpoly, &T.sh_zeta,
                                         &T.sh_refocc, &T.sh_eta, &T.sh_gam3};
-  for (int k = 0; k < 7; ++k) HIP_TRY(up(shd[k], *shdv[k]));
-  HIP_TRY(up(d_ao_sh, T.ao_sh));
-  HIP_TRY(up(d_ao_atom, T.ao_atom));
-  HIP_TRY(up(d_alpha, T.alpha));
-  HIP_TRY(up(d_coef, T.coef));
+  for (int k = 0; k < 7; ++k) HIP_TRY(up_topo(shd[k], *shdv[k]));
+  HIP_TRY(up_topo(d_ao_sh, T.ao_sh));
+  HIP_TRY(up_topo(d_ao_atom, T.ao_atom));
+  HIP_TRY(up_topo(d_alpha, T.alpha));
+  HIP_TRY(up_topo(d_coef, T.coef));
   HIP_TRY(up(d_pcxyz, T.pc_xyz));
   HIP_TRY(up(d_pcq, T.pc_q));
-  HIP_TRY(up(d_pcg, T.pc_gam));
-  HIP_TRY(up(d_full, full));
-  HIP_TRY(up(d_lower, lower));
-  HIP_TRY(up(d_back, back));
-  HIP_TRY(up(d_chunks, chunks));
-  HIP_TRY(up(d_part_off, part_off));
+  HIP_TRY(up_topo(d_pcg, T.pc_gam));
+  HIP_TRY(up_topo(d_full, full));
+  HIP_TRY(up_topo(d_lower, lower));
+  HIP_TRY(up_topo(d_back, back));
+  HIP_TRY(up_topo(d_chunks, chunks));
+  HIP_TRY(up_topo(d_part_off, part_off));
 
   t.atom_off = d_atom_off; t.shell_off = d_shell_off; t.ao_off = d_ao_off; t.vec_off = d_vec_off;
   t.pc_off = d_pc_off; t.mat_off = d_mat_off; t.gam_off = d_gam_off; t.apair_off = d_apair_off;
@@ -806,7 +824,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   double* d_gstage = cv.take<double>((size_t)kPairStageDoubles * N);
   int* d_gstagei = cv.take<int>((size_t)kPairStageInts * N);
   mx.ovl = cv.take<double>((size_t)mem * mem * B);
-  HIP_TRY(up(d_perm, T.perm));
+  HIP_TRY(up_topo(d_perm, T.perm));
   t.perm = d_perm;
   const bool big_systems = T.max_atoms > kPairStageAtoms;
   auto launch_potentials = [&](const unsigned char* mask) {
@@ -894,6 +912,10 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
       off += align_bytes<char>(u.bytes);
     }
   }
+  // the arena now holds this call's topology (stream order: every later call
+  // on this device reads it after these copies)
+  dev->resident_gen = cache ? cache->generation : 0;
+  dev->resident_bytes = bytes;
   // pinned per-parity round results: active counts, and (with kernel stats)
   // snapshots of the active mask and kocc at the end of each round
   int* h_count = reinterpret_cast<int*>(dev->pinned);
