@@ -257,3 +257,52 @@ def test_context_topology_cache_follows_the_batch(ctx):
         assert np.array_equal(got.energies, ref.energies)
         assert np.array_equal(got.scc_iterations, ref.scc_iterations)
         assert np.abs(got.forces - ref.forces).max() < 1e-11
+
+
+def test_enqueue_settles_inline_and_matches_compute(ctx):
+    # xtbloom_compute_enqueue / xtbloom_plan_compute_enqueue (reference
+    # xtbloom.h:836-1003): this backend completes accepted submissions inline
+    b = small_batch(20)
+    print("case enqueue", flush=True)
+    req = interface.Request(ctx)
+    try:
+        assert req.query() == dict(state=library.REQUEST_IDLE, completion_status=0, result_flags=0)
+        assert req.error() == ""
+        ref = run(ctx, b)
+        got = run(ctx, b, request=req)
+        st = req.wait()
+        assert st["state"] == library.REQUEST_COMPLETE and st["completion_status"] == 0
+        assert req.error() == ""
+        assert np.array_equal(got.energies, ref.energies)
+        assert np.array_equal(got.scc_iterations, ref.scc_iterations)
+        assert np.abs(got.forces - ref.forces).max() < 1e-11
+        # an admission error raises directly and leaves the request as it was
+        bad = b.positions.copy()
+        bad[0, 0] = np.nan
+        with pytest.raises(interface.XTBloomError) as err:
+            run(ctx, b, positions=bad, request=req)
+        assert err.value.status == library.STATUS_INVALID_ARGUMENT
+        assert req.query()["state"] == library.REQUEST_COMPLETE
+        # strict WARM through the request, then through a plan's enqueue
+        warm = run(ctx, b, positions=moved(b, 0.01, 21), scc_start_mode=WARM, request=req)
+        assert np.all(warm.per_system_status == 0)
+        plan = interface.Plan(ctx, b.atom_offsets, b.atomic_numbers, b.positions,
+                              b.molecular_charges, b.unpaired_electrons)
+        try:
+            pg = interface.compute_arrays(ctx, b.atom_offsets, b.atomic_numbers, b.positions,
+                                          b.molecular_charges, b.unpaired_electrons, plan=plan,
+                                          request=req)
+            assert np.array_equal(pg.energies, ref.energies)
+        finally:
+            plan.close()
+        # a request of another context is rejected at admission
+        other = interface.Context("hip", 0)
+        try:
+            foreign = interface.Request(other)
+            with pytest.raises(interface.XTBloomError):
+                run(ctx, b, request=foreign)
+            foreign.close()
+        finally:
+            other.close()
+    finally:
+        req.close()

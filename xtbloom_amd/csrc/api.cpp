@@ -77,6 +77,19 @@ struct xtbloom_context {
 // The inputs the packed host topology depends on: atom offsets, element
 // numbers, molecular charges, unpaired electrons, point-charge offsets and
 // gammas.
+// Asynchronous submission owner (reference xtbloom.h:41-47, 836-878).  This
+// backend settles every accepted submission inline: enqueue runs the call and
+// leaves the request COMPLETE, which the reference ABI allows ("or COMPLETE
+// when the backend must settle it inline"); admission errors return directly
+// and leave the request unchanged.
+struct xtbloom_request {
+  xtbloom_context_t* context = nullptr;
+  int32_t state = XTBLOOM_REQUEST_IDLE;
+  int32_t completion_status = XTBLOOM_STATUS_SUCCESS;
+  uint32_t result_flags = 0;
+  std::string error;
+};
+
 bool same_topology(const CallIdentity& a, const CallIdentity& b) {
   return a.atom_offsets == b.atom_offsets && a.Z == b.Z && a.charges == b.charges &&
          a.unpaired == b.unpaired && a.point_offsets == b.point_offsets && a.pc_gamma == b.pc_gamma;
@@ -377,10 +390,13 @@ std::vector<xtb::Device*> lanes_for(xtbloom_context_t* context, int64_t B) {
   return lanes;
 }
 
-// xtbloom_compute (plan == NULL) and xtbloom_plan_compute.
+// xtbloom_compute (plan == NULL) and xtbloom_plan_compute.  *admitted (if
+// given) tells whether the call got past admission (validation, plan and WARM
+// checks) before it returned.
 xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, const xtbloom_batch_t* batch,
                               const xtbloom_compute_options_t* options, xtbloom_batch_result_t* result,
-                              xtbloom_amd_extension_t* ext) {
+                              xtbloom_amd_extension_t* ext, bool* admitted = nullptr) {
+  if (admitted) *admitted = false;
   if (context == nullptr || context->device == nullptr)
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "context is NULL");
   std::lock_guard<std::mutex> guard(context->lock);
@@ -410,6 +426,7 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, 
     ro.warm_state = warm_state.data();
   }
   // an accepted call consumes the checkpoint whatever its start mode
+  if (admitted) *admitted = true;
   cp.valid = false;
   cp.state.clear();
   const int64_t B = hb.batch, N = hb.atoms, P = hb.points;
@@ -643,23 +660,85 @@ static xtbloom_status_t not_supported(const char* what) {
   return fail(XTBLOOM_STATUS_NOT_SUPPORTED, std::string(what) + " is not supported by the ROCm backend");
 }
 
-xtbloom_status_t xtbloom_request_create(xtbloom_context_t*, xtbloom_request_t** request) {
-  if (request) *request = nullptr;
-  return not_supported("asynchronous request");
+// ---- requests: submissions settled inline (reference xtbloom.h:836-937) ----
+xtbloom_status_t xtbloom_request_create(xtbloom_context_t* context, xtbloom_request_t** request) {
+  if (request == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request output is NULL");
+  *request = nullptr;
+  if (context == nullptr || context->device == nullptr)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "context is NULL");
+  try {
+    xtbloom_request_t* r = new xtbloom_request_t();
+    r->context = context;
+    *request = r;
+  } catch (const std::bad_alloc&) {
+    return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "request allocation failed");
+  }
+  g_last_error.clear();
+  return XTBLOOM_STATUS_SUCCESS;
 }
-xtbloom_status_t xtbloom_request_query(xtbloom_request_t*, xtbloom_request_info_t*) {
-  return not_supported("asynchronous request");
+
+static xtbloom_status_t request_info(xtbloom_request_t* request, xtbloom_request_info_t* info) {
+  if (request == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request is NULL");
+  if (!valid_header(info, XTBLOOM_REQUEST_INFO_V1_SIZE) || info->reserved != 0)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request info is NULL, too small, or has a different api_version");
+  info->state = request->state;
+  info->completion_status = request->state == XTBLOOM_REQUEST_COMPLETE ? request->completion_status
+                                                                      : XTBLOOM_STATUS_SUCCESS;
+  info->result_flags = request->state == XTBLOOM_REQUEST_COMPLETE ? request->result_flags : 0u;
+  g_last_error.clear();
+  return XTBLOOM_STATUS_SUCCESS;
 }
-xtbloom_status_t xtbloom_request_wait(xtbloom_request_t*, xtbloom_request_info_t*) {
-  return not_supported("asynchronous request");
+
+// every accepted submission is already COMPLETE: query and wait coincide
+xtbloom_status_t xtbloom_request_query(xtbloom_request_t* request, xtbloom_request_info_t* info) {
+  return request_info(request, info);
 }
-const char* xtbloom_request_get_error(const xtbloom_request_t*) { return ""; }
-void xtbloom_request_destroy(xtbloom_request_t*) {}
-xtbloom_status_t xtbloom_compute_enqueue(xtbloom_context_t*, const xtbloom_batch_t*,
-                                         const xtbloom_compute_options_t*,
-                                         const xtbloom_batch_result_t*, xtbloom_request_t*) {
-  return not_supported("xtbloom_compute_enqueue");
+xtbloom_status_t xtbloom_request_wait(xtbloom_request_t* request, xtbloom_request_info_t* info) {
+  return request_info(request, info);
 }
+
+const char* xtbloom_request_get_error(const xtbloom_request_t* request) {
+  if (request == nullptr) {
+    fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request is NULL");
+    return nullptr;
+  }
+  return request->state == XTBLOOM_REQUEST_COMPLETE ? request->error.c_str() : "";
+}
+
+void xtbloom_request_destroy(xtbloom_request_t* request) { delete request; }
+
+// Runs one submission inline: admission errors return directly and leave the
+// request and every output untouched; an admitted call completes the request
+// with its status.  The caller's result descriptor is copied and never
+// modified: completed result flags go to the request.
+static xtbloom_status_t enqueue_impl(xtbloom_context_t* context, xtbloom_plan_t* plan,
+                                     const xtbloom_batch_t* batch, const xtbloom_compute_options_t* options,
+                                     const xtbloom_batch_result_t* result, xtbloom_request_t* request) {
+  if (request == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request is NULL");
+  if (request->context != context)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request belongs to another context");
+  if (!valid_header(result, XTBLOOM_BATCH_RESULT_V1_SIZE))
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "batch result is NULL, too small, or has a different api_version");
+  xtbloom_batch_result_t copy;
+  std::memset(&copy, 0, sizeof(copy));
+  std::memcpy(&copy, result, std::min<size_t>(sizeof(copy), result->struct_size));
+  bool admitted = false;
+  const xtbloom_status_t s = compute_impl(context, plan, batch, options, &copy, nullptr, &admitted);
+  if (!admitted) return s;
+  request->state = XTBLOOM_REQUEST_COMPLETE;
+  request->completion_status = s;
+  request->result_flags = s == XTBLOOM_STATUS_SUCCESS ? copy.flags : 0u;
+  request->error = s == XTBLOOM_STATUS_SUCCESS ? std::string() : g_last_error;
+  g_last_error.clear();
+  return XTBLOOM_STATUS_SUCCESS;
+}
+
+xtbloom_status_t xtbloom_compute_enqueue(xtbloom_context_t* context, const xtbloom_batch_t* batch,
+                                         const xtbloom_compute_options_t* options,
+                                         const xtbloom_batch_result_t* result, xtbloom_request_t* request) {
+  return enqueue_impl(context, nullptr, batch, options, result, request);
+}
+
 // ---- fixed-topology plans (reference xtbloom.h:939-991) ----
 xtbloom_status_t xtbloom_plan_create(xtbloom_context_t* context, const xtbloom_batch_t* batch,
                                      const xtbloom_compute_options_t* options, xtbloom_plan_t** plan) {
@@ -731,10 +810,11 @@ xtbloom_status_t xtbloom_plan_compute(xtbloom_plan_t* plan, const xtbloom_batch_
   if (plan == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "plan is NULL");
   return compute_impl(plan->context, plan, batch, options, result, nullptr);
 }
-xtbloom_status_t xtbloom_plan_compute_enqueue(xtbloom_plan_t*, const xtbloom_batch_t*,
-                                              const xtbloom_compute_options_t*,
-                                              const xtbloom_batch_result_t*, xtbloom_request_t*) {
-  return not_supported("fixed-topology plan");
+xtbloom_status_t xtbloom_plan_compute_enqueue(xtbloom_plan_t* plan, const xtbloom_batch_t* batch,
+                                              const xtbloom_compute_options_t* options,
+                                              const xtbloom_batch_result_t* result, xtbloom_request_t* request) {
+  if (plan == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "plan is NULL");
+  return enqueue_impl(plan->context, plan, batch, options, result, request);
 }
 xtbloom_status_t xtbloom_result_owner_options_init(xtbloom_result_owner_options_t* o, size_t size) {
   xtbloom_status_t s = init_struct(o, size, XTBLOOM_RESULT_OWNER_OPTIONS_V1_SIZE, "result owner options");

@@ -226,6 +226,7 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                    max_scc_iterations=250, charge_tolerance=1e-6, energy_tolerance=1e-8,
                    electronic_temperature=300.0, scc_mixer_history=8, scc_mixer_damping=0.4,
                    scc_start_mode=library.SCC_START_FRESH, plan: "Plan | None" = None,
+                   request: "Request | None" = None,
                    extension=False, diagnostics=False, kernel_stats=False) -> ComputedBatch:
     """One ``xtbloom_compute`` over flat ragged arrays (the ``xtbloom_batch_t`` layout).
 
@@ -261,7 +262,23 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
     res.per_system_status = _out(status)
     ext_out = None
     dedq = None
-    if plan is not None:
+    if request is not None:
+        # xtbloom_compute_enqueue / xtbloom_plan_compute_enqueue; the request
+        # reports the completion status (admission errors raise here)
+        if extension or diagnostics or kernel_stats:
+            raise XTBloomValueError("enqueued calls take no extension outputs")
+        if plan is not None:
+            _check(lib.xtbloom_plan_compute_enqueue(plan.handle, ctypes.byref(batch),
+                                                    ctypes.byref(opts), ctypes.byref(res),
+                                                    request.handle))
+        else:
+            _check(lib.xtbloom_compute_enqueue(context.handle, ctypes.byref(batch),
+                                               ctypes.byref(opts), ctypes.byref(res),
+                                               request.handle))
+        done = request.wait()
+        if done["completion_status"] != library.STATUS_SUCCESS:
+            raise XTBloomError(done["completion_status"], request.error())
+    elif plan is not None:
         if extension or diagnostics or kernel_stats:
             raise XTBloomValueError("plan calls take no extension outputs")
         _check(lib.xtbloom_plan_compute(plan.handle, ctypes.byref(batch), ctypes.byref(opts),
@@ -299,6 +316,57 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                                    ctypes.byref(res)))
     return ComputedBatch(atom_offsets, point_offsets, energies, forces, q, pcf, dip, iters,
                          conv.astype(bool), status, dedq, ext_out)
+
+
+class Request:
+    """Reusable submission owner (``xtbloom_request_create``) bound to a context.
+
+    This backend settles each accepted submission inline, so after
+    :func:`compute_arrays` with ``request=`` the request is COMPLETE and
+    :meth:`wait` returns at once (reference ``xtbloom.h`` request API).
+    """
+
+    def __init__(self, context: Context):
+        lib = library.load()
+        handle = ctypes.c_void_p()
+        _check(lib.xtbloom_request_create(context.handle, ctypes.byref(handle)))
+        self._handle = handle
+        self._context = context
+
+    @property
+    def handle(self):
+        if not self._handle:
+            raise XTBloomError(library.STATUS_INVALID_ARGUMENT, "request is closed")
+        return self._handle
+
+    def _info(self, fn) -> dict:
+        lib = library.load()
+        info = library.RequestInfo()
+        _check(lib.xtbloom_request_info_init(ctypes.byref(info), ctypes.sizeof(info)))
+        _check(fn(self.handle, ctypes.byref(info)))
+        return dict(state=info.state, completion_status=info.completion_status,
+                    result_flags=info.result_flags)
+
+    def query(self) -> dict:
+        return self._info(library.load().xtbloom_request_query)
+
+    def wait(self) -> dict:
+        return self._info(library.load().xtbloom_request_wait)
+
+    def error(self) -> str:
+        msg = library.load().xtbloom_request_get_error(self.handle)
+        return msg.decode() if msg else ""
+
+    def close(self):
+        if getattr(self, "_handle", None):
+            library.load().xtbloom_request_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Plan:

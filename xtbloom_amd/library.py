@@ -128,6 +128,15 @@ class Extension(ctypes.Structure):
                 ("kernel_stats_mask", ctypes.c_uint32)]
 
 
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("api_version", ctypes.c_uint32),
+                ("state", ctypes.c_int32), ("completion_status", ctypes.c_int32),
+                ("result_flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+REQUEST_IDLE, REQUEST_PENDING, REQUEST_COMPLETE = 0, 1, 2
+
+
 class WorkspaceQuery(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("api_version", ctypes.c_uint32),
                 ("compute_flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
@@ -145,6 +154,7 @@ assert ctypes.sizeof(ComputeOptions) == 80
 assert ctypes.sizeof(BatchResult) == 280
 assert ctypes.sizeof(KernelStat) == 32
 assert ctypes.sizeof(WorkspaceQuery) == 48 and WorkspaceQuery.device_required_bytes.offset == 32
+assert ctypes.sizeof(RequestInfo) == 24
 
 EXPORTED_SYMBOLS = (
     "xtbloom_version_string", "xtbloom_status_string", "xtbloom_get_last_error",
@@ -220,6 +230,24 @@ def load() -> ctypes.CDLL:
     lib.xtbloom_plan_compute.argtypes = [vp, ctypes.POINTER(Batch), ctypes.POINTER(ComputeOptions),
                                          ctypes.POINTER(BatchResult)]
     lib.xtbloom_plan_compute.restype = ctypes.c_int32
+    lib.xtbloom_request_info_init.argtypes = [ctypes.POINTER(RequestInfo), ctypes.c_size_t]
+    lib.xtbloom_request_info_init.restype = ctypes.c_int32
+    lib.xtbloom_request_create.argtypes = [vp, ctypes.POINTER(vp)]
+    lib.xtbloom_request_create.restype = ctypes.c_int32
+    for name in ("xtbloom_request_query", "xtbloom_request_wait"):
+        getattr(lib, name).argtypes = [vp, ctypes.POINTER(RequestInfo)]
+        getattr(lib, name).restype = ctypes.c_int32
+    lib.xtbloom_request_get_error.argtypes = [vp]
+    lib.xtbloom_request_get_error.restype = ctypes.c_char_p
+    lib.xtbloom_request_destroy.argtypes = [vp]
+    lib.xtbloom_request_destroy.restype = None
+    lib.xtbloom_compute_enqueue.argtypes = [vp, ctypes.POINTER(Batch), ctypes.POINTER(ComputeOptions),
+                                            ctypes.POINTER(BatchResult), vp]
+    lib.xtbloom_compute_enqueue.restype = ctypes.c_int32
+    lib.xtbloom_plan_compute_enqueue.argtypes = [vp, ctypes.POINTER(Batch),
+                                                 ctypes.POINTER(ComputeOptions),
+                                                 ctypes.POINTER(BatchResult), vp]
+    lib.xtbloom_plan_compute_enqueue.restype = ctypes.c_int32
     _LIB = lib
     return lib
 
