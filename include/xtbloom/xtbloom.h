@@ -363,7 +363,10 @@ XTBLOOM_API xtbloom_status_t xtbloom_compute(xtbloom_context_t* context,
                                              const xtbloom_compute_options_t* options,
                                              xtbloom_batch_result_t* result);
 
-/* --- asynchronous requests (reference xtbloom.h:850-878, 933-937); NOT_SUPPORTED here --- */
+/* --- asynchronous requests (reference xtbloom.h:850-878, 933-937).  This
+ * backend settles every accepted submission inline: enqueue returns with the
+ * request COMPLETE (status, error text and result flags in the request);
+ * admission errors return directly and leave the request unchanged. --- */
 XTBLOOM_API xtbloom_status_t xtbloom_request_create(xtbloom_context_t* context,
                                                     xtbloom_request_t** request);
 XTBLOOM_API xtbloom_status_t xtbloom_request_query(xtbloom_request_t* request,
@@ -378,7 +381,9 @@ XTBLOOM_API xtbloom_status_t xtbloom_compute_enqueue(xtbloom_context_t* context,
                                                      const xtbloom_batch_result_t* result,
                                                      xtbloom_request_t* request);
 
-/* --- fixed-topology plans (reference xtbloom.h:960-1009); NOT_SUPPORTED here --- */
+/* --- fixed-topology plans (reference xtbloom.h:960-1009): the plan binds the
+ * topology and compute policy, keeps the packed host topology and its own
+ * strict-WARM checkpoint, and reserves the device workspace at creation. --- */
 XTBLOOM_API xtbloom_status_t xtbloom_plan_create(xtbloom_context_t* context,
                                                  const xtbloom_batch_t* batch,
                                                  const xtbloom_compute_options_t* options,
