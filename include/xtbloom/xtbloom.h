@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-/* Version of this build (the reference generates this from its packaging metadata). */
+/* Version of this library (the reference generates this from its packaging metadata). */
 #define XTBLOOM_VERSION_MAJOR 0
 #define XTBLOOM_VERSION_MINOR 1
 #define XTBLOOM_VERSION_PATCH 0

@@ -1,6 +1,6 @@
 """Parity of the MI355X path (through the C ABI) against the CPU oracle.
 
-Bars (north-star tolerances): energies 1e-7 Eh, forces 1e-6 Eh/bohr, charges
+Bars: energies 1e-7 Eh, forces 1e-6 Eh/bohr, charges
 1e-6 e, SCC iteration counts within +-1, integer layouts exact.  Every case
 prints its name before the GPU step so that a failing step is identifiable.
 """

@@ -108,7 +108,7 @@ def xtbloom_torch(positions, atomic_numbers, atom_offsets, molecular_charges, un
     if method not in ("GFN2-xTB", "GFN2"):
         raise XTBloomNotSupportedError(f"method {method!r} is not available on this backend")
     if backend not in ("hip", "auto"):
-        raise XTBloomNotSupportedError(f"backend {backend!r} is not available in this build")
+        raise XTBloomNotSupportedError(f"backend {backend!r} is not available in this library")
     if not torch.is_tensor(positions) or positions.dtype != torch.float64 or \
             positions.dim() != 2 or positions.shape[1] != 3:
         raise ValueError("positions must be a float64 tensor of shape (natoms, 3)")
