@@ -8,7 +8,7 @@
 // pipeline on its own stream lets the streaming work of one slice fill the
 // compute units that the latency-bound kernels of another slice leave idle,
 // and overlaps one slice's host bookkeeping with another slice's kernels.
-// On the 512-system ragged alkane batch three lanes run 4-7 % faster than one.
+// On the 512-system ragged alkane batch two lanes run 3-5 % faster than one.
 // Lanes are opt-in (XTBLOOM_AMD_LANES): every lane holds a stream, and once a
 // process has more busy streams than the runtime's hardware queues (four by
 // default) streams share queues and serialize, which costs far more than the
