@@ -539,7 +539,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     const int64_t ns = T.shell_off[s + 1] - T.shell_off[s];
     const int64_t na = T.atom_off[s + 1] - T.atom_off[s];
     cls_sys[c].push_back(s);
-    cls_sp[c].push_back(cls_sp[c].back() + ns * ns);
+    cls_sp[c].push_back(cls_sp[c].back() + (int64_t)ns * (ns + 1) / 2);
     cls_ap[c].push_back(cls_ap[c].back() + na * na);
   }
   // Mulliken partials (one per 32-row chunk and column), system by system;

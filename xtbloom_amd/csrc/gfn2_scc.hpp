@@ -139,6 +139,26 @@ __device__ __forceinline__ void h0_spatial(const Topo& t, int si, int sj, double
   dsp = kp * (ks * p2 + kt * p1) * rr / (2.0 * r);
 }
 
+// D and traceless Q of element (n, m) from those of element (m, n), m on atom
+// ar and n on atom ac ([xx, xy, yy, xz, yz, zz])
+__device__ __forceinline__ void transposed_moments(const Topo& t, int ar, int ac, double sv,
+                                                   const double (&dm)[3], const double (&qm)[6],
+                                                   double (&dt)[3], double (&qt)[6]) {
+  const double dl[3] = {t.xyz[3 * ac] - t.xyz[3 * ar], t.xyz[3 * ac + 1] - t.xyz[3 * ar + 1],
+                        t.xyz[3 * ac + 2] - t.xyz[3 * ar + 2]};
+#pragma unroll
+  for (int x = 0; x < 3; ++x) dt[x] = dm[x] + dl[x] * sv;
+  const double dd = dl[0] * dm[0] + dl[1] * dm[1] + dl[2] * dm[2];
+  const double l2 = dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2];
+  const double trc = 0.5 * (2.0 * dd + l2 * sv);
+  qt[0] = qm[0] + 1.5 * (2.0 * dl[0] * dm[0] + dl[0] * dl[0] * sv) - trc;
+  qt[1] = qm[1] + 1.5 * (dl[0] * dm[1] + dl[1] * dm[0] + dl[0] * dl[1] * sv);
+  qt[2] = qm[2] + 1.5 * (2.0 * dl[1] * dm[1] + dl[1] * dl[1] * sv) - trc;
+  qt[3] = qm[3] + 1.5 * (dl[0] * dm[2] + dl[2] * dm[0] + dl[0] * dl[2] * sv);
+  qt[4] = qm[4] + 1.5 * (dl[1] * dm[2] + dl[2] * dm[1] + dl[1] * dl[2] * sv);
+  qt[5] = qm[5] + 1.5 * (2.0 * dl[2] * dm[2] + dl[2] * dl[2] * sv) - trc;
+}
+
 template <int LB, int LK>
 __device__ void integral_block(const Topo& t, int s, int si, int sj, const double* cn, double* S,
                                double* D, double* Q, double* H0) {
@@ -176,6 +196,21 @@ __device__ void integral_block(const Topo& t, int s, int si, int sj, const doubl
       for (int c = 0; c < 3; ++c) D[3 * mo + c * nn + loc] = blk.m[c][u][w];
 #pragma unroll
       for (int c = 0; c < 6; ++c) Q[6 * mo + c * nn + loc] = blk.m[3 + c][u][w];
+      if (si < sj) {
+        // the (sj, si) block from the transposition identities (moments about
+        // the other atom): D_nm = D_mn + d S_mn and the traceless Q likewise
+        double dm[3], qm[6], dt[3], qt[6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dm[c] = blk.m[c][u][w];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) qm[c] = blk.m[3 + c][u][w];
+        transposed_moments(t, A, C, sval, dm, qm, dt, qt);
+        const int64_t tl = (int64_t)(c0 + w) * n + (r0 + u);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) D[3 * mo + c * nn + tl] = dt[c];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) Q[6 * mo + c * nn + tl] = qt[c];
+      }
     }
 }
 
@@ -197,9 +232,10 @@ __device__ __forceinline__ int syslist_find(const SysList& l, int64_t g, int64_t
   return l.sys[lo];
 }
 
-// One thread per ordered shell pair (i, j) of a system (ns x ns items).  HAS_D
-// selects the instantiation: systems whose shells are all s/p use the smaller
-// one, so they do not pay the register budget of d-shell blocks.
+// One thread per unordered shell pair i <= j of a system (ns (ns + 1) / 2
+// items, p = j (j + 1) / 2 + i); the block writes both (i, j) and (j, i).
+// HAS_D selects the instantiation: systems whose shells are all s/p use the
+// smaller one, so they do not pay the register budget of d-shell blocks.
 template <bool HAS_D>
 __global__ __launch_bounds__(128) void k_integrals(Topo t, SysList list, int64_t total,
                                                    const double* cn, double* S, double* D,
@@ -208,8 +244,11 @@ __global__ __launch_bounds__(128) void k_integrals(Topo t, SysList list, int64_t
   if (g >= total) return;
   int64_t loc;
   const int s = syslist_find(list, g, loc);
-  const int ns = t.shell_off[s + 1] - t.shell_off[s];
-  const int si = t.shell_off[s] + (int)(loc / ns), sj = t.shell_off[s] + (int)(loc % ns);
+  int jj = (int)((sqrt(8.0 * (double)loc + 1.0) - 1.0) * 0.5);
+  while ((int64_t)jj * (jj + 1) / 2 > loc) --jj;
+  while ((int64_t)(jj + 1) * (jj + 2) / 2 <= loc) ++jj;
+  const int ii = (int)(loc - (int64_t)jj * (jj + 1) / 2);
+  const int si = t.shell_off[s] + ii, sj = t.shell_off[s] + jj;
   const int key = t.sh_l[si] * 3 + t.sh_l[sj];
   if constexpr (!HAS_D) {
     switch (key) {
@@ -489,26 +528,6 @@ struct FockTile {
   int r0, c0;  // tile origin, r0 <= c0
 };
 constexpr int kFockTile = 32;
-
-// D and traceless Q of element (n, m) from those of element (m, n), m on atom
-// ar and n on atom ac ([xx, xy, yy, xz, yz, zz])
-__device__ __forceinline__ void transposed_moments(const Topo& t, int ar, int ac, double sv,
-                                                   const double (&dm)[3], const double (&qm)[6],
-                                                   double (&dt)[3], double (&qt)[6]) {
-  const double dl[3] = {t.xyz[3 * ac] - t.xyz[3 * ar], t.xyz[3 * ac + 1] - t.xyz[3 * ar + 1],
-                        t.xyz[3 * ac + 2] - t.xyz[3 * ar + 2]};
-#pragma unroll
-  for (int x = 0; x < 3; ++x) dt[x] = dm[x] + dl[x] * sv;
-  const double dd = dl[0] * dm[0] + dl[1] * dm[1] + dl[2] * dm[2];
-  const double l2 = dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2];
-  const double trc = 0.5 * (2.0 * dd + l2 * sv);
-  qt[0] = qm[0] + 1.5 * (2.0 * dl[0] * dm[0] + dl[0] * dl[0] * sv) - trc;
-  qt[1] = qm[1] + 1.5 * (dl[0] * dm[1] + dl[1] * dm[0] + dl[0] * dl[1] * sv);
-  qt[2] = qm[2] + 1.5 * (2.0 * dl[1] * dm[1] + dl[1] * dl[1] * sv) - trc;
-  qt[3] = qm[3] + 1.5 * (dl[0] * dm[2] + dl[2] * dm[0] + dl[0] * dl[2] * sv);
-  qt[4] = qm[4] + 1.5 * (dl[1] * dm[2] + dl[2] * dm[1] + dl[1] * dl[2] * sv);
-  qt[5] = qm[5] + 1.5 * (2.0 * dl[2] * dm[2] + dl[2] * dl[2] * sv) - trc;
-}
 
 __device__ __forceinline__ double fock_element(const Topo& t, int o, int r, int c, double sv,
                                                const double (&dm)[3], const double (&qm)[6],
