@@ -540,7 +540,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     const int64_t na = T.atom_off[s + 1] - T.atom_off[s];
     cls_sys[c].push_back(s);
     cls_sp[c].push_back(cls_sp[c].back() + (int64_t)ns * (ns + 1) / 2);
-    cls_ap[c].push_back(cls_ap[c].back() + na * na);
+    cls_ap[c].push_back(cls_ap[c].back() + na * (na + 1) / 2);
   }
   // Mulliken partials (one per 32-row chunk and column), system by system;
   // k_mulliken_rows runs over the Fock tile list
@@ -1147,10 +1147,12 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
         if (np == 0) continue;
         if (c == 0)
           hipLaunchKernelGGL((k_force_pairs<false>), dim3((unsigned)((np + 127) / 128)), dim3(128), 0, sm, t,
-                             ap_list[c], np, d_valid, d_cn, dP, dW, dS, d_vsh, d_vd, d_vqq, d_grad, d_dedcn);
+                             ap_list[c], np, d_valid, d_cn, dP, dW, dS, dD, d_vsh, d_vd, d_vqq, d_grad,
+                             d_dedcn);
         else
           hipLaunchKernelGGL((k_force_pairs<true>), dim3((unsigned)((np + 127) / 128)), dim3(128), 0, sm, t,
-                             ap_list[c], np, d_valid, d_cn, dP, dW, dS, d_vsh, d_vd, d_vqq, d_grad, d_dedcn);
+                             ap_list[c], np, d_valid, d_cn, dP, dW, dS, dD, d_vsh, d_vd, d_vqq, d_grad,
+                             d_dedcn);
       }
       HIP_TRY(probe_end(kKForcePairs, 0.0, 24.0 * v2));  // reads P, W, S
     }

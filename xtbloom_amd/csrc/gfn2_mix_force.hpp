@@ -248,98 +248,141 @@ __global__ __launch_bounds__(512) void k_energy_mix(Topo t, const int* order, un
 // contracted with recomputed derivative integrals per ordered atom pair; the
 // H0 CN adjoint and polynomial gradient (h0.cpp:469-597) ride along.
 // ---------------------------------------------------------------------------
+// Both ordered blocks (si, sj) and (sj, si) of one shell pair, si on atom A
+// and sj on atom C.  The (sj, si) block's moments are those of (si, sj) moved
+// to the other centre (D' = D + v S and the traceless Q' likewise, v = R_C -
+// R_A), so its adjoints fold into the (si, sj) adjoints and one derivative
+// sweep serves both blocks; the explicit dependence of D', Q' on v adds
+// K (D + v S) - S vd_A per element, with K = -(3 V - tr V) and V the symmetric
+// matrix of A's quadrupole potential.  For A == C only the CN chain is left.
 template <int LB, int LK>
 __device__ void force_block(const Topo& t, int s, int si, int sj, const double* cn,
-                            const double* P, const double* W, const double* Sm,
+                            const double* P, const double* W, const double* Sm, const double* Dm,
                             const double* vsh, const double* vd, const double* vqq, double gA[3],
                             double gC[3], double& dcnA, double& dcnC) {
   const int A = t.sh_atom[si], C = t.sh_atom[sj];
   const int n = t.ao_off[s + 1] - t.ao_off[s];
   const int64_t mo = t.mat_off[s];
   const int r0 = t.sh_ao[si] - t.ao_off[s], c0 = t.sh_ao[sj] - t.ao_off[s];
+  if (A == C) {
+    double wsum = 0.0;
+#pragma unroll
+    for (int u = 0; u < 2 * LB + 1; ++u)
+#pragma unroll
+      for (int w = 0; w < 2 * LK + 1; ++w) {
+        const int64_t idx = mo + (int64_t)(r0 + u) * n + (c0 + w);
+        wsum += P[idx] * Sm[idx];
+      }
+    const double lw = 0.5 * wsum;
+    dcnA -= t.sh_kcn[si] * lw;
+    dcnC -= t.sh_kcn[sj] * lw;
+    return;
+  }
   const double hs = shell_level(t, si, cn), ht = shell_level(t, sj, cn);
   const double avg = 0.5 * (hs + ht);
-  double sp = 1.0, dsp = 0.0, d[3] = {0, 0, 0}, r = 0.0;
-  if (A != C) {
-    for (int x = 0; x < 3; ++x) d[x] = t.xyz[3 * A + x] - t.xyz[3 * C + x];
-    r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    h0_spatial(t, si, sj, r, sp, dsp);
-  }
+  double d[3], sp, dsp;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) d[x] = t.xyz[3 * A + x] - t.xyz[3 * C + x];
+  const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  h0_spatial(t, si, sj, r, sp, dsp);
   const double f = avg * sp;
   const double sf = -0.5 * (vsh[si] + vsh[sj]);
-  double wsum = 0.0;
+  const double v[3] = {-d[0], -d[1], -d[2]};  // R_C - R_A
+  const double* vdA = vd + 3 * A;
+  const double* vqA = vqq + 6 * A;
+  const double trv = vqA[0] + vqA[2] + vqA[5];
+  // K = -(3 V - tr V I), V = [[q0, q1/2, q3/2], [q1/2, q2, q4/2], [q3/2, q4/2, q5]]
+  const double K[3][3] = {{trv - 3.0 * vqA[0], -1.5 * vqA[1], -1.5 * vqA[3]},
+                          {-1.5 * vqA[1], trv - 3.0 * vqA[2], -1.5 * vqA[4]},
+                          {-1.5 * vqA[3], -1.5 * vqA[4], trv - 3.0 * vqA[5]}};
+  double kv[3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) kv[x] = K[x][0] * v[0] + K[x][1] * v[1] + K[x][2] * v[2];
+  const double s_shift = 0.5 * (v[0] * kv[0] + v[1] * kv[1] + v[2] * kv[2]) -
+                         (vdA[0] * v[0] + vdA[1] * v[1] + vdA[2] * v[2]);
+  double wsum = 0.0, gx[3] = {0.0, 0.0, 0.0};
+  const int64_t nn = (int64_t)n * n;
+  const double* D0 = Dm + 3 * mo;
   PairAdjoint<LB, LK> adj;
 #pragma unroll
   for (int u = 0; u < 2 * LB + 1; ++u)
 #pragma unroll
     for (int w = 0; w < 2 * LK + 1; ++w) {
-      const int64_t idx = mo + (int64_t)(r0 + u) * n + (c0 + w);
-      const double p = P[idx];
-      wsum += p * Sm[idx];
-      adj.a[0][u][w] = p * (f + sf) - W[idx];
+      const int64_t loc = (int64_t)(r0 + u) * n + (c0 + w);
+      const double p = P[mo + loc], sv = Sm[mo + loc];
+      const double dm[3] = {D0[loc], D0[nn + loc], D0[2 * nn + loc]};
+      wsum += p * sv;
+      adj.a[0][u][w] = 2.0 * (p * (f + sf) - W[mo + loc]) + p * s_shift;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) adj.a[1 + c][u][w] = -p * vd[3 * C + c];
+      for (int c = 0; c < 3; ++c) adj.a[1 + c][u][w] = p * (kv[c] - vd[3 * C + c] - vdA[c]);
 #pragma unroll
-      for (int c = 0; c < 6; ++c) adj.a[4 + c][u][w] = -p * vqq[6 * C + c];
+      for (int c = 0; c < 6; ++c) adj.a[4 + c][u][w] = -p * (vqq[6 * C + c] + vqA[c]);
+#pragma unroll
+      for (int x = 0; x < 3; ++x)
+        gx[x] += p * (sv * (kv[x] - vdA[x]) + K[x][0] * dm[0] + K[x][1] * dm[1] + K[x][2] * dm[2]);
     }
-  const double lw = 0.5 * wsum * sp;
+  const double lw = wsum * sp;  // both blocks
   dcnA -= t.sh_kcn[si] * lw;
   dcnC -= t.sh_kcn[sj] * lw;
-  if (A == C) return;
-  const double sc = wsum * avg * dsp / r;
+  const double sc = 2.0 * wsum * avg * dsp / r;
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
     gA[x] += sc * d[x];
     gC[x] -= sc * d[x];
   }
-  const double v[3] = {-d[0], -d[1], -d[2]};  // R_C - R_A
   double g[3] = {0.0, 0.0, 0.0};
   pair_gradient<LB, LK>(t, si, sj, v, adj, g);
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
-    gC[x] += g[x];
-    gA[x] -= g[x];
+    gC[x] += g[x] + gx[x];
+    gA[x] -= g[x] + gx[x];
   }
 }
 
-// one thread per ordered atom pair (A, C) of a system (N x N items); HAS_D as
-// for k_integrals
+// One thread per unordered atom pair A <= C of a system (N (N + 1) / 2 items,
+// p = C (C + 1) / 2 + A); for A < C every shell pair covers both orders, for
+// A == C the thread runs all ordered shell pairs of the atom.  HAS_D as for
+// k_integrals.
 template <bool HAS_D>
 __global__ __launch_bounds__(128) void k_force_pairs(Topo t, SysList list, const int64_t total,
                                                      const unsigned char* valid, const double* cn,
                                                      const double* P, const double* W,
-                                                     const double* Sm, const double* vsh,
-                                                     const double* vd, const double* vqq,
-                                                     double* grad, double* dedcn) {
+                                                     const double* Sm, const double* Dm,
+                                                     const double* vsh, const double* vd,
+                                                     const double* vqq, double* grad,
+                                                     double* dedcn) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   int64_t loc;
   const int s = syslist_find(list, g, loc);
   if (!valid[s]) return;
-  const int a0 = t.atom_off[s], N = t.atom_off[s + 1] - a0;
-  const int A = a0 + (int)(loc / N), C = a0 + (int)(loc % N);
+  const int a0 = t.atom_off[s];
+  int cc = (int)((sqrt(8.0 * (double)loc + 1.0) - 1.0) * 0.5);
+  while ((int64_t)cc * (cc + 1) / 2 > loc) --cc;
+  while ((int64_t)(cc + 1) * (cc + 2) / 2 <= loc) ++cc;
+  const int A = a0 + (int)(loc - (int64_t)cc * (cc + 1) / 2), C = a0 + cc;
   double gA[3] = {0, 0, 0}, gC[3] = {0, 0, 0}, dA = 0.0, dC = 0.0;
   for (int si = t.atom_sh0[A]; si < t.atom_sh0[A + 1]; ++si)
     for (int sj = t.atom_sh0[C]; sj < t.atom_sh0[C + 1]; ++sj) {
       const int key = t.sh_l[si] * 3 + t.sh_l[sj];
       if constexpr (!HAS_D) {
         switch (key) {
-          case 0: force_block<0, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-          case 1: force_block<0, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-          case 3: force_block<1, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-          default: force_block<1, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 0: force_block<0, 0>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 1: force_block<0, 1>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 3: force_block<1, 0>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          default: force_block<1, 1>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
         }
       } else {
         switch (key) {
-          case 0: force_block<0, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-          case 1: force_block<0, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-          case 2: force_block<0, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-          case 3: force_block<1, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-          case 4: force_block<1, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-          case 5: force_block<1, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-          case 6: force_block<2, 0>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-          case 7: force_block<2, 1>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
-          default: force_block<2, 2>(t, s, si, sj, cn, P, W, Sm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 0: force_block<0, 0>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 1: force_block<0, 1>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 2: force_block<0, 2>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 3: force_block<1, 0>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 4: force_block<1, 1>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 5: force_block<1, 2>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 6: force_block<2, 0>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          case 7: force_block<2, 1>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
+          default: force_block<2, 2>(t, s, si, sj, cn, P, W, Sm, Dm, vsh, vd, vqq, gA, gC, dA, dC); break;
         }
       }
     }
