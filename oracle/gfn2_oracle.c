@@ -1363,9 +1363,11 @@ static int system_setup(System* sy, int32_t nat, const int32_t* Z, const double*
   for (int s = 0; s < ns; ++s) nel0 += xtbp_shell_refocc[sy->b.sh_par[s]];
   double nel = nel0 - charge;
   long nint = lround(nel);
-  if (nel < 0.0 || ((nint & 1) != (unpaired & 1))) return ST_INVALID;
+  if (unpaired < 0 || nel < 0.0 || ((nint & 1) != (unpaired & 1))) return ST_INVALID;
   *nalpha = 0.5 * (nel + unpaired);
   *nbeta = 0.5 * (nel - unpaired);
+  /* wavefunction.cpp:632-636: beta >= 0 and both counts within the orbital space */
+  if (*nbeta < 0.0 || *nalpha > n || *nbeta > n) return ST_INVALID;
   sy->S = malloc(sizeof(double) * nn); sy->D = malloc(sizeof(double) * 3 * nn);
   sy->Q = malloc(sizeof(double) * 6 * nn); sy->H0 = malloc(sizeof(double) * nn);
   sy->G = malloc(sizeof(double) * ns * ns); sy->cn = malloc(sizeof(double) * nat);

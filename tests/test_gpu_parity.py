@@ -260,3 +260,49 @@ def test_point_charge_potential_gradient_fd(ctx):
             vals.append(run(ctx, bb, charge_tolerance=1e-10, energy_tolerance=1e-12).energies[0])
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(fd - out.point_charge_potential_gradient[p]) < 1e-6
+
+
+def test_option_and_electron_count_validation(ctx):
+    # reference validation.cpp:426-461 and wavefunction.cpp:606-636; every
+    # refusal happens before any output is written
+    b = workloads.make("water8", seed=5, batch=2)
+
+    def expect_invalid(label, **kw):
+        print("case invalid option", label, flush=True)
+        with pytest.raises(interface.XTBloomError) as err:
+            run(ctx, b, **kw)
+        assert err.value.status == library.STATUS_INVALID_ARGUMENT, label
+
+    def setter(field, value):
+        return lambda o: setattr(o, field, value)
+
+    expect_invalid("reserved", options_hook=setter("reserved", 1))
+    expect_invalid("reserved_v2", options_hook=setter("reserved_v2", 1))
+    expect_invalid("reserved_v3", options_hook=setter("reserved_v3", 1))
+    expect_invalid("scc_start_mode 0", options_hook=setter("scc_start_mode", 0))
+    expect_invalid("scc_start_mode 3", options_hook=setter("scc_start_mode", 3))
+    expect_invalid("scc_mixer 0", options_hook=setter("scc_mixer", 0))
+    expect_invalid("determinism 2", determinism=2)
+    # charge +6 leaves two electrons, four unpaired need beta = -1
+    print("case invalid electron counts", flush=True)
+    with pytest.raises(interface.XTBloomError) as err:
+        interface.compute_arrays(ctx, b.atom_offsets, b.atomic_numbers, b.positions,
+                                 np.array([0.0, 6.0]), np.array([0, 4], np.int32))
+    assert err.value.status == library.STATUS_INVALID_ARGUMENT
+    assert "beta" in str(err.value)
+
+
+def test_reproducible_mode_is_bit_exact(ctx):
+    # XTBLOOM_DETERMINISM_REPRODUCIBLE: repeated calls agree bit for bit, forces
+    # included (order-independent sums of the scattered gradient contributions)
+    b = workloads.make("embedded256", seed=12, batch=24, kmin=2, kmax=20, points=8)
+    outs = []
+    for k in range(3):
+        print("case reproducible call", k, flush=True)
+        outs.append(run(ctx, b, determinism=library.DETERMINISM_REPRODUCIBLE))
+    for o in outs[1:]:
+        for name in ("energies", "forces", "charges", "point_charge_forces", "scc_iterations"):
+            assert np.array_equal(getattr(o, name), getattr(outs[0], name)), name
+    compare(b, outs[0], "reproducible-vs-oracle")
+    default = run(ctx, b)
+    assert np.abs(default.forces - outs[0].forces).max() < 1e-10

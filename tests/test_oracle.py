@@ -153,3 +153,17 @@ def test_batch_equals_sequential():
         assert out["status"][i] == 0 and out["iterations"][i] == r.iterations
         assert out["energies"][i] == r.energy
         assert np.array_equal(out["gradient"][a0:a1], r.gradient)
+
+
+def test_electron_count_validation():
+    # reference wavefunction.cpp:606-636: unpaired >= 0, matching parity,
+    # beta = (n_el - unpaired)/2 >= 0 and both spin counts within the orbital space
+    Z = np.array([8, 1, 1])
+    xyz = np.array([[0.0, 0.0, 0.0], [1.81, 0.0, 0.0], [-0.45, 1.75, 0.0]])
+    assert O.singlepoint(Z, xyz).status == 0
+    # charge +6 leaves 2 electrons: four unpaired electrons would need beta = -1
+    assert O.singlepoint(Z, xyz, charge=6.0, uhf=4).status == 1
+    assert O.singlepoint(Z, xyz, charge=0.0, uhf=-2).status == 1
+    assert O.singlepoint(Z, xyz, charge=0.0, uhf=1).status == 1   # parity
+    # 8 electrons in 6 orbitals: alpha = 7 > 6 with six unpaired electrons
+    assert O.singlepoint(Z, xyz, charge=0.0, uhf=6).status == 1

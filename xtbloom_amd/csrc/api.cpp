@@ -218,27 +218,39 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "SCC tolerances must be finite and positive");
   if (!(options->electronic_temperature >= 0.0) || !std::isfinite(options->electronic_temperature))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "electronic_temperature must be finite and nonnegative");
+  // reserved fields and enum ranges (reference validation.cpp:426-461)
+  if (options->reserved != 0)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "compute options reserved field must be zero");
   ro.max_iterations = options->max_scc_iterations;
   ro.charge_tolerance = options->charge_tolerance;
   ro.energy_tolerance = options->energy_tolerance;
   ro.kt = options->electronic_temperature;
   if (options->struct_size >= XTBLOOM_COMPUTE_OPTIONS_V2_SIZE) {
-    if (options->scc_start_mode != XTBLOOM_SCC_START_FRESH && options->scc_start_mode != XTBLOOM_SCC_START_WARM &&
-        options->scc_start_mode != 0)
+    if (options->scc_start_mode != XTBLOOM_SCC_START_FRESH && options->scc_start_mode != XTBLOOM_SCC_START_WARM)
       return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "scc_start_mode must be XTBLOOM_SCC_START_FRESH or XTBLOOM_SCC_START_WARM");
+    if (options->reserved_v2 != 0)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "compute options reserved_v2 field must be zero");
     warm = options->scc_start_mode == XTBLOOM_SCC_START_WARM;
   }
-  if (options->struct_size >= XTBLOOM_COMPUTE_OPTIONS_V3_SIZE &&
-      options->scc_mixer == XTBLOOM_SCC_MIXER_MODIFIED_BROYDEN) {
+  if (options->struct_size >= XTBLOOM_COMPUTE_OPTIONS_V3_SIZE) {
+    if (options->scc_mixer != XTBLOOM_SCC_MIXER_MODIFIED_BROYDEN)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "scc_mixer must be XTBLOOM_SCC_MIXER_MODIFIED_BROYDEN");
     if (options->scc_mixer_history < 1 || options->scc_mixer_history > 64)
       return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "scc_mixer_history must be in [1, 64]");
     if (!(options->scc_mixer_damping > 0.0) || !(options->scc_mixer_damping <= 1.0))
       return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "scc_mixer_damping must be in (0, 1]");
+    if (options->determinism != XTBLOOM_DETERMINISM_DEFAULT &&
+        options->determinism != XTBLOOM_DETERMINISM_REPRODUCIBLE)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT,
+                  "determinism must be XTBLOOM_DETERMINISM_DEFAULT or XTBLOOM_DETERMINISM_REPRODUCIBLE");
+    if (options->reserved_v3 != 0)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "compute options reserved_v3 field must be zero");
     ro.history = options->scc_mixer_history;
     ro.damping = options->scc_mixer_damping;
-  } else if (options->struct_size >= XTBLOOM_COMPUTE_OPTIONS_V3_SIZE && options->scc_mixer != 0) {
-    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "unknown scc_mixer");
+    ro.reproducible = options->determinism == XTBLOOM_DETERMINISM_REPRODUCIBLE;
   }
+  if (result != nullptr && result->reserved != 0)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "batch result reserved field must be zero");
   // result descriptor
   const bool want_e = flags & XTBLOOM_COMPUTE_ENERGY, want_f = flags & XTBLOOM_COMPUTE_FORCES;
   const bool want_q = flags & XTBLOOM_COMPUTE_ATOMIC_CHARGES;

@@ -32,6 +32,9 @@ struct RunOptions {
   int history = 8;
   double damping = 0.4;
   bool forces = true;
+  // XTBLOOM_DETERMINISM_REPRODUCIBLE: order-independent (fixed-point) sums of
+  // the scattered force contributions, so repeated calls agree bit for bit
+  bool reproducible = false;
   // WARM start: the SCC multipole state to start from instead of the FRESH
   // SAD state, per system (shell charges, then 3 dipole and 6 quadrupole
   // components per atom), systems in batch order; null = FRESH

@@ -247,8 +247,8 @@ int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
     }
     T.nalpha[s] = 0.5 * (nel + hb.unpaired[s]);
     T.nbeta[s] = 0.5 * (nel - hb.unpaired[s]);
-    if (T.nalpha[s] > nao || T.nbeta[s] > nao) {
-      error = "electron count exceeds the orbital space";
+    if (T.nbeta[s] < 0.0 || T.nalpha[s] > nao || T.nbeta[s] > nao) {
+      error = "alpha or beta electron count cannot be represented by the system orbital space";
       return 1;
     }
     T.pc_off[s + 1] = hb.point_offsets ? (int)hb.point_offsets[s + 1] : 0;
@@ -1145,14 +1145,25 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
       for (int c = 0; c < 2; ++c) {
         const int64_t np = cls_ap[c].back();
         if (np == 0) continue;
-        if (c == 0)
-          hipLaunchKernelGGL((k_force_pairs<false>), dim3((unsigned)((np + 127) / 128)), dim3(128), 0, sm, t,
-                             ap_list[c], np, d_valid, d_cn, dP, dW, dS, dD, d_vsh, d_vd, d_vqq, d_grad,
-                             d_dedcn);
+        const dim3 grid((unsigned)((np + 127) / 128));
+        if (c == 0 && !opt.reproducible)
+          hipLaunchKernelGGL((k_force_pairs<false, false>), grid, dim3(128), 0, sm, t, ap_list[c], np,
+                             d_valid, d_cn, dP, dW, dS, dD, d_vsh, d_vd, d_vqq, d_grad, d_dedcn);
+        else if (c == 0)
+          hipLaunchKernelGGL((k_force_pairs<false, true>), grid, dim3(128), 0, sm, t, ap_list[c], np,
+                             d_valid, d_cn, dP, dW, dS, dD, d_vsh, d_vd, d_vqq, d_grad, d_dedcn);
+        else if (!opt.reproducible)
+          hipLaunchKernelGGL((k_force_pairs<true, false>), grid, dim3(128), 0, sm, t, ap_list[c], np,
+                             d_valid, d_cn, dP, dW, dS, dD, d_vsh, d_vd, d_vqq, d_grad, d_dedcn);
         else
-          hipLaunchKernelGGL((k_force_pairs<true>), dim3((unsigned)((np + 127) / 128)), dim3(128), 0, sm, t,
-                             ap_list[c], np, d_valid, d_cn, dP, dW, dS, dD, d_vsh, d_vd, d_vqq, d_grad,
-                             d_dedcn);
+          hipLaunchKernelGGL((k_force_pairs<true, true>), grid, dim3(128), 0, sm, t, ap_list[c], np,
+                             d_valid, d_cn, dP, dW, dS, dD, d_vsh, d_vd, d_vqq, d_grad, d_dedcn);
+      }
+      if (opt.reproducible) {
+        hipLaunchKernelGGL(k_fix_to_double, dim3((unsigned)((3 * (int64_t)N + 255) / 256)), dim3(256), 0, sm,
+                           d_grad, 3 * (int64_t)N);
+        hipLaunchKernelGGL(k_fix_to_double, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, sm, d_dedcn,
+                           (int64_t)N);
       }
       HIP_TRY(probe_end(kKForcePairs, 0.0, 24.0 * v2));  // reads P, W, S
     }
@@ -1163,8 +1174,12 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
                      d_erep, d_pcgrad, d_dedq);
   HIP_TRY(probe_end(kKForceAtoms, 0.0, 0.0));
   HIP_TRY(probe_begin(kKForceAtm));
-  hipLaunchKernelGGL(k_force_atm, dim3(B), dim3(512), 0, sm, t, d_order, d_valid, d_cnd4, d_w4,
-                     d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm);
+  if (opt.reproducible)
+    hipLaunchKernelGGL(k_force_atm<true>, dim3(B), dim3(512), 0, sm, t, d_order, d_valid, d_cnd4, d_w4,
+                       d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm);
+  else
+    hipLaunchKernelGGL(k_force_atm<false>, dim3(B), dim3(512), 0, sm, t, d_order, d_valid, d_cnd4, d_w4,
+                       d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm);
   HIP_TRY(probe_end(kKForceAtm, 0.0, 0.0));
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(dev->ev[3], sm));

@@ -339,11 +339,38 @@ __device__ void force_block(const Topo& t, int s, int si, int sj, const double* 
   }
 }
 
+// Fixed-point accumulation for XTBLOOM_DETERMINISM_REPRODUCIBLE: every
+// contribution is rounded to a multiple of 2^-40 and added as a 64-bit integer,
+// so the sum does not depend on the order in which the atomics land (integer
+// addition is associative).  Resolution 9.1e-13, range +-8.4e6 (Eh/bohr for
+// gradients, Eh for dE/dCN), far beyond any gradient of a valid geometry.
+constexpr double kFixScale = 1099511627776.0;  // 2^40
+__device__ __forceinline__ unsigned long long fix_encode(double v) {
+  return (unsigned long long)(long long)__double2ll_rn(v * kFixScale);
+}
+__device__ __forceinline__ double fix_decode(unsigned long long v) {
+  return (double)(long long)v * (1.0 / kFixScale);
+}
+template <bool FIX>
+__device__ __forceinline__ void acc_add(double* p, double v) {
+  if constexpr (FIX)
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), fix_encode(v));
+  else
+    atomicAdd(p, v);
+}
+
+// Converts n fixed-point accumulators in place to doubles.
+__global__ void k_fix_to_double(double* buf, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) buf[i] = fix_decode(reinterpret_cast<unsigned long long*>(buf)[i]);
+}
+
 // One thread per unordered atom pair A <= C of a system (N (N + 1) / 2 items,
 // p = C (C + 1) / 2 + A); for A < C every shell pair covers both orders, for
 // A == C the thread runs all ordered shell pairs of the atom.  HAS_D as for
-// k_integrals.
-template <bool HAS_D>
+// k_integrals.  FIX: grad and dedcn hold fixed-point accumulators (reproducible
+// mode, zeroed before the launch and converted by k_fix_to_double after it).
+template <bool HAS_D, bool FIX>
 __global__ __launch_bounds__(128) void k_force_pairs(Topo t, SysList list, const int64_t total,
                                                      const unsigned char* valid, const double* cn,
                                                      const double* P, const double* W,
@@ -388,12 +415,12 @@ __global__ __launch_bounds__(128) void k_force_pairs(Topo t, SysList list, const
     }
   if (A != C) {
     for (int x = 0; x < 3; ++x) {
-      atomicAdd(&grad[3 * A + x], gA[x]);
-      atomicAdd(&grad[3 * C + x], gC[x]);
+      acc_add<FIX>(&grad[3 * A + x], gA[x]);
+      acc_add<FIX>(&grad[3 * C + x], gC[x]);
     }
   }
-  atomicAdd(&dedcn[A], dA);
-  atomicAdd(&dedcn[C], dC);
+  acc_add<FIX>(&dedcn[A], dA);
+  acc_add<FIX>(&dedcn[C], dC);
 }
 
 // AES2 fixed-multipole VJP of one unordered pair, first < second (reference aes2.cpp:601-706).
@@ -632,8 +659,11 @@ __global__ __launch_bounds__(512) void k_force_atoms(Topo t, const int* order,
 // length; gradient and CN adjoint of i go to LDS atomics (one system per block,
 // at most kMaxAO atoms), those of j and k stay in registers.  The D4-CN chain
 // finishes the kernel.
+// FIX: the LDS sums of the i-atom adjoints run in fixed point (reproducible
+// mode), so their value does not depend on the order of the LDS atomics.
 constexpr int kAtmFields = 5;
 
+template <bool FIX>
 __global__ __launch_bounds__(512) void k_force_atm(Topo t, const int* order, const unsigned char* valid,
                                                    const double* cnd4, double* w4, double* dw4c,
                                                    double* pairbuf, double* dedcn4, double* grad,
@@ -729,25 +759,33 @@ __global__ __launch_bounds__(512) void k_force_atm(Topo t, const int* order, con
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        atomicAdd(&gsh[3 * li + c], -(dg[0][c] + dg[1][c]));
+        acc_add<FIX>(&gsh[3 * li + c], -(dg[0][c] + dg[1][c]));
         gj[c] += dg[0][c] - dg[2][c];
         gk[c] += dg[1][c] + dg[2][c];
       }
       // dC6(x, y)/dCN_x / C6(x, y) sits at the ordered pair [x][y]
-      atomicAdd(&csh[li], -0.5 * de * (pdc[ij] + pdc[ik]));
+      acc_add<FIX>(&csh[li], -0.5 * de * (pdc[ij] + pdc[ik]));
       dcj -= 0.5 * de * (pdc[(int64_t)lj * N + li] + dcjk_j);
       dck -= 0.5 * de * (pdc[(int64_t)lk * N + li] + dcjk_k);
     }
     for (int c = 0; c < 3; ++c) {
-      atomicAdd(&gsh[3 * lj + c], gj[c]);
-      atomicAdd(&gsh[3 * lk + c], gk[c]);
+      acc_add<FIX>(&gsh[3 * lj + c], gj[c]);
+      acc_add<FIX>(&gsh[3 * lk + c], gk[c]);
     }
-    atomicAdd(&csh[lj], dcj);
-    atomicAdd(&csh[lk], dck);
+    acc_add<FIX>(&csh[lj], dcj);
+    acc_add<FIX>(&csh[lk], dck);
   }
   e = block_sum(e, red);
   if (tid == 0) eatm[s] = e;
   __syncthreads();
+  if constexpr (FIX) {
+    for (int la = tid; la < N; la += nt) {
+      for (int c = 0; c < 3; ++c)
+        gsh[3 * la + c] = fix_decode(reinterpret_cast<unsigned long long*>(gsh)[3 * la + c]);
+      csh[la] = fix_decode(reinterpret_cast<unsigned long long*>(csh)[la]);
+    }
+    __syncthreads();
+  }
   // D4-CN chain of the ATM adjoint
   for (int a = a0 + tid; a < a1; a += nt) {
     const int la = a - a0;

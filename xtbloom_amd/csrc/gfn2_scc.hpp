@@ -159,6 +159,9 @@ __device__ __forceinline__ void transposed_moments(const Topo& t, int ar, int ac
   qt[5] = qm[5] + 1.5 * (2.0 * dl[2] * dm[2] + dl[2] * dl[2] * sv) - trc;
 }
 
+// Tile edge of the Fock and Mulliken kernels (kFockTile below).
+constexpr int kIntMirrorTile = 32;
+
 template <int LB, int LK>
 __device__ void integral_block(const Topo& t, int s, int si, int sj, const double* cn, double* S,
                                double* D, double* Q, double* H0) {
@@ -196,9 +199,13 @@ __device__ void integral_block(const Topo& t, int s, int si, int sj, const doubl
       for (int c = 0; c < 3; ++c) D[3 * mo + c * nn + loc] = blk.m[c][u][w];
 #pragma unroll
       for (int c = 0; c < 6; ++c) Q[6 * mo + c * nn + loc] = blk.m[3 + c][u][w];
-      if (si < sj) {
-        // the (sj, si) block from the transposition identities (moments about
-        // the other atom): D_nm = D_mn + d S_mn and the traceless Q likewise
+      // the (sj, si) block from the transposition identities (moments about
+      // the other atom): D_nm = D_mn + d S_mn and the traceless Q likewise.
+      // Only elements inside a diagonal Fock tile are ever read (the Fock and
+      // Mulliken kernels take the transposed moments of off-diagonal tiles from
+      // the same identities, the force kernel reads forward blocks only), so
+      // the other mirrored elements are not stored.
+      if (si < sj && ((r0 + u) / kIntMirrorTile) == ((c0 + w) / kIntMirrorTile)) {
         double dm[3], qm[6], dt[3], qt[6];
 #pragma unroll
         for (int c = 0; c < 3; ++c) dm[c] = blk.m[c][u][w];
@@ -528,6 +535,7 @@ struct FockTile {
   int r0, c0;  // tile origin, r0 <= c0
 };
 constexpr int kFockTile = 32;
+static_assert(kFockTile == kIntMirrorTile, "the integral kernel stores the mirrored D/Q of diagonal Fock tiles");
 
 __device__ __forceinline__ double fock_element(const Topo& t, int o, int r, int c, double sv,
                                                const double (&dm)[3], const double (&qm)[6],

@@ -174,7 +174,7 @@ _DEFAULT_FLAGS = library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPU
 def _descriptors(lib, atom_offsets, numbers, positions, charges, uhf, point_offsets,
                  point_positions, point_values, point_gammas, model, flags, max_scc_iterations,
                  charge_tolerance, energy_tolerance, electronic_temperature, scc_mixer_history,
-                 scc_mixer_damping, scc_start_mode):
+                 scc_mixer_damping, scc_start_mode, determinism=0):
     """Batch and compute-options descriptors over flat host arrays (kept alive by the caller)."""
     atom_offsets = np.ascontiguousarray(atom_offsets, dtype=np.int64)
     numbers = np.ascontiguousarray(numbers, dtype=np.int32)
@@ -215,6 +215,7 @@ def _descriptors(lib, atom_offsets, numbers, positions, charges, uhf, point_offs
     opts.scc_mixer_history = scc_mixer_history
     opts.scc_mixer_damping = scc_mixer_damping
     opts.scc_start_mode = scc_start_mode
+    opts.determinism = determinism
     keep = (atom_offsets, numbers, positions, charges, uhf, point_offsets, point_positions,
             point_values, point_gammas)
     return batch, opts, keep, (B, N, P)
@@ -227,20 +228,26 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                    electronic_temperature=300.0, scc_mixer_history=8, scc_mixer_damping=0.4,
                    scc_start_mode=library.SCC_START_FRESH, plan: "Plan | None" = None,
                    request: "Request | None" = None,
-                   extension=False, diagnostics=False, kernel_stats=False) -> ComputedBatch:
+                   extension=False, diagnostics=False, kernel_stats=False,
+                   determinism=library.DETERMINISM_DEFAULT, options_hook=None) -> ComputedBatch:
     """One ``xtbloom_compute`` over flat ragged arrays (the ``xtbloom_batch_t`` layout).
 
     ``scc_start_mode=library.SCC_START_WARM`` starts from the converged state of
     the latest fully converged call with the same topology and policy (strict:
     no such call raises ``XTBloomError`` with INVALID_ARGUMENT).  With ``plan``
     the call goes through ``xtbloom_plan_compute`` of that plan.
+    ``determinism=library.DETERMINISM_REPRODUCIBLE`` asks for bit-identical
+    results on repeated calls.  ``options_hook`` (optional) receives the
+    compute-options struct right before the call (tests of the validation).
     """
     lib = library.load()
     batch, opts, keep, (B, N, P) = _descriptors(
         lib, atom_offsets, numbers, positions, charges, uhf, point_offsets, point_positions,
         point_values, point_gammas, model, flags, max_scc_iterations, charge_tolerance,
         energy_tolerance, electronic_temperature, scc_mixer_history, scc_mixer_damping,
-        scc_start_mode)
+        scc_start_mode, determinism)
+    if options_hook is not None:
+        options_hook(opts)
     atom_offsets, point_offsets = keep[0], keep[5]
     energies = np.full(B, np.nan)
     forces = np.full((N, 3), np.nan)

@@ -47,6 +47,8 @@ COMPUTE_DIPOLE_MOMENTS = 1 << 4
 SCC_START_FRESH = 1
 SCC_START_WARM = 2
 SCC_MIXER_MODIFIED_BROYDEN = 1
+DETERMINISM_DEFAULT = 0
+DETERMINISM_REPRODUCIBLE = 1
 DEFAULT_SCC_MIXER_HISTORY = 8
 DEFAULT_SCC_MIXER_DAMPING = 0.4
 
