@@ -58,13 +58,19 @@ def test_c20h42_128_cold():
     dit = np.abs(out.scc_iterations.astype(int) - ref["iterations"].astype(int)).max()
     print(f"c20h42_128: |dE| {de:.2e} |dF| {df:.2e} |dq| {dq:.2e} d_iter {dit}", flush=True)
     assert de <= E_TOL and df <= F_TOL and dq <= Q_TOL and dit <= 1
-    # size-independent properties of every member
+    # size-independent properties of every member.  The net force vanishes to
+    # rounding (every pair term is antisymmetric); the net torque vanishes only
+    # at exact SCC stationarity, so at the default charge tolerance (1e-6) it
+    # is bounded by the force tolerance and must equal the oracle's torque.
     for i in range(b.batch_size):
         a0, a1 = b.atom_offsets[i], b.atom_offsets[i + 1]
         f = out.forces[a0:a1]
         r = b.positions[a0:a1]
+        tau = np.cross(r, f).sum(axis=0)
+        tau_ref = np.cross(r, -ref["gradient"][a0:a1]).sum(axis=0)
         assert np.abs(f.sum(axis=0)).max() < 1e-9, i                     # translation
-        assert np.abs(np.cross(r, f).sum(axis=0)).max() < 1e-8, i         # rotation
+        assert np.abs(tau).max() < F_TOL, i                                # rotation
+        assert np.abs(tau - tau_ref).max() < 1e-9, i
         assert abs(out.charges[a0:a1].sum()) < 1e-9, i                     # neutral
     assert np.all(np.isfinite(out.energies))
 
