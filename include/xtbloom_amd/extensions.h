@@ -1,10 +1,12 @@
 /*
  * MI355X-backend extensions to the xtbloom C ABI.
  *
- * Neither entry point has a counterpart in the reference ABI.  They exist for
- * callers of this backend that need more than xtbloom_batch_result_t carries:
- * the differentiable PyTorch binding (dE/dQ of external point charges) and the
- * parity tests (packed intermediate matrices and energy components).
+ * No entry point here has a counterpart in the reference ABI.  The extended
+ * compute call and the packed-matrix size exist for callers of this backend
+ * that need more than xtbloom_batch_result_t carries: the differentiable
+ * PyTorch binding (dE/dQ of external point charges) and the parity tests
+ * (packed intermediate matrices and energy components).  The no-allocation
+ * compute call exists for callers that cannot allocate.
  */
 #ifndef XTBLOOM_AMD_EXTENSIONS_H
 #define XTBLOOM_AMD_EXTENSIONS_H
@@ -81,6 +83,28 @@ XTBLOOM_API xtbloom_status_t xtbloom_amd_compute_extended(xtbloom_context_t* con
                                                           const xtbloom_compute_options_t* options,
                                                           xtbloom_batch_result_t* result,
                                                           xtbloom_amd_extension_t* extension);
+
+/* Size of the fixed per-GPU device workspace of xtbloom_amd_compute_no_alloc. */
+#define XTBLOOM_AMD_NO_ALLOC_WORKSPACE_BYTES ((uint64_t)64 << 20)
+
+/* xtbloom_compute that allocates no device memory and no pinned host memory.
+ * No counterpart in the reference ABI: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The caller passes inputs and outputs as for xtbloom_compute;
+ * the device workspace is a fixed array of XTBLOOM_AMD_NO_ALLOC_WORKSPACE_BYTES
+ * compiled into the library, one per GPU, held until the process ends.  A
+ * batch whose workspace exceeds it returns XTBLOOM_STATUS_INVALID_ARGUMENT
+ * before any output is written (the message states the bytes it needs).
+ * Every no-allocation call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other contexts, other threads) race.  The call runs on
+ * the context's stream without concurrent lanes; host staging uses the
+ * context's pinned block when it is large enough and pageable copies
+ * otherwise.  Results, statuses, WARM checkpoints and errors are those of
+ * xtbloom_compute. */
+XTBLOOM_API xtbloom_status_t xtbloom_amd_compute_no_alloc(xtbloom_context_t* context,
+                                                          const xtbloom_batch_t* batch,
+                                                          const xtbloom_compute_options_t* options,
+                                                          xtbloom_batch_result_t* result);
 
 /* sum over systems of n_ao^2 for a host batch; -1 on invalid input. */
 XTBLOOM_API int64_t xtbloom_amd_packed_matrix_elements(const xtbloom_batch_t* batch);

@@ -27,6 +27,7 @@ static_assert(sizeof(xtb::KernelStat) == sizeof(xtbloom_amd_kernel_stat_t), "ker
 static_assert(offsetof(xtb::KernelStat, bytes) == offsetof(xtbloom_amd_kernel_stat_t, bytes),
               "kernel stat layout");
 static_assert((int)xtb::kKernelCount == (int)XTBLOOM_AMD_KERNEL_COUNT, "kernel family count");
+static_assert(xtb::kNoAllocWorkspaceBytes == XTBLOOM_AMD_NO_ALLOC_WORKSPACE_BYTES, "no-allocation workspace size");
 
 // Identity of a call for strict WARM starts and fixed-topology plans
 // (reference xtbloom.h:502-520): the topology (atom offsets, element numbers,
@@ -407,7 +408,8 @@ std::vector<xtb::Device*> lanes_for(xtbloom_context_t* context, int64_t B) {
 // checks) before it returned.
 xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, const xtbloom_batch_t* batch,
                               const xtbloom_compute_options_t* options, xtbloom_batch_result_t* result,
-                              xtbloom_amd_extension_t* ext, bool* admitted = nullptr) {
+                              xtbloom_amd_extension_t* ext, bool* admitted = nullptr,
+                              bool no_alloc = false) {
   if (admitted) *admitted = false;
   if (context == nullptr || context->device == nullptr)
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "context is NULL");
@@ -441,6 +443,7 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, 
   if (admitted) *admitted = true;
   cp.valid = false;
   cp.state.clear();
+  ro.no_alloc = no_alloc;
   const int64_t B = hb.batch, N = hb.atoms, P = hb.points;
   std::vector<int32_t> state_dim(B);
   {
@@ -491,7 +494,10 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, 
       }
       caches = &context->topo;
     }
-    rc = xtb::run_batch_lanes(lanes_for(context, B), hb, ro, out, &timing, err, caches);
+    // a no_alloc call runs on the context's own stream only (more lanes would
+    // create devices, which allocates)
+    rc = xtb::run_batch_lanes(no_alloc ? std::vector<xtb::Device*>{context->device} : lanes_for(context, B),
+                              hb, ro, out, &timing, err, caches);
   } catch (const std::bad_alloc&) {
     return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "host allocation failed");
   } catch (...) {
@@ -651,6 +657,12 @@ xtbloom_status_t xtbloom_amd_compute_extended(xtbloom_context_t* context, const 
                                               xtbloom_batch_result_t* result,
                                               xtbloom_amd_extension_t* extension) {
   return compute_impl(context, nullptr, batch, options, result, extension);
+}
+
+xtbloom_status_t xtbloom_amd_compute_no_alloc(xtbloom_context_t* context, const xtbloom_batch_t* batch,
+                                              const xtbloom_compute_options_t* options,
+                                              xtbloom_batch_result_t* result) {
+  return compute_impl(context, nullptr, batch, options, result, nullptr, nullptr, true);
 }
 
 int64_t xtbloom_amd_packed_matrix_elements(const xtbloom_batch_t* batch) {

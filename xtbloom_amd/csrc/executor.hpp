@@ -41,7 +41,15 @@ struct RunOptions {
   const double* warm_state = nullptr;
   // size the device workspace for the batch and return before any kernel
   bool reserve_only = false;
+  // allocate nothing: the device workspace is the fixed per-GPU array of
+  // kNoAllocWorkspaceBytes (a larger batch is rejected before any kernel),
+  // host staging is pinned only when the context's pinned block is already
+  // large enough, one lane, no per-launch events
+  bool no_alloc = false;
 };
+
+// Size of the per-GPU device workspace array behind RunOptions::no_alloc.
+constexpr size_t kNoAllocWorkspaceBytes = size_t(64) << 20;
 
 // Per-kernel-family timing record (layout of xtbloom_amd_kernel_stat_t).
 struct KernelStat {

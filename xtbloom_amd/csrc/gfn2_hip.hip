@@ -13,7 +13,9 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "../data/gfn2_tables.h"
@@ -303,6 +305,33 @@ bool topo_cache_retag(TopoCache* cache, int64_t tag) {
   return cache->valid;
 }
 
+// Device workspace of RunOptions::no_alloc calls: one copy of this array per
+// GPU, held from module load until the process ends.  Host code never passes
+// `scratch` itself to a kernel or a copy (in host code the name is only a
+// stand-in for the symbol): each device's address comes from
+// hipGetSymbolAddress, resolved once per device with that device current,
+// and every no_alloc call on a device uses that address.  Calls on one GPU
+// share the array, so no_alloc calls that overlap on one GPU (two contexts,
+// two threads) race; each call rewrites every byte it reads before reading it.
+static __device__ __align__(256) char scratch[kNoAllocWorkspaceBytes];
+
+std::mutex g_scratch_lock;
+std::vector<char*> g_scratch_addr;  // device address of `scratch` per device id
+
+// With device `id` current: the device address of `scratch` on that device.
+hipError_t scratch_address(int id, char** out) {
+  std::lock_guard<std::mutex> guard(g_scratch_lock);
+  if ((int)g_scratch_addr.size() <= id) g_scratch_addr.resize(id + 1, nullptr);
+  if (g_scratch_addr[id] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[id] = static_cast<char*>(p);
+  }
+  *out = g_scratch_addr[id];
+  return hipSuccess;
+}
+
 struct Device {
   int id = 0;
   hipStream_t stream = nullptr;
@@ -316,6 +345,8 @@ struct Device {
   // grow-only arena
   char* arena = nullptr;
   size_t arena_bytes = 0;
+  // this device's address of the no_alloc workspace array
+  char* scratch = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t round_ev[2] = {nullptr, nullptr};  // end of the SCC rounds of each parity
   std::vector<hipEvent_t> probe_pool;  // per-launch timing events, grow-only
@@ -329,6 +360,8 @@ struct Device {
   uint64_t resident_gen = 0;
   size_t resident_bytes = 0;
 };
+
+constexpr size_t kPinnedInitialBytes = 4096;
 
 Device* device_create(int device_id, void* stream, std::string& error) {
   int count = 0;
@@ -367,6 +400,15 @@ Device* device_create(int device_id, void* stream, std::string& error) {
        hipMemcpy(d->d4_refq, xtbp_d4_ref_q, sizeof(double) * 262, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(d->d4_refng, xtbp_d4_ref_ngauss, sizeof(int) * 262, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(d->kpair, xtbp_kpair, sizeof(double) * 9, hipMemcpyHostToDevice) == hipSuccess;
+  // the per-round active counts live in pinned memory from the start, so that
+  // a no_alloc call never needs to allocate it
+  ok = ok && hipHostMalloc((void**)&d->pinned, kPinnedInitialBytes) == hipSuccess;
+  if (ok) d->pinned_bytes = kPinnedInitialBytes;
+  if (ok && scratch_address(device_id, &d->scratch) != hipSuccess) {
+    error = "hipGetSymbolAddress failed for the no-allocation device workspace";
+    device_destroy(d);
+    return nullptr;
+  }
   for (int i = 0; i < 4 && ok; ++i) ok = hipEventCreate(&d->ev[i]) == hipSuccess;
   for (int i = 0; i < 2 && ok; ++i)
     ok = hipEventCreateWithFlags(&d->round_ev[i], hipEventDisableTiming) == hipSuccess;
@@ -607,7 +649,14 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addi(N);                     // element-grouped atom order
   addd((size_t)kPairStageDoubles * N); addi((size_t)kPairStageInts * N);  // pair staging (large systems)
   addd((size_t)mem * mem * B);  // Broyden overlaps
-  if (bytes > dev->arena_bytes) {
+  if (opt.no_alloc) {
+    if (bytes > kNoAllocWorkspaceBytes) {
+      error = "the batch needs " + std::to_string(bytes) +
+              " bytes of device workspace; the no-allocation entry point holds at most " +
+              std::to_string(kNoAllocWorkspaceBytes) + " bytes";
+      return 1;
+    }
+  } else if (bytes > dev->arena_bytes) {
     (void)hipFree(dev->arena);
     dev->arena = nullptr;
     dev->arena_bytes = 0;
@@ -621,7 +670,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   }
   if (timing) timing->workspace_bytes = bytes;
   if (opt.reserve_only) return 0;
-  Carver cv{dev->arena};
+  Carver cv{opt.no_alloc ? dev->scratch : dev->arena};
   hipStream_t sm = dev->stream;
   // uploads are collected here and issued together from pinned staging right
   // before the first kernel (the source vectors live until the call returns)
@@ -637,7 +686,9 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   };
   // Topology arrays stay in the arena between calls: a call with the same
   // cached topology and the same arena layout uploads only the geometry.
-  const bool resident = cache_hit && cache->generation == dev->resident_gen && bytes == dev->resident_bytes;
+  // (the no_alloc workspace is another memory region: it never holds them)
+  const bool resident = !opt.no_alloc && cache_hit && cache->generation == dev->resident_gen &&
+                        bytes == dev->resident_bytes;
   auto up_topo = [&](auto* dst, const auto& vec) -> hipError_t {
     return resident ? hipSuccess : up(dst, vec);
   };
@@ -843,7 +894,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   HIP_TRY(hipMemsetAsync(mx.omega, 0, sizeof(double) * (size_t)mem * B, sm));
 
   // ---- optional per-launch timing (HIP events on this stream) ----
-  KernelStat* ks = out.kernel_stats;
+  KernelStat* ks = opt.no_alloc ? nullptr : out.kernel_stats;
   struct ProbeRec {
     int kind;
     double flops, bytes;
@@ -898,24 +949,34 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     for (const Upload& u : uploads) need += align_bytes<char>(u.bytes);
     // copies of an earlier call that failed midway may still read the staging
     HIP_TRY(hipStreamSynchronize(sm));
-    if (need > dev->pinned_bytes) {
+    if (need > dev->pinned_bytes && !opt.no_alloc) {
       if (dev->pinned) (void)hipHostFree(dev->pinned);
       dev->pinned = nullptr;
       dev->pinned_bytes = 0;
       HIP_TRY(hipHostMalloc((void**)&dev->pinned, need + need / 4));
       dev->pinned_bytes = need + need / 4;
     }
-    size_t off = 256 + 2 * align_bytes<unsigned char>(B) + 2 * align_bytes<int>(B);
-    for (const Upload& u : uploads) {
-      std::memcpy(dev->pinned + off, u.src, u.bytes);
-      HIP_TRY(hipMemcpyAsync(u.dst, dev->pinned + off, u.bytes, hipMemcpyHostToDevice, sm));
-      off += align_bytes<char>(u.bytes);
+    if (need <= dev->pinned_bytes) {
+      size_t off = 256 + 2 * align_bytes<unsigned char>(B) + 2 * align_bytes<int>(B);
+      for (const Upload& u : uploads) {
+        std::memcpy(dev->pinned + off, u.src, u.bytes);
+        HIP_TRY(hipMemcpyAsync(u.dst, dev->pinned + off, u.bytes, hipMemcpyHostToDevice, sm));
+        off += align_bytes<char>(u.bytes);
+      }
+    } else {
+      // no_alloc with a smaller pinned block: copies from pageable memory
+      // (the runtime stages them before the call returns); the pinned block
+      // still holds the per-round counts (256 bytes, allocated with the device)
+      for (const Upload& u : uploads)
+        HIP_TRY(hipMemcpyAsync(u.dst, u.src, u.bytes, hipMemcpyHostToDevice, sm));
     }
   }
   // the arena now holds this call's topology (stream order: every later call
   // on this device reads it after these copies)
-  dev->resident_gen = cache ? cache->generation : 0;
-  dev->resident_bytes = bytes;
+  if (!opt.no_alloc) {
+    dev->resident_gen = cache ? cache->generation : 0;
+    dev->resident_bytes = bytes;
+  }
   // pinned per-parity round results: active counts, and (with kernel stats)
   // snapshots of the active mask and kocc at the end of each round
   int* h_count = reinterpret_cast<int*>(dev->pinned);

@@ -229,7 +229,8 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                    scc_start_mode=library.SCC_START_FRESH, plan: "Plan | None" = None,
                    request: "Request | None" = None,
                    extension=False, diagnostics=False, kernel_stats=False,
-                   determinism=library.DETERMINISM_DEFAULT, options_hook=None) -> ComputedBatch:
+                   determinism=library.DETERMINISM_DEFAULT, options_hook=None,
+                   no_alloc=False) -> ComputedBatch:
     """One ``xtbloom_compute`` over flat ragged arrays (the ``xtbloom_batch_t`` layout).
 
     ``scc_start_mode=library.SCC_START_WARM`` starts from the converged state of
@@ -239,6 +240,8 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
     ``determinism=library.DETERMINISM_REPRODUCIBLE`` asks for bit-identical
     results on repeated calls.  ``options_hook`` (optional) receives the
     compute-options struct right before the call (tests of the validation).
+    ``no_alloc=True`` goes through ``xtbloom_amd_compute_no_alloc`` (no device
+    allocation; the batch must fit its fixed workspace).
     """
     lib = library.load()
     batch, opts, keep, (B, N, P) = _descriptors(
@@ -269,7 +272,13 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
     res.per_system_status = _out(status)
     ext_out = None
     dedq = None
-    if request is not None:
+    if no_alloc and (request is not None or plan is not None or extension or diagnostics
+                     or kernel_stats):
+        raise XTBloomValueError("no-allocation calls take no plan, request or extension outputs")
+    if no_alloc:
+        _check(lib.xtbloom_amd_compute_no_alloc(context.handle, ctypes.byref(batch),
+                                                ctypes.byref(opts), ctypes.byref(res)))
+    elif request is not None:
         # xtbloom_compute_enqueue / xtbloom_plan_compute_enqueue; the request
         # reports the completion status (admission errors raise here)
         if extension or diagnostics or kernel_stats:

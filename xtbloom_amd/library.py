@@ -26,6 +26,8 @@ STATUS_INTERNAL_ERROR = 6
 STATUS_SCC_NOT_CONVERGED = 7
 STATUS_EIGENSOLVER_FAILED = 8
 
+NO_ALLOC_WORKSPACE_BYTES = 64 << 20  # XTBLOOM_AMD_NO_ALLOC_WORKSPACE_BYTES
+
 BACKEND_AUTO = 0
 BACKEND_CPU = 1
 BACKEND_CUDA = 2
@@ -171,6 +173,7 @@ EXPORTED_SYMBOLS = (
     "xtbloom_result_owner_create", "xtbloom_result_owner_buffer", "xtbloom_result_owner_retain",
     "xtbloom_result_owner_release", "xtbloom_result_owner_export_dltensor",
     "xtbloom_amd_compute_extended", "xtbloom_amd_packed_matrix_elements",
+    "xtbloom_amd_compute_no_alloc",
 )
 
 _LIB = None
@@ -218,6 +221,9 @@ def load() -> ctypes.CDLL:
                                                  ctypes.POINTER(BatchResult),
                                                  ctypes.POINTER(Extension)]
     lib.xtbloom_amd_compute_extended.restype = ctypes.c_int32
+    lib.xtbloom_amd_compute_no_alloc.argtypes = [vp, ctypes.POINTER(Batch), ctypes.POINTER(ComputeOptions),
+                                                 ctypes.POINTER(BatchResult)]
+    lib.xtbloom_amd_compute_no_alloc.restype = ctypes.c_int32
     lib.xtbloom_amd_packed_matrix_elements.argtypes = [ctypes.POINTER(Batch)]
     lib.xtbloom_amd_packed_matrix_elements.restype = ctypes.c_int64
     lib.xtbloom_workspace_query_init.argtypes = [ctypes.POINTER(WorkspaceQuery), ctypes.c_size_t]
