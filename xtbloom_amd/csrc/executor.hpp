@@ -48,6 +48,10 @@ struct RunOptions {
   bool no_alloc = false;
 };
 
+// Largest AO count of one system (the eigensolver's LDS arrays); larger
+// systems of a batch get per-system status NOT_SUPPORTED.
+constexpr int kMaxSystemAOs = 1024;
+
 // Size of the per-GPU device workspace array behind RunOptions::no_alloc.
 constexpr size_t kNoAllocWorkspaceBytes = size_t(64) << 20;
 
@@ -126,6 +130,8 @@ int64_t batch_matrix_elements(const HostBatch& batch, int32_t* max_ao, std::stri
 // status code like run_batch; warm_state and scc_state are cut per slice.
 // caches (optional) holds one topology cache per lane; a cache is reused only
 // for the same slice of the same lane count.
+// Systems above kMaxSystemAOs are left out of the slices and reported with
+// per-system status NOT_SUPPORTED and NaN slices.
 int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& batch,
                     const RunOptions& options, HostOutputs& out, RunTiming* timing,
                     std::string& error, const std::vector<TopoCache*>* caches = nullptr);

@@ -26,6 +26,8 @@
 #include "gfn2_mix_force.hpp"
 #include "gfn2_scc.hpp"
 
+static_assert(xtb::kMaxSystemAOs == xtb::kMaxAO, "AO limit of the executor and the eigensolver");
+
 namespace xtb {
 
 namespace {
@@ -347,6 +349,7 @@ struct Device {
   size_t arena_bytes = 0;
   // this device's address of the no_alloc workspace array
   char* scratch = nullptr;
+
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t round_ev[2] = {nullptr, nullptr};  // end of the SCC rounds of each parity
   std::vector<hipEvent_t> probe_pool;  // per-launch timing events, grow-only
@@ -390,6 +393,7 @@ Device* device_create(int device_id, void* stream, std::string& error) {
     }
     d->own_stream = true;
   }
+
   bool ok = hipMalloc(&d->d4_c6, sizeof(double) * 262 * 262) == hipSuccess &&
             hipMalloc(&d->d4_refcn, sizeof(double) * 262) == hipSuccess &&
             hipMalloc(&d->d4_refq, sizeof(double) * 262) == hipSuccess &&
@@ -448,6 +452,7 @@ void device_destroy(Device* d) {
 }
 
 int device_id(const Device* d) { return d ? d->id : -1; }
+
 
 int64_t batch_matrix_elements(const HostBatch& hb, int32_t* max_ao, std::string& error,
                               int32_t* system_aos, int32_t* system_state) {
@@ -1015,6 +1020,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   }
   HIP_TRY(probe_begin(kKCholesky));
   hipLaunchKernelGGL(k_cholesky_inverse, dim3(B), dim3(512), 0, sm, t, d_order, dS, dA, dX, d_st);
+  hipLaunchKernelGGL(k_overlap_rcond, dim3(B), dim3(256), 0, sm, t, d_order, dS, dX, d_st);
   HIP_TRY(probe_end(kKCholesky, all3 / 3.0, 0.0));
   hipLaunchKernelGGL(k_init_active, dim3((B + 255) / 256), dim3(256), 0, sm, B, d_st, d_active,
                      (int*)nullptr, d_st);

@@ -149,26 +149,9 @@ __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, con
 }
 
 // Cholesky S = L L^T and X = L^{-T} with everything in global memory (any n);
-// returns 1/(|S|_1 |X|_1 |X|_inf) through `bound`, or sets *bad on a
-// non-positive pivot.  `mx` holds blockDim.x doubles.
-__device__ void cholesky_global(const double* Sm, double* L, double* Xm, int n, double* mx, int& bad,
-                                double& bound) {
+// sets *bad on a non-positive pivot.
+__device__ void cholesky_global(const double* Sm, double* L, double* Xm, int n, int& bad) {
   for (int64_t idx = threadIdx.x; idx < (int64_t)n * n; idx += blockDim.x) L[idx] = Sm[idx];
-  __syncthreads();
-  // |S|_1
-  double colmax = 0.0;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    double sacc = 0.0;
-    for (int i = 0; i < n; ++i) sacc += fabs(Sm[(int64_t)i * n + j]);
-    colmax = fmax(colmax, sacc);
-  }
-  mx[threadIdx.x] = colmax;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + w]);
-    __syncthreads();
-  }
-  const double snorm = mx[0];
   __syncthreads();
   // right-looking column Cholesky
   for (int k = 0; k < n; ++k) {
@@ -203,34 +186,6 @@ __device__ void cholesky_global(const double* Sm, double* L, double* Xm, int n, 
       xr[i] = v / L[i + (int64_t)i * n];
     }
   }
-  __syncthreads();
-  // conditioning bound: |S^-1|_1 <= |Linv^T|_1 |Linv|_1 = (max row-sum of X^T...) use both norms of X
-  double c1 = 0.0, ci = 0.0;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    double rs = 0.0, cs = 0.0;
-    for (int i = 0; i < n; ++i) {
-      rs += fabs(Xm[(int64_t)j * n + i]);
-      cs += fabs(Xm[(int64_t)i * n + j]);
-    }
-    c1 = fmax(c1, rs);
-    ci = fmax(ci, cs);
-  }
-  mx[threadIdx.x] = c1;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + w]);
-    __syncthreads();
-  }
-  const double n1 = mx[0];
-  __syncthreads();
-  mx[threadIdx.x] = ci;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + w]);
-    __syncthreads();
-  }
-  const double n2 = mx[0];
-  bound = 1.0 / (snorm * n1 * n2);
   __syncthreads();
 }
 
@@ -698,13 +653,10 @@ __global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, doubl
 //   factorization  per column k: the owners broadcast column k through LDS, the
 //                  trailing lower triangle takes the rank-1 update l l^T / l_kk;
 //   inverse        Gauss-Jordan on the identity: row k of L^{-1} is scaled by
-//                  1/L_kk, then rows i > k lose L_ik times it;
-//   bound          1/(|S|_1 |X|_1 |X|_inf) from LDS-transposed row/column sums.
+//                  1/L_kk, then rows i > k lose L_ik times it.
 template <int NR>
 __device__ __forceinline__ void cholesky_registers(const double* Sm, double* L, double* Xm, int n,
-                                                   double* lcol, double* xr, double* rowpart,
-                                                   double* colpart, double* red, int& bad,
-                                                   double& bound) {
+                                                   double* lcol, double* xr, int& bad) {
   constexpr int NE = tri_slots(NR);
   const int tid = threadIdx.x, lr = tid >> 5, lc = tid & 31;
   double a[NE];
@@ -715,13 +667,6 @@ __device__ __forceinline__ void cholesky_registers(const double* Sm, double* L, 
     a[sl] = (i < n && j < n && i >= j) ? Sm[(int64_t)i * n + j] : 0.0;
     if constexpr (TJ == TI / 2) __builtin_amdgcn_sched_barrier(0);
   });
-  // |S|_1 (columns, coalesced)
-  double colmax = 0.0;
-  for (int j = tid; j < n; j += blockDim.x) {
-    double acc = 0.0;
-    for (int i = 0; i < n; ++i) acc += fabs(Sm[(int64_t)i * n + j]);
-    colmax = fmax(colmax, acc);
-  }
   // lcol is zero outside [k, n): the rank-1 update then needs no masks (column k
   // itself and the strict upper part of diagonal tiles take harmless updates
   // that are never read again)
@@ -808,56 +753,19 @@ __device__ __forceinline__ void cholesky_registers(const double* Sm, double* L, 
     const int r = (int)(idx / n), c = (int)(idx - (int64_t)r * n);
     if (c < r) Xm[idx] = 0.0;
   }
-  double pr = 0.0;
-  double pc[NR / 2];
-#pragma unroll
-  for (int TJ = 0; TJ < NR / 2; ++TJ) pc[TJ] = 0.0;
   static_for<0, NE>([&](auto S) __attribute__((always_inline)) {
     constexpr int sl = decltype(S)::value;
     constexpr int TI = tri_row(sl), TJ = tri_col(sl);
     const int i = 16 * TI + lr, j = 32 * TJ + lc;
-    const double x = (i >= j && i < n) ? a[sl] : 0.0;
-    if (i >= j && i < n) Xm[(int64_t)j * n + i] = x;
-    pr += fabs(x);
-    pc[TJ] += fabs(x);
-    if constexpr (TJ == TI / 2) {
-      rowpart[lc * kTriPad + 16 * TI + lr] = pr;
-      pr = 0.0;
-      __builtin_amdgcn_sched_barrier(0);  // one row tile of stores in flight
-    }
+    if (i >= j && i < n) Xm[(int64_t)j * n + i] = a[sl];
+    if constexpr (TJ == TI / 2) __builtin_amdgcn_sched_barrier(0);  // one row tile of stores in flight
   });
-#pragma unroll
-  for (int TJ = 0; TJ < NR / 2; ++TJ) colpart[lr * kTriPad + 32 * TJ + lc] = pc[TJ];
-  // rows of Linv never reach tiles outside the stored set; fill the gaps with zeros
-  for (int idx = tid; idx < 32 * kTriPad; idx += blockDim.x) {
-    const int row = idx % kTriPad;
-    if (row >= 16 * NR && row < kTriPad) rowpart[idx] = 0.0;
-  }
-  __syncthreads();
-  double rs = 0.0, cs = 0.0;
-  if (tid < n) {
-    for (int r = 0; r < 32; ++r) rs += rowpart[r * kTriPad + tid];
-    for (int r = 0; r < 16; ++r) cs += colpart[r * kTriPad + tid];
-  }
-  // block maxima: |S|_1, |X|_1 = max row sum of Linv, |X|_inf = max column sum of Linv
-  double m[3] = {colmax, rs, cs};
-  for (int q = 0; q < 3; ++q) {
-    double v = m[q];
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    __syncthreads();
-    if ((tid & 63) == 0) red[tid >> 6] = v;
-    __syncthreads();
-    v = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, red[w]);
-    m[q] = v;
-  }
-  bound = 1.0 / (m[0] * m[1] * m[2]);
   __syncthreads();
 }
 
 // Per-system overlap factorization, 512 threads: X = L^{-T} with L in `work`
-// (column-major).  status = 8 on a non-positive pivot or when the conditioning
-// bound 1/(|S|_1 |X|_1 |X|_inf) < 1e-12 (the bound never exceeds rcond_1).
+// (column-major).  status = 8 on a non-positive pivot; the conditioning gate
+// follows in k_overlap_rcond.
 __global__ __launch_bounds__(512) void k_cholesky_inverse(Topo t, const int* order, const double* S,
                                                           double* work, double* X, SysState* status) {
   const int s = order[blockIdx.x];
@@ -868,19 +776,169 @@ __global__ __launch_bounds__(512) void k_cholesky_inverse(Topo t, const int* ord
   double* Xm = X + mo;
   __shared__ double lcol[kMaxAO];
   __shared__ double xr[kMaxAO];
-  __shared__ double rowpart[32 * kTriPad];
-  __shared__ double colpart[16 * kTriPad];
-  __shared__ double red[16];
   __shared__ int bad;
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
-  double bound = 0.0;
-  if (n <= 64) cholesky_registers<4>(Sm, L, Xm, n, lcol, xr, rowpart, colpart, red, bad, bound);
-  else if (n <= 128) cholesky_registers<8>(Sm, L, Xm, n, lcol, xr, rowpart, colpart, red, bad, bound);
-  else if (n <= 192) cholesky_registers<12>(Sm, L, Xm, n, lcol, xr, rowpart, colpart, red, bad, bound);
-  else if (n <= 256) cholesky_registers<16>(Sm, L, Xm, n, lcol, xr, rowpart, colpart, red, bad, bound);
-  else cholesky_global(Sm, L, Xm, n, rowpart, bad, bound);
-  if (threadIdx.x == 0 && (bad || !(bound >= 1e-12) || !isfinite(bound))) status[s].status = 8;
+  if (n <= 64) cholesky_registers<4>(Sm, L, Xm, n, lcol, xr, bad);
+  else if (n <= 128) cholesky_registers<8>(Sm, L, Xm, n, lcol, xr, bad);
+  else if (n <= 192) cholesky_registers<12>(Sm, L, Xm, n, lcol, xr, bad);
+  else if (n <= 256) cholesky_registers<16>(Sm, L, Xm, n, lcol, xr, bad);
+  else cholesky_global(Sm, L, Xm, n, bad);
+  if (threadIdx.x == 0 && bad) status[s].status = 8;
+}
+
+// ---------------------------------------------------------------------------
+// Overlap conditioning gate of the reference (eigensolver.cpp:2388-2402):
+// rcond_1 = 1 / (|S|_1 est(|S^{-1}|_1)) >= 1e-12, where est is LAPACK's
+// Hager-Higham 1-norm estimate (dlacn2, the iteration dpocon runs; Higham,
+// ACM TOMS 14 (1988) 381-396, Algorithm 4.1): at most five power steps
+// x -> sign(S^{-1} x) -> S^{-1} sign(.) -> unit vector at the largest entry,
+// stopped on a repeated sign vector, no increase, or a repeated index, then
+// the alternating-sign test vector x_i = (-1)^i (1 + i/(n-1)) with estimate
+// 2 |S^{-1} x|_1 / (3n).  Here S^{-1} v = X (X^T v) with X = L^{-T} (row-major,
+// upper triangular) from k_cholesky_inverse.  One 256-thread block per system
+// whose factorization succeeded.
+// ---------------------------------------------------------------------------
+constexpr double kMinOverlapRcond = 1e-12;
+
+struct RcondShared {
+  double x[kMaxAO];
+  double y[kMaxAO];
+  signed char sgn[kMaxAO];
+  double red[8];
+  int ired[8];
+};
+
+// block sum of |v_i|
+__device__ double rc_asum(const double* v, int n, RcondShared& sh) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a += fabs(v[i]);
+  a = wave_sum(a);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh.red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh.red[w];
+  __syncthreads();
+  return t;
+}
+
+// first index of the largest |v_i| (LAPACK idamax)
+__device__ int rc_iamax(const double* v, int n, RcondShared& sh) {
+  double best = -1.0;
+  int bi = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double a = fabs(v[i]);
+    if (a > best) { best = a; bi = i; }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ob = __shfl_xor(best, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) { sh.red[threadIdx.x >> 6] = best; sh.ired[threadIdx.x >> 6] = bi; }
+  __syncthreads();
+  best = sh.red[0];
+  bi = sh.ired[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+    if (sh.red[w] > best || (sh.red[w] == best && sh.ired[w] < bi)) { best = sh.red[w]; bi = sh.ired[w]; }
+  __syncthreads();
+  return bi;
+}
+
+// sh.x <- S^{-1} sh.x  (y = X^T x, then x = X y); X row-major upper triangular
+__device__ void rc_apply(const double* __restrict__ Xm, int n, RcondShared& sh) {
+  __syncthreads();
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    double acc = 0.0;
+    for (int r = 0; r <= c; ++r) acc = fma(Xm[(int64_t)r * n + c], sh.x[r], acc);
+    sh.y[c] = acc;
+  }
+  __syncthreads();
+  // one wavefront per row, lanes along the row
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int r = wv; r < n; r += nw) {
+    double acc = 0.0;
+    for (int c = r + lane; c < n; c += 64) acc = fma(Xm[(int64_t)r * n + c], sh.y[c], acc);
+    acc = wave_sum(acc);
+    if (lane == 0) sh.x[r] = acc;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_overlap_rcond(Topo t, const int* order, const double* S,
+                                                       const double* X, SysState* status) {
+  const int s = order[blockIdx.x];
+  if (status[s].status != 0) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t mo = t.mat_off[s];
+  const double* Sm = S + mo;
+  const double* Xm = X + mo;
+  __shared__ RcondShared sh;
+  // |S|_1 (max column sum; S is symmetric)
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) acc += fabs(Sm[(int64_t)i * n + j]);
+    sh.y[j] = acc;
+  }
+  __syncthreads();
+  double anorm = 0.0;
+  {
+    double m = 0.0;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) m = fmax(m, sh.y[j]);
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh.red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) anorm = fmax(anorm, sh.red[w]);
+    __syncthreads();
+  }
+  // estimate of |S^{-1}|_1
+  for (int i = threadIdx.x; i < n; i += blockDim.x) sh.x[i] = 1.0 / n;
+  rc_apply(Xm, n, sh);
+  double est;
+  if (n == 1) {
+    est = fabs(sh.x[0]);
+  } else {
+    est = rc_asum(sh.x, n, sh);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const signed char g = sh.x[i] >= 0.0 ? 1 : -1;
+      sh.sgn[i] = g;
+      sh.x[i] = g;
+    }
+    rc_apply(Xm, n, sh);
+    int j = rc_iamax(sh.x, n, sh);
+    for (int iter = 2;; ++iter) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) sh.x[i] = i == j ? 1.0 : 0.0;
+      rc_apply(Xm, n, sh);
+      const double estold = est;
+      est = rc_asum(sh.x, n, sh);
+      __shared__ int differs;
+      if (threadIdx.x == 0) differs = 0;
+      __syncthreads();
+      for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if ((sh.x[i] >= 0.0 ? 1 : -1) != sh.sgn[i]) differs = 1;
+      __syncthreads();
+      if (!differs || est <= estold) break;  // repeated sign vector, or no increase
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const signed char g = sh.x[i] >= 0.0 ? 1 : -1;
+        sh.sgn[i] = g;
+        sh.x[i] = g;
+      }
+      rc_apply(Xm, n, sh);
+      const int jlast = j;
+      j = rc_iamax(sh.x, n, sh);
+      if (!(sh.x[jlast] != fabs(sh.x[j]) && iter < 5)) break;
+    }
+    // alternating-sign test vector
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+      sh.x[i] = (i & 1 ? -1.0 : 1.0) * (1.0 + (double)i / (double)(n - 1));
+    rc_apply(Xm, n, sh);
+    const double temp = 2.0 * (rc_asum(sh.x, n, sh) / (3.0 * n));
+    if (temp > est) est = temp;
+  }
+  const double rcond = (est != 0.0 && anorm != 0.0) ? (1.0 / est) / anorm : 0.0;
+  if (threadIdx.x == 0 && !(rcond >= kMinOverlapRcond && isfinite(rcond))) status[s].status = 8;
 }
 
 // x / y via the hardware reciprocal refined by one Newton step (a few ulp,
