@@ -60,8 +60,8 @@ WORKLOAD_TEXT = {
 }
 
 
-def cpu_baseline(batch, gpu_out, sample_stride, opts=None):
-    """FP64 C restatement on a strided sample; returns (baseline dict, parity dict)."""
+def cpu_baseline(batch, gpu_outs, sample_stride, opts=None):
+    """FP64 C restatement on a strided sample; returns (baseline dict, {path: parity dict})."""
     import numpy as np
 
     from oracle import oracle
@@ -74,26 +74,28 @@ def cpu_baseline(batch, gpu_out, sample_stride, opts=None):
     ref = oracle.batch(sub.atom_offsets, sub.atomic_numbers, sub.positions,
                        sub.molecular_charges, sub.unpaired_electrons, opts=opts, threads=threads)
     wall = time.perf_counter() - t0
-    e = gpu_out["energies"].cpu().numpy()
-    f = gpu_out["forces"].cpu().numpy()
-    q = gpu_out["charges"].cpu().numpy()
-    it = gpu_out["scc_iterations"].cpu().numpy()
-    de = df = dq = 0.0
-    dit = 0
-    for k, i in enumerate(idx):
-        a0, a1 = batch.atom_offsets[i], batch.atom_offsets[i + 1]
-        b0, b1 = sub.atom_offsets[k], sub.atom_offsets[k + 1]
-        de = max(de, abs(e[i] - ref["energies"][k]))
-        df = max(df, float(np.abs(f[a0:a1] + ref["gradient"][b0:b1]).max()))
-        dq = max(dq, float(np.abs(q[a0:a1] - ref["charges"][b0:b1]).max()))
-        dit = max(dit, abs(int(it[i]) - int(ref["iterations"][k])))
+    parity = {}
+    for name, gpu_out in gpu_outs.items():
+        e = gpu_out["energies"].cpu().numpy()
+        f = gpu_out["forces"].cpu().numpy()
+        q = gpu_out["charges"].cpu().numpy()
+        it = gpu_out["scc_iterations"].cpu().numpy()
+        de = df = dq = 0.0
+        dit = 0
+        for k, i in enumerate(idx):
+            a0, a1 = batch.atom_offsets[i], batch.atom_offsets[i + 1]
+            b0, b1 = sub.atom_offsets[k], sub.atom_offsets[k + 1]
+            de = max(de, abs(e[i] - ref["energies"][k]))
+            df = max(df, float(np.abs(f[a0:a1] + ref["gradient"][b0:b1]).max()))
+            dq = max(dq, float(np.abs(q[a0:a1] - ref["charges"][b0:b1]).max()))
+            dit = max(dit, abs(int(it[i]) - int(ref["iterations"][k])))
+        parity[name] = {"systems": len(idx), "max_abs_dE": de, "max_abs_dF": df,
+                        "max_abs_dq": dq, "max_abs_diter": dit}
     atoms = int(sub.atom_offsets[-1])
     base = {"value": len(idx) / wall, "unit": "systems/s", "cores": threads, "kind": "port",
             "sample": f"{len(idx)} of {batch.batch_size} systems (stride {sample_stride}, "
                       f"{atoms} atoms), FP64 C restatement, one system per thread, "
                       f"{wall:.2f} s wall"}
-    parity = {"systems": len(idx), "max_abs_dE": de, "max_abs_dF": df, "max_abs_dq": dq,
-              "max_abs_diter": dit}
     return base, parity
 
 
@@ -119,6 +121,9 @@ def main():
     ap.add_argument("--warm-step", type=float, default=0.0,
                     help="with --scc-start warm: Gaussian displacement (bohr) of every atom "
                          "before each timed call (0: the same geometry)")
+    ap.add_argument("--path", choices=("both", "plain", "batched"), default="both",
+                    help="execution paths timed: the headline value is the batched path, "
+                         "vs_baseline its ratio to the plain FP64 path")
     ap.add_argument("--plan", action="store_true",
                     help="timed calls through a fixed-topology plan (no per-launch events: "
                          "the roofline entry then comes from the warmup calls)")
@@ -152,24 +157,6 @@ def main():
             for k in ("launches", "ms", "flops", "bytes"):
                 agg[name][k] += st[k]
 
-    # warmup: events around every launch, for the per-family breakdown
-    wagg = new_agg()
-    for _ in range(max(args.warmup, 1)):
-        out, timing = interface.compute_tensors(ctx, **inp, flags=flags, outputs=out,
-                                                kernel_stats=True, **tol)
-        add(wagg, timing["kernel_stats"])
-    torch.cuda.synchronize()
-    dom = max(wagg, key=lambda k: wagg[k]["ms"])
-    agg = new_agg()
-    rounds = 0
-    device_ms = 0.0
-    plan = None
-    if args.plan:
-        plan = interface.Plan(ctx, batch.atom_offsets, batch.atomic_numbers, batch.positions,
-                              batch.molecular_charges, batch.unpaired_electrons, flags=flags,
-                              **tol)
-        # the plan's own checkpoint for WARM calls
-        out, _ = interface.compute_tensors(ctx, **inp, flags=flags, outputs=out, plan=plan, **tol)
     start = library.SCC_START_WARM if args.scc_start == "warm" else library.SCC_START_FRESH
     geoms = [inp["positions"]] * args.steps
     if args.scc_start == "warm" and args.warm_step > 0.0:
@@ -179,30 +166,78 @@ def main():
             cur = cur + args.warm_step * torch.randn(cur.shape, generator=gen, device=dev,
                                                      dtype=cur.dtype)
             geoms.append(cur)
-    # the timed loop sums iteration counts on the device: load torch's reduction
-    # kernel outside the timed region
-    iter_sum = 0 * int(out["scc_iterations"].sum())
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step_inp = dict(inp, positions=geoms[k])
-        out, timing = interface.compute_tensors(ctx, **step_inp, flags=flags, outputs=out,
-                                                kernel_stats=None if plan else [dom],
-                                                scc_start_mode=start, plan=plan, **tol)
-        if not plan:
-            add(agg, timing["kernel_stats"])
-        rounds = max(rounds, timing["scc_rounds"])
-        device_ms += timing["total_ms"]
-        if args.scc_start == "warm":
-            iter_sum += int(out["scc_iterations"].sum())
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = 1000.0 * wall / args.steps
+
+    def run_path(path, stats, timed_stats):
+        """Warmup and timed calls of one execution path (XTBLOOM_AMD_PATH)."""
+        os.environ["XTBLOOM_AMD_PATH"] = path
+        out = None
+        # warmup: with stats, events around every launch (the per-family breakdown)
+        wagg = new_agg()
+        for _ in range(max(args.warmup, 1)):
+            out, timing = interface.compute_tensors(ctx, **inp, flags=flags, outputs=out,
+                                                    kernel_stats=True if stats else None, **tol)
+            if stats:
+                add(wagg, timing["kernel_stats"])
+        torch.cuda.synchronize()
+        dom = max(wagg, key=lambda k: wagg[k]["ms"]) if stats else None
+        agg = new_agg()
+        rounds = 0
+        device_ms = 0.0
+        plan = None
+        if args.plan:
+            plan = interface.Plan(ctx, batch.atom_offsets, batch.atomic_numbers, batch.positions,
+                                  batch.molecular_charges, batch.unpaired_electrons, flags=flags,
+                                  **tol)
+            # the plan's own checkpoint for WARM calls
+            out, _ = interface.compute_tensors(ctx, **inp, flags=flags, outputs=out, plan=plan,
+                                               **tol)
+        # the timed loop sums iteration counts on the device: load torch's
+        # reduction kernel outside the timed region
+        iter_sum = 0 * int(out["scc_iterations"].sum())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step_inp = dict(inp, positions=geoms[k])
+            out, timing = interface.compute_tensors(
+                ctx, **step_inp, flags=flags, outputs=out,
+                kernel_stats=[dom] if (timed_stats and not plan) else None,
+                scc_start_mode=start, plan=plan, **tol)
+            if timed_stats and not plan:
+                add(agg, timing["kernel_stats"])
+            rounds = max(rounds, timing["scc_rounds"])
+            device_ms += timing["total_ms"]
+            if args.scc_start == "warm":
+                iter_sum += int(out["scc_iterations"].sum())
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if plan:
+            plan.close()
+        res = dict(out={k: v.clone() for k, v in out.items()}, ms_per_step=1000.0 * wall / args.steps,
+                   wagg=wagg, agg=agg, dom=dom, rounds=rounds, device_ms=device_ms,
+                   iter_sum=iter_sum, plan=bool(plan), timed_stats=timed_stats)
+        res["value"] = batch.batch_size / (res["ms_per_step"] / 1000.0)
+        return res
+
+    # the plain FP64 path (per-launch events on its dominant family give the
+    # roofline entry), then the batched path timed without events
+    runs = {}
+    if args.path in ("both", "plain"):
+        runs["plain"] = run_path("plain", True, True)
+    if args.path in ("both", "batched"):
+        # timed without events (events would split the captured rounds)
+        runs["batched"] = run_path("batched", args.path == "batched", False)
+    head = runs["batched"] if "batched" in runs else runs["plain"]
+    ref_run = runs["plain"] if "plain" in runs else runs["batched"]
+    out = head["out"]
+    ms_per_step = head["ms_per_step"]
+    value = head["value"]
     status = out["per_system_status"].cpu().numpy()
-    value = batch.batch_size / (ms_per_step / 1000.0)
+    wagg, agg, dom, plan = ref_run["wagg"], ref_run["agg"], ref_run["dom"], ref_run["plan"]
+    from_warmup = plan or not ref_run["timed_stats"]
+    rounds, device_ms, iter_sum = head["rounds"], head["device_ms"], head["iter_sum"]
 
     # roofline of the kernel family with the largest device time
-    d = wagg[dom] if plan else agg[dom]
+    d = wagg[dom] if from_warmup else agg[dom]
     avg_ms = d["ms"] / max(d["launches"], 1)
     if d["flops"] > 0.0:
         per_launch = d["flops"] / d["launches"]
@@ -223,10 +258,14 @@ def main():
         if tr.get("kernel") == dom:
             roof["traffic"] = tr["traffic_bytes_per_launch"]
             roof["traffic_source"] = f"profiles/hbm_traffic_{args.workload}_fp64.json (rocprofv3 --pmc)"
-    roof.update(kernel=dom, launches_per_step=d["launches"] / (max(args.warmup, 1) if plan else args.steps),
+    roof.update(kernel=dom, launches_per_step=d["launches"] / (max(args.warmup, 1) if from_warmup else args.steps),
                 avg_launch_ms=avg_ms, algorithmic_per_launch=per_launch)
-    if plan:
-        roof["source"] = "warmup calls (plan calls record no events)"
+    roof["source"] = (("warmup calls (" + ("plan calls" if plan else "graph-captured rounds")
+                       + " record no events)" if from_warmup else
+                       "timed calls, events on this family only") +
+                      (" of the plain path (the batched path runs the same kernels inside "
+                       "captured graphs, without events)" if "batched" in runs and "plain" in runs
+                       else ""))
     nw = max(args.warmup, 1)
     total_kernel_ms = sum(v["ms"] for v in wagg.values())
     kernels = {k: {"ms_per_step": v["ms"] / nw, "launches_per_step": v["launches"] / nw,
@@ -234,9 +273,12 @@ def main():
                for k, v in sorted(wagg.items(), key=lambda kv: -kv[1]["ms"]) if v["launches"]}
 
     cpu = parity = None
+    path_parity = {}
     if not args.no_cpu and args.scc_start == "fresh":
         from oracle import oracle as _o
-        cpu, parity = cpu_baseline(batch, out, args.cpu_stride, _o.options(**tol))
+        cpu, path_parity = cpu_baseline(batch, {name: r["out"] for name, r in runs.items()},
+                                        args.cpu_stride, _o.options(**tol))
+        parity = path_parity.get("batched", path_parity.get("plain"))
     n_atoms = int(batch.atom_offsets[-1])
     nao = workloads_nao(batch)
     n_ao = int(sum(nao))
@@ -249,7 +291,11 @@ def main():
     line = {
         "metric": METRIC, "value": value, "unit": "systems/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak",
+        # the metric's own baseline: the plain all-FP64 HIP path on the same batch
+        "vs_baseline": (runs["batched"]["value"] / runs["plain"]["value"]
+                        if len(runs) == 2 else None),
+        "dtype": "f64",
         "data": "synthetic (seeded alkane generator)",
         "config": {"workload": f"{args.workload}: {batch.batch_size} "
                                f"{WORKLOAD_TEXT.get(args.workload, 'systems')}, seed {args.seed}",
@@ -258,10 +304,13 @@ def main():
                           + (f" (step {args.warm_step:g} bohr)" if args.scc_start == "warm" else "")
                           + f", charge {args.charge_tol:g}, energy {args.energy_tol:g}, max 250, "
                           "Broyden 8/0.4, 300 K",
-                   "path": "plain FP64 HIP" + (", fixed-topology plan" if plan else ""),
+                   "path": ("batched (graph-captured SCC rounds)" if "batched" in runs
+                            else "plain FP64 HIP") + (", fixed-topology plan" if plan else ""),
                    "inputs": "device-resident (ROCm buffers)",
                    "lanes": lanes_for_batch(batch.batch_size)},
         "roofline": roof,
+        "paths": {name: {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                         "parity_vs_oracle": path_parity.get(name)} for name, r in runs.items()},
         "path_fp64_equivalent": {"flops_per_step": fpath, "achieved": path_tflops,
                                  "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                                  "frac": path_tflops / FP64_PEAK_TFLOPS},
@@ -275,8 +324,6 @@ def main():
         "kernels": kernels,
         "kernels_source": "warmup calls, events around every launch, summed over lanes",
     }
-    if plan:
-        plan.close()
     ctx.close()
     print(json.dumps(line), flush=True)
 

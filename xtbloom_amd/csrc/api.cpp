@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -403,6 +404,14 @@ std::vector<xtb::Device*> lanes_for(xtbloom_context_t* context, int64_t B) {
   return lanes;
 }
 
+// Execution path of a call, from XTBLOOM_AMD_PATH: "plain" (the FP64 path
+// with one host check per SCC round) or "batched" (graph-captured SCC rounds
+// driven from the device and the mixed-precision products).
+bool batched_path() {
+  const char* env = getenv("XTBLOOM_AMD_PATH");
+  return env != nullptr && std::strcmp(env, "batched") == 0;
+}
+
 // xtbloom_compute (plan == NULL) and xtbloom_plan_compute.  *admitted (if
 // given) tells whether the call got past admission (validation, plan and WARM
 // checks) before it returned.
@@ -444,6 +453,7 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, 
   cp.valid = false;
   cp.state.clear();
   ro.no_alloc = no_alloc;
+  ro.batched = batched_path();
   const int64_t B = hb.batch, N = hb.atoms, P = hb.points;
   std::vector<int32_t> state_dim(B);
   {

@@ -41,6 +41,11 @@ struct RunOptions {
   const double* warm_state = nullptr;
   // size the device workspace for the batch and return before any kernel
   bool reserve_only = false;
+  // batched path: the SCC rounds run as captured graphs of kGraphRounds
+  // rounds each, driven by a device-resident round counter (the host reads
+  // the active count once per graph, one graph ahead), and the mixed-precision
+  // products; false: the plain FP64 path with one host check per round
+  bool batched = false;
   // allocate nothing: the device workspace is the fixed per-GPU array of
   // kNoAllocWorkspaceBytes (a larger batch is rejected before any kernel),
   // host staging is pinned only when the context's pinned block is already

@@ -349,6 +349,9 @@ struct Device {
   size_t arena_bytes = 0;
   // this device's address of the no_alloc workspace array
   char* scratch = nullptr;
+  // captured SCC rounds of the batched path and what they were captured for
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<char> graph_key;
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t round_ev[2] = {nullptr, nullptr};  // end of the SCC rounds of each parity
@@ -365,6 +368,9 @@ struct Device {
 };
 
 constexpr size_t kPinnedInitialBytes = 4096;
+// SCC rounds per captured graph on the batched path: after convergence at
+// most this many rounds run with no active system (each a few microseconds)
+constexpr int kGraphRounds = 4;
 
 Device* device_create(int device_id, void* stream, std::string& error) {
   int count = 0;
@@ -447,6 +453,7 @@ void device_destroy(Device* d) {
   for (auto& e : d->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : d->probe_pool) (void)hipEventDestroy(e);
+  if (d->graph_exec) (void)hipGraphExecDestroy(d->graph_exec);
   if (d->own_stream && d->stream) (void)hipStreamDestroy(d->stream);
   delete d;
 }
@@ -646,6 +653,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   bytes += plan_bytes<SysState>(B);
   addb(B); addb(B);            // active, valid
   addi(2);                     // counter
+  addi(1);                     // device round counter (batched path)
+  addi((size_t)opt.max_iterations + kGraphRounds);  // active count after each round (batched path)
   addi(B);                     // kocc
   addd(3 * (size_t)N);         // grad
   addd(3 * (size_t)NPC); addd(NPC);
@@ -869,6 +878,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   unsigned char* d_active = cv.take<unsigned char>(B);
   unsigned char* d_valid = cv.take<unsigned char>(B);
   int* d_count = cv.take<int>(2);
+  int* d_round = cv.take<int>(1);
+  int* d_cnt_round = cv.take<int>((size_t)opt.max_iterations + kGraphRounds);
   int* d_kocc = cv.take<int>(B);
   double* d_grad = cv.take<double>(3 * (size_t)N);
   double* d_pcgrad = cv.take<double>(3 * (size_t)NPC);
@@ -1020,7 +1031,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   }
   HIP_TRY(probe_begin(kKCholesky));
   hipLaunchKernelGGL(k_cholesky_inverse, dim3(B), dim3(512), 0, sm, t, d_order, dS, dA, dX, d_st);
-  hipLaunchKernelGGL(k_overlap_rcond, dim3(B), dim3(256), 0, sm, t, d_order, dS, dX, d_st);
+  hipLaunchKernelGGL(k_overlap_rcond, dim3(B), dim3(kRcondThreads), 0, sm, t, d_order, dS, dX, d_st);
   HIP_TRY(probe_end(kKCholesky, all3 / 3.0, 0.0));
   hipLaunchKernelGGL(k_init_active, dim3((B + 255) / 256), dim3(256), 0, sm, B, d_st, d_active,
                      (int*)nullptr, d_st);
@@ -1080,13 +1091,11 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     count = h_count[par];
     return 0;
   };
-  int enqueued = 0;
-  bool stopped = false;
-  for (int it = 0; it < opt.max_iterations; ++it) {
-    const int par = it & 1;
-    int* d_cnt = d_count + par;
-    RoundProbes rp{};
-    HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(int), sm));
+  // One SCC round's launches.  rptr: null in the host-driven loop (round `it`,
+  // active count into cnt), else the device-resident round counter (the
+  // eigen kernel reads the round from it and k_energy_mix counts the systems
+  // still active into cnt[round]).
+  auto enqueue_round = [&](int it, int* cnt, const int* rptr, RoundProbes* rp) -> int {
     HIP_TRY(probe_begin(kKPotentials));
     launch_potentials(d_active);
     HIP_TRY(probe_end(kKPotentials, 0.0, 0.0));
@@ -1094,65 +1103,147 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     hipLaunchKernelGGL(k_fock, dim3((unsigned)chunks.size()), dim3(256), 0, sm, t, d_chunks,
                        d_active, dS, dD, dQ, dH0, d_vsh, d_vd, d_vqq, dH);
     HIP_TRY(probe_end(kKFock, 0.0, 0.0));
-    rp.fock = last_probe();
+    if (rp) rp->fock = last_probe();
     // H' = X^T H X
     HIP_TRY(probe_begin(kKReduce));
     hipLaunchKernelGGL((k_gemm<true, false, false, 1>), dim3(nfull), dim3(256), 0, sm, t, d_full,
                        dX, dH, dT, (const double*)nullptr, (const int*)nullptr,
                        (const int*)nullptr, d_active);
     HIP_TRY(probe_end(kKReduce, 0.0, 0.0));
-    rp.red1 = last_probe();
+    if (rp) rp->red1 = last_probe();
     HIP_TRY(probe_begin(kKReduce));
     hipLaunchKernelGGL((k_gemm<false, false, true, 2>), dim3(nlower), dim3(256), 0, sm, t, d_lower,
                        dT, dX, dA, (const double*)nullptr, (const int*)nullptr,
                        (const int*)nullptr, d_active);
     HIP_TRY(probe_end(kKReduce, 0.0, 0.0));
-    rp.red2 = last_probe();
+    if (rp) rp->red2 = last_probe();
     // eigenpairs of H' (A is overwritten by the reflectors; T, H, C, P, W serve as scratch)
     HIP_TRY(probe_begin(kKTridiag));
     hipLaunchKernelGGL(k_tridiag, dim3(B), dim3(512), 0, sm, t, d_order, dA, d_td, d_te, d_wyT,
                        d_active);
     HIP_TRY(probe_end(kKTridiag, 0.0, 0.0));
-    rp.tri = last_probe();
+    if (rp) rp->tri = last_probe();
     HIP_TRY(probe_begin(kKTridiagEigen));
     // eigenvalues, occupations (kocc), eigenvectors of the kocc lowest levels
     hipLaunchKernelGGL(k_tridiag_eigen, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_td, d_te,
-                       dZ, EigenScratch{dT, dH, dC, dP, dW}, d_eps, d_shift, it, d_zscale,
+                       dZ, EigenScratch{dT, dH, dC, dP, dW}, d_eps, d_shift, it, rptr, d_zscale,
                        FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt});
     HIP_TRY(probe_end(kKTridiagEigen, 0.0, 0.0));
     HIP_TRY(probe_begin(kKBacktransform));
     hipLaunchKernelGGL(k_backtransform, dim3((unsigned)back.size()), dim3(kBackThreads), back_lds, sm, t, d_back,
                        dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
     HIP_TRY(probe_end(kKBacktransform, 0.0, 0.0));
-    rp.bt = last_probe();
+    if (rp) rp->bt = last_probe();
     // C = X Y (the kocc leading columns), P = C diag(f) C^T
     HIP_TRY(probe_begin(kKBack));
     hipLaunchKernelGGL((k_gemm<false, false, false, 3>), dim3(nfull), dim3(256), 0, sm, t, d_full,
                        dX, dY, dC, (const double*)nullptr, (const int*)nullptr, d_kocc, d_active);
     HIP_TRY(probe_end(kKBack, 0.0, 0.0));
-    rp.back = last_probe();
+    if (rp) rp->back = last_probe();
     HIP_TRY(probe_begin(kKDensity));
     hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
                        dC, dP, d_occ, d_kocc, (const int*)nullptr, d_active);
     HIP_TRY(probe_end(kKDensity, 0.0, 0.0));
-    rp.dens = last_probe();
+    if (rp) rp->dens = last_probe();
     HIP_TRY(probe_begin(kKMulliken));
     hipLaunchKernelGGL(k_mulliken_rows, dim3((unsigned)chunks.size()), dim3(256), 0, sm, t,
                        d_chunks, d_active, d_part_off, dP, dS, dD, dQ, dH0, d_mpart);
     hipLaunchKernelGGL(k_mulliken_finish, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_part_off,
                        d_mpart, d_colsum, mx.raw, d_st);
     HIP_TRY(probe_end(kKMulliken, 0.0, 0.0));
-    rp.mull = last_probe();
+    if (rp) rp->mull = last_probe();
     HIP_TRY(probe_begin(kKEnergyMix));
     hipLaunchKernelGGL(k_energy_mix<true>, dim3(B), dim3(512), 0, sm, t, d_order, d_active,
-                       d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_cnt, d_gstage,
-                       d_gstagei);
+                       d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, cnt, d_gstage,
+                       d_gstagei, rptr);
     if (big_systems)
       hipLaunchKernelGGL(k_energy_mix<false>, dim3(B), dim3(512), 0, sm, t, d_order, d_active,
-                         d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, d_cnt, d_gstage,
-                         d_gstagei);
+                         d_cnd4, d_mrad, d_gam, d_vpc, d_qat, d_w4, mx, d_st, cnt, d_gstage,
+                         d_gstagei, rptr);
     HIP_TRY(probe_end(kKEnergyMix, 0.0, 0.0));
     HIP_TRY(hipGetLastError());
+    return 0;
+  };
+  int enqueued = 0;
+  bool stopped = false;
+  // a stream that cannot be captured (the legacy null stream) runs the
+  // host-driven loop
+  bool graph_rounds = opt.batched && !ks;
+  if (graph_rounds) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (sm == nullptr || hipStreamIsCapturing(sm, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+      graph_rounds = false;
+  }
+  if (graph_rounds) {
+    // Batched path: kGraphRounds rounds per captured graph, the round number
+    // and the per-round active counts on the device.  The counters are reset
+    // here, outside the graph (no memset node is captured).  The instantiated
+    // graph is kept on the device while the arena, the topology and the policy
+    // it was captured for are unchanged (every kernel argument is a function
+    // of those).
+    HIP_TRY(hipMemsetAsync(d_round, 0, sizeof(int), sm));
+    HIP_TRY(hipMemsetAsync(d_cnt_round, 0, sizeof(int) * ((size_t)opt.max_iterations + kGraphRounds), sm));
+    std::vector<char> key;
+    auto put = [&](const auto& v) {
+      const char* p = reinterpret_cast<const char*>(&v);
+      key.insert(key.end(), p, p + sizeof(v));
+    };
+    const char* arena_base = opt.no_alloc ? dev->scratch : dev->arena;
+    put(arena_base); put(bytes); put(B); put(N); put(NS); put(NAO); put(NPC);
+    put(cache ? cache->generation : ~uint64_t(0)); put(mem); put(opt.damping);
+    put(opt.charge_tolerance); put(opt.energy_tolerance); put(opt.kt); put(opt.max_iterations);
+    const bool reuse = cache != nullptr && dev->graph_exec != nullptr && dev->graph_key == key;
+    if (!reuse) {
+      if (dev->graph_exec) (void)hipGraphExecDestroy(dev->graph_exec);
+      dev->graph_exec = nullptr;
+      dev->graph_key.clear();
+      HIP_TRY(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+      int st_cap = 0;
+      for (int r = 0; r < kGraphRounds && st_cap == 0; ++r) {
+        st_cap = enqueue_round(0, d_cnt_round, d_round, nullptr);
+        if (st_cap == 0) hipLaunchKernelGGL(k_round_advance, dim3(1), dim3(64), 0, sm, d_round);
+      }
+      hipGraph_t graph = nullptr;
+      const hipError_t ce = hipStreamEndCapture(sm, &graph);
+      if (st_cap != 0) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st_cap;
+      }
+      HIP_TRY(ce);
+      const hipError_t ie = hipGraphInstantiate(&dev->graph_exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIP_TRY(ie);
+      dev->graph_key = key;
+    }
+    int launched = 0, g = 0;
+    while (launched < opt.max_iterations) {
+      HIP_TRY(hipGraphLaunch(dev->graph_exec, sm));
+      launched += kGraphRounds;
+      HIP_TRY(hipMemcpyAsync(h_count + (g & 1), d_cnt_round + launched - 1, sizeof(int),
+                             hipMemcpyDeviceToHost, sm));
+      HIP_TRY(hipEventRecord(dev->round_ev[g & 1], sm));
+      ++g;
+      if (g >= 2) {
+        HIP_TRY(hipEventSynchronize(dev->round_ev[g & 1]));  // graph g-2
+        if (h_count[g & 1] == 0) break;
+      }
+    }
+    HIP_TRY(hipEventSynchronize(dev->round_ev[(g - 1) & 1]));
+    // rounds that started with an active system: up to the first round after
+    // which none was left
+    std::vector<int> cnt((size_t)std::min(launched, opt.max_iterations));
+    HIP_TRY(hipMemcpyAsync(cnt.data(), d_cnt_round, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, sm));
+    HIP_TRY(hipStreamSynchronize(sm));
+    rounds = (int)cnt.size();
+    for (size_t r = 0; r < cnt.size(); ++r)
+      if (cnt[r] == 0) { rounds = (int)r + 1; break; }
+  }
+  for (int it = 0; it < opt.max_iterations && !graph_rounds; ++it) {
+    const int par = it & 1;
+    int* d_cnt = d_count + par;
+    RoundProbes rp{};
+    HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(int), sm));
+    if (const int st_round = enqueue_round(it, d_cnt, nullptr, &rp)) return st_round;
     HIP_TRY(hipMemcpyAsync(h_count + par, d_cnt, sizeof(int), hipMemcpyDeviceToHost, sm));
     if (ks) {
       HIP_TRY(hipMemcpyAsync(h_act_snap[par], d_active, B, hipMemcpyDeviceToHost, sm));

@@ -796,17 +796,21 @@ __global__ __launch_bounds__(512) void k_cholesky_inverse(Topo t, const int* ord
 // stopped on a repeated sign vector, no increase, or a repeated index, then
 // the alternating-sign test vector x_i = (-1)^i (1 + i/(n-1)) with estimate
 // 2 |S^{-1} x|_1 / (3n).  Here S^{-1} v = X (X^T v) with X = L^{-T} (row-major,
-// upper triangular) from k_cholesky_inverse.  One 256-thread block per system
+// upper triangular) from k_cholesky_inverse.  One 512-thread block per system
 // whose factorization succeeded.
 // ---------------------------------------------------------------------------
 constexpr double kMinOverlapRcond = 1e-12;
 
+constexpr int kRcondThreads = 512;
+constexpr int kRcondWaves = kRcondThreads / 64;
+
 struct RcondShared {
   double x[kMaxAO];
   double y[kMaxAO];
+  double part[kRcondWaves][kMaxAO];  // per-wave partial column sums of X^T x
   signed char sgn[kMaxAO];
-  double red[8];
-  int ired[8];
+  double red[kRcondWaves];
+  int ired[kRcondWaves];
 };
 
 // block sum of |v_i|
@@ -818,7 +822,7 @@ __device__ double rc_asum(const double* v, int n, RcondShared& sh) {
   if ((threadIdx.x & 63) == 0) sh.red[threadIdx.x >> 6] = a;
   __syncthreads();
   double t = 0.0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh.red[w];
+  for (int w = 0; w < kRcondWaves; ++w) t += sh.red[w];
   __syncthreads();
   return t;
 }
@@ -841,33 +845,83 @@ __device__ int rc_iamax(const double* v, int n, RcondShared& sh) {
   __syncthreads();
   best = sh.red[0];
   bi = sh.ired[0];
-  for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+  for (int w = 1; w < kRcondWaves; ++w)
     if (sh.red[w] > best || (sh.red[w] == best && sh.ired[w] < bi)) { best = sh.red[w]; bi = sh.ired[w]; }
   __syncthreads();
   return bi;
 }
 
 // sh.x <- S^{-1} sh.x  (y = X^T x, then x = X y); X row-major upper triangular
-__device__ void rc_apply(const double* __restrict__ Xm, int n, RcondShared& sh) {
+// with its strict lower part stored as zero.  Both products read X by rows,
+// lanes along the row, 16 / KC rows per wave at a time (their loads are in
+// flight together): X^T x as per-wave partial column sums (summed in
+// a fixed order), X y as one wave per row.  KC = column chunks of 64.
+template <int KC>
+__device__ void rc_apply_k(const double* __restrict__ Xm, int n, RcondShared& sh) {
+  constexpr int U = 16 / KC;  // rows in flight per wave: 16 loads per lane
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   __syncthreads();
-  for (int c = threadIdx.x; c < n; c += blockDim.x) {
-    double acc = 0.0;
-    for (int r = 0; r <= c; ++r) acc = fma(Xm[(int64_t)r * n + c], sh.x[r], acc);
-    sh.y[c] = acc;
+  double acc[KC];
+#pragma unroll
+  for (int m = 0; m < KC; ++m) acc[m] = 0.0;
+  for (int r0 = wv; r0 < n; r0 += U * kRcondWaves) {
+    double xv[U], v[U][KC];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + u * kRcondWaves;
+      xv[u] = r < n ? sh.x[r] : 0.0;
+#pragma unroll
+      for (int m = 0; m < KC; ++m) {
+        const int c = lane + 64 * m;
+        v[u][m] = (r < n && c >= r && c < n) ? Xm[(int64_t)r * n + c] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int m = 0; m < KC; ++m) acc[m] = fma(v[u][m], xv[u], acc[m]);
+  }
+#pragma unroll
+  for (int m = 0; m < KC; ++m) {
+    const int c = lane + 64 * m;
+    if (c < n) sh.part[wv][c] = acc[m];
   }
   __syncthreads();
-  // one wavefront per row, lanes along the row
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int r = wv; r < n; r += nw) {
-    double acc = 0.0;
-    for (int c = r + lane; c < n; c += 64) acc = fma(Xm[(int64_t)r * n + c], sh.y[c], acc);
-    acc = wave_sum(acc);
-    if (lane == 0) sh.x[r] = acc;
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    double t = 0.0;
+    for (int w = 0; w < kRcondWaves; ++w) t += sh.part[w][c];
+    sh.y[c] = t;
+  }
+  __syncthreads();
+  for (int r0 = wv; r0 < n; r0 += U * kRcondWaves) {
+    double a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + u * kRcondWaves;
+      a[u] = 0.0;
+#pragma unroll
+      for (int m = 0; m < KC; ++m) {
+        const int c = lane + 64 * m;
+        if (r < n && c >= r && c < n) a[u] = fma(Xm[(int64_t)r * n + c], sh.y[c], a[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + u * kRcondWaves;
+      const double t = wave_sum(a[u]);
+      if (lane == 0 && r < n) sh.x[r] = t;
+    }
   }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_overlap_rcond(Topo t, const int* order, const double* S,
+__device__ void rc_apply(const double* __restrict__ Xm, int n, RcondShared& sh) {
+  if (n <= 256) rc_apply_k<4>(Xm, n, sh);
+  else if (n <= 512) rc_apply_k<8>(Xm, n, sh);
+  else rc_apply_k<16>(Xm, n, sh);
+}
+
+__global__ __launch_bounds__(kRcondThreads) void k_overlap_rcond(Topo t, const int* order, const double* S,
                                                        const double* X, SysState* status) {
   const int s = order[blockIdx.x];
   if (status[s].status != 0) return;
@@ -876,21 +930,21 @@ __global__ __launch_bounds__(256) void k_overlap_rcond(Topo t, const int* order,
   const double* Sm = S + mo;
   const double* Xm = X + mo;
   __shared__ RcondShared sh;
-  // |S|_1 (max column sum; S is symmetric)
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    double acc = 0.0;
-    for (int i = 0; i < n; ++i) acc += fabs(Sm[(int64_t)i * n + j]);
-    sh.y[j] = acc;
-  }
-  __syncthreads();
+  // |S|_1: S is symmetric, so the largest row sum (one wave per row, lanes
+  // along the row)
   double anorm = 0.0;
   {
+    const int lane = threadIdx.x & 63;
     double m = 0.0;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) m = fmax(m, sh.y[j]);
+    for (int r = threadIdx.x >> 6; r < n; r += kRcondWaves) {
+      double a = 0.0;
+      for (int c = lane; c < n; c += 64) a += fabs(Sm[(int64_t)r * n + c]);
+      m = fmax(m, wave_sum(a));
+    }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
     if ((threadIdx.x & 63) == 0) sh.red[threadIdx.x >> 6] = m;
     __syncthreads();
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) anorm = fmax(anorm, sh.red[w]);
+    for (int w = 0; w < kRcondWaves; ++w) anorm = fmax(anorm, sh.red[w]);
     __syncthreads();
   }
   // estimate of |S^{-1}|_1
@@ -949,42 +1003,43 @@ __device__ __forceinline__ double fast_div(double x, double y) {
   return x * r;
 }
 
-// number of eigenvalues of T[b0:b1) below x (LDL^T pivots, tiny pivots set to -pivmin)
-__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int b0, int b1,
-                                           double x, double pivmin) {
-  double q = d[b0] - x;
-  if (fabs(q) < pivmin) q = -pivmin;
-  int c = q <= 0.0 ? 1 : 0;
-  for (int i = b0 + 1; i < b1; ++i) {
-    q = (d[i] - x) - fast_div(e2[i - 1], q);
-    if (fabs(q) < pivmin) q = -pivmin;
-    c += q <= 0.0 ? 1 : 0;
-  }
-  return c;
-}
-
-// Sturm count of T[b0:b1) - x I (as sturm_count) together with
-// L(x) = d/dx log|det(T - x I)| = sum_i 1/(x - lambda_i), accumulated from the
-// pivot recurrence q_i = (d_i - x) - e_{i-1}^2 / q_{i-1},
-// q_i' = -1 + e_{i-1}^2 q_{i-1}' / q_{i-1}^2, L = sum q_i' / q_i.
+// Sturm count of T[b0:b1) - x I (the number of eigenvalues below x) together
+// with L(x) = d/dx log|det(T - x I)| = sum_i 1/(x - lambda_i), from the
+// three-term recurrence of the leading principal minors of T - x I
+//   p_i = (d_i - x) p_{i-1} - e_{i-1}^2 p_{i-2},   p_{-1} = 1,
+// and of their derivatives
+//   p_i' = (d_i - x) p_{i-1}' - p_{i-1} - e_{i-1}^2 p_{i-2}',  p_{-1}' = 0.
+// The count is the number of sign changes along p_{-1}, p_0, ..., p_{b-1}
+// (a vanishing minor takes the sign opposite to its predecessor, as a
+// vanishing pivot of T - x I = L D L^T counts as negative), and L = p'/p of
+// the last minor.  The pivots of L D L^T are the ratios p_i / p_{i-1}, so this
+// is the pivot count without a division per row: one FMA per row sits on the
+// serial chain.  Every eight rows both minors and both derivatives are scaled
+// by the same power of two (the exponent of the larger minor), which keeps
+// them in range and changes neither a sign nor the ratio p'/p.
 __device__ __forceinline__ int sturm_count_logd(const double* d, const double* e2, int b0, int b1,
-                                                double x, double pivmin, double& L) {
-  double q = d[b0] - x;
-  if (fabs(q) < pivmin) q = -pivmin;
-  int c = q <= 0.0 ? 1 : 0;
-  double r = fast_div(1.0, q), dq = -1.0;
-  double acc = dq * r;
+                                                double x, double& L) {
+  double pm2 = 1.0, pm1 = d[b0] - x;  // p_{i-2}, p_{i-1}
+  double qm2 = 0.0, qm1 = -1.0;       // their derivatives
+  int neg = pm1 <= 0.0 ? 1 : 0;       // sign of p_{i-1} (p_{-1} = 1 > 0)
+  int c = neg;
+  // sign bookkeeping on the bits (selects, no branches: the count must not
+  // put control flow on the chain)
   auto step = [&](double di, double ei2) __attribute__((always_inline)) {
-    const double tq = ei2 * r;
-    dq = fma(tq * r, dq, -1.0);
-    q = (di - x) - tq;
-    if (fabs(q) < pivmin) q = -pivmin;
-    c += q <= 0.0 ? 1 : 0;
-    r = fast_div(1.0, q);
-    acc = fma(dq, r, acc);
+    const double dx = di - x;
+    const double p = fma(dx, pm1, -ei2 * pm2);
+    const double q = fma(dx, qm1, -fma(ei2, qm2, pm1));
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(p);
+    const int sgn = (int)(bits >> 63);
+    const int sn = (bits << 1) == 0ull ? (neg ^ 1) : sgn;  // a zero minor flips the sign
+    c += sn ^ neg;
+    neg = sn;
+    pm2 = pm1;
+    pm1 = p;
+    qm2 = qm1;
+    qm1 = q;
   };
-  // the recurrence is a serial chain: load eight entries of d and e^2 ahead so
-  // that LDS latency stays off it
+  // load eight entries of d and e^2 ahead so that LDS latency stays off the chain
   int i = b0 + 1;
   for (; i + 7 < b1; i += 8) {
     double dv[8], ev[8];
@@ -995,9 +1050,14 @@ __device__ __forceinline__ int sturm_count_logd(const double* d, const double* e
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) step(dv[u], ev[u]);
+    const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(pm1), fabs(pm2)));
+    pm1 = ldexp(pm1, -ex);
+    pm2 = ldexp(pm2, -ex);
+    qm1 = ldexp(qm1, -ex);
+    qm2 = ldexp(qm2, -ex);
   }
   for (; i < b1; ++i) step(d[i], e2[i - 1]);
-  L = acc;
+  L = qm1 / pm1;
   return c;
 }
 
@@ -1074,10 +1134,9 @@ __device__ __forceinline__ bool eig_start(EigState& st, int j, const double* d, 
 }
 
 // one pass at st.x; returns true (result in `out`) when the eigenvalue has converged
-__device__ __forceinline__ bool eig_step(EigState& st, const double* d, const double* e2,
-                                         double pivmin, double& out) {
+__device__ __forceinline__ bool eig_step(EigState& st, const double* d, const double* e2, double& out) {
   double L;
-  const int c = sturm_count_logd(d, e2, st.b0, st.b1, st.x, pivmin, L);
+  const int c = sturm_count_logd(d, e2, st.b0, st.b1, st.x, L);
   if (st.phase < 2) {
     // guess bracket: only narrow where the count confirms the side
     if (st.x > st.lo && st.x < st.hi) {
@@ -1327,7 +1386,8 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
                                                        const unsigned char* active,
                                                        const double* dall, const double* eall,
                                                        double* Zall, EigenScratch sc,
-                                                       double* wall, double* shift, int round,
+                                                       double* wall, double* shift, int round_arg,
+                                                       const int* round_ptr,
                                                        double* zscale_all, FermiOut fo) {
   const int s = order[blockIdx.x];
   if (active && !active[s]) return;
@@ -1385,6 +1445,9 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
     if (e[i] == 0.0) atomicAdd(&nsplit, 1);
   __syncthreads();
   const bool unsplit = nsplit == 0;
+  // the SCC round: the launch argument, or the device-resident counter of
+  // rounds enqueued ahead of the host (graph-captured rounds)
+  const int round = round_ptr ? *round_ptr : round_arg;
   const bool have_old = round >= 1 && unsplit;
   const bool guess_ok = round >= 2 && unsplit;
   const double delta = guess_ok ? 2.0 * shift[s] : 0.0;
@@ -1430,7 +1493,7 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
         if (!busy) continue;
       }
       double x;
-      if (eig_step(st, d, e2, pivmin, x)) {
+      if (eig_step(st, d, e2, x)) {
         lam[st.j] = x;
         if (have_old) dmax = fmax(dmax, fabs(x - wall[t.ao_off[s] + st.j]));
         busy = false;
@@ -1493,15 +1556,37 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
   // ---- eigenvectors: thread per cluster start ----
   // A cluster is needed when its first (lowest) member is below kocc; every
   // member of a needed cluster is computed so the cluster stays orthogonal.
+  // an unsplit T is one block [0, n) whose 1-norm is the block-wide maximum
+  // row sum; split blocks are found and measured by the thread that needs them
+  __shared__ double tnorm1;
+  if (unsplit) {
+    double m = 0.0;
+    for (int i = tid; i < n; i += nt)
+      m = fmax(m, fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0));
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((tid & 63) == 0) red[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < (nt >> 6); ++w) m = fmax(m, red[w]);
+      tnorm1 = m;
+    }
+    __syncthreads();
+  }
   for (int j = tid; j < n; j += nt) {
     if (rnk[j] >= kocc) continue;
     int b0 = j, b1 = j + 1;
-    while (b0 > 0 && e[b0 - 1] != 0.0) --b0;
-    while (b1 < n && e[b1 - 1] != 0.0) ++b1;
-    const int bn = b1 - b0;
     double onenrm = 0.0;
-    for (int i = b0; i < b1; ++i)
-      onenrm = fmax(onenrm, fabs(d[i]) + (i > b0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < b1 ? fabs(e[i]) : 0.0));
+    if (unsplit) {
+      b0 = 0;
+      b1 = n;
+      onenrm = tnorm1;
+    } else {
+      while (b0 > 0 && e[b0 - 1] != 0.0) --b0;
+      while (b1 < n && e[b1 - 1] != 0.0) ++b1;
+      for (int i = b0; i < b1; ++i)
+        onenrm = fmax(onenrm, fabs(d[i]) + (i > b0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < b1 ? fabs(e[i]) : 0.0));
+    }
+    const int bn = b1 - b0;
     const double gtol = kClusterTol * fmax(onenrm, kSafeMin);
     if (j > b0 && lam[j] - lam[j - 1] <= gtol) continue;  // not a cluster start
     int g1 = j + 1;

@@ -18,7 +18,8 @@ __global__ __launch_bounds__(512) void k_energy_mix(Topo t, const int* order, un
                                                     const double* gam, const double* vpc,
                                                     double* qat_raw, double* w4, MixerArrays mx,
                                                     SysState* st, int* active_count,
-                                                    double* gstage, int* gstagei) {
+                                                    double* gstage, int* gstagei,
+                                                    const int* round_ptr) {
   const int s = order[blockIdx.x];
   if (!active[s]) return;
   const int a0 = t.atom_off[s], a1 = t.atom_off[s + 1], N = a1 - a0;
@@ -237,8 +238,14 @@ __global__ __launch_bounds__(512) void k_energy_mix(Topo t, const int* order, un
     }
     if (!isfinite(F) || !isfinite(rms)) { z.status = 6; stop = true; }
     if (stop) active[s] = 0;
-    else atomicAdd(active_count, 1);
+    else atomicAdd(round_ptr ? active_count + *round_ptr : active_count, 1);
   }
+}
+
+// End of a device-driven SCC round: the next round's number.  active_count
+// (indexed by round) was counted by k_energy_mix.
+__global__ void k_round_advance(int* round) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *round += 1;
 }
 
 // ---------------------------------------------------------------------------
