@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--path", choices=("both", "plain", "batched"), default="both",
                     help="execution paths timed: the headline value is the batched path, "
                          "vs_baseline its ratio to the plain FP64 path")
+    ap.add_argument("--split32", action="store_true",
+                    help="also time the batched path with split-precision FP32 density products")
     ap.add_argument("--plan", action="store_true",
                     help="timed calls through a fixed-topology plan (no per-launch events: "
                          "the roofline entry then comes from the warmup calls)")
@@ -167,9 +169,10 @@ def main():
                                                      dtype=cur.dtype)
             geoms.append(cur)
 
-    def run_path(path, stats, timed_stats):
+    def run_path(path, stats, timed_stats, split32=False):
         """Warmup and timed calls of one execution path (XTBLOOM_AMD_PATH)."""
         os.environ["XTBLOOM_AMD_PATH"] = path
+        os.environ["XTBLOOM_AMD_SPLIT32"] = "1" if split32 else "0"
         out = None
         # warmup: with stats, events around every launch (the per-family breakdown)
         wagg = new_agg()
@@ -226,6 +229,11 @@ def main():
     if args.path in ("both", "batched"):
         # timed without events (events would split the captured rounds)
         runs["batched"] = run_path("batched", args.path == "batched", False)
+    if args.split32:
+        # the split-precision FP32 density products on the batched path (reported
+        # beside the headline, which stays the FP64 batched path)
+        runs["batched_split32"] = run_path("batched", False, False, split32=True)
+    os.environ.pop("XTBLOOM_AMD_SPLIT32", None)
     head = runs["batched"] if "batched" in runs else runs["plain"]
     ref_run = runs["plain"] if "plain" in runs else runs["batched"]
     out = head["out"]
@@ -294,7 +302,7 @@ def main():
         "higher_is_better": True, "scaling": "weak",
         # the metric's own baseline: the plain all-FP64 HIP path on the same batch
         "vs_baseline": (runs["batched"]["value"] / runs["plain"]["value"]
-                        if len(runs) == 2 else None),
+                        if "batched" in runs and "plain" in runs else None),
         "dtype": "f64",
         "data": "synthetic (seeded alkane generator)",
         "config": {"workload": f"{args.workload}: {batch.batch_size} "

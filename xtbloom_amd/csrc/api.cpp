@@ -454,6 +454,10 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, 
   cp.state.clear();
   ro.no_alloc = no_alloc;
   ro.batched = batched_path();
+  {
+    const char* env = getenv("XTBLOOM_AMD_SPLIT32");
+    ro.split32 = env != nullptr && std::strcmp(env, "1") == 0;
+  }
   const int64_t B = hb.batch, N = hb.atoms, P = hb.points;
   std::vector<int32_t> state_dim(B);
   {

@@ -46,6 +46,9 @@ struct RunOptions {
   // the active count once per graph, one graph ahead), and the mixed-precision
   // products; false: the plain FP64 path with one host check per round
   bool batched = false;
+  // split-precision FP32 matrix-core density products: P in the SCC rounds
+  // whose mixer residual is above 1e-3 (and the first round), W always
+  bool split32 = false;
   // allocate nothing: the device workspace is the fixed per-GPU array of
   // kNoAllocWorkspaceBytes (a larger batch is rejected before any kernel),
   // host staging is pinned only when the context's pinned block is already

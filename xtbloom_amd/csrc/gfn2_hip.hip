@@ -1142,7 +1142,11 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     if (rp) rp->back = last_probe();
     HIP_TRY(probe_begin(kKDensity));
     hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
-                       dC, dP, d_occ, d_kocc, (const int*)nullptr, d_active);
+                       dC, dP, d_occ, d_kocc, (const int*)nullptr, d_active,
+                       opt.split32 ? (const SysState*)d_st : nullptr);
+    if (opt.split32)
+      hipLaunchKernelGGL(k_gemm_split32, dim3(nlower), dim3(256), 0, sm, t, d_lower, dC, dP, d_occ, d_kocc,
+                         (const SysState*)d_st, d_active);
     HIP_TRY(probe_end(kKDensity, 0.0, 0.0));
     if (rp) rp->dens = last_probe();
     HIP_TRY(probe_begin(kKMulliken));
@@ -1192,6 +1196,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     put(arena_base); put(bytes); put(B); put(N); put(NS); put(NAO); put(NPC);
     put(cache ? cache->generation : ~uint64_t(0)); put(mem); put(opt.damping);
     put(opt.charge_tolerance); put(opt.energy_tolerance); put(opt.kt); put(opt.max_iterations);
+    put(opt.split32);
     const bool reuse = cache != nullptr && dev->graph_exec != nullptr && dev->graph_key == key;
     if (!reuse) {
       if (dev->graph_exec) (void)hipGraphExecDestroy(dev->graph_exec);
@@ -1295,8 +1300,12 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   if (opt.forces) {
     // W = C diag(f eps) C^T
     HIP_TRY(probe_begin(kKEnergyWeighted));
-    hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
-                       dC, dW, d_fe, d_kocc, (const int*)nullptr, d_valid);
+    if (opt.split32)
+      hipLaunchKernelGGL(k_gemm_split32, dim3(nlower), dim3(256), 0, sm, t, d_lower, dC, dW, d_fe, d_kocc,
+                         (const SysState*)nullptr, d_valid);
+    else
+      hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
+                         dC, dW, d_fe, d_kocc, (const int*)nullptr, d_valid);
     HIP_TRY(probe_end(kKEnergyWeighted, work_occ(h_valid), 0.0));  // lower half, K = kocc
     if (NAP > 0) {
       HIP_TRY(probe_begin(kKForcePairs));

@@ -39,14 +39,23 @@ __device__ __forceinline__ mfma_d4 mfma_16x16x4(double a, double b, mfma_d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Systems whose density product runs in split-precision FP32 (k_gemm_split32):
+// those in their first SCC round or with a mixer residual above kSplitResidual.
+constexpr double kSplitResidual = 1e-3;
+__device__ __forceinline__ bool split_round(const SysState& z) {
+  return z.iterations == 0 || z.residual_rms > kSplitResidual;
+}
+
 template <bool TA, bool TB, bool SYMM, int TRI = 0>
 __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, const double* A,
                                               const double* Bm, double* C, const double* scale,
                                               const int* kdim, const int* ncol,
-                                              const unsigned char* active) {
+                                              const unsigned char* active,
+                                              const SysState* split = nullptr) {
   const TileTask task = tasks[blockIdx.x];
   const int s = task.sys;
   if (active && !active[s]) return;
+  if (split && split_round(split[s])) return;  // computed by k_gemm_split32
   if (ncol && task.tj * kTile >= ncol[s]) return;  // columns past ncol are not needed
   const int n = t.ao_off[s + 1] - t.ao_off[s];
   const int64_t mo = t.mat_off[s];
@@ -144,6 +153,118 @@ __global__ __launch_bounds__(256) void k_gemm(Topo t, const TileTask* tasks, con
         if (gj >= n) continue;
         c[(int64_t)gi * n + gj] = acc[p][q][r];
         if (SYMM && task.ti != task.tj) c[(int64_t)gj * n + gi] = acc[p][q][r];
+      }
+    }
+}
+
+// Split-precision FP32 matrix-core form of the symmetric density products
+// P = C diag(f) C^T and W = C diag(f eps) C^T (lower tiles of the tile queue,
+// K = kocc, mirrored).  Each FP64 operand x is split into x_hi = fp32(x) and
+// x_lo = fp32(x - x_hi), and a_hi b_hi + a_hi b_lo + a_lo b_hi run on
+// v_mfma_f32_16x16x4f32 with FP32 accumulation over one 16-deep K stage; the
+// stage sums are added in FP64.  The dropped a_lo b_lo term is below 2^-48 of
+// a product; the FP32 accumulation bounds the result near 1e-7 relative to
+// the stage sums.  `only_early` (P): systems still in their first SCC round
+// or with a mixer residual above kSplitResidual; null (W): every system.
+// k_gemm skips the same systems through its `split` argument.
+
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_gemm_split32(Topo t, const TileTask* tasks, const double* A,
+                                                      double* C, const double* scale, const int* kdim,
+                                                      const SysState* only_early,
+                                                      const unsigned char* active) {
+  const TileTask task = tasks[blockIdx.x];
+  const int s = task.sys;
+  if (active && !active[s]) return;
+  if (only_early && !split_round(only_early[s])) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t mo = t.mat_off[s];
+  const double* a = A + mo;
+  double* c = C + mo;
+  const double* sc = scale + t.ao_off[s];
+  const int K = kdim[s];
+  const int i0 = task.ti * kTile, j0 = task.tj * kTile;
+  // operands of one K stage: A = C[i][k] f_k (rows i0..), B^T = C[j][k] (rows j0..)
+  __shared__ float Ah[kTileK][kTile + 1], Al[kTileK][kTile + 1];
+  __shared__ float Bh[kTileK][kTile + 1], Bl[kTileK][kTile + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int qi = 32 * (w >> 1), qj = 32 * (w & 1);
+  const int l16 = lane & 15, lk = lane >> 4;
+  double acc[2][2][4];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[p][q][r] = 0.0;
+  constexpr int kPer = kTile * kTileK / 256;
+  for (int k0 = 0; k0 < K; k0 += kTileK) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int idx = threadIdx.x + 256 * u;
+      const int kk = idx % kTileK, ii = idx / kTileK;
+      const int gk = k0 + kk;
+      double va = 0.0, vb = 0.0;
+      if (gk < K) {
+        if (i0 + ii < n) va = a[(int64_t)(i0 + ii) * n + gk] * sc[gk];
+        if (j0 + ii < n) vb = a[(int64_t)(j0 + ii) * n + gk];
+      }
+      const float ha = (float)va, hb = (float)vb;
+      Ah[kk][ii] = ha;
+      Al[kk][ii] = (float)(va - (double)ha);
+      Bh[kk][ii] = hb;
+      Bl[kk][ii] = (float)(vb - (double)hb);
+    }
+    __syncthreads();
+    mfma_f4 part[2][2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) part[p][q] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k4 = 0; k4 < kTileK; k4 += 4) {
+      float ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        ah[p] = Ah[k4 + lk][qi + 16 * p + l16];
+        al[p] = Al[k4 + lk][qi + 16 * p + l16];
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        bh[q] = Bh[k4 + lk][qj + 16 * q + l16];
+        bl[q] = Bl[k4 + lk][qj + 16 * q + l16];
+      }
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          part[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[p], bh[q], part[p][q], 0, 0, 0);
+          part[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[p], bl[q], part[p][q], 0, 0, 0);
+          part[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[p], bh[q], part[p][q], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[p][q][r] += (double)part[p][q][r];
+    __syncthreads();
+  }
+  // v_mfma_f32_16x16x4f32 leaves C[4 (l / 16) + r][l % 16] in lane l, item r
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gi = i0 + qi + 16 * p + 4 * lk + r;
+      if (gi >= n) continue;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int gj = j0 + qj + 16 * q + l16;
+        if (gj >= n) continue;
+        c[(int64_t)gi * n + gj] = acc[p][q][r];
+        if (task.ti != task.tj) c[(int64_t)gj * n + gi] = acc[p][q][r];
       }
     }
 }
