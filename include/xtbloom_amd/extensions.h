@@ -75,6 +75,10 @@ typedef struct xtbloom_amd_extension {
   /* input: bit f set = per-launch events for kernel family f only; 0 = every
    * family (fewer events perturb concurrent lanes less) */
   uint32_t kernel_stats_mask;
+  /* [sum n] orbital energies of the final SCC round, ascending per system,
+   * and the total (alpha + beta) occupation of each level */
+  double* orbital_energies;
+  double* occupations;
 } xtbloom_amd_extension_t;
 
 /* xtbloom_compute plus the extension outputs above. */

@@ -1054,6 +1054,293 @@ static long double fermi_sum(const double* eps, int n, long double ref, long dou
   return s;
 }
 
+/* ---- binary64 occupation policy (gfn2/occupation_binary64_policy.hpp) ----
+ * The rare path of compute_occupations (eigensolver.cpp:1162-1218): a root in
+ * the translated frame scaled by kT with saturated arithmetic and
+ * compensated sums (:185-258), one retry at a unique changing degenerate
+ * frontier when bisection exhausts the binary64 spacing (:260-348), and a
+ * publication that replaces at most two degenerate blocks by one shared
+ * occupation each so that the published occupations conserve the electron
+ * (or hole) count within 64 eps, or within 2 eps per block member when no
+ * strict candidate exists (:351-600).  Restated here, not copied. */
+#define P64_MAX 1.79769313486231570814527423731704357e308
+#define P64_EPS 2.220446049250313080847263336181640625e-16
+
+static double p64_sat_add(double a, double b) {
+  const double r = a + b;
+  if (isfinite(r)) return r;
+  return (signbit(a) != 0) == (signbit(b) != 0) && signbit(a) ? -P64_MAX : P64_MAX;
+}
+static double p64_sat_sub(double a, double b) {
+  const double r = a - b;
+  if (isfinite(r)) return r;
+  return a < b ? -P64_MAX : P64_MAX;
+}
+static double p64_sat_mul_nonneg(double a, double b) {
+  if (a == 0.0 || b == 0.0) return 0.0;
+  return a > P64_MAX / b ? P64_MAX : a * b;
+}
+static double p64_sat_affine(double ref, double mult, double scale) {
+  const double prod = mult * scale;
+  if (isfinite(prod)) return p64_sat_add(ref, prod);
+  const double nrm = ref / P64_MAX + mult * (scale / P64_MAX);
+  if (nrm >= 1.0) return P64_MAX;
+  if (nrm <= -1.0) return -P64_MAX;
+  return nrm * P64_MAX;
+}
+/* (e - ref) / T without spurious overflow */
+static double p64_scaled(double e, double ref, double t) {
+  double r;
+  if ((signbit(e) != 0) == (signbit(ref) != 0)) r = (e - ref) / t;
+  else r = e / t - ref / t;
+  if (isfinite(r)) return r;
+  return e < ref ? -P64_MAX : P64_MAX;
+}
+static double p64_fermi(double se, double smu) {
+  const double a = p64_sat_sub(se, smu);
+  if (a >= 0.0) { const double v = exp(-a); return v / (1.0 + v); }
+  return 1.0 / (exp(a) + 1.0);
+}
+static double p64_hole(double se, double smu) {
+  const double a = p64_sat_sub(se, smu);
+  if (a >= 0.0) return 1.0 / (exp(-a) + 1.0);
+  const double v = exp(a);
+  return v / (1.0 + v);
+}
+typedef struct { double v, c; } p64_ksum;  /* Kahan sum and its compensation */
+static void p64_kadd(p64_ksum* s, double x) {
+  const double y = x - s->c, t = s->v + y;
+  s->c = (t - s->v) - y;
+  s->v = t;
+}
+static double p64_quantity(const double* e, int n, double ref, double smu, double t, int holes) {
+  p64_ksum s = {0.0, 0.0};
+  for (int i = 0; i < n; ++i) {
+    const double se = p64_scaled(e[i], ref, t);
+    p64_kadd(&s, holes ? p64_hole(se, smu) : p64_fermi(se, smu));
+  }
+  return s.v;
+}
+typedef struct {
+  double ref, lo, hi, smu, q;
+  int exhausted, retried, fb, fe;
+} p64_root;
+static int p64_root_once(const double* e, int n, double target, double t, int holes, double ref,
+                         p64_root* r) {
+  const double lf = log(target) - log((double)n);
+  const double steps = fmax(64.0, -lf + 8.0);
+  const double smin = p64_scaled(e[0], ref, t), smax = p64_scaled(e[n - 1], ref, t);
+  const double scale = fmax(1.0, fabs(p64_sat_sub(smax, smin)));
+  const double margin = p64_sat_add(steps, p64_sat_mul_nonneg(64.0 * P64_EPS, scale));
+  double lo = p64_sat_sub(smin, margin), hi = p64_sat_add(smax, margin);
+  const double qlo = p64_quantity(e, n, ref, lo, t, holes), qhi = p64_quantity(e, n, ref, hi, t, holes);
+  const int brk = holes ? (qlo >= target && qhi <= target) : (qlo <= target && qhi >= target);
+  if (!isfinite(lo) || !isfinite(hi) || !(lo < hi) || !isfinite(qlo) || !isfinite(qhi) || !brk) return 0;
+  const double tol = 64.0 * P64_EPS * target;
+  int exhausted = 0;
+  for (int it = 0; it < 4096; ++it) {
+    const double mid = 0.5 * lo + 0.5 * hi;
+    const double q = p64_quantity(e, n, ref, mid, t, holes);
+    if (!isfinite(q)) return 0;
+    if (fabs(q - target) <= tol) { lo = hi = mid; break; }
+    if (mid == lo || mid == hi) { exhausted = 1; break; }
+    if ((!holes && q < target) || (holes && q > target)) lo = mid; else hi = mid;
+  }
+  const double smu = 0.5 * lo + 0.5 * hi;
+  const double q = p64_quantity(e, n, ref, smu, t, holes);
+  if (!isfinite(q)) return 0;
+  r->ref = ref; r->lo = lo; r->hi = hi; r->smu = smu; r->q = q;
+  r->exhausted = exhausted; r->retried = 0; r->fb = -1; r->fe = -1;
+  return 1;
+}
+/* the one degenerate block whose occupation differs between the bracket ends
+ * (n: none, -1: more than one) */
+static int p64_frontier(const double* e, int n, double t, int holes, const p64_root* r) {
+  int fr = n;
+  for (int b = 0; b < n;) {
+    int en = b + 1;
+    while (en < n && e[en] == e[b]) ++en;
+    if (en - b > 1) {
+      const double se = p64_scaled(e[b], r->ref, t);
+      const double lv = holes ? p64_hole(se, r->lo) : p64_fermi(se, r->lo);
+      const double uv = holes ? p64_hole(se, r->hi) : p64_fermi(se, r->hi);
+      if (lv != uv) {
+        if (fr != n) return -1;
+        fr = b;
+      }
+    }
+    b = en;
+  }
+  return fr;
+}
+static int p64_solve_root(const double* e, int n, double target, double t, int holes, p64_root* r) {
+  if (!p64_root_once(e, n, target, t, holes, holes ? e[n - 1] : e[0], r)) return 0;
+  const double tol = 64.0 * P64_EPS * target;
+  const double qlo = p64_quantity(e, n, r->ref, r->lo, t, holes), qhi = p64_quantity(e, n, r->ref, r->hi, t, holes);
+  const int straddles = isfinite(qlo) && isfinite(qhi) &&
+                        (holes ? (qlo >= target && qhi <= target) : (qlo <= target && qhi >= target));
+  if (!r->exhausted || fabs(r->q - target) <= tol || !straddles) return 1;
+  const int fr = p64_frontier(e, n, t, holes, r);
+  if (fr < 0 || fr >= n) return 1;
+  r->fb = fr;
+  r->fe = fr + 1;
+  while (r->fe < n && e[r->fe] == e[fr]) ++r->fe;
+  if (e[fr] == r->ref) return 1;
+  p64_root again;
+  if (!p64_root_once(e, n, target, t, holes, e[fr], &again)) return 1;
+  if (fabs(again.q - target) < fabs(r->q - target)) {
+    again.retried = 1;
+    again.fb = r->fb;
+    again.fe = r->fe;
+    *r = again;
+  }
+  return 1;
+}
+static int p64_largest_block(const double* e, int n) {
+  int big = 0;
+  for (int b = 0; b < n;) {
+    int en = b + 1;
+    while (en < n && e[en] == e[b]) ++en;
+    if (en - b > 1 && en - b > big) big = en - b;
+    b = en;
+  }
+  return big;
+}
+typedef struct { int b, e; double occ; } p64_fix;
+typedef struct { p64_fix fix[2]; int nfix, relaxed; double rtol, cerr; } p64_pub;
+static double p64_published(const double* e, int i, double t, const p64_root* r, const p64_pub* p) {
+  for (int c = p->nfix; c > 0; --c)
+    if (i >= p->fix[c - 1].b && i < p->fix[c - 1].e) return p->fix[c - 1].occ;
+  return p64_fermi(p64_scaled(e[i], r->ref, t), r->smu);
+}
+static double p64_pub_error(const double* e, int n, double t, const p64_root* r, const p64_pub* p,
+                            int holes, double target) {
+  p64_ksum s = {0.0, 0.0};
+  for (int i = 0; i < n; ++i) {
+    const double o = p64_published(e, i, t, r, p);
+    p64_kadd(&s, holes ? 1.0 - o : o);
+  }
+  return fabs((s.v - target) - s.c);
+}
+typedef struct { double hi, lo; } p64_dd;  /* double-double accumulator */
+static void p64_dd_add(p64_dd* d, double x) {
+  const double s = d->hi + x, v = s - d->hi;
+  const double err = (d->hi - (s - v)) + (x - v) + d->lo;
+  const double h = s + err;
+  d->lo = err - (h - s);
+  d->hi = h;
+}
+static int p64_dd_sign(p64_dd d) {
+  if (d.hi > 0.0) return 1;
+  if (d.hi < 0.0) return -1;
+  return d.lo > 0.0 ? 1 : (d.lo < 0.0 ? -1 : 0);
+}
+static int p64_within(const double* e, int n, double t, const p64_root* r, const p64_pub* p, int holes,
+                      double target, double tol) {
+  p64_dd d = {0.0, 0.0};
+  for (int i = 0; i < n; ++i) {
+    const double o = p64_published(e, i, t, r, p);
+    p64_dd_add(&d, holes ? 1.0 - o : o);
+  }
+  p64_dd_add(&d, -target);
+  p64_dd below = d, above = d;
+  p64_dd_add(&below, -tol);
+  p64_dd_add(&above, tol);
+  return p64_dd_sign(below) <= 0 && p64_dd_sign(above) >= 0;
+}
+static double p64_entropy(const double* e, int n, double t, const p64_root* r, const p64_pub* p) {
+  p64_ksum s = {0.0, 0.0};
+  for (int i = 0; i < n; ++i) {
+    const double o = p64_published(e, i, t, r, p);
+    if (o > 0.0 && o < 1.0) {
+      const double h = 1.0 - o;
+      p64_kadd(&s, -o * log(o) - h * log(h));
+    }
+  }
+  return s.v;
+}
+typedef struct { int found, frac, b, e; double err, occ, tol; } p64_cand;
+static int p64_better(double err, int frac, int b, double occ, const p64_cand* best) {
+  return !best->found || err < best->err || (err == best->err && frac && !best->frac) ||
+         (err == best->err && frac == best->frac && occ < best->occ) ||
+         (err == best->err && frac == best->frac && occ == best->occ && b < best->b);
+}
+static int p64_select(const double* e, int n, double target, double t, int holes, const p64_root* r,
+                      p64_pub* pub) {
+  memset(pub, 0, sizeof(*pub));
+  const double strict = 64.0 * P64_EPS * target;
+  pub->cerr = p64_pub_error(e, n, t, r, pub, holes, target);
+  if (p64_within(e, n, t, r, pub, holes, target, strict)) return 1;
+  for (int phase = 0; phase < 2; ++phase) {
+    p64_cand sc, rc;
+    memset(&sc, 0, sizeof sc);
+    memset(&rc, 0, sizeof rc);
+    for (int b = 0; b < n;) {
+      int en = b + 1;
+      while (en < n && e[en] == e[b]) ++en;
+      const int cnt = en - b;
+      const double old = p64_published(e, b, t, r, pub);
+      p64_ksum other = {0.0, 0.0};
+      for (int i = 0; i < n; ++i) {
+        if (i >= b && i < en) continue;
+        const double o = p64_published(e, i, t, r, pub);
+        p64_kadd(&other, holes ? 1.0 - o : o);
+      }
+      const double want = ((target - other.v) + other.c) / (double)cnt;
+      const double near = holes ? 1.0 - want : want;
+      const double nd = nextafter(near, -P64_MAX), nu = nextafter(near, P64_MAX);
+      const double cands[8] = {near, nd, nu, nextafter(nd, -P64_MAX), nextafter(nu, P64_MAX),
+                               old, nextafter(old, -P64_MAX), nextafter(old, P64_MAX)};
+      for (int ci = 0; ci < 8; ++ci) {
+        const double occ = cands[ci];
+        int dup = 0;
+        for (int pj = 0; pj < ci; ++pj) dup = dup || occ == cands[pj];
+        if (!isfinite(occ) || occ < 0.0 || occ > 1.0 || dup) continue;
+        if (pub->nfix >= 2) continue;
+        p64_pub trial = *pub;
+        trial.fix[trial.nfix].b = b;
+        trial.fix[trial.nfix].e = en;
+        trial.fix[trial.nfix].occ = occ;
+        trial.nfix++;
+        const double err = p64_pub_error(e, n, t, r, &trial, holes, target);
+        const int frac = occ > 0.0 && occ < 1.0;
+        if (p64_within(e, n, t, r, &trial, holes, target, strict) && p64_better(err, frac, b, occ, &sc)) {
+          sc.found = 1; sc.err = err; sc.frac = frac; sc.b = b; sc.e = en; sc.occ = occ; sc.tol = strict;
+        }
+        if (phase == 0 && cnt > 1) {
+          const double rt = 2.0 * P64_EPS * (double)cnt;
+          if (p64_within(e, n, t, r, &trial, holes, target, rt) && p64_better(err, frac, b, occ, &rc)) {
+            rc.found = 1; rc.err = err; rc.frac = frac; rc.b = b; rc.e = en; rc.occ = occ; rc.tol = rt;
+          }
+        }
+      }
+      b = en;
+    }
+    const p64_cand* sel = sc.found ? &sc : &rc;
+    if (!sel->found)
+      return pub->relaxed && p64_within(e, n, t, r, pub, holes, target, pub->rtol);
+    pub->fix[pub->nfix].b = sel->b;
+    pub->fix[pub->nfix].e = sel->e;
+    pub->fix[pub->nfix].occ = sel->occ;
+    pub->nfix++;
+    pub->cerr = sel->err;
+    if (sc.found) {
+      pub->relaxed = 0;
+      pub->rtol = 0.0;
+      break;
+    }
+    pub->relaxed = 1;
+    pub->rtol = sel->tol;
+  }
+  return p64_within(e, n, t, r, pub, holes, target, strict) ||
+         (pub->relaxed && p64_within(e, n, t, r, pub, holes, target, pub->rtol));
+}
+
+/* Fermi occupations of one spin channel (gfn2/eigensolver.cpp:1026-1219):
+ * long-double bisection in the frame translated to the lowest (electrons) or
+ * highest (holes) level; when the rounded occupations fail to conserve the
+ * count, the binary64 policy above takes over (degenerate frontiers only).
+ * Returns 0 on a data failure (the system fails with EIGENSOLVER_FAILED). */
 int oracle_fermi(int32_t n, const double* eps, double nel, double kt, double* occ, double* mu,
                  double* entropy) {
   *mu = 0.0;
@@ -1070,8 +1357,11 @@ int oracle_fermi(int32_t n, const double* eps, double nel, double kt, double* oc
   long double target = nel, cap = n, t = kt;
   if (target == cap) {
     for (int i = 0; i < n; ++i) occ[i] = 1.0;
-    *mu = (double)((long double)eps[n - 1] + 50.0L * t);
-    return 1;
+    long double m = (long double)eps[n - 1] + 50.0L * t;
+    if (m > (long double)DBL_MAX) m = DBL_MAX;
+    if (m < -(long double)DBL_MAX) m = -DBL_MAX;
+    *mu = (double)m;
+    return isfinite(*mu);
   }
   int holes = target > 0.5L * cap;
   long double qt = holes ? cap - target : target;
@@ -1081,6 +1371,11 @@ int oracle_fermi(int32_t n, const double* eps, double nel, double kt, double* oc
   long double scale = fmaxl(1.0L, fabsl(hi0 - lo0));
   long double margin = steps * t + 64.0L * LDBL_EPSILON * scale;
   long double lo = lo0 - margin, hi = hi0 + margin;
+  {
+    const long double ql = fermi_sum(eps, n, ref, lo, t, holes), qh = fermi_sum(eps, n, ref, hi, t, holes);
+    const int brk = holes ? (ql >= qt && qh <= qt) : (ql <= qt && qh >= qt);
+    if (!isfinite(lo) || !isfinite(hi) || !(lo < hi) || !brk) return 0;
+  }
   long double tol = 64.0L * (long double)DBL_EPSILON * qt;
   for (int it = 0; it < 4096; ++it) {
     long double mid = lo + 0.5L * (hi - lo);
@@ -1090,15 +1385,52 @@ int oracle_fermi(int32_t n, const double* eps, double nel, double kt, double* oc
     if ((!holes && q < qt) || (holes && q > qt)) lo = mid; else hi = mid;
   }
   long double m = lo + 0.5L * (hi - lo);
-  *mu = (double)(ref + m);
-  long double s = 0.0L;
-  for (int i = 0; i < n; ++i) {
-    long double f = fermi_val((long double)eps[i] - ref, m, t);
-    occ[i] = (double)f;
-    long double fo = (long double)occ[i];
-    if (fo > 0.0L && fo < 1.0L) s -= fo * logl(fo) + (1.0L - fo) * logl(1.0L - fo);
+  {
+    long double am = ref + m;
+    if (am > (long double)DBL_MAX) am = DBL_MAX;
+    if (am < -(long double)DBL_MAX) am = -DBL_MAX;
+    *mu = (double)am;
   }
-  *entropy = (double)s;
+  long double ideal = 0.0L, published = 0.0L;
+  for (int i = 0; i < n; ++i) {
+    const long double se = (long double)eps[i] - ref;
+    const long double f = fermi_val(se, m, t);
+    ideal += holes ? fermi_hole(se, m, t) : f;
+    occ[i] = (double)f;
+    published += holes ? 1.0L - (long double)occ[i] : (long double)occ[i];
+  }
+  const int ideal_ok = fabsl(ideal - qt) <= tol;
+  const int pub_ok = fabsl(published - qt) <= tol;
+  if (ideal_ok && pub_ok) {
+    long double s = 0.0L;
+    for (int i = 0; i < n; ++i) {
+      long double fo = (long double)occ[i];
+      if (fo > 0.0L && fo < 1.0L) s -= fo * logl(fo) + (1.0L - fo) * logl(1.0L - fo);
+    }
+    *entropy = (double)s;
+    return isfinite(*mu) && isfinite(*entropy);
+  }
+  /* rare path: the binary64 policy */
+  const int big = p64_largest_block(eps, n);
+  if (!ideal_ok && big <= 1) return 0;
+  const double cap64 = (double)n, t64 = holes ? cap64 - nel : nel;
+  p64_root root;
+  p64_pub pub;
+  if (!(t64 > 0.0) || !isfinite(t64) || !p64_solve_root(eps, n, t64, kt, holes, &root)) return 0;
+  if (!p64_select(eps, n, t64, kt, holes, &root, &pub)) return 0;
+  double rtol = 1024.0 * P64_EPS * t64;
+  if (root.fb >= 0 && root.fe > root.fb) rtol = fmax(rtol, 2.0 * P64_EPS * (double)(root.fe - root.fb));
+  if (!isfinite(root.q) || fabs(root.q - t64) > rtol) return 0;
+  *mu = p64_sat_affine(root.ref, root.smu, kt);
+  if (big == n && t64 / cap64 == 0.0) {
+    /* a subnormal total over a fully degenerate block: the analytic logit */
+    const double lf = log(t64) - log(cap64);
+    const double lc = log1p(-exp(lf));
+    *mu = p64_sat_affine(root.ref, holes ? lc - lf : lf - lc, kt);
+  }
+  *entropy = p64_entropy(eps, n, kt, &root, &pub);
+  if (!isfinite(*mu) || !isfinite(*entropy)) return 0;
+  for (int i = 0; i < n; ++i) occ[i] = p64_published(eps, i, kt, &root, &pub);
   return 1;
 }
 

@@ -486,6 +486,8 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, 
     out.dbg_h0 = ext->core_hamiltonian;
     out.dbg_density = ext->density;
     out.dbg_cn = ext->coordination_numbers;
+    out.dbg_levels = ext->orbital_energies;
+    out.dbg_occ = ext->occupations;
     out.dbg_components = ext->energy_components;
     out.kernel_stats = reinterpret_cast<xtb::KernelStat*>(ext->kernel_stats);
     out.kernel_stats_mask = ext->kernel_stats_mask;

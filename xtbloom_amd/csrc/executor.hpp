@@ -94,6 +94,8 @@ struct HostOutputs {
   double* dbg_density = nullptr;    // [sum n^2]
   double* dbg_cn = nullptr;         // [N]
   double* dbg_components = nullptr; // [B*8] core es2 es3 aes2 d4 pc rep atm
+  double* dbg_levels = nullptr;     // [sum n] final-round orbital energies, ascending per system
+  double* dbg_occ = nullptr;        // [sum n] their total occupations
   double* scc_state = nullptr;        // [sum (shells + 9 atoms)] final SCC multipole state
   KernelStat* kernel_stats = nullptr; // [kKernelCount]; enables per-launch events
   uint32_t kernel_stats_mask = 0;     // bit f: events for family f (0 = every family)

@@ -1375,6 +1375,10 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     HIP_TRY(hipMemcpyAsync(out.dbg_density, dP, sizeof(double) * NN, hipMemcpyDeviceToHost, sm));
   if (out.dbg_cn)
     HIP_TRY(hipMemcpyAsync(out.dbg_cn, d_cn, sizeof(double) * N, hipMemcpyDeviceToHost, sm));
+  if (out.dbg_levels)
+    HIP_TRY(hipMemcpyAsync(out.dbg_levels, d_eps, sizeof(double) * NAO, hipMemcpyDeviceToHost, sm));
+  if (out.dbg_occ)
+    HIP_TRY(hipMemcpyAsync(out.dbg_occ, d_occ, sizeof(double) * NAO, hipMemcpyDeviceToHost, sm));
   HIP_TRY(hipStreamSynchronize(sm));
   if (out.scc_state) std::copy(hx.begin(), hx.end(), out.scc_state);
   if (timing) {

@@ -6,6 +6,7 @@
 #include <type_traits>
 
 #include "gfn2_device.hpp"
+#include "gfn2_occupation.hpp"
 
 namespace xtb {
 
@@ -1439,22 +1440,30 @@ __device__ double fermi_sum_wave(const double* eps, int n, double ref, double mu
   return wave_sum(acc);
 }
 
-__device__ void fermi_wave(const double* eps, int n, double nel, double kt, double* occ,
-                           double& entropy) {
+// Occupations of one spin channel on one wavefront.  Returns 0 when the
+// occupations in `occ` are final, 1 when the rare path (fermi_policy) must
+// publish them -- the root missed the count or the rounded occupations miss
+// it by more than 64 eps (checked in double-double across the wave) -- and 2
+// when the count is not bracketed (a data failure), as the reference's
+// compute_occupations decides (eigensolver.cpp:1026-1161).  ideal_ok: the
+// root met the count.
+__device__ int fermi_wave(const double* eps, int n, double nel, double kt, double* occ,
+                          double& entropy, bool& ideal_ok) {
   const int lane = threadIdx.x & 63;
   entropy = 0.0;
-  if (nel == 0.0) {
-    for (int i = lane; i < n; i += 64) occ[i] = 0.0;
-    return;
-  }
+  ideal_ok = true;
   if (kt == 0.0) {
     const int full = (int)floor(nel);
     for (int i = lane; i < n; i += 64) occ[i] = i < full ? 1.0 : (i == full ? nel - full : 0.0);
-    return;
+    return 0;
+  }
+  if (nel == 0.0) {
+    for (int i = lane; i < n; i += 64) occ[i] = 0.0;
+    return 0;
   }
   if (nel == (double)n) {
     for (int i = lane; i < n; i += 64) occ[i] = 1.0;
-    return;
+    return 0;
   }
   const bool holes = nel > 0.5 * n;
   const double qt = holes ? n - nel : nel;
@@ -1463,6 +1472,12 @@ __device__ void fermi_wave(const double* eps, int n, double nel, double kt, doub
   const double lo0 = eps[0] - ref, hi0 = eps[n - 1] - ref;
   const double margin = steps * kt + 64.0 * 2.220446049250313e-16 * fmax(1.0, fabs(hi0 - lo0));
   double lo = lo0 - margin, hi = hi0 + margin;
+  {
+    const double ql = fermi_sum_wave(eps, n, ref, lo, kt, holes);
+    const double qh = fermi_sum_wave(eps, n, ref, hi, kt, holes);
+    const bool brk = holes ? (ql >= qt && qh <= qt) : (ql <= qt && qh >= qt);
+    if (!isfinite(lo) || !isfinite(hi) || !(lo < hi) || !brk) return 2;
+  }
   const double tol = 64.0 * 2.220446049250313e-16 * qt;
   for (int it = 0; it < 4096; ++it) {
     const double mid = lo + 0.5 * (hi - lo);
@@ -1472,14 +1487,40 @@ __device__ void fermi_wave(const double* eps, int n, double nel, double kt, doub
     if ((!holes && q < qt) || (holes && q > qt)) lo = mid; else hi = mid;
   }
   const double mu = lo + 0.5 * (hi - lo);
+  ideal_ok = fabs(fermi_sum_wave(eps, n, ref, mu, kt, holes) - qt) <= tol;
+  // published count in double-double: lane sums, then a butterfly whose
+  // result is taken from lane 0
+  double ph = 0.0, pl = 0.0;
+  auto dd_add = [](double& h, double& l, double x) __attribute__((always_inline)) {
+    const double sm = h + x, v = sm - h;
+    const double err = (h - (sm - v)) + (x - v) + l;
+    const double nh = sm + err;
+    l = err - (nh - sm);
+    h = nh;
+  };
   double sacc = 0.0;
   for (int i = lane; i < n; i += 64) {
     const double a = ((eps[i] - ref) - mu) / kt;
     const double f = a >= 0.0 ? exp(-a) / (1.0 + exp(-a)) : 1.0 / (exp(a) + 1.0);
     occ[i] = f;
+    dd_add(ph, pl, holes ? 1.0 - f : f);
     if (f > 0.0 && f < 1.0) sacc -= f * log(f) + (1.0 - f) * log(1.0 - f);
   }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double oh = __shfl_xor(ph, off, 64), ol = __shfl_xor(pl, off, 64);
+    dd_add(ph, pl, oh);
+    dd_add(ph, pl, ol);
+  }
+  ph = __shfl(ph, 0, 64);
+  pl = __shfl(pl, 0, 64);
+  dd_add(ph, pl, -qt);
+  double bh = ph, bl = pl, ah = ph, al = pl;
+  dd_add(bh, bl, -tol);
+  dd_add(ah, al, tol);
+  auto sgn = [](double h, double l) { return h > 0.0 ? 1 : (h < 0.0 ? -1 : (l > 0.0 ? 1 : (l < 0.0 ? -1 : 0))); };
+  const bool pub_ok = sgn(bh, bl) <= 0 && sgn(ah, al) >= 0;
   entropy = wave_sum(sacc);
+  return ideal_ok && pub_ok ? 0 : 1;
 }
 
 // Occupations are negligible below kOccFloor: orbitals past the last one
@@ -1654,8 +1695,22 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
   if (tid < 64) {
     const int o = t.ao_off[s];
     double sa, sb;
-    fermi_wave(srt, n, t.nalpha[s], fo.kt, fo.fa + o, sa);
-    fermi_wave(srt, n, t.nbeta[s], fo.kt, fo.fb + o, sb);
+    bool ia, ib;
+    int ra = fermi_wave(srt, n, t.nalpha[s], fo.kt, fo.fa + o, sa, ia);
+    int rb = fermi_wave(srt, n, t.nbeta[s], fo.kt, fo.fb + o, sb, ib);
+    // the reference's binary64 rare path, serial on lane 0
+    if (ra == 1 || rb == 1) {
+      if (tid == 0) {
+        if (ra == 1) ra = fermi_policy(srt, n, t.nalpha[s], fo.kt, fo.fa + o, sa) ? 0 : 2;
+        if (rb == 1) rb = fermi_policy(srt, n, t.nbeta[s], fo.kt, fo.fb + o, sb) ? 0 : 2;
+      }
+      __threadfence_block();
+      ra = __shfl(ra, 0, 64);
+      rb = __shfl(rb, 0, 64);
+      sa = __shfl(sa, 0, 64);
+      sb = __shfl(sb, 0, 64);
+    }
+    if ((ra == 2 || rb == 2) && tid == 0) fo.st[s].status = 8;
     int last = 0;
     for (int i = tid; i < n; i += 64) {
       const double f = fo.fa[o + i] + fo.fb[o + i];

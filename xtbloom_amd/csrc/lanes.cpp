@@ -115,7 +115,7 @@ struct Gathered {
   std::vector<int32_t> Z, unpaired, iterations, status;
   std::vector<double> xyz, charges, pc_xyz, pc_q, pc_gamma, warm;
   std::vector<double> energies, gradient, atom_charges, pc_gradient, pc_dedq, dipoles, scc_state,
-      dbg_overlap, dbg_h0, dbg_density, dbg_cn, dbg_components;
+      dbg_overlap, dbg_h0, dbg_density, dbg_cn, dbg_components, dbg_levels, dbg_occ;
   std::vector<uint8_t> converged;
 };
 
@@ -125,7 +125,7 @@ struct Gathered {
 void fill_unsupported(const HostBatch& hb, const std::vector<int32_t>& nao,
                       const std::vector<int32_t>& ndim, HostOutputs& out, int limit) {
   const double nan = std::numeric_limits<double>::quiet_NaN();
-  int64_t state = 0;
+  int64_t state = 0, ao = 0;
   for (int64_t s = 0; s < hb.batch; ++s) {
     const int64_t ns = ndim[s];
     if (nao[s] > limit) {
@@ -150,10 +150,15 @@ void fill_unsupported(const HostBatch& hb, const std::vector<int32_t>& nao,
         }
       if (out.dbg_components)
         for (int c = 0; c < 8; ++c) out.dbg_components[8 * s + c] = nan;
+      for (int64_t i = ao; i < ao + nao[s]; ++i) {
+        if (out.dbg_levels) out.dbg_levels[i] = nan;
+        if (out.dbg_occ) out.dbg_occ[i] = nan;
+      }
       if (out.scc_state)
         for (int64_t i = 0; i < ns; ++i) out.scc_state[state + i] = nan;
     }
     state += ns;
+    ao += nao[s];
   }
 }
 
@@ -188,10 +193,11 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
   }
   const LaneMembers members = contiguous_split(nao, kept, K);
 
-  std::vector<int64_t> mat_prefix(hb.batch + 1, 0), state_prefix(hb.batch + 1, 0);
+  std::vector<int64_t> mat_prefix(hb.batch + 1, 0), state_prefix(hb.batch + 1, 0), ao_prefix(hb.batch + 1, 0);
   for (int64_t s = 0; s < hb.batch; ++s) {
     mat_prefix[s + 1] = mat_prefix[s] + (int64_t)nao[s] * nao[s];
     state_prefix[s + 1] = state_prefix[s] + ndim[s];
+    ao_prefix[s + 1] = ao_prefix[s] + nao[s];
   }
   std::vector<RunOptions> sopt(K, opt);
   std::vector<Slice> sl(K);
@@ -242,6 +248,8 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
       o.dbg_density = at(out.dbg_density, m0);
       o.dbg_cn = at(out.dbg_cn, a0);
       o.dbg_components = at(out.dbg_components, 8 * x.s0);
+      o.dbg_levels = at(out.dbg_levels, ao_prefix[x.s0]);
+      o.dbg_occ = at(out.dbg_occ, ao_prefix[x.s0]);
       o.scc_state = at(out.scc_state, state_prefix[x.s0]);
       sopt[k].warm_state = at(opt.warm_state, state_prefix[x.s0]);
     } else {
@@ -250,7 +258,7 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
       Gathered& g = gat[k];
       g.atom_offsets.assign(1, 0);
       if (hb.point_offsets) g.point_offsets.assign(1, 0);
-      int64_t nmat = 0, nstate = 0;
+      int64_t nmat = 0, nstate = 0, nlev = 0;
       for (int64_t s : m) {
         const int64_t a0 = hb.atom_offsets[s], a1 = hb.atom_offsets[s + 1];
         g.atom_offsets.push_back(g.atom_offsets.back() + (a1 - a0));
@@ -269,6 +277,7 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
           g.warm.insert(g.warm.end(), opt.warm_state + state_prefix[s], opt.warm_state + state_prefix[s + 1]);
         nmat += (int64_t)nao[s] * nao[s];
         nstate += ndim[s];
+        nlev += nao[s];
       }
       const int64_t nb = b.batch, na = g.atom_offsets.back();
       const int64_t np = hb.point_offsets ? g.point_offsets.back() : 0;
@@ -304,6 +313,8 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
       o.dbg_density = own(g.dbg_density, out.dbg_density, nmat);
       o.dbg_cn = own(g.dbg_cn, out.dbg_cn, na);
       o.dbg_components = own(g.dbg_components, out.dbg_components, 8 * nb);
+      o.dbg_levels = own(g.dbg_levels, out.dbg_levels, nlev);
+      o.dbg_occ = own(g.dbg_occ, out.dbg_occ, nlev);
       o.scc_state = own(g.scc_state, out.scc_state, nstate);
       sopt[k].warm_state = opt.warm_state ? g.warm.data() : nullptr;
     }
@@ -344,7 +355,7 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
     const std::vector<int64_t>& m = members[k];
     if (is_contiguous(m)) continue;
     const Gathered& g = gat[k];
-    int64_t ga = 0, gp = 0, gm = 0, gs = 0;
+    int64_t ga = 0, gp = 0, gm = 0, gs = 0, gl = 0;
     for (size_t i = 0; i < m.size(); ++i) {
       const int64_t s = m[i];
       const int64_t a0 = hb.atom_offsets[s], na = hb.atom_offsets[s + 1] - a0;
@@ -369,6 +380,9 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
       put(at(out.dbg_cn, a0), g.dbg_cn, ga, na);
       put(at(out.dbg_components, 8 * s), g.dbg_components, 8 * (int64_t)i, 8);
       put(at(out.scc_state, state_prefix[s]), g.scc_state, gs, ns);
+      put(at(out.dbg_levels, ao_prefix[s]), g.dbg_levels, gl, nao[s]);
+      put(at(out.dbg_occ, ao_prefix[s]), g.dbg_occ, gl, nao[s]);
+      gl += nao[s];
       ga += na;
       gp += np;
       gm += nm;

@@ -316,6 +316,10 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
             ext.coordination_numbers = bufs["coordination_numbers"].ctypes.data
             bufs["energy_components"] = np.zeros((B, 8))
             ext.energy_components = bufs["energy_components"].ctypes.data
+            # packed per system over its n levels; 9 AOs per atom bound the total
+            for name in ("orbital_energies", "occupations"):
+                bufs[name] = np.full(9 * N, np.nan)
+                setattr(ext, name, bufs[name].ctypes.data)
         stats = (library.KernelStat * len(library.KERNEL_FAMILIES))() if kernel_stats else None
         if stats is not None:
             ext.kernel_stats = ctypes.addressof(stats)

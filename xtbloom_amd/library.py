@@ -129,7 +129,8 @@ class Extension(ctypes.Structure):
                 ("setup_ms", ctypes.c_float),
                 ("scc_ms", ctypes.c_float), ("force_ms", ctypes.c_float),
                 ("total_ms", ctypes.c_float), ("scc_rounds", ctypes.c_int32),
-                ("kernel_stats_mask", ctypes.c_uint32)]
+                ("kernel_stats_mask", ctypes.c_uint32),
+                ("orbital_energies", ctypes.c_void_p), ("occupations", ctypes.c_void_p)]
 
 
 class RequestInfo(ctypes.Structure):
