@@ -15,6 +15,13 @@ Fixtures (all data -- inputs and expected outputs):
   mixer_known_answer.json
                      the modified-Broyden known answer of
                      tests/scc_mixer_test.cpp:254-319 (memory 3, damping 0.4).
+  gfn2_parameters_hcno.json
+                     the H, C, N, O entries of the reference's parameter
+                     headers data/parameters/gfn2.hpp (kElements, kShells,
+                     kGlobal) and data/parameters/d4.hpp (kElements,
+                     kReferences, kReferenceC6 among their references): the
+                     elements of the generated workloads, read from a source
+                     independent of the gfn2.json the tables are generated from.
 """
 import json
 import os
@@ -110,6 +117,52 @@ def traces():
     return {"source": "data/conformance/scc-traces (reference)", "cases": out}
 
 
+def _cxx_rows(text, name):
+    """Rows ``{...}`` (or ``Type{...}``) of ``name{{ ... }};`` as lists of numbers."""
+    body = text[text.index(name + "{{") + len(name) + 2:]
+    body = body[:body.index("}};")]
+    return [[float(tok.strip().rstrip("u")) for tok in row.split(",") if tok.strip()]
+            for row in re.findall(r"\{([^{}]*)\}", body)]
+
+
+def parameters():
+    gfn2 = open(os.path.join(REF, "data/parameters/gfn2.hpp")).read()
+    d4 = open(os.path.join(REF, "data/parameters/d4.hpp")).read()
+    elements = _cxx_rows(gfn2, "kElements")
+    shells = _cxx_rows(gfn2, "kShells")
+    d4el = _cxx_rows(d4, "kElements")
+    d4ref = _cxx_rows(d4, "kReferences")
+    c6body = d4[d4.index("kReferenceC6{{") + 14:]
+    c6 = [float(x) for x in re.findall(r"[-+]?(?:\d+\.\d*|\d+)(?:[eE][-+]?\d+)?",
+                                       c6body[:c6body.index("}};")])]
+    nref = len(d4ref)
+    assert len(elements) == 86 and len(shells) == 237 and len(d4el) == 86 and len(c6) == nref * nref
+    out = {"source": "data/parameters/gfn2.hpp, data/parameters/d4.hpp (reference)", "elements": {}}
+    for z in (1, 6, 7, 8):
+        e = elements[z - 1]
+        off, cnt = int(e[0]), int(e[2])
+        assert int(e[1]) == z
+        d = d4el[z - 1]
+        r0, rn = int(d[0]), int(d[1])
+        out["elements"][str(z)] = {
+            "gam": e[3], "gam3": e[4], "zeff": e[5], "arep": e[6], "en": e[8],
+            "dkernel": e[9], "qkernel": e[10], "mprad": e[11], "mpvcn": e[12],
+            "shell_offset": off, "shell_count": cnt,
+            "shells": [{"n": int(sh[0]), "l": int(sh[1]), "ngauss": int(sh[2]), "level_ev": sh[4],
+                        "slater": sh[5], "refocc": sh[6], "kpoly": sh[7], "kcn_ev": sh[8],
+                        "hubbard_scale": sh[9]} for sh in shells[off:off + cnt]],
+            "d4": {"ref_offset": r0, "ref_count": rn, "covalent_radius": d[2], "en": d[3],
+                   "zeff": d[4], "hardness": d[5], "r4r2": d[6],
+                   "ref_cn": [d4ref[r][0] for r in range(r0, r0 + rn)],
+                   "ref_q": [d4ref[r][1] for r in range(r0, r0 + rn)],
+                   "ref_ngauss": [int(d4ref[r][2]) for r in range(r0, r0 + rn)]}}
+    # C6 among the H/C/N/O references (row-major over all references)
+    refs = [r for z in (1, 6, 7, 8) for r in range(int(d4el[z - 1][0]),
+                                                   int(d4el[z - 1][0]) + int(d4el[z - 1][1]))]
+    out["c6"] = [[i, j, c6[i * nref + j]] for i in refs for j in refs]
+    return out
+
+
 def mixer():
     text = open(os.path.join(REF, "tests/scc_mixer_test.cpp")).read()
     body = text[text.index("int test_tblite_trace_and_ring_history()"):]
@@ -145,7 +198,8 @@ def mixer():
 
 def main():
     for name, data in (("conformance.json", conformance()), ("scc_traces.json", traces()),
-                       ("mixer_known_answer.json", mixer())):
+                       ("mixer_known_answer.json", mixer()),
+                       ("gfn2_parameters_hcno.json", parameters())):
         path = os.path.join(OUT, name)
         with open(path, "w") as fh:
             json.dump(data, fh, separators=(",", ":"))
