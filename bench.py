@@ -71,8 +71,13 @@ def cpu_baseline(batch, gpu_outs, sample_stride, opts=None):
     threads = host_threads()
     oracle.lib()
     t0 = time.perf_counter()
+    pcs = {}
+    if sub.point_offsets is not None:
+        pcs = dict(point_offsets=sub.point_offsets, pc_xyz=sub.point_positions,
+                   pc_q=sub.point_charges, pc_gamma=sub.point_gammas)
     ref = oracle.batch(sub.atom_offsets, sub.atomic_numbers, sub.positions,
-                       sub.molecular_charges, sub.unpaired_electrons, opts=opts, threads=threads)
+                       sub.molecular_charges, sub.unpaired_electrons, opts=opts, threads=threads,
+                       **pcs)
     wall = time.perf_counter() - t0
     parity = {}
     for name, gpu_out in gpu_outs.items():

@@ -1440,11 +1440,66 @@ __device__ double fermi_sum_wave(const double* eps, int n, double ref, double mu
   return wave_sum(acc);
 }
 
+// (h, l) += x in double-double (two-sum of the high parts, then renormalized)
+__device__ __forceinline__ void dd_accumulate(double& h, double& l, double x) {
+  const double sm = h + x, v = sm - h;
+  const double err = (h - (sm - v)) + (x - v) + l;
+  const double nh = sm + err;
+  l = err - (nh - sm);
+  h = nh;
+}
+
+// double-double sum over the wavefront, every lane receiving lane 0's result
+__device__ __forceinline__ double dd_wave_excess(double h, double l, double qt) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double oh = __shfl_xor(h, off, 64), ol = __shfl_xor(l, off, 64);
+    dd_accumulate(h, l, oh);
+    dd_accumulate(h, l, ol);
+  }
+  h = __shfl(h, 0, 64);
+  l = __shfl(l, 0, 64);
+  dd_accumulate(h, l, -qt);
+  return h + l;
+}
+
+// The counts of one spin channel at mu minus the target qt, both accumulated
+// in double-double over the wavefront:
+//   ideal  the occupations before rounding (holes: the hole occupations
+//          1 / (e^-a + 1) as such), to the accuracy of exp and the division;
+//   pub    the binary64 occupations f that are published (holes: 1 - f,
+//          taken exactly).
+// For electrons the two take the same terms.  The reference's long-double
+// main path accepts its root when both lie within 64 eps qt of the target
+// (eigensolver.cpp:1135-1161); with both counts this accurate the bisection
+// below applies that same acceptance at every step, so the published count
+// meets the target whenever a root is accepted.
+__device__ __forceinline__ void fermi_counts_wave(const double* eps, int n, double ref, double mu,
+                                                  double kt, bool holes, double qt, double& ideal,
+                                                  double& pub) {
+  double ih = 0.0, il = 0.0, ph = 0.0, pl = 0.0;
+  for (int i = threadIdx.x & 63; i < n; i += 64) {
+    const double a = ((eps[i] - ref) - mu) / kt;
+    const double x = exp(-fabs(a));
+    const double big = 1.0 / (x + 1.0), small = x / (1.0 + x);
+    const double f = a >= 0.0 ? small : big;
+    if (holes) {
+      dd_accumulate(ih, il, a >= 0.0 ? big : small);
+      const double h = 1.0 - f;
+      dd_accumulate(ph, pl, h);
+      pl += (1.0 - h) - f;  // the rounding error of 1 - f, exact for 0 <= f <= 1
+    } else {
+      dd_accumulate(ph, pl, f);
+    }
+  }
+  pub = dd_wave_excess(ph, pl, qt);
+  ideal = holes ? dd_wave_excess(ih, il, qt) : pub;
+}
+
 // Occupations of one spin channel on one wavefront.  Returns 0 when the
 // occupations in `occ` are final, 1 when the rare path (fermi_policy) must
-// publish them -- the root missed the count or the rounded occupations miss
-// it by more than 64 eps (checked in double-double across the wave) -- and 2
-// when the count is not bracketed (a data failure), as the reference's
+// publish them -- no binary64 chemical potential puts both the count before
+// rounding and the published count within 64 eps of the target -- and 2 when
+// the count is not bracketed (a data failure), as the reference's
 // compute_occupations decides (eigensolver.cpp:1026-1161).  ideal_ok: the
 // root met the count.
 __device__ int fermi_wave(const double* eps, int n, double nel, double kt, double* occ,
@@ -1479,46 +1534,31 @@ __device__ int fermi_wave(const double* eps, int n, double nel, double kt, doubl
     if (!isfinite(lo) || !isfinite(hi) || !(lo < hi) || !brk) return 2;
   }
   const double tol = 64.0 * 2.220446049250313e-16 * qt;
+  double xi = 0.0, xp = 0.0;
+  bool met = false;
   for (int it = 0; it < 4096; ++it) {
     const double mid = lo + 0.5 * (hi - lo);
-    const double q = fermi_sum_wave(eps, n, ref, mid, kt, holes);
-    if (fabs(q - qt) <= tol) { lo = hi = mid; break; }
+    fermi_counts_wave(eps, n, ref, mid, kt, holes, qt, xi, xp);
+    if (fabs(xi) <= tol && fabs(xp) <= tol) {
+      lo = hi = mid;
+      met = true;
+      break;
+    }
     if (mid == lo || mid == hi) break;
-    if ((!holes && q < qt) || (holes && q > qt)) lo = mid; else hi = mid;
+    if ((!holes && xi < 0.0) || (holes && xi > 0.0)) lo = mid; else hi = mid;
   }
   const double mu = lo + 0.5 * (hi - lo);
-  ideal_ok = fabs(fermi_sum_wave(eps, n, ref, mu, kt, holes) - qt) <= tol;
-  // published count in double-double: lane sums, then a butterfly whose
-  // result is taken from lane 0
-  double ph = 0.0, pl = 0.0;
-  auto dd_add = [](double& h, double& l, double x) __attribute__((always_inline)) {
-    const double sm = h + x, v = sm - h;
-    const double err = (h - (sm - v)) + (x - v) + l;
-    const double nh = sm + err;
-    l = err - (nh - sm);
-    h = nh;
-  };
+  if (!met) fermi_counts_wave(eps, n, ref, mu, kt, holes, qt, xi, xp);
+  ideal_ok = fabs(xi) <= tol;
+  const bool pub_ok = fabs(xp) <= tol;
   double sacc = 0.0;
   for (int i = lane; i < n; i += 64) {
     const double a = ((eps[i] - ref) - mu) / kt;
-    const double f = a >= 0.0 ? exp(-a) / (1.0 + exp(-a)) : 1.0 / (exp(a) + 1.0);
+    const double x = exp(-fabs(a));
+    const double f = a >= 0.0 ? x / (1.0 + x) : 1.0 / (x + 1.0);
     occ[i] = f;
-    dd_add(ph, pl, holes ? 1.0 - f : f);
     if (f > 0.0 && f < 1.0) sacc -= f * log(f) + (1.0 - f) * log(1.0 - f);
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    const double oh = __shfl_xor(ph, off, 64), ol = __shfl_xor(pl, off, 64);
-    dd_add(ph, pl, oh);
-    dd_add(ph, pl, ol);
-  }
-  ph = __shfl(ph, 0, 64);
-  pl = __shfl(pl, 0, 64);
-  dd_add(ph, pl, -qt);
-  double bh = ph, bl = pl, ah = ph, al = pl;
-  dd_add(bh, bl, -tol);
-  dd_add(ah, al, tol);
-  auto sgn = [](double h, double l) { return h > 0.0 ? 1 : (h < 0.0 ? -1 : (l > 0.0 ? 1 : (l < 0.0 ? -1 : 0))); };
-  const bool pub_ok = sgn(bh, bl) <= 0 && sgn(ah, al) >= 0;
   entropy = wave_sum(sacc);
   return ideal_ok && pub_ok ? 0 : 1;
 }
@@ -1691,36 +1731,50 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
     wall[t.ao_off[s] + r] = lj;
   }
   __syncthreads();
-  // ---- occupations (first wavefront) ----
+  // ---- occupations: alpha on the first wavefront, beta on the second (a
+  // closed shell solves once and copies) ----
+  __shared__ int chan_r[2];
+  __shared__ double chan_s[2];
+  const bool same_channels = t.nalpha[s] == t.nbeta[s];
+  if (tid < 128 && (tid < 64 || !same_channels)) {
+    const int w = tid >> 6;
+    const int o = t.ao_off[s];
+    const double nel = w == 0 ? t.nalpha[s] : t.nbeta[s];
+    double* f = (w == 0 ? fo.fa : fo.fb) + o;
+    double sw;
+    bool ideal;
+    int r = fermi_wave(srt, n, nel, fo.kt, f, sw, ideal);
+    // the reference's binary64 rare path, serial on lane 0
+    if (r == 1) {
+      if ((tid & 63) == 0) r = fermi_policy(srt, n, nel, fo.kt, f, sw) ? 0 : 2;
+      __threadfence_block();
+      r = __shfl(r, 0, 64);
+      sw = __shfl(sw, 0, 64);
+    }
+    if ((tid & 63) == 0) {
+      chan_r[w] = r;
+      chan_s[w] = sw;
+    }
+  }
+  __syncthreads();
   if (tid < 64) {
     const int o = t.ao_off[s];
-    double sa, sb;
-    bool ia, ib;
-    int ra = fermi_wave(srt, n, t.nalpha[s], fo.kt, fo.fa + o, sa, ia);
-    int rb = fermi_wave(srt, n, t.nbeta[s], fo.kt, fo.fb + o, sb, ib);
-    // the reference's binary64 rare path, serial on lane 0
-    if (ra == 1 || rb == 1) {
-      if (tid == 0) {
-        if (ra == 1) ra = fermi_policy(srt, n, t.nalpha[s], fo.kt, fo.fa + o, sa) ? 0 : 2;
-        if (rb == 1) rb = fermi_policy(srt, n, t.nbeta[s], fo.kt, fo.fb + o, sb) ? 0 : 2;
-      }
-      __threadfence_block();
-      ra = __shfl(ra, 0, 64);
-      rb = __shfl(rb, 0, 64);
-      sa = __shfl(sa, 0, 64);
-      sb = __shfl(sb, 0, 64);
-    }
-    if ((ra == 2 || rb == 2) && tid == 0) fo.st[s].status = 8;
+    const int rb = same_channels ? chan_r[0] : chan_r[1];
+    const double sb = same_channels ? chan_s[0] : chan_s[1];
+    if ((chan_r[0] == 2 || rb == 2) && tid == 0) fo.st[s].status = 8;
     int last = 0;
     for (int i = tid; i < n; i += 64) {
-      const double f = fo.fa[o + i] + fo.fb[o + i];
+      const double fa = fo.fa[o + i];
+      const double fb = same_channels ? fa : fo.fb[o + i];
+      if (same_channels) fo.fb[o + i] = fa;
+      const double f = fa + fb;
       fo.occ[o + i] = f;
       fo.fe[o + i] = f * srt[i];
       if (f > kOccFloor) last = max(last, i + 1);
     }
     for (int off = 32; off > 0; off >>= 1) last = max(last, __shfl_xor(last, off, 64));
     if (tid == 0) {
-      fo.st[s].entropy = sa + sb;
+      fo.st[s].entropy = chan_s[0] + sb;
       fo.st[s].kocc = last;
       fo.kocc[s] = last;
       kocc_sh = last;
