@@ -561,9 +561,11 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   std::vector<FockTile> chunks;
   std::vector<int64_t> part_off(B + 1, 0);
   size_t back_lds = 0;
+  int n_large = 0;  // systems above 256 AOs (a prefix of the size order)
   for (int i = 0; i < B; ++i) {
     const int s = T.order[i];
     const int n = T.ao_off[s + 1] - T.ao_off[s];
+    n_large += n > 256;
     const int nt = (n + kTile - 1) / kTile;
     for (int a = 0; a < nt; ++a)
       for (int b = 0; b < nt; ++b) {
@@ -1119,8 +1121,14 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     if (rp) rp->red2 = last_probe();
     // eigenpairs of H' (A is overwritten by the reflectors; T, H, C, P, W serve as scratch)
     HIP_TRY(probe_begin(kKTridiag));
-    hipLaunchKernelGGL(k_tridiag, dim3(B), dim3(512), 0, sm, t, d_order, dA, d_td, d_te, d_wyT,
-                       d_active);
+    // systems are ordered largest first: the n_large above 256 AOs take the
+    // blocked reduction, the rest the register one
+    if (n_large > 0)
+      hipLaunchKernelGGL(k_tridiag_blocked, dim3(n_large), dim3(512), 0, sm, t, d_order, dA, d_td,
+                         d_te, dT, d_active);
+    if (n_large < B)
+      hipLaunchKernelGGL(k_tridiag, dim3(B - n_large), dim3(512), 0, sm, t, d_order + n_large, dA,
+                         d_td, d_te, d_wyT, d_active);
     HIP_TRY(probe_end(kKTridiag, 0.0, 0.0));
     if (rp) rp->tri = last_probe();
     HIP_TRY(probe_begin(kKTridiagEigen));

@@ -332,63 +332,350 @@ constexpr double kSafeMin = 2.2250738585072014e-308;
 constexpr double kClusterTol = 1e-6;  // relative gap (to the block 1-norm) below which vectors are orthogonalized
 constexpr int kInverseIterations = 3;
 
-// Householder reduction with the trailing matrix in global memory (any n).
-__device__ __forceinline__ void tridiag_global(double* A, int n, double* d, double* e, double* v, double* pq,
-                               double* red, double* bc) {
+// Householder reduction for n > 256 in panels of kTriPanel reflectors, the
+// arrangement of LAPACK's dsytrd/dlatrd.  Within a panel the trailing matrix
+// is not written: after reflectors 0 .. j-1 of the panel it equals
+//   A_j = A_0 - V W^T - W V^T,
+// V holding the panel's unit reflectors v_i (H_i = I - 2 v_i v_i^T, stored
+// in A's columns below the subdiagonal as the register path stores them) and
+// W the vectors w_i = 2 (p_i - (v_i^T p_i) v_i), p_i = A_i v_i (in the
+// scratch matrix W, same layout).  Reflector j therefore reads column k of A_0
+// corrected by V and W, the lower triangle of A_0's trailing block once (the
+// symmetric product), and V and W twice; the trailing lower triangle is
+// rewritten once per panel, on the matrix cores.  Only the lower triangle of
+// A is read or written.  512 threads, n <= kMaxAO.
+//
+// Every phase issues all of its global loads before the first use (fully
+// unrolled, out-of-range lanes clamped to valid addresses and their values
+// selected away), so that a wavefront waits once per column pair rather than
+// once per 64-row slot.
+constexpr int kTriPanel = 16;
+// LDS: `part` >= 8 kMaxAO doubles, `aux` >= kMaxAO + 4 kTriPanel doubles
+constexpr int kTriAux = kMaxAO + 4 * kTriPanel;
+
+// One unit of the symmetric product p = A v over an m x m trailing block
+// (column-major at `blk`, leading dimension n, lower triangle): the columns
+// c = cu .. cu + 2 kTriPairs - 1 (cu a multiple of 2 kTriPairs, so that all
+// start in row slot t0 = cu / 64), rows 64 (t0 + u) + lane, u < NS.  Column
+// c adds a_rc x_c to the row partials (registers, flushed to the wavefront's
+// LDS row `prow` at the end) and its dot with x over the rows below the
+// diagonal, plus the diagonal term, to pcol[c].  The next column pair is
+// fetched while the current one is consumed.
+//
+// Rows past m read row m - 1 (a valid address) and meet x_r = 0, so they
+// drop out of the dots without masks, and their row partials are never read;
+// only slot 0, which holds the diagonal, is masked.  Column bases are
+// wavefront-uniform and the row offsets 32-bit, so each load is one
+// scalar-base instruction.
+constexpr int kTriPairs = 8;
+
+// Wavefront sums of two values at once: one exchange across the halves
+// leaves column 0's partials in lanes 0..31 and column 1's in lanes 32..63,
+// then five DPP steps sum each half (results in lanes 31 and 63).
+__device__ __forceinline__ void wave_sum_pair(double d0, double d1, double& s0, double& s1) {
+  const bool hi = (threadIdx.x & 32) != 0;
+  double v = (hi ? d1 : d0) + __shfl_xor(hi ? d0 : d1, 32, 64);
+  v += dpp_move<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x124, 0xf>(v);  // row_ror:4
+  v += dpp_move<0x128, 0xf>(v);  // row_ror:8
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  const long long b = __double_as_longlong(v);
+  s0 = __longlong_as_double(((long long)__builtin_amdgcn_readlane((int)(b >> 32), 31) << 32) |
+                            (unsigned int)__builtin_amdgcn_readlane((int)b, 31));
+  s1 = __longlong_as_double(((long long)__builtin_amdgcn_readlane((int)(b >> 32), 63) << 32) |
+                            (unsigned int)__builtin_amdgcn_readlane((int)b, 63));
+}
+
+template <int NS>
+__device__ __forceinline__ void symv_fetch(const double* __restrict__ blk, int64_t n, int m, int c,
+                                           const unsigned (&boff)[NS], double (&f0)[NS], double (&f1)[NS]) {
+  const char* p0 = reinterpret_cast<const char*>(blk + (int64_t)min(c, m - 1) * n);
+  const char* p1 = reinterpret_cast<const char*>(blk + (int64_t)min(c + 1, m - 1) * n);
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    f0[u] = *reinterpret_cast<const double*>(p0 + boff[u]);
+    f1[u] = *reinterpret_cast<const double*>(p1 + boff[u]);
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ void symv_consume(int c, int m, int r0, const double* __restrict__ x,
+                                             const double (&xr)[NS], const double (&f0)[NS],
+                                             const double (&f1)[NS], double (&acc)[NS],
+                                             double* __restrict__ pcol) {
+  const double xc0 = c < m ? x[c] : 0.0;
+  const double xc1 = c + 1 < m ? x[c + 1] : 0.0;
+  // slot 0: strictly lower rows, and the diagonal on the lane that holds it
+  const double e0 = r0 > c ? f0[0] : 0.0;
+  const double e1 = r0 > c + 1 ? f1[0] : 0.0;
+  double d0 = fma(e0, xr[0], r0 == c ? f0[0] * xc0 : 0.0);
+  double d1 = fma(e1, xr[0], r0 == c + 1 ? f1[0] * xc1 : 0.0);
+  acc[0] = fma(e0, xc0, fma(e1, xc1, acc[0]));
+#pragma unroll
+  for (int u = 1; u < NS; ++u) {
+    d0 = fma(f0[u], xr[u], d0);
+    d1 = fma(f1[u], xr[u], d1);
+    acc[u] = fma(f0[u], xc0, fma(f1[u], xc1, acc[u]));
+  }
+  wave_sum_pair(d0, d1, d0, d1);
+  if ((threadIdx.x & 63) == 0) {
+    if (c < m) pcol[c] = d0;
+    if (c + 1 < m) pcol[c + 1] = d1;
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ void symv_unit(const double* __restrict__ blk, int64_t n, int m, int cu,
+                                          const double* __restrict__ x, double* __restrict__ prow,
+                                          double* __restrict__ pcol) {
+  const int lane = threadIdx.x & 63;
+  const int t0 = cu >> 6;
+  const int r0 = 64 * t0 + lane;  // this lane's row in slot 0
+  unsigned boff[NS];              // byte offsets of this lane's rows in a column
+  double xr[NS], acc[NS];
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const int rr = r0 + 64 * u;
+    const int rc = min(rr, m - 1);
+    boff[u] = (unsigned)rc * 8u;
+    xr[u] = rr < m ? x[rc] : 0.0;
+    acc[u] = 0.0;
+  }
+  // two column pairs per trip, in two buffers: pair q + 1 is in flight while
+  // pair q is consumed, with no register copies between trips
+  double a0[NS], a1[NS], b0[NS], b1[NS];
+  symv_fetch<NS>(blk, n, m, cu, boff, a0, a1);
+#pragma unroll 1
+  for (int q = 0; q < kTriPairs; q += 2) {
+    const int c = cu + 2 * q;
+    symv_fetch<NS>(blk, n, m, c + 2, boff, b0, b1);
+    symv_consume<NS>(c, m, r0, x, xr, a0, a1, acc, pcol);
+    if (q + 2 < kTriPairs) symv_fetch<NS>(blk, n, m, c + 4, boff, a0, a1);
+    symv_consume<NS>(c + 2, m, r0, x, xr, b0, b1, acc, pcol);
+  }
+#pragma unroll
+  for (int u = 0; u < NS; ++u) prow[r0 + 64 * u] += acc[u];
+}
+
+template <int NS>
+__device__ __forceinline__ void dots_pair(const double* __restrict__ col0, const double* __restrict__ col1,
+                                          int m, const double* __restrict__ x, double* __restrict__ out0,
+                                          double* __restrict__ out1) {
+  const int lane = threadIdx.x & 63;
+  double a0[NS], a1[NS];
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const int rr = min(64 * u + lane, m - 1);
+    a0[u] = col0[rr];
+    a1[u] = col1[rr];
+  }
+  double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    const int rr = 64 * u + lane;
+    const double xv = rr < m ? x[min(rr, m - 1)] : 0.0;
+    d0 = fma(a0[u], xv, d0);
+    d1 = fma(a1[u], xv, d1);
+  }
+  wave_sum_pair(d0, d1, d0, d1);
+  if (lane == 0) {
+    *out0 = d0;
+    *out1 = d1;
+  }
+}
+
+#define XTB_SLOT_SWITCH(NSV, CALL)                                                   \
+  switch (NSV) {                                                                     \
+    case 1: { constexpr int NS = 1; CALL; } break;                                   \
+    case 2: { constexpr int NS = 2; CALL; } break;                                   \
+    case 3: { constexpr int NS = 3; CALL; } break;                                   \
+    case 4: { constexpr int NS = 4; CALL; } break;                                   \
+    case 5: { constexpr int NS = 5; CALL; } break;                                   \
+    case 6: { constexpr int NS = 6; CALL; } break;                                   \
+    case 7: { constexpr int NS = 7; CALL; } break;                                   \
+    case 8: { constexpr int NS = 8; CALL; } break;                                   \
+    case 9: { constexpr int NS = 9; CALL; } break;                                   \
+    case 10: { constexpr int NS = 10; CALL; } break;                                 \
+    case 11: { constexpr int NS = 11; CALL; } break;                                 \
+    case 12: { constexpr int NS = 12; CALL; } break;                                 \
+    case 13: { constexpr int NS = 13; CALL; } break;                                 \
+    case 14: { constexpr int NS = 14; CALL; } break;                                 \
+    case 15: { constexpr int NS = 15; CALL; } break;                                 \
+    case 16: { constexpr int NS = 16; CALL; } break;                                 \
+    default: break;                                                                  \
+  }
+static_assert(kMaxAO == 16 * 64, "the slot switch covers up to 16 slots of 64 rows");
+
+__device__ __forceinline__ void tridiag_blocked(double* __restrict__ A, double* __restrict__ W, int n,
+                                                double* d, double* e, double* v, double* pq,
+                                                double* part, double* aux, double* red, double* bc) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int k = 0; k + 2 < n; ++k) {
-    const int m = n - k - 1;
-    double ss = 0.0;
-    for (int i = tid; i < m; i += nt) {
-      const double x = A[(k + 1 + i) + (int64_t)k * n];
-      v[i] = x;
-      ss += x * x;
-    }
-    ss = block_sum(ss, red);
-    if (tid == 0) {
-      const double x0 = v[0];
-      const double xn = sqrt(ss);
-      d[k] = A[k + (int64_t)k * n];
-      if (xn == 0.0 || ss - x0 * x0 == 0.0) {
-        e[k] = x0;  // already reduced: no reflection
-        bc[0] = 0.0;
-      } else {
-        const double alpha = x0 >= 0.0 ? -xn : xn;
-        const double v0 = x0 - alpha;
-        const double vv = ss - x0 * x0 + v0 * v0;
-        e[k] = alpha;
-        v[0] = v0;
-        bc[0] = 1.0 / sqrt(vv);
+  // the wavefront index is uniform: keep it (and the column indices derived
+  // from it) in scalar registers
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nt >> 6;
+  double* pcol = aux;                   // [kMaxAO] column dots of the symmetric product
+  double* rowk = aux + kMaxAO;          // [2 kTriPanel] row k of V, of W
+  double* yv = rowk + 2 * kTriPanel;    // [2 kTriPanel] W^T v, V^T v
+  double* prow = part + wv * kMaxAO;    // this wavefront's row partials
+  auto at = [n](int r, int c) __attribute__((always_inline)) { return (int64_t)r + (int64_t)c * n; };
+
+  for (int k0 = 0; k0 + 2 < n; k0 += kTriPanel) {
+    const int jn = min(kTriPanel, n - 2 - k0);
+    for (int j = 0; j < jn; ++j) {
+      const int k = k0 + j, b = k + 1, m = n - b;
+      if (tid < kTriPanel) {
+        const bool on = tid < j;
+        const int ct = min(k0 + tid, n - 1);
+        const double vk = A[at(k, ct)], wk = W[at(k, ct)];
+        rowk[tid] = on ? vk : 0.0;
+        rowk[kTriPanel + tid] = on ? wk : 0.0;
       }
-    }
-    __syncthreads();
-    const double inv = bc[0];
-    if (inv == 0.0) {
-      for (int i = tid; i < m; i += nt) A[(k + 1 + i) + (int64_t)k * n] = 0.0;
       __syncthreads();
-      continue;
+      // column k of A_j, rows k .. n-1 (the panel's later columns are read and
+      // weighted by zero: one wait for all loads)
+      double ss = 0.0;
+      for (int r = k + tid; r < n; r += nt) {
+        double av[kTriPanel], wv_[kTriPanel];
+#pragma unroll
+        for (int i = 0; i < kTriPanel; ++i) {
+          av[i] = A[at(r, min(k0 + i, n - 1))];
+          wv_[i] = W[at(r, min(k0 + i, n - 1))];
+        }
+        double x = A[at(r, k)];
+#pragma unroll
+        for (int i = 0; i < kTriPanel; ++i) {
+          const bool on = i < j;
+          x -= (on ? av[i] : 0.0) * rowk[kTriPanel + i] + (on ? wv_[i] : 0.0) * rowk[i];
+        }
+        if (r == k) {
+          d[k] = x;
+        } else {
+          v[r - b] = x;
+          ss += x * x;
+        }
+      }
+      ss = block_sum(ss, red);
+      if (tid == 0) {
+        const double x0 = v[0];
+        const double xn = sqrt(ss);
+        if (xn == 0.0 || ss - x0 * x0 == 0.0) {
+          e[k] = x0;  // already reduced: no reflection
+          bc[0] = 0.0;
+        } else {
+          const double alpha = x0 >= 0.0 ? -xn : xn;
+          const double v0 = x0 - alpha;
+          const double vv = ss - x0 * x0 + v0 * v0;
+          e[k] = alpha;
+          v[0] = v0;
+          bc[0] = 1.0 / sqrt(vv);
+        }
+      }
+      __syncthreads();
+      const double inv = bc[0];
+      if (inv == 0.0) {
+        for (int i = tid; i < m; i += nt) {
+          A[at(b + i, k)] = 0.0;
+          W[at(b + i, k)] = 0.0;
+        }
+        __syncthreads();
+        continue;
+      }
+      const int tend = (m + 63) >> 6;
+      for (int i = tid; i < m; i += nt) {
+        const double vi = v[i] * inv;
+        v[i] = vi;
+        A[at(b + i, k)] = vi;
+      }
+      for (int i = lane; i < 64 * tend; i += 64) prow[i] = 0.0;
+      __syncthreads();
+      // p = A_0 v over the trailing block's lower triangle in units of
+      // 2 kTriPairs columns, handed out in a snake over the wavefronts (the
+      // units' work falls with their first column), then the 2j dots W^T v
+      // and V^T v on the wavefront after the last unit's
+      const int nunits = (m + 2 * kTriPairs - 1) / (2 * kTriPairs);
+      for (int u0 = 0; u0 < nunits; u0 += nw) {
+        const int u = u0 + (((u0 / nw) & 1) ? nw - 1 - wv : wv);
+        if (u >= nunits) continue;
+        const int cu = 2 * kTriPairs * u;
+        const double* blk = A + at(b, b);
+        XTB_SLOT_SWITCH(tend - (cu >> 6), (symv_unit<NS>(blk, n, m, cu, v, prow, pcol)))
+      }
+      for (int q = (wv + nw - nunits % nw) % nw; q < j; q += nw)
+        XTB_SLOT_SWITCH(tend, (dots_pair<NS>(W + at(b, k0 + q), A + at(b, k0 + q), m, v, yv + q,
+                                              yv + kTriPanel + q)))
+      __syncthreads();
+      // p_j = A_j v = p - V (W^T v) - W (V^T v);  K = v^T p_j
+      double kp = 0.0;
+      for (int i = tid; i < m; i += nt) {
+        double av[kTriPanel], wv_[kTriPanel];
+#pragma unroll
+        for (int q = 0; q < kTriPanel; ++q) {
+          av[q] = A[at(b + i, min(k0 + q, n - 1))];
+          wv_[q] = W[at(b + i, min(k0 + q, n - 1))];
+        }
+        double p = pcol[i];
+        for (int w = 0; w < nw; ++w) p += part[w * kMaxAO + i];
+#pragma unroll
+        for (int q = 0; q < kTriPanel; ++q) {
+          const bool on = q < j;
+          p -= (on ? av[q] : 0.0) * (on ? yv[q] : 0.0) + (on ? wv_[q] : 0.0) * (on ? yv[kTriPanel + q] : 0.0);
+        }
+        pq[i] = p;
+        kp += v[i] * p;
+      }
+      const double K = block_sum(kp, red);
+      for (int i = tid; i < m; i += nt) W[at(b + i, k)] = 2.0 * (pq[i] - K * v[i]);
+      __syncthreads();
     }
-    for (int i = tid; i < m; i += nt) {
-      v[i] *= inv;
-      A[(k + 1 + i) + (int64_t)k * n] = v[i];
-    }
-    __syncthreads();
-    // p = A22 v, q = p - (v^T p) v
-    double kp = 0.0;
-    for (int i = tid; i < m; i += nt) {
-      double p = 0.0;
-      for (int j = 0; j < m; ++j) p += A[(k + 1 + i) + (int64_t)(k + 1 + j) * n] * v[j];
-      pq[i] = p;
-      kp += v[i] * p;
-    }
-    const double K = block_sum(kp, red);
-    for (int i = tid; i < m; i += nt) pq[i] -= K * v[i];
-    __syncthreads();
-    // A22 -= 2 (v q^T + q v^T)
-    for (int j = 0; j < m; ++j) {
-      const double vj = v[j], qj = pq[j];
-      for (int i = tid; i < m; i += nt)
-        A[(k + 1 + i) + (int64_t)(k + 1 + j) * n] -= 2.0 * (v[i] * qj + pq[i] * vj);
+    // trailing lower triangle, rows and columns >= c0:
+    //   A -= V W^T + W V^T = [V W] [W V]^T
+    // in 16 x 16 tiles on the matrix cores (K = 2 kTriPanel).  The product is
+    // formed transposed, so that a lane's outputs run along a column of A
+    // (row = l16, column = lk + 4 r) and the update reads and writes whole
+    // column segments.
+    const int c0 = k0 + jn, mt = n - c0;
+    const int nb16 = (mt + 15) >> 4;
+    const int ntiles = nb16 * (nb16 + 1) / 2;
+    const int l16 = lane & 15, lk = lane >> 4;
+    for (int tile = wv; tile < ntiles; tile += nw) {
+      // tile -> (TI, TJ), TI >= TJ, row-major over the lower triangle
+      int TI = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+      while (TI * (TI + 1) / 2 > tile) --TI;
+      while ((TI + 1) * (TI + 2) / 2 <= tile) ++TI;
+      const int TJ = tile - TI * (TI + 1) / 2;
+      const int rr = 16 * TI + l16, cc = 16 * TJ + l16;  // this lane's operand row and column
+      const int rrc = c0 + min(rr, mt - 1), ccc = c0 + min(cc, mt - 1);
+      double xa[2 * kTriPanel / 4], xb[2 * kTriPanel / 4];
+#pragma unroll
+      for (int q = 0; q < kTriPanel / 4; ++q) {
+        const int i = 4 * q + lk;
+        const int ci = k0 + min(i, jn - 1);
+        // A operand: column side [W V] at row cc; B operand: row side [V W] at row rr
+        xa[q] = W[at(ccc, ci)];
+        xa[kTriPanel / 4 + q] = A[at(ccc, ci)];
+        xb[q] = A[at(rrc, ci)];
+        xb[kTriPanel / 4 + q] = W[at(rrc, ci)];
+      }
+      mfma_d4 acc = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < 2 * kTriPanel / 4; ++q) {
+        const bool on = 4 * (q % (kTriPanel / 4)) + lk < jn;
+        acc = mfma_16x16x4(on && cc < mt ? xa[q] : 0.0, on && rr < mt ? xb[q] : 0.0, acc);
+      }
+      // acc[r] = D[row = 16 TI + l16][col = 16 TJ + lk + 4 r]
+      const int row = 16 * TI + l16;
+      double old[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = 16 * TJ + lk + 4 * r;
+        old[r] = A[at(c0 + min(row, mt - 1), c0 + min(col, mt - 1))];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = 16 * TJ + lk + 4 * r;
+        if (row < mt && col < mt && row >= col) A[at(c0 + row, c0 + col)] = old[r] - acc[r];
+      }
     }
     __syncthreads();
   }
@@ -739,7 +1026,8 @@ __device__ __forceinline__ void wy_factors(const double* A, int n, double* Ts, d
   __syncthreads();
 }
 
-// 512 threads per system (two waves per SIMD, 256 VGPRs per thread)
+// 512 threads per system (two waves per SIMD, 256 VGPRs per thread), n <= 256:
+// the systems order[blockIdx.x] (the host launches the small systems only)
 __global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, double* Aall, double* dall,
                                                  double* eall, double* Tall,
                                                  const unsigned char* active) {
@@ -754,19 +1042,35 @@ __global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, doubl
   __shared__ double col[512];  // published column of the register path
   __shared__ double rowpart[32 * kTriPad];
   __shared__ double colpart[16 * kTriPad];
-  __shared__ double red[16];
-  __shared__ double bc[2];
   if (n <= 64) tridiag_registers<4>(A, n, d, e, v, col, pq, rowpart, colpart);
   else if (n <= 128) tridiag_registers<8>(A, n, d, e, v, col, pq, rowpart, colpart);
   else if (n <= 192) tridiag_registers<12>(A, n, d, e, v, col, pq, rowpart, colpart);
-  else if (n <= 256) tridiag_registers<16>(A, n, d, e, v, col, pq, rowpart, colpart);
-  else tridiag_global(A, n, d, e, v, pq, red, bc);
-  // WY factors for the register back-transformation (n <= 256); the stored
-  // reflectors must be visible to the whole block first
-  if (n <= 256 && n > 2) {
+  else tridiag_registers<16>(A, n, d, e, v, col, pq, rowpart, colpart);
+  // WY factors for the register back-transformation; the stored reflectors
+  // must be visible to the whole block first
+  if (n > 2) {
     __syncthreads();
     wy_factors(A, n, Tall + wy_offset(t, s), rowpart);
   }
+}
+
+// n > 256: the blocked reduction, one 512-thread block per system (the host
+// launches the large systems only); W is per-system n x n scratch
+__global__ __launch_bounds__(512) void k_tridiag_blocked(Topo t, const int* order, double* Aall,
+                                                         double* dall, double* eall, double* Wall,
+                                                         const unsigned char* active) {
+  const int s = order[blockIdx.x];
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  __shared__ double v[kMaxAO];
+  __shared__ double pq[kMaxAO];
+  __shared__ double part[8 * kMaxAO];
+  __shared__ double aux[kTriAux];
+  static_assert(kTriPanel % 4 == 0, "panel in K steps of 4");
+  __shared__ double red[16];
+  __shared__ double bc[2];
+  tridiag_blocked(Aall + t.mat_off[s], Wall + t.mat_off[s], n, dall + t.ao_off[s],
+                  eall + t.ao_off[s], v, pq, part, aux, red, bc);
 }
 
 // Overlap Cholesky S = L L^T (L column-major into `L`) and X = L^{-T} (row-major,
