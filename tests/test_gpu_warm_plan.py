@@ -259,9 +259,9 @@ def test_context_topology_cache_follows_the_batch(ctx):
         assert np.abs(got.forces - ref.forces).max() < 1e-11
 
 
-def test_enqueue_settles_inline_and_matches_compute(ctx):
+def test_enqueue_matches_compute(ctx):
     # xtbloom_compute_enqueue / xtbloom_plan_compute_enqueue (reference
-    # xtbloom.h:836-1003): this backend completes accepted submissions inline
+    # xtbloom.h:836-1003): submitted, then waited for
     b = small_batch(20)
     print("case enqueue", flush=True)
     req = interface.Request(ctx)

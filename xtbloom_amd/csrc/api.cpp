@@ -8,16 +8,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstddef>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <memory>
 #include <mutex>
 #include <new>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/xtbloom/xtbloom.h"
@@ -60,6 +64,8 @@ struct WarmCheckpoint {
   std::vector<double> state;
 };
 
+struct Job;
+
 struct xtbloom_context {
   xtb::Device* device = nullptr;
   // additional execution lanes (own streams and arenas) for large batches,
@@ -74,22 +80,33 @@ struct xtbloom_context {
     for (xtb::TopoCache* c : topo) xtb::topo_cache_destroy(c);
   }
   std::mutex lock;  // one context is one serialized transaction domain (reference backend.hpp:28-32)
+  // Asynchronous submissions (reference xtbloom.h:892-925): admitted jobs run
+  // in submission order on one worker thread; synchronous calls wait until the
+  // queue has drained, so every call on a context keeps submission order.
+  // queue, busy and stop are guarded by `lock`.
+  std::condition_variable cv;
+  std::deque<Job*> queue;
+  bool busy = false, stop = false;
+  std::thread worker;
 };
 
 // The inputs the packed host topology depends on: atom offsets, element
 // numbers, molecular charges, unpaired electrons, point-charge offsets and
 // gammas.
-// Asynchronous submission owner (reference xtbloom.h:41-47, 836-878).  This
-// backend settles every accepted submission inline: enqueue runs the call and
-// leaves the request COMPLETE, which the reference ABI allows ("or COMPLETE
-// when the backend must settle it inline"); admission errors return directly
-// and leave the request unchanged.
+// Asynchronous submission owner (reference xtbloom.h:41-47, 836-878):
+// IDLE after creation, PENDING from an accepted enqueue until the context's
+// worker has run the call and published its outputs, then COMPLETE with the
+// call's status, result flags and diagnostic.  Admission errors return from
+// enqueue directly and leave the request unchanged.  The fields are guarded by
+// `m`; the worker completes the request and notifies under it.
 struct xtbloom_request {
   xtbloom_context_t* context = nullptr;
   int32_t state = XTBLOOM_REQUEST_IDLE;
   int32_t completion_status = XTBLOOM_STATUS_SUCCESS;
   uint32_t result_flags = 0;
   std::string error;
+  std::mutex m;
+  std::condition_variable cv;
 };
 
 bool same_topology(const CallIdentity& a, const CallIdentity& b) {
@@ -111,6 +128,67 @@ struct xtbloom_plan {
   }
 };
 
+// A result arena (reference xtbloom.h:1011-1145): host memory or ROCm device
+// memory, freed when the last reference (the creator's, an explicit retain,
+// or one per exported tensor) is released.
+struct xtbloom_result_owner {
+  std::atomic<int64_t> refs{1};
+  xtbloom_memory_space_t memory_space = XTBLOOM_MEMORY_HOST;
+  int32_t device_id = -1;
+  uint64_t size = 0;
+  void* data = nullptr;
+  ~xtbloom_result_owner();
+};
+
+// DLPack 1.0 managed-tensor layouts, written from the published DLPack
+// specification (dlpack.h v1.0: DLDevice, DLDataType, DLTensor,
+// DLManagedTensor, DLManagedTensorVersioned); the byte layout is the
+// interchange format, so it must be exactly this.
+enum { kDLCPU = 1, kDLROCM = 10 };
+enum { kDLInt = 0, kDLUInt = 1, kDLFloat = 2 };
+struct DLTensorLayout {
+  void* data;
+  struct {
+    int32_t device_type;
+    int32_t device_id;
+  } device;
+  int32_t ndim;
+  struct {
+    uint8_t code;
+    uint8_t bits;
+    uint16_t lanes;
+  } dtype;
+  int64_t* shape;
+  int64_t* strides;
+  uint64_t byte_offset;
+};
+struct DLManagedLegacy {
+  DLTensorLayout dl_tensor;
+  void* manager_ctx;
+  void (*deleter)(DLManagedLegacy*);
+};
+struct DLManagedVersioned {
+  uint32_t version_major;
+  uint32_t version_minor;
+  void* manager_ctx;
+  void (*deleter)(DLManagedVersioned*);
+  uint64_t flags;
+  DLTensorLayout dl_tensor;
+};
+static_assert(sizeof(DLTensorLayout) == 48 && sizeof(DLManagedLegacy) == 64 && sizeof(DLManagedVersioned) == 80,
+              "DLPack 1.0 layouts");
+
+// One exported tensor: its managed struct (one of the two forms), its shape
+// storage and an arena reference, all released by the consumer's deleter.
+struct ManagedExport {
+  DLManagedLegacy legacy;
+  DLManagedVersioned versioned;
+  int64_t shape[XTBLOOM_DLPACK_MAX_NDIM];
+  xtbloom_result_owner_t* owner = nullptr;
+  static void delete_legacy(DLManagedLegacy* t);
+  static void delete_versioned(DLManagedVersioned* t);
+};
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -119,6 +197,47 @@ xtbloom_status_t fail(xtbloom_status_t status, std::string message) {
   g_last_error = std::move(message);
   return status;
 }
+
+constexpr uint64_t kArenaAlign = 256;
+
+// selects a device for the scope and restores the caller's on exit
+struct DeviceScope {
+  int previous = -1;
+  bool ok = false;
+  explicit DeviceScope(int device) {
+    if (hipGetDevice(&previous) != hipSuccess) previous = -1;
+    ok = hipSetDevice(device) == hipSuccess;
+  }
+  ~DeviceScope() {
+    if (previous >= 0) (void)hipSetDevice(previous);
+  }
+};
+
+}  // namespace
+
+xtbloom_result_owner::~xtbloom_result_owner() {
+  if (data == nullptr) return;
+  if (memory_space == XTBLOOM_MEMORY_HOST) {
+    std::free(data);
+  } else {
+    DeviceScope scope(device_id);
+    (void)hipFree(data);
+  }
+}
+
+void ManagedExport::delete_legacy(DLManagedLegacy* t) {
+  ManagedExport* m = static_cast<ManagedExport*>(t->manager_ctx);
+  xtbloom_result_owner_release(m->owner);
+  delete m;
+}
+
+void ManagedExport::delete_versioned(DLManagedVersioned* t) {
+  ManagedExport* m = static_cast<ManagedExport*>(t->manager_ctx);
+  xtbloom_result_owner_release(m->owner);
+  delete m;
+}
+
+namespace {
 
 template <typename T>
 xtbloom_status_t init_struct(T* value, size_t caller_size, size_t minimum, const char* name) {
@@ -415,54 +534,78 @@ bool batched_path() {
 // xtbloom_compute (plan == NULL) and xtbloom_plan_compute.  *admitted (if
 // given) tells whether the call got past admission (validation, plan and WARM
 // checks) before it returned.
-xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, const xtbloom_batch_t* batch,
-                              const xtbloom_compute_options_t* options, xtbloom_batch_result_t* result,
-                              xtbloom_amd_extension_t* ext, bool* admitted = nullptr,
-                              bool no_alloc = false) {
-  if (admitted) *admitted = false;
-  if (context == nullptr || context->device == nullptr)
-    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "context is NULL");
-  std::lock_guard<std::mutex> guard(context->lock);
+}  // namespace
+
+// One admitted call: its staged inputs, options and identity, the consumed
+// WARM state, and a copy of the caller's result descriptor (the output
+// buffers it names stay borrowed until the call completes).
+struct Job {
+  xtbloom_plan_t* plan = nullptr;
   Staged st;
   xtb::HostBatch hb;
   xtb::RunOptions ro;
   uint32_t flags = 0;
   bool warm = false;
-  xtbloom_status_t s = validate_and_stage(batch, options, result, st, hb, ro, flags, warm);
+  CallIdentity id;
+  std::vector<double> warm_state;
+  xtbloom_batch_result_t result;
+  xtbloom_amd_extension_t* ext = nullptr;
+  xtbloom_request_t* request = nullptr;
+  bool all_converged = false;
+  std::vector<double> scc_state;
+};
+
+namespace {
+
+// Admission (context lock held): validation and staging of every input,
+// the plan identity, and strict WARM (reference xtbloom.h:502-523: the latest
+// fully converged call with the same identity, never a fallback to FRESH).
+// An admitted call consumes the checkpoint whatever its start mode.
+xtbloom_status_t admit(xtbloom_context_t* context, xtbloom_plan_t* plan, const xtbloom_batch_t* batch,
+                       const xtbloom_compute_options_t* options, const xtbloom_batch_result_t* result,
+                       xtbloom_amd_extension_t* ext, bool no_alloc, Job& job) {
+  xtbloom_status_t s = validate_and_stage(batch, options, result, job.st, job.hb, job.ro, job.flags, job.warm);
   if (s != XTBLOOM_STATUS_SUCCESS) return s;
   if (ext != nullptr && !valid_header(ext, sizeof(xtbloom_amd_extension_t)))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "extension descriptor is invalid");
-  CallIdentity id = make_identity(st, options, ro, flags);
-  if (plan != nullptr && !(id == plan->id))
+  job.id = make_identity(job.st, options, job.ro, job.flags);
+  if (plan != nullptr && !(job.id == plan->id))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT,
                 "the batch topology or compute policy does not match the fixed-topology plan");
-  // strict WARM: the latest fully converged call with the same identity, never
-  // a fallback to FRESH (reference xtbloom.h:502-523)
   WarmCheckpoint& cp = plan ? plan->warm : context->warm;
-  std::vector<double> warm_state;
-  if (warm) {
-    if (!cp.valid || !(cp.id == id))
+  if (job.warm) {
+    if (!cp.valid || !(cp.id == job.id))
       return fail(XTBLOOM_STATUS_INVALID_ARGUMENT,
                   "WARM SCC start requires a previous fully converged call with identical topology and "
                   "compute policy on this context");
-    warm_state.swap(cp.state);
-    ro.warm_state = warm_state.data();
+    job.warm_state.swap(cp.state);
   }
-  // an accepted call consumes the checkpoint whatever its start mode
-  if (admitted) *admitted = true;
   cp.valid = false;
   cp.state.clear();
-  ro.no_alloc = no_alloc;
-  ro.batched = batched_path();
+  job.plan = plan;
+  job.ext = ext;
+  job.ro.no_alloc = no_alloc;
+  job.ro.batched = batched_path();
   {
     const char* env = getenv("XTBLOOM_AMD_SPLIT32");
-    ro.split32 = env != nullptr && std::strcmp(env, "1") == 0;
+    job.ro.split32 = env != nullptr && std::strcmp(env, "1") == 0;
   }
-  const int64_t B = hb.batch, N = hb.atoms, P = hb.points;
+  std::memset(&job.result, 0, sizeof(job.result));
+  std::memcpy(&job.result, result, std::min<size_t>(sizeof(job.result), result->struct_size));
+  return XTBLOOM_STATUS_SUCCESS;
+}
+
+// Execution of an admitted call: the device pipeline, then publication into
+// the caller's output buffers.  Runs with no other execution on the context
+// (the worker, or a synchronous caller after the queue has drained).
+xtbloom_status_t execute(xtbloom_context_t* context, Job& job) {
+  job.ro.warm_state = job.warm ? job.warm_state.data() : nullptr;
+  xtbloom_status_t s = XTBLOOM_STATUS_SUCCESS;
+  const int64_t B = job.hb.batch, N = job.hb.atoms, P = job.hb.points;
   std::vector<int32_t> state_dim(B);
   {
     std::string dim_err;
-    if (xtb::batch_matrix_elements(hb, nullptr, dim_err, nullptr, state_dim.data()) < 0)
+    if (xtb::batch_matrix_elements(job.hb, nullptr, dim_err, nullptr, state_dim.data()) < 0)
       return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, dim_err);
   }
   int64_t nstate = 0;
@@ -480,6 +623,7 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, 
   out.iterations = iters.data();
   out.converged = conv.data();
   out.status = status.data();
+  xtbloom_amd_extension_t* ext = job.ext;
   if (ext) {
     out.pc_dedq = ext->point_charge_potential_gradient;
     out.dbg_overlap = ext->overlap;
@@ -498,33 +642,32 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, 
   int rc = 0;
   try {
     const std::vector<xtb::TopoCache*>* caches = nullptr;
-    if (plan) {
-      caches = &plan->caches;
+    if (job.plan) {
+      caches = &job.plan->caches;
     } else {
       // the context's own topology cache: emptied when the topology changes
       if (context->topo.empty())
         for (int k = 0; k < 4; ++k) context->topo.push_back(xtb::topo_cache_create());
-      if (!same_topology(context->topo_id, id)) {
+      if (!same_topology(context->topo_id, job.id)) {
         for (xtb::TopoCache* c : context->topo) xtb::topo_cache_retag(c, -1);
-        context->topo_id = id;
+        context->topo_id = job.id;
       }
       caches = &context->topo;
     }
     // a no_alloc call runs on the context's own stream only (more lanes would
     // create devices, which allocates)
-    rc = xtb::run_batch_lanes(no_alloc ? std::vector<xtb::Device*>{context->device} : lanes_for(context, B),
-                              hb, ro, out, &timing, err, caches);
+    rc = xtb::run_batch_lanes(job.ro.no_alloc ? std::vector<xtb::Device*>{context->device} : lanes_for(context, B),
+                              job.hb, job.ro, out, &timing, err, caches);
   } catch (const std::bad_alloc&) {
     return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "host allocation failed");
   } catch (...) {
     return fail(XTBLOOM_STATUS_INTERNAL_ERROR, "unexpected exception in the ROCm executor");
   }
   if (rc != 0) return fail(rc, err);
-  if (std::all_of(status.begin(), status.end(), [](int32_t v) { return v == 0; })) {
-    cp.id = std::move(id);
-    cp.state = std::move(scc_state);
-    cp.valid = true;
-  }
+  // the SCC state becomes the WARM checkpoint when every system converged
+  // (committed by the caller of execute, under the context lock)
+  job.all_converged = std::all_of(status.begin(), status.end(), [](int32_t v) { return v == 0; });
+  if (job.all_converged) job.scc_state = std::move(scc_state);
   if (ext) {
     ext->setup_ms = timing.setup_ms;
     ext->scc_ms = timing.scc_ms;
@@ -535,23 +678,106 @@ xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, 
   // forces are -dE/dR
   for (auto& v : grad) v = -v;
   for (auto& v : pcgrad) v = -v;
-  if ((s = publish(result->energies, energies.data(), (flags & XTBLOOM_COMPUTE_ENERGY) ? sizeof(double) * B : 0)) ||
-      (s = publish(result->forces, grad.data(), (flags & XTBLOOM_COMPUTE_FORCES) ? sizeof(double) * 3 * N : 0)) ||
-      (s = publish(result->atomic_charges, charges.data(), (flags & XTBLOOM_COMPUTE_ATOMIC_CHARGES) ? sizeof(double) * N : 0)) ||
-      (s = publish(result->point_charge_forces, pcgrad.data(),
-                   (flags & XTBLOOM_COMPUTE_POINT_CHARGE_FORCES) ? sizeof(double) * 3 * P : 0)) ||
-      (s = publish(result->scc_iterations, iters.data(), sizeof(int32_t) * B)) ||
-      (s = publish(result->scc_converged, conv.data(), sizeof(uint8_t) * B)) ||
-      (s = publish(result->per_system_status, status.data(), sizeof(int32_t) * B)))
+  if ((s = publish(job.result.energies, energies.data(), (job.flags & XTBLOOM_COMPUTE_ENERGY) ? sizeof(double) * B : 0)) ||
+      (s = publish(job.result.forces, grad.data(), (job.flags & XTBLOOM_COMPUTE_FORCES) ? sizeof(double) * 3 * N : 0)) ||
+      (s = publish(job.result.atomic_charges, charges.data(), (job.flags & XTBLOOM_COMPUTE_ATOMIC_CHARGES) ? sizeof(double) * N : 0)) ||
+      (s = publish(job.result.point_charge_forces, pcgrad.data(),
+                   (job.flags & XTBLOOM_COMPUTE_POINT_CHARGE_FORCES) ? sizeof(double) * 3 * P : 0)) ||
+      (s = publish(job.result.scc_iterations, iters.data(), sizeof(int32_t) * B)) ||
+      (s = publish(job.result.scc_converged, conv.data(), sizeof(uint8_t) * B)) ||
+      (s = publish(job.result.per_system_status, status.data(), sizeof(int32_t) * B)))
     return s;
   uint32_t rflags = 0;
-  if ((flags & XTBLOOM_COMPUTE_DIPOLE_MOMENTS) && result->struct_size >= XTBLOOM_BATCH_RESULT_V2_SIZE) {
-    if ((s = publish(result->dipole_moments, dipoles.data(), sizeof(double) * 3 * B))) return s;
+  if ((job.flags & XTBLOOM_COMPUTE_DIPOLE_MOMENTS) && job.result.struct_size >= XTBLOOM_BATCH_RESULT_V2_SIZE) {
+    if ((s = publish(job.result.dipole_moments, dipoles.data(), sizeof(double) * 3 * B))) return s;
     rflags |= XTBLOOM_RESULT_DIPOLE_MOMENTS;
   }
-  result->flags = rflags;
+  job.result.flags = rflags;
   g_last_error.clear();
   return XTBLOOM_STATUS_SUCCESS;
+}
+
+
+void checkpoint_commit(xtbloom_context_t* context, Job& job) {
+  if (!job.all_converged) return;
+  WarmCheckpoint& cp = job.plan ? job.plan->warm : context->warm;
+  cp.id = job.id;
+  cp.state = std::move(job.scc_state);
+  cp.valid = true;
+}
+
+// waits (context lock held through `guard`) until no submission is queued or running
+void drain(xtbloom_context_t* context, std::unique_lock<std::mutex>& guard) {
+  context->cv.wait(guard, [context] { return context->queue.empty() && !context->busy; });
+}
+
+// A synchronous call: after the submissions queued before it, admission and
+// execution under the context lock.
+xtbloom_status_t compute_impl(xtbloom_context_t* context, xtbloom_plan_t* plan, const xtbloom_batch_t* batch,
+                              const xtbloom_compute_options_t* options, xtbloom_batch_result_t* result,
+                              xtbloom_amd_extension_t* ext, bool* admitted = nullptr,
+                              bool no_alloc = false) {
+  if (admitted) *admitted = false;
+  if (context == nullptr || context->device == nullptr)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "context is NULL");
+  std::unique_lock<std::mutex> guard(context->lock);
+  drain(context, guard);
+  std::unique_ptr<Job> job;
+  try {
+    job.reset(new Job());
+  } catch (const std::bad_alloc&) {
+    return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "host allocation failed");
+  }
+  xtbloom_status_t s = admit(context, plan, batch, options, result, ext, no_alloc, *job);
+  if (s != XTBLOOM_STATUS_SUCCESS) return s;
+  if (admitted) *admitted = true;
+  try {
+    s = execute(context, *job);
+  } catch (const std::bad_alloc&) {
+    s = fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "host allocation failed");
+  }
+  checkpoint_commit(context, *job);
+  if (s == XTBLOOM_STATUS_SUCCESS) result->flags = job->result.flags;
+  return s;
+}
+
+// The context's worker: runs queued submissions in order and completes their
+// requests.
+void worker_loop(xtbloom_context_t* context) {
+  std::unique_lock<std::mutex> guard(context->lock);
+  for (;;) {
+    context->cv.wait(guard, [context] { return context->stop || !context->queue.empty(); });
+    if (context->queue.empty()) return;  // stopped and drained
+    std::unique_ptr<Job> job(context->queue.front());
+    context->queue.pop_front();
+    context->busy = true;
+    guard.unlock();
+    xtbloom_status_t s;
+    try {
+      s = execute(context, *job);
+    } catch (const std::bad_alloc&) {
+      s = fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "host allocation failed");
+    } catch (...) {
+      s = fail(XTBLOOM_STATUS_INTERNAL_ERROR, "unexpected exception in the ROCm executor");
+    }
+    std::string err = s == XTBLOOM_STATUS_SUCCESS ? std::string() : g_last_error;
+    g_last_error.clear();
+    guard.lock();
+    checkpoint_commit(context, *job);
+    context->busy = false;
+    xtbloom_request_t* r = job->request;
+    {
+      // completed and notified under the request's mutex: a destroy waiting
+      // for completion cannot free the request before the notification
+      std::lock_guard<std::mutex> rg(r->m);
+      r->completion_status = s;
+      r->result_flags = s == XTBLOOM_STATUS_SUCCESS ? job->result.flags : 0u;
+      r->error.swap(err);
+      r->state = XTBLOOM_REQUEST_COMPLETE;
+      r->cv.notify_all();
+    }
+    context->cv.notify_all();
+  }
 }
 
 }  // namespace
@@ -649,6 +875,13 @@ xtbloom_status_t xtbloom_context_create(const xtbloom_context_options_t* options
 
 void xtbloom_context_destroy(xtbloom_context_t* context) {
   if (context == nullptr) return;
+  {
+    // queued submissions complete, then the worker stops
+    std::lock_guard<std::mutex> guard(context->lock);
+    context->stop = true;
+  }
+  context->cv.notify_all();
+  if (context->worker.joinable()) context->worker.join();
   for (xtb::Device* d : context->lanes) xtb::device_destroy(d);
   xtb::device_destroy(context->device);
   delete context;
@@ -717,24 +950,25 @@ xtbloom_status_t xtbloom_request_create(xtbloom_context_t* context, xtbloom_requ
   return XTBLOOM_STATUS_SUCCESS;
 }
 
-static xtbloom_status_t request_info(xtbloom_request_t* request, xtbloom_request_info_t* info) {
+static xtbloom_status_t request_info(xtbloom_request_t* request, xtbloom_request_info_t* info, bool wait) {
   if (request == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request is NULL");
   if (!valid_header(info, XTBLOOM_REQUEST_INFO_V1_SIZE) || info->reserved != 0)
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request info is NULL, too small, or has a different api_version");
+  std::unique_lock<std::mutex> rg(request->m);
+  if (wait) request->cv.wait(rg, [request] { return request->state != XTBLOOM_REQUEST_PENDING; });
   info->state = request->state;
   info->completion_status = request->state == XTBLOOM_REQUEST_COMPLETE ? request->completion_status
-                                                                      : XTBLOOM_STATUS_SUCCESS;
+                                                                         : XTBLOOM_STATUS_SUCCESS;
   info->result_flags = request->state == XTBLOOM_REQUEST_COMPLETE ? request->result_flags : 0u;
   g_last_error.clear();
   return XTBLOOM_STATUS_SUCCESS;
 }
 
-// every accepted submission is already COMPLETE: query and wait coincide
 xtbloom_status_t xtbloom_request_query(xtbloom_request_t* request, xtbloom_request_info_t* info) {
-  return request_info(request, info);
+  return request_info(request, info, false);
 }
 xtbloom_status_t xtbloom_request_wait(xtbloom_request_t* request, xtbloom_request_info_t* info) {
-  return request_info(request, info);
+  return request_info(request, info, true);
 }
 
 const char* xtbloom_request_get_error(const xtbloom_request_t* request) {
@@ -742,33 +976,69 @@ const char* xtbloom_request_get_error(const xtbloom_request_t* request) {
     fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request is NULL");
     return nullptr;
   }
-  return request->state == XTBLOOM_REQUEST_COMPLETE ? request->error.c_str() : "";
+  xtbloom_request_t* r = const_cast<xtbloom_request_t*>(request);
+  std::lock_guard<std::mutex> rg(r->m);
+  return r->state == XTBLOOM_REQUEST_COMPLETE ? r->error.c_str() : "";
 }
 
-void xtbloom_request_destroy(xtbloom_request_t* request) { delete request; }
+// a PENDING request is destroyed after its submission completes
+void xtbloom_request_destroy(xtbloom_request_t* request) {
+  if (request == nullptr) return;
+  {
+    std::unique_lock<std::mutex> rg(request->m);
+    request->cv.wait(rg, [request] { return request->state != XTBLOOM_REQUEST_PENDING; });
+  }
+  delete request;
+}
 
-// Runs one submission inline: admission errors return directly and leave the
-// request and every output untouched; an admitted call completes the request
-// with its status.  The caller's result descriptor is copied and never
-// modified: completed result flags go to the request.
+// Admits one submission and queues it for the context's worker (reference
+// xtbloom.h:892-925): admission errors return directly and leave the request
+// and every output untouched; an admitted call leaves the request PENDING
+// and returns at once.  Host inputs are staged during admission, so the
+// caller may reuse them on return; output buffers (and device inputs, staged
+// here as well) stay borrowed until COMPLETE.  The caller's result
+// descriptor is copied and never modified: completed result flags go to the
+// request.
 static xtbloom_status_t enqueue_impl(xtbloom_context_t* context, xtbloom_plan_t* plan,
                                      const xtbloom_batch_t* batch, const xtbloom_compute_options_t* options,
                                      const xtbloom_batch_result_t* result, xtbloom_request_t* request) {
   if (request == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request is NULL");
+  if (context == nullptr || context->device == nullptr)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "context is NULL");
   if (request->context != context)
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request belongs to another context");
   if (!valid_header(result, XTBLOOM_BATCH_RESULT_V1_SIZE))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "batch result is NULL, too small, or has a different api_version");
-  xtbloom_batch_result_t copy;
-  std::memset(&copy, 0, sizeof(copy));
-  std::memcpy(&copy, result, std::min<size_t>(sizeof(copy), result->struct_size));
-  bool admitted = false;
-  const xtbloom_status_t s = compute_impl(context, plan, batch, options, &copy, nullptr, &admitted);
-  if (!admitted) return s;
-  request->state = XTBLOOM_REQUEST_COMPLETE;
-  request->completion_status = s;
-  request->result_flags = s == XTBLOOM_STATUS_SUCCESS ? copy.flags : 0u;
-  request->error = s == XTBLOOM_STATUS_SUCCESS ? std::string() : g_last_error;
+  std::unique_lock<std::mutex> guard(context->lock);
+  {
+    std::lock_guard<std::mutex> rg(request->m);
+    if (request->state == XTBLOOM_REQUEST_PENDING)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "request already has a pending submission");
+  }
+  std::unique_ptr<Job> job;
+  try {
+    job.reset(new Job());
+  } catch (const std::bad_alloc&) {
+    return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "host allocation failed");
+  }
+  const xtbloom_status_t s = admit(context, plan, batch, options, result, nullptr, false, *job);
+  if (s != XTBLOOM_STATUS_SUCCESS) return s;
+  job->request = request;
+  try {
+    if (!context->worker.joinable()) context->worker = std::thread(worker_loop, context);
+    context->queue.push_back(job.get());
+  } catch (...) {
+    return fail(XTBLOOM_STATUS_INTERNAL_ERROR, "the submission worker could not be started");
+  }
+  job.release();
+  {
+    std::lock_guard<std::mutex> rg(request->m);
+    request->state = XTBLOOM_REQUEST_PENDING;
+    request->completion_status = XTBLOOM_STATUS_SUCCESS;
+    request->result_flags = 0;
+    request->error.clear();
+  }
+  context->cv.notify_all();
   g_last_error.clear();
   return XTBLOOM_STATUS_SUCCESS;
 }
@@ -786,7 +1056,8 @@ xtbloom_status_t xtbloom_plan_create(xtbloom_context_t* context, const xtbloom_b
   *plan = nullptr;
   if (context == nullptr || context->device == nullptr)
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "context is NULL");
-  std::lock_guard<std::mutex> guard(context->lock);
+  std::unique_lock<std::mutex> guard(context->lock);
+  drain(context, guard);  // the reservation runs on the device
   Staged st;
   xtb::HostBatch hb;
   xtb::RunOptions ro;
@@ -829,7 +1100,15 @@ xtbloom_status_t xtbloom_plan_create(xtbloom_context_t* context, const xtbloom_b
   return XTBLOOM_STATUS_SUCCESS;
 }
 
-void xtbloom_plan_destroy(xtbloom_plan_t* plan) { delete plan; }
+// queued submissions may still reference the plan: they complete first
+void xtbloom_plan_destroy(xtbloom_plan_t* plan) {
+  if (plan == nullptr) return;
+  {
+    std::unique_lock<std::mutex> guard(plan->context->lock);
+    drain(plan->context, guard);
+  }
+  delete plan;
+}
 
 xtbloom_status_t xtbloom_plan_query_workspace(const xtbloom_plan_t* plan, xtbloom_workspace_query_t* query) {
   if (plan == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "plan is NULL");
@@ -858,22 +1137,163 @@ xtbloom_status_t xtbloom_plan_compute_enqueue(xtbloom_plan_t* plan, const xtbloo
 }
 xtbloom_status_t xtbloom_result_owner_options_init(xtbloom_result_owner_options_t* o, size_t size) {
   xtbloom_status_t s = init_struct(o, size, XTBLOOM_RESULT_OWNER_OPTIONS_V1_SIZE, "result owner options");
-  if (s == XTBLOOM_STATUS_SUCCESS) o->device_id = -1;
+  if (s == XTBLOOM_STATUS_SUCCESS) {
+    o->memory_space = XTBLOOM_MEMORY_HOST;
+    o->device_id = -1;
+  }
   return s;
 }
-xtbloom_status_t xtbloom_result_owner_create(const xtbloom_result_owner_options_t*,
+
+xtbloom_status_t xtbloom_result_owner_create(const xtbloom_result_owner_options_t* options,
                                              xtbloom_result_owner_t** owner) {
-  if (owner) *owner = nullptr;
-  return not_supported("DLPack result owner");
+  if (owner == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "result owner output is NULL");
+  *owner = nullptr;
+  if (!valid_header(options, XTBLOOM_RESULT_OWNER_OPTIONS_V1_SIZE) || options->reserved != 0)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT,
+                "result owner options are NULL, too small, have a different api_version or a nonzero reserved field");
+  if (options->size_bytes == 0) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "result arena size must be nonzero");
+  const bool host = options->memory_space == XTBLOOM_MEMORY_HOST;
+  if (host && options->device_id != -1)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "a host result arena takes device_id -1");
+  if (!host && options->memory_space != XTBLOOM_MEMORY_ROCM_DEVICE)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "result arenas live in host or ROCm device memory");
+  if (!host && options->device_id < 0)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "a device result arena needs a device ordinal");
+  std::unique_ptr<xtbloom_result_owner_t> o;
+  try {
+    o.reset(new xtbloom_result_owner_t());
+  } catch (const std::bad_alloc&) {
+    return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "result owner allocation failed");
+  }
+  o->memory_space = options->memory_space;
+  o->device_id = options->device_id;
+  o->size = options->size_bytes;
+  if (host) {
+    const uint64_t rounded = (o->size + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+    o->data = std::aligned_alloc(kArenaAlign, rounded);
+    if (o->data == nullptr) return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "host result arena allocation failed");
+    std::memset(o->data, 0, rounded);
+  } else {
+    DeviceScope scope(o->device_id);
+    if (!scope.ok) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "device result arena: no such device");
+    if (hipMalloc(&o->data, o->size) != hipSuccess || o->data == nullptr) {
+      o->data = nullptr;
+      return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "device result arena allocation failed");
+    }
+    if (hipMemset(o->data, 0, o->size) != hipSuccess) {
+      (void)hipFree(o->data);
+      o->data = nullptr;
+      return fail(XTBLOOM_STATUS_INTERNAL_ERROR, "device result arena initialization failed");
+    }
+  }
+  *owner = o.release();
+  g_last_error.clear();
+  return XTBLOOM_STATUS_SUCCESS;
 }
-xtbloom_status_t xtbloom_result_owner_buffer(const xtbloom_result_owner_t*, xtbloom_buffer_t*) {
-  return not_supported("DLPack result owner");
+
+xtbloom_status_t xtbloom_result_owner_buffer(const xtbloom_result_owner_t* owner, xtbloom_buffer_t* buffer) {
+  if (owner == nullptr || buffer == nullptr)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "result owner or buffer is NULL");
+  buffer->data = owner->data;
+  buffer->size_bytes = (size_t)owner->size;
+  buffer->memory_space = owner->memory_space;
+  buffer->reserved = 0;
+  g_last_error.clear();
+  return XTBLOOM_STATUS_SUCCESS;
 }
-void xtbloom_result_owner_retain(xtbloom_result_owner_t*) {}
-void xtbloom_result_owner_release(xtbloom_result_owner_t*) {}
-xtbloom_status_t xtbloom_result_owner_export_dltensor(const xtbloom_result_owner_t*,
-                                                      const xtbloom_dlpack_view_t*, int, void**) {
-  return not_supported("DLPack result owner");
+
+void xtbloom_result_owner_retain(xtbloom_result_owner_t* owner) {
+  if (owner != nullptr) owner->refs.fetch_add(1, std::memory_order_relaxed);
+}
+
+void xtbloom_result_owner_release(xtbloom_result_owner_t* owner) {
+  if (owner == nullptr) return;
+  if (owner->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete owner;
+}
+
+xtbloom_status_t xtbloom_result_owner_export_dltensor(const xtbloom_result_owner_t* owner,
+                                                      const xtbloom_dlpack_view_t* view, int version,
+                                                      void** out_managed) {
+  if (out_managed == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "managed tensor output is NULL");
+  *out_managed = nullptr;
+  if (owner == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "result owner is NULL");
+  if (version != 0 && version != 1)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "DLPack export version must be 0 (legacy) or 1 (versioned)");
+  if (!valid_header(view, XTBLOOM_DLPACK_VIEW_V1_SIZE) || view->reserved != 0)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT,
+                "DLPack view is NULL, too small, has a different api_version or a nonzero reserved field");
+  // scalar int8/16/32/64, uint8, float32/64 (reference xtbloom.h:1067-1076)
+  const int32_t code = view->dtype_code, bits = view->dtype_bits;
+  const bool dtype_ok = view->dtype_lanes == 1 &&
+                        ((code == kDLInt && (bits == 8 || bits == 16 || bits == 32 || bits == 64)) ||
+                         (code == kDLUInt && bits == 8) || (code == kDLFloat && (bits == 32 || bits == 64)));
+  if (!dtype_ok) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "unsupported DLPack dtype");
+  if (view->ndim < 0 || view->ndim > XTBLOOM_DLPACK_MAX_NDIM)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "DLPack ndim must be 0..8");
+  if (view->ndim > 0 && view->shape == nullptr) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "DLPack shape is NULL");
+  const uint64_t item = (uint64_t)bits / 8;
+  if (view->byte_offset % item != 0)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "DLPack byte offset breaks the dtype alignment");
+  uint64_t count = 1;
+  bool empty = false;
+  for (int32_t d = 0; d < view->ndim; ++d) {
+    const int64_t extent = view->shape[d];
+    if (extent < 0) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "negative DLPack extent");
+    if (extent == 0) empty = true;
+  }
+  if (!empty) {
+    for (int32_t d = 0; d < view->ndim; ++d) {
+      const uint64_t extent = (uint64_t)view->shape[d];
+      if (count > std::numeric_limits<uint64_t>::max() / extent)
+        return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "DLPack extent product overflows");
+      count *= extent;
+    }
+    if (count > (std::numeric_limits<uint64_t>::max() - view->byte_offset) / item ||
+        view->byte_offset + count * item > owner->size)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "DLPack view exceeds the result arena");
+  } else if (view->byte_offset > owner->size) {
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "DLPack view exceeds the result arena");
+  }
+  ManagedExport* m = nullptr;
+  try {
+    m = new ManagedExport();
+  } catch (const std::bad_alloc&) {
+    return fail(XTBLOOM_STATUS_ALLOCATION_FAILED, "managed tensor allocation failed");
+  }
+  std::memset(&m->legacy, 0, sizeof(m->legacy));
+  std::memset(&m->versioned, 0, sizeof(m->versioned));
+  for (int32_t d = 0; d < view->ndim; ++d) m->shape[d] = view->shape[d];
+  m->owner = const_cast<xtbloom_result_owner_t*>(owner);
+  DLTensorLayout t;
+  std::memset(&t, 0, sizeof(t));
+  t.data = empty ? nullptr : static_cast<char*>(owner->data) + view->byte_offset;
+  const bool host = owner->memory_space == XTBLOOM_MEMORY_HOST;
+  t.device.device_type = host ? kDLCPU : kDLROCM;
+  t.device.device_id = host ? 0 : owner->device_id;
+  t.ndim = view->ndim;
+  t.dtype.code = (uint8_t)code;
+  t.dtype.bits = (uint8_t)bits;
+  t.dtype.lanes = 1;
+  t.shape = view->ndim > 0 ? m->shape : nullptr;
+  t.strides = nullptr;
+  t.byte_offset = 0;
+  xtbloom_result_owner_retain(m->owner);
+  if (version == 1) {
+    m->versioned.version_major = 1;
+    m->versioned.version_minor = 0;
+    m->versioned.manager_ctx = m;
+    m->versioned.deleter = &ManagedExport::delete_versioned;
+    m->versioned.flags = 0;
+    m->versioned.dl_tensor = t;
+    *out_managed = &m->versioned;
+  } else {
+    m->legacy.dl_tensor = t;
+    m->legacy.manager_ctx = m;
+    m->legacy.deleter = &ManagedExport::delete_legacy;
+    *out_managed = &m->legacy;
+  }
+  g_last_error.clear();
+  return XTBLOOM_STATUS_SUCCESS;
 }
 
 }  // extern "C"

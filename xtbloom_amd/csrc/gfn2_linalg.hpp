@@ -2186,21 +2186,34 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
         for (int i = b0; i < b1; ++i) rmax = fmax(rmax, fabs(Z[(int64_t)i * n + cj]));
         const double rs = rmax > 0.0 ? 1.0 / rmax : 1.0;
         for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + cj] *= rs;
-        for (int k = 0; k < bn - 1; ++k) {
-          const int64_t at = (int64_t)(b0 + k) * n + cj, nx = at + n;
-          double rk = Z[at], rn = Z[nx];
-          if (FP[at] != 0.0) { const double tmp = rk; rk = rn; rn = tmp; }
-          Z[at] = rk;
-          Z[nx] = rn - FM[at] * rk;
+        // forward elimination; the running entry stays in a register (it is
+        // the one stored one row earlier), so only independent loads remain
+        {
+          double cur = Z[(int64_t)b0 * n + cj];
+          for (int k = 0; k < bn - 1; ++k) {
+            const int64_t at = (int64_t)(b0 + k) * n + cj;
+            double rk = cur, rn = Z[at + n];
+            if (FP[at] != 0.0) { const double tmp = rk; rk = rn; rn = tmp; }
+            Z[at] = rk;
+            cur = rn - FM[at] * rk;
+          }
+          Z[(int64_t)(b1 - 1) * n + cj] = cur;
         }
-        for (int k = bn - 1; k >= 0; --k) {
-          const int64_t at = (int64_t)(b0 + k) * n + cj;
-          double r = Z[at];
-          if (k + 1 < bn) r -= F1[at] * Z[at + n];
-          if (k + 2 < bn) r -= F2[at] * Z[at + 2 * (int64_t)n];
-          double u = F0[at];
-          if (fabs(u) < ptol) u = u < 0.0 ? -ptol : ptol;
-          Z[at] = r / u;
+        // back substitution with the two entries below in registers
+        {
+          double z1 = 0.0, z2 = 0.0;  // solution entries k + 1, k + 2
+          for (int k = bn - 1; k >= 0; --k) {
+            const int64_t at = (int64_t)(b0 + k) * n + cj;
+            double r = Z[at];
+            if (k + 1 < bn) r -= F1[at] * z1;
+            if (k + 2 < bn) r -= F2[at] * z2;
+            double u = F0[at];
+            if (fabs(u) < ptol) u = u < 0.0 ? -ptol : ptol;
+            const double zk = r / u;
+            Z[at] = zk;
+            z2 = z1;
+            z1 = zk;
+          }
         }
         // normalize
         double nrm = 0.0, ymax = 0.0;

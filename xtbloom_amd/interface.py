@@ -230,7 +230,7 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                    request: "Request | None" = None,
                    extension=False, diagnostics=False, kernel_stats=False,
                    determinism=library.DETERMINISM_DEFAULT, options_hook=None,
-                   no_alloc=False) -> ComputedBatch:
+                   no_alloc=False, wait=True) -> "ComputedBatch | PendingBatch":
     """One ``xtbloom_compute`` over flat ragged arrays (the ``xtbloom_batch_t`` layout).
 
     ``scc_start_mode=library.SCC_START_WARM`` starts from the converged state of
@@ -241,7 +241,10 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
     results on repeated calls.  ``options_hook`` (optional) receives the
     compute-options struct right before the call (tests of the validation).
     ``no_alloc=True`` goes through ``xtbloom_amd_compute_no_alloc`` (no device
-    allocation; the batch must fit its fixed workspace).
+    allocation; the batch must fit its fixed workspace).  With ``request`` the
+    call is submitted through ``xtbloom_compute_enqueue`` (or the plan's
+    enqueue); ``wait=False`` returns a :class:`PendingBatch` at once, whose
+    ``result()`` waits for the request.
     """
     lib = library.load()
     batch, opts, keep, (B, N, P) = _descriptors(
@@ -291,9 +294,10 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
             _check(lib.xtbloom_compute_enqueue(context.handle, ctypes.byref(batch),
                                                ctypes.byref(opts), ctypes.byref(res),
                                                request.handle))
-        done = request.wait()
-        if done["completion_status"] != library.STATUS_SUCCESS:
-            raise XTBloomError(done["completion_status"], request.error())
+        computed = ComputedBatch(atom_offsets, point_offsets, energies, forces, q, pcf, dip, iters,
+                                 conv, status, None, None)
+        pending = PendingBatch(request, computed)
+        return pending if not wait else pending.result()
     elif plan is not None:
         if extension or diagnostics or kernel_stats:
             raise XTBloomValueError("plan calls take no extension outputs")
@@ -338,12 +342,40 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                          conv.astype(bool), status, dedq, ext_out)
 
 
+class PendingBatch:
+    """A submitted call: its request and the output arrays the library fills.
+
+    ``done()`` polls the request without blocking; ``result()`` waits for it,
+    raises ``XTBloomError`` with the completion status of a failed call, and
+    returns the :class:`ComputedBatch`.
+    """
+
+    def __init__(self, request: "Request", computed: ComputedBatch):
+        self.request = request
+        self._computed = computed
+        self._result = None
+
+    def done(self) -> bool:
+        return self.request.query()["state"] == library.REQUEST_COMPLETE
+
+    def result(self) -> ComputedBatch:
+        if self._result is None:
+            info = self.request.wait()
+            if info["completion_status"] != library.STATUS_SUCCESS:
+                raise XTBloomError(info["completion_status"], self.request.error())
+            c = self._computed
+            c.scc_converged = c.scc_converged.astype(bool)
+            self._result = c
+        return self._result
+
+
 class Request:
     """Reusable submission owner (``xtbloom_request_create``) bound to a context.
 
-    This backend settles each accepted submission inline, so after
-    :func:`compute_arrays` with ``request=`` the request is COMPLETE and
-    :meth:`wait` returns at once (reference ``xtbloom.h`` request API).
+    An accepted submission is PENDING until the context's worker has run it
+    and published its outputs, then COMPLETE (reference ``xtbloom.h`` request
+    API); submissions on one context run in order, and a synchronous call
+    waits for the ones queued before it.  Close a request before its context.
     """
 
     def __init__(self, context: Context):

@@ -142,6 +142,20 @@ class RequestInfo(ctypes.Structure):
 REQUEST_IDLE, REQUEST_PENDING, REQUEST_COMPLETE = 0, 1, 2
 
 
+class ResultOwnerOptions(ctypes.Structure):  # xtbloom_result_owner_options_t
+    _fields_ = [("struct_size", ctypes.c_uint32), ("api_version", ctypes.c_uint32),
+                ("memory_space", ctypes.c_int32), ("device_id", ctypes.c_int32),
+                ("size_bytes", ctypes.c_uint64), ("reserved", ctypes.c_uint32)]
+
+
+class DLPackView(ctypes.Structure):  # xtbloom_dlpack_view_t
+    _fields_ = [("struct_size", ctypes.c_uint32), ("api_version", ctypes.c_uint32),
+                ("byte_offset", ctypes.c_uint64), ("dtype_code", ctypes.c_int32),
+                ("dtype_bits", ctypes.c_int32), ("dtype_lanes", ctypes.c_int32),
+                ("ndim", ctypes.c_int32), ("reserved", ctypes.c_uint32),
+                ("shape", ctypes.POINTER(ctypes.c_int64))]
+
+
 class WorkspaceQuery(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("api_version", ctypes.c_uint32),
                 ("compute_flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
@@ -160,6 +174,7 @@ assert ctypes.sizeof(BatchResult) == 280
 assert ctypes.sizeof(KernelStat) == 32
 assert ctypes.sizeof(WorkspaceQuery) == 48 and WorkspaceQuery.device_required_bytes.offset == 32
 assert ctypes.sizeof(RequestInfo) == 24
+assert ctypes.sizeof(ResultOwnerOptions) == 32 and ctypes.sizeof(DLPackView) == 48
 
 EXPORTED_SYMBOLS = (
     "xtbloom_version_string", "xtbloom_status_string", "xtbloom_get_last_error",
@@ -257,6 +272,19 @@ def load() -> ctypes.CDLL:
                                                  ctypes.POINTER(ComputeOptions),
                                                  ctypes.POINTER(BatchResult), vp]
     lib.xtbloom_plan_compute_enqueue.restype = ctypes.c_int32
+    lib.xtbloom_result_owner_options_init.argtypes = [ctypes.POINTER(ResultOwnerOptions), ctypes.c_size_t]
+    lib.xtbloom_result_owner_options_init.restype = ctypes.c_int32
+    lib.xtbloom_result_owner_create.argtypes = [ctypes.POINTER(ResultOwnerOptions), ctypes.POINTER(vp)]
+    lib.xtbloom_result_owner_create.restype = ctypes.c_int32
+    lib.xtbloom_result_owner_buffer.argtypes = [vp, ctypes.POINTER(Buffer)]
+    lib.xtbloom_result_owner_buffer.restype = ctypes.c_int32
+    lib.xtbloom_result_owner_retain.argtypes = [vp]
+    lib.xtbloom_result_owner_retain.restype = None
+    lib.xtbloom_result_owner_release.argtypes = [vp]
+    lib.xtbloom_result_owner_release.restype = None
+    lib.xtbloom_result_owner_export_dltensor.argtypes = [vp, ctypes.POINTER(DLPackView), ctypes.c_int,
+                                                         ctypes.POINTER(vp)]
+    lib.xtbloom_result_owner_export_dltensor.restype = ctypes.c_int32
     _LIB = lib
     return lib
 
