@@ -332,6 +332,79 @@ constexpr double kSafeMin = 2.2250738585072014e-308;
 constexpr double kClusterTol = 1e-6;  // relative gap (to the block 1-norm) below which vectors are orthogonalized
 constexpr int kInverseIterations = 3;
 
+// Householder reduction with the trailing matrix in global memory, one
+// reflector at a time (any n).  The register kernel below keeps it as its
+// general branch; the host sends systems above 256 AOs to the blocked
+// reduction (k_tridiag_blocked), which reads the matrix a sixth as often.
+__device__ __forceinline__ void tridiag_global(double* A, int n, double* d, double* e, double* v, double* pq,
+                               double* red, double* bc) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;
+    double ss = 0.0;
+    for (int i = tid; i < m; i += nt) {
+      const double x = A[(k + 1 + i) + (int64_t)k * n];
+      v[i] = x;
+      ss += x * x;
+    }
+    ss = block_sum(ss, red);
+    if (tid == 0) {
+      const double x0 = v[0];
+      const double xn = sqrt(ss);
+      d[k] = A[k + (int64_t)k * n];
+      if (xn == 0.0 || ss - x0 * x0 == 0.0) {
+        e[k] = x0;  // already reduced: no reflection
+        bc[0] = 0.0;
+      } else {
+        const double alpha = x0 >= 0.0 ? -xn : xn;
+        const double v0 = x0 - alpha;
+        const double vv = ss - x0 * x0 + v0 * v0;
+        e[k] = alpha;
+        v[0] = v0;
+        bc[0] = 1.0 / sqrt(vv);
+      }
+    }
+    __syncthreads();
+    const double inv = bc[0];
+    if (inv == 0.0) {
+      for (int i = tid; i < m; i += nt) A[(k + 1 + i) + (int64_t)k * n] = 0.0;
+      __syncthreads();
+      continue;
+    }
+    for (int i = tid; i < m; i += nt) {
+      v[i] *= inv;
+      A[(k + 1 + i) + (int64_t)k * n] = v[i];
+    }
+    __syncthreads();
+    // p = A22 v, q = p - (v^T p) v
+    double kp = 0.0;
+    for (int i = tid; i < m; i += nt) {
+      double p = 0.0;
+      for (int j = 0; j < m; ++j) p += A[(k + 1 + i) + (int64_t)(k + 1 + j) * n] * v[j];
+      pq[i] = p;
+      kp += v[i] * p;
+    }
+    const double K = block_sum(kp, red);
+    for (int i = tid; i < m; i += nt) pq[i] -= K * v[i];
+    __syncthreads();
+    // A22 -= 2 (v q^T + q v^T)
+    for (int j = 0; j < m; ++j) {
+      const double vj = v[j], qj = pq[j];
+      for (int i = tid; i < m; i += nt)
+        A[(k + 1 + i) + (int64_t)(k + 1 + j) * n] -= 2.0 * (v[i] * qj + pq[i] * vj);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (n >= 2) {
+      d[n - 2] = A[(n - 2) + (int64_t)(n - 2) * n];
+      e[n - 2] = A[(n - 1) + (int64_t)(n - 2) * n];
+    }
+    d[n - 1] = A[(n - 1) + (int64_t)(n - 1) * n];
+    e[n - 1] = 0.0;
+  }
+}
+
 // Householder reduction for n > 256 in panels of kTriPanel reflectors, the
 // arrangement of LAPACK's dsytrd/dlatrd.  Within a panel the trailing matrix
 // is not written: after reflectors 0 .. j-1 of the panel it equals
@@ -1026,8 +1099,9 @@ __device__ __forceinline__ void wy_factors(const double* A, int n, double* Ts, d
   __syncthreads();
 }
 
-// 512 threads per system (two waves per SIMD, 256 VGPRs per thread), n <= 256:
-// the systems order[blockIdx.x] (the host launches the small systems only)
+// 512 threads per system (two waves per SIMD, 256 VGPRs per thread): the
+// register reduction for n <= 256, the global-memory one otherwise (the host
+// launches the systems of at most 256 AOs only)
 __global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, double* Aall, double* dall,
                                                  double* eall, double* Tall,
                                                  const unsigned char* active) {
@@ -1042,13 +1116,16 @@ __global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, doubl
   __shared__ double col[512];  // published column of the register path
   __shared__ double rowpart[32 * kTriPad];
   __shared__ double colpart[16 * kTriPad];
+  __shared__ double red[16];
+  __shared__ double bc[2];
   if (n <= 64) tridiag_registers<4>(A, n, d, e, v, col, pq, rowpart, colpart);
   else if (n <= 128) tridiag_registers<8>(A, n, d, e, v, col, pq, rowpart, colpart);
   else if (n <= 192) tridiag_registers<12>(A, n, d, e, v, col, pq, rowpart, colpart);
-  else tridiag_registers<16>(A, n, d, e, v, col, pq, rowpart, colpart);
-  // WY factors for the register back-transformation; the stored reflectors
-  // must be visible to the whole block first
-  if (n > 2) {
+  else if (n <= 256) tridiag_registers<16>(A, n, d, e, v, col, pq, rowpart, colpart);
+  else tridiag_global(A, n, d, e, v, pq, red, bc);
+  // WY factors for the register back-transformation (n <= 256); the stored
+  // reflectors must be visible to the whole block first
+  if (n <= 256 && n > 2) {
     __syncthreads();
     wy_factors(A, n, Tall + wy_offset(t, s), rowpart);
   }
