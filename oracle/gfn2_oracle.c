@@ -69,6 +69,7 @@ void oracle_default_options(oracle_options* o) {
   o->energy_tolerance = 1.0e-8;
   o->kt = 300.0 * 3.166808578545117e-6;
   o->damping = 0.4;
+  o->efield[0] = o->efield[1] = o->efield[2] = 0.0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1561,6 +1562,12 @@ typedef struct {
   double *L;         /* Cholesky factor, column-major lower */
   AesPair* aes;
   D4Geo d4;
+  /* uniform electric field (gfn2_cpu_execution.cpp:1294-1309): per atom the
+   * scalar potential vat = -E.r on the charge channel and the dipolar
+   * potential -E on the dipole channel; `field` 0 leaves both out */
+  int field;
+  double E[3];
+  const double* xyz;
 } System;
 
 static void mulliken(const System* sy, const double* P, double* qsh, double* qat, double* dip,
@@ -1605,6 +1612,15 @@ static void potentials(const System* sy, const double* qsh, const double* qat, c
   aes2_potential(sy->nat, b->Z, sy->aes, qat, dip, quad, vat_scratch, vd, vqq);
   d4_two_body(&sy->d4, qat, d4v, NULL);
   for (int a = 0; a < sy->nat; ++a) vat_scratch[a] += d4v[a];
+  /* field after D4, on the charge-channel atom and dipole potentials
+   * (gfn2_cpu_execution.cpp:1422-1434, scc_driver.cpp:2553-2580) */
+  if (sy->field) {
+    for (int a = 0; a < sy->nat; ++a) {
+      const double* r = sy->xyz + 3 * a;
+      vat_scratch[a] += -(sy->E[0] * r[0] + sy->E[1] * r[1] + sy->E[2] * r[2]);
+      for (int x = 0; x < 3; ++x) vd[3 * a + x] += -sy->E[x];
+    }
+  }
   for (int s = 0; s < ns; ++s) vsh[s] += vat_scratch[b->sh_atom[s]];
 }
 
@@ -1684,6 +1700,7 @@ static int system_setup(System* sy, int32_t nat, const int32_t* Z, const double*
                         int32_t unpaired, int32_t npc, const double* pc_xyz, const double* pc_q,
                         const double* pc_gamma, double* nalpha, double* nbeta) {
   memset(sy, 0, sizeof(*sy));
+  sy->xyz = xyz;
   int st = basis_make(nat, Z, &sy->b);
   if (st != ST_OK) return st;
   const int n = sy->b.nao, ns = sy->b.nsh;
@@ -1771,9 +1788,23 @@ static int scc_step(System* sy, const double* x, double nalpha, double nbeta, do
   }
   double eaes = aes2_energy(nat, sy->b.Z, sy->aes, qat_scratch, rd, rQ);
   double ed4 = d4_two_body(&sy->d4, qat_scratch, NULL, NULL);
-  double e_int = ecore + ees2 + ees3 + eaes + ed4 + epc;
+  /* field: sum_i q_i vat_i + vdp . d_i over the raw multipoles
+   * (scc_driver.cpp:2211-2232), summed after the point charges */
+  double efield = 0.0;
+  if (sy->field) {
+    for (int a = 0; a < nat; ++a) {
+      const double* r = sy->xyz + 3 * a;
+      const double vat = -(sy->E[0] * r[0] + sy->E[1] * r[1] + sy->E[2] * r[2]);
+      double c = fma(vat, qat_scratch[a], 0.0);
+      for (int x = 0; x < 3; ++x) c = fma(-sy->E[x], rd[3 * a + x], c);
+      efield += c;
+    }
+  }
+  double e_int = ecore + ees2 + ees3 + eaes + ed4 + epc + efield;
   *F = fma(-kt, entropy, e_int);
-  ecomp[0] = ecore; ecomp[1] = ees2; ecomp[2] = ees3; ecomp[3] = eaes; ecomp[4] = ed4; ecomp[5] = epc;
+  /* component 5 holds the external terms: point charges and the field */
+  ecomp[0] = ecore; ecomp[1] = ees2; ecomp[2] = ees3; ecomp[3] = eaes; ecomp[4] = ed4;
+  ecomp[5] = epc + efield;
   return ST_OK;
 }
 
@@ -1822,6 +1853,8 @@ int oracle_singlepoint_state(int32_t nat, const int32_t* Z, const double* xyz, d
   double nalpha = 0.0, nbeta = 0.0;
   int status = system_setup(&sy, nat, Z, xyz, charge, unpaired, npc, pc_xyz, pc_q, pc_gamma,
                             &nalpha, &nbeta);
+  for (int k = 0; k < 3; ++k) sy.E[k] = opt->efield[k];
+  sy.field = opt->efield[0] != 0.0 || opt->efield[1] != 0.0 || opt->efield[2] != 0.0;
   if (status == ST_INVALID || sy.S == NULL) { system_free(&sy); return status == ST_OK ? ST_INVALID : status; }
   const int n = sy.n, ns = sy.ns;
   const size_t nn = (size_t)n * n;
@@ -1973,6 +2006,12 @@ int oracle_singlepoint_state(int32_t nat, const int32_t* Z, const double* xyz, d
         }
       }
     }
+    /* field: the stationary potentials above carry the density response; the
+     * explicit derivative of -sum_i q_i E.r_i is -q_i E in the gradient
+     * (gfn2_cpu_execution.cpp:1588-1598) */
+    if (sy.field)
+      for (int a = 0; a < nat; ++a)
+        for (int k = 0; k < 3; ++k) g[3 * a + k] = fma(-qat[a], sy.E[k], g[3 * a + k]);
     if (grad_out) memcpy(grad_out, g, sizeof(double) * 3 * nat);
     free(aS); free(aD); free(aQ); free(dEdcn); free(g); free(vsh);
   }
@@ -1998,6 +2037,7 @@ typedef struct {
   int32_t *iters, *status;
   long next;
   pthread_mutex_t lock;
+  const double* efield; /* [batch][3] or NULL */
 } BatchJob;
 
 static void* batch_worker(void* arg) {
@@ -2012,9 +2052,12 @@ static void* batch_worker(void* arg) {
     int nat = (int)(a1 - a0), npc = (int)(p1 - p0);
     double e = NAN;
     int32_t it = 0;
+    oracle_options o = *job->opt;
+    if (job->efield)
+      for (int k = 0; k < 3; ++k) o.efield[k] = job->efield[3 * s + k];
     int st = oracle_singlepoint(nat, job->Z + a0, job->xyz + 3 * a0, job->chg[s], job->uhf[s], npc,
                                 npc ? job->pxyz + 3 * p0 : NULL, npc ? job->pq + p0 : NULL,
-                                npc ? job->pg + p0 : NULL, job->opt, &e,
+                                npc ? job->pg + p0 : NULL, &o, &e,
                                 job->g ? job->g + 3 * a0 : NULL, job->q ? job->q + a0 : NULL,
                                 (job->pgr && npc) ? job->pgr + 3 * p0 : NULL, NULL, &it, NULL);
     if (st != ST_OK) {
@@ -2034,11 +2077,12 @@ int oracle_batch(int64_t batch, const int64_t* atom_offsets, const int32_t* Z, c
                  const double* charges, const int32_t* unpaired, const int64_t* point_offsets,
                  const double* pc_xyz, const double* pc_q, const double* pc_gamma,
                  const oracle_options* options, int32_t threads, double* energies, double* grad,
-                 double* atomic_charges, double* pc_grad, int32_t* iterations, int32_t* status) {
+                 double* atomic_charges, double* pc_grad, int32_t* iterations, int32_t* status,
+                 const double* efield) {
   if (!la_dsyevd) return ST_INVALID;
   BatchJob job = {batch, atom_offsets, point_offsets, Z, unpaired, xyz, charges, pc_xyz, pc_q,
                   pc_gamma, options, energies, grad, atomic_charges, pc_grad, iterations, status,
-                  0, PTHREAD_MUTEX_INITIALIZER};
+                  0, PTHREAD_MUTEX_INITIALIZER, efield};
   if (threads <= 0) threads = (int32_t)sysconf(_SC_NPROCESSORS_ONLN);
   if (threads > 256) threads = 256;
   pthread_t tid[256];

@@ -26,6 +26,7 @@ typedef struct oracle_options {
   double energy_tolerance;  /* default 1e-8 */
   double kt;                /* k_B*T in Hartree, default 300 K */
   double damping;           /* default 0.4 */
+  double efield[3];         /* uniform electric field, Hartree/(e bohr); default 0 (none) */
 } oracle_options;
 
 /* Optional diagnostics for term-level pinning; every pointer may be NULL. */
@@ -87,7 +88,8 @@ int oracle_batch(int64_t batch, const int64_t* atom_offsets, const int32_t* Z, c
                  const double* charges, const int32_t* unpaired, const int64_t* point_offsets,
                  const double* pc_xyz, const double* pc_q, const double* pc_gamma,
                  const oracle_options* options, int32_t threads, double* energies, double* grad,
-                 double* atomic_charges, double* pc_grad, int32_t* iterations, int32_t* status);
+                 double* atomic_charges, double* pc_grad, int32_t* iterations, int32_t* status,
+                 const double* efield /* [batch][3] uniform fields, or NULL */);
 
 /*
  * Replay exactly one SCC step from a given mixed state [qsh, dip(3N), quad(6N)]

@@ -38,12 +38,17 @@ def run(ctx, batch, **kw):
                                     batch.unpaired_electrons, **pc, flags=flags, **kw)
 
 
-def compare(batch, out, label, opts=None, iter_tol=1):
+def compare(batch, out, label, opts=None, iter_tol=1, fields=None, opt_kw=None):
+    """fields: per-system uniform field (or None); such a system's oracle run
+    uses O.options(**opt_kw, efield=field)."""
     worst = dict(E=0.0, F=0.0, q=0.0, PF=0.0, it=0)
     for i in range(batch.batch_size):
         s = batch.system(i)
         kw = {k: s[k] for k in ("pc_xyz", "pc_q", "pc_gamma") if k in s and len(s["pc_q"])}
-        ref = O.singlepoint(s["Z"], s["xyz"], charge=s["charge"], uhf=s["uhf"], opts=opts, **kw)
+        o = opts
+        if fields is not None and fields[i] is not None:
+            o = O.options(**(opt_kw or {}), efield=fields[i])
+        ref = O.singlepoint(s["Z"], s["xyz"], charge=s["charge"], uhf=s["uhf"], opts=o, **kw)
         a0, a1 = batch.atom_offsets[i], batch.atom_offsets[i + 1]
         assert ref.status == 0, (label, i)
         assert out.per_system_status[i] == 0, (label, i, out.per_system_status[i])
@@ -103,22 +108,27 @@ def test_conformance_goldens(ctx):
         systems.append(s)
         names.append(c["id"])
     b = workloads.from_systems(systems, names)
+    fields = [c.get("efield") for c in cases]
     print("case conformance", names, flush=True)
-    out = run(ctx, b, charge_tolerance=1e-8, energy_tolerance=1e-10)
+    out = run(ctx, b, charge_tolerance=1e-8, energy_tolerance=1e-10,
+              electric_field=fields if any(f is not None for f in fields) else None)
     tol = conf["tolerances"]
     for i, c in enumerate(cases):
         a0, a1 = b.atom_offsets[i], b.atom_offsets[i + 1]
         assert out.per_system_status[i] == 0, c["id"]
         assert abs(out.energies[i] - c["energy"]) <= tol["energy"], c["id"]
-        assert np.abs(out.forces[a0:a1].ravel() - np.array(c["forces"])).max() <= tol["forces"], c["id"]
+        # a case may pin only its energy (the field case: its forces are
+        # checked against the oracle below and against central differences)
+        if "forces_hartree_per_bohr" in c.get("oracle_properties", ["forces_hartree_per_bohr"]):
+            assert np.abs(out.forces[a0:a1].ravel() - np.array(c["forces"])).max() <= tol["forces"], c["id"]
         if "charges" in c:
             assert np.abs(out.charges[a0:a1] - np.array(c["charges"])).max() <= tol["charges"]
         if "pc_forces" in c:
             p0, p1 = b.point_offsets[i], b.point_offsets[i + 1]
             assert np.abs(out.point_charge_forces[p0:p1].ravel()
                           - np.array(c["pc_forces"])).max() <= tol["point_charge_forces"], c["id"]
-    compare(b, out, "conformance-vs-oracle", opts=O.options(charge_tolerance=1e-8,
-                                                            energy_tolerance=1e-10))
+    compare(b, out, "conformance-vs-oracle", opts=O.options(charge_tolerance=1e-8, energy_tolerance=1e-10),
+            fields=fields, opt_kw=dict(charge_tolerance=1e-8, energy_tolerance=1e-10))
 
 
 def test_ragged_alkanes(ctx):

@@ -33,11 +33,15 @@ def test_conformance_golden(case):
     if "pc_q" in case:
         kw = dict(pc_xyz=case["pc_xyz"], pc_q=case["pc_q"], pc_gamma=case["pc_gamma"])
     # goldens were produced at --acc 1e-4; converge tighter so only oracle noise remains
-    r = O.singlepoint(case["Z"], case["xyz"], charge=case["charge"], uhf=case["unpaired"],
-                      opts=O.options(charge_tolerance=1e-8, energy_tolerance=1e-10), **kw)
+    opts = O.options(charge_tolerance=1e-8, energy_tolerance=1e-10, efield=case.get("efield", (0, 0, 0)))
+    r = O.singlepoint(case["Z"], case["xyz"], charge=case["charge"], uhf=case["unpaired"], opts=opts, **kw)
     assert r.status == 0
     assert abs(r.energy - case["energy"]) <= tol["energy"]
-    assert np.abs(-r.gradient.ravel() - np.array(case["forces"])).max() <= tol["forces"]
+    # a case may pin only some properties (the field case pins its energy;
+    # its forces are checked against central differences below)
+    props = case.get("oracle_properties", ["energy_hartree", "forces_hartree_per_bohr"])
+    if "forces_hartree_per_bohr" in props:
+        assert np.abs(-r.gradient.ravel() - np.array(case["forces"])).max() <= tol["forces"]
     if "charges" in case:
         assert np.abs(r.charges - np.array(case["charges"])).max() <= tol["charges"]
     if "pc_forces" in case:
@@ -139,6 +143,69 @@ def test_forces_central_difference():
         em = O.singlepoint(s["Z"], s["xyz"], opts=tight, pc_xyz=xm, pc_q=s["pc_q"],
                            pc_gamma=s["pc_gamma"]).energy
         assert abs((ep - em) / (2 * h) - r.pc_gradient[p, 1]) < 2e-7
+
+
+def test_field_forces_central_difference():
+    """Uniform electric field: E = E0 - E.(sum_A q_A R_A + sum_A mu_A); its force
+    term and the field's response through the SCC both enter the gradient."""
+    from xtbloom_amd import workloads as W
+    s = W.water(np.random.default_rng(4))
+    field = (0.003, -0.004, 0.005)
+    tight = O.options(charge_tolerance=1e-10, energy_tolerance=1e-12, efield=field)
+    r = O.singlepoint(s["Z"], s["xyz"], opts=tight)
+    plain = O.singlepoint(s["Z"], s["xyz"], opts=O.options(charge_tolerance=1e-10, energy_tolerance=1e-12))
+    assert abs(r.energy - plain.energy) > 1e-5  # the field acts
+    h = 1e-4
+    for atom, axis in ((0, 0), (1, 1), (2, 2), (0, 2)):
+        xp = s["xyz"].copy(); xp[atom, axis] += h
+        xm = s["xyz"].copy(); xm[atom, axis] -= h
+        ep = O.singlepoint(s["Z"], xp, opts=tight).energy
+        em = O.singlepoint(s["Z"], xm, opts=tight).energy
+        assert abs((ep - em) / (2 * h) - r.gradient[atom, axis]) < 2e-7
+    # a neutral molecule's energy does not depend on the origin; the net
+    # force is -(total charge) E = 0
+    assert np.abs(r.gradient.sum(axis=0)).max() < 1e-9
+    shifted = O.singlepoint(s["Z"], s["xyz"] + np.array([1.5, -2.0, 0.7]), opts=tight)
+    assert abs(shifted.energy - r.energy) < 1e-10
+    # Hellmann-Feynman: the SCC energy is stationary in the density, so
+    # dE/dE_k at E = 0 is minus the dipole moment sum_A (q_A R_A + mu_A)
+    e = 1e-4
+    base = O.singlepoint(s["Z"], s["xyz"], opts=O.options(charge_tolerance=1e-10, energy_tolerance=1e-12),
+                         trace=True)
+    mu = (base.charges[:, None] * s["xyz"]).sum(axis=0) + base.trace["dipoles"].sum(axis=0)
+    assert np.abs(mu).min() > 1e-2
+    for k in range(3):
+        f = [0.0, 0.0, 0.0]
+        f[k] = e
+        ep = O.singlepoint(s["Z"], s["xyz"], opts=O.options(charge_tolerance=1e-10, energy_tolerance=1e-12,
+                                                            efield=tuple(f))).energy
+        f[k] = -e
+        em = O.singlepoint(s["Z"], s["xyz"], opts=O.options(charge_tolerance=1e-10, energy_tolerance=1e-12,
+                                                            efield=tuple(f))).energy
+        assert abs((ep - em) / (2 * e) + mu[k]) < 1e-7
+
+
+# water in the tblite h2o.xyz orientation (angstrom -> bohr) at the field
+# (0.001, 0.002, -0.0015) a.u.: energy -4.7652477392228 Eh at abs 1e-7
+# (reference python/tests/test_interface.py:459-488)
+TBLITE_WATER_Z = [8, 1, 1]
+TBLITE_WATER_XYZ = np.array([[0.0, 0.0, -0.2358784530], [0.0, 1.4270063049, 1.0081495306],
+                             [0.0, -1.4270063049, 1.0081495306]]) * 1.8897261246257702
+TBLITE_WATER_FIELD = (0.001, 0.002, -0.0015)
+TBLITE_WATER_ENERGY = -4.7652477392228
+
+
+def test_field_golden_tblite_water():
+    r = O.singlepoint(TBLITE_WATER_Z, TBLITE_WATER_XYZ,
+                      opts=O.options(charge_tolerance=1e-10, energy_tolerance=1e-12, efield=TBLITE_WATER_FIELD))
+    assert r.status == 0
+    assert abs(r.energy - TBLITE_WATER_ENERGY) <= 1e-7
+    # the field polarizes the molecule: charges move, their sum stays zero
+    plain = O.singlepoint(TBLITE_WATER_Z, TBLITE_WATER_XYZ,
+                          opts=O.options(charge_tolerance=1e-10, energy_tolerance=1e-12))
+    assert abs(r.charges.sum()) < 1e-12
+    assert abs(r.charges[0] - plain.charges[0]) > 1e-6
+    assert np.linalg.norm(r.gradient - plain.gradient) > 1e-4
 
 
 def test_batch_equals_sequential():

@@ -11,6 +11,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstddef>
+#include <cstdint>
 #include <cstdlib>
 #include <condition_variable>
 #include <cstring>
@@ -22,6 +23,7 @@
 #include <numeric>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/xtbloom/xtbloom.h"
@@ -38,13 +40,21 @@ static_assert(xtb::kNoAllocWorkspaceBytes == XTBLOOM_AMD_NO_ALLOC_WORKSPACE_BYTE
 // (reference xtbloom.h:502-520): the topology (atom offsets, element numbers,
 // point-charge offsets and hardnesses, spin channels), molecular charges,
 // unpaired electrons and the compute policy.  Geometry (atom and point-charge
-// positions, point-charge values) is not part of it.
+// positions, point-charge values) is not part of it.  The uniform-field
+// attachments (presence per system and the field vectors, reference
+// gfn2_cpu_execution.cpp:564-597) belong to the strict WARM identity only: a
+// plan accepts another field, as it accepts another geometry
+// (gfn2_cpu_execution.cpp:657-671).
 struct CallIdentity {
   std::vector<char> atom_offsets, Z, charges, unpaired, point_offsets, pc_gamma, spin;
   uint32_t flags = 0;
   int32_t model = 0, max_iterations = 0, history = 0, determinism = 0;
   double charge_tolerance = 0.0, energy_tolerance = 0.0, kt = 0.0, damping = 0.0;
-  bool operator==(const CallIdentity& o) const {
+  // empty when the call attaches no field, else one entry (three) per system;
+  // the vectors compare by value, as the reference's key does
+  std::vector<uint8_t> field_on;
+  std::vector<double> field;
+  bool same_plan(const CallIdentity& o) const {
     // doubles compare by their bits: the identity is exact
     auto same = [](double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; };
     return atom_offsets == o.atom_offsets && Z == o.Z && charges == o.charges && unpaired == o.unpaired &&
@@ -52,6 +62,9 @@ struct CallIdentity {
            model == o.model && max_iterations == o.max_iterations && history == o.history &&
            determinism == o.determinism && same(charge_tolerance, o.charge_tolerance) &&
            same(energy_tolerance, o.energy_tolerance) && same(kt, o.kt) && same(damping, o.damping);
+  }
+  bool operator==(const CallIdentity& o) const {
+    return same_plan(o) && field_on == o.field_on && field == o.field;
   }
 };
 
@@ -308,7 +321,122 @@ xtbloom_status_t publish(const xtbloom_buffer_t& b, const void* src, size_t byte
 
 struct Staged {
   std::vector<char> atom_offsets, Z, xyz, charges, unpaired, poff, pxyz, pq, pg, spin;
+  // uniform-field attachments: empty without any, else [B] flags and [3B] vectors
+  std::vector<uint8_t> field_on;
+  std::vector<double> field;
 };
+
+bool known_interaction(int32_t type) {
+  switch (type) {
+    case XTBLOOM_INTERACTION_ELECTRIC_FIELD:
+    case XTBLOOM_INTERACTION_ELECTRIC_FIELD_GRADIENT:
+    case XTBLOOM_INTERACTION_POINT_CHARGES_MULTIPOLE:
+    case XTBLOOM_INTERACTION_ATOMIC_POTENTIAL_GRID:
+    case XTBLOOM_INTERACTION_ALPB_SOLVATION:
+    case XTBLOOM_INTERACTION_GBSA_SOLVATION:
+    case XTBLOOM_INTERACTION_GB_SOLVATION:
+    case XTBLOOM_INTERACTION_GBE_SOLVATION:
+    case XTBLOOM_INTERACTION_DDX_SOLVATION:
+    case XTBLOOM_INTERACTION_D3_DISPERSION:
+    case XTBLOOM_INTERACTION_D4_VARIANT_DISPERSION:
+    case XTBLOOM_INTERACTION_HALOGEN_BOND:
+      return true;
+    default:
+      return false;
+  }
+}
+
+// ABI-v3 interaction attachments (reference xtbloom.h:160-222, validation.cpp:
+// 552-625, 920-1010, 1115-1140): every descriptor is checked structurally
+// (flags zero, a known tag other than NONE, system index in the batch, the
+// block inside the payload, an aligned block_version header), the uniform
+// electric field block exactly (32 bytes, 8-byte aligned, block_version 1,
+// reserved zero, finite components), at most one attachment per (system,
+// tag); only then do the reserved tags return NOT_IMPLEMENTED.  Descriptors
+// and payload may be host or ROCm device memory; they are staged once.
+xtbloom_status_t stage_interactions(const xtbloom_batch_t* batch, int64_t B, Staged& st) {
+  st.field_on.clear();
+  st.field.clear();
+  if (batch->struct_size < XTBLOOM_BATCH_V3_SIZE) return XTBLOOM_STATUS_SUCCESS;
+  const int64_t count = batch->total_interactions;
+  if (count < 0) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "total_interactions must be nonnegative");
+  if (count == 0) return XTBLOOM_STATUS_SUCCESS;
+  if (count > ((int64_t)1 << 40) / (int64_t)sizeof(xtbloom_interaction_t))
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "total_interactions overflows the addressable byte size");
+  std::vector<char> desc, payload;
+  xtbloom_status_t s;
+  if ((s = stage_input(batch->interaction_descriptors, sizeof(xtbloom_interaction_t) * (size_t)count,
+                       "interaction_descriptors", desc, true)))
+    return s;
+  if (!buffer_active(batch->interaction_payload))
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "interaction_payload is required when interactions are present");
+  const uint64_t pbytes = batch->interaction_payload.size_bytes;
+  const uintptr_t pbase = reinterpret_cast<uintptr_t>(batch->interaction_payload.data);
+  std::vector<xtbloom_interaction_t> items((size_t)count);
+  std::memcpy(items.data(), desc.data(), desc.size());
+  bool any_field = false, reserved_tag = false;
+  for (const xtbloom_interaction_t& it : items) {
+    if (it.flags != 0) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "interaction descriptor flags must be zero");
+    if (it.type == XTBLOOM_INTERACTION_NONE || !known_interaction(it.type))
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "an interaction descriptor uses an unknown or NONE type tag");
+    if (it.system_index < 0 || it.system_index >= B)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "an interaction system_index lies outside the batch");
+    if (it.payload_offset > pbytes || it.payload_size > pbytes - it.payload_offset)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "an interaction payload block extends past interaction_payload");
+    if (it.payload_offset > (uint64_t)(UINTPTR_MAX - pbase))
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "an interaction payload block address overflows uintptr_t");
+    const uintptr_t addr = pbase + (uintptr_t)it.payload_offset;
+    if (it.type != XTBLOOM_INTERACTION_ELECTRIC_FIELD) {
+      if (it.payload_size < sizeof(int32_t) || addr % alignof(int32_t) != 0)
+        return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "an interaction payload block must contain an aligned block_version");
+      reserved_tag = true;
+      continue;
+    }
+    if (it.payload_size < 32 || addr % alignof(double) != 0)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "an electric-field interaction payload block is undersized or misaligned");
+    if (it.payload_size > 32)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "an electric-field interaction payload block exceeds the released contract");
+    any_field = true;
+  }
+  // the field blocks' contents
+  if (any_field) {
+    if ((s = stage_input(batch->interaction_payload, (size_t)pbytes, "interaction_payload", payload, true))) return s;
+    for (const xtbloom_interaction_t& it : items) {
+      if (it.type != XTBLOOM_INTERACTION_ELECTRIC_FIELD) continue;
+      const char* block = payload.data() + it.payload_offset;
+      int32_t version = 0, reserved = 0;
+      double E[3];
+      std::memcpy(&version, block, sizeof(version));
+      std::memcpy(&reserved, block + 4, sizeof(reserved));
+      std::memcpy(E, block + 8, sizeof(E));
+      if (version != 1)
+        return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "an electric-field interaction uses an unsupported block_version");
+      if (reserved != 0)
+        return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "an electric-field interaction reserved payload field must be zero");
+      if (!std::isfinite(E[0]) || !std::isfinite(E[1]) || !std::isfinite(E[2]))
+        return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "an electric-field interaction contains NaN or infinity");
+    }
+  }
+  // at most one attachment per (system, tag)
+  {
+    std::vector<std::pair<int64_t, int32_t>> keys;
+    keys.reserve(items.size());
+    for (const xtbloom_interaction_t& it : items) keys.emplace_back(it.system_index, it.type);
+    std::sort(keys.begin(), keys.end());
+    if (std::adjacent_find(keys.begin(), keys.end()) != keys.end())
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "the same system has two attachments of the same interaction type");
+  }
+  if (reserved_tag)
+    return fail(XTBLOOM_STATUS_NOT_IMPLEMENTED,
+                "an interaction attachment uses a tag whose execution is reserved but not implemented");
+  st.field_on.assign((size_t)B, 0);
+  st.field.assign(3 * (size_t)B, 0.0);
+  for (const xtbloom_interaction_t& it : items) {
+    std::memcpy(&st.field[3 * (size_t)it.system_index], payload.data() + it.payload_offset + 8, 3 * sizeof(double));
+    st.field_on[(size_t)it.system_index] = 1;
+  }
+  return XTBLOOM_STATUS_SUCCESS;
+}
 
 // result may be NULL (plan creation: no outputs are checked).
 xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_compute_options_t* options,
@@ -399,8 +527,6 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
   if (buffer_active(batch->atomic_potential_shifts) || buffer_active(batch->charge_response_offsets) ||
       buffer_active(batch->charge_response_matrix) || batch->total_charge_response_elements != 0)
     return fail(XTBLOOM_STATUS_NOT_SUPPORTED, "periodic b + A q embedding is not supported by the ROCm backend");
-  if (batch->struct_size >= XTBLOOM_BATCH_V3_SIZE && batch->total_interactions != 0)
-    return fail(XTBLOOM_STATUS_NOT_SUPPORTED, "interaction attachments are not supported by the ROCm backend");
   // stage inputs
   if ((s = stage_input(batch->atom_offsets, sizeof(int64_t) * (B + 1), "atom_offsets", st.atom_offsets, true)) ||
       (s = stage_input(batch->atomic_numbers, sizeof(int32_t) * N, "atomic_numbers", st.Z, true)) ||
@@ -416,6 +542,7 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
       if (ch[i] != 1) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "spin_channels must be 1 or 2");
     }
   }
+  if ((s = stage_interactions(batch, B, st))) return s;
   if (batch->struct_size >= XTBLOOM_BATCH_V4_SIZE &&
       (buffer_active(batch->cell_matrices) || buffer_active(batch->periodic_axes)))
     return fail(XTBLOOM_STATUS_NOT_IMPLEMENTED, "periodic batches are not implemented");
@@ -482,6 +609,7 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
     hb.pc_q = reinterpret_cast<const double*>(st.pq.data());
     hb.pc_gamma = reinterpret_cast<const double*>(st.pg.data());
   }
+  hb.field = st.field.empty() ? nullptr : st.field.data();
   ro.forces = want_f || want_pf;
   return XTBLOOM_STATUS_SUCCESS;
 }
@@ -496,6 +624,8 @@ CallIdentity make_identity(const Staged& st, const xtbloom_compute_options_t* op
   id.point_offsets = st.poff;
   id.pc_gamma = st.pg;
   id.spin = st.spin;
+  id.field_on = st.field_on;
+  id.field = st.field;
   id.flags = flags;
   id.model = options->model;
   id.max_iterations = ro.max_iterations;
@@ -569,7 +699,7 @@ xtbloom_status_t admit(xtbloom_context_t* context, xtbloom_plan_t* plan, const x
   if (ext != nullptr && !valid_header(ext, sizeof(xtbloom_amd_extension_t)))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "extension descriptor is invalid");
   job.id = make_identity(job.st, options, job.ro, job.flags);
-  if (plan != nullptr && !(job.id == plan->id))
+  if (plan != nullptr && !job.id.same_plan(plan->id))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT,
                 "the batch topology or compute policy does not match the fixed-topology plan");
   WarmCheckpoint& cp = plan ? plan->warm : context->warm;

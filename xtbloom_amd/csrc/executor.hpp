@@ -22,6 +22,7 @@ struct HostBatch {
   const double* pc_xyz = nullptr;         // [3P]
   const double* pc_q = nullptr;           // [P]
   const double* pc_gamma = nullptr;       // [P]
+  const double* field = nullptr;          // [3B] uniform electric field per system, or null
 };
 
 struct RunOptions {

@@ -73,6 +73,9 @@ struct Topo {
   const double* pc_xyz;
   const double* pc_q;
   const double* pc_gam;
+  // uniform electric field per system [3B] (Hartree / (e bohr)), or null when
+  // no system of the call carries one
+  const double* field;
   // global tables
   const double* d4_c6;      // [262*262]
   const double* d4_refcn;

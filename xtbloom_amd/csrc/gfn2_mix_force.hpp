@@ -60,6 +60,12 @@ __global__ __launch_bounds__(512) void k_energy_mix(Topo t, const int* order, un
     e[1] += t.sh_gam3[sh] * qsh[i] * qsh[i] * qsh[i] / 3.0;
     e[2] = fma(qsh[i], vpc[sh], e[2]);
   }
+  // a uniform field's dipole term -E.mu_A joins the external energy (its
+  // charge term -q_A E.R_A is part of vpc)
+  if (t.field) {
+    const double* E = t.field + 3 * s;
+    for (int i = tid; i < 3 * N; i += nt) e[2] -= E[i % 3] * dip[i];
+  }
   // AES2 and D4 two-body: every ordered pair once, halved.  The AES2 pair
   // energy equals q_a V_q + d_a.V_d + Q_a.V_Q of atom a in the field of b,
   // symmetric in (a, b).
@@ -650,6 +656,13 @@ __global__ __launch_bounds__(512) void k_force_atoms(Topo t, const int* order,
       for (int c = 0; c < 3; ++c) pc_grad[3 * p + c] = g[c];
       dedq[p] = dq;
     }
+  }
+  // phase 5: explicit force of a uniform field on the atomic charges,
+  // dE/dR_A = -q_A E (same atom-to-thread map as phase 4)
+  if (t.field) {
+    const double* E = t.field + 3 * s;
+    for (int a = a0 + tid; a < a1; a += nt)
+      for (int c = 0; c < 3; ++c) grad[3 * a + c] = fma(-qat[a], E[c], grad[3 * a + c]);
   }
 }
 

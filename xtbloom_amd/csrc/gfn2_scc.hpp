@@ -79,6 +79,8 @@ __global__ __launch_bounds__(256) void k_geometry(Topo t, const int* order, doub
     }
   }
   // point-charge shell potential (reference external_point_charges.hpp:59)
+  // and the scalar potential -E.R_A of a uniform field on the charge of the
+  // shell's atom (reference gfn2_cpu_execution.cpp:1294-1302)
   const int p0 = t.pc_off[s], p1 = t.pc_off[s + 1];
   for (int sh = s0 + tid; sh < s1; sh += nt) {
     const int A = t.sh_atom[sh];
@@ -88,6 +90,10 @@ __global__ __launch_bounds__(256) void k_geometry(Topo t, const int* order, doub
                    dz = t.xyz[3 * A + 2] - t.pc_xyz[3 * p + 2];
       const double ia = 2.0 / (t.sh_eta[sh] + t.pc_gam[p]);
       v += t.pc_q[p] / sqrt(dx * dx + dy * dy + dz * dz + ia * ia);
+    }
+    if (t.field) {
+      const double* E = t.field + 3 * s;
+      v -= E[0] * t.xyz[3 * A] + E[1] * t.xyz[3 * A + 1] + E[2] * t.xyz[3 * A + 2];
     }
     vpc[sh] = v;
   }
@@ -509,8 +515,10 @@ __global__ __launch_bounds__(512, 4) void k_potentials(Topo t, const int* order,
         double pd4 = 0.0;
         for (int r = 0; r < kD4MaxRef; ++r) pd4 -= dw4[kD4MaxRef * a + r] * o.z[r];
         vat[a] = o.pq + pd4;
+        // a uniform field adds -E to every atomic dipole potential
+        const double* E = t.field ? t.field + 3 * s : nullptr;
         for (int c = 0; c < 3; ++c)
-          vd[3 * a + c] = 2.0 * t.at_dk[a] * S.dp[c * N + pa] + o.pd[c];
+          vd[3 * a + c] = 2.0 * t.at_dk[a] * S.dp[c * N + pa] + o.pd[c] - (E ? E[c] : 0.0);
         for (int c = 0; c < 6; ++c)
           vqq[6 * a + c] = 2.0 * t.at_qk[a] * qscale[c] * S.qp[c * N + pa] + o.pQ[c];
       }

@@ -113,7 +113,7 @@ bool is_contiguous(const std::vector<int64_t>& m) {
 struct Gathered {
   std::vector<int64_t> atom_offsets, point_offsets;
   std::vector<int32_t> Z, unpaired, iterations, status;
-  std::vector<double> xyz, charges, pc_xyz, pc_q, pc_gamma, warm;
+  std::vector<double> xyz, charges, pc_xyz, pc_q, pc_gamma, field, warm;
   std::vector<double> energies, gradient, atom_charges, pc_gradient, pc_dedq, dipoles, scc_state,
       dbg_overlap, dbg_h0, dbg_density, dbg_cn, dbg_components, dbg_levels, dbg_occ;
   std::vector<uint8_t> converged;
@@ -221,6 +221,7 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
       b.xyz = hb.xyz + 3 * a0;
       b.charges = hb.charges + x.s0;
       b.unpaired = hb.unpaired + x.s0;
+      b.field = at(hb.field, 3 * x.s0);
       int64_t p0 = 0;
       if (hb.point_offsets) {
         p0 = hb.point_offsets[x.s0];
@@ -266,6 +267,7 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
         g.xyz.insert(g.xyz.end(), hb.xyz + 3 * a0, hb.xyz + 3 * a1);
         g.charges.push_back(hb.charges[s]);
         g.unpaired.push_back(hb.unpaired[s]);
+        if (hb.field) g.field.insert(g.field.end(), hb.field + 3 * s, hb.field + 3 * s + 3);
         if (hb.point_offsets) {
           const int64_t p0 = hb.point_offsets[s], p1 = hb.point_offsets[s + 1];
           g.point_offsets.push_back(g.point_offsets.back() + (p1 - p0));
@@ -287,6 +289,7 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
       b.xyz = g.xyz.data();
       b.charges = g.charges.data();
       b.unpaired = g.unpaired.data();
+      b.field = hb.field ? g.field.data() : nullptr;
       if (hb.point_offsets) {
         b.points = np;
         b.point_offsets = g.point_offsets.data();

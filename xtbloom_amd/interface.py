@@ -63,11 +63,30 @@ class PointCharge:
             raise XTBloomValueError("point charge inputs must be finite")
 
 
+def _normalize_efield(efield):
+    """A uniform electric field (Hartree per elementary charge per bohr) as a
+    frozen length-3 vector; ``None`` and the zero vector mean no attachment
+    (reference python/xtbloom/interface.py:273-289)."""
+    if efield is None:
+        return None
+    value = np.asarray(efield, dtype=float)
+    if value.ndim != 1 or value.shape[0] != 3:
+        raise XTBloomValueError("efield must be a vector of length 3")
+    if not np.isfinite(value).all():
+        raise XTBloomValueError("efield must be finite")
+    if np.any(value != 0.0):
+        value = np.array(value)
+        value.flags.writeable = False
+        return value
+    return None
+
+
 class Structure:
-    """One molecule: atomic numbers, positions (bohr), charge, uhf, optional point charges."""
+    """One molecule: atomic numbers, positions (bohr), charge, uhf, optional
+    point charges and an optional uniform electric field (atomic units)."""
 
     def __init__(self, numbers, positions, charge: float = 0.0, uhf: int | None = None,
-                 point_charges: PointCharge | None = None):
+                 point_charges: PointCharge | None = None, efield=None):
         self._numbers = np.ascontiguousarray(numbers, dtype=np.int32)
         self._positions = np.ascontiguousarray(positions, dtype=float).reshape(-1, 3)
         if self._numbers.ndim != 1 or len(self._numbers) != len(self._positions):
@@ -75,12 +94,14 @@ class Structure:
         self._charge = float(charge)
         self._uhf = int(uhf or 0)
         self._point_charges = point_charges
+        self._efield = _normalize_efield(efield)
 
     numbers = property(lambda self: self._numbers)
     positions = property(lambda self: self._positions)
     charge = property(lambda self: self._charge)
     uhf = property(lambda self: self._uhf)
     point_charges = property(lambda self: self._point_charges)
+    efield = property(lambda self: self._efield)
 
     def update(self, positions=None, point_charges=None):
         if positions is not None:
@@ -171,10 +192,36 @@ class ComputedBatch:
 _DEFAULT_FLAGS = library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES
 
 
+def _field_attachments(B: int, electric_field):
+    """Interaction descriptors and the payload for uniform-field attachments
+    (reference python/xtbloom/interface.py:1172-1184).  ``electric_field`` is a
+    (B, 3) array (every system attached, zero rows included) or a sequence of B
+    entries, each None (no attachment) or a length-3 vector."""
+    rows = list(electric_field)
+    if len(rows) != B:
+        raise XTBloomValueError("electric_field needs one entry per system")
+    attached = [(i, np.asarray(r, dtype=float)) for i, r in enumerate(rows) if r is not None]
+    desc = (library.Interaction * max(1, len(attached)))()
+    payload = np.zeros(max(1, len(attached)) * library.ELECTRIC_FIELD_BLOCK_BYTES // 8, np.float64)
+    head = payload.view(np.int32)
+    for k, (i, field) in enumerate(attached):
+        if field.shape != (3,):
+            raise XTBloomValueError("an electric field must be a vector of length 3")
+        desc[k].type = library.INTERACTION_ELECTRIC_FIELD
+        desc[k].flags = 0
+        desc[k].system_index = i
+        desc[k].payload_offset = k * library.ELECTRIC_FIELD_BLOCK_BYTES
+        desc[k].payload_size = library.ELECTRIC_FIELD_BLOCK_BYTES
+        head[8 * k] = 1  # block_version
+        head[8 * k + 1] = 0  # reserved
+        payload[4 * k + 1:4 * k + 4] = field
+    return len(attached), desc, payload
+
+
 def _descriptors(lib, atom_offsets, numbers, positions, charges, uhf, point_offsets,
                  point_positions, point_values, point_gammas, model, flags, max_scc_iterations,
                  charge_tolerance, energy_tolerance, electronic_temperature, scc_mixer_history,
-                 scc_mixer_damping, scc_start_mode, determinism=0):
+                 scc_mixer_damping, scc_start_mode, determinism=0, electric_field=None):
     """Batch and compute-options descriptors over flat host arrays (kept alive by the caller)."""
     atom_offsets = np.ascontiguousarray(atom_offsets, dtype=np.int64)
     numbers = np.ascontiguousarray(numbers, dtype=np.int32)
@@ -204,6 +251,15 @@ def _descriptors(lib, atom_offsets, numbers, positions, charges, uhf, point_offs
         batch.point_charge_positions = _const(point_positions)
         batch.point_charge_values = _const(point_values)
         batch.point_charge_gammas = _const(point_gammas)
+    fields = None
+    if electric_field is not None:
+        count, desc, payload = _field_attachments(B, electric_field)
+        if count:
+            fields = (desc, payload)
+            batch.total_interactions = count
+            batch.interaction_descriptors = library.ConstBuffer(ctypes.addressof(desc), ctypes.sizeof(desc),
+                                                                library.MEMORY_HOST, 0)
+            batch.interaction_payload = _const(payload)
     opts = library.ComputeOptions()
     _check(lib.xtbloom_compute_options_init(ctypes.byref(opts), ctypes.sizeof(opts)))
     opts.model = model
@@ -217,7 +273,7 @@ def _descriptors(lib, atom_offsets, numbers, positions, charges, uhf, point_offs
     opts.scc_start_mode = scc_start_mode
     opts.determinism = determinism
     keep = (atom_offsets, numbers, positions, charges, uhf, point_offsets, point_positions,
-            point_values, point_gammas)
+            point_values, point_gammas, fields)
     return batch, opts, keep, (B, N, P)
 
 
@@ -230,7 +286,7 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                    request: "Request | None" = None,
                    extension=False, diagnostics=False, kernel_stats=False,
                    determinism=library.DETERMINISM_DEFAULT, options_hook=None,
-                   no_alloc=False, wait=True) -> "ComputedBatch | PendingBatch":
+                   no_alloc=False, wait=True, electric_field=None) -> "ComputedBatch | PendingBatch":
     """One ``xtbloom_compute`` over flat ragged arrays (the ``xtbloom_batch_t`` layout).
 
     ``scc_start_mode=library.SCC_START_WARM`` starts from the converged state of
@@ -244,14 +300,16 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
     allocation; the batch must fit its fixed workspace).  With ``request`` the
     call is submitted through ``xtbloom_compute_enqueue`` (or the plan's
     enqueue); ``wait=False`` returns a :class:`PendingBatch` at once, whose
-    ``result()`` waits for the request.
+    ``result()`` waits for the request.  ``electric_field`` attaches uniform
+    fields (a (B, 3) array, or B entries of None or a length-3 vector) as
+    ABI-v3 interaction attachments.
     """
     lib = library.load()
     batch, opts, keep, (B, N, P) = _descriptors(
         lib, atom_offsets, numbers, positions, charges, uhf, point_offsets, point_positions,
         point_values, point_gammas, model, flags, max_scc_iterations, charge_tolerance,
         energy_tolerance, electronic_temperature, scc_mixer_history, scc_mixer_damping,
-        scc_start_mode, determinism)
+        scc_start_mode, determinism, electric_field)
     if options_hook is not None:
         options_hook(opts)
     atom_offsets, point_offsets = keep[0], keep[5]
@@ -461,11 +519,13 @@ class Plan:
         return self._handle
 
     def compute(self, positions, point_positions=None, point_values=None, *,
-                scc_start_mode=library.SCC_START_FRESH) -> ComputedBatch:
-        """One ``xtbloom_plan_compute`` with new geometry."""
+                scc_start_mode=library.SCC_START_FRESH, electric_field=None) -> ComputedBatch:
+        """One ``xtbloom_plan_compute`` with new geometry (and, optionally, new
+        uniform fields: a plan accepts any field, a WARM start only the
+        checkpoint's)."""
         return compute_arrays(self._context, positions=positions, point_positions=point_positions,
                               point_values=point_values, scc_start_mode=scc_start_mode, plan=self,
-                              **self._topology, **self._policy)
+                              electric_field=electric_field, **self._topology, **self._policy)
 
     def query_workspace(self) -> dict:
         """Reserved workspace (``xtbloom_plan_query_workspace``)."""
@@ -710,6 +770,8 @@ def _flatten(structures: Sequence[Structure]):
                   point_positions=np.concatenate([p.positions for p in pcs]),
                   point_values=np.concatenate([p.charges for p in pcs]),
                   point_gammas=np.concatenate([p.gammas for p in pcs]))
+    if any(s.efield is not None for s in structures):
+        kw["electric_field"] = [s.efield for s in structures]
     return offs, numbers, positions, charges, uhf, kw
 
 
@@ -764,8 +826,8 @@ class Calculator(Structure):
 
     def __init__(self, method: str, numbers, positions, charge: float = 0.0, uhf: int | None = None,
                  point_charges: PointCharge | None = None, *, backend: str | int = "hip",
-                 device_id: int | None = None, warm_start: bool = False, **settings):
-        super().__init__(numbers, positions, charge, uhf, point_charges)
+                 device_id: int | None = None, warm_start: bool = False, efield=None, **settings):
+        super().__init__(numbers, positions, charge, uhf, point_charges, efield)
         self._model = _resolve_method(method)
         self._method = method
         self._context = Context(backend, device_id)
@@ -789,6 +851,9 @@ class Calculator(Structure):
         flags = library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES
         if self.point_charges is not None:
             flags |= library.COMPUTE_POINT_CHARGE_FORCES
+        if self.efield is not None:
+            # a field run publishes the molecular dipole (reference interface.py:1926-1927)
+            flags |= library.COMPUTE_DIPOLE_MOMENTS
         offs, numbers, positions, charges, uhf, kw = _flatten([self])
         computed = _compute_auto_warm(self._context, self._warm_start, offs, numbers, positions,
                                       charges, uhf, **kw, model=self._model, flags=flags,
@@ -825,8 +890,10 @@ class BatchCalculator:
         offs, numbers, positions, charges, uhf, kw = _flatten(self._structures)
         if flags is None:
             flags = library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES
-            if kw:
+            if "point_offsets" in kw:
                 flags |= library.COMPUTE_POINT_CHARGE_FORCES
+            if "electric_field" in kw:
+                flags |= library.COMPUTE_DIPOLE_MOMENTS
         computed = _compute_auto_warm(self._context, self._warm_start, offs, numbers, positions,
                                       charges, uhf, **kw, model=self._model, flags=flags,
                                       **self._settings.kwargs())

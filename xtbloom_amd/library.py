@@ -54,6 +54,15 @@ DETERMINISM_REPRODUCIBLE = 1
 DEFAULT_SCC_MIXER_HISTORY = 8
 DEFAULT_SCC_MIXER_DAMPING = 0.4
 
+# interaction attachment tags (ABI-v3 batch suffix); the uniform electric field
+# (block_version 1: int32 version, int32 reserved, three doubles) is executed,
+# the others are reserved
+INTERACTION_NONE = 0
+INTERACTION_ELECTRIC_FIELD = 0x0101
+INTERACTION_ELECTRIC_FIELD_GRADIENT = 0x0102
+INTERACTION_ALPB_SOLVATION = 0x0201
+ELECTRIC_FIELD_BLOCK_BYTES = 32
+
 
 class ContextOptions(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("api_version", ctypes.c_uint32),
@@ -70,6 +79,14 @@ class ConstBuffer(ctypes.Structure):
 class Buffer(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("size_bytes", ctypes.c_size_t),
                 ("memory_space", ctypes.c_int32), ("reserved", ctypes.c_uint32)]
+
+
+class Interaction(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("flags", ctypes.c_uint32), ("system_index", ctypes.c_int64),
+                ("payload_offset", ctypes.c_uint64), ("payload_size", ctypes.c_uint64)]
+
+
+assert ctypes.sizeof(Interaction) == 32
 
 
 class Batch(ctypes.Structure):
