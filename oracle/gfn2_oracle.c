@@ -70,6 +70,8 @@ void oracle_default_options(oracle_options* o) {
   o->kt = 300.0 * 3.166808578545117e-6;
   o->damping = 0.4;
   o->efield[0] = o->efield[1] = o->efield[2] = 0.0;
+  o->potential_shifts = NULL;
+  o->charge_response = NULL;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1568,7 +1570,27 @@ typedef struct {
   int field;
   double E[3];
   const double* xyz;
+  /* external charge operator (periodic_embedding.hpp:55-60): v_A = b_A +
+   * sum_B A_AB q_B on the charge-channel atom potential and E = q.b + q.A.q/2;
+   * either may be NULL */
+  const double* pshift;
+  const double* presp;
 } System;
+
+/* v = b + A q and its energy q.b + q.(A q)/2 (periodic_embedding.cpp:352-434) */
+static double charge_operator(const System* sy, const double* q, double* v) {
+  double lin = 0.0, quad = 0.0;
+  for (int a = 0; a < sy->nat; ++a) {
+    double r = 0.0;
+    if (sy->presp)
+      for (int c = 0; c < sy->nat; ++c) r = fma(sy->presp[(size_t)a * sy->nat + c], q[c], r);
+    const double b = sy->pshift ? sy->pshift[a] : 0.0;
+    if (v) v[a] = b + r;
+    lin = fma(q[a], b, lin);
+    quad = fma(q[a], r, quad);
+  }
+  return fma(0.5, quad, lin);
+}
 
 static void mulliken(const System* sy, const double* P, double* qsh, double* qat, double* dip,
                      double* quad) {
@@ -1610,6 +1632,11 @@ static void potentials(const System* sy, const double* qsh, const double* qat, c
     vsh[s] = v;
   }
   aes2_potential(sy->nat, b->Z, sy->aes, qat, dip, quad, vat_scratch, vd, vqq);
+  /* the external charge operator joins before D4 (scc_driver.cpp:2515-2531) */
+  if (sy->pshift || sy->presp) {
+    charge_operator(sy, qat, d4v);
+    for (int a = 0; a < sy->nat; ++a) vat_scratch[a] += d4v[a];
+  }
   d4_two_body(&sy->d4, qat, d4v, NULL);
   for (int a = 0; a < sy->nat; ++a) vat_scratch[a] += d4v[a];
   /* field after D4, on the charge-channel atom and dipole potentials
@@ -1800,11 +1827,14 @@ static int scc_step(System* sy, const double* x, double nalpha, double nbeta, do
       efield += c;
     }
   }
-  double e_int = ecore + ees2 + ees3 + eaes + ed4 + epc + efield;
+  /* external charge operator at the raw atomic charges, summed last
+   * (scc_driver.cpp:2236-2262) */
+  const double eop = (sy->pshift || sy->presp) ? charge_operator(sy, qat_scratch, NULL) : 0.0;
+  double e_int = ecore + ees2 + ees3 + eaes + ed4 + epc + efield + eop;
   *F = fma(-kt, entropy, e_int);
-  /* component 5 holds the external terms: point charges and the field */
+  /* component 5 holds the external terms: point charges, field and operator */
   ecomp[0] = ecore; ecomp[1] = ees2; ecomp[2] = ees3; ecomp[3] = eaes; ecomp[4] = ed4;
-  ecomp[5] = epc + efield;
+  ecomp[5] = epc + efield + eop;
   return ST_OK;
 }
 
@@ -1855,6 +1885,8 @@ int oracle_singlepoint_state(int32_t nat, const int32_t* Z, const double* xyz, d
                             &nalpha, &nbeta);
   for (int k = 0; k < 3; ++k) sy.E[k] = opt->efield[k];
   sy.field = opt->efield[0] != 0.0 || opt->efield[1] != 0.0 || opt->efield[2] != 0.0;
+  sy.pshift = opt->potential_shifts;
+  sy.presp = opt->charge_response;
   if (status == ST_INVALID || sy.S == NULL) { system_free(&sy); return status == ST_OK ? ST_INVALID : status; }
   const int n = sy.n, ns = sy.ns;
   const size_t nn = (size_t)n * n;
@@ -2038,6 +2070,9 @@ typedef struct {
   long next;
   pthread_mutex_t lock;
   const double* efield; /* [batch][3] or NULL */
+  const double* shifts;  /* [total atoms] or NULL */
+  const double* response; /* packed nat*nat per system or NULL */
+  const int64_t* roff;    /* [batch+1] start of each system's A */
 } BatchJob;
 
 static void* batch_worker(void* arg) {
@@ -2055,6 +2090,8 @@ static void* batch_worker(void* arg) {
     oracle_options o = *job->opt;
     if (job->efield)
       for (int k = 0; k < 3; ++k) o.efield[k] = job->efield[3 * s + k];
+    o.potential_shifts = job->shifts ? job->shifts + a0 : NULL;
+    o.charge_response = job->response ? job->response + job->roff[s] : NULL;
     int st = oracle_singlepoint(nat, job->Z + a0, job->xyz + 3 * a0, job->chg[s], job->uhf[s], npc,
                                 npc ? job->pxyz + 3 * p0 : NULL, npc ? job->pq + p0 : NULL,
                                 npc ? job->pg + p0 : NULL, &o, &e,
@@ -2078,15 +2115,23 @@ int oracle_batch(int64_t batch, const int64_t* atom_offsets, const int32_t* Z, c
                  const double* pc_xyz, const double* pc_q, const double* pc_gamma,
                  const oracle_options* options, int32_t threads, double* energies, double* grad,
                  double* atomic_charges, double* pc_grad, int32_t* iterations, int32_t* status,
-                 const double* efield) {
+                 const double* efield, const double* shifts, const double* response) {
   if (!la_dsyevd) return ST_INVALID;
+  int64_t* roff = malloc(sizeof(int64_t) * (size_t)(batch + 1));
+  if (!roff) return ST_INVALID;
+  roff[0] = 0;
+  for (int64_t s = 0; s < batch; ++s) {
+    const int64_t n = atom_offsets[s + 1] - atom_offsets[s];
+    roff[s + 1] = roff[s] + n * n;
+  }
   BatchJob job = {batch, atom_offsets, point_offsets, Z, unpaired, xyz, charges, pc_xyz, pc_q,
                   pc_gamma, options, energies, grad, atomic_charges, pc_grad, iterations, status,
-                  0, PTHREAD_MUTEX_INITIALIZER, efield};
+                  0, PTHREAD_MUTEX_INITIALIZER, efield, shifts, response, roff};
   if (threads <= 0) threads = (int32_t)sysconf(_SC_NPROCESSORS_ONLN);
   if (threads > 256) threads = 256;
   pthread_t tid[256];
   for (int t = 0; t < threads; ++t) pthread_create(&tid[t], NULL, batch_worker, &job);
   for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+  free(roff);
   return ST_OK;
 }

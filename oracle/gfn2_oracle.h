@@ -27,6 +27,11 @@ typedef struct oracle_options {
   double kt;                /* k_B*T in Hartree, default 300 K */
   double damping;           /* default 0.4 */
   double efield[3];         /* uniform electric field, Hartree/(e bohr); default 0 (none) */
+  /* external charge operator v = b + A q, E = q.b + q.A.q/2 on the atomic
+   * charges (periodic QM/MM embedding): b [nat] and A [nat*nat, row-major,
+   * symmetric], each NULL when absent (default) */
+  const double* potential_shifts;
+  const double* charge_response;
 } oracle_options;
 
 /* Optional diagnostics for term-level pinning; every pointer may be NULL. */
@@ -89,7 +94,9 @@ int oracle_batch(int64_t batch, const int64_t* atom_offsets, const int32_t* Z, c
                  const double* pc_xyz, const double* pc_q, const double* pc_gamma,
                  const oracle_options* options, int32_t threads, double* energies, double* grad,
                  double* atomic_charges, double* pc_grad, int32_t* iterations, int32_t* status,
-                 const double* efield /* [batch][3] uniform fields, or NULL */);
+                 const double* efield /* [batch][3] uniform fields, or NULL */,
+                 const double* shifts /* [total atoms] b, or NULL */,
+                 const double* response /* per system nat*nat A packed in system order, or NULL */);
 
 /*
  * Replay exactly one SCC step from a given mixed state [qsh, dip(3N), quad(6N)]

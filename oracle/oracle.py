@@ -25,7 +25,8 @@ class Options(ctypes.Structure):
     _fields_ = [("max_iterations", ctypes.c_int32), ("history", ctypes.c_int32),
                 ("charge_tolerance", ctypes.c_double), ("energy_tolerance", ctypes.c_double),
                 ("kt", ctypes.c_double), ("damping", ctypes.c_double),
-                ("efield", ctypes.c_double * 3)]
+                ("efield", ctypes.c_double * 3), ("potential_shifts", _c_double_p),
+                ("charge_response", _c_double_p)]
 
 
 class Trace(ctypes.Structure):
@@ -77,7 +78,7 @@ def lib():
             ctypes.c_int64, _c_int64_p, _c_int32_p, _c_double_p, _c_double_p, _c_int32_p,
             _c_int64_p, _c_double_p, _c_double_p, _c_double_p, ctypes.POINTER(Options),
             ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_int32_p,
-            _c_int32_p, _c_double_p]
+            _c_int32_p, _c_double_p, _c_double_p, _c_double_p]
         L.oracle_scc_replay.argtypes = [
             ctypes.c_int32, _c_int32_p, _c_double_p, ctypes.c_double, ctypes.c_int32,
             ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p, ctypes.c_double, _c_double_p,
@@ -108,9 +109,20 @@ def _p(a, t=_c_double_p):
 
 def options(max_iterations=250, charge_tolerance=1e-6, energy_tolerance=1e-8,
             kt=300.0 * 3.166808578545117e-6, history=8, damping=0.4,
-            efield=(0.0, 0.0, 0.0)) -> Options:
-    return Options(max_iterations, history, charge_tolerance, energy_tolerance, kt, damping,
-                   (ctypes.c_double * 3)(*efield))
+            efield=(0.0, 0.0, 0.0), shifts=None, response=None) -> Options:
+    """``shifts`` (b, per atom) and ``response`` (A, nat x nat symmetric) add
+    the external charge operator v = b + A q, E = q.b + q.A.q/2."""
+    o = Options(max_iterations, history, charge_tolerance, energy_tolerance, kt, damping,
+                (ctypes.c_double * 3)(*efield))
+    keep = []
+    if shifts is not None:
+        keep.append(np.ascontiguousarray(shifts, dtype=np.float64))
+        o.potential_shifts = _p(keep[-1])
+    if response is not None:
+        keep.append(np.ascontiguousarray(response, dtype=np.float64))
+        o.charge_response = _p(keep[-1])
+    o._keep = keep  # the arrays live as long as the options
+    return o
 
 
 @dataclass
@@ -223,7 +235,8 @@ def scc_replay(Z, xyz, mixed, charge=0.0, uhf=0, pc_xyz=None, pc_q=None, pc_gamm
 
 
 def batch(atom_offsets, Z, xyz, charges, uhf, point_offsets=None, pc_xyz=None, pc_q=None,
-          pc_gamma=None, opts=None, threads=0, want_point_gradient=False, efield=None):
+          pc_gamma=None, opts=None, threads=0, want_point_gradient=False, efield=None,
+          shifts=None, response=None):
     atom_offsets = np.ascontiguousarray(atom_offsets, dtype=np.int64)
     B = len(atom_offsets) - 1
     Z = np.ascontiguousarray(Z, dtype=np.int32)
@@ -246,8 +259,13 @@ def batch(atom_offsets, Z, xyz, charges, uhf, point_offsets=None, pc_xyz=None, p
     o = opts or options()
     if efield is not None:  # [B, 3] uniform fields (zero rows: none)
         efield = np.ascontiguousarray(efield, dtype=np.float64).reshape(B, 3)
+    if shifts is not None:  # [N] b per atom
+        shifts = np.ascontiguousarray(shifts, dtype=np.float64)
+    if response is not None:  # A of each system, nat x nat, packed in system order
+        response = np.ascontiguousarray(response, dtype=np.float64)
     lib().oracle_batch(B, _p(atom_offsets, _c_int64_p), _p(Z, _c_int32_p), _p(xyz), _p(charges),
                        _p(uhf, _c_int32_p), _p(point_offsets, _c_int64_p), _p(pc_xyz), _p(pc_q),
                        _p(pc_gamma), ctypes.byref(o), int(threads), _p(e), _p(g), _p(q), _p(pg),
-                       _p(it, _c_int32_p), _p(st, _c_int32_p), _p(efield))
+                       _p(it, _c_int32_p), _p(st, _c_int32_p), _p(efield), _p(shifts),
+                       _p(response))
     return dict(energies=e, gradient=g, charges=q, iterations=it, status=st, pc_gradient=pg)

@@ -208,6 +208,86 @@ def test_field_golden_tblite_water():
     assert np.linalg.norm(r.gradient - plain.gradient) > 1e-4
 
 
+def _operator(nat, seed):
+    rng = np.random.default_rng(seed)
+    b = rng.normal(scale=0.01, size=nat)
+    A = rng.normal(scale=0.01, size=(nat, nat))
+    return b, 0.5 * (A + A.T) + 0.05 * np.eye(nat)
+
+
+def test_charge_operator_derivatives():
+    """External charge operator v = b + A q, E = q.b + q.A.q/2 (periodic QM/MM
+    coupling): the SCC energy is stationary in the density, so dE/db_A = q_A,
+    dE/dA_AB = q_A q_B for a symmetric off-diagonal change and q_A^2/2 on the
+    diagonal; b and A are fixed in space, so the forces are the exact
+    coordinate derivative."""
+    from xtbloom_amd import workloads as W
+    s = W.water(np.random.default_rng(4))
+    b, A = _operator(3, 1)
+
+    def energy(xyz=s["xyz"], bb=b, AA=A, trace=False):
+        r = O.singlepoint(s["Z"], xyz, opts=O.options(charge_tolerance=1e-10, energy_tolerance=1e-12,
+                                                       shifts=bb, response=AA))
+        assert r.status == 0
+        return r
+
+    r = energy()
+    plain = O.singlepoint(s["Z"], s["xyz"], opts=O.options(charge_tolerance=1e-10, energy_tolerance=1e-12))
+    assert abs(r.energy - plain.energy) > 1e-3 and np.abs(r.charges - plain.charges).max() > 1e-3
+    q = r.charges
+    h = 1e-5
+    for a in range(3):
+        bp, bm = b.copy(), b.copy()
+        bp[a] += h
+        bm[a] -= h
+        assert abs((energy(bb=bp).energy - energy(bb=bm).energy) / (2 * h) - q[a]) < 1e-8
+    for a, c in ((0, 1), (1, 2), (2, 2)):
+        Ap, Am = A.copy(), A.copy()
+        Ap[a, c] += h
+        Am[a, c] -= h
+        if a != c:
+            Ap[c, a] += h
+            Am[c, a] -= h
+        want = q[a] * q[c] if a != c else 0.5 * q[a] ** 2
+        assert abs((energy(AA=Ap).energy - energy(AA=Am).energy) / (2 * h) - want) < 1e-8
+    hx = 1e-4
+    for atom, axis in ((0, 0), (1, 1), (2, 2)):
+        xp = s["xyz"].copy(); xp[atom, axis] += hx
+        xm = s["xyz"].copy(); xm[atom, axis] -= hx
+        assert abs((energy(xyz=xp).energy - energy(xyz=xm).energy) / (2 * hx) - r.gradient[atom, axis]) < 2e-7
+
+
+def test_charge_operator_invariances():
+    """On a neutral molecule a uniform shift b = c (A = 0) shifts every level
+    by -c and changes no charge and no energy; A = a 11^T acts on the total
+    charge only, which is zero."""
+    from xtbloom_amd import workloads as W
+    s = W.alkane(3, np.random.default_rng(2), substituent="NH2")
+    nat = len(s["Z"])
+    tight = dict(charge_tolerance=1e-10, energy_tolerance=1e-12)
+    plain = O.singlepoint(s["Z"], s["xyz"], opts=O.options(**tight))
+    for kw in (dict(shifts=np.full(nat, 0.03)), dict(response=np.full((nat, nat), 0.02))):
+        r = O.singlepoint(s["Z"], s["xyz"], opts=O.options(**tight, **kw))
+        assert abs(r.energy - plain.energy) < 1e-10
+        assert np.abs(r.charges - plain.charges).max() < 1e-8
+        assert np.abs(r.gradient - plain.gradient).max() < 1e-8
+
+
+def test_charge_operator_batch_packing():
+    from xtbloom_amd import workloads as W
+    bt = W.make("ragged512", seed=3, batch=4, kmax=5)
+    nat = np.diff(bt.atom_offsets)
+    ops = [_operator(int(n), 10 + i) for i, n in enumerate(nat)]
+    shifts = np.concatenate([o[0] for o in ops])
+    resp = np.concatenate([o[1].ravel() for o in ops])
+    out = O.batch(bt.atom_offsets, bt.atomic_numbers, bt.positions, bt.molecular_charges,
+                  bt.unpaired_electrons, shifts=shifts, response=resp, threads=2)
+    for i in range(bt.batch_size):
+        s = bt.system(i)
+        r = O.singlepoint(s["Z"], s["xyz"], opts=O.options(shifts=ops[i][0], response=ops[i][1]))
+        assert out["energies"][i] == r.energy
+
+
 def test_batch_equals_sequential():
     from xtbloom_amd import workloads as W
     b = W.make("ragged512", seed=5, batch=6, kmax=8)

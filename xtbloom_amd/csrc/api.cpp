@@ -54,6 +54,9 @@ struct CallIdentity {
   // the vectors compare by value, as the reference's key does
   std::vector<uint8_t> field_on;
   std::vector<double> field;
+  // the external charge operator's presence (its b and A are numerical
+  // inputs, gfn2_cpu_execution.cpp:627-643)
+  bool charge_operator = false;
   bool same_plan(const CallIdentity& o) const {
     // doubles compare by their bits: the identity is exact
     auto same = [](double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; };
@@ -61,7 +64,8 @@ struct CallIdentity {
            point_offsets == o.point_offsets && pc_gamma == o.pc_gamma && spin == o.spin && flags == o.flags &&
            model == o.model && max_iterations == o.max_iterations && history == o.history &&
            determinism == o.determinism && same(charge_tolerance, o.charge_tolerance) &&
-           same(energy_tolerance, o.energy_tolerance) && same(kt, o.kt) && same(damping, o.damping);
+           same(energy_tolerance, o.energy_tolerance) && same(kt, o.kt) && same(damping, o.damping) &&
+           charge_operator == o.charge_operator;
   }
   bool operator==(const CallIdentity& o) const {
     return same_plan(o) && field_on == o.field_on && field == o.field;
@@ -324,7 +328,80 @@ struct Staged {
   // uniform-field attachments: empty without any, else [B] flags and [3B] vectors
   std::vector<uint8_t> field_on;
   std::vector<double> field;
+  // external charge operator: b [N] and A (packed n_atoms^2 per system), each
+  // empty when absent
+  std::vector<double> pshift, presp;
 };
+
+// External charge operator v = b + A q, E = q.b + q.A.q/2 (periodic QM/MM
+// coupling; reference xtbloom.h:424-431, validation.cpp:401-403, 676-700,
+// 1308-1360, gfn2_cpu_execution.cpp:448-465, 538-560): b is active when its
+// buffer has data and bytes; A needs its count, offsets and matrix together,
+// the offsets pack one square n_atoms^2 block per system in order, and every
+// block is exactly symmetric.  All values finite.
+xtbloom_status_t stage_charge_operator(const xtbloom_batch_t* batch, int64_t B, int64_t N,
+                                       const int64_t* aoff, Staged& st) {
+  xtbloom_status_t s;
+  st.pshift.clear();
+  st.presp.clear();
+  std::vector<char> raw;
+  if (batch->atomic_potential_shifts.data != nullptr && batch->atomic_potential_shifts.size_bytes != 0) {
+    if ((s = stage_input(batch->atomic_potential_shifts, sizeof(double) * (size_t)N, "atomic_potential_shifts",
+                         raw, true)))
+      return s;
+    st.pshift.resize((size_t)N);
+    std::memcpy(st.pshift.data(), raw.data(), raw.size());
+  }
+  const int64_t total = batch->total_charge_response_elements;
+  if (total < 0) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "total_charge_response_elements must be nonnegative");
+  const bool enabled = total != 0 || buffer_active(batch->charge_response_offsets) ||
+                       buffer_active(batch->charge_response_matrix);
+  if (enabled) {
+    if (total == 0 || !buffer_active(batch->charge_response_offsets) || !buffer_active(batch->charge_response_matrix))
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "charge response count, offsets, and matrix must be supplied together");
+    if (total > ((int64_t)1 << 40))
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "total_charge_response_elements overflows the addressable byte size");
+    std::vector<char> offs;
+    if ((s = stage_input(batch->charge_response_offsets, sizeof(int64_t) * (size_t)(B + 1), "charge_response_offsets",
+                         offs, true)))
+      return s;
+    const int64_t* ro = reinterpret_cast<const int64_t*>(offs.data());
+    if (ro[0] != 0) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "charge_response_offsets must begin with zero");
+    int64_t packed = 0;
+    for (int64_t i = 0; i < B; ++i) {
+      if (ro[i + 1] < ro[i])
+        return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "charge_response_offsets must be monotonically nondecreasing");
+      const int64_t n = aoff[i + 1] - aoff[i];
+      if (ro[i + 1] - ro[i] != n * n)
+        return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "charge_response_offsets do not pack one square matrix per system");
+      packed += n * n;
+    }
+    if (ro[B] != total || packed != total)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "total_charge_response_elements does not match the packed square matrices");
+    if ((s = stage_input(batch->charge_response_matrix, sizeof(double) * (size_t)total, "charge_response_matrix", raw,
+                         true)))
+      return s;
+    st.presp.resize((size_t)total);
+    std::memcpy(st.presp.data(), raw.data(), raw.size());
+  }
+  for (double v : st.pshift)
+    if (!std::isfinite(v)) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "periodic b/A inputs contain NaN or infinity");
+  for (double v : st.presp)
+    if (!std::isfinite(v)) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "periodic b/A inputs contain NaN or infinity");
+  if (!st.presp.empty()) {
+    int64_t base = 0;
+    for (int64_t i = 0; i < B; ++i) {
+      const int64_t n = aoff[i + 1] - aoff[i];
+      const double* A = st.presp.data() + base;
+      for (int64_t r = 0; r < n; ++r)
+        for (int64_t c = r + 1; c < n; ++c)
+          if (A[r * n + c] != A[c * n + r])
+            return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "charge_response_matrix must be exactly symmetric per system");
+      base += n * n;
+    }
+  }
+  return XTBLOOM_STATUS_SUCCESS;
+}
 
 bool known_interaction(int32_t type) {
   switch (type) {
@@ -523,10 +600,6 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
   if (result != nullptr && result->struct_size >= XTBLOOM_BATCH_RESULT_V2_SIZE &&
       (result->quadrupole_moments.data || result->wiberg_orders.data || result->spin_populations.data))
     return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "reserved result outputs must be NULL");
-  // unsupported optional inputs on this backend
-  if (buffer_active(batch->atomic_potential_shifts) || buffer_active(batch->charge_response_offsets) ||
-      buffer_active(batch->charge_response_matrix) || batch->total_charge_response_elements != 0)
-    return fail(XTBLOOM_STATUS_NOT_SUPPORTED, "periodic b + A q embedding is not supported by the ROCm backend");
   // stage inputs
   if ((s = stage_input(batch->atom_offsets, sizeof(int64_t) * (B + 1), "atom_offsets", st.atom_offsets, true)) ||
       (s = stage_input(batch->atomic_numbers, sizeof(int32_t) * N, "atomic_numbers", st.Z, true)) ||
@@ -557,6 +630,7 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
   if (aoff[0] != 0 || aoff[B] != N) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "atom_offsets must start at 0 and end at total_atoms");
   for (int64_t i = 0; i < B; ++i)
     if (aoff[i + 1] <= aoff[i]) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "atom_offsets must be strictly increasing");
+  if ((s = stage_charge_operator(batch, B, N, aoff, st))) return s;
   const double* xyz = reinterpret_cast<const double*>(st.xyz.data());
   for (int64_t i = 0; i < 3 * N; ++i)
     if (!std::isfinite(xyz[i])) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "positions contain NaN or infinity");
@@ -610,6 +684,8 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
     hb.pc_gamma = reinterpret_cast<const double*>(st.pg.data());
   }
   hb.field = st.field.empty() ? nullptr : st.field.data();
+  hb.pshift = st.pshift.empty() ? nullptr : st.pshift.data();
+  hb.presp = st.presp.empty() ? nullptr : st.presp.data();
   ro.forces = want_f || want_pf;
   return XTBLOOM_STATUS_SUCCESS;
 }
@@ -626,6 +702,7 @@ CallIdentity make_identity(const Staged& st, const xtbloom_compute_options_t* op
   id.spin = st.spin;
   id.field_on = st.field_on;
   id.field = st.field;
+  id.charge_operator = !st.pshift.empty() || !st.presp.empty();
   id.flags = flags;
   id.model = options->model;
   id.max_iterations = ro.max_iterations;
@@ -822,6 +899,9 @@ xtbloom_status_t execute(xtbloom_context_t* context, Job& job) {
     if ((s = publish(job.result.dipole_moments, dipoles.data(), sizeof(double) * 3 * B))) return s;
     rflags |= XTBLOOM_RESULT_DIPOLE_MOMENTS;
   }
+  // forces leave out the coordinate derivatives of a caller's b and A
+  // (reference gfn2_cpu_execution.cpp:1983-1989)
+  if (job.id.charge_operator) rflags |= XTBLOOM_RESULT_FORCES_EXCLUDE_EXTERNAL_OPERATOR_DERIVATIVES;
   job.result.flags = rflags;
   g_last_error.clear();
   return XTBLOOM_STATUS_SUCCESS;

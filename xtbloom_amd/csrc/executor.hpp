@@ -23,6 +23,10 @@ struct HostBatch {
   const double* pc_q = nullptr;           // [P]
   const double* pc_gamma = nullptr;       // [P]
   const double* field = nullptr;          // [3B] uniform electric field per system, or null
+  // external charge operator v = b + A q on the atomic charges: b [N] and
+  // A (n_atoms^2 per system, row-major, packed in system order), each or null
+  const double* pshift = nullptr;
+  const double* presp = nullptr;
 };
 
 struct RunOptions {

@@ -76,6 +76,10 @@ struct Topo {
   // uniform electric field per system [3B] (Hartree / (e bohr)), or null when
   // no system of the call carries one
   const double* field;
+  // external charge operator: b per atom [N] and A per system at apair_off
+  // (row-major n_atoms^2), each null when the call has none
+  const double* pshift;
+  const double* presp;
   // global tables
   const double* d4_c6;      // [262*262]
   const double* d4_refcn;

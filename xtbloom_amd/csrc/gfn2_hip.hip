@@ -113,6 +113,7 @@ struct HostTopo {
   std::vector<double> alpha, coef;
   std::vector<double> pc_xyz, pc_q, pc_gam;
   std::vector<double> field;  // [3B] when the call carries a uniform field, else empty
+  std::vector<double> pshift, presp;  // external charge operator b [N], A [apair_off[B]], or empty
   std::vector<int> setup_status;
   int max_ao = 0;
   int max_atoms = 0;
@@ -548,11 +549,19 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
       cache->generation = ++g_topo_generation;
     }
   }
-  // the field is a per-call input like the geometry
+  // the field and the charge operator are per-call inputs like the geometry
   if (hb.field)
     T.field.assign(hb.field, hb.field + 3 * (size_t)hb.batch);
   else
     T.field.clear();
+  if (hb.pshift)
+    T.pshift.assign(hb.pshift, hb.pshift + (size_t)hb.atoms);
+  else
+    T.pshift.clear();
+  if (hb.presp)
+    T.presp.assign(hb.presp, hb.presp + (size_t)T.apair_off[T.B]);
+  else
+    T.presp.clear();
   const int B = T.B, N = T.N, NS = T.NS, NAO = T.NAO, NPC = T.NPC;
   const int64_t NN = T.mat_off[B];
   const int64_t NG = T.gam_off[B];
@@ -631,6 +640,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(T.NP); addd(T.NP);
   addd(3 * (size_t)NPC); addd(NPC); addd(NPC);
   addd(3 * (size_t)B);  // uniform field per system
+  addd(T.pshift.size()); addd(T.presp.size());  // charge operator b, A
   bytes += plan_bytes<TileTask>(full.size()) + plan_bytes<TileTask>(lower.size());
   bytes += plan_bytes<BackTask>(back.size());
   bytes += plan_bytes<FockTile>(chunks.size());
@@ -750,6 +760,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   double* d_pcq = cv.take<double>(NPC);
   double* d_pcg = cv.take<double>(NPC);
   double* d_field = cv.take<double>(3 * (size_t)B);
+  double* d_pshift = cv.take<double>(T.pshift.size());
+  double* d_presp = cv.take<double>(T.presp.size());
   TileTask* d_full = cv.take<TileTask>(full.size());
   TileTask* d_lower = cv.take<TileTask>(lower.size());
   BackTask* d_back = cv.take<BackTask>(back.size());
@@ -804,6 +816,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   HIP_TRY(up(d_pcq, T.pc_q));
   HIP_TRY(up_topo(d_pcg, T.pc_gam));
   HIP_TRY(up(d_field, T.field));
+  HIP_TRY(up(d_pshift, T.pshift));
+  HIP_TRY(up(d_presp, T.presp));
   HIP_TRY(up_topo(d_full, full));
   HIP_TRY(up_topo(d_lower, lower));
   HIP_TRY(up_topo(d_back, back));
@@ -825,6 +839,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   t.alpha = d_alpha; t.coef = d_coef;
   t.pc_xyz = d_pcxyz; t.pc_q = d_pcq; t.pc_gam = d_pcg;
   t.field = T.field.empty() ? nullptr : d_field;
+  t.pshift = T.pshift.empty() ? nullptr : d_pshift;
+  t.presp = T.presp.empty() ? nullptr : d_presp;
   t.d4_c6 = dev->d4_c6; t.d4_refcn = dev->d4_refcn; t.d4_refq = dev->d4_refq;
   t.d4_refng = dev->d4_refng; t.kpair = dev->kpair;
 
@@ -1214,7 +1230,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     put(arena_base); put(bytes); put(B); put(N); put(NS); put(NAO); put(NPC);
     put(cache ? cache->generation : ~uint64_t(0)); put(mem); put(opt.damping);
     put(opt.charge_tolerance); put(opt.energy_tolerance); put(opt.kt); put(opt.max_iterations);
-    put(opt.split32); put(t.field);
+    put(opt.split32); put(t.field); put(t.pshift); put(t.presp);
     const bool reuse = cache != nullptr && dev->graph_exec != nullptr && dev->graph_key == key;
     if (!reuse) {
       if (dev->graph_exec) (void)hipGraphExecDestroy(dev->graph_exec);

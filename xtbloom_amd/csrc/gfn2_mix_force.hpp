@@ -66,6 +66,19 @@ __global__ __launch_bounds__(512) void k_energy_mix(Topo t, const int* order, un
     const double* E = t.field + 3 * s;
     for (int i = tid; i < 3 * N; i += nt) e[2] -= E[i % 3] * dip[i];
   }
+  // external charge operator q.b + q.(A q)/2 at the raw atomic charges
+  // (reference periodic_embedding.cpp:352-434)
+  if (t.pshift || t.presp) {
+    for (int la = tid; la < N; la += nt) {
+      double r = 0.0;
+      if (t.presp) {
+        const double* row = t.presp + t.apair_off[s] + (int64_t)la * N;
+        for (int c = 0; c < N; ++c) r = fma(row[c], qat_raw[a0 + c], r);
+      }
+      const double b = t.pshift ? t.pshift[a0 + la] : 0.0;
+      e[2] = fma(qat_raw[a0 + la], fma(0.5, r, b), e[2]);
+    }
+  }
   // AES2 and D4 two-body: every ordered pair once, halved.  The AES2 pair
   // energy equals q_a V_q + d_a.V_d + Q_a.V_Q of atom a in the field of b,
   // symmetric in (a, b).

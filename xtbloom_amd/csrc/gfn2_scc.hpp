@@ -514,7 +514,16 @@ __global__ __launch_bounds__(512, 4) void k_potentials(Topo t, const int* order,
         const int a = t.perm[a0 + pa];
         double pd4 = 0.0;
         for (int r = 0; r < kD4MaxRef; ++r) pd4 -= dw4[kD4MaxRef * a + r] * o.z[r];
-        vat[a] = o.pq + pd4;
+        // external charge operator b_A + sum_B A_AB q_B, before D4 as in
+        // the reference (scc_driver.cpp:2515-2531); qat of the whole system
+        // was written before the barrier above
+        double vop = 0.0;
+        if (t.presp) {
+          const double* row = t.presp + t.apair_off[s] + (int64_t)(a - a0) * N;
+          for (int c = 0; c < N; ++c) vop = fma(row[c], qat[a0 + c], vop);
+        }
+        if (t.pshift) vop += t.pshift[a];
+        vat[a] = (o.pq + vop) + pd4;
         // a uniform field adds -E to every atomic dipole potential
         const double* E = t.field ? t.field + 3 * s : nullptr;
         for (int c = 0; c < 3; ++c)

@@ -113,7 +113,7 @@ bool is_contiguous(const std::vector<int64_t>& m) {
 struct Gathered {
   std::vector<int64_t> atom_offsets, point_offsets;
   std::vector<int32_t> Z, unpaired, iterations, status;
-  std::vector<double> xyz, charges, pc_xyz, pc_q, pc_gamma, field, warm;
+  std::vector<double> xyz, charges, pc_xyz, pc_q, pc_gamma, field, pshift, presp, warm;
   std::vector<double> energies, gradient, atom_charges, pc_gradient, pc_dedq, dipoles, scc_state,
       dbg_overlap, dbg_h0, dbg_density, dbg_cn, dbg_components, dbg_levels, dbg_occ;
   std::vector<uint8_t> converged;
@@ -193,8 +193,11 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
   }
   const LaneMembers members = contiguous_split(nao, kept, K);
 
-  std::vector<int64_t> mat_prefix(hb.batch + 1, 0), state_prefix(hb.batch + 1, 0), ao_prefix(hb.batch + 1, 0);
+  std::vector<int64_t> mat_prefix(hb.batch + 1, 0), state_prefix(hb.batch + 1, 0), ao_prefix(hb.batch + 1, 0),
+      resp_prefix(hb.batch + 1, 0);
   for (int64_t s = 0; s < hb.batch; ++s) {
+    const int64_t na = hb.atom_offsets[s + 1] - hb.atom_offsets[s];
+    resp_prefix[s + 1] = resp_prefix[s] + na * na;
     mat_prefix[s + 1] = mat_prefix[s] + (int64_t)nao[s] * nao[s];
     state_prefix[s + 1] = state_prefix[s] + ndim[s];
     ao_prefix[s + 1] = ao_prefix[s] + nao[s];
@@ -222,6 +225,8 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
       b.charges = hb.charges + x.s0;
       b.unpaired = hb.unpaired + x.s0;
       b.field = at(hb.field, 3 * x.s0);
+      b.pshift = at(hb.pshift, a0);
+      b.presp = at(hb.presp, resp_prefix[x.s0]);
       int64_t p0 = 0;
       if (hb.point_offsets) {
         p0 = hb.point_offsets[x.s0];
@@ -268,6 +273,8 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
         g.charges.push_back(hb.charges[s]);
         g.unpaired.push_back(hb.unpaired[s]);
         if (hb.field) g.field.insert(g.field.end(), hb.field + 3 * s, hb.field + 3 * s + 3);
+        if (hb.pshift) g.pshift.insert(g.pshift.end(), hb.pshift + a0, hb.pshift + a1);
+        if (hb.presp) g.presp.insert(g.presp.end(), hb.presp + resp_prefix[s], hb.presp + resp_prefix[s + 1]);
         if (hb.point_offsets) {
           const int64_t p0 = hb.point_offsets[s], p1 = hb.point_offsets[s + 1];
           g.point_offsets.push_back(g.point_offsets.back() + (p1 - p0));
@@ -290,6 +297,8 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
       b.charges = g.charges.data();
       b.unpaired = g.unpaired.data();
       b.field = hb.field ? g.field.data() : nullptr;
+      b.pshift = hb.pshift ? g.pshift.data() : nullptr;
+      b.presp = hb.presp ? g.presp.data() : nullptr;
       if (hb.point_offsets) {
         b.points = np;
         b.point_offsets = g.point_offsets.data();

@@ -63,6 +63,31 @@ class PointCharge:
             raise XTBloomValueError("point charge inputs must be finite")
 
 
+@dataclass(frozen=True)
+class ChargeResponse:
+    """External charge operator of one molecule (periodic QM/MM coupling,
+    reference python/xtbloom/interface.py:231-267): an SCC shift ``b + A q``
+    on the atomic charges and the energy ``q.b + q.A.q / 2``.  ``shifts`` is b
+    (n, Hartree/e), ``matrix`` the symmetric A (n x n, Hartree/e^2); both are
+    constant operators, so forces leave out their coordinate derivatives."""
+    shifts: np.ndarray
+    matrix: np.ndarray
+
+    def __post_init__(self):
+        shifts = np.asarray(self.shifts, dtype=float)
+        matrix = np.asarray(self.matrix, dtype=float)
+        if shifts.ndim != 1:
+            raise XTBloomValueError("charge response shifts must be one-dimensional")
+        if matrix.ndim != 2 or matrix.shape[0] != matrix.shape[1]:
+            raise XTBloomValueError("charge response matrix must be square")
+        if matrix.shape[0] != shifts.size:
+            raise XTBloomValueError("charge response matrix must match the shift count")
+        if not (np.isfinite(shifts).all() and np.isfinite(matrix).all()):
+            raise XTBloomValueError("charge response inputs must be finite")
+        object.__setattr__(self, "shifts", np.ascontiguousarray(shifts))
+        object.__setattr__(self, "matrix", np.ascontiguousarray(matrix))
+
+
 def _normalize_efield(efield):
     """A uniform electric field (Hartree per elementary charge per bohr) as a
     frozen length-3 vector; ``None`` and the zero vector mean no attachment
@@ -86,7 +111,8 @@ class Structure:
     point charges and an optional uniform electric field (atomic units)."""
 
     def __init__(self, numbers, positions, charge: float = 0.0, uhf: int | None = None,
-                 point_charges: PointCharge | None = None, efield=None):
+                 point_charges: PointCharge | None = None, efield=None,
+                 charge_response: ChargeResponse | None = None):
         self._numbers = np.ascontiguousarray(numbers, dtype=np.int32)
         self._positions = np.ascontiguousarray(positions, dtype=float).reshape(-1, 3)
         if self._numbers.ndim != 1 or len(self._numbers) != len(self._positions):
@@ -95,6 +121,9 @@ class Structure:
         self._uhf = int(uhf or 0)
         self._point_charges = point_charges
         self._efield = _normalize_efield(efield)
+        if charge_response is not None and charge_response.shifts.size != len(self._numbers):
+            raise XTBloomValueError("charge response shifts must match the atom count")
+        self._charge_response = charge_response
 
     numbers = property(lambda self: self._numbers)
     positions = property(lambda self: self._positions)
@@ -102,6 +131,7 @@ class Structure:
     uhf = property(lambda self: self._uhf)
     point_charges = property(lambda self: self._point_charges)
     efield = property(lambda self: self._efield)
+    charge_response = property(lambda self: self._charge_response)
 
     def update(self, positions=None, point_charges=None):
         if positions is not None:
@@ -187,6 +217,7 @@ class ComputedBatch:
     per_system_status: np.ndarray
     point_charge_potential_gradient: np.ndarray | None = None
     extension: dict | None = None
+    result_flags: int = 0  # xtbloom_batch_result_t.flags of the call
 
 
 _DEFAULT_FLAGS = library.COMPUTE_ENERGY | library.COMPUTE_FORCES | library.COMPUTE_ATOMIC_CHARGES
@@ -221,7 +252,9 @@ def _field_attachments(B: int, electric_field):
 def _descriptors(lib, atom_offsets, numbers, positions, charges, uhf, point_offsets,
                  point_positions, point_values, point_gammas, model, flags, max_scc_iterations,
                  charge_tolerance, energy_tolerance, electronic_temperature, scc_mixer_history,
-                 scc_mixer_damping, scc_start_mode, determinism=0, electric_field=None):
+                 scc_mixer_damping, scc_start_mode, determinism=0, electric_field=None,
+                 atomic_potential_shifts=None, charge_response_offsets=None,
+                 charge_response_matrix=None):
     """Batch and compute-options descriptors over flat host arrays (kept alive by the caller)."""
     atom_offsets = np.ascontiguousarray(atom_offsets, dtype=np.int64)
     numbers = np.ascontiguousarray(numbers, dtype=np.int32)
@@ -260,6 +293,17 @@ def _descriptors(lib, atom_offsets, numbers, positions, charges, uhf, point_offs
             batch.interaction_descriptors = library.ConstBuffer(ctypes.addressof(desc), ctypes.sizeof(desc),
                                                                 library.MEMORY_HOST, 0)
             batch.interaction_payload = _const(payload)
+    operator = []
+    if atomic_potential_shifts is not None:
+        operator.append(np.ascontiguousarray(atomic_potential_shifts, dtype=np.float64))
+        batch.atomic_potential_shifts = _const(operator[-1])
+    if charge_response_matrix is not None or charge_response_offsets is not None:
+        roff = np.ascontiguousarray(charge_response_offsets, dtype=np.int64)
+        rmat = np.ascontiguousarray(charge_response_matrix, dtype=np.float64)
+        operator += [roff, rmat]
+        batch.total_charge_response_elements = rmat.size
+        batch.charge_response_offsets = _const(roff)
+        batch.charge_response_matrix = _const(rmat)
     opts = library.ComputeOptions()
     _check(lib.xtbloom_compute_options_init(ctypes.byref(opts), ctypes.sizeof(opts)))
     opts.model = model
@@ -273,7 +317,7 @@ def _descriptors(lib, atom_offsets, numbers, positions, charges, uhf, point_offs
     opts.scc_start_mode = scc_start_mode
     opts.determinism = determinism
     keep = (atom_offsets, numbers, positions, charges, uhf, point_offsets, point_positions,
-            point_values, point_gammas, fields)
+            point_values, point_gammas, fields, operator)
     return batch, opts, keep, (B, N, P)
 
 
@@ -286,7 +330,9 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                    request: "Request | None" = None,
                    extension=False, diagnostics=False, kernel_stats=False,
                    determinism=library.DETERMINISM_DEFAULT, options_hook=None,
-                   no_alloc=False, wait=True, electric_field=None) -> "ComputedBatch | PendingBatch":
+                   no_alloc=False, wait=True, electric_field=None, atomic_potential_shifts=None,
+                   charge_response_offsets=None,
+                   charge_response_matrix=None) -> "ComputedBatch | PendingBatch":
     """One ``xtbloom_compute`` over flat ragged arrays (the ``xtbloom_batch_t`` layout).
 
     ``scc_start_mode=library.SCC_START_WARM`` starts from the converged state of
@@ -302,14 +348,17 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
     enqueue); ``wait=False`` returns a :class:`PendingBatch` at once, whose
     ``result()`` waits for the request.  ``electric_field`` attaches uniform
     fields (a (B, 3) array, or B entries of None or a length-3 vector) as
-    ABI-v3 interaction attachments.
+    ABI-v3 interaction attachments.  ``atomic_potential_shifts`` (b, [N]),
+    ``charge_response_offsets`` ([B + 1]) and ``charge_response_matrix`` (A,
+    n_atoms^2 per system, packed) add the external charge operator.
     """
     lib = library.load()
     batch, opts, keep, (B, N, P) = _descriptors(
         lib, atom_offsets, numbers, positions, charges, uhf, point_offsets, point_positions,
         point_values, point_gammas, model, flags, max_scc_iterations, charge_tolerance,
         energy_tolerance, electronic_temperature, scc_mixer_history, scc_mixer_damping,
-        scc_start_mode, determinism, electric_field)
+        scc_start_mode, determinism, electric_field, atomic_potential_shifts,
+        charge_response_offsets, charge_response_matrix)
     if options_hook is not None:
         options_hook(opts)
     atom_offsets, point_offsets = keep[0], keep[5]
@@ -397,7 +446,7 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
         _check(lib.xtbloom_compute(context.handle, ctypes.byref(batch), ctypes.byref(opts),
                                    ctypes.byref(res)))
     return ComputedBatch(atom_offsets, point_offsets, energies, forces, q, pcf, dip, iters,
-                         conv.astype(bool), status, dedq, ext_out)
+                         conv.astype(bool), status, dedq, ext_out, int(res.flags))
 
 
 class PendingBatch:
@@ -423,6 +472,7 @@ class PendingBatch:
                 raise XTBloomError(info["completion_status"], self.request.error())
             c = self._computed
             c.scc_converged = c.scc_converged.astype(bool)
+            c.result_flags = int(info.get("result_flags", 0))
             self._result = c
         return self._result
 
@@ -493,8 +543,15 @@ class Plan:
                  uhf=None, point_offsets=None, point_positions=None, point_values=None,
                  point_gammas=None, *, model=library.MODEL_GFN2_XTB, flags=_DEFAULT_FLAGS,
                  max_scc_iterations=250, charge_tolerance=1e-6, energy_tolerance=1e-8,
-                 electronic_temperature=300.0, scc_mixer_history=8, scc_mixer_damping=0.4):
+                 electronic_temperature=300.0, scc_mixer_history=8, scc_mixer_damping=0.4,
+                 atomic_potential_shifts=None, charge_response_offsets=None,
+                 charge_response_matrix=None):
         lib = library.load()
+        # the operator's presence belongs to the plan; its values are inputs
+        # of each call (defaulting to these)
+        self._operator = dict(atomic_potential_shifts=atomic_potential_shifts,
+                              charge_response_offsets=charge_response_offsets,
+                              charge_response_matrix=charge_response_matrix)
         self._policy = dict(model=model, flags=flags, max_scc_iterations=max_scc_iterations,
                             charge_tolerance=charge_tolerance, energy_tolerance=energy_tolerance,
                             electronic_temperature=electronic_temperature,
@@ -502,7 +559,8 @@ class Plan:
                             scc_mixer_damping=scc_mixer_damping)
         batch, opts, keep, _ = _descriptors(
             lib, atom_offsets, numbers, positions, charges, uhf, point_offsets, point_positions,
-            point_values, point_gammas, scc_start_mode=library.SCC_START_FRESH, **self._policy)
+            point_values, point_gammas, scc_start_mode=library.SCC_START_FRESH, **self._policy,
+            **self._operator)
         self._topology = dict(atom_offsets=keep[0], numbers=keep[1], charges=keep[3],
                               uhf=keep[4], point_offsets=keep[5], point_gammas=keep[8])
         self._flags = int(opts.flags)
@@ -519,13 +577,19 @@ class Plan:
         return self._handle
 
     def compute(self, positions, point_positions=None, point_values=None, *,
-                scc_start_mode=library.SCC_START_FRESH, electric_field=None) -> ComputedBatch:
+                scc_start_mode=library.SCC_START_FRESH, electric_field=None,
+                atomic_potential_shifts=None, charge_response_matrix=None) -> ComputedBatch:
         """One ``xtbloom_plan_compute`` with new geometry (and, optionally, new
         uniform fields: a plan accepts any field, a WARM start only the
-        checkpoint's)."""
+        checkpoint's; and new b or A values for a plan made with them)."""
+        op = dict(self._operator)
+        if atomic_potential_shifts is not None:
+            op["atomic_potential_shifts"] = atomic_potential_shifts
+        if charge_response_matrix is not None:
+            op["charge_response_matrix"] = charge_response_matrix
         return compute_arrays(self._context, positions=positions, point_positions=point_positions,
                               point_values=point_values, scc_start_mode=scc_start_mode, plan=self,
-                              electric_field=electric_field, **self._topology, **self._policy)
+                              electric_field=electric_field, **self._topology, **self._policy, **op)
 
     def query_workspace(self) -> dict:
         """Reserved workspace (``xtbloom_plan_query_workspace``)."""
@@ -772,6 +836,19 @@ def _flatten(structures: Sequence[Structure]):
                   point_gammas=np.concatenate([p.gammas for p in pcs]))
     if any(s.efield is not None for s in structures):
         kw["electric_field"] = [s.efield for s in structures]
+    if any(s.charge_response is not None for s in structures):
+        # systems without an operator get zero b and A: the ABI takes one
+        # batch-wide operator (reference python/xtbloom/interface.py:679-710)
+        shifts, mats = [], []
+        for s in structures:
+            n = len(s.numbers)
+            r = s.charge_response
+            shifts.append(np.zeros(n) if r is None else r.shifts)
+            mats.append(np.zeros(n * n) if r is None else r.matrix.ravel())
+        roff = np.zeros(len(structures) + 1, np.int64)
+        roff[1:] = np.cumsum([m.size for m in mats])
+        kw.update(atomic_potential_shifts=np.concatenate(shifts), charge_response_offsets=roff,
+                  charge_response_matrix=np.concatenate(mats))
     return offs, numbers, positions, charges, uhf, kw
 
 
@@ -826,8 +903,9 @@ class Calculator(Structure):
 
     def __init__(self, method: str, numbers, positions, charge: float = 0.0, uhf: int | None = None,
                  point_charges: PointCharge | None = None, *, backend: str | int = "hip",
-                 device_id: int | None = None, warm_start: bool = False, efield=None, **settings):
-        super().__init__(numbers, positions, charge, uhf, point_charges, efield)
+                 device_id: int | None = None, warm_start: bool = False, efield=None,
+                 charge_response: ChargeResponse | None = None, **settings):
+        super().__init__(numbers, positions, charge, uhf, point_charges, efield, charge_response)
         self._model = _resolve_method(method)
         self._method = method
         self._context = Context(backend, device_id)

@@ -46,6 +46,10 @@ COMPUTE_ATOMIC_CHARGES = 1 << 2
 COMPUTE_POINT_CHARGE_FORCES = 1 << 3
 COMPUTE_DIPOLE_MOMENTS = 1 << 4
 
+# xtbloom_batch_result_t.flags
+RESULT_FORCES_EXCLUDE_EXTERNAL_OPERATOR_DERIVATIVES = 1 << 0
+RESULT_DIPOLE_MOMENTS = 1 << 4
+
 SCC_START_FRESH = 1
 SCC_START_WARM = 2
 SCC_MIXER_MODIFIED_BROYDEN = 1
