@@ -84,3 +84,71 @@ def test_system_above_ao_limit_is_reported_per_system():
         a0, a1 = b.atom_offsets[i], b.atom_offsets[i + 1]
         assert abs(out.energies[i] - ref.energy) < 1e-9
         assert np.abs(out.forces[a0:a1] + ref.gradient).max() < 1e-9
+
+
+def _lattice_call(ctx, b, cells, axes):
+    import ctypes
+    lib = library.load()
+    batch, opts, keep, (B, N, _) = interface._descriptors(
+        lib, b.atom_offsets, b.atomic_numbers, b.positions, b.molecular_charges,
+        b.unpaired_electrons, None, None, None, None, library.MODEL_GFN2_XTB,
+        interface._DEFAULT_FLAGS, 250, 1e-6, 1e-8, 300.0, 8, 0.4, library.SCC_START_FRESH)
+    held = []
+    if cells is not None:
+        held.append(np.ascontiguousarray(cells, dtype=np.float64))
+        batch.cell_matrices = interface._const(held[-1])
+    if axes is not None:
+        held.append(np.ascontiguousarray(axes, dtype=np.int32))
+        batch.periodic_axes = interface._const(held[-1])
+    e = np.full(B, np.nan)
+    res = library.BatchResult()
+    assert lib.xtbloom_batch_result_init(ctypes.byref(res), ctypes.sizeof(res)) == 0
+    f, q = np.full((N, 3), np.nan), np.full(N, np.nan)
+    it, conv, st = np.zeros(B, np.int32), np.zeros(B, np.uint8), np.zeros(B, np.int32)
+    res.energies, res.forces, res.atomic_charges = (interface._out(x) for x in (e, f, q))
+    res.scc_iterations, res.scc_converged, res.per_system_status = (interface._out(x) for x in (it, conv, st))
+    status = lib.xtbloom_compute(ctx.handle, ctypes.byref(batch), ctypes.byref(opts), ctypes.byref(res))
+    return status, e
+
+
+def test_native_lattice_descriptors():
+    """ABI-v4 lattice descriptors (reference validation.cpp:626-648,
+    1013-1089): all-NONE items with zero cells run as molecules; a valid XYZ
+    cell is NOT_IMPLEMENTED; malformed descriptors are INVALID_ARGUMENT;
+    one- and two-dimensional masks are NOT_SUPPORTED."""
+    b = workloads.water_batch(batch=2, seed=3)
+    zero = np.zeros((2, 9))
+    cube = np.array([[20.0, 0, 0, 0, 20.0, 0, 0, 0, 20.0]] * 2)
+    ctx = interface.Context("hip", 0)
+    try:
+        print("case native lattice descriptors", flush=True)
+        plain = interface.compute_arrays(ctx, b.atom_offsets, b.atomic_numbers, b.positions,
+                                         b.molecular_charges, b.unpaired_electrons)
+        st, e = _lattice_call(ctx, b, zero, [0, 0])
+        assert st == 0 and np.array_equal(e, plain.energies)
+        first = cube.copy()
+        first[1] = 0.0  # a cell for item 0 only (item 1 is NONE)
+        left = first.copy()
+        left[0, 8] = -20.0  # left-handed
+        flat = first.copy()
+        flat[0, 6:9] = [20.0, 20.0, 0.0]  # third row in the plane of the first two
+        nan = first.copy()
+        nan[0, 4] = np.nan
+        inv, ns, ni = library.STATUS_INVALID_ARGUMENT, library.STATUS_NOT_SUPPORTED, library.STATUS_NOT_IMPLEMENTED
+        cases = {
+            "valid XYZ": ((first, [7, 0]), ni),
+            "valid XYZ, both items": ((cube, [7, 7]), ni),
+            "cells only": ((zero, None), inv),
+            "axes only": ((None, [0, 0]), inv),
+            "NONE with a cell": ((first, [0, 0]), inv),
+            "unknown bits": ((first, [15, 0]), inv),
+            "two-dimensional": ((first, [3, 0]), ns),
+            "left-handed": ((left, [7, 0]), inv),
+            "singular": ((flat, [7, 0]), inv),
+            "NaN cell": ((nan, [7, 0]), inv),
+        }
+        got = {k: _lattice_call(ctx, b, *args)[0] for k, (args, _) in cases.items()}
+        want = {k: w for k, (_, w) in cases.items()}
+        assert got == want
+    finally:
+        ctx.close()

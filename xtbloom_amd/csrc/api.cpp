@@ -333,6 +333,69 @@ struct Staged {
   std::vector<double> pshift, presp;
 };
 
+// Native lattice descriptors (ABI-v4 suffix; reference xtbloom.h:186-200,
+// validation.cpp:626-648, 1013-1089, lattice.hpp:85-123): cell_matrices and
+// periodic_axes come together; a NONE item needs an all-zero cell, a mask
+// with unknown bits is invalid, one- and two-dimensional masks are not
+// supported, an XYZ cell must be finite, right-handed and nonsingular (the
+// determinant of the rows, each scaled to unit length, above 2^-46).  Valid
+// XYZ items are NOT_IMPLEMENTED; a batch whose items are all NONE runs as a
+// molecular batch.
+xtbloom_status_t check_lattice(const xtbloom_batch_t* batch, int64_t B) {
+  if (batch->struct_size < XTBLOOM_BATCH_V4_SIZE) return XTBLOOM_STATUS_SUCCESS;
+  const bool cells_on = buffer_active(batch->cell_matrices), axes_on = buffer_active(batch->periodic_axes);
+  if (!cells_on && !axes_on) return XTBLOOM_STATUS_SUCCESS;
+  if (!cells_on || !axes_on)
+    return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "cell_matrices and periodic_axes must be supplied together");
+  std::vector<char> cells, axes;
+  xtbloom_status_t s;
+  if ((s = stage_input(batch->cell_matrices, sizeof(double) * 9 * (size_t)B, "cell_matrices", cells, true)) ||
+      (s = stage_input(batch->periodic_axes, sizeof(int32_t) * (size_t)B, "periodic_axes", axes, true)))
+    return s;
+  bool periodic = false;
+  for (int64_t i = 0; i < B; ++i) {
+    int32_t mask;
+    double cell[9];
+    std::memcpy(&mask, axes.data() + sizeof(int32_t) * i, sizeof(mask));
+    std::memcpy(cell, cells.data() + sizeof(cell) * i, sizeof(cell));
+    if (mask == XTBLOOM_PERIODIC_AXES_NONE) {
+      for (double v : cell)
+        if (v != 0.0) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "a nonperiodic batch item must use an all-zero cell matrix");
+      continue;
+    }
+    if ((mask & ~XTBLOOM_PERIODIC_AXES_XYZ) != 0)
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "periodic_axes contains unknown mask bits");
+    if (mask != XTBLOOM_PERIODIC_AXES_XYZ)
+      return fail(XTBLOOM_STATUS_NOT_SUPPORTED, "one- and two-dimensional periodic axes are reserved but not supported");
+    for (double v : cell)
+      if (!std::isfinite(v)) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "a periodic cell matrix contains NaN or infinity");
+    double u[9];
+    bool ok = true;
+    for (int r = 0; r < 3 && ok; ++r) {
+      double m = 0.0;
+      for (int c = 0; c < 3; ++c) m = std::max(m, std::fabs(cell[3 * r + c]));
+      if (!(m > 0.0)) { ok = false; break; }
+      double nn = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        u[3 * r + c] = cell[3 * r + c] / m;
+        nn += u[3 * r + c] * u[3 * r + c];
+      }
+      const double norm = std::sqrt(nn);
+      for (int c = 0; c < 3; ++c) u[3 * r + c] /= norm;
+    }
+    const double det = ok ? u[0] * (u[4] * u[8] - u[5] * u[7]) + u[1] * (u[5] * u[6] - u[3] * u[8]) +
+                                u[2] * (u[3] * u[7] - u[4] * u[6])
+                          : 0.0;
+    if (!(det > 0x1p-46) || !std::isfinite(det))
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "a periodic cell must be finite, right-handed, and nonsingular");
+    periodic = true;
+  }
+  if (periodic)
+    return fail(XTBLOOM_STATUS_NOT_IMPLEMENTED,
+                "native lattice/PBC descriptors are valid but periodic GFN2 execution is not implemented");
+  return XTBLOOM_STATUS_SUCCESS;
+}
+
 // External charge operator v = b + A q, E = q.b + q.A.q/2 (periodic QM/MM
 // coupling; reference xtbloom.h:424-431, validation.cpp:401-403, 676-700,
 // 1308-1360, gfn2_cpu_execution.cpp:448-465, 538-560): b is active when its
@@ -616,9 +679,7 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
     }
   }
   if ((s = stage_interactions(batch, B, st))) return s;
-  if (batch->struct_size >= XTBLOOM_BATCH_V4_SIZE &&
-      (buffer_active(batch->cell_matrices) || buffer_active(batch->periodic_axes)))
-    return fail(XTBLOOM_STATUS_NOT_IMPLEMENTED, "periodic batches are not implemented");
+  if ((s = check_lattice(batch, B))) return s;
   if (P > 0) {
     if ((s = stage_input(batch->point_charge_offsets, sizeof(int64_t) * (B + 1), "point_charge_offsets", st.poff, true)) ||
         (s = stage_input(batch->point_charge_positions, sizeof(double) * 3 * P, "point_charge_positions", st.pxyz, true)) ||
