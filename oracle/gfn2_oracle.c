@@ -72,6 +72,8 @@ void oracle_default_options(oracle_options* o) {
   o->efield[0] = o->efield[1] = o->efield[2] = 0.0;
   o->potential_shifts = NULL;
   o->charge_response = NULL;
+  o->spin_channels = 1;
+  o->reserved = 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1575,7 +1577,38 @@ typedef struct {
    * either may be NULL */
   const double* pshift;
   const double* presp;
+  /* spin channels (wavefunction.hpp:55-65): 1 restricted, 2 unrestricted.
+   * With two, H/C/eps/occ hold the alpha channel and H2/C2/eps2/occ2 the beta
+   * one; P and W are the totals, Pa/Pb the channel densities, Pm = Pa - Pb */
+  int nch;
+  double *H2, *C2, *eps2, *occ2, *Pa, *Pb, *Pm, *W2;
 } System;
+
+/* tblite spin constant index of an l pair (ss sp pp sd pd dd), symmetric */
+static int spin_index(int l1, int l2) {
+  if (l1 > l2) { int t = l1; l1 = l2; l2 = t; }
+  if (l1 == 0) return l2 == 0 ? 0 : (l2 == 1 ? 1 : 3);
+  if (l1 == 1) return l2 == 1 ? 2 : 4;
+  return 5;
+}
+
+/* magnetization shell potential v_m = W_A m_A per atom and the spin energy
+ * m.W.m / 2 (spin.hpp:106-118, spin.cpp:135-172) */
+static double spin_potential(const System* sy, const double* m, double* vm) {
+  const Basis* b = &sy->b;
+  double e = 0.0;
+  for (int a = 0; a < sy->nat; ++a) {
+    const int z = b->Z[a];
+    for (int s = b->atom_sh[a]; s < b->atom_sh[a + 1]; ++s) {
+      double v = 0.0;
+      for (int t = b->atom_sh[a]; t < b->atom_sh[a + 1]; ++t)
+        v = fma(xtbp_spin_w[6 * (z - 1) + spin_index(b->sh_l[s], b->sh_l[t])], m[t], v);
+      if (vm) vm[s] = v;
+      e = fma(0.5 * m[s], v, e);
+    }
+  }
+  return e;
+}
 
 /* v = b + A q and its energy q.b + q.(A q)/2 (periodic_embedding.cpp:352-434) */
 static double charge_operator(const System* sy, const double* q, double* v) {
@@ -1592,8 +1625,8 @@ static double charge_operator(const System* sy, const double* q, double* v) {
   return fma(0.5, quad, lin);
 }
 
-static void mulliken(const System* sy, const double* P, double* qsh, double* qat, double* dip,
-                     double* quad) {
+static void mulliken(const System* sy, const double* P, int add_ref, double* qsh, double* qat,
+                     double* dip, double* quad) {
   const Basis* b = &sy->b;
   const int n = sy->n;
   const size_t nn = (size_t)n * n;
@@ -1613,7 +1646,7 @@ static void mulliken(const System* sy, const double* P, double* qsh, double* qat
     }
   for (int a = 0; a < sy->nat; ++a) qat[a] = 0.0;
   for (int s = 0; s < sy->ns; ++s) {
-    qsh[s] += xtbp_shell_refocc[b->sh_par[s]];
+    if (add_ref) qsh[s] += xtbp_shell_refocc[b->sh_par[s]];
     qat[b->sh_atom[s]] += qsh[s];
   }
 }
@@ -1672,12 +1705,16 @@ static void fock(const System* sy, const double* vsh, const double* vd, const do
   }
 }
 
-/* solve_one_spin + occupations + density (eigensolver.cpp:1221-1400) */
-static int solve(System* sy, double nalpha, double nbeta, double kt, double* entropy) {
+/* solve_one_spin + occupations + density (eigensolver.cpp:1221-1400): one
+ * orbital set from H (C, eps), occupied by fa (nalpha electrons) and, when
+ * fb is given, fb (nbeta electrons) on the same orbitals; P and W receive
+ * C diag(f) C^T and C diag(f eps) C^T with f = fa + fb */
+static int solve_channel(System* sy, const double* H, double* A, double* eps, double* fa, double* fb,
+                         double nalpha, double nbeta, double kt, double* entropy, double* P,
+                         double* W) {
   const int n = sy->n;
-  double* A = sy->C;
   for (int r = 0; r < n; ++r)
-    for (int c = 0; c < n; ++c) A[(size_t)c * n + r] = sy->H[(size_t)r * n + c];
+    for (int c = 0; c < n; ++c) A[(size_t)c * n + r] = H[(size_t)r * n + c];
   la_dtrsm(kColMajor, kLeft, kLower, kNoTrans, kNonUnit, n, n, 1.0, sy->L, n, A, n);
   la_dtrsm(kColMajor, kRight, kLower, kCblasTrans, kNonUnit, n, n, 1.0, sy->L, n, A, n);
   for (int c = 0; c < n; ++c)
@@ -1689,28 +1726,49 @@ static int solve(System* sy, double nalpha, double nbeta, double kt, double* ent
   int lwork = 1 + 6 * n + 2 * n * n, liwork = 3 + 5 * n;
   double* work = malloc(sizeof(double) * lwork);
   int* iwork = malloc(sizeof(int) * liwork);
-  int info = la_dsyevd(kColMajor, 'V', 'L', n, A, n, sy->eps, work, lwork, iwork, liwork);
+  int info = la_dsyevd(kColMajor, 'V', 'L', n, A, n, eps, work, lwork, iwork, liwork);
   free(work);
   free(iwork);
   if (info != 0) return 0;
   la_dtrsm(kColMajor, kLeft, kLower, kCblasTrans, kNonUnit, n, n, 1.0, sy->L, n, A, n);
-  double mu, sa, sb;
-  double* fa = sy->occ;
-  double* fb = sy->occ + n;
-  if (!oracle_fermi(n, sy->eps, nalpha, kt, fa, &mu, &sa)) return 0;
-  if (!oracle_fermi(n, sy->eps, nbeta, kt, fb, &mu, &sb)) return 0;
+  double mu, sa, sb = 0.0;
+  if (!oracle_fermi(n, eps, nalpha, kt, fa, &mu, &sa)) return 0;
+  if (fb && !oracle_fermi(n, eps, nbeta, kt, fb, &mu, &sb)) return 0;
   *entropy = sa + sb;
   double* wc = malloc(sizeof(double) * (size_t)n * n);
   for (int pass = 0; pass < 2; ++pass) {
     for (int o = 0; o < n; ++o) {
-      double wgt = fa[o] + fb[o];
-      if (pass == 1) wgt *= sy->eps[o];
+      double wgt = fa[o] + (fb ? fb[o] : 0.0);
+      if (pass == 1) wgt *= eps[o];
       for (int r = 0; r < n; ++r) wc[r + (size_t)o * n] = A[r + (size_t)o * n] * wgt;
     }
     /* P and W are symmetric, so column-major equals row-major */
-    la_dgemm(kColMajor, kNoTrans, kCblasTrans, n, n, n, 1.0, wc, n, A, n, 0.0, pass ? sy->W : sy->P, n);
+    la_dgemm(kColMajor, kNoTrans, kCblasTrans, n, n, n, 1.0, wc, n, A, n, 0.0, pass ? W : P, n);
   }
   free(wc);
+  return 1;
+}
+
+/* Both channels: restricted, one orbital set occupied by alpha and beta
+ * electrons; unrestricted, alpha electrons in the orbitals of H and beta
+ * electrons in those of H2 (eigensolver.cpp:1166-1400), with P, W the totals */
+static int solve(System* sy, double nalpha, double nbeta, double kt, double* entropy) {
+  const int n = sy->n;
+  const size_t nn = (size_t)n * n;
+  if (sy->nch == 1)
+    return solve_channel(sy, sy->H, sy->C, sy->eps, sy->occ, sy->occ + n, nalpha, nbeta, kt, entropy,
+                         sy->P, sy->W);
+  double sa = 0.0, sb = 0.0;
+  if (!solve_channel(sy, sy->H, sy->C, sy->eps, sy->occ, NULL, nalpha, 0.0, kt, &sa, sy->Pa, sy->W))
+    return 0;
+  if (!solve_channel(sy, sy->H2, sy->C2, sy->eps2, sy->occ2, NULL, nbeta, 0.0, kt, &sb, sy->Pb, sy->W2))
+    return 0;
+  *entropy = sa + sb;
+  for (size_t k = 0; k < nn; ++k) {
+    sy->P[k] = sy->Pa[k] + sy->Pb[k];
+    sy->Pm[k] = sy->Pa[k] - sy->Pb[k];
+    sy->W[k] += sy->W2[k];
+  }
   return 1;
 }
 
@@ -1718,6 +1776,8 @@ static void system_free(System* sy) {
   free(sy->S); free(sy->D); free(sy->Q); free(sy->H0); free(sy->G); free(sy->Vpc); free(sy->cn);
   free(sy->H); free(sy->C); free(sy->eps); free(sy->P); free(sy->W); free(sy->occ); free(sy->L);
   free(sy->aes);
+  free(sy->H2); free(sy->C2); free(sy->eps2); free(sy->occ2); free(sy->Pa); free(sy->Pb); free(sy->Pm);
+  free(sy->W2);
   if (sy->d4.cn) d4_free(&sy->d4);
   basis_free(&sy->b);
 }
@@ -1782,26 +1842,72 @@ static int system_setup(System* sy, int32_t nat, const int32_t* Z, const double*
   return ST_OK;
 }
 
+/* Second-channel buffers of an unrestricted system. */
+static void system_channels(System* sy, int nch) {
+  sy->nch = nch;
+  if (nch != 2) return;
+  const size_t nn = (size_t)sy->n * sy->n;
+  sy->H2 = malloc(sizeof(double) * nn); sy->C2 = malloc(sizeof(double) * nn);
+  sy->eps2 = malloc(sizeof(double) * sy->n); sy->occ2 = malloc(sizeof(double) * sy->n);
+  sy->Pa = malloc(sizeof(double) * nn); sy->Pb = malloc(sizeof(double) * nn);
+  sy->Pm = malloc(sizeof(double) * nn); sy->W2 = malloc(sizeof(double) * nn);
+}
+
+/* Offsets of the fields of a state vector in the reference's packing
+ * (scc_driver.cpp:1262-1293, wavefunction.hpp:55-65): every field holds its
+ * channels one after the other, [qsh(nch*ns), dip(nch*3N), quad(nch*6N)];
+ * channel 1 is the magnetization. */
+typedef struct { int qsh, mag, dip, dipm, quad, quadm, dim; } StateLayout;
+static StateLayout state_layout(const System* sy) {
+  const int ns = sy->ns, nat = sy->nat, c = sy->nch;
+  StateLayout l;
+  l.qsh = 0; l.mag = ns;
+  l.dip = c * ns; l.dipm = l.dip + 3 * nat;
+  l.quad = c * (ns + 3 * nat); l.quadm = l.quad + 6 * nat;
+  l.dim = c * (ns + 9 * nat);
+  return l;
+}
+
 /*
- * One SCC step from the mixed state x = [qsh, dip(3N), quad(6N)] (scc_driver.cpp:706-860):
- * potentials -> H -> eigensolve -> P -> raw Mulliken multipoles -> F = E - kT S.
+ * One SCC step from the mixed state x (scc_driver.cpp:706-860): potentials ->
+ * H (per channel) -> eigensolve -> P -> raw Mulliken multipoles -> F = E - kT S.
+ * Unrestricted: the magnetization potential v_m = W m enters as vsh + v_m in
+ * the alpha and vsh - v_m in the beta Hamiltonian (scc_driver.cpp:2593-2640);
+ * the raw magnetization multipoles are those of Pa - Pb.
  */
 static int scc_step(System* sy, const double* x, double nalpha, double nbeta, double kt,
                     double* raw, double* qat_scratch, double* F, double ecomp[8]) {
   const int ns = sy->ns, nat = sy->nat;
   const size_t nn = (size_t)sy->n * sy->n;
-  const double *qsh = x, *dip = x + ns, *quad = x + ns + 3 * nat;
-  double* vsh = malloc(sizeof(double) * (ns + 11 * nat));
+  const StateLayout L = state_layout(sy);
+  const double *qsh = x, *dip = x + L.dip, *quad = x + L.quad;
+  double* vsh = malloc(sizeof(double) * (3 * ns + 11 * nat));
   double *vd = vsh + ns, *vqq = vd + 3 * nat, *vat = vqq + 6 * nat, *d4v = vat + nat;
+  double *vm = d4v + nat, *vs = vm + ns;
   for (int a = 0; a < nat; ++a) qat_scratch[a] = 0.0;
   for (int s = 0; s < ns; ++s) qat_scratch[sy->b.sh_atom[s]] += qsh[s];
   potentials(sy, qsh, qat_scratch, dip, quad, vsh, vd, vqq, vat, d4v);
-  fock(sy, vsh, vd, vqq, sy->H);
+  if (sy->nch == 1) {
+    fock(sy, vsh, vd, vqq, sy->H);
+  } else {
+    spin_potential(sy, x + L.mag, vm);
+    for (int s = 0; s < ns; ++s) vs[s] = vsh[s] + vm[s];
+    fock(sy, vs, vd, vqq, sy->H);
+    for (int s = 0; s < ns; ++s) vs[s] = vsh[s] - vm[s];
+    fock(sy, vs, vd, vqq, sy->H2);
+  }
   free(vsh);
   double entropy = 0.0;
   if (!solve(sy, nalpha, nbeta, kt, &entropy)) return ST_EIGEN_FAILED;
-  double *rq = raw, *rd = raw + ns, *rQ = raw + ns + 3 * nat;
-  mulliken(sy, sy->P, rq, qat_scratch, rd, rQ);
+  double *rq = raw, *rd = raw + L.dip, *rQ = raw + L.quad;
+  mulliken(sy, sy->P, 1, rq, qat_scratch, rd, rQ);
+  double espin = 0.0;
+  if (sy->nch == 2) {
+    double* qm = malloc(sizeof(double) * nat);
+    mulliken(sy, sy->Pm, 0, raw + L.mag, qm, raw + L.dipm, raw + L.quadm);
+    free(qm);
+    espin = spin_potential(sy, raw + L.mag, NULL);
+  }
   /* evaluate_scc_energy_system (scc_driver.cpp:2105-2286) */
   double ecore = 0.0;
   for (size_t k = 0; k < nn; ++k) ecore = fma(sy->H0[k], sy->P[k], ecore);
@@ -1830,34 +1936,57 @@ static int scc_step(System* sy, const double* x, double nalpha, double nbeta, do
   /* external charge operator at the raw atomic charges, summed last
    * (scc_driver.cpp:2236-2262) */
   const double eop = (sy->pshift || sy->presp) ? charge_operator(sy, qat_scratch, NULL) : 0.0;
-  double e_int = ecore + ees2 + ees3 + eaes + ed4 + epc + efield + eop;
+  /* the spin energy joins after AES2 (scc_driver.cpp:2258-2262) */
+  double e_int = ecore + ees2 + ees3 + eaes + espin + ed4 + epc + efield + eop;
   *F = fma(-kt, entropy, e_int);
   /* component 5 holds the external terms: point charges, field and operator */
-  ecomp[0] = ecore; ecomp[1] = ees2; ecomp[2] = ees3; ecomp[3] = eaes; ecomp[4] = ed4;
+  /* component 3 holds AES2 and the spin polarization */
+  ecomp[0] = ecore; ecomp[1] = ees2; ecomp[2] = ees3; ecomp[3] = eaes + espin; ecomp[4] = ed4;
   ecomp[5] = epc + efield + eop;
   return ST_OK;
+}
+
+int oracle_scc_replay_channels(int32_t nat, const int32_t* Z, const double* xyz, double charge,
+                               int32_t unpaired, int32_t spin_channels, int32_t npc, const double* pc_xyz,
+                               const double* pc_q, const double* pc_gamma, double kt, const double* mixed,
+                               double* raw, double* hamiltonian, double* density, double* eigenvalues,
+                               double* occupations, double* free_energy) {
+  if (!la_dsyevd) return ST_INVALID;
+  if (spin_channels != 1 && spin_channels != 2) return ST_INVALID;
+  System sy;
+  double na, nb, ecomp[8];
+  int st = system_setup(&sy, nat, Z, xyz, charge, unpaired, npc, pc_xyz, pc_q, pc_gamma, &na, &nb);
+  if (st == ST_OK) {
+    system_channels(&sy, spin_channels);
+    double* qat = malloc(sizeof(double) * nat);
+    st = scc_step(&sy, mixed, na, nb, kt, raw, qat, free_energy, ecomp);
+    free(qat);
+    if (st == ST_OK) {
+      const int n = sy.n;
+      const size_t nn = (size_t)n * n;
+      if (hamiltonian) memcpy(hamiltonian, sy.H, sizeof(double) * nn);
+      if (eigenvalues) memcpy(eigenvalues, sy.eps, sizeof(double) * n);
+      if (spin_channels == 1) {
+        if (density) memcpy(density, sy.P, sizeof(double) * nn);
+        if (occupations) memcpy(occupations, sy.occ, sizeof(double) * 2 * n);
+      } else {
+        if (hamiltonian) memcpy(hamiltonian + nn, sy.H2, sizeof(double) * nn);
+        if (density) { memcpy(density, sy.Pa, sizeof(double) * nn); memcpy(density + nn, sy.Pb, sizeof(double) * nn); }
+        if (eigenvalues) memcpy(eigenvalues + n, sy.eps2, sizeof(double) * n);
+        if (occupations) { memcpy(occupations, sy.occ, sizeof(double) * n); memcpy(occupations + n, sy.occ2, sizeof(double) * n); }
+      }
+    }
+  }
+  system_free(&sy);
+  return st;
 }
 
 int oracle_scc_replay(int32_t nat, const int32_t* Z, const double* xyz, double charge,
                       int32_t unpaired, int32_t npc, const double* pc_xyz, const double* pc_q,
                       const double* pc_gamma, double kt, const double* mixed, double* raw,
                       double* hamiltonian, double* density, double* free_energy) {
-  if (!la_dsyevd) return ST_INVALID;
-  System sy;
-  double na, nb, ecomp[8];
-  int st = system_setup(&sy, nat, Z, xyz, charge, unpaired, npc, pc_xyz, pc_q, pc_gamma, &na, &nb);
-  if (st == ST_OK) {
-    double* qat = malloc(sizeof(double) * nat);
-    st = scc_step(&sy, mixed, na, nb, kt, raw, qat, free_energy, ecomp);
-    free(qat);
-    if (st == ST_OK) {
-      const size_t nn = (size_t)sy.n * sy.n;
-      if (hamiltonian) memcpy(hamiltonian, sy.H, sizeof(double) * nn);
-      if (density) memcpy(density, sy.P, sizeof(double) * nn);
-    }
-  }
-  system_free(&sy);
-  return st;
+  return oracle_scc_replay_channels(nat, Z, xyz, charge, unpaired, 1, npc, pc_xyz, pc_q, pc_gamma, kt,
+                                    mixed, raw, hamiltonian, density, NULL, NULL, free_energy);
 }
 
 int oracle_singlepoint(int32_t nat, const int32_t* Z, const double* xyz, double charge,
@@ -1887,14 +2016,18 @@ int oracle_singlepoint_state(int32_t nat, const int32_t* Z, const double* xyz, d
   sy.field = opt->efield[0] != 0.0 || opt->efield[1] != 0.0 || opt->efield[2] != 0.0;
   sy.pshift = opt->potential_shifts;
   sy.presp = opt->charge_response;
+  if (opt->spin_channels != 1 && opt->spin_channels != 2) status = ST_INVALID;
   if (status == ST_INVALID || sy.S == NULL) { system_free(&sy); return status == ST_OK ? ST_INVALID : status; }
+  system_channels(&sy, opt->spin_channels);
   const int n = sy.n, ns = sy.ns;
   const size_t nn = (size_t)n * n;
+  const StateLayout SL = state_layout(&sy);
 
-  /* FRESH SAD state (wavefunction.cpp:733-776), or the WARM checkpoint's
-   * converged multipoles (gfn2_cpu_execution.cpp:1261-1268: the restored
-   * wavefunction seeds a reset mixer) */
-  const int dim = ns + 9 * nat;
+  /* FRESH SAD state (wavefunction.cpp:733-776: magnetization, dipoles and
+   * quadrupoles zero), or the WARM checkpoint's converged multipoles
+   * (gfn2_cpu_execution.cpp:1261-1268: the restored wavefunction seeds a
+   * reset mixer) */
+  const int dim = SL.dim;
   double* x = calloc(dim, sizeof(double));
   double* raw = calloc(dim, sizeof(double));
   double* qat = calloc(nat, sizeof(double));
@@ -1951,9 +2084,10 @@ int oracle_singlepoint_state(int32_t nat, const int32_t* Z, const double* xyz, d
   if (iterations) *iterations = it;
 
   if (status == ST_OK) {
-    const double *qsh = x, *dip = x + ns, *quad = x + ns + 3 * nat;
-    double* vsh = malloc(sizeof(double) * (ns + 11 * nat));
+    const double *qsh = x, *dip = x + SL.dip, *quad = x + SL.quad;
+    double* vsh = malloc(sizeof(double) * (2 * ns + 11 * nat));
     double *vd = vsh + ns, *vqq = vd + 3 * nat, *vat = vqq + 6 * nat, *d4v = vat + nat;
+    double* vm = d4v + nat;
     for (int a = 0; a < nat; ++a) qat[a] = 0.0;
     for (int s = 0; s < ns; ++s) qat[sy.b.sh_atom[s]] += qsh[s];
     if (atomic_charges) memcpy(atomic_charges, qat, sizeof(double) * nat);
@@ -1972,7 +2106,10 @@ int oracle_singlepoint_state(int32_t nat, const int32_t* Z, const double* xyz, d
 
     /* stationary potentials at the converged raw multipoles (gfn2_cpu_execution.cpp:1358-1441) */
     potentials(&sy, qsh, qat, dip, quad, vsh, vd, vqq, vat, d4v);
-    /* force composer (force.cpp:501-612) */
+    if (sy.nch == 2) spin_potential(&sy, x + SL.mag, vm);
+    /* force composer (force.cpp:501-612); unrestricted, the magnetization
+     * potential acts on the spin density Pa - Pb through the overlap
+     * derivative (force.cpp:506-533) */
     double* aS = calloc(nn, sizeof(double));
     double* aD = calloc(3 * nn, sizeof(double));
     double* aQ = calloc(6 * nn, sizeof(double));
@@ -1986,6 +2123,10 @@ int oracle_singlepoint_state(int32_t nat, const int32_t* Z, const double* xyz, d
         size_t f = (size_t)r * n + c, rv = (size_t)c * n + r;
         double pd = sy.P[f] + (f == rv ? 0.0 : sy.P[rv]);
         aS[f] += pd * (-0.5 * (vsh[sr] + vsh[sc]));
+        if (sy.nch == 2) {
+          const double pm = sy.Pm[f] + (f == rv ? 0.0 : sy.Pm[rv]);
+          aS[f] += pm * (-0.5 * (vm[sr] + vm[sc]));
+        }
         for (int k = 0; k < 3; ++k) {
           aD[k * nn + f] += -0.5 * pd * vd[3 * ac + k];
           aD[k * nn + rv] += -0.5 * pd * vd[3 * ar + k];
@@ -2073,6 +2214,7 @@ typedef struct {
   const double* shifts;  /* [total atoms] or NULL */
   const double* response; /* packed nat*nat per system or NULL */
   const int64_t* roff;    /* [batch+1] start of each system's A */
+  const int32_t* spin;    /* [batch] spin channels or NULL */
 } BatchJob;
 
 static void* batch_worker(void* arg) {
@@ -2092,6 +2234,7 @@ static void* batch_worker(void* arg) {
       for (int k = 0; k < 3; ++k) o.efield[k] = job->efield[3 * s + k];
     o.potential_shifts = job->shifts ? job->shifts + a0 : NULL;
     o.charge_response = job->response ? job->response + job->roff[s] : NULL;
+    if (job->spin) o.spin_channels = job->spin[s];
     int st = oracle_singlepoint(nat, job->Z + a0, job->xyz + 3 * a0, job->chg[s], job->uhf[s], npc,
                                 npc ? job->pxyz + 3 * p0 : NULL, npc ? job->pq + p0 : NULL,
                                 npc ? job->pg + p0 : NULL, &o, &e,
@@ -2115,7 +2258,8 @@ int oracle_batch(int64_t batch, const int64_t* atom_offsets, const int32_t* Z, c
                  const double* pc_xyz, const double* pc_q, const double* pc_gamma,
                  const oracle_options* options, int32_t threads, double* energies, double* grad,
                  double* atomic_charges, double* pc_grad, int32_t* iterations, int32_t* status,
-                 const double* efield, const double* shifts, const double* response) {
+                 const double* efield, const double* shifts, const double* response,
+                 const int32_t* spin_channels) {
   if (!la_dsyevd) return ST_INVALID;
   int64_t* roff = malloc(sizeof(int64_t) * (size_t)(batch + 1));
   if (!roff) return ST_INVALID;
@@ -2126,7 +2270,7 @@ int oracle_batch(int64_t batch, const int64_t* atom_offsets, const int32_t* Z, c
   }
   BatchJob job = {batch, atom_offsets, point_offsets, Z, unpaired, xyz, charges, pc_xyz, pc_q,
                   pc_gamma, options, energies, grad, atomic_charges, pc_grad, iterations, status,
-                  0, PTHREAD_MUTEX_INITIALIZER, efield, shifts, response, roff};
+                  0, PTHREAD_MUTEX_INITIALIZER, efield, shifts, response, roff, spin_channels};
   if (threads <= 0) threads = (int32_t)sysconf(_SC_NPROCESSORS_ONLN);
   if (threads > 256) threads = 256;
   pthread_t tid[256];

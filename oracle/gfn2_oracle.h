@@ -32,6 +32,8 @@ typedef struct oracle_options {
    * symmetric], each NULL when absent (default) */
   const double* potential_shifts;
   const double* charge_response;
+  int32_t spin_channels;    /* 1 restricted (default), 2 unrestricted */
+  int32_t reserved;
 } oracle_options;
 
 /* Optional diagnostics for term-level pinning; every pointer may be NULL. */
@@ -96,7 +98,20 @@ int oracle_batch(int64_t batch, const int64_t* atom_offsets, const int32_t* Z, c
                  double* atomic_charges, double* pc_grad, int32_t* iterations, int32_t* status,
                  const double* efield /* [batch][3] uniform fields, or NULL */,
                  const double* shifts /* [total atoms] b, or NULL */,
-                 const double* response /* per system nat*nat A packed in system order, or NULL */);
+                 const double* response /* per system nat*nat A packed in system order, or NULL */,
+                 const int32_t* spin_channels /* [batch] 1 or 2, or NULL (all 1) */);
+
+/*
+ * One SCC step like oracle_scc_replay with 1 or 2 spin channels: mixed and
+ * raw in the reference's state packing ([qsh, dip, quad], each field holding
+ * its channels in turn); hamiltonian and density [nch][nao*nao] (alpha,
+ * beta), eigenvalues [nch][nao], occupations [2][nao] (alpha, beta).
+ */
+int oracle_scc_replay_channels(int32_t nat, const int32_t* Z, const double* xyz, double charge,
+                               int32_t unpaired, int32_t spin_channels, int32_t npc, const double* pc_xyz,
+                               const double* pc_q, const double* pc_gamma, double kt, const double* mixed,
+                               double* raw, double* hamiltonian, double* density, double* eigenvalues,
+                               double* occupations, double* free_energy);
 
 /*
  * Replay exactly one SCC step from a given mixed state [qsh, dip(3N), quad(6N)]

@@ -26,7 +26,8 @@ class Options(ctypes.Structure):
                 ("charge_tolerance", ctypes.c_double), ("energy_tolerance", ctypes.c_double),
                 ("kt", ctypes.c_double), ("damping", ctypes.c_double),
                 ("efield", ctypes.c_double * 3), ("potential_shifts", _c_double_p),
-                ("charge_response", _c_double_p)]
+                ("charge_response", _c_double_p), ("spin_channels", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class Trace(ctypes.Structure):
@@ -78,11 +79,15 @@ def lib():
             ctypes.c_int64, _c_int64_p, _c_int32_p, _c_double_p, _c_double_p, _c_int32_p,
             _c_int64_p, _c_double_p, _c_double_p, _c_double_p, ctypes.POINTER(Options),
             ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_int32_p,
-            _c_int32_p, _c_double_p, _c_double_p, _c_double_p]
+            _c_int32_p, _c_double_p, _c_double_p, _c_double_p, _c_int32_p]
         L.oracle_scc_replay.argtypes = [
             ctypes.c_int32, _c_int32_p, _c_double_p, ctypes.c_double, ctypes.c_int32,
             ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p, ctypes.c_double, _c_double_p,
             _c_double_p, _c_double_p, _c_double_p, _c_double_p]
+        L.oracle_scc_replay_channels.argtypes = [
+            ctypes.c_int32, _c_int32_p, _c_double_p, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+            ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p, ctypes.c_double, _c_double_p,
+            _c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_double_p, _c_double_p]
         L.oracle_broyden.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                      _c_double_p, ctypes.c_int32, _c_double_p, _c_double_p]
         L.oracle_basis_counts.argtypes = [ctypes.c_int32, _c_int32_p, _c_int32_p, _c_int32_p,
@@ -109,11 +114,12 @@ def _p(a, t=_c_double_p):
 
 def options(max_iterations=250, charge_tolerance=1e-6, energy_tolerance=1e-8,
             kt=300.0 * 3.166808578545117e-6, history=8, damping=0.4,
-            efield=(0.0, 0.0, 0.0), shifts=None, response=None) -> Options:
+            efield=(0.0, 0.0, 0.0), shifts=None, response=None, spin_channels=1) -> Options:
     """``shifts`` (b, per atom) and ``response`` (A, nat x nat symmetric) add
     the external charge operator v = b + A q, E = q.b + q.A.q/2."""
     o = Options(max_iterations, history, charge_tolerance, energy_tolerance, kt, damping,
                 (ctypes.c_double * 3)(*efield))
+    o.spin_channels = spin_channels
     keep = []
     if shifts is not None:
         keep.append(np.ascontiguousarray(shifts, dtype=np.float64))
@@ -234,9 +240,34 @@ def scc_replay(Z, xyz, mixed, charge=0.0, uhf=0, pc_xyz=None, pc_q=None, pc_gamm
     return raw, H, P, F.value
 
 
+def scc_replay_channels(Z, xyz, mixed, charge=0.0, uhf=0, spin_channels=1,
+                        kt=300.0 * 3.166808578545117e-6):
+    """One SCC step with 1 or 2 spin channels from a state in the reference's
+    packing ([qsh, dip, quad], each field's channels in turn); returns raw,
+    H [nch, nao, nao], P [nch, nao, nao] (alpha, beta; restricted: total),
+    eigenvalues [nch, nao], occupations [2, nao] and F."""
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    mixed = np.ascontiguousarray(mixed, dtype=np.float64)
+    _, nao, _ = basis_counts(Z)
+    nch = int(spin_channels)
+    raw = np.zeros_like(mixed)
+    H = np.zeros((nch, nao, nao))
+    P = np.zeros((nch, nao, nao))
+    eps = np.zeros((nch, nao))
+    occ = np.zeros((2, nao))
+    F = ctypes.c_double()
+    st = lib().oracle_scc_replay_channels(len(Z), _p(Z, _c_int32_p), _p(xyz), float(charge), int(uhf), nch,
+                                          0, None, None, None, float(kt), _p(mixed), _p(raw), _p(H), _p(P),
+                                          _p(eps), _p(occ), ctypes.byref(F))
+    if st != 0:
+        raise RuntimeError(f"oracle replay failed with status {st}")
+    return raw, H, P, eps, occ, F.value
+
+
 def batch(atom_offsets, Z, xyz, charges, uhf, point_offsets=None, pc_xyz=None, pc_q=None,
           pc_gamma=None, opts=None, threads=0, want_point_gradient=False, efield=None,
-          shifts=None, response=None):
+          shifts=None, response=None, spin_channels=None):
     atom_offsets = np.ascontiguousarray(atom_offsets, dtype=np.int64)
     B = len(atom_offsets) - 1
     Z = np.ascontiguousarray(Z, dtype=np.int32)
@@ -263,9 +294,10 @@ def batch(atom_offsets, Z, xyz, charges, uhf, point_offsets=None, pc_xyz=None, p
         shifts = np.ascontiguousarray(shifts, dtype=np.float64)
     if response is not None:  # A of each system, nat x nat, packed in system order
         response = np.ascontiguousarray(response, dtype=np.float64)
+    spin = None if spin_channels is None else np.ascontiguousarray(spin_channels, dtype=np.int32)
     lib().oracle_batch(B, _p(atom_offsets, _c_int64_p), _p(Z, _c_int32_p), _p(xyz), _p(charges),
                        _p(uhf, _c_int32_p), _p(point_offsets, _c_int64_p), _p(pc_xyz), _p(pc_q),
                        _p(pc_gamma), ctypes.byref(o), int(threads), _p(e), _p(g), _p(q), _p(pg),
                        _p(it, _c_int32_p), _p(st, _c_int32_p), _p(efield), _p(shifts),
-                       _p(response))
+                       _p(response), _p(spin, _c_int32_p))
     return dict(energies=e, gradient=g, charges=q, iterations=it, status=st, pc_gradient=pg)

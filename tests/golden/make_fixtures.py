@@ -121,6 +121,35 @@ def traces():
     return {"source": "data/conformance/scc-traces (reference)", "cases": out}
 
 
+def traces_unrestricted():
+    """The unrestricted (spin_channels = 2) traces of manifest-v2.json.  Fields
+    keep the reference's packing: every multipole field holds the charge then
+    the magnetization channel; H, P and eigenvalues hold alpha then beta.  H is
+    the solver Hamiltonian, H0 + V_q +/- V_m (tblite doubles its half-converted
+    assembly, so the assembled one is the same operator at half scale)."""
+    root = os.path.join(REF, "data/conformance/scc-traces")
+    index = json.load(open(os.path.join(root, "manifest-v2.json")))
+    out = []
+    for cid, meta in sorted(index["cases"].items()):
+        d = json.load(open(os.path.join(root, meta["path"])))
+        inp = d["input"]
+        item = {"id": cid, "Z": inp["atomic_numbers"], "xyz": flat(inp["positions"]),
+                "charge": inp["molecular_charge"], "unpaired": inp["unpaired_electrons"],
+                "spin_channels": inp["spin_channels"], "temperature_kelvin": inp["temperature"],
+                "overlap": flat(d["statics"]["overlap"][0]),
+                "h0": flat(d["statics"]["core_hamiltonian"][0]),
+                "terminal_iterations": d["terminal"]["iterations"], "iterations": []}
+        for it in d["iterations"]:
+            item["iterations"].append({
+                "mixed": flat(it["mixed_qsh"]) + flat(it["mixed_dipoles"]) + flat(it["mixed_quadrupoles"]),
+                "raw": flat(it["raw_qsh"]) + flat(it["raw_dipoles"]) + flat(it["raw_quadrupoles"]),
+                "energy": it["energy"], "hamiltonian": flat(it["solver_hamiltonian"]),
+                "density": flat(it["density"]), "eigenvalues": flat(it["eigenvalues"]),
+                "occupations": flat(it["occupations"]), "residual_rms": it["residual_rms"]})
+        out.append(item)
+    return {"source": "data/conformance/scc-traces/manifest-v2.json (reference)", "cases": out}
+
+
 def _cxx_rows(text, name):
     """Rows ``{...}`` (or ``Type{...}``) of ``name{{ ... }};`` as lists of numbers."""
     body = text[text.index(name + "{{") + len(name) + 2:]
@@ -202,6 +231,7 @@ def mixer():
 
 def main():
     for name, data in (("conformance.json", conformance()), ("scc_traces.json", traces()),
+                       ("scc_traces_unrestricted.json", traces_unrestricted()),
                        ("mixer_known_answer.json", mixer()),
                        ("gfn2_parameters_hcno.json", parameters())):
         path = os.path.join(OUT, name)

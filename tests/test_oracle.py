@@ -71,6 +71,72 @@ def test_scc_trace_replay(case):
     assert nao > 0
 
 
+UTRACES = load("scc_traces_unrestricted.json")
+
+
+@pytest.mark.parametrize("case", UTRACES["cases"], ids=lambda c: c["id"])
+def test_unrestricted_trace_replay(case):
+    """Every iteration of the reference's unrestricted SCC trace, replayed on
+    its own mixed input: both Hamiltonians, both densities, both eigenvalue
+    sets and occupations, the raw charge and magnetization multipoles, F."""
+    xyz = np.array(case["xyz"]).reshape(-1, 3)
+    kt = case["temperature_kelvin"] * 3.166808578545117e-6
+    for it in case["iterations"]:
+        raw, H, P, eps, occ, F = O.scc_replay_channels(case["Z"], xyz, it["mixed"], charge=case["charge"],
+                                                       uhf=case["unpaired"],
+                                                       spin_channels=case["spin_channels"], kt=kt)
+        assert abs(F - it["energy"]) < 1e-11
+        assert np.abs(H.ravel() - it["hamiltonian"]).max() < 1e-12
+        assert np.abs(P.ravel() - it["density"]).max() < 1e-11
+        assert np.abs(raw - it["raw"]).max() < 1e-11
+        assert np.abs(eps.ravel() - it["eigenvalues"]).max() < 1e-12
+        assert np.abs(occ.ravel() - it["occupations"]).max() < 1e-11
+
+
+@pytest.mark.parametrize("case", UTRACES["cases"], ids=lambda c: c["id"])
+def test_unrestricted_loop_follows_trace(case):
+    """From FRESH, the oracle's own loop reproduces the trace's first four
+    free energies and residual norms (the trace's later rounds follow the
+    generating program's mixer settings, as the restricted traces do): this
+    pins the two-channel state packing and its initial state."""
+    xyz = np.array(case["xyz"]).reshape(-1, 3)
+    r = O.singlepoint(case["Z"], xyz, charge=case["charge"], uhf=case["unpaired"],
+                      opts=O.options(spin_channels=2, charge_tolerance=1e-9, energy_tolerance=1e-11),
+                      trace=True)
+    assert r.status == 0
+    for k, it in enumerate(case["iterations"][:4]):
+        assert abs(r.trace["free_energy"][k] - it["energy"]) < 1e-12
+        assert abs(r.trace["residual_rms"][k] - it["residual_rms"]) < 1e-12
+
+
+def test_unrestricted_forces_and_invariances():
+    """Forces of the unrestricted OH radical against central differences (the
+    magnetization acts through the overlap derivative); no net force; a
+    closed shell with two channels stays unpolarized and equals the
+    restricted result."""
+    Z = [8, 1]
+    xyz = np.array([[0.0, 0.0, 0.0], [1.8, 0.1, -0.05]])
+    tight = O.options(spin_channels=2, charge_tolerance=1e-10, energy_tolerance=1e-12)
+    r = O.singlepoint(Z, xyz, uhf=1, opts=tight)
+    restricted = O.singlepoint(Z, xyz, uhf=1, opts=O.options(charge_tolerance=1e-10, energy_tolerance=1e-12))
+    assert r.status == 0 and r.energy < restricted.energy - 1e-4
+    h = 1e-4
+    for atom, axis in ((0, 0), (1, 0), (1, 1), (1, 2)):
+        xp = xyz.copy(); xp[atom, axis] += h
+        xm = xyz.copy(); xm[atom, axis] -= h
+        fd = (O.singlepoint(Z, xp, uhf=1, opts=tight).energy - O.singlepoint(Z, xm, uhf=1, opts=tight).energy) / (2 * h)
+        assert abs(fd - r.gradient[atom, axis]) < 2e-7
+    assert np.abs(r.gradient.sum(axis=0)).max() < 1e-10
+    from xtbloom_amd import workloads as W
+    s = W.water(np.random.default_rng(4))
+    two = O.singlepoint(s["Z"], s["xyz"], opts=O.options(spin_channels=2, charge_tolerance=1e-10,
+                                                         energy_tolerance=1e-12))
+    one = O.singlepoint(s["Z"], s["xyz"], opts=O.options(charge_tolerance=1e-10, energy_tolerance=1e-12))
+    assert abs(two.energy - one.energy) < 1e-10
+    assert np.abs(two.gradient - one.gradient).max() < 1e-9
+    assert np.abs(two.charges - one.charges).max() < 1e-9
+
+
 def test_mixer_known_answer():
     ka = load("mixer_known_answer.json")
     out = O.broyden(ka["initial"], ka["residuals"], memory=ka["memory"], damping=ka["damping"])
