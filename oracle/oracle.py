@@ -189,7 +189,8 @@ def singlepoint(Z, xyz, charge=0.0, uhf=0, pc_xyz=None, pc_q=None, pc_gamma=None
             "shell_charges", "dipoles", "quadrupoles", "coordination", "components")])
     o = opts or options()
     nsh, _, _ = basis_counts(Z)
-    state = np.zeros(nsh + 9 * nat)
+    # the state holds every spin channel of [qsh, dip(3N), quad(6N)]
+    state = np.zeros(int(o.spin_channels) * (nsh + 9 * nat))
     if warm_state is not None:
         warm_state = np.ascontiguousarray(warm_state, dtype=np.float64)
         if warm_state.shape != state.shape:
