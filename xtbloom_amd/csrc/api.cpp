@@ -70,6 +70,21 @@ struct CallIdentity {
   bool operator==(const CallIdentity& o) const {
     return same_plan(o) && field_on == o.field_on && field == o.field;
   }
+  // the first part of the identity that differs from o (for diagnostics)
+  const char* difference(const CallIdentity& o) const {
+    auto same = [](double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; };
+    if (atom_offsets != o.atom_offsets || Z != o.Z) return "atoms";
+    if (charges != o.charges || unpaired != o.unpaired) return "molecular charges or unpaired electrons";
+    if (point_offsets != o.point_offsets || pc_gamma != o.pc_gamma) return "point-charge layout";
+    if (spin != o.spin) return "spin channels";
+    if (charge_operator != o.charge_operator) return "external charge operator";
+    if (field_on != o.field_on || field != o.field) return "electric field";
+    if (flags != o.flags || model != o.model || determinism != o.determinism) return "flags, model or determinism";
+    if (max_iterations != o.max_iterations || history != o.history || !same(charge_tolerance, o.charge_tolerance) ||
+        !same(energy_tolerance, o.energy_tolerance) || !same(kt, o.kt) || !same(damping, o.damping))
+      return "SCC policy";
+    return "nothing";
+  }
 };
 
 // Converged SCC multipole state of the latest fully converged call, the
@@ -107,9 +122,6 @@ struct xtbloom_context {
   std::thread worker;
 };
 
-// The inputs the packed host topology depends on: atom offsets, element
-// numbers, molecular charges, unpaired electrons, point-charge offsets and
-// gammas.
 // Asynchronous submission owner (reference xtbloom.h:41-47, 836-878):
 // IDLE after creation, PENDING from an accepted enqueue until the context's
 // worker has run the call and published its outputs, then COMPLETE with the
@@ -126,9 +138,13 @@ struct xtbloom_request {
   std::condition_variable cv;
 };
 
+// The inputs the packed host topology depends on: atom offsets, element
+// numbers, molecular charges, unpaired electrons, point-charge offsets and
+// gammas, spin channels.
 bool same_topology(const CallIdentity& a, const CallIdentity& b) {
   return a.atom_offsets == b.atom_offsets && a.Z == b.Z && a.charges == b.charges &&
-         a.unpaired == b.unpaired && a.point_offsets == b.point_offsets && a.pc_gamma == b.pc_gamma;
+         a.unpaired == b.unpaired && a.point_offsets == b.point_offsets && a.pc_gamma == b.pc_gamma &&
+         a.spin == b.spin;
 }
 
 // Fixed-topology plan (reference xtbloom.h:939-991): the identity it was
@@ -673,10 +689,14 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
   if (batch->struct_size >= XTBLOOM_BATCH_V2_SIZE && buffer_active(batch->spin_channels)) {
     if ((s = stage_input(batch->spin_channels, sizeof(int32_t) * B, "spin_channels", st.spin, true))) return s;
     const int32_t* ch = reinterpret_cast<const int32_t*>(st.spin.data());
-    for (int64_t i = 0; i < B; ++i) {
-      if (ch[i] == 2) return fail(XTBLOOM_STATUS_NOT_SUPPORTED, "unrestricted spin channels are not supported by the ROCm backend");
-      if (ch[i] != 1) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "spin_channels must be 1 or 2");
-    }
+    for (int64_t i = 0; i < B; ++i)
+      if (ch[i] != 1 && ch[i] != 2) return fail(XTBLOOM_STATUS_INVALID_ARGUMENT, "spin_channels must be 1 or 2");
+  } else {
+    // an absent buffer means one restricted channel per system; the identity
+    // of a call therefore does not depend on whether the ones were given
+    st.spin.resize(sizeof(int32_t) * (size_t)B);
+    const int32_t one = 1;
+    for (int64_t i = 0; i < B; ++i) std::memcpy(st.spin.data() + sizeof(int32_t) * i, &one, sizeof(one));
   }
   if ((s = stage_interactions(batch, B, st))) return s;
   if ((s = check_lattice(batch, B))) return s;
@@ -745,6 +765,7 @@ xtbloom_status_t validate_and_stage(const xtbloom_batch_t* batch, const xtbloom_
     hb.pc_gamma = reinterpret_cast<const double*>(st.pg.data());
   }
   hb.field = st.field.empty() ? nullptr : st.field.data();
+  hb.spin = st.spin.empty() ? nullptr : reinterpret_cast<const int32_t*>(st.spin.data());
   hb.pshift = st.pshift.empty() ? nullptr : st.pshift.data();
   hb.presp = st.presp.empty() ? nullptr : st.presp.data();
   ro.forces = want_f || want_pf;
@@ -842,10 +863,15 @@ xtbloom_status_t admit(xtbloom_context_t* context, xtbloom_plan_t* plan, const x
                 "the batch topology or compute policy does not match the fixed-topology plan");
   WarmCheckpoint& cp = plan ? plan->warm : context->warm;
   if (job.warm) {
-    if (!cp.valid || !(cp.id == job.id))
+    if (!cp.valid)
       return fail(XTBLOOM_STATUS_INVALID_ARGUMENT,
                   "WARM SCC start requires a previous fully converged call with identical topology and "
-                  "compute policy on this context");
+                  "compute policy on this context (there is none)");
+    if (!(cp.id == job.id))
+      return fail(XTBLOOM_STATUS_INVALID_ARGUMENT,
+                  std::string("WARM SCC start requires a previous fully converged call with identical topology "
+                              "and compute policy on this context (the ") +
+                      job.id.difference(cp.id) + " differ)");
     job.warm_state.swap(cp.state);
   }
   cp.valid = false;

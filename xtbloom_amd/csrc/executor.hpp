@@ -27,6 +27,7 @@ struct HostBatch {
   // A (n_atoms^2 per system, row-major, packed in system order), each or null
   const double* pshift = nullptr;
   const double* presp = nullptr;
+  const int32_t* spin = nullptr;          // [B] spin channels (1 or 2), or null (all 1)
 };
 
 struct RunOptions {
@@ -134,7 +135,9 @@ int run_batch(Device* dev, const HostBatch& batch, const RunOptions& options, Ho
               RunTiming* timing, std::string& error, TopoCache* cache = nullptr);
 // Packed-matrix element count and AO count limit checks used by validation;
 // system_aos (optional, [B]) receives each system's AO count and
-// system_state (optional, [B]) the length of its SCC multipole state.
+// system_state (optional, [B]) the length of its SCC multipole state (two
+// channels of shell charges, dipoles and quadrupoles for an unrestricted
+// system).
 int64_t batch_matrix_elements(const HostBatch& batch, int32_t* max_ao, std::string& error,
                               int32_t* system_aos = nullptr, int32_t* system_state = nullptr);
 

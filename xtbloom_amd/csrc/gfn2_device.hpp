@@ -80,6 +80,14 @@ struct Topo {
   // (row-major n_atoms^2), each null when the call has none
   const double* pshift;
   const double* presp;
+  // unrestricted systems: each runs as two systems, its alpha channel (the
+  // batch member) and a beta copy appended after every member; partner[s] is
+  // the other one (-1 for a restricted system).  The alpha system's SCC
+  // state holds the charge block [qsh, dip, quad] and then the magnetization
+  // block.  sh_wspin[3 sh + k] couples shell sh to the k-th shell of its atom
+  // (spin constants W_ll').  Both null when the call has no such system.
+  const int* partner;
+  const double* sh_wspin;
   // global tables
   const double* d4_c6;      // [262*262]
   const double* d4_refcn;

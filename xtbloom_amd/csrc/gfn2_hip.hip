@@ -114,12 +114,19 @@ struct HostTopo {
   std::vector<double> pc_xyz, pc_q, pc_gam;
   std::vector<double> field;  // [3B] when the call carries a uniform field, else empty
   std::vector<double> pshift, presp;  // external charge operator b [N], A [apair_off[B]], or empty
+  // unrestricted systems (empty when there are none): the partner of each
+  // system (-1: restricted) and the spin couplings of each shell
+  std::vector<int> partner;
+  std::vector<double> sh_wspin;
   std::vector<int> setup_status;
   int max_ao = 0;
   int max_atoms = 0;
 };
 
-int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
+// partner (optional, [B]): the alpha system of an unrestricted member and
+// its beta copy name each other; the alpha system holds alpha electrons only
+// and a state of both channels, the copy beta electrons only.
+int build_topology(const HostBatch& hb, HostTopo& T, std::string& error, const int* partner) {
   const int B = (int)hb.batch;
   T.B = B;
   T.atom_off.assign(B + 1, 0);
@@ -239,7 +246,7 @@ int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
     T.max_ao = std::max(T.max_ao, nao);
     T.shell_off[s + 1] = T.shell_off[s] + nsh;
     T.ao_off[s + 1] = T.ao_off[s] + nao;
-    T.vec_off[s + 1] = T.vec_off[s] + nsh + 9 * N;
+    T.vec_off[s + 1] = T.vec_off[s] + (partner && partner[s] > s ? 2 : 1) * (nsh + 9 * N);
     T.mat_off[s + 1] = T.mat_off[s] + (int64_t)nao * nao;
     T.gam_off[s + 1] = T.gam_off[s] + (int64_t)nsh * nsh;
     T.apair_off[s + 1] = T.apair_off[s] + (int64_t)N * N;
@@ -257,10 +264,31 @@ int build_topology(const HostBatch& hb, HostTopo& T, std::string& error) {
       error = "alpha or beta electron count cannot be represented by the system orbital space";
       return 1;
     }
+    if (partner && partner[s] >= 0) {
+      if (partner[s] > s)
+        T.nbeta[s] = 0.0;
+      else
+        T.nalpha[s] = 0.0;
+    }
     T.pc_off[s + 1] = hb.point_offsets ? (int)hb.point_offsets[s + 1] : 0;
   }
   T.atom_off[B] = (int)hb.atom_offsets[B];
   T.atom_sh0.push_back((int)T.sh_atom.size());
+  if (partner) {
+    // spin constants of each shell with the shells of its atom
+    // (reference spin.cpp:200-263: W_ll' by element, symmetric in l, l')
+    T.partner.assign(partner, partner + B);
+    T.sh_wspin.assign(3 * T.sh_atom.size(), 0.0);
+    for (size_t sh = 0; sh < T.sh_atom.size(); ++sh) {
+      const int a = T.sh_atom[sh], z = T.Z[a];
+      for (int k = 0; T.atom_sh0[a] + k < T.atom_sh0[a + 1]; ++k) {
+        int l1 = T.sh_l[sh], l2 = T.sh_l[T.atom_sh0[a] + k];
+        if (l1 > l2) std::swap(l1, l2);
+        const int idx = l1 == 0 ? (l2 == 0 ? 0 : (l2 == 1 ? 1 : 3)) : (l1 == 1 ? (l2 == 1 ? 2 : 4) : 5);
+        T.sh_wspin[3 * sh + k] = xtbp_spin_w[6 * (z - 1) + idx];
+      }
+    }
+  }
   T.N = (int)T.Z.size();
   T.NS = (int)T.sh_atom.size();
   T.NAO = (int)T.ao_sh.size();
@@ -480,7 +508,8 @@ int64_t batch_matrix_elements(const HostBatch& hb, int32_t* max_ao, std::string&
       nsh += cnt;
     }
     if (system_state)
-      system_state[s] = (int32_t)(nsh + 9 * (hb.atom_offsets[s + 1] - hb.atom_offsets[s]));
+      system_state[s] = (int32_t)((hb.spin && hb.spin[s] == 2 ? 2 : 1) *
+                                  (nsh + 9 * (hb.atom_offsets[s + 1] - hb.atom_offsets[s])));
     mx = std::max<int32_t>(mx, (int32_t)nao);
     if (system_aos) system_aos[s] = (int32_t)nao;
     total += nao * nao;
@@ -523,8 +552,10 @@ __global__ void k_valid(int B, const SysState* st, unsigned char* valid) {
 
 }  // namespace
 
-int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutputs& out,
-              RunTiming* timing, std::string& error, TopoCache* cache) {
+// The whole pipeline over a batch whose unrestricted members are already
+// split into alpha systems and beta copies (partner, or null).
+static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutputs& out,
+                          RunTiming* timing, std::string& error, TopoCache* cache, const int* partner) {
   if (hipSetDevice(dev->id) != hipSuccess) {
     error = "hipSetDevice failed";
     return 6;
@@ -542,7 +573,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
       cache->valid = false;
       T = HostTopo();  // build_topology appends to empty vectors
     }
-    const int st = build_topology(hb, T, error);
+    const int st = build_topology(hb, T, error, partner);
     if (st != 0) return st;
     if (cache) {
       cache->valid = true;
@@ -562,6 +593,11 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     T.presp.assign(hb.presp, hb.presp + (size_t)T.apair_off[T.B]);
   else
     T.presp.clear();
+  // alpha systems of the unrestricted members (one block each in the spin kernels)
+  std::vector<int> alpha_sys;
+  for (size_t s = 0; s < T.partner.size(); ++s)
+    if (T.partner[s] > (int)s) alpha_sys.push_back((int)s);
+  const int U = (int)alpha_sys.size();
   const int B = T.B, N = T.N, NS = T.NS, NAO = T.NAO, NPC = T.NPC;
   const int64_t NN = T.mat_off[B];
   const int64_t NG = T.gam_off[B];
@@ -641,6 +677,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   addd(3 * (size_t)NPC); addd(NPC); addd(NPC);
   addd(3 * (size_t)B);  // uniform field per system
   addd(T.pshift.size()); addd(T.presp.size());  // charge operator b, A
+  addi(T.partner.size()); addi(alpha_sys.size()); addd(T.sh_wspin.size());  // unrestricted systems
   bytes += plan_bytes<TileTask>(full.size()) + plan_bytes<TileTask>(lower.size());
   bytes += plan_bytes<BackTask>(back.size());
   bytes += plan_bytes<FockTile>(chunks.size());
@@ -762,6 +799,9 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   double* d_field = cv.take<double>(3 * (size_t)B);
   double* d_pshift = cv.take<double>(T.pshift.size());
   double* d_presp = cv.take<double>(T.presp.size());
+  int* d_partner = cv.take<int>(T.partner.size());
+  int* d_alpha_sys = cv.take<int>(alpha_sys.size());
+  double* d_wspin = cv.take<double>(T.sh_wspin.size());
   TileTask* d_full = cv.take<TileTask>(full.size());
   TileTask* d_lower = cv.take<TileTask>(lower.size());
   BackTask* d_back = cv.take<BackTask>(back.size());
@@ -818,6 +858,9 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   HIP_TRY(up(d_field, T.field));
   HIP_TRY(up(d_pshift, T.pshift));
   HIP_TRY(up(d_presp, T.presp));
+  HIP_TRY(up_topo(d_partner, T.partner));
+  HIP_TRY(up_topo(d_alpha_sys, alpha_sys));
+  HIP_TRY(up_topo(d_wspin, T.sh_wspin));
   HIP_TRY(up_topo(d_full, full));
   HIP_TRY(up_topo(d_lower, lower));
   HIP_TRY(up_topo(d_back, back));
@@ -841,6 +884,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   t.field = T.field.empty() ? nullptr : d_field;
   t.pshift = T.pshift.empty() ? nullptr : d_pshift;
   t.presp = T.presp.empty() ? nullptr : d_presp;
+  t.partner = U > 0 ? d_partner : nullptr;
+  t.sh_wspin = U > 0 ? d_wspin : nullptr;
   t.d4_c6 = dev->d4_c6; t.d4_refcn = dev->d4_refcn; t.d4_refq = dev->d4_refq;
   t.d4_refng = dev->d4_refng; t.kpair = dev->kpair;
 
@@ -1126,6 +1171,9 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
   auto enqueue_round = [&](int it, int* cnt, const int* rptr, RoundProbes* rp) -> int {
     HIP_TRY(probe_begin(kKPotentials));
     launch_potentials(d_active);
+    if (U > 0)
+      hipLaunchKernelGGL(k_spin_potentials, dim3(U), dim3(256), 0, sm, t, d_alpha_sys, d_active, mx.x, d_vsh, d_vd,
+                         d_vqq);
     HIP_TRY(probe_end(kKPotentials, 0.0, 0.0));
     HIP_TRY(probe_begin(kKFock));
     hipLaunchKernelGGL(k_fock, dim3((unsigned)chunks.size()), dim3(256), 0, sm, t, d_chunks,
@@ -1188,6 +1236,8 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
                        d_chunks, d_active, d_part_off, dP, dS, dD, dQ, dH0, d_mpart);
     hipLaunchKernelGGL(k_mulliken_finish, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_part_off,
                        d_mpart, d_colsum, mx.raw, d_st);
+    if (U > 0)
+      hipLaunchKernelGGL(k_spin_combine, dim3(U), dim3(256), 0, sm, t, d_alpha_sys, d_active, mx.raw, d_st);
     HIP_TRY(probe_end(kKMulliken, 0.0, 0.0));
     if (rp) rp->mull = last_probe();
     HIP_TRY(probe_begin(kKEnergyMix));
@@ -1230,7 +1280,7 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     put(arena_base); put(bytes); put(B); put(N); put(NS); put(NAO); put(NPC);
     put(cache ? cache->generation : ~uint64_t(0)); put(mem); put(opt.damping);
     put(opt.charge_tolerance); put(opt.energy_tolerance); put(opt.kt); put(opt.max_iterations);
-    put(opt.split32); put(t.field); put(t.pshift); put(t.presp);
+    put(opt.split32); put(t.field); put(t.pshift); put(t.presp); put(U);
     const bool reuse = cache != nullptr && dev->graph_exec != nullptr && dev->graph_key == key;
     if (!reuse) {
       if (dev->graph_exec) (void)hipGraphExecDestroy(dev->graph_exec);
@@ -1341,6 +1391,9 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
       hipLaunchKernelGGL((k_gemm<false, true, true>), dim3(nlower), dim3(256), 0, sm, t, d_lower, dC,
                          dC, dW, d_fe, d_kocc, (const int*)nullptr, d_valid);
     HIP_TRY(probe_end(kKEnergyWeighted, work_occ(h_valid), 0.0));  // lower half, K = kocc
+    // unrestricted: total P and W with the magnetization term, beta copies out
+    if (U > 0)
+      hipLaunchKernelGGL(k_spin_fold, dim3(U), dim3(256), 0, sm, t, d_alpha_sys, d_valid, mx.x, dP, dW);
     if (NAP > 0) {
       HIP_TRY(probe_begin(kKForcePairs));
       for (int c = 0; c < 2; ++c) {
@@ -1471,6 +1524,135 @@ int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutpu
     }
   }
   error.clear();
+  return 0;
+}
+
+// Unrestricted members (spin channel count 2) run as an alpha system in
+// their own place and a beta copy appended after all members: the copy has
+// the member's atoms and geometry, no point charges, and zero field, b and A
+// (its potentials are the alpha system's, set by the spin kernels).  The
+// outputs of the members are the leading slices of the expanded run's.
+int run_batch(Device* dev, const HostBatch& hb, const RunOptions& opt, HostOutputs& out,
+              RunTiming* timing, std::string& error, TopoCache* cache) {
+  int64_t U = 0;
+  if (hb.spin)
+    for (int64_t s = 0; s < hb.batch; ++s) U += hb.spin[s] == 2;
+  if (U == 0) return run_batch_core(dev, hb, opt, out, timing, error, cache, nullptr);
+  const int64_t B = hb.batch, N = hb.atoms, B2 = B + U;
+  std::vector<int64_t> aoff(hb.atom_offsets, hb.atom_offsets + B + 1), poff;
+  std::vector<int32_t> Z(hb.Z, hb.Z + N), unpaired(hb.unpaired, hb.unpaired + B);
+  std::vector<double> xyz(hb.xyz, hb.xyz + 3 * N), chg(hb.charges, hb.charges + B), field, pshift, presp;
+  std::vector<int> partner((size_t)B2, -1);
+  if (hb.point_offsets) poff.assign(hb.point_offsets, hb.point_offsets + B + 1);
+  if (hb.field) field.assign(hb.field, hb.field + 3 * B);
+  if (hb.pshift) pshift.assign(hb.pshift, hb.pshift + N);
+  if (hb.presp) {
+    int64_t nresp = 0;
+    for (int64_t s = 0; s < B; ++s) nresp += (aoff[s + 1] - aoff[s]) * (aoff[s + 1] - aoff[s]);
+    presp.assign(hb.presp, hb.presp + nresp);
+  }
+  std::vector<int32_t> state_dim((size_t)B);
+  if (batch_matrix_elements(hb, nullptr, error, nullptr, state_dim.data()) < 0) return 1;
+  int64_t nstate = 0, beta_state = 0;
+  for (int64_t s = 0; s < B; ++s) {
+    nstate += state_dim[s];
+    if (hb.spin[s] != 2) continue;
+    const int64_t a0 = hb.atom_offsets[s], a1 = hb.atom_offsets[s + 1], n = a1 - a0;
+    const int b = (int)aoff.size() - 1;
+    partner[s] = b;
+    partner[b] = (int)s;
+    Z.insert(Z.end(), hb.Z + a0, hb.Z + a1);
+    xyz.insert(xyz.end(), hb.xyz + 3 * a0, hb.xyz + 3 * a1);
+    aoff.push_back(aoff.back() + n);
+    chg.push_back(hb.charges[s]);
+    unpaired.push_back(hb.unpaired[s]);
+    if (!poff.empty()) poff.push_back(poff.back());
+    if (hb.field) field.insert(field.end(), 3, 0.0);
+    if (hb.pshift) pshift.insert(pshift.end(), (size_t)n, 0.0);
+    if (hb.presp) presp.insert(presp.end(), (size_t)(n * n), 0.0);
+    beta_state += state_dim[s] / 2;
+  }
+  HostBatch e = hb;
+  e.batch = B2;
+  e.atoms = (int64_t)Z.size();
+  e.atom_offsets = aoff.data();
+  e.Z = Z.data();
+  e.xyz = xyz.data();
+  e.charges = chg.data();
+  e.unpaired = unpaired.data();
+  e.point_offsets = poff.empty() ? nullptr : poff.data();
+  e.field = hb.field ? field.data() : nullptr;
+  e.pshift = hb.pshift ? pshift.data() : nullptr;
+  e.presp = hb.presp ? presp.data() : nullptr;
+  e.spin = nullptr;
+  const int64_t N2 = e.atoms;
+  // a WARM state covers the members; the copies' slots are scratch
+  RunOptions eopt = opt;
+  std::vector<double> warm;
+  if (opt.warm_state) {
+    warm.assign(opt.warm_state, opt.warm_state + nstate);
+    warm.resize((size_t)(nstate + beta_state), 0.0);
+    eopt.warm_state = warm.data();
+  }
+  // expanded outputs; point-charge outputs are shared (the copies have none)
+  std::vector<int32_t> nao2((size_t)B2);
+  int64_t nn2 = 0, nao_total = 0;
+  {
+    std::string err;
+    nn2 = batch_matrix_elements(e, nullptr, err, nao2.data(), nullptr);
+    if (nn2 < 0) { error = err; return 1; }
+    for (int32_t v : nao2) nao_total += v;
+  }
+  std::vector<double> en, grad, q, dip, ov, h0, den, cn, comp, lev, occ, state;
+  std::vector<int32_t> iters, status;
+  std::vector<uint8_t> conv;
+  HostOutputs eo = out;
+  auto own = [](auto& vec, const void* want, int64_t count) {
+    if (want == nullptr) return (decltype(vec.data()))nullptr;
+    vec.assign((size_t)count, {});
+    return vec.data();
+  };
+  eo.energies = own(en, out.energies, B2);
+  eo.gradient = own(grad, out.gradient, 3 * N2);
+  eo.charges = own(q, out.charges, N2);
+  eo.dipoles = own(dip, out.dipoles, 3 * B2);
+  eo.iterations = own(iters, out.iterations, B2);
+  eo.converged = own(conv, out.converged, B2);
+  eo.status = own(status, out.status, B2);
+  eo.dbg_overlap = own(ov, out.dbg_overlap, nn2);
+  eo.dbg_h0 = own(h0, out.dbg_h0, nn2);
+  eo.dbg_density = own(den, out.dbg_density, nn2);
+  eo.dbg_cn = own(cn, out.dbg_cn, N2);
+  eo.dbg_components = own(comp, out.dbg_components, 8 * B2);
+  eo.dbg_levels = own(lev, out.dbg_levels, nao_total);
+  eo.dbg_occ = own(occ, out.dbg_occ, nao_total);
+  eo.scc_state = own(state, out.scc_state, nstate + beta_state);
+  const int rc = run_batch_core(dev, e, eopt, eo, timing, error, cache, partner.data());
+  if (rc != 0) return rc;
+  // the members' slices lead every packed output
+  int64_t nn = 0, nao_members = 0;
+  for (int64_t s = 0; s < B; ++s) {
+    nn += (int64_t)nao2[s] * nao2[s];
+    nao_members += nao2[s];
+  }
+  auto give = [](const auto& vec, auto* dst, int64_t count) {
+    if (dst) std::copy(vec.begin(), vec.begin() + count, dst);
+  };
+  give(en, out.energies, B);
+  give(grad, out.gradient, 3 * N);
+  give(q, out.charges, N);
+  give(dip, out.dipoles, 3 * B);
+  give(iters, out.iterations, B);
+  give(conv, out.converged, B);
+  give(status, out.status, B);
+  give(ov, out.dbg_overlap, nn);
+  give(h0, out.dbg_h0, nn);
+  give(den, out.dbg_density, nn);
+  give(cn, out.dbg_cn, N);
+  give(comp, out.dbg_components, 8 * B);
+  give(lev, out.dbg_levels, nao_members);
+  give(occ, out.dbg_occ, nao_members);
+  give(state, out.scc_state, nstate);
   return 0;
 }
 

@@ -22,6 +22,9 @@ __global__ __launch_bounds__(512) void k_energy_mix(Topo t, const int* order, un
                                                     const int* round_ptr) {
   const int s = order[blockIdx.x];
   if (!active[s]) return;
+  // the beta copy of an unrestricted system has no state of its own: its
+  // alpha system mixes both channels and sets its flags
+  if (t.partner && t.partner[s] >= 0 && t.partner[s] < s) return;
   const int a0 = t.atom_off[s], a1 = t.atom_off[s + 1], N = a1 - a0;
   if (STAGED != (N <= kPairStageAtoms)) return;
   const int s0 = t.shell_off[s], ns = t.shell_off[s + 1] - s0;
@@ -65,6 +68,18 @@ __global__ __launch_bounds__(512) void k_energy_mix(Topo t, const int* order, un
   if (t.field) {
     const double* E = t.field + 3 * s;
     for (int i = tid; i < 3 * N; i += nt) e[2] -= E[i % 3] * dip[i];
+  }
+  // spin polarization m.W.m / 2 of the raw magnetization (reference
+  // spin.cpp:135-172), kept with AES2 in e[3]
+  if (t.partner && t.partner[s] > s) {
+    const double* m = raw + ns + 9 * N;
+    for (int i = tid; i < ns; i += nt) {
+      const int sh = s0 + i, f = t.atom_sh0[t.sh_atom[sh]] - s0;
+      const int cnt = t.atom_sh0[t.sh_atom[sh] + 1] - s0 - f;
+      double v = 0.0;
+      for (int k = 0; k < cnt; ++k) v = fma(t.sh_wspin[3 * sh + k], m[f + k], v);
+      e[3] = fma(0.5 * m[i], v, e[3]);
+    }
   }
   // external charge operator q.b + q.(A q)/2 at the raw atomic charges
   // (reference periodic_embedding.cpp:352-434)
@@ -258,7 +273,114 @@ __global__ __launch_bounds__(512) void k_energy_mix(Topo t, const int* order, un
     if (!isfinite(F) || !isfinite(rms)) { z.status = 6; stop = true; }
     if (stop) active[s] = 0;
     else atomicAdd(round_ptr ? active_count + *round_ptr : active_count, 1);
+    // the beta copy follows its alpha system
+    if (t.partner && t.partner[s] >= 0) {
+      const int sb = t.partner[s];
+      if (stop) active[sb] = 0;
+      st[sb].converged = z.converged;
+      st[sb].iterations = z.iterations;
+      st[sb].residual_rms = z.residual_rms;  // the split-precision round choice
+    }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Unrestricted systems (reference scc_driver.cpp:2593-2640, 2699-2720,
+// spin.cpp:135-172, force.cpp:506-533).  One block per alpha system; the
+// list holds the alpha systems only.
+// ---------------------------------------------------------------------------
+
+// magnetization shell potential v_m = W_A m_A from the state's magnetization
+// block; the alpha Hamiltonian takes vsh + v_m, the beta copy vsh - v_m and
+// the same dipole and quadrupole potentials
+__global__ __launch_bounds__(256) void k_spin_potentials(Topo t, const int* alpha, const unsigned char* active,
+                                                         const double* xs, double* vsh, double* vd,
+                                                         double* vqq) {
+  const int s = alpha[blockIdx.x];
+  if (active && !active[s]) return;
+  const int sb = t.partner[s];
+  const int s0 = t.shell_off[s], ns = t.shell_off[s + 1] - s0;
+  const int a0 = t.atom_off[s], N = t.atom_off[s + 1] - a0;
+  const int b0 = t.atom_off[sb], sb0 = t.shell_off[sb];
+  const double* m = xs + t.vec_off[s] + ns + 9 * N;
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    const int sh = s0 + i, f = t.atom_sh0[t.sh_atom[sh]] - s0;
+    const int cnt = t.atom_sh0[t.sh_atom[sh] + 1] - s0 - f;
+    double v = 0.0;
+    for (int k = 0; k < cnt; ++k) v = fma(t.sh_wspin[3 * sh + k], m[f + k], v);
+    const double q = vsh[sh];
+    vsh[sh] = q + v;
+    vsh[sb0 + i] = q - v;
+  }
+  for (int i = threadIdx.x; i < 3 * N; i += blockDim.x) vd[3 * b0 + i] = vd[3 * a0 + i];
+  for (int i = threadIdx.x; i < 6 * N; i += blockDim.x) vqq[6 * b0 + i] = vqq[6 * a0 + i];
+}
+
+// after the Mulliken pass: the alpha and beta raw multipoles (each with the
+// reference occupations in its shell charges) become the charge block
+// a + b - n0 and the magnetization block a - b of the alpha state; the core
+// energy and the entropy add, and a beta failure fails the system
+__global__ __launch_bounds__(256) void k_spin_combine(Topo t, const int* alpha, const unsigned char* active,
+                                                      double* raw, SysState* st) {
+  const int s = alpha[blockIdx.x];
+  if (active && !active[s]) return;
+  const int sb = t.partner[s];
+  const int s0 = t.shell_off[s], ns = t.shell_off[s + 1] - s0;
+  const int N = t.atom_off[s + 1] - t.atom_off[s];
+  const int dim = ns + 9 * N;
+  double* ra = raw + t.vec_off[s];
+  const double* rb = raw + t.vec_off[sb];
+  for (int i = threadIdx.x; i < dim; i += blockDim.x) {
+    const double a = ra[i], b = rb[i];
+    const double ref = i < ns ? t.sh_refocc[s0 + i] : 0.0;
+    ra[i] = (a + b) - ref;
+    ra[dim + i] = a - b;
+  }
+  if (threadIdx.x == 0) {
+    st[s].e_core += st[sb].e_core;
+    st[s].entropy += st[sb].entropy;
+    if (st[sb].status != 0 && st[s].status == 0) st[s].status = st[sb].status;
+  }
+}
+
+// before the forces: P <- Pa + Pb and W <- Wa + Wb + (Pa - Pb)(v_m,m + v_m,n)/2
+// in the alpha system's matrices, so that the overlap-derivative term of the
+// force kernels carries the magnetization potential; the beta copy is then
+// left out of the forces
+__global__ __launch_bounds__(256) void k_spin_fold(Topo t, const int* alpha, unsigned char* valid,
+                                                   const double* xs, double* P, double* W) {
+  const int s = alpha[blockIdx.x];
+  const int sb = t.partner[s];
+  if (!valid[s]) {
+    if (threadIdx.x == 0) valid[sb] = 0;
+    return;
+  }
+  const int s0 = t.shell_off[s], ns = t.shell_off[s + 1] - s0;
+  const int N = t.atom_off[s + 1] - t.atom_off[s];
+  const int o = t.ao_off[s], n = t.ao_off[s + 1] - o;
+  __shared__ double vm[kMaxAO];
+  const double* m = xs + t.vec_off[s] + ns + 9 * N;
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    const int sh = s0 + i, f = t.atom_sh0[t.sh_atom[sh]] - s0;
+    const int cnt = t.atom_sh0[t.sh_atom[sh] + 1] - s0 - f;
+    double v = 0.0;
+    for (int k = 0; k < cnt; ++k) v = fma(t.sh_wspin[3 * sh + k], m[f + k], v);
+    vm[i] = v;
+  }
+  __syncthreads();
+  double* Pa = P + t.mat_off[s];
+  const double* Pb = P + t.mat_off[sb];
+  double* Wa = W + t.mat_off[s];
+  const double* Wb = W + t.mat_off[sb];
+  for (int64_t idx = threadIdx.x; idx < (int64_t)n * n; idx += blockDim.x) {
+    const int r = (int)(idx / n), c = (int)(idx % n);
+    const double pa = Pa[idx], pb = Pb[idx];
+    const double v = vm[t.ao_sh[o + r] - s0] + vm[t.ao_sh[o + c] - s0];
+    Pa[idx] = pa + pb;
+    Wa[idx] = (Wa[idx] + Wb[idx]) + 0.5 * (pa - pb) * v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) valid[sb] = 0;
 }
 
 // End of a device-driven SCC round: the next round's number.  active_count

@@ -112,7 +112,7 @@ bool is_contiguous(const std::vector<int64_t>& m) {
 // Slice-owned copies of the inputs and outputs of a non-contiguous lane.
 struct Gathered {
   std::vector<int64_t> atom_offsets, point_offsets;
-  std::vector<int32_t> Z, unpaired, iterations, status;
+  std::vector<int32_t> Z, unpaired, spin, iterations, status;
   std::vector<double> xyz, charges, pc_xyz, pc_q, pc_gamma, field, pshift, presp, warm;
   std::vector<double> energies, gradient, atom_charges, pc_gradient, pc_dedq, dipoles, scc_state,
       dbg_overlap, dbg_h0, dbg_density, dbg_cn, dbg_components, dbg_levels, dbg_occ;
@@ -225,6 +225,7 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
       b.charges = hb.charges + x.s0;
       b.unpaired = hb.unpaired + x.s0;
       b.field = at(hb.field, 3 * x.s0);
+      b.spin = at(hb.spin, x.s0);
       b.pshift = at(hb.pshift, a0);
       b.presp = at(hb.presp, resp_prefix[x.s0]);
       int64_t p0 = 0;
@@ -273,6 +274,7 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
         g.charges.push_back(hb.charges[s]);
         g.unpaired.push_back(hb.unpaired[s]);
         if (hb.field) g.field.insert(g.field.end(), hb.field + 3 * s, hb.field + 3 * s + 3);
+        if (hb.spin) g.spin.push_back(hb.spin[s]);
         if (hb.pshift) g.pshift.insert(g.pshift.end(), hb.pshift + a0, hb.pshift + a1);
         if (hb.presp) g.presp.insert(g.presp.end(), hb.presp + resp_prefix[s], hb.presp + resp_prefix[s + 1]);
         if (hb.point_offsets) {
@@ -297,6 +299,7 @@ int run_batch_lanes(const std::vector<Device*>& lanes, const HostBatch& hb, cons
       b.charges = g.charges.data();
       b.unpaired = g.unpaired.data();
       b.field = hb.field ? g.field.data() : nullptr;
+      b.spin = hb.spin ? g.spin.data() : nullptr;
       b.pshift = hb.pshift ? g.pshift.data() : nullptr;
       b.presp = hb.presp ? g.presp.data() : nullptr;
       if (hb.point_offsets) {

@@ -108,17 +108,24 @@ def _normalize_efield(efield):
 
 class Structure:
     """One molecule: atomic numbers, positions (bohr), charge, uhf, optional
-    point charges and an optional uniform electric field (atomic units)."""
+    point charges and an optional uniform electric field (atomic units).
+    ``spin_channels`` 1 (restricted) or 2 (unrestricted) defaults to 2 for an
+    open shell (uhf != 0), as in the reference (python/xtbloom/interface.py:
+    158-162, 350-371)."""
 
     def __init__(self, numbers, positions, charge: float = 0.0, uhf: int | None = None,
                  point_charges: PointCharge | None = None, efield=None,
-                 charge_response: ChargeResponse | None = None):
+                 charge_response: ChargeResponse | None = None, spin_channels: int | None = None):
         self._numbers = np.ascontiguousarray(numbers, dtype=np.int32)
         self._positions = np.ascontiguousarray(positions, dtype=float).reshape(-1, 3)
         if self._numbers.ndim != 1 or len(self._numbers) != len(self._positions):
             raise XTBloomValueError("numbers and positions must describe the same atoms")
         self._charge = float(charge)
         self._uhf = int(uhf or 0)
+        spin = (2 if self._uhf != 0 else 1) if spin_channels is None else int(spin_channels)
+        if spin not in (1, 2):
+            raise XTBloomValueError("spin_channels must be 1 (restricted) or 2 (unrestricted)")
+        self._spin_channels = spin
         self._point_charges = point_charges
         self._efield = _normalize_efield(efield)
         if charge_response is not None and charge_response.shifts.size != len(self._numbers):
@@ -131,6 +138,7 @@ class Structure:
     uhf = property(lambda self: self._uhf)
     point_charges = property(lambda self: self._point_charges)
     efield = property(lambda self: self._efield)
+    spin_channels = property(lambda self: self._spin_channels)
     charge_response = property(lambda self: self._charge_response)
 
     def update(self, positions=None, point_charges=None):
@@ -254,7 +262,7 @@ def _descriptors(lib, atom_offsets, numbers, positions, charges, uhf, point_offs
                  charge_tolerance, energy_tolerance, electronic_temperature, scc_mixer_history,
                  scc_mixer_damping, scc_start_mode, determinism=0, electric_field=None,
                  atomic_potential_shifts=None, charge_response_offsets=None,
-                 charge_response_matrix=None):
+                 charge_response_matrix=None, spin_channels=None):
     """Batch and compute-options descriptors over flat host arrays (kept alive by the caller)."""
     atom_offsets = np.ascontiguousarray(atom_offsets, dtype=np.int64)
     numbers = np.ascontiguousarray(numbers, dtype=np.int32)
@@ -294,6 +302,9 @@ def _descriptors(lib, atom_offsets, numbers, positions, charges, uhf, point_offs
                                                                 library.MEMORY_HOST, 0)
             batch.interaction_payload = _const(payload)
     operator = []
+    if spin_channels is not None:  # [B] 1 or 2 (absent: every system restricted)
+        operator.append(np.ascontiguousarray(spin_channels, dtype=np.int32))
+        batch.spin_channels = _const(operator[-1])
     if atomic_potential_shifts is not None:
         operator.append(np.ascontiguousarray(atomic_potential_shifts, dtype=np.float64))
         batch.atomic_potential_shifts = _const(operator[-1])
@@ -331,8 +342,8 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
                    extension=False, diagnostics=False, kernel_stats=False,
                    determinism=library.DETERMINISM_DEFAULT, options_hook=None,
                    no_alloc=False, wait=True, electric_field=None, atomic_potential_shifts=None,
-                   charge_response_offsets=None,
-                   charge_response_matrix=None) -> "ComputedBatch | PendingBatch":
+                   charge_response_offsets=None, charge_response_matrix=None,
+                   spin_channels=None) -> "ComputedBatch | PendingBatch":
     """One ``xtbloom_compute`` over flat ragged arrays (the ``xtbloom_batch_t`` layout).
 
     ``scc_start_mode=library.SCC_START_WARM`` starts from the converged state of
@@ -351,6 +362,7 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
     ABI-v3 interaction attachments.  ``atomic_potential_shifts`` (b, [N]),
     ``charge_response_offsets`` ([B + 1]) and ``charge_response_matrix`` (A,
     n_atoms^2 per system, packed) add the external charge operator.
+    ``spin_channels`` ([B], 1 or 2) selects unrestricted systems.
     """
     lib = library.load()
     batch, opts, keep, (B, N, P) = _descriptors(
@@ -358,7 +370,7 @@ def compute_arrays(context: Context, atom_offsets, numbers, positions, charges=N
         point_values, point_gammas, model, flags, max_scc_iterations, charge_tolerance,
         energy_tolerance, electronic_temperature, scc_mixer_history, scc_mixer_damping,
         scc_start_mode, determinism, electric_field, atomic_potential_shifts,
-        charge_response_offsets, charge_response_matrix)
+        charge_response_offsets, charge_response_matrix, spin_channels)
     if options_hook is not None:
         options_hook(opts)
     atom_offsets, point_offsets = keep[0], keep[5]
@@ -545,7 +557,7 @@ class Plan:
                  max_scc_iterations=250, charge_tolerance=1e-6, energy_tolerance=1e-8,
                  electronic_temperature=300.0, scc_mixer_history=8, scc_mixer_damping=0.4,
                  atomic_potential_shifts=None, charge_response_offsets=None,
-                 charge_response_matrix=None):
+                 charge_response_matrix=None, spin_channels=None):
         lib = library.load()
         # the operator's presence belongs to the plan; its values are inputs
         # of each call (defaulting to these)
@@ -560,9 +572,10 @@ class Plan:
         batch, opts, keep, _ = _descriptors(
             lib, atom_offsets, numbers, positions, charges, uhf, point_offsets, point_positions,
             point_values, point_gammas, scc_start_mode=library.SCC_START_FRESH, **self._policy,
-            **self._operator)
+            **self._operator, spin_channels=spin_channels)
         self._topology = dict(atom_offsets=keep[0], numbers=keep[1], charges=keep[3],
-                              uhf=keep[4], point_offsets=keep[5], point_gammas=keep[8])
+                              uhf=keep[4], point_offsets=keep[5], point_gammas=keep[8],
+                              spin_channels=spin_channels)
         self._flags = int(opts.flags)
         handle = ctypes.c_void_p()
         _check(lib.xtbloom_plan_create(context.handle, ctypes.byref(batch), ctypes.byref(opts),
@@ -836,6 +849,8 @@ def _flatten(structures: Sequence[Structure]):
                   point_gammas=np.concatenate([p.gammas for p in pcs]))
     if any(s.efield is not None for s in structures):
         kw["electric_field"] = [s.efield for s in structures]
+    if any(s.spin_channels == 2 for s in structures):
+        kw["spin_channels"] = np.array([s.spin_channels for s in structures], np.int32)
     if any(s.charge_response is not None for s in structures):
         # systems without an operator get zero b and A: the ABI takes one
         # batch-wide operator (reference python/xtbloom/interface.py:679-710)
@@ -904,8 +919,10 @@ class Calculator(Structure):
     def __init__(self, method: str, numbers, positions, charge: float = 0.0, uhf: int | None = None,
                  point_charges: PointCharge | None = None, *, backend: str | int = "hip",
                  device_id: int | None = None, warm_start: bool = False, efield=None,
-                 charge_response: ChargeResponse | None = None, **settings):
-        super().__init__(numbers, positions, charge, uhf, point_charges, efield, charge_response)
+                 charge_response: ChargeResponse | None = None, spin_channels: int | None = None,
+                 **settings):
+        super().__init__(numbers, positions, charge, uhf, point_charges, efield, charge_response,
+                         spin_channels)
         self._model = _resolve_method(method)
         self._method = method
         self._context = Context(backend, device_id)
