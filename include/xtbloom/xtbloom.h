@@ -363,10 +363,12 @@ XTBLOOM_API xtbloom_status_t xtbloom_compute(xtbloom_context_t* context,
                                              const xtbloom_compute_options_t* options,
                                              xtbloom_batch_result_t* result);
 
-/* --- asynchronous requests (reference xtbloom.h:850-878, 933-937).  This
- * backend settles every accepted submission inline: enqueue returns with the
- * request COMPLETE (status, error text and result flags in the request);
- * admission errors return directly and leave the request unchanged. --- */
+/* --- asynchronous requests (reference xtbloom.h:850-878, 892-937).  An
+ * admitted enqueue leaves the request PENDING and returns at once; the
+ * context's worker thread runs submissions in order and completes the request
+ * (status, error text and result flags in the request).  Host inputs are
+ * staged at admission; output buffers stay borrowed until COMPLETE.
+ * Admission errors return directly and leave the request unchanged. --- */
 XTBLOOM_API xtbloom_status_t xtbloom_request_create(xtbloom_context_t* context,
                                                     xtbloom_request_t** request);
 XTBLOOM_API xtbloom_status_t xtbloom_request_query(xtbloom_request_t* request,
@@ -401,7 +403,8 @@ XTBLOOM_API xtbloom_status_t xtbloom_plan_compute_enqueue(xtbloom_plan_t* plan,
                                                           const xtbloom_batch_result_t* result,
                                                           xtbloom_request_t* request);
 
-/* --- DLPack result owners (reference xtbloom.h:1126-1145); NOT_SUPPORTED here --- */
+/* --- result owners (reference xtbloom.h:1011-1145): reference-counted
+ * arenas in host or ROCm device memory, exported as DLPack 1.0 tensors. --- */
 XTBLOOM_API xtbloom_status_t
 xtbloom_result_owner_options_init(xtbloom_result_owner_options_t* options, size_t struct_size);
 XTBLOOM_API xtbloom_status_t xtbloom_result_owner_create(
