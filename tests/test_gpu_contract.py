@@ -14,8 +14,8 @@
   converges within the iteration limit is decided by that noise (the oracle
   converges at d = 1e-5 and not at d = 4.681e-6).  Only their status class
   is checked there; the water member is checked against the oracle in full.
-* Systems above the 1024-AO limit of this backend's eigensolver get per-system
-  status NOT_SUPPORTED with NaN slices; the rest of the batch is computed.
+* Systems above the 8192-AO limit of this backend get per-system status
+  NOT_SUPPORTED with NaN slices; the rest of the batch is computed.
 """
 import numpy as np
 import pytest
@@ -66,12 +66,12 @@ def test_overlap_conditioning_gate_matches_oracle():
 
 def test_system_above_ao_limit_is_reported_per_system():
     rng = np.random.default_rng(9)
-    big = workloads.alkane(171, rng)  # C171H344: 6 * 171 + 2 = 1028 AOs
-    assert O.basis_counts(big["Z"])[1] == 1028
+    big = workloads.alkane(1366, rng)  # C1366H2734: 6 * 1366 + 2 = 8198 AOs
+    assert O.basis_counts(big["Z"])[1] == 8198
     b = workloads.from_systems([workloads.water(rng), big, workloads.water(rng)])
     ctx = interface.Context("hip", 0)
     try:
-        print("case 1028-AO system between two waters", flush=True)
+        print("case 8198-AO system between two waters", flush=True)
         out = interface.compute_arrays(ctx, b.atom_offsets, b.atomic_numbers, b.positions,
                                        b.molecular_charges, b.unpaired_electrons)
     finally:

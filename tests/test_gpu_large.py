@@ -1,0 +1,149 @@
+"""Systems above 1024 AOs on the GPU.
+
+Systems of at most 1024 AOs keep their per-system work vectors in LDS.  Larger
+ones, up to 8192 AOs, run instantiations of the same kernels with these
+vectors in a global slice per system:
+- the conditioning estimate;
+- the blocked Householder reduction, whose symmetric product and panel dots
+  run over row chunks of 16 slots of 64 rows;
+- the eigensolver;
+- the back-transformation;
+- the spin fold;
+- the three-body force kernel.
+The reference CPU path has no size limit (LAPACK dsyevd per system,
+eigensolver.cpp:1221-1254).
+
+Checks:
+- a ragged batch against the oracle: C190H382 (1142 AOs), an unrestricted
+  C190H381 radical (1141 AOs), and members on the blocked (362 AOs) and
+  register (122 AOs) paths;
+- C350H702 (2102 AOs, 33 slots: three row chunks) against the committed
+  oracle fixture;
+- central differences of the GPU's own energies at 1142 AOs;
+- the plain and graph-captured paths agree, and reproducible mode repeats
+  bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from xtbloom_amd import interface, library, workloads
+
+pytestmark = pytest.mark.gpu
+
+E_TOL, F_TOL, Q_TOL = 1e-7, 1e-6, 1e-6
+TIGHT = dict(charge_tolerance=1e-10, energy_tolerance=1e-12, max_scc_iterations=500)
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "large_c350.npz")
+
+
+def _radical(k, rng):
+    s = workloads.alkane(k, rng)
+    return dict(Z=s["Z"][:-1], xyz=s["xyz"][:-1], charge=0.0, uhf=1)
+
+
+def _members():
+    rng = np.random.default_rng(190)
+    members = [workloads.water(rng), workloads.alkane(190, rng), _radical(190, rng),
+               workloads.alkane(60, rng, substituent="NH2"), workloads.alkane(20, rng)]
+    spin = np.array([1, 1, 2, 1, 1], np.int32)
+    return members, spin
+
+
+def _run(ctx, b, spin=None, **kw):
+    return interface.compute_arrays(ctx, b.atom_offsets, b.atomic_numbers, b.positions,
+                                    b.molecular_charges, b.unpaired_electrons, spin_channels=spin, **kw)
+
+
+def test_ragged_batch_against_oracle():
+    members, spin = _members()
+    b = workloads.from_systems(members)
+    naos = [O.basis_counts(m["Z"])[1] for m in members]
+    assert naos[1] == 1142 and naos[2] == 1141 and 256 < naos[3] <= 1024 and naos[4] <= 256
+    ctx = interface.Context("hip", 0)
+    try:
+        print("case large ragged batch, AOs", naos, flush=True)
+        out = _run(ctx, b, spin)
+    finally:
+        ctx.close()
+    ref = O.batch(b.atom_offsets, b.atomic_numbers, b.positions, b.molecular_charges,
+                  b.unpaired_electrons, spin_channels=spin, threads=4)
+    assert np.all(out.per_system_status == 0) and np.all(ref["status"] == 0)
+    dE = np.abs(out.energies - ref["energies"])
+    dF = np.abs(out.forces + ref["gradient"]).max()
+    dq = np.abs(out.charges - ref["charges"]).max()
+    print(f"large batch vs oracle: dE {dE.max():.2e} dF {dF:.2e} dq {dq:.2e} iterations "
+          f"{out.scc_iterations.tolist()} oracle {ref['iterations'].tolist()}", flush=True)
+    assert dE.max() <= E_TOL and dF <= F_TOL and dq <= Q_TOL
+    assert np.abs(out.scc_iterations - ref["iterations"]).max() <= 1
+
+
+def test_2102_aos_against_fixture():
+    fx = np.load(FIXTURE)
+    s = dict(Z=fx["Z"], xyz=fx["xyz"], charge=0.0, uhf=0)
+    assert O.basis_counts(s["Z"])[1] == 2102
+    b = workloads.from_systems([s, workloads.water(np.random.default_rng(1))])
+    ctx = interface.Context("hip", 0)
+    try:
+        print("case C350H702 (2102 AOs)", flush=True)
+        out = _run(ctx, b)
+    finally:
+        ctx.close()
+    a1 = b.atom_offsets[1]
+    assert out.per_system_status.tolist() == [0, 0]
+    dE = abs(out.energies[0] - float(fx["energy"]))
+    dF = np.abs(out.forces[:a1] + fx["gradient"]).max()
+    dq = np.abs(out.charges[:a1] - fx["charges"]).max()
+    print(f"2102 AOs vs oracle fixture: dE {dE:.2e} dF {dF:.2e} dq {dq:.2e} iterations "
+          f"{int(out.scc_iterations[0])} oracle {int(fx['iterations'])}", flush=True)
+    assert dE <= E_TOL and dF <= F_TOL and dq <= Q_TOL
+    assert abs(int(out.scc_iterations[0]) - int(fx["iterations"])) <= 1
+    assert np.abs(out.forces[:a1].sum(axis=0)).max() < 1e-8  # translation invariance
+
+
+def test_forces_central_difference():
+    s = workloads.alkane(190, np.random.default_rng(3))
+    ctx = interface.Context("hip", 0)
+    try:
+        print("case 1142 AOs: forces vs central differences", flush=True)
+        base = _run(ctx, workloads.from_systems([s]), **TIGHT)
+        assert base.per_system_status[0] == 0
+        h = 1e-4
+        for atom, axis in ((0, 0), (200, 1), (len(s["Z"]) - 1, 2)):
+            e = []
+            for sign in (1, -1):
+                xyz = s["xyz"].copy()
+                xyz[atom, axis] += sign * h
+                e.append(_run(ctx, workloads.from_systems([dict(s, xyz=xyz)]), **TIGHT).energies[0])
+            fd = -(e[0] - e[1]) / (2 * h)
+            print(f"atom {atom} axis {axis}: analytic {base.forces[atom, axis]:.9f} fd {fd:.9f}", flush=True)
+            assert abs(fd - base.forces[atom, axis]) < 2e-7
+    finally:
+        ctx.close()
+
+
+def test_paths_and_reproducible_mode(monkeypatch):
+    members, spin = _members()
+    b = workloads.from_systems(members[1:3] + members[:1])
+    spin = np.array([1, 2, 1], np.int32)
+    ctx = interface.Context("hip", 0)
+    try:
+        print("case large members: plain and batched paths, reproducible mode", flush=True)
+        monkeypatch.setenv("XTBLOOM_AMD_PATH", "plain")
+        plain = _run(ctx, b, spin)
+        monkeypatch.setenv("XTBLOOM_AMD_PATH", "batched")
+        batched = _run(ctx, b, spin)
+        r = library.DETERMINISM_REPRODUCIBLE
+        first = _run(ctx, b, spin, determinism=r)
+        second = _run(ctx, b, spin, determinism=r)
+    finally:
+        ctx.close()
+    assert np.all(plain.per_system_status == 0)
+    assert np.array_equal(batched.per_system_status, plain.per_system_status)
+    assert np.abs(batched.energies - plain.energies).max() <= E_TOL
+    assert np.abs(batched.forces - plain.forces).max() <= F_TOL
+    for name in ("energies", "forces", "charges", "scc_iterations"):
+        assert np.array_equal(getattr(first, name), getattr(second, name)), name
+    assert np.abs(first.energies - plain.energies).max() <= 1e-9
+    assert np.abs(first.forces - plain.forces).max() <= 1e-9

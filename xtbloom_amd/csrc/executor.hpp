@@ -62,9 +62,11 @@ struct RunOptions {
   bool no_alloc = false;
 };
 
-// Largest AO count of one system (the eigensolver's LDS arrays); larger
-// systems of a batch get per-system status NOT_SUPPORTED.
-constexpr int kMaxSystemAOs = 1024;
+// Largest AO count of one system; larger systems of a batch get per-system
+// status NOT_SUPPORTED.  (Systems above 1024 AOs run kernel instantiations
+// whose per-system work vectors live in global memory instead of LDS; a
+// system of 8192 AOs holds about 20 n^2 doubles = 10.7 GB of device arena.)
+constexpr int kMaxSystemAOs = 8192;
 
 // Size of the per-GPU device workspace array behind RunOptions::no_alloc.
 constexpr size_t kNoAllocWorkspaceBytes = size_t(64) << 20;

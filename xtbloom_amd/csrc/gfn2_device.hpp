@@ -11,7 +11,17 @@
 
 namespace xtb {
 
-constexpr int kMaxAO = 1024;      // largest per-system AO count the eigensolver accepts
+// Systems of at most kMaxAO AOs keep their per-system work vectors (the
+// eigensolver's tridiagonal and eigenvalue arrays, the reduction's partial
+// sums, the conditioning estimate's vectors, per-atom force accumulators) in
+// LDS.  Larger systems, up to kMaxSystemAO, run instantiations of the same
+// kernels that keep these vectors in a global slice of kBigPerAO doubles per
+// AO plus kBigPerSys per system, at big_offset(); the host launches them over
+// the prefix of the size order above kMaxAO.
+constexpr int kMaxAO = 1024;
+constexpr int kMaxSystemAO = 8192;
+constexpr int kBigPerAO = 12;
+constexpr int kBigPerSys = 1024;
 constexpr int kD4MaxRef = 7;
 constexpr double kPi = 3.141592653589793238462643383279502884;
 
@@ -95,6 +105,12 @@ struct Topo {
   const int* d4_refng;
   const double* kpair;      // [9]
 };
+
+// First double of system s's slice of the global work vectors (systems above
+// kMaxAO AOs only; the slice holds kBigPerAO n + kBigPerSys doubles)
+__device__ __forceinline__ int64_t big_offset(const Topo& t, int s) {
+  return (int64_t)kBigPerAO * t.ao_off[s] + (int64_t)kBigPerSys * s;
+}
 
 // Per-system scalar state of the SCC driver (device resident).
 struct SysState {

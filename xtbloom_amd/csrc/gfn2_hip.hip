@@ -26,7 +26,7 @@
 #include "gfn2_mix_force.hpp"
 #include "gfn2_scc.hpp"
 
-static_assert(xtb::kMaxSystemAOs == xtb::kMaxAO, "AO limit of the executor and the eigensolver");
+static_assert(xtb::kMaxSystemAOs == xtb::kMaxSystemAO, "AO limit of the executor and the kernels");
 
 namespace xtb {
 
@@ -239,8 +239,8 @@ int build_topology(const HostBatch& hb, HostTopo& T, std::string& error, const i
       error = "every batch member needs at least one atom";
       return 1;
     }
-    if (nao > kMaxAO) {
-      error = "system exceeds the 1024-AO limit of the MI355X eigensolver";
+    if (nao > kMaxSystemAO) {
+      error = "system exceeds the 8192-AO limit of the MI355X executor";
       return 3;
     }
     T.max_ao = std::max(T.max_ao, nao);
@@ -456,7 +456,7 @@ Device* device_create(int device_id, void* stream, std::string& error) {
   int lds = 0;
   if (ok && hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device_id) == hipSuccess &&
       lds > 65536 &&
-      hipFuncSetAttribute((const void*)k_backtransform, hipFuncAttributeMaxDynamicSharedMemorySize,
+      hipFuncSetAttribute((const void*)k_backtransform<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           lds - (int)kBackStaticLds) == hipSuccess)
     d->lds_limit = (size_t)lds;
   if (!ok) {
@@ -594,10 +594,14 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
   else
     T.presp.clear();
   // alpha systems of the unrestricted members (one block each in the spin kernels)
+  // (those above kMaxAO AOs first)
   std::vector<int> alpha_sys;
   for (size_t s = 0; s < T.partner.size(); ++s)
     if (T.partner[s] > (int)s) alpha_sys.push_back((int)s);
+  const auto xl_end = std::stable_partition(alpha_sys.begin(), alpha_sys.end(),
+                                            [&](int s) { return T.ao_off[s + 1] - T.ao_off[s] > kMaxAO; });
   const int U = (int)alpha_sys.size();
+  const int U_xl = (int)(xl_end - alpha_sys.begin());
   const int B = T.B, N = T.N, NS = T.NS, NAO = T.NAO, NPC = T.NPC;
   const int64_t NN = T.mat_off[B];
   const int64_t NG = T.gam_off[B];
@@ -613,23 +617,27 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
   std::vector<int64_t> part_off(B + 1, 0);
   size_t back_lds = 0;
   int n_large = 0;  // systems above 256 AOs (a prefix of the size order)
+  int n_xl = 0;     // systems above kMaxAO AOs (a prefix of that prefix)
+  size_t n_back_xl = 0;  // their back-transformation tasks (a prefix of `back`)
   for (int i = 0; i < B; ++i) {
     const int s = T.order[i];
     const int n = T.ao_off[s + 1] - T.ao_off[s];
     n_large += n > 256;
+    n_xl += n > kMaxAO;
     const int nt = (n + kTile - 1) / kTile;
     for (int a = 0; a < nt; ++a)
       for (int b = 0; b < nt; ++b) {
         full.push_back({s, a, b});
         if (a >= b) lower.push_back({s, a, b});
       }
-    int cw = 64;  // register path (n <= 256): 64 columns per block, no dynamic LDS
-    if (n > 256) {
+    int cw = 64;  // register path (n <= 256) and global path (n > kMaxAO): no dynamic LDS
+    if (n > 256 && n <= kMaxAO) {
       const size_t budget = dev->lds_limit - kBackStaticLds;
       while (cw > 1 && (size_t)(n * cw + 2 * n + kBackThreads) * sizeof(double) > budget) cw /= 2;
       back_lds = std::max(back_lds, (size_t)(n * cw + 2 * n + kBackThreads) * sizeof(double));
     }
     for (int j0 = 0; j0 < n; j0 += cw) back.push_back({s, j0, cw, 0});
+    if (n > kMaxAO) n_back_xl = back.size();
     for (int r0 = 0; r0 < n; r0 += kFockTile)
       for (int c0 = r0; c0 < n; c0 += kFockTile) chunks.push_back({s, r0, c0});
   }
@@ -719,6 +727,9 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
   addi(N);                     // element-grouped atom order
   addd((size_t)kPairStageDoubles * N); addi((size_t)kPairStageInts * N);  // pair staging (large systems)
   addd((size_t)mem * mem * B);  // Broyden overlaps
+  // work vectors of the systems above kMaxAO AOs (big_offset)
+  const size_t xl_doubles = n_xl > 0 ? (size_t)kBigPerAO * NAO + (size_t)kBigPerSys * B : 0;
+  if (xl_doubles > 0) addd(xl_doubles);
   if (opt.no_alloc) {
     if (bytes > kNoAllocWorkspaceBytes) {
       error = "the batch needs " + std::to_string(bytes) +
@@ -964,6 +975,7 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
   double* d_gstage = cv.take<double>((size_t)kPairStageDoubles * N);
   int* d_gstagei = cv.take<int>((size_t)kPairStageInts * N);
   mx.ovl = cv.take<double>((size_t)mem * mem * B);
+  double* d_xl = n_xl > 0 ? cv.take<double>(xl_doubles) : nullptr;
   HIP_TRY(up_topo(d_perm, T.perm));
   t.perm = d_perm;
   const bool big_systems = T.max_atoms > kPairStageAtoms;
@@ -1104,7 +1116,12 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
   }
   HIP_TRY(probe_begin(kKCholesky));
   hipLaunchKernelGGL(k_cholesky_inverse, dim3(B), dim3(512), 0, sm, t, d_order, dS, dA, dX, d_st);
-  hipLaunchKernelGGL(k_overlap_rcond, dim3(B), dim3(kRcondThreads), 0, sm, t, d_order, dS, dX, d_st);
+  if (n_xl > 0)
+    hipLaunchKernelGGL(k_overlap_rcond<true>, dim3(n_xl), dim3(kRcondThreads), 0, sm, t, d_order, dS, dX, d_st,
+                       d_xl);
+  if (n_xl < B)
+    hipLaunchKernelGGL(k_overlap_rcond<false>, dim3(B - n_xl), dim3(kRcondThreads), 0, sm, t, d_order + n_xl,
+                       dS, dX, d_st, d_xl);
   HIP_TRY(probe_end(kKCholesky, all3 / 3.0, 0.0));
   hipLaunchKernelGGL(k_init_active, dim3((B + 255) / 256), dim3(256), 0, sm, B, d_st, d_active,
                      (int*)nullptr, d_st);
@@ -1196,10 +1213,14 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
     // eigenpairs of H' (A is overwritten by the reflectors; T, H, C, P, W serve as scratch)
     HIP_TRY(probe_begin(kKTridiag));
     // systems are ordered largest first: the n_large above 256 AOs take the
-    // blocked reduction, the rest the register one
-    if (n_large > 0)
-      hipLaunchKernelGGL(k_tridiag_blocked, dim3(n_large), dim3(512), 0, sm, t, d_order, dA, d_td,
-                         d_te, dT, d_active);
+    // blocked reduction (its first n_xl, above kMaxAO, with global work
+    // vectors), the rest the register one
+    if (n_xl > 0)
+      hipLaunchKernelGGL(k_tridiag_blocked<true>, dim3(n_xl), dim3(512), 0, sm, t, d_order, dA, d_td,
+                         d_te, dT, d_active, d_xl);
+    if (n_large > n_xl)
+      hipLaunchKernelGGL(k_tridiag_blocked<false>, dim3(n_large - n_xl), dim3(512), 0, sm, t, d_order + n_xl,
+                         dA, d_td, d_te, dT, d_active, d_xl);
     if (n_large < B)
       hipLaunchKernelGGL(k_tridiag, dim3(B - n_large), dim3(512), 0, sm, t, d_order + n_large, dA,
                          d_td, d_te, d_wyT, d_active);
@@ -1207,13 +1228,22 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
     if (rp) rp->tri = last_probe();
     HIP_TRY(probe_begin(kKTridiagEigen));
     // eigenvalues, occupations (kocc), eigenvectors of the kocc lowest levels
-    hipLaunchKernelGGL(k_tridiag_eigen, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_td, d_te,
-                       dZ, EigenScratch{dT, dH, dC, dP, dW}, d_eps, d_shift, it, rptr, d_zscale,
-                       FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt});
+    if (n_xl > 0)
+      hipLaunchKernelGGL(k_tridiag_eigen<true>, dim3(n_xl), dim3(256), 0, sm, t, d_order, d_active, d_td,
+                         d_te, dZ, EigenScratch{dT, dH, dC, dP, dW}, d_eps, d_shift, it, rptr, d_zscale,
+                         FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt}, d_xl);
+    if (n_xl < B)
+      hipLaunchKernelGGL(k_tridiag_eigen<false>, dim3(B - n_xl), dim3(256), 0, sm, t, d_order + n_xl,
+                         d_active, d_td, d_te, dZ, EigenScratch{dT, dH, dC, dP, dW}, d_eps, d_shift, it,
+                         rptr, d_zscale, FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt}, d_xl);
     HIP_TRY(probe_end(kKTridiagEigen, 0.0, 0.0));
     HIP_TRY(probe_begin(kKBacktransform));
-    hipLaunchKernelGGL(k_backtransform, dim3((unsigned)back.size()), dim3(kBackThreads), back_lds, sm, t, d_back,
-                       dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
+    if (n_back_xl > 0)
+      hipLaunchKernelGGL(k_backtransform<true>, dim3((unsigned)n_back_xl), dim3(kBackThreads), 0, sm, t,
+                         d_back, dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
+    if (back.size() > n_back_xl)
+      hipLaunchKernelGGL(k_backtransform<false>, dim3((unsigned)(back.size() - n_back_xl)), dim3(kBackThreads),
+                         back_lds, sm, t, d_back + n_back_xl, dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
     HIP_TRY(probe_end(kKBacktransform, 0.0, 0.0));
     if (rp) rp->bt = last_probe();
     // C = X Y (the kocc leading columns), P = C diag(f) C^T
@@ -1392,8 +1422,12 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
                          dC, dW, d_fe, d_kocc, (const int*)nullptr, d_valid);
     HIP_TRY(probe_end(kKEnergyWeighted, work_occ(h_valid), 0.0));  // lower half, K = kocc
     // unrestricted: total P and W with the magnetization term, beta copies out
-    if (U > 0)
-      hipLaunchKernelGGL(k_spin_fold, dim3(U), dim3(256), 0, sm, t, d_alpha_sys, d_valid, mx.x, dP, dW);
+    if (U_xl > 0)
+      hipLaunchKernelGGL(k_spin_fold<true>, dim3(U_xl), dim3(256), 0, sm, t, d_alpha_sys, d_valid, mx.x, dP,
+                         dW, d_xl);
+    if (U > U_xl)
+      hipLaunchKernelGGL(k_spin_fold<false>, dim3(U - U_xl), dim3(256), 0, sm, t, d_alpha_sys + U_xl, d_valid,
+                         mx.x, dP, dW, d_xl);
     if (NAP > 0) {
       HIP_TRY(probe_begin(kKForcePairs));
       for (int c = 0; c < 2; ++c) {
@@ -1428,12 +1462,21 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
                      d_erep, d_pcgrad, d_dedq);
   HIP_TRY(probe_end(kKForceAtoms, 0.0, 0.0));
   HIP_TRY(probe_begin(kKForceAtm));
-  if (opt.reproducible)
-    hipLaunchKernelGGL(k_force_atm<true>, dim3(B), dim3(512), 0, sm, t, d_order, d_valid, d_cnd4, d_w4,
-                       d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm);
-  else
-    hipLaunchKernelGGL(k_force_atm<false>, dim3(B), dim3(512), 0, sm, t, d_order, d_valid, d_cnd4, d_w4,
-                       d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm);
+  if (opt.reproducible) {
+    if (n_xl > 0)
+      hipLaunchKernelGGL((k_force_atm<true, true>), dim3(n_xl), dim3(512), 0, sm, t, d_order, d_valid, d_cnd4,
+                         d_w4, d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm, d_xl);
+    if (n_xl < B)
+      hipLaunchKernelGGL((k_force_atm<true, false>), dim3(B - n_xl), dim3(512), 0, sm, t, d_order + n_xl,
+                         d_valid, d_cnd4, d_w4, d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm, d_xl);
+  } else {
+    if (n_xl > 0)
+      hipLaunchKernelGGL((k_force_atm<false, true>), dim3(n_xl), dim3(512), 0, sm, t, d_order, d_valid, d_cnd4,
+                         d_w4, d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm, d_xl);
+    if (n_xl < B)
+      hipLaunchKernelGGL((k_force_atm<false, false>), dim3(B - n_xl), dim3(512), 0, sm, t, d_order + n_xl,
+                         d_valid, d_cnd4, d_w4, d_dw4, d_c6pair, d_dedcn4, d_grad, d_eatm, d_xl);
+  }
   HIP_TRY(probe_end(kKForceAtm, 0.0, 0.0));
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(dev->ev[3], sm));

@@ -476,7 +476,7 @@ template <int NS>
 __device__ __forceinline__ void symv_consume(int c, int m, int r0, const double* __restrict__ x,
                                              const double (&xr)[NS], const double (&f0)[NS],
                                              const double (&f1)[NS], double (&acc)[NS],
-                                             double* __restrict__ pcol) {
+                                             double* __restrict__ pcol, bool add) {
   const double xc0 = c < m ? x[c] : 0.0;
   const double xc1 = c + 1 < m ? x[c + 1] : 0.0;
   // slot 0: strictly lower rows, and the diagonal on the lane that holds it
@@ -493,17 +493,21 @@ __device__ __forceinline__ void symv_consume(int c, int m, int r0, const double*
   }
   wave_sum_pair(d0, d1, d0, d1);
   if ((threadIdx.x & 63) == 0) {
-    if (c < m) pcol[c] = d0;
-    if (c + 1 < m) pcol[c + 1] = d1;
+    if (c < m) pcol[c] = add ? pcol[c] + d0 : d0;
+    if (c + 1 < m) pcol[c + 1] = add ? pcol[c + 1] + d1 : d1;
   }
 }
 
-template <int NS>
-__device__ __forceinline__ void symv_unit(const double* __restrict__ blk, int64_t n, int m, int cu,
+// CH: the unit's rows start at slot tb instead of cu / 64 (row chunks of a
+// trailing block above 16 slots); chunks after the first add their column
+// dots to pcol, and their rows lie below every column of the unit.
+template <int NS, bool CH = false>
+__device__ __forceinline__ void symv_unit(const double* __restrict__ blk, int64_t n, int m, int cu, int tb,
                                           const double* __restrict__ x, double* __restrict__ prow,
                                           double* __restrict__ pcol) {
   const int lane = threadIdx.x & 63;
-  const int t0 = cu >> 6;
+  const int t0 = CH ? tb : cu >> 6;
+  const bool add = CH && tb != (cu >> 6);
   const int r0 = 64 * t0 + lane;  // this lane's row in slot 0
   unsigned boff[NS];              // byte offsets of this lane's rows in a column
   double xr[NS], acc[NS];
@@ -523,9 +527,9 @@ __device__ __forceinline__ void symv_unit(const double* __restrict__ blk, int64_
   for (int q = 0; q < kTriPairs; q += 2) {
     const int c = cu + 2 * q;
     symv_fetch<NS>(blk, n, m, c + 2, boff, b0, b1);
-    symv_consume<NS>(c, m, r0, x, xr, a0, a1, acc, pcol);
+    symv_consume<NS>(c, m, r0, x, xr, a0, a1, acc, pcol, add);
     if (q + 2 < kTriPairs) symv_fetch<NS>(blk, n, m, c + 4, boff, a0, a1);
-    symv_consume<NS>(c + 2, m, r0, x, xr, b0, b1, acc, pcol);
+    symv_consume<NS>(c + 2, m, r0, x, xr, b0, b1, acc, pcol, add);
   }
 #pragma unroll
   for (int u = 0; u < NS; ++u) prow[r0 + 64 * u] += acc[u];
@@ -534,7 +538,7 @@ __device__ __forceinline__ void symv_unit(const double* __restrict__ blk, int64_
 template <int NS>
 __device__ __forceinline__ void dots_pair(const double* __restrict__ col0, const double* __restrict__ col1,
                                           int m, const double* __restrict__ x, double* __restrict__ out0,
-                                          double* __restrict__ out1) {
+                                          double* __restrict__ out1, bool add = false) {
   const int lane = threadIdx.x & 63;
   double a0[NS], a1[NS];
 #pragma unroll
@@ -553,8 +557,8 @@ __device__ __forceinline__ void dots_pair(const double* __restrict__ col0, const
   }
   wave_sum_pair(d0, d1, d0, d1);
   if (lane == 0) {
-    *out0 = d0;
-    *out1 = d1;
+    *out0 = add ? *out0 + d0 : d0;
+    *out1 = add ? *out1 + d1 : d1;
   }
 }
 
@@ -580,6 +584,10 @@ __device__ __forceinline__ void dots_pair(const double* __restrict__ col0, const
   }
 static_assert(kMaxAO == 16 * 64, "the slot switch covers up to 16 slots of 64 rows");
 
+// BIG (n > kMaxAO): v, pq, part and aux are global, with a row stride of
+// `part` of n rounded up to 64; the symmetric product and the W^T v, V^T v
+// dots run over row chunks of at most 16 slots.
+template <bool BIG = false>
 __device__ __forceinline__ void tridiag_blocked(double* __restrict__ A, double* __restrict__ W, int n,
                                                 double* d, double* e, double* v, double* pq,
                                                 double* part, double* aux, double* red, double* bc) {
@@ -587,10 +595,11 @@ __device__ __forceinline__ void tridiag_blocked(double* __restrict__ A, double* 
   // the wavefront index is uniform: keep it (and the column indices derived
   // from it) in scalar registers
   const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nt >> 6;
-  double* pcol = aux;                   // [kMaxAO] column dots of the symmetric product
-  double* rowk = aux + kMaxAO;          // [2 kTriPanel] row k of V, of W
+  const int ldp = BIG ? (n + 63) & ~63 : kMaxAO;
+  double* pcol = aux;                   // [ldp] column dots of the symmetric product
+  double* rowk = aux + ldp;             // [2 kTriPanel] row k of V, of W
   double* yv = rowk + 2 * kTriPanel;    // [2 kTriPanel] W^T v, V^T v
-  double* prow = part + wv * kMaxAO;    // this wavefront's row partials
+  double* prow = part + wv * ldp;       // this wavefront's row partials
   auto at = [n](int r, int c) __attribute__((always_inline)) { return (int64_t)r + (int64_t)c * n; };
 
   for (int k0 = 0; k0 + 2 < n; k0 += kTriPanel) {
@@ -672,11 +681,24 @@ __device__ __forceinline__ void tridiag_blocked(double* __restrict__ A, double* 
         if (u >= nunits) continue;
         const int cu = 2 * kTriPairs * u;
         const double* blk = A + at(b, b);
-        XTB_SLOT_SWITCH(tend - (cu >> 6), (symv_unit<NS>(blk, n, m, cu, v, prow, pcol)))
+        if constexpr (BIG) {
+          for (int tb = cu >> 6; tb < tend; tb += 16)
+            XTB_SLOT_SWITCH(min(16, tend - tb), (symv_unit<NS, true>(blk, n, m, cu, tb, v, prow, pcol)))
+        } else {
+          XTB_SLOT_SWITCH(tend - (cu >> 6), (symv_unit<NS>(blk, n, m, cu, 0, v, prow, pcol)))
+        }
       }
-      for (int q = (wv + nw - nunits % nw) % nw; q < j; q += nw)
-        XTB_SLOT_SWITCH(tend, (dots_pair<NS>(W + at(b, k0 + q), A + at(b, k0 + q), m, v, yv + q,
-                                              yv + kTriPanel + q)))
+      for (int q = (wv + nw - nunits % nw) % nw; q < j; q += nw) {
+        if constexpr (BIG) {
+          for (int tb = 0; tb < tend; tb += 16)
+            XTB_SLOT_SWITCH(min(16, tend - tb),
+                            (dots_pair<NS>(W + at(b + 64 * tb, k0 + q), A + at(b + 64 * tb, k0 + q),
+                                           m - 64 * tb, v + 64 * tb, yv + q, yv + kTriPanel + q, tb > 0)))
+        } else {
+          XTB_SLOT_SWITCH(tend, (dots_pair<NS>(W + at(b, k0 + q), A + at(b, k0 + q), m, v, yv + q,
+                                                yv + kTriPanel + q)))
+        }
+      }
       __syncthreads();
       // p_j = A_j v = p - V (W^T v) - W (V^T v);  K = v^T p_j
       double kp = 0.0;
@@ -688,7 +710,7 @@ __device__ __forceinline__ void tridiag_blocked(double* __restrict__ A, double* 
           wv_[q] = W[at(b + i, min(k0 + q, n - 1))];
         }
         double p = pcol[i];
-        for (int w = 0; w < nw; ++w) p += part[w * kMaxAO + i];
+        for (int w = 0; w < nw; ++w) p += part[w * ldp + i];
 #pragma unroll
         for (int q = 0; q < kTriPanel; ++q) {
           const bool on = q < j;
@@ -1132,22 +1154,33 @@ __global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, doubl
 }
 
 // n > 256: the blocked reduction, one 512-thread block per system (the host
-// launches the large systems only); W is per-system n x n scratch
+// launches the large systems only); W is per-system n x n scratch.  BIG: the
+// systems above kMaxAO AOs, with the work vectors in their global slice of `big`.
+template <bool BIG>
 __global__ __launch_bounds__(512) void k_tridiag_blocked(Topo t, const int* order, double* Aall,
                                                          double* dall, double* eall, double* Wall,
-                                                         const unsigned char* active) {
+                                                         const unsigned char* active, double* big) {
   const int s = order[blockIdx.x];
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
-  __shared__ double v[kMaxAO];
-  __shared__ double pq[kMaxAO];
-  __shared__ double part[8 * kMaxAO];
-  __shared__ double aux[kTriAux];
+  __shared__ double v_l[BIG ? 1 : kMaxAO];
+  __shared__ double pq_l[BIG ? 1 : kMaxAO];
+  __shared__ double part_l[BIG ? 1 : 8 * kMaxAO];
+  __shared__ double aux_l[BIG ? 1 : kTriAux];
   static_assert(kTriPanel % 4 == 0, "panel in K steps of 4");
   __shared__ double red[16];
   __shared__ double bc[2];
-  tridiag_blocked(Aall + t.mat_off[s], Wall + t.mat_off[s], n, dall + t.ao_off[s],
-                  eall + t.ao_off[s], v, pq, part, aux, red, bc);
+  double *v = v_l, *pq = pq_l, *part = part_l, *aux = aux_l;
+  if constexpr (BIG) {
+    // 2 n + 8 ldp + ldp + 4 kTriPanel <= kBigPerAO n + kBigPerSys doubles
+    const int ldp = (n + 63) & ~63;
+    v = big + big_offset(t, s);
+    pq = v + n;
+    part = pq + n;
+    aux = part + 8 * ldp;
+  }
+  tridiag_blocked<BIG>(Aall + t.mat_off[s], Wall + t.mat_off[s], n, dall + t.ao_off[s],
+                       eall + t.ao_off[s], v, pq, part, aux, red, bc);
 }
 
 // Overlap Cholesky S = L L^T (L column-major into `L`) and X = L^{-T} (row-major,
@@ -1316,8 +1349,19 @@ struct RcondShared {
   int ired[kRcondWaves];
 };
 
+// The same vectors for systems above kMaxAO AOs, in their global slice (x, y:
+// n doubles each, sgn: n bytes); red and ired stay in LDS.
+struct RcondBig {
+  double* x;
+  double* y;
+  signed char* sgn;
+  double* red;
+  int* ired;
+};
+
 // block sum of |v_i|
-__device__ double rc_asum(const double* v, int n, RcondShared& sh) {
+template <typename SH>
+__device__ double rc_asum(const double* v, int n, SH& sh) {
   double a = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) a += fabs(v[i]);
   a = wave_sum(a);
@@ -1331,7 +1375,8 @@ __device__ double rc_asum(const double* v, int n, RcondShared& sh) {
 }
 
 // first index of the largest |v_i| (LAPACK idamax)
-__device__ int rc_iamax(const double* v, int n, RcondShared& sh) {
+template <typename SH>
+__device__ int rc_iamax(const double* v, int n, SH& sh) {
   double best = -1.0;
   int bi = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -1424,15 +1469,30 @@ __device__ void rc_apply(const double* __restrict__ Xm, int n, RcondShared& sh) 
   else rc_apply_k<16>(Xm, n, sh);
 }
 
-__global__ __launch_bounds__(kRcondThreads) void k_overlap_rcond(Topo t, const int* order, const double* S,
-                                                       const double* X, SysState* status) {
-  const int s = order[blockIdx.x];
-  if (status[s].status != 0) return;
-  const int n = t.ao_off[s + 1] - t.ao_off[s];
-  const int64_t mo = t.mat_off[s];
-  const double* Sm = S + mo;
-  const double* Xm = X + mo;
-  __shared__ RcondShared sh;
+// n > kMaxAO: y = X^T x with a thread per column (the column's entries of
+// consecutive rows are adjacent across the threads), then x = X y with a
+// wave per row
+__device__ void rc_apply(const double* __restrict__ Xm, int n, RcondBig& sh) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    double a = 0.0;
+    for (int r = 0; r <= c; ++r) a = fma(Xm[(int64_t)r * n + c], sh.x[r], a);
+    sh.y[c] = a;
+  }
+  __syncthreads();
+  for (int r = wv; r < n; r += kRcondWaves) {
+    double a = 0.0;
+    for (int c = r + lane; c < n; c += 64) a = fma(Xm[(int64_t)r * n + c], sh.y[c], a);
+    a = wave_sum(a);
+    if (lane == 0) sh.x[r] = a;
+  }
+  __syncthreads();
+}
+
+template <typename SH>
+__device__ void overlap_rcond(const double* __restrict__ Sm, const double* __restrict__ Xm, int n, SH& sh,
+                              SysState& status) {
   // |S|_1: S is symmetric, so the largest row sum (one wave per row, lanes
   // along the row)
   double anorm = 0.0;
@@ -1495,7 +1555,28 @@ __global__ __launch_bounds__(kRcondThreads) void k_overlap_rcond(Topo t, const i
     if (temp > est) est = temp;
   }
   const double rcond = (est != 0.0 && anorm != 0.0) ? (1.0 / est) / anorm : 0.0;
-  if (threadIdx.x == 0 && !(rcond >= kMinOverlapRcond && isfinite(rcond))) status[s].status = 8;
+  if (threadIdx.x == 0 && !(rcond >= kMinOverlapRcond && isfinite(rcond))) status.status = 8;
+}
+
+// BIG: the systems above kMaxAO AOs, with x, y and the signs in their global
+// slice of `big`
+template <bool BIG>
+__global__ __launch_bounds__(kRcondThreads) void k_overlap_rcond(Topo t, const int* order, const double* S,
+                                                       const double* X, SysState* status, double* big) {
+  const int s = order[blockIdx.x];
+  if (status[s].status != 0) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t mo = t.mat_off[s];
+  if constexpr (BIG) {
+    __shared__ double red[kRcondWaves];
+    __shared__ int ired[kRcondWaves];
+    double* x = big + big_offset(t, s);
+    RcondBig sh{x, x + n, reinterpret_cast<signed char*>(x + 2 * n), red, ired};
+    overlap_rcond(S + mo, X + mo, n, sh, status[s]);
+  } else {
+    __shared__ RcondShared sh;
+    overlap_rcond(S + mo, X + mo, n, sh, status[s]);
+  }
 }
 
 // x / y via the hardware reciprocal refined by one Newton step (a few ulp,
@@ -1964,14 +2045,17 @@ struct FermiOut {
 // kocc lowest eigenvalues only, written to Z column = ascending rank.
 // From the second SCC round on, wall holds the previous round's ascending
 // eigenvalues on entry; shift[s] carries the largest eigenvalue change of the
-// system's previous round (written here for the next one).
+// system's previous round (written here for the next one).  BIG: the systems
+// above kMaxAO AOs, with d, e, e^2, the eigenvalues and their ranks in their
+// global slice of `big`.
+template <bool BIG>
 __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
                                                        const unsigned char* active,
                                                        const double* dall, const double* eall,
                                                        double* Zall, EigenScratch sc,
                                                        double* wall, double* shift, int round_arg,
                                                        const int* round_ptr,
-                                                       double* zscale_all, FermiOut fo) {
+                                                       double* zscale_all, FermiOut fo, double* big) {
   const int s = order[blockIdx.x];
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
@@ -1983,10 +2067,17 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
   double* __restrict__ FM = sc.mult + mo;
   double* __restrict__ FP = sc.piv + mo;
   double* zscale = zscale_all + t.ao_off[s];
-  __shared__ double d[kMaxAO];
-  __shared__ double e[kMaxAO];   // split off-diagonal (negligible entries zeroed)
-  __shared__ double e2[kMaxAO];
-  __shared__ double lam[kMaxAO];
+  __shared__ double d_l[BIG ? 1 : kMaxAO];
+  __shared__ double e_l[BIG ? 1 : kMaxAO];   // split off-diagonal (negligible entries zeroed)
+  __shared__ double e2_l[BIG ? 1 : kMaxAO];
+  __shared__ double lam_l[BIG ? 1 : kMaxAO];
+  double *d = d_l, *e = e_l, *e2 = e2_l, *lam = lam_l;
+  if constexpr (BIG) {
+    d = big + big_offset(t, s);  // d, e, e2, lam, rnk: 5 n <= kBigPerAO n doubles
+    e = d + n;
+    e2 = e + n;
+    lam = e2 + n;
+  }
   __shared__ double red[16];
   const int tid = threadIdx.x, nt = blockDim.x;
   double emax2 = 0.0;
@@ -2097,7 +2188,8 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
   __syncthreads();
 
   // ---- ascending rank (stable on ties) and the sorted spectrum ----
-  __shared__ int rnk[kMaxAO];
+  __shared__ int rnk_l[BIG ? 1 : kMaxAO];
+  int* rnk = BIG ? reinterpret_cast<int*>(lam + n) : rnk_l;
   __shared__ int kocc_sh;
   double* srt = e2;  // e2 is not needed after the eigenvalues
   for (int j = tid; j < n; j += nt) {
@@ -2520,7 +2612,39 @@ __device__ __forceinline__ void backtransform_lds(const double* __restrict__ A,
   }
 }
 
-// two blocks per CU (four wavefronts per SIMD): at most 128 VGPRs
+// n > kMaxAO: the same reflector sweep on Y's columns j0 .. j0 + cw - 1 in
+// place in global memory (cw = 64: a wavefront reads 64 adjacent columns of a
+// row), the reflectors read from A; `part` holds kBackThreads doubles
+__device__ void backtransform_global(const double* __restrict__ A, const double* __restrict__ Z,
+                                     double* __restrict__ Y, int kocc, const double* zscale, int n,
+                                     int j0, int cw, double* part) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int c = tid % cw, rg = tid / cw, nrg = nt / cw;
+  const int col = j0 + c;
+  const bool on = col < kocc;
+  if (on)
+    for (int i = rg; i < n; i += nrg) Y[(int64_t)i * n + col] = zscale[col] * Z[(int64_t)i * n + col];
+  __syncthreads();
+  for (int k = n - 3; k >= 0; --k) {
+    const int m = n - k - 1;
+    const double* hv = A + (k + 1) + (int64_t)k * n;  // reflector k, column-major below the subdiagonal
+    double p = 0.0;
+    if (on)
+      for (int i = rg; i < m; i += nrg) p = fma(hv[i], Y[(int64_t)(k + 1 + i) * n + col], p);
+    part[rg * cw + c] = p;
+    __syncthreads();
+    double dot = 0.0;
+    for (int w = 0; w < nrg; ++w) dot += part[w * cw + c];
+    dot *= 2.0;
+    if (on)
+      for (int i = rg; i < m; i += nrg) Y[(int64_t)(k + 1 + i) * n + col] -= dot * hv[i];
+    __syncthreads();
+  }
+}
+
+// two blocks per CU (four wavefronts per SIMD): at most 128 VGPRs.  BIG: the
+// column blocks of systems above kMaxAO AOs (a prefix of the task list).
+template <bool BIG>
 __global__ __launch_bounds__(kBackThreads, 4) void k_backtransform(Topo t, const BackTask* tasks,
                                                                 const double* Aall,
                                                                 const double* Zall,
@@ -2540,7 +2664,9 @@ __global__ __launch_bounds__(kBackThreads, 4) void k_backtransform(Topo t, const
   const int kocc = kocc_all[s];
   if (task.j0 >= kocc) return;
   const double* zscale = zscale_all + t.ao_off[s];
-  if (n <= 256)
+  if constexpr (BIG)
+    backtransform_global(Aall + mo, Zall + mo, Yall + mo, kocc, zscale, n, task.j0, task.cw, part);
+  else if (n <= 256)
     backtransform_registers(Aall + mo, Zall + mo, Tall + wy_offset(t, s),
                             Yall + mo, kocc, zscale, n, task.j0, ring, part, wf);
   else

@@ -346,9 +346,11 @@ __global__ __launch_bounds__(256) void k_spin_combine(Topo t, const int* alpha, 
 // before the forces: P <- Pa + Pb and W <- Wa + Wb + (Pa - Pb)(v_m,m + v_m,n)/2
 // in the alpha system's matrices, so that the overlap-derivative term of the
 // force kernels carries the magnetization potential; the beta copy is then
-// left out of the forces
+// left out of the forces.  BIG: alpha systems above kMaxAO AOs, with the
+// shell potentials in their global slice of `big`.
+template <bool BIG>
 __global__ __launch_bounds__(256) void k_spin_fold(Topo t, const int* alpha, unsigned char* valid,
-                                                   const double* xs, double* P, double* W) {
+                                                   const double* xs, double* P, double* W, double* big) {
   const int s = alpha[blockIdx.x];
   const int sb = t.partner[s];
   if (!valid[s]) {
@@ -358,7 +360,8 @@ __global__ __launch_bounds__(256) void k_spin_fold(Topo t, const int* alpha, uns
   const int s0 = t.shell_off[s], ns = t.shell_off[s + 1] - s0;
   const int N = t.atom_off[s + 1] - t.atom_off[s];
   const int o = t.ao_off[s], n = t.ao_off[s + 1] - o;
-  __shared__ double vm[kMaxAO];
+  __shared__ double vm_l[BIG ? 1 : kMaxAO];
+  double* vm = BIG ? big + big_offset(t, s) : vm_l;
   const double* m = xs + t.vec_off[s] + ns + 9 * N;
   for (int i = threadIdx.x; i < ns; i += blockDim.x) {
     const int sh = s0 + i, f = t.atom_sh0[t.sh_atom[sh]] - s0;
@@ -816,21 +819,29 @@ __global__ __launch_bounds__(512) void k_force_atoms(Topo t, const int* order,
 // finishes the kernel.
 // FIX: the LDS sums of the i-atom adjoints run in fixed point (reproducible
 // mode), so their value does not depend on the order of the LDS atomics.
+// BIG: systems above kMaxAO AOs (and so possibly above kMaxAO atoms), with the
+// per-atom sums and coordinates in their global slice of `big`.
 constexpr int kAtmFields = 5;
 
-template <bool FIX>
+template <bool FIX, bool BIG>
 __global__ __launch_bounds__(512) void k_force_atm(Topo t, const int* order, const unsigned char* valid,
                                                    const double* cnd4, double* w4, double* dw4c,
                                                    double* pairbuf, double* dedcn4, double* grad,
-                                                   double* eatm) {
+                                                   double* eatm, double* big) {
   const int s = order[blockIdx.x];
   if (!valid[s]) return;
   const int a0 = t.atom_off[s], a1 = t.atom_off[s + 1], N = a1 - a0;
   const int tid = threadIdx.x, nt = blockDim.x;
   __shared__ double red[16];
-  __shared__ double gsh[3 * kMaxAO];
-  __shared__ double csh[kMaxAO];
-  __shared__ double xs[3 * kMaxAO];
+  __shared__ double gsh_l[BIG ? 1 : 3 * kMaxAO];
+  __shared__ double csh_l[BIG ? 1 : kMaxAO];
+  __shared__ double xs_l[BIG ? 1 : 3 * kMaxAO];
+  double *gsh = gsh_l, *csh = csh_l, *xs = xs_l;
+  if constexpr (BIG) {
+    gsh = big + big_offset(t, s);  // 7 N <= kBigPerAO n doubles
+    csh = gsh + 3 * N;
+    xs = csh + N;
+  }
   const int64_t NN = (int64_t)N * N;
   double* pr2 = pairbuf + kAtmFields * t.apair_off[s];
   double* pir = pr2 + NN;
