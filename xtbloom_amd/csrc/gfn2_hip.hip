@@ -1115,7 +1115,13 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
     HIP_TRY(probe_end(kKIntegrals, 0.0, 88.0 * all2));  // S + 3 D + 6 Q + H0 written
   }
   HIP_TRY(probe_begin(kKCholesky));
-  hipLaunchKernelGGL(k_cholesky_inverse, dim3(B), dim3(512), 0, sm, t, d_order, dS, dA, dX, d_st);
+  // systems are ordered largest first: the n_large above 256 AOs take the
+  // blocked factorization, the rest the register one
+  if (n_large > 0)
+    hipLaunchKernelGGL(k_cholesky_blocked, dim3(n_large), dim3(512), 0, sm, t, d_order, dS, dA, dX, d_st);
+  if (n_large < B)
+    hipLaunchKernelGGL(k_cholesky_inverse, dim3(B - n_large), dim3(512), 0, sm, t, d_order + n_large, dS, dA,
+                       dX, d_st);
   if (n_xl > 0)
     hipLaunchKernelGGL(k_overlap_rcond<true>, dim3(n_xl), dim3(kRcondThreads), 0, sm, t, d_order, dS, dX, d_st,
                        d_xl);

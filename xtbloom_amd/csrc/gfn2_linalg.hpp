@@ -1323,6 +1323,184 @@ __global__ __launch_bounds__(512) void k_cholesky_inverse(Topo t, const int* ord
   if (threadIdx.x == 0 && bad) status[s].status = 8;
 }
 
+// Overlap factorization for n > 256, 512 threads, X = L^{-T} as above (L in
+// `work`, column-major lower; X row-major upper with its strict lower part
+// zero).
+//   factorization  right-looking in panels of kCholPanel columns: inside a
+//                  panel column by column (the pivot row's panel entries go
+//                  through LDS, every thread owns rows); after the panel the
+//                  trailing lower triangle takes A -= L21 L21^T once, in
+//                  16 x 16 tiles on the FP64 matrix cores (K = the panel).
+//   inverse        L^{-1} by 16 x 16 blocks: the diagonal blocks by
+//                  substitution (a thread per block column), then block row
+//                  I from the rows above it,
+//                    Linv_IJ = -Linv_II sum_{K=J}^{I-1} L_IK Linv_KJ,
+//                  one block J per wavefront at a time (matrix cores, the sum
+//                  staged through the wavefront's LDS tile), one barrier per
+//                  block row.  Linv_ij is stored as X_ji.
+// Matrix-core lane mapping (v_mfma_f64_16x16x4f64): for D = P Q^T over k,
+// lane (l16, lk) supplies a = Q[l16][k], b = P[l16][k] for k = 4 q + lk and
+// receives D[l16][lk + 4 r], r < 4.
+constexpr int kCholPanel = 16;
+
+__global__ __launch_bounds__(512) void k_cholesky_blocked(Topo t, const int* order, const double* S,
+                                                          double* work, double* X, SysState* status) {
+  const int s = order[blockIdx.x];
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int64_t mo = t.mat_off[s];
+  const double* __restrict__ Sm = S + mo;
+  double* __restrict__ L = work + mo;
+  double* __restrict__ Xm = X + mo;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nt >> 6;
+  const int l16 = lane & 15, lk = lane >> 4;
+  __shared__ double pl[kCholPanel];
+  __shared__ double sh_ipiv;
+  __shared__ double sS[8][16][17];
+  __shared__ int bad;
+  auto at = [n](int r, int c) __attribute__((always_inline)) { return (int64_t)r + (int64_t)c * n; };
+  if (tid == 0) bad = 0;
+  for (int64_t idx = tid; idx < (int64_t)n * n; idx += nt) L[idx] = Sm[idx];
+  __syncthreads();
+  for (int k0 = 0; k0 < n; k0 += kCholPanel) {
+    const int jn = min(kCholPanel, n - k0);
+    for (int j = k0; j < k0 + jn; ++j) {
+      if (tid == 0) {
+        double d = L[at(j, j)];
+        if (!(d > 0.0)) {
+          bad = 1;
+          d = 1.0;
+        }
+        const double piv = sqrt(d);
+        const double ipiv = 1.0 / piv;
+        L[at(j, j)] = piv;
+        for (int jj = j + 1; jj < k0 + jn; ++jj) pl[jj - k0] = L[at(jj, j)] * ipiv;
+        sh_ipiv = ipiv;
+      }
+      __syncthreads();
+      const double ipiv = sh_ipiv;
+      for (int i = j + 1 + tid; i < n; i += nt) {
+        const double v = L[at(i, j)] * ipiv;
+        L[at(i, j)] = v;
+        const int je = min(i, k0 + jn - 1);
+        for (int jj = j + 1; jj <= je; ++jj) L[at(i, jj)] -= v * pl[jj - k0];
+      }
+      __syncthreads();
+    }
+    // trailing lower triangle, rows and columns >= c0: A -= L21 L21^T
+    const int c0 = k0 + jn, mt = n - c0;
+    const int nb16 = (mt + 15) >> 4;
+    const int ntiles = nb16 * (nb16 + 1) / 2;
+    for (int tile = wv; tile < ntiles; tile += nw) {
+      int TI = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+      while (TI * (TI + 1) / 2 > tile) --TI;
+      while ((TI + 1) * (TI + 2) / 2 <= tile) ++TI;
+      const int TJ = tile - TI * (TI + 1) / 2;
+      const int rr = 16 * TI + l16, cc = 16 * TJ + l16;  // this lane's row of P (rows), of Q (columns)
+      const int rrc = c0 + min(rr, mt - 1), ccc = c0 + min(cc, mt - 1);
+      double xa[kCholPanel / 4], xb[kCholPanel / 4];
+#pragma unroll
+      for (int q = 0; q < kCholPanel / 4; ++q) {
+        const int ci = k0 + min(4 * q + lk, jn - 1);
+        xa[q] = L[at(ccc, ci)];
+        xb[q] = L[at(rrc, ci)];
+      }
+      mfma_d4 acc = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < kCholPanel / 4; ++q) {
+        const bool on = 4 * q + lk < jn;
+        acc = mfma_16x16x4(on && cc < mt ? xa[q] : 0.0, on && rr < mt ? xb[q] : 0.0, acc);
+      }
+      // acc[r] = D[row = 16 TI + l16][col = 16 TJ + lk + 4 r]
+      const int row = 16 * TI + l16;
+      double old[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = 16 * TJ + lk + 4 * r;
+        old[r] = L[at(c0 + min(row, mt - 1), c0 + min(col, mt - 1))];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = 16 * TJ + lk + 4 * r;
+        if (row < mt && col < mt && row >= col) L[at(c0 + row, c0 + col)] = old[r] - acc[r];
+      }
+    }
+    __syncthreads();
+  }
+  // ---- X = L^{-T}: strict lower part of X (upper part of L^{-1}) is zero ----
+  for (int64_t idx = tid; idx < (int64_t)n * n; idx += nt) {
+    const int r = (int)(idx / n), c = (int)(idx - (int64_t)r * n);
+    if (c < r) Xm[idx] = 0.0;
+  }
+  // diagonal blocks: thread (I, c) solves column c of L_II^{-1}
+  const int NB = (n + 15) >> 4;
+  for (int idx = tid; idx < 16 * NB; idx += nt) {
+    const int I = idx >> 4, c = idx & 15, o = 16 * I, b = min(16, n - o);
+    if (c >= b) continue;
+    double y[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      y[r] = 0.0;
+      if (r < c || r >= b) continue;
+      double v = r == c ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < r; ++k)
+        if (k >= c) v -= L[at(o + r, o + k)] * y[k];
+      y[r] = v / L[at(o + r, o + r)];
+      Xm[(int64_t)(o + c) * n + o + r] = y[r];
+    }
+  }
+  __syncthreads();
+  // block rows I >= 1: blocks J < I, one per wavefront at a time
+  for (int I = 1; I < NB; ++I) {
+    const int oi = 16 * I;
+    const int ri = oi + l16;  // this lane's row of block row I
+    const bool rin = ri < n;
+    const int ric = min(ri, n - 1);
+    for (int J = wv; J < I; J += nw) {
+      const int oj = 16 * J;
+      mfma_d4 acc = mfma_d4{0.0, 0.0, 0.0, 0.0};
+      for (int K = J; K < I; ++K) {
+        const int ok = 16 * K;
+        double xa[4], xb[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int k = 4 * q + lk;
+          xa[q] = Xm[(int64_t)(oj + l16) * n + ok + k];  // Linv(ok + k, oj + l16)
+          xb[q] = L[at(ric, ok + k)];                    // L(oi + l16, ok + k)
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc = mfma_16x16x4(xa[q], rin ? xb[q] : 0.0, acc);
+      }
+      // S[l16][lk + 4 r] -> LDS; then Linv_IJ = -Linv_II S
+      double(*st)[17] = sS[wv];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) st[l16][lk + 4 * r] = acc[r];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      mfma_d4 acc2 = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = 4 * q + lk;
+        const bool kin = oi + k < n;
+        const double a = st[k][l16];                                          // S[k][l16]
+        const double bv = Xm[(int64_t)(oi + (kin ? k : 0)) * n + ric];       // Linv(oi + l16, oi + k)
+        acc2 = mfma_16x16x4(a, rin && kin ? bv : 0.0, acc2);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (rin) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Xm[(int64_t)(oj + lk + 4 * r) * n + ri] = -acc2[r];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && bad) status[s].status = 8;
+}
+
 // ---------------------------------------------------------------------------
 // Overlap conditioning gate of the reference (eigensolver.cpp:2388-2402):
 // rcond_1 = 1 / (|S|_1 est(|S^{-1}|_1)) >= 1e-12, where est is LAPACK's
