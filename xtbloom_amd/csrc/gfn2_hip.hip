@@ -452,12 +452,14 @@ Device* device_create(int device_id, void* stream, std::string& error) {
   for (int i = 0; i < 4 && ok; ++i) ok = hipEventCreate(&d->ev[i]) == hipSuccess;
   for (int i = 0; i < 2 && ok; ++i)
     ok = hipEventCreateWithFlags(&d->round_ev[i], hipEventDisableTiming) == hipSuccess;
-  // gfx950 grants up to 160 KB of LDS to one workgroup; opt the back-transformation in
+  // gfx950 grants up to 160 KB of LDS to one workgroup; opt the back-transformations in
   int lds = 0;
   if (ok && hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device_id) == hipSuccess &&
       lds > 65536 &&
-      hipFuncSetAttribute((const void*)k_backtransform<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          lds - (int)kBackStaticLds) == hipSuccess)
+      hipFuncSetAttribute((const void*)k_backtransform, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          lds - (int)kBackStaticLds) == hipSuccess &&
+      hipFuncSetAttribute((const void*)k_backtransform_wy<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          lds) == hipSuccess)
     d->lds_limit = (size_t)lds;
   if (!ok) {
     error = "failed to allocate or upload the device parameter tables";
@@ -615,10 +617,10 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
   std::vector<BackTask> back;
   std::vector<FockTile> chunks;
   std::vector<int64_t> part_off(B + 1, 0);
-  size_t back_lds = 0;
+  size_t back_lds = 0, back_lds_xl = 0;  // dynamic LDS of k_backtransform_wy, <false> and <true>
   int n_large = 0;  // systems above 256 AOs (a prefix of the size order)
   int n_xl = 0;     // systems above kMaxAO AOs (a prefix of that prefix)
-  size_t n_back_xl = 0;  // their back-transformation tasks (a prefix of `back`)
+  size_t n_back_large = 0, n_back_xl = 0;  // their back-transformation tasks (prefixes of `back`)
   for (int i = 0; i < B; ++i) {
     const int s = T.order[i];
     const int n = T.ao_off[s + 1] - T.ao_off[s];
@@ -630,14 +632,20 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
         full.push_back({s, a, b});
         if (a >= b) lower.push_back({s, a, b});
       }
-    int cw = 64;  // register path (n <= 256) and global path (n > kMaxAO): no dynamic LDS
+    int cw = 64;  // columns per task: register path (n <= 256), WY path in place (n > kMaxAO)
     if (n > 256 && n <= kMaxAO) {
-      const size_t budget = dev->lds_limit - kBackStaticLds;
-      while (cw > 1 && (size_t)(n * cw + 2 * n + kBackThreads) * sizeof(double) > budget) cw /= 2;
-      back_lds = std::max(back_lds, (size_t)(n * cw + 2 * n + kBackThreads) * sizeof(double));
+      while (cw > 4 && backtransform_wy_lds(n, cw, false) > dev->lds_limit) cw /= 2;
+      if (backtransform_wy_lds(n, cw, false) > dev->lds_limit) {
+        error = "the device grants too little LDS for the back-transformation";
+        return 6;
+      }
+      back_lds = std::max(back_lds, backtransform_wy_lds(n, cw, false));
+    } else if (n > kMaxAO) {
+      back_lds_xl = backtransform_wy_lds(n, cw, true);
     }
     for (int j0 = 0; j0 < n; j0 += cw) back.push_back({s, j0, cw, 0});
     if (n > kMaxAO) n_back_xl = back.size();
+    if (n > 256) n_back_large = back.size();
     for (int r0 = 0; r0 < n; r0 += kFockTile)
       for (int c0 = r0; c0 < n; c0 += kFockTile) chunks.push_back({s, r0, c0});
   }
@@ -1244,12 +1252,24 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
                          rptr, d_zscale, FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt}, d_xl);
     HIP_TRY(probe_end(kKTridiagEigen, 0.0, 0.0));
     HIP_TRY(probe_begin(kKBacktransform));
+    // systems above 256 AOs: the WY factors of their reflectors, then the WY
+    // back-transformation (in place in Y above kMaxAO); the rest: registers
+    if (n_xl > 0)
+      hipLaunchKernelGGL(k_wy_factors<true>, dim3(n_xl), dim3(512), 0, sm, t, d_order, dA, d_wyT, d_active,
+                         d_xl);
+    if (n_large > n_xl)
+      hipLaunchKernelGGL(k_wy_factors<false>, dim3(n_large - n_xl), dim3(512), 0, sm, t, d_order + n_xl, dA,
+                         d_wyT, d_active, d_xl);
     if (n_back_xl > 0)
-      hipLaunchKernelGGL(k_backtransform<true>, dim3((unsigned)n_back_xl), dim3(kBackThreads), 0, sm, t,
-                         d_back, dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
-    if (back.size() > n_back_xl)
-      hipLaunchKernelGGL(k_backtransform<false>, dim3((unsigned)(back.size() - n_back_xl)), dim3(kBackThreads),
-                         back_lds, sm, t, d_back + n_back_xl, dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
+      hipLaunchKernelGGL(k_backtransform_wy<true>, dim3((unsigned)n_back_xl), dim3(kWyThreads), back_lds_xl, sm,
+                         t, d_back, dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
+    if (n_back_large > n_back_xl)
+      hipLaunchKernelGGL(k_backtransform_wy<false>, dim3((unsigned)(n_back_large - n_back_xl)),
+                         dim3(kWyThreads), back_lds, sm, t, d_back + n_back_xl, dA, dZ, d_wyT, dY, d_kocc,
+                         d_zscale, d_active);
+    if (back.size() > n_back_large)
+      hipLaunchKernelGGL(k_backtransform, dim3((unsigned)(back.size() - n_back_large)), dim3(kBackThreads), 0,
+                         sm, t, d_back + n_back_large, dA, dZ, d_wyT, dY, d_kocc, d_zscale, d_active);
     HIP_TRY(probe_end(kKBacktransform, 0.0, 0.0));
     if (rp) rp->bt = last_probe();
     // C = X Y (the kocc leading columns), P = C diag(f) C^T

@@ -1068,6 +1068,9 @@ __device__ __forceinline__ int64_t wy_offset(const Topo& t, int s) {
 // strict lower part, column k = reflector k), by the whole block: one thread
 // per (block, pair) inner product over rows > k_i (where both reflectors live),
 // then one thread per block runs the recurrence.  gsh holds 28 doubles per block.
+// (Each caller takes its own instance: a second caller of one instance changes
+// the register kernel's register allocation.)
+template <int INSTANCE>
 __device__ __forceinline__ void wy_factors(const double* A, int n, double* Ts, double* gsh) {
   constexpr int kPairs = kBackChunk * (kBackChunk - 1) / 2;  // 28
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -1149,7 +1152,7 @@ __global__ __launch_bounds__(512) void k_tridiag(Topo t, const int* order, doubl
   // reflectors must be visible to the whole block first
   if (n <= 256 && n > 2) {
     __syncthreads();
-    wy_factors(A, n, Tall + wy_offset(t, s), rowpart);
+    wy_factors<0>(A, n, Tall + wy_offset(t, s), rowpart);
   }
 }
 
@@ -2790,39 +2793,142 @@ __device__ __forceinline__ void backtransform_lds(const double* __restrict__ A,
   }
 }
 
-// n > kMaxAO: the same reflector sweep on Y's columns j0 .. j0 + cw - 1 in
-// place in global memory (cw = 64: a wavefront reads 64 adjacent columns of a
-// row), the reflectors read from A; `part` holds kBackThreads doubles
-__device__ void backtransform_global(const double* __restrict__ A, const double* __restrict__ Z,
-                                     double* __restrict__ Y, int kocc, const double* zscale, int n,
-                                     int j0, int cw, double* part) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int c = tid % cw, rg = tid / cw, nrg = nt / cw;
+// T factors of the WY blocks of the systems above 256 AOs (the register
+// reduction forms its own), one 512-thread block per system after the blocked
+// reduction.  BIG (n > kMaxAO): the Gram entries in the system's global slice.
+constexpr int kWyGram = kBackChunk * (kBackChunk - 1) / 2 * ((kMaxAO - 2 + kBackChunk - 1) / kBackChunk);
+template <bool BIG>
+__global__ __launch_bounds__(512) void k_wy_factors(Topo t, const int* order, const double* Aall,
+                                                    double* Tall, const unsigned char* active, double* big) {
+  const int s = order[blockIdx.x];
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  __shared__ double gram_l[BIG ? 1 : kWyGram];
+  double* gram = BIG ? big + big_offset(t, s) : gram_l;  // 28 ceil((n-2)/8) <= kBigPerAO n doubles
+  wy_factors<1>(Aall + t.mat_off[s], n, Tall + wy_offset(t, s), gram);
+}
+
+// n > 256: Y = Q Z on a block of cw columns (cw | 64), the reflectors applied
+// kBackChunk at a time in compact WY form with the T factors of k_wy_factors:
+//   w = V^T z, y = T w, z -= V y,
+// four barriers per block of eight reflectors.  Thread (c, g) = (tid % cw,
+// tid / cw) holds column j0 + c, rows i0 + g, i0 + g + 512 / cw, ... of the
+// rows the block touches (i0 = one past its lowest reflector); the dot
+// partials are summed over the row groups of a wavefront by lane exchanges,
+// then over the wavefronts in LDS.
+// Dynamic LDS (doubles): non-BIG, the column block [n][cw], the block's
+// reflectors [8][n] (zero-padded to absolute rows), partials [8][8][cw] and
+// y [8][cw]; BIG (n > kMaxAO), partials and y only: the column block is
+// updated in place in Y and the reflectors are read from A.
+constexpr int kWyThreads = 512;
+__host__ __device__ constexpr size_t backtransform_wy_lds(int n, int cw, bool big) {
+  return ((big ? 0 : (size_t)n * cw + (size_t)kBackChunk * n) + (size_t)kBackChunk * 9 * cw) * sizeof(double);
+}
+
+template <bool BIG>
+__global__ __launch_bounds__(kWyThreads) void k_backtransform_wy(Topo t, const BackTask* tasks,
+                                                                 const double* Aall, const double* Zall,
+                                                                 const double* Tall, double* Yall,
+                                                                 const int* kocc_all, const double* zscale_all,
+                                                                 const unsigned char* active) {
+  extern __shared__ double lds[];
+  const BackTask task = tasks[blockIdx.x];
+  const int s = task.sys;
+  if (active && !active[s]) return;
+  const int n = t.ao_off[s + 1] - t.ao_off[s];
+  const int kocc = kocc_all[s];
+  if (task.j0 >= kocc) return;
+  const int64_t mo = t.mat_off[s];
+  const double* __restrict__ A = Aall + mo;
+  const double* __restrict__ Z = Zall + mo;
+  double* __restrict__ Y = Yall + mo;
+  const double* zscale = zscale_all + t.ao_off[s];
+  const double* Tf = Tall + wy_offset(t, s);
+  const int cw = task.cw, j0 = task.j0;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int c = tid % cw, g = tid / cw, ng = kWyThreads / cw;
   const int col = j0 + c;
   const bool on = col < kocc;
-  if (on)
-    for (int i = rg; i < n; i += nrg) Y[(int64_t)i * n + col] = zscale[col] * Z[(int64_t)i * n + col];
+  double* zb = BIG ? Y + j0 : lds;  // z(i, c) at zb[i * ldz + c]
+  const int64_t ldz = BIG ? n : cw;
+  double* vs = lds + (BIG ? 0 : (int64_t)n * cw);          // [8][n]
+  double* part = vs + (BIG ? 0 : (int64_t)kBackChunk * n);  // [8][8][cw]
+  double* yv = part + kBackChunk * 8 * cw;                  // [8][cw]
+  for (int i = g; i < n; i += ng) {
+    const double zv = on ? zscale[col] * Z[(int64_t)i * n + col] : 0.0;
+    if (!BIG || on) zb[i * ldz + c] = zv;
+  }
   __syncthreads();
-  for (int k = n - 3; k >= 0; --k) {
-    const int m = n - k - 1;
-    const double* hv = A + (k + 1) + (int64_t)k * n;  // reflector k, column-major below the subdiagonal
-    double p = 0.0;
-    if (on)
-      for (int i = rg; i < m; i += nrg) p = fma(hv[i], Y[(int64_t)(k + 1 + i) * n + col], p);
-    part[rg * cw + c] = p;
+  const int nref = n - 2;  // reflectors k = 0 .. n-3
+  for (int ch = 0; ch * kBackChunk < nref; ++ch) {
+    const int kb = n - 3 - ch * kBackChunk;  // k_0; k_u = kb - u
+    const int i0 = max(kb - (kBackChunk - 1), 0) + 1;
+    const int m = n - i0;
+    if constexpr (!BIG) {
+      for (int x = tid; x < kBackChunk * m; x += kWyThreads) {
+        const int u = x / m, i = i0 + (x - u * m), k = kb - u;
+        vs[u * n + i] = (k >= 0 && i > k) ? A[i + (int64_t)k * n] : 0.0;
+      }
+      __syncthreads();
+    }
+    auto vat = [&](int i, int u) __attribute__((always_inline)) -> double {
+      if constexpr (BIG) {
+        const int k = kb - u;
+        return (k >= 0 && i > k) ? A[i + (int64_t)k * n] : 0.0;
+      } else {
+        return vs[u * n + i];
+      }
+    };
+    double p[kBackChunk];
+#pragma unroll
+    for (int u = 0; u < kBackChunk; ++u) p[u] = 0.0;
+    for (int i = i0 + g; i < n; i += ng) {
+      const double zv = (!BIG || on) ? zb[i * ldz + c] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kBackChunk; ++u) p[u] = fma(vat(i, u), zv, p[u]);
+    }
+    for (int off = cw; off < 64; off <<= 1)
+#pragma unroll
+      for (int u = 0; u < kBackChunk; ++u) p[u] += __shfl_xor(p[u], off, 64);
+    if (lane < cw)
+#pragma unroll
+      for (int u = 0; u < kBackChunk; ++u) part[(u * 8 + w) * cw + c] = p[u];
     __syncthreads();
-    double dot = 0.0;
-    for (int w = 0; w < nrg; ++w) dot += part[w * cw + c];
-    dot *= 2.0;
-    if (on)
-      for (int i = rg; i < m; i += nrg) Y[(int64_t)(k + 1 + i) * n + col] -= dot * hv[i];
+    // y = T w, T lower triangular (thread (u, c))
+    if (tid < kBackChunk * cw) {
+      const int u = tid / cw, cc = tid - u * cw;
+      double acc = 0.0;
+      for (int i = 0; i <= u; ++i) {
+        double wsum = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) wsum += part[(i * 8 + q) * cw + cc];
+        acc = fma(Tf[ch * kBackChunk * kBackChunk + u * kBackChunk + i], wsum, acc);
+      }
+      yv[u * cw + cc] = acc;
+    }
     __syncthreads();
+    double yc[kBackChunk];
+#pragma unroll
+    for (int u = 0; u < kBackChunk; ++u) yc[u] = yv[u * cw + c];
+    if (!BIG || on)
+      for (int i = i0 + g; i < n; i += ng) {
+        double zv = zb[i * ldz + c];
+#pragma unroll
+        for (int u = 0; u < kBackChunk; ++u) zv = fma(-vat(i, u), yc[u], zv);
+        zb[i * ldz + c] = zv;
+      }
+    __syncthreads();
+  }
+  if constexpr (!BIG) {
+    if (on)
+      for (int i = g; i < n; i += ng) Y[(int64_t)i * n + col] = zb[i * cw + c];
   }
 }
 
-// two blocks per CU (four wavefronts per SIMD): at most 128 VGPRs.  BIG: the
-// column blocks of systems above kMaxAO AOs (a prefix of the task list).
-template <bool BIG>
+// two blocks per CU (four wavefronts per SIMD): at most 128 VGPRs.  The host
+// sends the column blocks of systems of at most 256 AOs here (the register
+// path) and larger ones to k_backtransform_wy; the LDS branch stays as the
+// kernel's general form.
 __global__ __launch_bounds__(kBackThreads, 4) void k_backtransform(Topo t, const BackTask* tasks,
                                                                 const double* Aall,
                                                                 const double* Zall,
@@ -2842,9 +2948,7 @@ __global__ __launch_bounds__(kBackThreads, 4) void k_backtransform(Topo t, const
   const int kocc = kocc_all[s];
   if (task.j0 >= kocc) return;
   const double* zscale = zscale_all + t.ao_off[s];
-  if constexpr (BIG)
-    backtransform_global(Aall + mo, Zall + mo, Yall + mo, kocc, zscale, n, task.j0, task.cw, part);
-  else if (n <= 256)
+  if (n <= 256)
     backtransform_registers(Aall + mo, Zall + mo, Tall + wy_offset(t, s),
                             Yall + mo, kocc, zscale, n, task.j0, ring, part, wf);
   else
