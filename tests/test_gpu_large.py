@@ -21,7 +21,9 @@ Checks:
   oracle fixture;
 - central differences of the GPU's own energies at 1142 AOs;
 - the plain and graph-captured paths agree, and reproducible mode repeats
-  bit for bit.
+  bit for bit;
+- the overlap conditioning gate (status 8 exactly where the oracle's dpocon
+  gate fails) after the blocked factorization, at 364 and 1144 AOs.
 """
 import os
 
@@ -147,3 +149,39 @@ def test_paths_and_reproducible_mode(monkeypatch):
         assert np.array_equal(getattr(first, name), getattr(second, name)), name
     assert np.abs(first.energies - plain.energies).max() <= 1e-9
     assert np.abs(first.forces - plain.forces).max() <= 1e-9
+
+
+def _near_dependent(k, d, rng):
+    """CkH(2k+2) with an extra H at distance d from one H (and one more H for parity)."""
+    base = workloads.alkane(k, rng)
+    Z0, X0 = base["Z"], base["xyz"]
+    h = np.where(Z0 == 1)[0]
+    xyz = np.vstack([X0, X0[h[0]] + [d, 0.0, 0.0], X0[h[3]] + [0.0, 0.0, 0.5]])
+    return dict(Z=np.concatenate([Z0, [1, 1]]), xyz=xyz, charge=0.0, uhf=0)
+
+
+def test_overlap_gate_on_large_systems():
+    rng = np.random.default_rng(11)
+    members = [workloads.water(rng), _near_dependent(60, 3e-6, rng), _near_dependent(190, 3e-6, rng),
+               _near_dependent(60, 1e-3, rng)]
+    assert [O.basis_counts(m["Z"])[1] for m in members[1:]] == [364, 1144, 364]
+    b = workloads.from_systems(members)
+    iters = 40
+    ctx = interface.Context("hip", 0)
+    try:
+        print("case near-dependent overlaps at 364 and 1144 AOs", flush=True)
+        out = _run(ctx, b, max_scc_iterations=iters)
+    finally:
+        ctx.close()
+    ref = O.batch(b.atom_offsets, b.atomic_numbers, b.positions, b.molecular_charges,
+                  b.unpaired_electrons, opts=O.options(max_iterations=iters), threads=4)
+    print("statuses", out.per_system_status.tolist(), "oracle", ref["status"].tolist(), flush=True)
+    failed = library.STATUS_EIGENSOLVER_FAILED
+    assert ref["status"][1] == failed and ref["status"][2] == failed and ref["status"][3] != failed
+    assert [st == failed for st in out.per_system_status] == [st == failed for st in ref["status"]]
+    for i in (1, 2):
+        a0, a1 = b.atom_offsets[i], b.atom_offsets[i + 1]
+        assert np.isnan(out.energies[i]) and np.all(np.isnan(out.forces[a0:a1]))
+    assert out.per_system_status[0] == 0 and abs(out.energies[0] - ref["energies"][0]) <= 1e-10
+    if ref["status"][3] == 0 and out.per_system_status[3] == 0:
+        assert abs(out.energies[3] - ref["energies"][3]) <= E_TOL

@@ -738,6 +738,7 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
   // work vectors of the systems above kMaxAO AOs (big_offset)
   const size_t xl_doubles = n_xl > 0 ? (size_t)kBigPerAO * NAO + (size_t)kBigPerSys * B : 0;
   if (xl_doubles > 0) addd(xl_doubles);
+  addi(B); addi(NAO); addd(NAO); addi(NAO);  // eigenvalue clusters handed to k_cluster_vectors
   if (opt.no_alloc) {
     if (bytes > kNoAllocWorkspaceBytes) {
       error = "the batch needs " + std::to_string(bytes) +
@@ -984,6 +985,11 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
   int* d_gstagei = cv.take<int>((size_t)kPairStageInts * N);
   mx.ovl = cv.take<double>((size_t)mem * mem * B);
   double* d_xl = n_xl > 0 ? cv.take<double>(xl_doubles) : nullptr;
+  ClusterOut co{};
+  co.flag = cv.take<int>(B);
+  co.end = cv.take<int>(NAO);
+  co.lam = cv.take<double>(NAO);
+  co.rank = cv.take<int>(NAO);
   HIP_TRY(up_topo(d_perm, T.perm));
   t.perm = d_perm;
   const bool big_systems = T.max_atoms > kPairStageAtoms;
@@ -998,6 +1004,8 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
   };
 
   HIP_TRY(hipMemsetAsync(d_st, 0, sizeof(SysState) * B, sm));
+  HIP_TRY(hipMemsetAsync(co.flag, 0, sizeof(int) * B, sm));
+  HIP_TRY(hipMemsetAsync(co.end, 0, sizeof(int) * (size_t)NAO, sm));
   HIP_TRY(hipMemsetAsync(mx.df, 0, sizeof(double) * (size_t)mem * NV, sm));
   HIP_TRY(hipMemsetAsync(mx.u, 0, sizeof(double) * (size_t)mem * NV, sm));
   HIP_TRY(hipMemsetAsync(mx.omega, 0, sizeof(double) * (size_t)mem * B, sm));
@@ -1245,11 +1253,14 @@ static int run_batch_core(Device* dev, const HostBatch& hb, const RunOptions& op
     if (n_xl > 0)
       hipLaunchKernelGGL(k_tridiag_eigen<true>, dim3(n_xl), dim3(256), 0, sm, t, d_order, d_active, d_td,
                          d_te, dZ, EigenScratch{dT, dH, dC, dP, dW}, d_eps, d_shift, it, rptr, d_zscale,
-                         FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt}, d_xl);
+                         FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt}, d_xl, co);
     if (n_xl < B)
       hipLaunchKernelGGL(k_tridiag_eigen<false>, dim3(B - n_xl), dim3(256), 0, sm, t, d_order + n_xl,
                          d_active, d_td, d_te, dZ, EigenScratch{dT, dH, dC, dP, dW}, d_eps, d_shift, it,
-                         rptr, d_zscale, FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt}, d_xl);
+                         rptr, d_zscale, FermiOut{d_fa, d_fb, d_occ, d_fe, d_st, d_kocc, opt.kt}, d_xl, co);
+    // clusters of more than kSerialCluster eigenvalues (rare: near-dependent bases)
+    hipLaunchKernelGGL(k_cluster_vectors, dim3(B), dim3(256), 0, sm, t, d_order, d_active, d_td, d_te, dZ,
+                       EigenScratch{dT, dH, dC, dP, dW}, d_zscale, co);
     HIP_TRY(probe_end(kKTridiagEigen, 0.0, 0.0));
     HIP_TRY(probe_begin(kKBacktransform));
     // systems above 256 AOs: the WY factors of their reflectors, then the WY

@@ -330,6 +330,20 @@ __device__ void cholesky_global(const double* Sm, double* L, double* Xm, int n, 
 constexpr double kEps = 2.220446049250313e-16;
 constexpr double kSafeMin = 2.2250738585072014e-308;
 constexpr double kClusterTol = 1e-6;  // relative gap (to the block 1-norm) below which vectors are orthogonalized
+// Clusters of more than kSerialCluster eigenvalues go to k_cluster_vectors
+// (a near-dependent basis inflates |T| and can chain hundreds of eigenvalues
+// into one cluster, whose Gram-Schmidt is quadratic in its size)
+constexpr int kSerialCluster = 8;
+// Hand-over of those clusters, per system: flag[s] set when it has one; for
+// a cluster of indices j .. g1-1 (block-local eigenvalue order), end[ao + j]
+// = g1 and lam / rank[ao + jj] hold each member's eigenvalue and Z column.
+// k_cluster_vectors clears end[] entries after use.
+struct ClusterOut {
+  int* flag;
+  int* end;
+  double* lam;
+  int* rank;
+};
 constexpr int kInverseIterations = 3;
 
 // Householder reduction with the trailing matrix in global memory, one
@@ -2228,7 +2242,8 @@ struct FermiOut {
 // eigenvalues on entry; shift[s] carries the largest eigenvalue change of the
 // system's previous round (written here for the next one).  BIG: the systems
 // above kMaxAO AOs, with d, e, e^2, the eigenvalues and their ranks in their
-// global slice of `big`.
+// global slice of `big`.  Clusters of more than kSerialCluster close
+// eigenvalues are left to k_cluster_vectors through `co`.
 template <bool BIG>
 __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
                                                        const unsigned char* active,
@@ -2236,7 +2251,8 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
                                                        double* Zall, EigenScratch sc,
                                                        double* wall, double* shift, int round_arg,
                                                        const int* round_ptr,
-                                                       double* zscale_all, FermiOut fo, double* big) {
+                                                       double* zscale_all, FermiOut fo, double* big,
+                                                       ClusterOut co) {
   const int s = order[blockIdx.x];
   if (active && !active[s]) return;
   const int n = t.ao_off[s + 1] - t.ao_off[s];
@@ -2261,6 +2277,7 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
   }
   __shared__ double red[16];
   const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) co.flag[s] = 0;
   double emax2 = 0.0;
   for (int i = tid; i < n; i += nt) {
     d[i] = dall[t.ao_off[s] + i];
@@ -2475,6 +2492,16 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
     if (j > b0 && lam[j] - lam[j - 1] <= gtol) continue;  // not a cluster start
     int g1 = j + 1;
     while (g1 < b1 && lam[g1] - lam[g1 - 1] <= gtol) ++g1;
+    if (g1 - j > kSerialCluster) {  // the whole block orthogonalizes it (k_cluster_vectors)
+      const int o = t.ao_off[s];
+      for (int jj = j; jj < g1; ++jj) {
+        co.lam[o + jj] = lam[jj];
+        co.rank[o + jj] = rnk[jj];
+      }
+      co.end[o + j] = g1;
+      co.flag[s] = 1;
+      continue;
+    }
     const double ptol = kEps * fmax(onenrm, kSafeMin);       // smallest admissible pivot
     const double pertol = 10.0 * kEps * fmax(onenrm, kSafeMin);
     double xprev = 0.0;
@@ -2586,6 +2613,218 @@ __global__ __launch_bounds__(256) void k_tridiag_eigen(Topo t, const int* order,
       const double f = (big < 0.0 ? -1.0 : 1.0) / sqrt(nrm);
       for (int i = b0; i < b1; ++i) Z[(int64_t)i * n + cj] *= f;
     }
+  }
+}
+
+// Block-wide maximum (blockDim.x <= 1024; red holds blockDim.x / 64 doubles).
+__device__ __forceinline__ double block_max(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double m = red[0];
+  for (int i = 1; i < nw; ++i) m = fmax(m, red[i]);
+  __syncthreads();
+  return m;
+}
+
+// Eigenvectors of the clusters k_tridiag_eigen hands over (more than
+// kSerialCluster members), one 256-thread block per system with such a
+// cluster.  The vectors follow the serial path's inverse iteration: the same
+// shift perturbation, partially pivoted LU of T_b - x I and start vector, the
+// same iteration count, normalization and sign.  The orthogonalization
+// against the cluster's earlier vectors runs on the whole block as classical
+// Gram-Schmidt applied twice, over chunks of kCoopChunk earlier vectors (their
+// dots a thread per vector, the update a thread per row); only the LU and the
+// two substitutions (O(n) per vector) stay on one thread.  T's splitting, the
+// block norms are recomputed from d and e as k_tridiag_eigen forms them.
+constexpr int kCoopChunk = 256;
+
+__global__ __launch_bounds__(256) void k_cluster_vectors(Topo t, const int* order, const unsigned char* active,
+                                                         const double* dall, const double* eall, double* Zall,
+                                                         EigenScratch sc, double* zscale_all, ClusterOut co) {
+  const int s = order[blockIdx.x];
+  if ((active && !active[s]) || !co.flag[s]) return;
+  const int o = t.ao_off[s], n = t.ao_off[s + 1] - o;
+  const int64_t mo = t.mat_off[s];
+  double* __restrict__ Z = Zall + mo;
+  double* __restrict__ F0 = sc.u0 + mo;
+  double* __restrict__ F1 = sc.u1 + mo;
+  double* __restrict__ F2 = sc.u2 + mo;
+  double* __restrict__ FM = sc.mult + mo;
+  double* __restrict__ FP = sc.piv + mo;
+  double* zscale = zscale_all + o;
+  const double* d = dall + o;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  __shared__ double red[16];
+  __shared__ int ired[16];
+  __shared__ double dots[kCoopChunk];
+  // off-diagonal after the split (negligible entries zeroed)
+  auto esp = [&](int i) -> double {
+    if (i + 1 >= n) return 0.0;
+    double ei = eall[o + i];
+    if (ei * ei <= kEps * kEps * fabs(d[i] * d[i + 1]) + kSafeMin) ei = 0.0;
+    return ei * ei == 0.0 ? 0.0 : ei;
+  };
+  double rows1 = 0.0, splits = 0.0;
+  for (int i = tid; i < n; i += nt) {
+    const double ei = esp(i), ep = i > 0 ? esp(i - 1) : 0.0;
+    rows1 = fmax(rows1, fabs(d[i]) + fabs(ep) + fabs(ei));
+    if (i + 1 < n && ei == 0.0) splits += 1.0;
+  }
+  const double tnorm1 = block_max(rows1, red);
+  const bool unsplit = block_sum(splits, red) == 0.0;
+  for (int j = 0; j < n; ++j) {
+    const int g1 = co.end[o + j];
+    if (g1 == 0) continue;
+    int b0 = j, b1 = j + 1;
+    double onenrm = 0.0;
+    if (unsplit) {
+      b0 = 0;
+      b1 = n;
+      onenrm = tnorm1;
+    } else {
+      while (b0 > 0 && esp(b0 - 1) != 0.0) --b0;
+      while (b1 < n && esp(b1 - 1) != 0.0) ++b1;
+      for (int i = b0; i < b1; ++i)
+        onenrm = fmax(onenrm, fabs(d[i]) + (i > b0 ? fabs(esp(i - 1)) : 0.0) + (i + 1 < b1 ? fabs(esp(i)) : 0.0));
+    }
+    const int bn = b1 - b0;
+    const double ptol = kEps * fmax(onenrm, kSafeMin);
+    const double pertol = 10.0 * kEps * fmax(onenrm, kSafeMin);
+    double xprev = 0.0;
+    for (int jj = j; jj < g1; ++jj) {
+      const int cj = co.rank[o + jj];
+      for (int i = tid; i < n; i += nt)
+        if (i < b0 || i >= b1) Z[(int64_t)i * n + cj] = 0.0;
+      if (tid == 0) zscale[cj] = 1.0;
+      double x = co.lam[o + jj];
+      if (jj > j && x - xprev < pertol) x = xprev + pertol;
+      xprev = x;
+      if (bn == 1) {
+        if (tid == 0) Z[(int64_t)b0 * n + cj] = 1.0;
+        __syncthreads();
+        continue;
+      }
+      if (tid == 0) {  // LU with partial pivoting of T_b - x I
+        double c0 = d[b0] - x, c1 = esp(b0), c2 = 0.0;
+        for (int k = 0; k < bn - 1; ++k) {
+          const int gi = b0 + k;
+          const double n0 = esp(gi), n1 = d[gi + 1] - x, n2 = (k + 2 < bn) ? esp(gi + 1) : 0.0;
+          const int64_t at = (int64_t)gi * n + cj;
+          if (fabs(c0) >= fabs(n0)) {
+            const double m = c0 != 0.0 ? n0 / c0 : 0.0;
+            F0[at] = c0; F1[at] = c1; F2[at] = c2; FM[at] = m; FP[at] = 0.0;
+            c0 = n1 - m * c1;
+            c1 = n2 - m * c2;
+          } else {
+            const double m = c0 / n0;
+            F0[at] = n0; F1[at] = n1; F2[at] = n2; FM[at] = m; FP[at] = 1.0;
+            const double t0 = c1 - m * n1, t1 = c2 - m * n2;
+            c0 = t0;
+            c1 = t1;
+          }
+          c2 = 0.0;
+        }
+        const int64_t at = (int64_t)(b1 - 1) * n + cj;
+        F0[at] = c0; F1[at] = 0.0; F2[at] = 0.0; FM[at] = 0.0; FP[at] = 0.0;
+      }
+      for (int i = b0 + tid; i < b1; i += nt) Z[(int64_t)i * n + cj] = start_component(jj, i - b0);
+      __syncthreads();
+      for (int it = 0; it <= kInverseIterations; ++it) {
+        // orthogonalize against the cluster members already finished (CGS twice)
+        for (int pass = 0; pass < 2; ++pass)
+          for (int p0 = j; p0 < jj; p0 += kCoopChunk) {
+            const int pe = min(jj, p0 + kCoopChunk);
+            for (int p = p0 + tid; p < pe; p += nt) {
+              const int cp = co.rank[o + p];
+              double dot = 0.0;
+              for (int i = b0; i < b1; ++i) dot = fma(Z[(int64_t)i * n + cp], Z[(int64_t)i * n + cj], dot);
+              dots[p - p0] = dot;
+            }
+            __syncthreads();
+            for (int i = b0 + tid; i < b1; i += nt) {
+              double z = Z[(int64_t)i * n + cj];
+              for (int p = p0; p < pe; ++p) z = fma(-dots[p - p0], Z[(int64_t)i * n + co.rank[o + p]], z);
+              Z[(int64_t)i * n + cj] = z;
+            }
+            __syncthreads();
+          }
+        if (it == kInverseIterations) break;
+        // scale to unit max-norm, then solve (T_b - x I) y = r
+        double rmax = 0.0;
+        for (int i = b0 + tid; i < b1; i += nt) rmax = fmax(rmax, fabs(Z[(int64_t)i * n + cj]));
+        rmax = block_max(rmax, red);
+        const double rs = rmax > 0.0 ? 1.0 / rmax : 1.0;
+        for (int i = b0 + tid; i < b1; i += nt) Z[(int64_t)i * n + cj] *= rs;
+        __syncthreads();
+        if (tid == 0) {
+          double cur = Z[(int64_t)b0 * n + cj];
+          for (int k = 0; k < bn - 1; ++k) {
+            const int64_t at = (int64_t)(b0 + k) * n + cj;
+            double rk = cur, rn = Z[at + n];
+            if (FP[at] != 0.0) { const double tmp = rk; rk = rn; rn = tmp; }
+            Z[at] = rk;
+            cur = rn - FM[at] * rk;
+          }
+          Z[(int64_t)(b1 - 1) * n + cj] = cur;
+          double z1 = 0.0, z2 = 0.0;
+          for (int k = bn - 1; k >= 0; --k) {
+            const int64_t at = (int64_t)(b0 + k) * n + cj;
+            double r = Z[at];
+            if (k + 1 < bn) r -= F1[at] * z1;
+            if (k + 2 < bn) r -= F2[at] * z2;
+            double u = F0[at];
+            if (fabs(u) < ptol) u = u < 0.0 ? -ptol : ptol;
+            const double zk = r / u;
+            Z[at] = zk;
+            z2 = z1;
+            z1 = zk;
+          }
+        }
+        __syncthreads();
+        // normalize
+        double ymax = 0.0;
+        for (int i = b0 + tid; i < b1; i += nt) ymax = fmax(ymax, fabs(Z[(int64_t)i * n + cj]));
+        ymax = block_max(ymax, red);
+        const double ys = ymax > 0.0 ? 1.0 / ymax : 1.0;
+        double nrm = 0.0;
+        for (int i = b0 + tid; i < b1; i += nt) {
+          const double y = Z[(int64_t)i * n + cj] * ys;
+          nrm += y * y;
+        }
+        nrm = block_sum(nrm, red);
+        const double sc2 = ys / sqrt(nrm);
+        for (int i = b0 + tid; i < b1; i += nt) Z[(int64_t)i * n + cj] *= sc2;
+        __syncthreads();
+      }
+      // final normalization; sign: the first largest-magnitude component positive
+      double nrm = 0.0, big = 0.0;
+      int bi = n;
+      for (int i = b0 + tid; i < b1; i += nt) {
+        const double y = Z[(int64_t)i * n + cj];
+        nrm += y * y;
+        if (fabs(y) > fabs(big)) { big = y; bi = i; }
+      }
+      nrm = block_sum(nrm, red);
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(big, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (fabs(ob) > fabs(big) || (fabs(ob) == fabs(big) && oi < bi)) { big = ob; bi = oi; }
+      }
+      if ((tid & 63) == 0) { red[tid >> 6] = big; ired[tid >> 6] = bi; }
+      __syncthreads();
+      big = red[0];
+      bi = ired[0];
+      for (int w = 1; w < (nt >> 6); ++w)
+        if (fabs(red[w]) > fabs(big) || (fabs(red[w]) == fabs(big) && ired[w] < bi)) { big = red[w]; bi = ired[w]; }
+      __syncthreads();
+      const double f = (big < 0.0 ? -1.0 : 1.0) / sqrt(nrm);
+      for (int i = b0 + tid; i < b1; i += nt) Z[(int64_t)i * n + cj] *= f;
+      __syncthreads();
+    }
+    if (tid == 0) co.end[o + j] = 0;  // consumed
   }
 }
 
