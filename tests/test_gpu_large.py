@@ -23,7 +23,9 @@ Checks:
 - the plain and graph-captured paths agree, and reproducible mode repeats
   bit for bit;
 - the overlap conditioning gate (status 8 exactly where the oracle's dpocon
-  gate fails) after the blocked factorization, at 364 and 1144 AOs.
+  gate fails) after the blocked factorization, at 364 and 1144 AOs;
+- two concurrent lanes and a fixed-topology plan with a WARM start give the
+  single-lane results.
 """
 import os
 
@@ -185,3 +187,36 @@ def test_overlap_gate_on_large_systems():
     assert out.per_system_status[0] == 0 and abs(out.energies[0] - ref["energies"][0]) <= 1e-10
     if ref["status"][3] == 0 and out.per_system_status[3] == 0:
         assert abs(out.energies[3] - ref["energies"][3]) <= E_TOL
+
+
+def test_lanes_plans_and_warm_start(monkeypatch):
+    rng = np.random.default_rng(21)
+    # 64 members: two lanes take 32 or more each
+    members = [workloads.alkane(190, rng), workloads.alkane(60, rng)]
+    members += [workloads.water(rng) if i % 2 else workloads.alkane(2 + i % 9, rng) for i in range(62)]
+    b = workloads.from_systems(members)
+    ctx = interface.Context("hip", 0)
+    try:
+        print("case large members: one lane", flush=True)
+        ref = _run(ctx, b)
+        assert np.all(ref.per_system_status == 0)
+        monkeypatch.setenv("XTBLOOM_AMD_LANES", "2")
+        print("case large members: two lanes", flush=True)
+        two = _run(ctx, b)
+        monkeypatch.setenv("XTBLOOM_AMD_LANES", "1")
+        for name in ("energies", "charges", "scc_iterations", "per_system_status"):
+            assert np.array_equal(getattr(two, name), getattr(ref, name)), name
+        assert np.abs(two.forces - ref.forces).max() < 1e-11
+        print("case large members: plan and WARM start", flush=True)
+        plan = interface.Plan(ctx, b.atom_offsets, b.atomic_numbers, b.positions, b.molecular_charges,
+                              b.unpaired_electrons)
+        try:
+            got = plan.compute(b.positions)
+            assert np.array_equal(got.energies, ref.energies)
+            warm = plan.compute(b.positions, scc_start_mode=library.SCC_START_WARM)
+            assert np.all(warm.per_system_status == 0) and warm.scc_iterations.max() <= 3
+            assert np.abs(warm.energies - ref.energies).max() < 1e-7
+        finally:
+            plan.close()
+    finally:
+        ctx.close()
