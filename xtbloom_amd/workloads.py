@@ -219,6 +219,8 @@ WORKLOADS = {
                                          r_min=10.0, r_max=20.0)),
     "long64": (ragged_alkanes, dict(batch=64, kmin=50, kmax=99, substitute=False,
                                     torsion_sigma_deg=20.0, points=1000, r_min=15.0, r_max=30.0)),
+    # systems above 1024 AOs (1082-1322): the global-memory kernel instantiations
+    "large4": (ragged_alkanes, dict(batch=4, kmin=180, kmax=220, substitute=False)),
 }
 
 
