@@ -25,7 +25,9 @@ Checks:
 - the overlap conditioning gate (status 8 exactly where the oracle's dpocon
   gate fails) after the blocked factorization, at 364 and 1144 AOs;
 - two concurrent lanes and a fixed-topology plan with a WARM start give the
-  single-lane results.
+  single-lane results;
+- large eigenvalue clusters (near-dependent bases) in an unsplit and in a
+  split tridiagonal matrix, against the oracle.
 """
 import os
 
@@ -220,3 +222,29 @@ def test_lanes_plans_and_warm_start(monkeypatch):
             plan.close()
     finally:
         ctx.close()
+
+
+def test_large_clusters_in_a_split_tridiagonal():
+    # two copies of a near-dependent C60H122 (+2 H) 120 bohr apart: S and H are
+    # block diagonal, T splits between the fragments, and each block holds a
+    # cluster of about 360 eigenvalues (block-local eigenvalue order)
+    rng = np.random.default_rng(11)
+    frag = _near_dependent(60, 1e-3, rng)
+    both = dict(Z=np.concatenate([frag["Z"], frag["Z"]]),
+                xyz=np.vstack([frag["xyz"], frag["xyz"] + [120.0, 0.0, 0.0]]), charge=0.0, uhf=0)
+    b = workloads.from_systems([both, workloads.water(rng)])
+    iters = 40
+    ctx = interface.Context("hip", 0)
+    try:
+        print("case two near-dependent fragments in one system (728 AOs)", flush=True)
+        out = _run(ctx, b, max_scc_iterations=iters)
+    finally:
+        ctx.close()
+    ref = O.batch(b.atom_offsets, b.atomic_numbers, b.positions, b.molecular_charges,
+                  b.unpaired_electrons, opts=O.options(max_iterations=iters), threads=4)
+    print("statuses", out.per_system_status.tolist(), "oracle", ref["status"].tolist(),
+          "iterations", out.scc_iterations.tolist(), "oracle", ref["iterations"].tolist(), flush=True)
+    assert ref["status"].tolist() == [0, 0] and out.per_system_status.tolist() == [0, 0]
+    assert np.abs(out.energies - ref["energies"]).max() <= E_TOL
+    a1 = b.atom_offsets[1]
+    assert np.abs(out.charges[:a1] - ref["charges"][:a1]).max() <= 1e-5
